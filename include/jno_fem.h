@@ -1,0 +1,207 @@
+/*
+ * jno_fem.h -- C-ABI of the MI355X (gfx950) finite-element hot path of jNO:
+ * Lagrange P1/P2 tetrahedral assembly into CSR, symmetric Dirichlet elimination and
+ * Jacobi-preconditioned CG (plain FP64, FP32-inner with FP64 refinement, batched), plus the
+ * adjoint dL/dk contraction.
+ *
+ * Conventions
+ *   - Every pointer argument named d_* is DEVICE memory owned by the caller.  Host pointers are
+ *     named h_*.  Integer index arrays are int32 (the reference's BCOO index dtype).
+ *   - `stream` is a hipStream_t passed as void*; NULL means the legacy default stream.
+ *   - Entry points return 0 on success or a negative JNO_E* status; jno_last_error() gives the
+ *     message of the calling thread's last failure.
+ *   - Nothing here allocates device memory on the hot path: scratch comes in through `d_ws`
+ *     whose size the matching *_workspace_size() query returns.
+ *   - Reference citations are relative to the FhG-IISB/jNO 0.3.1 checkout.
+ */
+#ifndef JNO_FEM_H
+#define JNO_FEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  JNO_OK = 0,
+  JNO_EINVAL = -1,       /* bad argument (ValueError in Python)                                 */
+  JNO_EHIP = -2,         /* HIP runtime error                                                   */
+  JNO_ENOTCONV = -3,     /* iteration cap reached before the tolerance                          */
+  JNO_ENONFINITE = -4,   /* NaN/Inf in the iteration                                            */
+  JNO_EWORKSPACE = -5,   /* workspace too small                                                 */
+  JNO_ECAPACITY = -6     /* a row or node exceeds a compiled capacity (e.g. > 64 P1 neighbours)  */
+};
+
+/* Message of the calling thread's last failure; returns its full length. */
+int jno_last_error(char* buf, size_t len);
+/* Library version string (static storage). */
+const char* jno_version(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * A1 mesh.  Replaces Geometries.equi_distant_box (jno/domain/geometries.py:204-301) as reached by
+ * Shape.box(..).structured(n) (jno/geometry/shape.py:415-458): (n+1)^3 lattice, node id
+ * (i*(ny+1)+j)*(nz+1)+k, six Kuhn tets per voxel in voxel-major order.
+ * d_points: f64[(nx+1)(ny+1)(nz+1)*3]; d_cells: i32[6*nx*ny*nz*4].
+ * ------------------------------------------------------------------------------------------- */
+int jno_mesh_kuhn_box(int nx, int ny, int nz, const double* h_lo, const double* h_hi,
+                      double* d_points, int32_t* d_cells, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * A7 sparsity pattern ("plan").  Replaces the static pattern hoist + compress_plan
+ * (jno/utils/solver/fem_native.py:2264-2298, jno/utils/solver/fem_utils.py:4009-4060): the CSR
+ * index structure = the (row, col)-sorted unique element pairs, dof = node*vec + comp.
+ * Two phases: count fills d_rowptr (ndof+1) and the node->(element,local) incidence that assembly
+ * reuses, and returns nnz in *h_nnz; fill writes d_col (nnz).
+ * d_inc_ptr: i32[n_nodes+1]; d_inc: i32[T*nloc] (entries e*nloc+a, ascending per node).
+ * ------------------------------------------------------------------------------------------- */
+size_t jno_pattern_workspace_size(int64_t n_cells, int nloc, int64_t n_nodes);
+int jno_pattern_count(const int32_t* d_cells, int64_t n_cells, int nloc, int vec, int64_t n_nodes,
+                      int32_t* d_inc_ptr, int32_t* d_inc, int32_t* d_rowptr, int64_t* h_nnz,
+                      int32_t* h_max_row, void* d_ws, size_t ws_bytes, void* stream);
+int jno_pattern_fill(const int32_t* d_cells, int64_t n_cells, int nloc, int vec, int64_t n_nodes,
+                     const int32_t* d_inc_ptr, const int32_t* d_inc, const int32_t* d_rowptr,
+                     int32_t* d_col, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * A10 boundary.  Replaces _boundary_facets (jno/_fem.py:2695-2773) + _boundary_node_ids
+ * (jno/utils/solver/fem_native.py:2563-2636): d_mask[n] = 1 for every node of a facet used by
+ * exactly one cell (P2: the facet's edge nodes too).  jno_mask_plane keeps the masked nodes whose
+ * coordinate `axis` equals `value` exactly (a named box face: left/right/... ).
+ * ------------------------------------------------------------------------------------------- */
+int jno_boundary_mask(const int32_t* d_cells, int64_t n_cells, int nloc, int64_t n_nodes,
+                      const int32_t* d_inc_ptr, const int32_t* d_inc, uint8_t* d_mask, void* stream);
+int jno_mask_plane(const double* d_points, int64_t n_nodes, int axis, double value,
+                   const uint8_t* d_mask_in, uint8_t* d_mask_out, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * A4-A6, A8, A9, A11, A14, A19  P1 scalar assembly.  Replaces FemLinearSystem.evaluate
+ * (jno/trace/__init__.py:3612-3615 -> _assemble_at, jno/utils/solver/fem_native.py:3311-3318) for
+ * the forms  kind=0: k grad u.grad v - f v  (stiffness + load)   kind=1: u v  (mass, k ignored).
+ * Row-pull, atomic-free: every row sums its incident element rows in ascending element order.
+ * d_k: f64[T] per-element coefficient (P0 parameter, jno/architectures/models.py:97-107).
+ * f: if d_f_nodal is NULL the load is the constant f_const, else the P1 nodal field d_f_nodal[N].
+ * d_dmask (nullable): u8[N] Dirichlet mask; d_g (nullable = zeros): f64[N] Dirichlet values.  With a
+ * mask the symmetric elimination of _apply_dirichlet_symmetric (jno/utils/solver/fem_1d.py:667-686)
+ * is fused: b -= A e(g), constrained rows/cols zeroed (pattern kept), unit diagonal, b[d] = g.
+ * d_b (nullable) f64[N]; d_dinv (nullable) f64[N] Jacobi 1/diag with the |d| > 1e-30 guard
+ * (jno/precond.py:105-109).  max_row = the plan's longest row (from jno_pattern_count).
+ * ------------------------------------------------------------------------------------------- */
+int jno_assemble_p1(int kind, const double* d_points, const int32_t* d_cells, int64_t n_cells,
+                    int64_t n_nodes, const int32_t* d_inc_ptr, const int32_t* d_inc,
+                    const int32_t* d_rowptr, const int32_t* d_col, int max_row, const double* d_k,
+                    double f_const, const double* d_f_nodal, const uint8_t* d_dmask,
+                    const double* d_g, double* d_vals, double* d_b, double* d_dinv, void* stream);
+
+/* A11 on any assembled CSR (P2 elasticity uses it): bcoo_zero_rows_cols + bcoo_set_unit_diag
+ * (jno/utils/solver/fem_utils.py:3332-3359) with the lift b -= A e(g) and b[d] = g.  In place. */
+int jno_dirichlet_eliminate(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                            double* d_vals, double* d_b, const uint8_t* d_dmask, const double* d_g,
+                            void* stream);
+
+/* A14 matrix_diagonal + _Jacobi.materialize (jno/utils/solver/linear.py:88-94,
+ * jno/precond.py:105-109): dinv = 1/d where |d| > 1e-30 else 1. */
+int jno_jacobi(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+               double* d_dinv, void* stream);
+
+/* A15 BCOO matvec LinearOperator.mv (jno/utils/solver/solver_api.py:137-147) on CSR: y = A x. */
+int jno_spmv_f64(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                 const double* d_x, double* d_y, void* stream);
+
+/* A12 compress_eager (jno/utils/solver/fem_utils.py:4063-4091): drop exact-zero entries.
+ * Count phase writes d_rowptr_out and *h_nnz_out; compact phase writes d_col_out/d_vals_out. */
+int jno_drop_zeros_count(int64_t n_rows, const int32_t* d_rowptr, const double* d_vals,
+                         int32_t* d_rowptr_out, int64_t* h_nnz_out, void* d_ws, size_t ws_bytes,
+                         void* stream);
+int jno_drop_zeros_fill(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                        const double* d_vals, const int32_t* d_rowptr_out, int32_t* d_col_out,
+                        double* d_vals_out, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Solver operators.  A "solver operator" is an opaque device-side re-layout of an assembled CSR
+ * for the PCG kernels (it is to the CSR what XLA's chosen layout is to a BCOO).  Formats:
+ *   JNO_FMT_CSR  : CSR with `lanes` lanes per row (1..64), FP64 and FP32 value copies.
+ *   JNO_FMT_DIA  : symmetric diagonal storage (main + upper diagonals; lower read by symmetry),
+ *                  chosen when the operator has at most 2*JNO_DIA_MAX-1 distinct diagonals and
+ *                  is structurally symmetric.  Index-free, fully coalesced.
+ * jno_op_analyze inspects a CSR and reports whether DIA applies; jno_op_fill_dia lays it out.
+ * ------------------------------------------------------------------------------------------- */
+enum { JNO_FMT_CSR = 0, JNO_FMT_DIA = 1 };
+#define JNO_DIA_MAX 16
+
+typedef struct jno_op {
+  int32_t format;      /* JNO_FMT_*                                                     */
+  int32_t lanes;       /* CSR: lanes per row                                            */
+  int64_t n;           /* rows                                                          */
+  int64_t nnz;         /* CSR nnz                                                       */
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* vals64;   /* CSR FP64 values (or DIA FP64 values [ndiag][n])             */
+  const float* vals32;    /* CSR FP32 values (or DIA FP32 values [ndiag][n]); nullable   */
+  int32_t ndiag;       /* DIA: stored diagonals (main first, then ascending upper)       */
+  int32_t offsets[JNO_DIA_MAX]; /* DIA: stored offsets (0 first)                        */
+} jno_op;
+
+/* Distinct column-row offsets (up to JNO_DIA_MAX upper ones) and symmetry of a CSR.
+ * *h_ndiag = number of distinct offsets >= 0 (0 is always counted), or -1 when there are more than
+ * JNO_DIA_MAX or the matrix is not symmetric (bitwise, when d_vals is given; structurally when it
+ * is NULL); h_offsets (JNO_DIA_MAX entries) receives them ascending from 0. */
+size_t jno_op_analyze_workspace_size(void);
+int jno_op_analyze(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                   int32_t* h_ndiag, int32_t* h_offsets, void* d_ws, size_t ws_bytes, void* stream);
+/* DIA fill from CSR: d_dia64 f64[ndiag*n] (nullable), d_dia32 f32[ndiag*n] (nullable). */
+int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                    const double* d_vals, int ndiag, const int32_t* h_offsets, double* d_dia64,
+                    float* d_dia32, void* stream);
+/* FP32 copy of CSR values. */
+int jno_cast_f32(int64_t n, const double* d_in, float* d_out, void* stream);
+/* y = A x with a solver operator, FP64 values. */
+int jno_op_spmv_f64(const jno_op* h_op, const double* d_x, double* d_y, void* stream);
+int jno_op_spmv_f32(const jno_op* h_op, const float* d_x, float* d_y, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * A16 + A17  Jacobi-PCG.  Replaces jno.solve.cg -> jax.scipy.sparse.linalg.cg (jno/solve.py:191-206)
+ * with M = jno.precond.jacobi(): x0 (or 0), r = b - A x0, z = M r, p = z; loop while r.r >
+ * max(tol^2 b.b, atol^2) and k < maxiter.  The final relative residual ||b - A x||/||b|| (true,
+ * FP64) is returned in *h_relres so the caller can apply the 1e-4 residual guard of
+ * _maybe_residual_check (jno/utils/solver/solver_api.py:224-240).
+ * d_x: in = x0 (if use_x0) / out = solution.  Returns JNO_ENOTCONV at maxiter (x still written).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct jno_pcg_stats {
+  int64_t iterations;      /* inner iterations (all precisions)          */
+  int64_t refinements;     /* FP64 residual replacements (mixed only)    */
+  double relres;           /* true FP64 ||b - A x|| / ||b||              */
+  double relres_recursive; /* recursive residual at exit                 */
+  double seconds;          /* wall time of the iteration on the stream   */
+} jno_pcg_stats;
+
+size_t jno_pcg_workspace_size(int64_t n, int mixed);
+/* Iterations enqueued per host poll of the device-side convergence state (default 8).  No
+ * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
+int jno_pcg_set_chunk(int chunk);
+int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
+                double tol, double atol, int64_t maxiter, jno_pcg_stats* h_stats, void* d_ws,
+                size_t ws_bytes, void* stream);
+/* FP32 inner PCG iterations with FP64 residual replacement ("reliable updates"): x accumulates in
+ * FP64, the true residual b - A x is recomputed in FP64 with the FP64 operator whenever the FP32
+ * recursive residual has dropped by `replace_factor` since the last replacement, and the stopping
+ * test is the true FP64 residual <= max(tol ||b||, atol).  Needs h_op->vals32. */
+int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x,
+                  int use_x0, double tol, double atol, int64_t maxiter, double replace_factor,
+                  jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * A18  adjoint contraction.  Replaces the VJP chain of custom_linear_solve(symmetric=True) ->
+ * BCOO data -> Dirichlet mask -> segment_sum -> jacfwd (jno/utils/solver/fem_native.py:2326-2345,
+ * jno/utils/solver/fem_utils.py:3332-3336):  d_grad[e] = -|T_e| (G_e P lam_e).(G_e u_e) where P zeroes
+ * the Dirichlet dofs (d_dmask, nullable).  Caller solves A lam = dL/du with jno_pcg_* first.
+ * ------------------------------------------------------------------------------------------- */
+int jno_adjoint_grad_k_p1(const double* d_points, const int32_t* d_cells, int64_t n_cells,
+                          const double* d_u, const double* d_lam, const uint8_t* d_dmask,
+                          double* d_grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JNO_FEM_H */

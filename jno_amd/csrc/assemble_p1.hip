@@ -1,0 +1,203 @@
+// P1 scalar assembly (stiffness k grad u.grad v with load f, or mass u v) straight into the CSR
+// values of the plan, with the symmetric Dirichlet elimination, the load vector and the Jacobi
+// inverse diagonal fused into the same pass; and the P1 adjoint contraction dL/dk.
+//
+// Row-pull, atomic-free: one thread owns one row, walks the row's incident (element, local)
+// entries in ascending element order (the plan sorted them), recomputes that element's geometry
+// and adds the element-matrix row into register accumulators matched to the row's CSR columns.
+// Every value is written exactly once; the summation order per entry is fixed, so results are
+// bit-reproducible run to run.
+#include "common.h"
+
+namespace jno {
+
+struct TetGeom {
+  double g[4][3];  // physical gradients of the barycentric basis
+  double vol;      // |detJ| / 6
+};
+
+// J = [v1-v0, v2-v0, v3-v0] (columns).  Rows of inv(J) are (e2 x e3, e3 x e1, e1 x e2) / det,
+// i.e. grad(lambda_1..3); grad(lambda_0) = -(sum).  (_cell_fields, fem_native.py:1461-1497)
+__device__ __forceinline__ TetGeom tet_geom(const double* __restrict__ pts, int4 c) {
+  const double* p0 = pts + 3 * (int64_t)c.x;
+  const double* p1 = pts + 3 * (int64_t)c.y;
+  const double* p2 = pts + 3 * (int64_t)c.z;
+  const double* p3 = pts + 3 * (int64_t)c.w;
+  double e1[3], e2[3], e3[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+    e3[i] = p3[i] - p0[i];
+  }
+  double c23[3] = {e2[1] * e3[2] - e2[2] * e3[1], e2[2] * e3[0] - e2[0] * e3[2], e2[0] * e3[1] - e2[1] * e3[0]};
+  double c31[3] = {e3[1] * e1[2] - e3[2] * e1[1], e3[2] * e1[0] - e3[0] * e1[2], e3[0] * e1[1] - e3[1] * e1[0]};
+  double c12[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  const double det = e1[0] * c23[0] + e1[1] * c23[1] + e1[2] * c23[2];
+  const double inv = 1.0 / det;
+  TetGeom t;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    t.g[1][i] = c23[i] * inv;
+    t.g[2][i] = c31[i] * inv;
+    t.g[3][i] = c12[i] * inv;
+    t.g[0][i] = -(t.g[1][i] + t.g[2][i] + t.g[3][i]);
+  }
+  t.vol = fabs(det) / 6.0;
+  return t;
+}
+
+template <int MAXR>
+__global__ void __launch_bounds__(256) assemble_p1_k(
+    int kind, const double* __restrict__ pts, const int4* __restrict__ cells, int64_t n_nodes,
+    const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc, const int32_t* __restrict__ rowptr,
+    const int32_t* __restrict__ col, const double* __restrict__ kcoef, double f_const,
+    const double* __restrict__ f_nodal, const uint8_t* __restrict__ dmask, const double* __restrict__ g,
+    double* __restrict__ vals, double* __restrict__ b, double* __restrict__ dinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_nodes;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int rb = rowptr[r];
+    const int len = rowptr[r + 1] - rb;
+    int cols[MAXR];
+    double acc[MAXR];
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) {
+      cols[s] = s < len ? col[rb + s] : -1;
+      acc[s] = 0.0;
+    }
+    double bl = 0.0;
+    const int ib = inc_ptr[r], ie = inc_ptr[r + 1];
+    for (int q = ib; q < ie; ++q) {
+      const int ent = inc[q];
+      const int e = ent >> 2, a = ent & 3;
+      const int4 c = cells[e];
+      const TetGeom t = tet_geom(pts, c);
+      const int nb[4] = {c.x, c.y, c.z, c.w};
+      if (kind == 0) {
+        const double coef = kcoef[e] * t.vol;
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+          const double v = coef * (t.g[a][0] * t.g[bb][0] + t.g[a][1] * t.g[bb][1] + t.g[a][2] * t.g[bb][2]);
+#pragma unroll
+          for (int s = 0; s < MAXR; ++s) acc[s] += (cols[s] == nb[bb]) ? v : 0.0;
+        }
+        if (f_nodal) {
+          const double fs = f_nodal[c.x] + f_nodal[c.y] + f_nodal[c.z] + f_nodal[c.w];
+          bl += t.vol * (f_nodal[nb[a]] + fs) / 20.0;
+        } else {
+          bl += f_const * t.vol / 4.0;
+        }
+      } else {
+        const double m = t.vol / 20.0;
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+          const double v = (bb == a) ? 2.0 * m : m;
+#pragma unroll
+          for (int s = 0; s < MAXR; ++s) acc[s] += (cols[s] == nb[bb]) ? v : 0.0;
+        }
+      }
+    }
+    // symmetric Dirichlet elimination (_apply_dirichlet_symmetric, fem_1d.py:667-686)
+    if (dmask) {
+      if (dmask[r]) {
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) acc[s] = (cols[s] == (int)r) ? 1.0 : 0.0;
+        bl = g ? g[r] : 0.0;
+      } else {
+        double lift = 0.0;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s) {
+          if (s < len && dmask[cols[s]]) {
+            if (g) lift += acc[s] * g[cols[s]];
+            acc[s] = 0.0;
+          }
+        }
+        bl -= lift;
+      }
+    }
+    double d = 0.0;
+#pragma unroll
+    for (int s = 0; s < MAXR; ++s) {
+      if (s < len) vals[rb + s] = acc[s];
+      if (cols[s] == (int)r) d += acc[s];
+    }
+    if (b) b[r] = bl;
+    if (dinv) dinv[r] = fabs(d) > 1e-30 ? 1.0 / d : 1.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) adjoint_grad_p1_k(const double* __restrict__ pts, const int4* __restrict__ cells,
+                                                         int64_t T, const double* __restrict__ u,
+                                                         const double* __restrict__ lam,
+                                                         const uint8_t* __restrict__ dmask,
+                                                         double* __restrict__ grad) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < T; e += (int64_t)gridDim.x * blockDim.x) {
+    const int4 c = cells[e];
+    const TetGeom t = tet_geom(pts, c);
+    const int nb[4] = {c.x, c.y, c.z, c.w};
+    double gl[3] = {0, 0, 0}, gu[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double la = (dmask && dmask[nb[a]]) ? 0.0 : lam[nb[a]];
+      const double ua = u[nb[a]];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        gl[i] += t.g[a][i] * la;
+        gu[i] += t.g[a][i] * ua;
+      }
+    }
+    grad[e] = -t.vol * (gl[0] * gu[0] + gl[1] * gu[1] + gl[2] * gu[2]);
+  }
+}
+
+template <int MAXR>
+static void launch_assemble(hipStream_t s, int kind, const double* pts, const int4* cells, int64_t N,
+                            const int32_t* inc_ptr, const int32_t* inc, const int32_t* rowptr, const int32_t* col,
+                            const double* k, double fc, const double* fn, const uint8_t* dm, const double* g,
+                            double* vals, double* b, double* dinv) {
+  assemble_p1_k<MAXR><<<grid_for(N, 256, 8192), 256, 0, s>>>(kind, pts, cells, N, inc_ptr, inc, rowptr, col, k, fc,
+                                                             fn, dm, g, vals, b, dinv);
+}
+
+}  // namespace jno
+
+using namespace jno;
+
+extern "C" int jno_assemble_p1(int kind, const double* d_points, const int32_t* d_cells, int64_t n_cells,
+                               int64_t n_nodes, const int32_t* d_inc_ptr, const int32_t* d_inc,
+                               const int32_t* d_rowptr, const int32_t* d_col, int max_row, const double* d_k,
+                               double f_const, const double* d_f_nodal, const uint8_t* d_dmask, const double* d_g,
+                               double* d_vals, double* d_b, double* d_dinv, void* stream) {
+  JNO_REQUIRE(kind == 0 || kind == 1, JNO_EINVAL, "jno_assemble_p1: kind must be 0 (stiffness) or 1 (mass)");
+  JNO_REQUIRE(kind == 1 || d_k, JNO_EINVAL, "jno_assemble_p1: stiffness needs the per-element coefficient k");
+  JNO_REQUIRE(max_row >= 0 && max_row <= 64, JNO_ECAPACITY,
+              "jno_assemble_p1: a row has %d entries; the P1 row-pull kernel holds at most 64", max_row);
+  JNO_REQUIRE(d_points && d_cells && d_rowptr && d_col && d_vals && d_inc_ptr && d_inc, JNO_EINVAL,
+              "jno_assemble_p1: null pointer");
+  (void)n_cells;
+  if (n_nodes == 0) return JNO_OK;
+  hipStream_t s = as_stream(stream);
+  const int4* cells = reinterpret_cast<const int4*>(d_cells);
+  if (max_row <= 16)
+    launch_assemble<16>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
+                        d_dmask, d_g, d_vals, d_b, d_dinv);
+  else if (max_row <= 32)
+    launch_assemble<32>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
+                        d_dmask, d_g, d_vals, d_b, d_dinv);
+  else
+    launch_assemble<64>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
+                        d_dmask, d_g, d_vals, d_b, d_dinv);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_adjoint_grad_k_p1(const double* d_points, const int32_t* d_cells, int64_t n_cells,
+                                     const double* d_u, const double* d_lam, const uint8_t* d_dmask, double* d_grad,
+                                     void* stream) {
+  JNO_REQUIRE(d_points && d_cells && d_u && d_lam && d_grad, JNO_EINVAL, "jno_adjoint_grad_k_p1: null pointer");
+  if (n_cells == 0) return JNO_OK;
+  adjoint_grad_p1_k<<<grid_for(n_cells, 256, 8192), 256, 0, as_stream(stream)>>>(
+      d_points, reinterpret_cast<const int4*>(d_cells), n_cells, d_u, d_lam, d_dmask, d_grad);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}

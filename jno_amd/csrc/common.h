@@ -1,0 +1,83 @@
+// Shared helpers of the jno_fem HIP library (gfx950 / CDNA4, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/jno_fem.h"
+
+namespace jno {
+
+void set_error(const char* fmt, ...);
+
+#define JNO_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (call);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      ::jno::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(_e));   \
+      return JNO_EHIP;                                                                       \
+    }                                                                                        \
+  } while (0)
+
+#define JNO_REQUIRE(cond, code, ...)   \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::jno::set_error(__VA_ARGS__);   \
+      return (code);                   \
+    }                                  \
+  } while (0)
+
+#define JNO_LAUNCH_CHECK() JNO_HIP(hipGetLastError())
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int WAVE = 64;
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Number of workgroups for a grid-stride kernel: enough to fill all 256 CUs several times over.
+inline int grid_for(int64_t work, int block, int max_blocks = 4096) {
+  int64_t g = cdiv(work, block);
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return static_cast<int>(g);
+}
+
+// ---- wavefront reductions (wave64, DPP/permute via __shfl_xor) ---------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum for blockDim.x = NT (multiple of 64).  All threads get the result.
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T v, T* smem /* >= NT/64 */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
+  T t = (threadIdx.x < NT / 64) ? smem[threadIdx.x] : T(0);
+  if (wid == 0) t = wave_sum(t);
+  if (threadIdx.x == 0) smem[0] = t;
+  __syncthreads();
+  T r = smem[0];
+  __syncthreads();
+  return r;
+}
+
+// Exclusive scan of int32 counts into int32 offsets (n+1 entries, last = total).  Three-phase,
+// workspace: cdiv(n, 1024) + 1 int64.  Writes the total to *d_total (int64, device).
+int exclusive_scan_i32(const int32_t* d_in, int32_t* d_out, int64_t n, int64_t* d_total, void* d_ws,
+                       size_t ws_bytes, hipStream_t stream);
+size_t exclusive_scan_ws(int64_t n);
+
+}  // namespace jno

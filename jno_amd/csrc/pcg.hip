@@ -1,0 +1,614 @@
+// Jacobi-preconditioned CG: plain FP64 (jax.scipy.sparse.linalg.cg semantics as called by
+// jno.solve.cg, jno/solve.py:191-206) and FP32 inner iterations with FP64 residual replacement.
+//
+// One iteration is two kernels:
+//   K1  p_k = z_k + beta p_{k-1}  (z = dinv r, recomputed for every gathered neighbour, so p is
+//       never a separate pass),  x += alpha_{k-1} p_{k-1} (the x update is deferred one iteration
+//       and fused here),  q = A p_k,  partial p.q
+//   K2  alpha_k = gamma_k / p.q,  r -= alpha_k q,  partials r.z and r.r
+// Scalars never leave the device: each kernel's last workgroup (ticket counter + fences) reduces
+// the per-workgroup partials in a fixed order, so every run gives bitwise identical scalars, and
+// writes them into a small state block read by the next kernel.  Convergence, the iteration cap
+// and non-finite scalars are decided on the device; launches after the decision exit at once.
+// The host enqueues fixed chunks of iterations and polls a pinned copy of the state one chunk
+// behind, so the queue never drains while it checks.
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "sparse.h"
+
+namespace jno {
+
+constexpr int PCG_NT = 256;
+constexpr int PCG_MAX_BLOCKS = 2048;
+
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
+
+struct PcgState {
+  int32_t done;    // written by K2 (block 0), read by K1
+  int32_t done1;   // written by K1 (block 0), read by K2
+  int32_t status;  // why K1 stopped
+  int32_t pad;
+  int64_t k;       // iterations done (written by K2 block 0)
+  int64_t k_stop;  // iteration at which K1 stopped
+  double gamma_next;  // gamma_k = r_k . z_k    (K2 last block / prep)
+  double rr;          // r_k . r_k              (K2 last block / prep)
+  double gamma_cur;   // copy of gamma_k for K2 (K1 block 0)
+  double gamma_prev;  // gamma_{k-1}            (K2 block 0)
+  double alpha;       // alpha_{k-1}            (K2 block 0)
+  double pq;          // p_k . A p_k            (K1 last block)
+  double atol2;       // stop when r.r <= atol2
+  double rr_replace;  // FP32 inner: replace the residual when r.r <= rr_replace
+  double rr_true;     // ||b - A x||^2 (prep)
+  double bb;          // b . b (prep)
+  int64_t maxiter;
+};
+
+template <typename T, typename V>
+struct PcgArgs {
+  int64_t n;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const V* vals;
+  int ndiag;
+  DiaOffsets off;
+  const T* dinv;
+  T* r;
+  const T* p_old;
+  T* p_new;
+  T* q;
+  T* x;
+  PcgState* st;
+  double* partials;    // 2 * PCG_MAX_BLOCKS
+  unsigned* counter;
+};
+
+// Block-reduce two doubles; the last block to finish reduces all blocks' partials in a fixed order
+// and returns true (with the totals in *t0, *t1) on thread 0 of that block.
+__device__ __forceinline__ bool reduce2_last(double v0, double v1, double* partials, unsigned* counter, double* t0,
+                                             double* t1) {
+  __shared__ double sm0[PCG_NT / 64], sm1[PCG_NT / 64];
+  __shared__ bool s_last;
+  v0 = wave_sum(v0);
+  v1 = wave_sum(v1);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    sm0[wid] = v0;
+    sm1[wid] = v1;
+  }
+  __syncthreads();
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int w = 0; w < PCG_NT / 64; ++w) {
+      a += sm0[w];
+      b += sm1[w];
+    }
+    partials[blockIdx.x] = a;
+    partials[PCG_MAX_BLOCKS + blockIdx.x] = b;
+    __threadfence();
+    const unsigned ticket = atomicAdd(counter, 1u);
+    s_last = (ticket == (unsigned)nb - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;  // block-uniform
+  __threadfence();
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nb; i += PCG_NT) {
+    a += partials[i];
+    b += partials[PCG_MAX_BLOCKS + i];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if (lane == 0) {
+    sm0[wid] = a;
+    sm1[wid] = b;
+  }
+  __syncthreads();
+  bool res = false;
+  if (threadIdx.x == 0) {
+    double s0 = 0, s1 = 0;
+    for (int w = 0; w < PCG_NT / 64; ++w) {
+      s0 += sm0[w];
+      s1 += sm1[w];
+    }
+    *t0 = s0;
+    *t1 = s1;
+    *counter = 0u;
+    res = true;
+  }
+  __syncthreads();  // the shared scratch may be reused by a following reduction
+  return res;
+}
+
+// ---------------------------------------------------------------------------------------------
+// K1: p update (+ deferred x update), q = A p, partial p.q
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T pnew_at(const T* __restrict__ dinv, const T* __restrict__ r, const T* __restrict__ p_old,
+                                     T beta, int64_t j) {
+  return dinv[j] * r[j] + beta * p_old[j];
+}
+
+template <int L, bool DIA, typename T, typename V>
+__global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
+  PcgState* st = a.st;
+  if (st->done) return;
+  const int64_t k = st->k;
+  const double gamma = st->gamma_next, rr = st->rr;
+  const double alpha_prev = st->alpha;
+  const bool stop_conv = !(rr > st->atol2);
+  const bool stop_cap = k >= st->maxiter;
+  const bool stop_repl = rr <= st->rr_replace;
+  const bool bad = !isfinite(gamma) || !isfinite(rr);
+  const T ap = (T)alpha_prev;
+  if (stop_conv || stop_cap || stop_repl || bad) {
+    // finish the deferred x update and stop
+    if (k > 0 && alpha_prev != 0.0) {
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
+        a.x[i] += ap * a.p_old[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done1 = 1;
+      st->k_stop = k;
+      st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
+    }
+    return;
+  }
+  const double beta_d = (k == 0) ? 0.0 : gamma / st->gamma_prev;
+  const T beta = (T)beta_d;
+  double part = 0.0;
+  if (DIA) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+      const T po = a.p_old[i];
+      const T pi = a.dinv[i] * a.r[i] + beta * po;
+      T s = (T)a.vals[i] * pi;
+      for (int d = 1; d < a.ndiag; ++d) {
+        const int64_t o = a.off.o[d];
+        const V* v = a.vals + (int64_t)d * a.n;
+        if (i + o < a.n) s += (T)v[i] * pnew_at(a.dinv, a.r, a.p_old, beta, i + o);
+        if (i - o >= 0) s += (T)v[i - o] * pnew_at(a.dinv, a.r, a.p_old, beta, i - o);
+      }
+      a.p_new[i] = pi;
+      a.q[i] = s;
+      if (k > 0) a.x[i] += ap * po;
+      part += (double)pi * (double)s;
+    }
+  } else {
+    const int sub = threadIdx.x % L;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x / L;
+    for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L; row < a.n; row += step) {
+      const int b = a.rowptr[row], e = a.rowptr[row + 1];
+      T s = 0;
+      for (int j = b + sub; j < e; j += L) s += (T)a.vals[j] * pnew_at(a.dinv, a.r, a.p_old, beta, a.col[j]);
+#pragma unroll
+      for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+      if (sub == 0) {
+        const T po = a.p_old[row];
+        const T pi = a.dinv[row] * a.r[row] + beta * po;
+        a.p_new[row] = pi;
+        a.q[row] = s;
+        if (k > 0) a.x[row] += ap * po;
+        part += (double)pi * (double)s;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
+  double t0, t1;
+  if (reduce2_last(part, 0.0, a.partials, a.counter, &t0, &t1)) st->pq = t0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// K2: r -= alpha q, partials r.z and r.r
+// ---------------------------------------------------------------------------------------------
+template <typename T, typename V>
+__global__ void __launch_bounds__(PCG_NT) pcg_k2(PcgArgs<T, V> a) {
+  PcgState* st = a.st;
+  if (st->done1) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+    return;
+  }
+  const double gamma = st->gamma_cur;
+  const double alpha = gamma / st->pq;
+  const T al = (T)alpha;
+  double pg = 0.0, pr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T ri = a.r[i] - al * a.q[i];
+    a.r[i] = ri;
+    const T zi = a.dinv[i] * ri;
+    pg += (double)ri * (double)zi;
+    pr += (double)ri * (double)ri;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alpha;
+    st->gamma_prev = gamma;
+    st->k = st->k + 1;
+  }
+  double t0, t1;
+  if (reduce2_last(pg, pr, a.partials, a.counter + 1, &t0, &t1)) {
+    st->gamma_next = t0;
+    st->rr = t1;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// residual preparation: r64 = b - Ax (Ax nullable = 0), rs = (T) r64, optional x-accumulation,
+// totals rr_true = r64.r64, gamma = rs.dinv.rs, rr = rs.rs, bb = b.b
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, const double* __restrict__ b,
+                                                          const double* __restrict__ Ax, const T* __restrict__ dinv,
+                                                          T* __restrict__ rs, PcgState* st, double* partials,
+                                                          unsigned* counter, int write_rs) {
+  double prt = 0, pg = 0, pr = 0, pb = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double bi = b[i];
+    const double r = Ax ? bi - Ax[i] : bi;
+    prt += r * r;
+    pb += bi * bi;
+    const T ri = (T)r;
+    if (write_rs) rs[i] = ri;
+    pg += (double)ri * (double)(dinv[i] * ri);
+    pr += (double)ri * (double)ri;
+  }
+  double t0, t1;
+  if (reduce2_last(prt, pb, partials, counter, &t0, &t1)) {
+    st->rr_true = t0;
+    st->bb = t1;
+  }
+  // second pair through the other counter slot (separate tickets, separate partial area)
+  if (reduce2_last(pg, pr, partials + 2 * PCG_MAX_BLOCKS, counter + 1, &t0, &t1)) {
+    st->gamma_next = t0;
+    st->rr = t1;
+  }
+}
+
+template <typename T>
+__global__ void zero_k(T* __restrict__ a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = T(0);
+}
+
+__global__ void accumulate_x_k(int64_t n, double* __restrict__ x, float* __restrict__ xs) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += (double)xs[i];
+    xs[i] = 0.0f;
+  }
+}
+
+__global__ void cast_f32_vec_k(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+__global__ void state_init_k(PcgState* st, double tol, double atol, int64_t maxiter, double replace2, int first) {
+  // runs after residual_prep: atol2 = max(tol^2 b.b, atol^2)
+  const double atol2 = fmax(tol * tol * st->bb, atol * atol);
+  st->atol2 = atol2;
+  st->maxiter = maxiter;
+  st->done = 0;
+  st->done1 = 0;
+  st->status = ST_RUNNING;
+  st->alpha = 0.0;
+  if (first) st->k = 0;
+  st->rr_replace = replace2 > 0.0 ? replace2 * st->rr : -1.0;
+}
+
+__global__ void counters_zero_k(unsigned* c, int n) {
+  if (threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------------------------
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Ws {
+  PcgState* st;
+  unsigned* counters;
+  double* partials;  // 4 * PCG_MAX_BLOCKS
+  void* r;
+  void* p0;
+  void* p1;
+  void* q;
+  void* xs;      // FP32 accumulator (mixed)
+  float* dinv32; // mixed
+  double* ax;    // FP64 A x
+};
+
+static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
+  const size_t vb = mixed ? 4 : 8;
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* p = base ? base + off : nullptr;
+    off += a256(bytes);
+    return p;
+  };
+  Ws t;
+  t.st = (PcgState*)take(sizeof(PcgState));
+  t.counters = (unsigned*)take(16 * sizeof(unsigned));
+  t.partials = (double*)take(4 * PCG_MAX_BLOCKS * sizeof(double));
+  t.r = take(n * vb);
+  t.p0 = take(n * vb);
+  t.p1 = take(n * vb);
+  t.q = take(n * vb);
+  t.xs = mixed ? take(n * 4) : nullptr;
+  t.dinv32 = mixed ? (float*)take(n * 4) : nullptr;
+  t.ax = (double*)take(n * 8);
+  if (w) *w = t;
+  return off;
+}
+
+struct HostPoll {
+  PcgState* h = nullptr;  // pinned, 2 slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool ok = false;
+  int init() {
+    if (ok) return JNO_OK;
+    JNO_HIP(hipHostMalloc((void**)&h, 2 * sizeof(PcgState), hipHostMallocDefault));
+    JNO_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    JNO_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    ok = true;
+    return JNO_OK;
+  }
+};
+static thread_local HostPoll g_poll;
+
+template <typename T, typename V>
+static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  if (op->format == JNO_FMT_DIA) {
+    pcg_k1<1, true, T, V><<<grid, PCG_NT, 0, s>>>(a);
+  } else {
+    switch (op->lanes) {
+      case 1: pcg_k1<1, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 2: pcg_k1<2, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 4: pcg_k1<4, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 8: pcg_k1<8, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 16: pcg_k1<16, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 32: pcg_k1<32, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 64: pcg_k1<64, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: lanes per row must be a power of two in 1..64");
+    }
+  }
+  JNO_LAUNCH_CHECK();
+  pcg_k2<T, V><<<grid, PCG_NT, 0, s>>>(a);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+// Runs K1/K2 chunks until the device reports a stop; returns the final state in *out.
+template <typename T, typename V>
+static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, hipStream_t s,
+                     PcgState* out) {
+  JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
+  const int64_t work = op->format == JNO_FMT_DIA ? a.n : a.n * op->lanes;
+  const int grid = grid_for(work, PCG_NT, PCG_MAX_BLOCKS);
+  int64_t k = k0;
+  int slot = 0, pending = 0;
+  for (;;) {
+    for (int i = 0; i < chunk; ++i, ++k) {
+      a.p_old = pbuf[k & 1];
+      a.p_new = pbuf[(k + 1) & 1];
+      int rc = launch_iter<T, V>(op, a, grid, s);
+      if (rc) return rc;
+    }
+    JNO_HIP(hipMemcpyAsync(&g_poll.h[slot], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipEventRecord(g_poll.ev[slot], s));
+    if (pending) {
+      const int prev = slot ^ 1;
+      JNO_HIP(hipEventSynchronize(g_poll.ev[prev]));
+      if (g_poll.h[prev].done1) break;
+    }
+    pending = 1;
+    slot ^= 1;
+  }
+  JNO_HIP(hipMemcpyAsync(&g_poll.h[0], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *out = g_poll.h[0];
+  return JNO_OK;
+}
+
+static int read_state(PcgState* d, PcgState* out, hipStream_t s) {
+  JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
+  JNO_HIP(hipMemcpyAsync(&g_poll.h[0], d, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *out = g_poll.h[0];
+  return JNO_OK;
+}
+
+static int finish_status(const PcgState& f, jno_pcg_stats* stats, int64_t iters) {
+  if (f.status == ST_NONFINITE) {
+    set_error("pcg: non-finite value in the iteration after %lld iterations", (long long)iters);
+    return JNO_ENONFINITE;
+  }
+  if (f.status == ST_MAXITER) {
+    set_error("pcg: no convergence in %lld iterations (relative residual %.3e)", (long long)iters,
+              stats ? stats->relres : -1.0);
+    return JNO_ENOTCONV;
+  }
+  return JNO_OK;
+}
+
+static int check_op(const jno_op* op, int need32) {
+  JNO_REQUIRE(op, JNO_EINVAL, "pcg: null operator");
+  JNO_REQUIRE(op->format == JNO_FMT_CSR || op->format == JNO_FMT_DIA, JNO_EINVAL, "pcg: unknown operator format");
+  JNO_REQUIRE(op->vals64, JNO_EINVAL, "pcg: operator has no FP64 values");
+  JNO_REQUIRE(!need32 || op->vals32, JNO_EINVAL, "pcg: mixed precision needs the operator's FP32 values");
+  if (op->format == JNO_FMT_CSR) JNO_REQUIRE(op->rowptr && op->col, JNO_EINVAL, "pcg: CSR operator without indices");
+  if (op->format == JNO_FMT_DIA)
+    JNO_REQUIRE(op->ndiag >= 1 && op->ndiag <= JNO_DIA_MAX && op->offsets[0] == 0, JNO_EINVAL,
+                "pcg: bad DIA operator");
+  return JNO_OK;
+}
+
+template <typename T, typename V>
+static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T* r, T* q, T* x, const Ws& w) {
+  PcgArgs<T, V> a;
+  a.n = op->n;
+  a.rowptr = op->rowptr;
+  a.col = op->col;
+  a.vals = vals;
+  a.ndiag = op->ndiag;
+  for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
+  a.dinv = dinv;
+  a.r = r;
+  a.q = q;
+  a.x = x;
+  a.st = w.st;
+  a.partials = w.partials;
+  a.counter = w.counters;
+  a.p_old = nullptr;
+  a.p_new = nullptr;
+  return a;
+}
+
+static int g_chunk = 8;
+
+}  // namespace jno
+
+using namespace jno;
+
+extern "C" size_t jno_pcg_workspace_size(int64_t n, int mixed) { return ws_layout(n, mixed ? 1 : 0, nullptr, nullptr); }
+
+extern "C" int jno_pcg_set_chunk(int chunk) {
+  JNO_REQUIRE(chunk >= 1 && chunk <= 1024, JNO_EINVAL, "jno_pcg_set_chunk: chunk must be in 1..1024");
+  g_chunk = chunk;
+  return JNO_OK;
+}
+
+extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
+                           double tol, double atol, int64_t maxiter, jno_pcg_stats* h_stats, void* d_ws,
+                           size_t ws_bytes, void* stream) {
+  int rc = check_op(h_op, 0);
+  if (rc) return rc;
+  const int64_t n = h_op->n;
+  JNO_REQUIRE(d_dinv && d_b && d_x, JNO_EINVAL, "jno_pcg_f64: null vector");
+  JNO_REQUIRE(ws_bytes >= jno_pcg_workspace_size(n, 0), JNO_EWORKSPACE, "jno_pcg_f64: workspace too small");
+  JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_f64: maxiter must be >= 0");
+  hipStream_t s = as_stream(stream);
+  Ws w;
+  ws_layout(n, 0, static_cast<char*>(d_ws), &w);
+  const auto t0 = std::chrono::steady_clock::now();
+  const int g = grid_for(n, PCG_NT, PCG_MAX_BLOCKS);
+  counters_zero_k<<<1, 64, 0, s>>>(w.counters, 16);
+  double* r = (double*)w.r;
+  double* pb[2] = {(double*)w.p0, (double*)w.p1};
+  zero_k<double><<<g, PCG_NT, 0, s>>>(pb[0], n);
+  zero_k<double><<<g, PCG_NT, 0, s>>>(pb[1], n);
+  if (use_x0) {
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    if (rc) return rc;
+  } else {
+    zero_k<double><<<g, PCG_NT, 0, s>>>(d_x, n);
+  }
+  residual_prep_k<double><<<g, PCG_NT, 0, s>>>(n, d_b, use_x0 ? w.ax : nullptr, d_dinv, r, w.st, w.partials,
+                                               w.counters + 2, 1);
+  state_init_k<<<1, 1, 0, s>>>(w.st, tol, atol, maxiter, 0.0, 1);
+  JNO_LAUNCH_CHECK();
+  PcgArgs<double, double> a = make_args<double, double>(h_op, h_op->vals64, d_dinv, r, (double*)w.q, d_x, w);
+  PcgState f;
+  rc = run_inner<double, double>(h_op, a, pb, 0, g_chunk, s, &f);
+  if (rc) return rc;
+  // true residual ||b - A x|| / ||b||  (the eager guard of _maybe_residual_check)
+  rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+  if (rc) return rc;
+  residual_prep_k<double><<<g, PCG_NT, 0, s>>>(n, d_b, w.ax, d_dinv, r, w.st, w.partials, w.counters + 2, 0);
+  JNO_LAUNCH_CHECK();
+  PcgState e;
+  rc = read_state(w.st, &e, s);
+  if (rc) return rc;
+  const auto t1 = std::chrono::steady_clock::now();
+  const double bn = sqrt(e.bb);
+  if (h_stats) {
+    h_stats->iterations = f.k_stop;
+    h_stats->refinements = 0;
+    h_stats->relres = bn > 0 ? sqrt(e.rr_true) / bn : sqrt(e.rr_true);
+    h_stats->relres_recursive = bn > 0 ? sqrt(f.rr) / bn : sqrt(f.rr);
+    h_stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+  return finish_status(f, h_stats, f.k_stop);
+}
+
+extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
+                             double tol, double atol, int64_t maxiter, double replace_factor,
+                             jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream) {
+  int rc = check_op(h_op, 1);
+  if (rc) return rc;
+  const int64_t n = h_op->n;
+  JNO_REQUIRE(d_dinv && d_b && d_x, JNO_EINVAL, "jno_pcg_mixed: null vector");
+  JNO_REQUIRE(ws_bytes >= jno_pcg_workspace_size(n, 1), JNO_EWORKSPACE, "jno_pcg_mixed: workspace too small");
+  JNO_REQUIRE(replace_factor > 0.0 && replace_factor < 1.0, JNO_EINVAL,
+              "jno_pcg_mixed: replace_factor must be in (0, 1)");
+  JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_mixed: maxiter must be >= 0");
+  hipStream_t s = as_stream(stream);
+  Ws w;
+  ws_layout(n, 1, static_cast<char*>(d_ws), &w);
+  const auto t0 = std::chrono::steady_clock::now();
+  const int g = grid_for(n, PCG_NT, PCG_MAX_BLOCKS);
+  counters_zero_k<<<1, 64, 0, s>>>(w.counters, 16);
+  float* rs = (float*)w.r;
+  float* pb[2] = {(float*)w.p0, (float*)w.p1};
+  float* xs = (float*)w.xs;
+  zero_k<float><<<g, PCG_NT, 0, s>>>(pb[0], n);
+  zero_k<float><<<g, PCG_NT, 0, s>>>(pb[1], n);
+  zero_k<float><<<g, PCG_NT, 0, s>>>(xs, n);
+  cast_f32_vec_k<<<g, PCG_NT, 0, s>>>(n, d_dinv, w.dinv32);
+  if (use_x0) {
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    if (rc) return rc;
+  } else {
+    zero_k<double><<<g, PCG_NT, 0, s>>>(d_x, n);
+  }
+  const double repl2 = replace_factor * replace_factor;
+  residual_prep_k<float><<<g, PCG_NT, 0, s>>>(n, d_b, use_x0 ? w.ax : nullptr, w.dinv32, rs, w.st, w.partials,
+                                              w.counters + 2, 1);
+  state_init_k<<<1, 1, 0, s>>>(w.st, tol, atol, maxiter, repl2, 1);
+  JNO_LAUNCH_CHECK();
+  PcgArgs<float, float> a = make_args<float, float>(h_op, h_op->vals32, w.dinv32, rs, (float*)w.q, xs, w);
+  int64_t k = 0, refinements = 0;
+  PcgState f, e;
+  double rr_true = 0.0, bb = 0.0;
+  for (;;) {
+    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, s, &f);
+    if (rc) return rc;
+    k = f.k_stop;
+    // FP64 refinement: x += xs, r = b - A x, rs = (float) r
+    accumulate_x_k<<<g, PCG_NT, 0, s>>>(n, d_x, xs);
+    JNO_LAUNCH_CHECK();
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    if (rc) return rc;
+    residual_prep_k<float><<<g, PCG_NT, 0, s>>>(n, d_b, w.ax, w.dinv32, rs, w.st, w.partials, w.counters + 2, 1);
+    JNO_LAUNCH_CHECK();
+    rc = read_state(w.st, &e, s);
+    if (rc) return rc;
+    ++refinements;
+    rr_true = e.rr_true;
+    bb = e.bb;
+    const double atol2 = fmax(tol * tol * bb, atol * atol);
+    if (!(rr_true > atol2)) {
+      f.status = ST_CONVERGED;
+      break;
+    }
+    if (f.status == ST_MAXITER || f.status == ST_NONFINITE || !std::isfinite(rr_true)) {
+      if (!std::isfinite(rr_true)) f.status = ST_NONFINITE;
+      break;
+    }
+    // resume the Krylov recurrence on the replaced residual: alpha = 0 so the deferred x update of
+    // the next K1 is void (it was applied at the stop); p_{k-1} and gamma_{k-1} are kept.
+    state_init_k<<<1, 1, 0, s>>>(w.st, tol, atol, maxiter, repl2, 0);
+    JNO_LAUNCH_CHECK();
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  const double bn = sqrt(bb);
+  if (h_stats) {
+    h_stats->iterations = k;
+    h_stats->refinements = refinements;
+    h_stats->relres = bn > 0 ? sqrt(rr_true) / bn : sqrt(rr_true);
+    h_stats->relres_recursive = bn > 0 ? sqrt(e.rr) / bn : sqrt(e.rr);
+    h_stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+  return finish_status(f, h_stats, k);
+}

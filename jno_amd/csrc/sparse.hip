@@ -1,0 +1,401 @@
+// CSR utilities: SpMV (lanes-per-row CSR and symmetric DIA), Jacobi diagonal, Dirichlet
+// elimination on an assembled CSR, exact-zero compaction, and the operator analysis that decides
+// whether the index-free symmetric diagonal layout applies.
+#include "common.h"
+#include "sparse.h"
+
+namespace jno {
+
+// ---------------------------------------------------------------------------------------------
+// SpMV
+// ---------------------------------------------------------------------------------------------
+template <int L, typename V, typename X>
+__global__ void __launch_bounds__(256) csr_spmv_k(int64_t n, const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ col, const V* __restrict__ vals,
+                                                  const X* __restrict__ x, X* __restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x % L;
+  for (int64_t row = gid / L; row < n; row += (int64_t)gridDim.x * blockDim.x / L) {
+    const int b = rowptr[row], e = rowptr[row + 1];
+    X s = 0;
+    for (int j = b + sub; j < e; j += L) s += (X)vals[j] * x[col[j]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+    if (sub == 0) y[row] = s;
+  }
+}
+
+template <typename V, typename X>
+__global__ void __launch_bounds__(256) dia_spmv_k(int64_t n, int ndiag, DiaOffsets off, const V* __restrict__ dia,
+                                                  const X* __restrict__ x, X* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    X s = (X)dia[i] * x[i];
+    for (int d = 1; d < ndiag; ++d) {
+      const int64_t o = off.o[d];
+      const V* v = dia + (int64_t)d * n;
+      if (i + o < n) s += (X)v[i] * x[i + o];
+      if (i - o >= 0) s += (X)v[i - o] * x[i - o];
+    }
+    y[i] = s;
+  }
+}
+
+template <typename V, typename X>
+int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s) {
+  JNO_REQUIRE(op && vals && x && y, JNO_EINVAL, "spmv: null pointer (is the FP32 copy built?)");
+  const int64_t n = op->n;
+  if (n == 0) return JNO_OK;
+  if (op->format == JNO_FMT_DIA) {
+    DiaOffsets off;
+    for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = op->offsets[d];
+    dia_spmv_k<V, X><<<grid_for(n, 256, 16384), 256, 0, s>>>(n, op->ndiag, off, vals, x, y);
+  } else {
+    const int L = op->lanes;
+    const int g = grid_for(n * L, 256, 16384);
+    switch (L) {
+      case 1: csr_spmv_k<1, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      case 2: csr_spmv_k<2, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      case 4: csr_spmv_k<4, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      case 8: csr_spmv_k<8, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      case 16: csr_spmv_k<16, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      case 32: csr_spmv_k<32, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      case 64: csr_spmv_k<64, V, X><<<g, 256, 0, s>>>(n, op->rowptr, op->col, vals, x, y); break;
+      default: JNO_REQUIRE(false, JNO_EINVAL, "spmv: lanes per row must be a power of two in 1..64");
+    }
+  }
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+template int op_spmv<double, double>(const jno_op*, const double*, const double*, double*, hipStream_t);
+template int op_spmv<float, float>(const jno_op*, const float*, const float*, float*, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
+// Jacobi, Dirichlet, cast
+// ---------------------------------------------------------------------------------------------
+__global__ void jacobi_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                         const double* __restrict__ vals, double* __restrict__ dinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double d = 0.0;  // duplicates on the diagonal are summed (matrix_diagonal, linear.py:88-94)
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j)
+      if (col[j] == (int)r) d += vals[j];
+    dinv[r] = fabs(d) > 1e-30 ? 1.0 / d : 1.0;
+  }
+}
+
+__global__ void dirichlet_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                            double* __restrict__ vals, double* __restrict__ b, const uint8_t* __restrict__ dmask,
+                            const double* __restrict__ g) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int rb = rowptr[r], re = rowptr[r + 1];
+    if (dmask[r]) {
+      bool seen = false;  // a duplicated diagonal would sum to 1 only if one copy carries it
+      for (int j = rb; j < re; ++j) {
+        const bool diag = col[j] == (int)r && !seen;
+        seen |= diag;
+        vals[j] = diag ? 1.0 : 0.0;
+      }
+      if (b) b[r] = g ? g[r] : 0.0;
+    } else {
+      double lift = 0.0;
+      for (int j = rb; j < re; ++j) {
+        const int c = col[j];
+        if (dmask[c]) {
+          if (g) lift += vals[j] * g[c];
+          vals[j] = 0.0;
+        }
+      }
+      if (b) b[r] -= lift;
+    }
+  }
+}
+
+__global__ void cast_f32_k(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// exact-zero compaction (compress_eager, fem_utils.py:4063-4091)
+// ---------------------------------------------------------------------------------------------
+__global__ void nz_count_k(int64_t n, const int32_t* __restrict__ rowptr, const double* __restrict__ vals,
+                           int32_t* __restrict__ cnt) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int c = 0;
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) c += fabs(vals[j]) > 0.0 ? 1 : 0;
+    cnt[r] = c;
+  }
+}
+
+__global__ void nz_fill_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                          const double* __restrict__ vals, const int32_t* __restrict__ rowptr_o,
+                          int32_t* __restrict__ col_o, double* __restrict__ vals_o) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int o = rowptr_o[r];
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      const double v = vals[j];
+      if (fabs(v) > 0.0) {
+        col_o[o] = col[j];
+        vals_o[o] = v;
+        ++o;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// operator analysis: distinct diagonal offsets (LDS hash per block, merged into a global table)
+// and numerical symmetry
+// ---------------------------------------------------------------------------------------------
+constexpr int OFF_SLOTS = 128;
+constexpr int32_t OFF_EMPTY = INT32_MIN;
+
+__device__ __forceinline__ bool hash_insert(int32_t* table, int32_t key) {
+  uint32_t h = ((uint32_t)key * 2654435761u) & (OFF_SLOTS - 1);
+  for (int p = 0; p < OFF_SLOTS; ++p) {
+    int32_t cur = table[h];
+    if (cur == key) return true;
+    if (cur == OFF_EMPTY) {
+      int32_t old = atomicCAS(&table[h], OFF_EMPTY, key);
+      if (old == OFF_EMPTY || old == key) return true;
+    }
+    h = (h + 1) & (OFF_SLOTS - 1);
+  }
+  return false;
+}
+
+// Entries whose value is exactly 0.0 (when vals is given) are skipped: they are the couplings
+// compress_eager drops, and a DIA layout does not need a diagonal for them.
+__global__ void __launch_bounds__(256) offsets_k(int64_t n, const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ col, const double* __restrict__ vals,
+                                                 int32_t* __restrict__ gtable, int32_t* __restrict__ overflow) {
+  __shared__ int32_t table[OFF_SLOTS];
+  __shared__ int s_over;
+  for (int i = threadIdx.x; i < OFF_SLOTS; i += blockDim.x) table[i] = OFF_EMPTY;
+  if (threadIdx.x == 0) s_over = 0;
+  __syncthreads();
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    if (s_over) break;
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      if (vals && vals[j] == 0.0) continue;
+      if (!hash_insert(table, col[j] - (int32_t)r)) {
+        s_over = 1;
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  if (s_over) {
+    if (threadIdx.x == 0) atomicExch(overflow, 1);
+    return;
+  }
+  for (int i = threadIdx.x; i < OFF_SLOTS; i += blockDim.x) {
+    const int32_t k = table[i];
+    if (k != OFF_EMPTY && !hash_insert(gtable, k)) atomicExch(overflow, 1);
+  }
+}
+
+__global__ void symmetry_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ vals, int32_t* __restrict__ bad) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      const int c = col[j];
+      if (c == (int)r || (vals && vals[j] == 0.0)) continue;
+      // binary search (c, r)
+      int lo = rowptr[c], hi = rowptr[c + 1] - 1, hit = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int cm = col[mid];
+        if (cm == (int)r) { hit = mid; break; }
+        if (cm < (int)r) lo = mid + 1; else hi = mid - 1;
+      }
+      const bool ok = hit >= 0 ? (vals ? vals[hit] == vals[j] : true) : (vals ? vals[j] == 0.0 : false);
+      if (!ok) atomicAdd(bad, 1);
+    }
+  }
+}
+
+__global__ void dia_fill_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ vals, int ndiag, DiaOffsets off, double* __restrict__ d64,
+                           float* __restrict__ d32) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double acc[JNO_DIA_MAX];
+#pragma unroll
+    for (int d = 0; d < JNO_DIA_MAX; ++d) acc[d] = 0.0;
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      const int o = col[j] - (int)r;
+      const double v = vals[j];
+#pragma unroll
+      for (int d = 0; d < JNO_DIA_MAX; ++d)
+        if (d < ndiag && off.o[d] == o) acc[d] += v;
+    }
+#pragma unroll
+    for (int d = 0; d < JNO_DIA_MAX; ++d) {
+      if (d < ndiag) {
+        if (d64) d64[(int64_t)d * n + r] = acc[d];
+        if (d32) d32[(int64_t)d * n + r] = (float)acc[d];
+      }
+    }
+  }
+}
+
+}  // namespace jno
+
+using namespace jno;
+
+extern "C" int jno_jacobi(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                          double* d_dinv, void* stream) {
+  if (n_rows == 0) return JNO_OK;
+  jacobi_k<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, d_rowptr, d_col, d_vals, d_dinv);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_dirichlet_eliminate(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                                       double* d_vals, double* d_b, const uint8_t* d_dmask, const double* d_g,
+                                       void* stream) {
+  JNO_REQUIRE(d_dmask, JNO_EINVAL, "jno_dirichlet_eliminate: mask is required");
+  if (n_rows == 0) return JNO_OK;
+  dirichlet_k<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, d_rowptr, d_col, d_vals, d_b, d_dmask,
+                                                                    d_g);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_spmv_f64(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                            const double* d_x, double* d_y, void* stream) {
+  jno_op op{};
+  op.format = JNO_FMT_CSR;
+  op.lanes = 8;
+  op.n = n_rows;
+  op.rowptr = d_rowptr;
+  op.col = d_col;
+  op.vals64 = d_vals;
+  return op_spmv<double, double>(&op, d_vals, d_x, d_y, as_stream(stream));
+}
+
+extern "C" int jno_op_spmv_f64(const jno_op* h_op, const double* d_x, double* d_y, void* stream) {
+  JNO_REQUIRE(h_op, JNO_EINVAL, "jno_op_spmv_f64: null operator");
+  return op_spmv<double, double>(h_op, h_op->vals64, d_x, d_y, as_stream(stream));
+}
+
+extern "C" int jno_op_spmv_f32(const jno_op* h_op, const float* d_x, float* d_y, void* stream) {
+  JNO_REQUIRE(h_op, JNO_EINVAL, "jno_op_spmv_f32: null operator");
+  return op_spmv<float, float>(h_op, h_op->vals32, d_x, d_y, as_stream(stream));
+}
+
+extern "C" int jno_cast_f32(int64_t n, const double* d_in, float* d_out, void* stream) {
+  if (n == 0) return JNO_OK;
+  cast_f32_k<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(n, d_in, d_out);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_drop_zeros_count(int64_t n_rows, const int32_t* d_rowptr, const double* d_vals,
+                                    int32_t* d_rowptr_out, int64_t* h_nnz_out, void* d_ws, size_t ws_bytes,
+                                    void* stream) {
+  hipStream_t s = as_stream(stream);
+  const size_t need = (size_t)(n_rows + 1) * 4 + 256 + exclusive_scan_ws(n_rows);
+  JNO_REQUIRE(ws_bytes >= need, JNO_EWORKSPACE, "jno_drop_zeros_count: workspace %zu < %zu", ws_bytes, need);
+  int32_t* cnt = static_cast<int32_t*>(d_ws);
+  char* w = static_cast<char*>(d_ws) + (((size_t)(n_rows + 1) * 4 + 255) & ~(size_t)255);
+  int64_t* total = reinterpret_cast<int64_t*>(w);
+  w += 256;
+  nz_count_k<<<grid_for(n_rows, 256), 256, 0, s>>>(n_rows, d_rowptr, d_vals, cnt);
+  JNO_LAUNCH_CHECK();
+  int rc = exclusive_scan_i32(cnt, d_rowptr_out, n_rows, total, w, exclusive_scan_ws(n_rows), s);
+  if (rc) return rc;
+  int64_t h_total = 0;
+  JNO_HIP(hipMemcpyAsync(&h_total, total, 8, hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  if (h_nnz_out) *h_nnz_out = h_total;
+  return JNO_OK;
+}
+
+extern "C" int jno_drop_zeros_fill(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                                   const double* d_vals, const int32_t* d_rowptr_out, int32_t* d_col_out,
+                                   double* d_vals_out, void* stream) {
+  if (n_rows == 0) return JNO_OK;
+  nz_fill_k<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, d_rowptr, d_col, d_vals, d_rowptr_out,
+                                                                   d_col_out, d_vals_out);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" size_t jno_op_analyze_workspace_size(void) { return 1024; }
+
+extern "C" int jno_op_analyze(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                              int32_t* h_ndiag, int32_t* h_offsets, void* d_ws, size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(ws_bytes >= jno_op_analyze_workspace_size(), JNO_EWORKSPACE, "jno_op_analyze: workspace too small");
+  JNO_REQUIRE(h_ndiag && h_offsets, JNO_EINVAL, "jno_op_analyze: null output");
+  hipStream_t s = as_stream(stream);
+  int32_t* table = static_cast<int32_t*>(d_ws);
+  int32_t* flags = table + OFF_SLOTS;  // [0] overflow, [1] asymmetric count
+  {
+    int32_t h_init[OFF_SLOTS + 2];
+    for (int i = 0; i < OFF_SLOTS; ++i) h_init[i] = OFF_EMPTY;
+    h_init[OFF_SLOTS] = 0;
+    h_init[OFF_SLOTS + 1] = 0;
+    JNO_HIP(hipMemcpyAsync(table, h_init, sizeof(h_init), hipMemcpyHostToDevice, s));
+    JNO_HIP(hipStreamSynchronize(s));
+  }
+  offsets_k<<<grid_for(n_rows, 256, 2048), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_vals, table, flags);
+  JNO_LAUNCH_CHECK();
+  symmetry_k<<<grid_for(n_rows, 256, 8192), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_vals, flags + 1);
+  JNO_LAUNCH_CHECK();
+  int32_t h[OFF_SLOTS + 2];
+  JNO_HIP(hipMemcpyAsync(h, table, sizeof(h), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  if (h[OFF_SLOTS] || h[OFF_SLOTS + 1]) {
+    *h_ndiag = -1;
+    return JNO_OK;
+  }
+  int32_t offs[OFF_SLOTS];
+  int cnt = 0;
+  for (int i = 0; i < OFF_SLOTS; ++i)
+    if (h[i] != OFF_EMPTY) offs[cnt++] = h[i];
+  // ascending insertion sort
+  for (int i = 1; i < cnt; ++i)
+    for (int j = i; j > 0 && offs[j - 1] > offs[j]; --j) {
+      int32_t t = offs[j];
+      offs[j] = offs[j - 1];
+      offs[j - 1] = t;
+    }
+  int nd = 0;
+  bool has0 = false;
+  for (int i = 0; i < cnt; ++i) {
+    if (offs[i] == 0) has0 = true;
+    if (offs[i] >= 0) {
+      if (nd >= JNO_DIA_MAX) {
+        *h_ndiag = -1;
+        return JNO_OK;
+      }
+      nd++;
+    }
+  }
+  int k = 0;
+  if (!has0) {
+    if (nd >= JNO_DIA_MAX) {
+      *h_ndiag = -1;
+      return JNO_OK;
+    }
+    h_offsets[k++] = 0;
+  }
+  for (int i = 0; i < cnt; ++i)
+    if (offs[i] >= 0) h_offsets[k++] = offs[i];
+  *h_ndiag = k;
+  return JNO_OK;
+}
+
+extern "C" int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                               int ndiag, const int32_t* h_offsets, double* d_dia64, float* d_dia32, void* stream) {
+  JNO_REQUIRE(ndiag >= 1 && ndiag <= JNO_DIA_MAX && h_offsets, JNO_EINVAL, "jno_op_fill_dia: bad offsets");
+  JNO_REQUIRE(h_offsets[0] == 0, JNO_EINVAL, "jno_op_fill_dia: first offset must be the main diagonal");
+  DiaOffsets off;
+  for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = d < ndiag ? h_offsets[d] : 0;
+  if (n_rows == 0) return JNO_OK;
+  dia_fill_k<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, d_rowptr, d_col, d_vals, ndiag, off,
+                                                                    d_dia64, d_dia32);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}

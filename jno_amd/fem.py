@@ -1,0 +1,241 @@
+"""Assembly front-end: the CSR plan, the assembled operator and ``FemLinearSystem.evaluate``.
+
+Numeric analogue of the reference's parametric FEM operator: a mesh, a per-element coefficient
+(a P0 ``jno.np.parameter``), a load and a Dirichlet set go in; the assembled, symmetrically
+eliminated CSR ``A`` and load ``b`` come out (``FemLinearSystem.evaluate``,
+jno/trace/__init__.py:3612-3615 -> ``_assemble_at``, jno/utils/solver/fem_native.py:3311-3318).
+All work runs in the HIP library; this module only allocates and wires device buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Union
+
+import torch
+
+from . import _lib
+from .mesh import Mesh
+
+
+@dataclass
+class Plan:
+    """CSR index structure (the bit-exact contract) + the node->(element, local) incidence.
+
+    ``rowptr``/``col`` equal ``compress_plan``'s (row, col)-sorted unique pairs
+    (jno/utils/solver/fem_utils.py:4009-4060) for dof = node*vec + comp."""
+
+    rowptr: torch.Tensor
+    col: torch.Tensor
+    inc_ptr: torch.Tensor
+    inc: torch.Tensor
+    nnz: int
+    max_row: int
+    ndof: int
+    vec: int
+
+    @staticmethod
+    def build(mesh: Mesh, vec: int = 1) -> "Plan":
+        dev = mesh.device
+        N, T, nloc = mesh.n_nodes, mesh.n_cells, mesh.nloc
+        ndof = N * vec
+        inc_ptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        inc = torch.empty(max(T * nloc, 1), dtype=torch.int32, device=dev)
+        rowptr = torch.empty(ndof + 1, dtype=torch.int32, device=dev)
+        ws_bytes = _lib.load().jno_pattern_workspace_size(T, nloc, N)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        nnz = C.c_int64(0)
+        max_row = C.c_int32(0)
+        stream = _lib.stream_ptr(dev)
+        _lib.call(
+            "jno_pattern_count", _lib.ptr(mesh.cells), T, nloc, vec, N, _lib.ptr(inc_ptr), _lib.ptr(inc),
+            _lib.ptr(rowptr), C.byref(nnz), C.byref(max_row), _lib.ptr(ws), ws_bytes, stream,
+        )
+        col = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+        _lib.call(
+            "jno_pattern_fill", _lib.ptr(mesh.cells), T, nloc, vec, N, _lib.ptr(inc_ptr), _lib.ptr(inc),
+            _lib.ptr(rowptr), _lib.ptr(col), stream,
+        )
+        return Plan(rowptr, col[: nnz.value], inc_ptr, inc, int(nnz.value), int(max_row.value), ndof, vec)
+
+
+@dataclass
+class CSR:
+    """An assembled sparse operator in CSR form (FP64 values) on the device.
+
+    Plays the role of the reference's ``LinearOperator`` over a BCOO
+    (jno/utils/solver/solver_api.py:82-184): ``mv``/``@``, ``diag``, ``T`` (symmetric operators
+    are their own transpose here), ``dense`` for small checks, and ``bcoo`` = (data, idx)."""
+
+    rowptr: torch.Tensor
+    col: torch.Tensor
+    vals: torch.Tensor
+    shape: tuple
+    dinv: Optional[torch.Tensor] = None  # Jacobi inverse diagonal, when assembly produced it
+    _cache: dict = field(default_factory=dict, repr=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.shape[0])
+
+    @property
+    def device(self):
+        return self.vals.device
+
+    def mv(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.to(torch.float64).contiguous()
+        y = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+        _lib.call("jno_spmv_f64", self.shape[0], _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.vals),
+                  _lib.ptr(x), _lib.ptr(y), _lib.stream_ptr(self.device))
+        return y
+
+    __matmul__ = mv
+    __call__ = mv
+
+    @property
+    def T(self) -> "CSR":
+        return self
+
+    def diag_inv(self) -> torch.Tensor:
+        """Jacobi inverse diagonal with the |d| > 1e-30 guard (jno/precond.py:105-109)."""
+        if self.dinv is None:
+            d = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+            _lib.call("jno_jacobi", self.shape[0], _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.vals),
+                      _lib.ptr(d), _lib.stream_ptr(self.device))
+            self.dinv = d
+        return self.dinv
+
+    def diag(self) -> torch.Tensor:
+        """The diagonal (duplicates summed, ``matrix_diagonal``, jno/utils/solver/linear.py:88-94)."""
+        rows = self.row_indices()
+        on = rows == self.col.long()
+        d = torch.zeros(self.shape[0], dtype=torch.float64, device=self.device)
+        d.index_add_(0, rows[on], self.vals[on])
+        return d
+
+    def row_indices(self) -> torch.Tensor:
+        counts = (self.rowptr[1:] - self.rowptr[:-1]).long()
+        return torch.repeat_interleave(torch.arange(self.shape[0], device=self.device), counts)
+
+    def dense(self) -> torch.Tensor:
+        Dm = torch.zeros(self.shape, dtype=torch.float64, device=self.device)
+        Dm.index_put_((self.row_indices(), self.col.long()), self.vals, accumulate=True)
+        return Dm
+
+    @property
+    def bcoo(self):
+        """(data, idx) with idx (nnz, 2) int32 sorted by (row, col), the BCOO view."""
+        return self.vals, torch.stack([self.row_indices().to(torch.int32), self.col], dim=1)
+
+    def drop_zeros(self) -> "CSR":
+        """``compress_eager`` (jno/utils/solver/fem_utils.py:4063-4091): entries exactly 0.0 removed."""
+        n = self.shape[0]
+        rowptr_o = torch.empty(n + 1, dtype=torch.int32, device=self.device)
+        ws_bytes = (n + 1) * 4 + 512 + ((n + 1023) // 1024 + 1) * 8 + 256
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=self.device)
+        nnz = C.c_int64(0)
+        stream = _lib.stream_ptr(self.device)
+        _lib.call("jno_drop_zeros_count", n, _lib.ptr(self.rowptr), _lib.ptr(self.vals), _lib.ptr(rowptr_o),
+                  C.byref(nnz), _lib.ptr(ws), ws_bytes, stream)
+        col_o = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=self.device)
+        vals_o = torch.empty(max(nnz.value, 1), dtype=torch.float64, device=self.device)
+        _lib.call("jno_drop_zeros_fill", n, _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.vals),
+                  _lib.ptr(rowptr_o), _lib.ptr(col_o), _lib.ptr(vals_o), stream)
+        return CSR(rowptr_o, col_o[: nnz.value], vals_o[: nnz.value], self.shape, self.dinv)
+
+
+def _as_k(k, T: int, device) -> torch.Tensor:
+    k = torch.as_tensor(k, dtype=torch.float64, device=device)
+    if k.ndim == 0:
+        k = k.expand(T)
+    if k.shape != (T,):
+        raise ValueError(f"the per-element coefficient must have shape ({T},) (one value per cell), got {tuple(k.shape)}")
+    return k.contiguous()
+
+
+class FemLinearSystem:
+    """-div(k grad u) = f on a P1 tet mesh with u = g on ``dirichlet``; ``k`` is a per-element (P0)
+    runtime parameter named ``param``.
+
+    ``evaluate({"k": k})`` re-assembles ``A(k), b(k)`` like the reference's parametric operator
+    (jno/trace/__init__.py:3612-3615): CSR on the structural pattern P_struct, with the symmetric
+    Dirichlet elimination of jno/utils/solver/fem_1d.py:667-686 fused into assembly.
+    """
+
+    def __init__(self, mesh: Mesh, f: Union[float, torch.Tensor] = 1.0, dirichlet: Optional[str] = "boundary",
+                 g: Union[float, torch.Tensor] = 0.0, param: str = "k"):
+        if mesh.nloc != 4:
+            raise ValueError("FemLinearSystem (P1 Poisson) needs a 4-node tet mesh")
+        self.mesh = mesh
+        self.param = param
+        self.plan = mesh.plan(1)
+        N = mesh.n_nodes
+        dev = mesh.device
+        self.f_const = 0.0
+        self.f_nodal = None
+        if isinstance(f, torch.Tensor) and f.ndim > 0:
+            if f.shape != (N,):
+                raise ValueError(f"nodal load must have shape ({N},)")
+            self.f_nodal = f.to(device=dev, dtype=torch.float64).contiguous()
+        else:
+            self.f_const = float(f)
+        self.dmask = mesh.boundary_mask(dirichlet) if dirichlet else None
+        self.g = None
+        if isinstance(g, torch.Tensor) and g.ndim > 0:
+            self.g = g.to(device=dev, dtype=torch.float64).contiguous()
+        elif float(g) != 0.0:
+            self.g = torch.full((N,), float(g), dtype=torch.float64, device=dev)
+
+    @property
+    def n(self) -> int:
+        return self.mesh.n_nodes
+
+    def evaluate(self, args: dict, out: Optional[CSR] = None):
+        """(A, b) at the parameter values ``args[param]`` (shape (T,))."""
+        if self.param not in args:
+            raise KeyError(f"evaluate() needs the runtime parameter {self.param!r}")
+        m, plan = self.mesh, self.plan
+        k = _as_k(args[self.param], m.n_cells, m.device)
+        N = m.n_nodes
+        if out is None:
+            vals = torch.empty(plan.nnz, dtype=torch.float64, device=m.device)
+            dinv = torch.empty(N, dtype=torch.float64, device=m.device)
+            A = CSR(plan.rowptr, plan.col, vals, (N, N), dinv)
+        else:
+            A = out
+            A._cache.clear()
+        b = torch.empty(N, dtype=torch.float64, device=m.device)
+        _lib.call(
+            "jno_assemble_p1", 0, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
+            _lib.ptr(plan.inc), _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.max_row, _lib.ptr(k), self.f_const,
+            _lib.ptr(self.f_nodal), _lib.ptr(self.dmask), _lib.ptr(self.g), _lib.ptr(A.vals), _lib.ptr(b),
+            _lib.ptr(A.dinv), _lib.stream_ptr(m.device),
+        )
+        return A, b
+
+    def mass(self) -> CSR:
+        """Consistent P1 mass matrix (the u*v form) on the same pattern, no Dirichlet elimination."""
+        m, plan = self.mesh, self.plan
+        N = m.n_nodes
+        vals = torch.empty(plan.nnz, dtype=torch.float64, device=m.device)
+        _lib.call(
+            "jno_assemble_p1", 1, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
+            _lib.ptr(plan.inc), _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.max_row, None, 0.0, None, None, None,
+            _lib.ptr(vals), None, None, _lib.stream_ptr(m.device),
+        )
+        return CSR(plan.rowptr, plan.col, vals, (N, N))
+
+    def grad_k(self, u: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
+        """dL/dk_e = -(P lam)_e^T (K_e/k_e) u_e for the adjoint state ``lam`` (P zeroes Dirichlet dofs)."""
+        m = self.mesh
+        grad = torch.empty(m.n_cells, dtype=torch.float64, device=m.device)
+        u = u.to(torch.float64).contiguous()
+        lam = lam.to(torch.float64).contiguous()
+        _lib.call("jno_adjoint_grad_k_p1", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, _lib.ptr(u), _lib.ptr(lam),
+                  _lib.ptr(self.dmask), _lib.ptr(grad), _lib.stream_ptr(m.device))
+        return grad
+
+
+def fem_poisson(mesh: Mesh, f=1.0, dirichlet="boundary", g=0.0, param="k") -> FemLinearSystem:
+    """Spelling of ``jno.fem([k*(grad u . grad v) - f*v, u(boundary) - g])`` with ``k`` a P0 parameter."""
+    return FemLinearSystem(mesh, f=f, dirichlet=dirichlet, g=g, param=param)

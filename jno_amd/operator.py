@@ -1,0 +1,116 @@
+"""Solver operators: the device layout the PCG kernels stream, derived from an assembled CSR.
+
+Two layouts, picked per operator (``format="auto"`` lets :mod:`jno_amd.autotune` time them):
+  * ``dia``  symmetric diagonal storage -- the main and upper diagonals of the nonzero pattern,
+             the lower ones read through symmetry.  Index-free and fully coalesced; it applies when
+             the nonzero entries lie on at most 31 distinct diagonals and the matrix is bitwise
+             symmetric (structured meshes, e.g. the Kuhn lattice).
+  * ``csr``  CSR with ``lanes`` lanes of a wavefront per row (1..64), exact zeros dropped first
+             (``compress_eager``, jno/utils/solver/fem_utils.py:4063-4091).
+Both carry FP64 values (plain PCG, FP64 residuals) and FP32 values (mixed-precision inner loop).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import torch
+
+from . import _lib
+from .fem import CSR
+
+
+class SolverOperator:
+    def __init__(self, A: CSR, fmt: str = "auto", lanes: Optional[int] = None, fp32: bool = True):
+        self.A = A
+        self.n = A.shape[0]
+        self.device = A.device
+        self._keep = []  # tensors referenced by the C struct
+        self.op = _lib.JnoOp()
+        fmt = fmt.lower()
+        if fmt not in ("auto", "dia", "csr"):
+            raise ValueError("format must be 'auto', 'dia' or 'csr'")
+        if fmt == "auto":
+            from .autotune import choose_format
+
+            fmt, lanes = choose_format(A)
+        self.ndiag = -1
+        if fmt == "dia":
+            nd, offs = self.analyze(A)
+            if nd < 0:
+                raise ValueError("this operator has no symmetric DIA layout (too many diagonals or not symmetric)")
+            self._build_dia(nd, offs, fp32)
+        else:
+            self._build_csr(A.drop_zeros(), lanes or 4, fp32)
+        self.format = fmt
+
+    @staticmethod
+    def analyze(A: CSR):
+        """(ndiag, offsets) of the nonzero pattern, ndiag = -1 when DIA does not apply."""
+        lib = _lib.load()
+        wsb = lib.jno_op_analyze_workspace_size()
+        ws = torch.empty(wsb, dtype=torch.uint8, device=A.device)
+        nd = C.c_int32(0)
+        offs = (C.c_int32 * _lib.JNO_DIA_MAX)()
+        _lib.call("jno_op_analyze", A.shape[0], _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), C.byref(nd),
+                  offs, _lib.ptr(ws), wsb, _lib.stream_ptr(A.device))
+        return int(nd.value), [int(offs[i]) for i in range(max(nd.value, 0))]
+
+    def _build_dia(self, nd, offs, fp32):
+        n, A = self.n, self.A
+        d64 = torch.empty(nd * n, dtype=torch.float64, device=self.device)
+        d32 = torch.empty(nd * n, dtype=torch.float32, device=self.device) if fp32 else None
+        offs_c = (C.c_int32 * _lib.JNO_DIA_MAX)(*offs)
+        _lib.call("jno_op_fill_dia", n, _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), nd, offs_c, _lib.ptr(d64),
+                  _lib.ptr(d32), _lib.stream_ptr(self.device))
+        self._keep += [d64, d32]
+        op = self.op
+        op.format = _lib.JNO_FMT_DIA
+        op.lanes = 1
+        op.n = n
+        op.nnz = A.nnz
+        op.vals64 = d64.data_ptr()
+        op.vals32 = d32.data_ptr() if d32 is not None else None
+        op.ndiag = nd
+        for i, o in enumerate(offs):
+            op.offsets[i] = o
+        self.ndiag = nd
+        self.offsets = offs
+        self.lanes = 1
+        self.nnz_stored = A.nnz
+
+    def _build_csr(self, Az: CSR, lanes, fp32):
+        if lanes not in (1, 2, 4, 8, 16, 32, 64):
+            raise ValueError("lanes per row must be a power of two in 1..64")
+        v32 = None
+        if fp32:
+            v32 = torch.empty(max(Az.nnz, 1), dtype=torch.float32, device=self.device)
+            _lib.call("jno_cast_f32", Az.nnz, _lib.ptr(Az.vals), _lib.ptr(v32), _lib.stream_ptr(self.device))
+        self._keep += [Az.rowptr, Az.col, Az.vals, v32]
+        op = self.op
+        op.format = _lib.JNO_FMT_CSR
+        op.lanes = lanes
+        op.n = self.n
+        op.nnz = Az.nnz
+        op.rowptr = Az.rowptr.data_ptr()
+        op.col = Az.col.data_ptr()
+        op.vals64 = Az.vals.data_ptr()
+        op.vals32 = v32.data_ptr() if v32 is not None else None
+        op.ndiag = 0
+        self.lanes = lanes
+        self.nnz_stored = Az.nnz
+
+    def mv(self, x: torch.Tensor) -> torch.Tensor:
+        if x.dtype == torch.float32:
+            y = torch.empty(self.n, dtype=torch.float32, device=self.device)
+            _lib.call("jno_op_spmv_f32", C.byref(self.op), _lib.ptr(x.contiguous()), _lib.ptr(y), _lib.stream_ptr(self.device))
+            return y
+        y = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        _lib.call("jno_op_spmv_f64", C.byref(self.op), _lib.ptr(x.to(torch.float64).contiguous()), _lib.ptr(y),
+                  _lib.stream_ptr(self.device))
+        return y
+
+    def describe(self) -> dict:
+        if self.format == "dia":
+            return {"format": "dia", "ndiag": self.ndiag, "offsets": self.offsets}
+        return {"format": "csr", "lanes": self.lanes, "nnz": self.nnz_stored}

@@ -1,0 +1,302 @@
+"""Parity of the HIP P1 path (mesh, pattern, assembly, Dirichlet, PCG, adjoint) with the oracle.
+
+Bars: bit-exact for meshes, masks and the CSR index structure; 1e-12 relative for assembled values;
+<= 1e-9 relative L2 for solutions at rtol 1e-10 (the FP64 oracle is scipy's direct solve)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SOL_TOL = 1e-9  # relative L2 vs the FP64 oracle at rtol 1e-10
+VAL_TOL = 1e-12  # relative, assembled values
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 4), (5, 5, 5), (16, 16, 16)])
+def test_mesh_bit_exact(dev, dims):
+    import jno_amd as J
+
+    m = J.kuhn_box(*dims)
+    p, t = O.kuhn_box(*dims)
+    assert np.array_equal(_np(m.points), p)
+    assert np.array_equal(_np(m.cells), t)
+
+
+def test_mesh_non_unit_box(dev):
+    import jno_amd as J
+
+    lo, hi = (-1.0, 0.5, 2.0), (3.0, 1.7, 2.3)
+    m = J.Shape.box(*lo, *hi).structured((3, 5, 7)).domain()
+    p, t = O.kuhn_box(3, 5, 7, lo, hi)
+    assert np.array_equal(_np(m.points), p) and np.array_equal(_np(m.cells), t)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 16])
+def test_pattern_bit_exact(dev, n):
+    import jno_amd as J
+
+    m = J.kuhn_box(n)
+    plan = m.plan(1)
+    p, t = O.kuhn_box(n)
+    rp, col, _, nse = O.pattern_struct(t, 1)
+    assert plan.nnz == nse
+    assert np.array_equal(_np(plan.rowptr), rp)
+    assert np.array_equal(_np(plan.col), col)
+    # incidence: ascending (element*4 + local) per node, covering every cell slot once
+    ip, inc = _np(plan.inc_ptr), _np(plan.inc)
+    assert ip[-1] == 4 * t.shape[0]
+    for r in range(0, p.shape[0], max(1, p.shape[0] // 50)):
+        seg = inc[ip[r]: ip[r + 1]]
+        assert np.all(np.diff(seg) > 0)
+        assert np.all(t.reshape(-1)[seg] == r)
+
+
+def test_pattern_vec3_bit_exact(dev):
+    import jno_amd as J
+
+    m = J.kuhn_box(2)
+    plan = m.plan(3)
+    p, t = O.kuhn_box(2)
+    rp, col, _, _ = O.pattern_struct(t, 3)
+    assert np.array_equal(_np(plan.rowptr), rp) and np.array_equal(_np(plan.col), col)
+
+
+def test_pattern_matches_golden(dev):
+    import jno_amd as J
+
+    g = np.load("tests/golden/p1_patterns.npz")
+    for n in (2, 4, 16):
+        plan = J.kuhn_box(n).plan(1)
+        assert np.array_equal(_np(plan.rowptr), g[f"rowptr_{n}"])
+        assert np.array_equal(_np(plan.col), g[f"col_{n}"])
+
+
+@pytest.mark.parametrize("n", [2, 5, 16])
+def test_boundary_masks(dev, n):
+    import jno_amd as J
+
+    m = J.kuhn_box(n)
+    p, t = O.kuhn_box(n)
+    assert np.array_equal(np.flatnonzero(_np(m.boundary_mask())), O.boundary_nodes(p, t))
+    for face in ("left", "right", "bottom", "top", "front", "back"):
+        assert np.array_equal(np.flatnonzero(_np(m.boundary_mask(face))), O.boundary_nodes(p, t, face))
+
+
+def _system(n, k, **kw):
+    import jno_amd as J
+
+    m = J.kuhn_box(n)
+    s = J.fem_poisson(m, **kw)
+    A, b = s.evaluate({"k": torch.as_tensor(k, device="cuda")})
+    return m, s, A, b
+
+
+def test_assembly_c1_values(dev):
+    n = 16
+    p, t = O.kuhn_box(n)
+    k = O.k_sin(p, t)
+    m, s, A, b = _system(n, k)
+    Ao, bo, d = O.poisson_p1_system(p, t, k)
+    Ao.sort_indices()
+    assert np.array_equal(_np(A.col), Ao.indices) and np.array_equal(_np(A.rowptr), Ao.indptr)
+    v = _np(A.vals)
+    assert np.abs(v - Ao.data).max() <= VAL_TOL * np.abs(Ao.data).max()
+    assert np.abs(_np(b) - bo).max() <= VAL_TOL * np.abs(bo).max()
+    assert np.abs(_np(A.dinv) - O.jacobi_inv(Ao)).max() <= VAL_TOL * np.abs(O.jacobi_inv(Ao)).max()
+    # exact-zero couplings on the dyadic Kuhn mesh: the zero-dropped operator is the oracle's P_eager
+    Ae = A.drop_zeros()
+    assert Ae.nnz == O.compress_eager(Ao).nnz == 23_813
+    # symmetric, bitwise
+    D = _np(A.dense())
+    assert np.array_equal(D, D.T)
+
+
+def test_assembly_nonzero_dirichlet_and_nodal_load(dev):
+    import jno_amd as J
+
+    n = 6
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 11)
+    f = 1.0 + p[:, 0] * p[:, 1]
+    g = np.sin(p[:, 0]) + p[:, 2]
+    m = J.kuhn_box(n)
+    s = J.fem_poisson(m, f=torch.tensor(f, device="cuda"), dirichlet="left", g=torch.tensor(g, device="cuda"))
+    A, b = s.evaluate({"k": torch.tensor(k, device="cuda")})
+    N = p.shape[0]
+    rp, col, vals = O.assemble(O.p1_stiffness(p, t, k), t, 1, N)
+    dofs = O.boundary_nodes(p, t, "left")
+    Ao, bo = O.dirichlet_eliminate(O.csr_matrix(rp, col, vals, N), O.p1_load(p, t, f, N), dofs, g[dofs])
+    assert np.abs(_np(A.vals) - Ao.data).max() <= VAL_TOL * np.abs(Ao.data).max()
+    assert np.abs(_np(b) - bo).max() <= VAL_TOL * np.abs(bo).max()
+
+
+def test_mass_matrix(dev):
+    n = 5
+    p, t = O.kuhn_box(n)
+    m, s, A, b = _system(n, np.ones(6 * n**3))
+    M = s.mass()
+    Mo = O.mass_matrix_p1(p, t)
+    Mo.sort_indices()
+    assert np.abs(_np(M.vals) - Mo.data).max() <= VAL_TOL * Mo.data.max()
+
+
+def test_spmv_and_generic_dirichlet(dev):
+    import jno_amd as J
+    from jno_amd import _lib
+
+    n = 7
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 5)
+    m = J.kuhn_box(n)
+    s = J.fem_poisson(m, dirichlet=None)
+    A, b = s.evaluate({"k": torch.tensor(k, device="cuda")})
+    Ao = O.csr_matrix(*O.assemble(O.p1_stiffness(p, t, k), t, 1, p.shape[0]), p.shape[0])
+    x = np.random.default_rng(0).standard_normal(p.shape[0])
+    assert _rel(_np(A.mv(torch.tensor(x, device="cuda"))), Ao @ x) < 1e-13
+    # jno_dirichlet_eliminate on the assembled CSR == fused elimination == oracle
+    mask = m.boundary_mask()
+    g = torch.tensor(np.cos(p[:, 1]), device="cuda")
+    _lib.call("jno_dirichlet_eliminate", A.shape[0], _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), _lib.ptr(b),
+              _lib.ptr(mask), _lib.ptr(g), _lib.stream_ptr())
+    dofs = O.boundary_nodes(p, t)
+    A2, b2 = O.dirichlet_eliminate(Ao, O.p1_load(p, t, 1.0, p.shape[0]), dofs, np.cos(p[dofs, 1]))
+    assert np.abs(_np(A.vals) - A2.data).max() <= VAL_TOL * np.abs(A2.data).max()
+    assert np.abs(_np(b) - b2).max() <= VAL_TOL * np.abs(b2).max()
+
+
+@pytest.mark.parametrize("fmt", ["csr", "dia", "auto"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_c1(dev, fmt, precision):
+    from jno_amd import precond, solve
+
+    n = 16
+    p, t = O.kuhn_box(n)
+    k = O.k_sin(p, t)
+    m, s, A, b = _system(n, k)
+    Ao, bo, d = O.poisson_p1_system(p, t, k)
+    xd = O.direct_solve(Ao, bo)
+    solver = solve.cg(tol=1e-10, precision=precision, format=fmt)
+    u = solver(A, b, M=precond.jacobi())
+    st = solver.last_stats
+    assert _rel(_np(u), xd) <= SOL_TOL, st
+    assert st["relres"] <= 1e-10, st
+    if precision == "fp64":
+        # jax cg semantics: the oracle's restated iteration stops after 64 iterations
+        assert abs(st["iterations"] - 64) <= 1, st
+    gold = np.load("tests/golden/c1_solution.npz")
+    assert _rel(_np(u), gold["u"]) <= SOL_TOL
+
+
+def test_solve_semantics(dev):
+    from jno_amd import precond, solve
+
+    n = 8
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 9)
+    m, s, A, b = _system(n, k)
+    Ao, bo, d = O.poisson_p1_system(p, t, k)
+    # b = 0: zero iterations, x = 0
+    sv = solve.cg(tol=1e-10)
+    x = sv(A, torch.zeros_like(b), M=precond.jacobi())
+    assert sv.last_stats["iterations"] == 0 and not _np(x).any()
+    # the iteration count follows the oracle's restated jax cg
+    xo, ito = O.pcg(Ao, bo, O.jacobi_inv(Ao), tol=1e-10)
+    sv(A, b, M=precond.jacobi())
+    assert abs(sv.last_stats["iterations"] - ito) <= 1
+    # x0 = solution: converged at once
+    x = sv(A, b, M=precond.jacobi(), x0=torch.tensor(O.direct_solve(Ao, bo), device="cuda"))
+    assert sv.last_stats["iterations"] <= 1
+    # maxiter too small: the 1e-4 residual guard raises like _maybe_residual_check
+    with pytest.raises(RuntimeError, match="did not solve the system"):
+        solve.cg(tol=1e-10, maxiter=2)(A, b, M=precond.jacobi())
+    # M = None (identity) still converges
+    x = solve.cg(tol=1e-10)(A, b)
+    assert _rel(_np(x), O.direct_solve(Ao, bo)) <= SOL_TOL
+    # a non-diagonal applier is refused with a remedy
+    with pytest.raises(TypeError, match="diagonal"):
+        solve.cg()(A, b, M=lambda v: v)
+
+
+def test_solve_nonfinite_coefficient_is_detected(dev):
+    from jno_amd import precond, solve
+
+    n = 4
+    k = np.ones(6 * n**3)
+    k[7] = np.nan
+    m, s, A, b = _system(n, k)
+    with pytest.raises(RuntimeError, match="did not solve"):
+        solve.cg(tol=1e-10)(A, b, M=precond.jacobi())
+
+
+def test_operator_layouts_agree(dev):
+    from jno_amd.operator import SolverOperator
+
+    n = 10
+    p, t = O.kuhn_box(n)
+    m, s, A, b = _system(n, O.k_lognormal(t.shape[0], 2))
+    x = torch.randn(A.shape[0], dtype=torch.float64, device="cuda")
+    ref = _np(A.mv(x))
+    dia = SolverOperator(A, fmt="dia")
+    assert dia.ndiag == 4 and dia.offsets == [0, 1, n + 1, (n + 1) ** 2]
+    assert _rel(_np(dia.mv(x)), ref) < 1e-14
+    for L in (1, 2, 4, 8, 16, 32, 64):
+        op = SolverOperator(A, fmt="csr", lanes=L)
+        assert _rel(_np(op.mv(x)), ref) < 1e-14
+        assert _rel(_np(op.mv(x.float())).astype(np.float64), ref) < 1e-6
+
+
+def test_adjoint_gradient_vs_oracle_and_fd(dev):
+    import jno_amd as J
+    from jno_amd import adjoint, precond, solve
+
+    n = 8
+    p, t = O.kuhn_box(n)
+    T = t.shape[0]
+    k = O.k_lognormal(T, 0)
+    kstar = O.k_lognormal(T, 1)
+    m = J.kuhn_box(n)
+    s = J.fem_poisson(m)
+    M = s.mass()
+    sv = solve.cg(tol=1e-12)
+    Astar, bstar = s.evaluate({"k": torch.tensor(kstar, device="cuda")})
+    ustar = sv(Astar, bstar, M=precond.jacobi())
+    kt = torch.tensor(k, device="cuda", requires_grad=True)
+    u = adjoint.solve(s, kt, sv)
+    e = u - ustar
+    L = 0.5 * torch.dot(e, M.mv(e))
+    L.backward()
+    grad = _np(kt.grad)
+    # oracle: direct solves + closed-form contraction
+    Ao, bo, d = O.poisson_p1_system(p, t, k)
+    Mo = O.mass_matrix_p1(p, t)
+    uo = O.direct_solve(Ao, bo)
+    uso = O.direct_solve(*O.poisson_p1_system(p, t, kstar)[:2])
+    rhs = Mo @ (uo - uso)
+    rhs[d] = 0
+    go = O.adjoint_grad_k_p1(p, t, uo, O.direct_solve(Ao, rhs), d)
+    assert _rel(grad, go) < 1e-8
+    # central FD with defect-correction solves on seeded elements (h = 1e-6 k_e)
+    rng = np.random.default_rng(3)
+    for el in rng.choice(T, 6, replace=False):
+        h = 1e-6 * k[el]
+        Ls = []
+        for sg in (1, -1):
+            kk = k.copy()
+            kk[el] += sg * h
+            A2, b2 = s.evaluate({"k": torch.tensor(kk, device="cuda")})
+            du = solve.cg(tol=1e-13)(A2, b2 - A2.mv(u.detach()), M=precond.jacobi())
+            uu = u.detach() + du
+            ee = uu - ustar
+            Ls.append(float(0.5 * torch.dot(ee, M.mv(ee))))
+        fd = (Ls[0] - Ls[1]) / (2 * h)
+        assert abs(fd - grad[el]) <= 1e-6 * abs(grad[el]) + 1e-15, (el, fd, grad[el])
