@@ -1,0 +1,250 @@
+"""Benchmark of the jno_amd FEM hot path on one MI355X.
+
+Workload (default): P1 Poisson -div(k grad u) = 1 on the unit cube, 128^3 hexes x 6 Kuhn tets
+(12,582,912 tets, 2,146,689 nodes), k_e = exp(0.5 z_e) with z ~ N(0,1) from numpy
+default_rng(0), zero Dirichlet on the boundary.  One step = re-assemble A(k), b (the parametric
+``evaluate``) + solve to relative residual 1e-10 with FP32 Jacobi-PCG inner iterations and FP64
+residual replacement, stopping on the true FP64 residual.  The CSR plan (pattern) is built once,
+as the reference caches it.
+
+Also reported: the plain FP64 GPU PCG on the same operator layout, the forward+adjoint gradient
+rate on 64^3 (dL/dk for all 1,572,864 elements), the live HIP-event roofline of the dominant
+kernel, and the CPU baseline (the C + OpenMP FP64 restatement, a bounded sample on the host).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-grad]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128, help="lattice cells per axis")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the bounded CPU PCG sample")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the path would shard by instances or by mesh "
+              "slabs; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import jno_amd as J
+    from jno_amd import _lib, precond, solve
+    from jno_amd.operator import SolverOperator
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    log(f"[bench] mesh {n}^3 + plan")
+    mesh = J.kuhn_box(n, device=dev)
+    T, N = mesh.n_cells, mesh.n_nodes
+    k_host = np.exp(0.5 * np.random.default_rng(args.seed).standard_normal(T))
+    k = torch.tensor(k_host, device=dev)
+    system = J.fem_poisson(mesh)
+    t0 = time.perf_counter()
+    plan = mesh.plan(1)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+
+    mixed = solve.cg(tol=1e-10, precision="mixed")
+    fp64 = solve.cg(tol=1e-10, precision="fp64")
+
+    def step(solver):
+        A, b = system.evaluate({"k": k})
+        u = solver(A, b, M=precond.jacobi())
+        return A, b, u
+
+    log("[bench] warmup")
+    for _ in range(args.warmup):
+        step(mixed)
+    torch.cuda.synchronize()
+
+    log(f"[bench] timed: {args.steps} steps (assemble + mixed solve to 1e-10)")
+    lib = _lib.load()
+    lib.jno_pcg_profile(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        A, b, u = step(mixed)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    k1_ms, k1_n, k2_ms, k2_n = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+    lib.jno_pcg_profile_read(C.byref(k1_ms), C.byref(k1_n), C.byref(k2_ms), C.byref(k2_n))
+    lib.jno_pcg_profile(0)
+    st_mixed = dict(mixed.last_stats)
+    op = mixed.operator(A)
+
+    # component times (one extra, untimed-in-value step split into its parts)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    A2, b2 = system.evaluate({"k": k})
+    torch.cuda.synchronize()
+    t_assemble = time.perf_counter() - ta
+    ta = time.perf_counter()
+    op2 = SolverOperator(A2, fmt=op.format, lanes=op.lanes, fp32=True)
+    torch.cuda.synchronize()
+    t_layout = time.perf_counter() - ta
+
+    log("[bench] plain FP64 PCG on the same operator")
+    fp64(A, b, M=precond.jacobi())  # warm
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    u64 = fp64(A, b, M=precond.jacobi())
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - ta
+    st_fp64 = dict(fp64.last_stats)
+    agree = float(torch.linalg.norm(u - u64) / torch.linalg.norm(u64))
+
+    # dominant kernel: K1 of the FP32 inner loop (SpMV + p/x update + p.q)
+    k1_avg_ms = k1_ms.value / max(k1_n.value, 1)
+    k2_avg_ms = k2_ms.value / max(k2_n.value, 1)
+    if op.format == "dia":
+        # per row: ndiag stored FP32 values, reads r, dinv, p_old, x and writes p_new, q, x (FP32)
+        k1_bytes_row = 4 * op.ndiag + 4 * 4 + 3 * 4
+        k1_bytes = N * k1_bytes_row
+    else:
+        nnz = op.nnz_stored
+        k1_bytes = nnz * (4 + 4) + 4 * (N + 1) + N * (4 * 4 + 3 * 4)
+    achieved = k1_bytes / (k1_avg_ms * 1e-3) / 1e9
+    k2_bytes = N * (3 * 4 + 4)
+    k2_gbs = k2_bytes / (k2_avg_ms * 1e-3) / 1e9
+    iters = st_mixed["iterations"]
+    # the survey's byte model of one FP32-inner iteration on P_eager CSR (nnz*(4+4) + 4(N+1) + 13*4N)
+    nnz_eager = op.nnz_stored if op.format == "csr" else None
+
+    out = {
+        "metric": "P1 Poisson 128³ time-to-1e-10 + HBM GB/s; fwd+adjoint grads/s vs FP64 plain",
+        "value": round(t_step, 6),
+        "unit": "s per assemble+solve to rtol 1e-10 (128³ P1, mixed FP32/FP64)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement",
+        "data": "synthetic (Kuhn lattice, lognormal k from numpy default_rng seed)",
+        "config": {"workload": f"P1 Poisson {n}^3 Kuhn tets (T={T}, N={N}), k=exp(0.5 z) seed {args.seed}, "
+                               "f=1, zero Dirichlet, rtol 1e-10", "parallelism": "single GPU"},
+        "solve": {
+            "mixed_s": round(st_mixed["seconds"], 6), "mixed_iterations": iters,
+            "mixed_refinements": st_mixed["refinements"], "mixed_relres_true": st_mixed["relres"],
+            "fp64_s": round(t_fp64, 6), "fp64_iterations": st_fp64["iterations"], "fp64_relres_true": st_fp64["relres"],
+            "speedup_mixed_vs_fp64": round(t_fp64 / max(st_mixed["seconds"], 1e-12), 3),
+            "mixed_vs_fp64_rel_l2": agree,
+            "dof_iterations_per_s": N * iters / max(st_mixed["seconds"], 1e-12),
+            "assemble_s": round(t_assemble, 6), "layout_s": round(t_layout, 6), "plan_s": round(t_plan, 6),
+            "operator": op.describe() | {"nnz_struct": plan.nnz},
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": f"pcg_k1<{op.format}, f32>", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": k1_bytes, "avg_launch_ms": k1_avg_ms, "launches": k1_n.value,
+            "k2_gbs": round(k2_gbs, 1), "k2_avg_ms": k2_avg_ms,
+        },
+    }
+    del nnz_eager
+
+    if not args.no_grad:
+        log("[bench] forward + adjoint gradient on 64^3")
+        out["gradient"] = grad_rate(J, torch, dev)
+
+    if not args.no_cpu:
+        log("[bench] CPU baseline (C + OpenMP FP64 restatement, bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_seconds, st_fp64["iterations"])
+    print(json.dumps(out), flush=True)
+
+
+def grad_rate(J, torch, dev, n=64, reps=3):
+    from jno_amd import adjoint, solve
+
+    mesh = J.kuhn_box(n, device=dev)
+    T = mesh.n_cells
+    system = J.fem_poisson(mesh)
+    kt = torch.tensor(np.exp(0.5 * np.random.default_rng(0).standard_normal(T)), device=dev)
+    ks = torch.tensor(np.exp(0.5 * np.random.default_rng(1).standard_normal(T)), device=dev)
+    M = system.mass()
+    res = {}
+    for prec in ("mixed", "fp64"):
+        sv = solve.cg(tol=1e-10, precision=prec)
+        from jno_amd import precond
+
+        Ast, bst = system.evaluate({"k": ks})
+        ustar = sv(Ast, bst, M=precond.jacobi())
+
+        def once():
+            k = kt.clone().requires_grad_(True)
+            u = adjoint.solve(system, k, sv)
+            e = u - ustar
+            L = 0.5 * torch.dot(e, M.mv(e))
+            L.backward()
+            return k.grad
+
+        once()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g = once()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        res[f"{prec}_grads_per_s"] = round(1.0 / dt, 3)
+        res[f"{prec}_s_per_grad"] = round(dt, 6)
+        res[f"{prec}_grad_norm"] = float(torch.linalg.norm(g))
+    res["workload"] = f"P1 Poisson {n}^3 (T={T}), L = 1/2 (u-u*)^T M (u-u*), dL/dk for all elements"
+    res["speedup_mixed_vs_fp64"] = round(res["fp64_s_per_grad"] / res["mixed_s_per_grad"], 3)
+    return res
+
+
+def cpu_baseline(n, seed, budget_s, full_iters):
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import cpu_fem
+    from oracle import fem_oracle as O
+
+    p, t = O.kuhn_box(n)
+    k = np.exp(0.5 * np.random.default_rng(seed).standard_normal(t.shape[0]))
+    mask = np.any((p == 0.0) | (p == 1.0), axis=1).astype(np.uint8)
+    s = cpu_fem.P1System(p, t, k, 1.0, mask)
+    # a short probe sizes the bounded sample to the budget
+    _, it0, _, dt0 = s.pcg(1e-10, maxiter=10)
+    per_it = dt0 / max(it0, 1)
+    sample = int(max(20, min(full_iters, budget_s / max(per_it, 1e-9))))
+    _, its, _, dts = s.pcg(1e-10, maxiter=sample)
+    per_it = dts / max(its, 1)
+    est = s.t_assemble + per_it * full_iters
+    return {
+        "value": round(est, 3),
+        "unit": "s per assemble+solve to rtol 1e-10 (estimated: assembly + per-iteration time x full iteration count)",
+        "cores": cpu_fem.threads(),
+        "kind": "port",
+        "sample": f"C+OpenMP FP64 restatement on the structural CSR (nnz {len(s.col)}): full assembly "
+                  f"({s.t_assemble:.3f} s, pattern {s.t_pattern:.3f} s untimed) + {its} of {full_iters} PCG iterations "
+                  f"({per_it * 1e3:.2f} ms/iteration)",
+    }
+
+
+if __name__ == "__main__":
+    main()

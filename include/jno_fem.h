@@ -180,6 +180,12 @@ size_t jno_pcg_workspace_size(int64_t n, int mixed);
 /* Iterations enqueued per host poll of the device-side convergence state (default 8).  No
  * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
 int jno_pcg_set_chunk(int chunk);
+/* Live kernel timing of the PCG iteration kernels (no reference counterpart: measurement hook).
+ * When enabled, HIP events are recorded on the solve's stream around every iteration kernel and
+ * the summed durations of the iterations that did work are accumulated; enabling resets them.
+ * K1 = SpMV + p/x update + p.q, K2 = r update + r.z, r.r. */
+int jno_pcg_profile(int enable);
+int jno_pcg_profile_read(double* h_k1_ms, int64_t* h_k1_count, double* h_k2_ms, int64_t* h_k2_count);
 int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
                 double tol, double atol, int64_t maxiter, jno_pcg_stats* h_stats, void* d_ws,
                 size_t ws_bytes, void* stream);

@@ -358,7 +358,8 @@ struct HostPoll {
 static thread_local HostPoll g_poll;
 
 template <typename T, typename V>
-static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s) {
+static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t after_k1,
+                       hipEvent_t after_k2) {
   if (op->format == JNO_FMT_DIA) {
     pcg_k1<1, true, T, V><<<grid, PCG_NT, 0, s>>>(a);
   } else {
@@ -374,10 +375,30 @@ static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t
     }
   }
   JNO_LAUNCH_CHECK();
+  if (after_k1) JNO_HIP(hipEventRecord(after_k1, s));
   pcg_k2<T, V><<<grid, PCG_NT, 0, s>>>(a);
   JNO_LAUNCH_CHECK();
+  if (after_k2) JNO_HIP(hipEventRecord(after_k2, s));
   return JNO_OK;
 }
+
+// Optional live kernel timing: HIP events recorded on the solve's stream between consecutive
+// K1/K2 launches (E0 K1 E1 K2 E2 K1 E3 ...), summed over the iterations that did work.
+struct KernelProf {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  double k1_ms = 0, k2_ms = 0;
+  int64_t k1_n = 0, k2_n = 0;
+  int ensure(size_t n) {
+    while (ev.size() < n) {
+      hipEvent_t e;
+      JNO_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    return JNO_OK;
+  }
+};
+static KernelProf g_prof;
 
 // Runs K1/K2 chunks until the device reports a stop; returns the final state in *out.
 template <typename T, typename V>
@@ -388,12 +409,24 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   const int grid = grid_for(work, PCG_NT, PCG_MAX_BLOCKS);
   int64_t k = k0;
   int slot = 0, pending = 0;
+  size_t nev = 0;
+  if (g_prof.on) {
+    if (g_prof.ensure(1) != JNO_OK) return JNO_EHIP;
+    JNO_HIP(hipEventRecord(g_prof.ev[nev++], s));
+  }
   for (;;) {
     for (int i = 0; i < chunk; ++i, ++k) {
       a.p_old = pbuf[k & 1];
       a.p_new = pbuf[(k + 1) & 1];
-      int rc = launch_iter<T, V>(op, a, grid, s);
-      if (rc) return rc;
+      if (g_prof.on) {
+        if (g_prof.ensure(nev + 2) != JNO_OK) return JNO_EHIP;
+        int rc = launch_iter<T, V>(op, a, grid, s, g_prof.ev[nev], g_prof.ev[nev + 1]);
+        nev += 2;
+        if (rc) return rc;
+      } else {
+        int rc = launch_iter<T, V>(op, a, grid, s, nullptr, nullptr);
+        if (rc) return rc;
+      }
     }
     JNO_HIP(hipMemcpyAsync(&g_poll.h[slot], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     JNO_HIP(hipEventRecord(g_poll.ev[slot], s));
@@ -408,6 +441,18 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   JNO_HIP(hipMemcpyAsync(&g_poll.h[0], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
   JNO_HIP(hipStreamSynchronize(s));
   *out = g_poll.h[0];
+  if (g_prof.on) {
+    const int64_t work_iters = out->k_stop - k0;
+    for (int64_t i = 0; i < work_iters && (size_t)(2 * i + 2) < nev; ++i) {
+      float t1 = 0, t2 = 0;
+      JNO_HIP(hipEventElapsedTime(&t1, g_prof.ev[2 * i], g_prof.ev[2 * i + 1]));
+      JNO_HIP(hipEventElapsedTime(&t2, g_prof.ev[2 * i + 1], g_prof.ev[2 * i + 2]));
+      g_prof.k1_ms += t1;
+      g_prof.k2_ms += t2;
+      g_prof.k1_n += 1;
+      g_prof.k2_n += 1;
+    }
+  }
   return JNO_OK;
 }
 
@@ -472,6 +517,21 @@ static int g_chunk = 8;
 using namespace jno;
 
 extern "C" size_t jno_pcg_workspace_size(int64_t n, int mixed) { return ws_layout(n, mixed ? 1 : 0, nullptr, nullptr); }
+
+extern "C" int jno_pcg_profile(int enable) {
+  g_prof.on = enable != 0;
+  g_prof.k1_ms = g_prof.k2_ms = 0;
+  g_prof.k1_n = g_prof.k2_n = 0;
+  return JNO_OK;
+}
+
+extern "C" int jno_pcg_profile_read(double* h_k1_ms, int64_t* h_k1_n, double* h_k2_ms, int64_t* h_k2_n) {
+  if (h_k1_ms) *h_k1_ms = g_prof.k1_ms;
+  if (h_k1_n) *h_k1_n = g_prof.k1_n;
+  if (h_k2_ms) *h_k2_ms = g_prof.k2_ms;
+  if (h_k2_n) *h_k2_n = g_prof.k2_n;
+  return JNO_OK;
+}
 
 extern "C" int jno_pcg_set_chunk(int chunk) {
   JNO_REQUIRE(chunk >= 1 && chunk <= 1024, JNO_EINVAL, "jno_pcg_set_chunk: chunk must be in 1..1024");

@@ -168,3 +168,23 @@ def test_adjoint_gradient_sign_on_a_compliance():
     u = O.direct_solve(A, b)
     g = O.adjoint_grad_k_p1(p, t, u, u, d)
     assert np.all(g <= 1e-18)
+
+
+def test_c_restatement_matches_numpy_oracle():
+    from oracle import cpu_fem
+
+    n = 12
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 6)
+    mask = np.zeros(p.shape[0], np.uint8)
+    mask[O.boundary_nodes(p, t)] = 1
+    s = cpu_fem.P1System(p, t, k, 1.0, mask)
+    A, b, d = O.poisson_p1_system(p, t, k)
+    A.sort_indices()
+    assert np.array_equal(s.rowptr, A.indptr) and np.array_equal(s.col, A.indices)
+    assert np.abs(s.vals - A.data).max() <= 1e-12 * np.abs(A.data).max()
+    assert np.abs(s.b - b).max() <= 1e-14
+    x, it, rel, _ = s.pcg(1e-10)
+    xo, ito = O.pcg(A, b, O.jacobi_inv(A), tol=1e-10)
+    assert abs(it - ito) <= 1 and rel <= 1e-10 * 1.001
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
