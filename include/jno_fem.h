@@ -207,6 +207,49 @@ int jno_adjoint_grad_k_p1(const double* d_points, const int32_t* d_cells, int64_
                           const double* d_u, const double* d_lam, const uint8_t* d_dmask,
                           double* d_grad, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * A2 P2 promotion.  Replaces _promote_to_degree (jno/utils/solver/fem_utils.py:1979-2044) with the
+ * basix P2 edge order (jno/utils/solver/fem_topology.py:44): vertices keep ids 0..N-1, edge nodes
+ * get N.. in cell-major / local-dof-minor first-appearance order.  Needs the P1 plan's incidence.
+ * d_cells10: i32[T*10]; d_points2: f64 capacity (N + 6T)*3 (only the first *h_n_nodes2 rows are
+ * written).
+ * ------------------------------------------------------------------------------------------- */
+size_t jno_promote_p2_workspace_size(int64_t n_cells);
+int jno_promote_p2(const double* d_points, const int32_t* d_cells, int64_t n_cells, int64_t n_nodes,
+                   const int32_t* d_inc_ptr, const int32_t* d_inc, int32_t* d_cells10,
+                   double* d_points2, int64_t* h_n_nodes2, void* d_ws, size_t ws_bytes, void* stream);
+
+/* Element coloring for the atomic-free scatter (no reference counterpart: the reference scatters
+ * with XLA's segment_sum, jno/utils/solver/fem_native.py:2326).  Cells sharing a vertex get
+ * different colors; d_order (T) lists cells grouped by color, h_color_ptr (max_colors+1) the
+ * group bounds.  The incidence is the plan's (entries e*nloc+a). */
+size_t jno_color_workspace_size(int64_t n_cells);
+int jno_color_cells(const int32_t* d_cells, int64_t n_cells, int nloc, const int32_t* d_inc_ptr,
+                    const int32_t* d_inc, int32_t* d_order, int32_t* h_color_ptr, int32_t max_colors,
+                    int32_t* h_ncolors, void* d_ws, size_t ws_bytes, void* stream);
+
+/* A4-A6, A8 for P2 vector (3 dof/node) linear elasticity: the element Jacobian of
+ * E_e (lam tr eps(u) tr eps(v) + 2 mu eps(u):eps(v)) (weak form of
+ * tests/test_topology_optimisation.py:59-66; jacfwd element kernel fem_native.py:2331-2345)
+ * contracted on MFMA (FP64 16x16x4) and scattered color by color into the CSR values d_vals (nnz)
+ * of the vec=3 plan.  lam, mu are the Lame parameters of E = 1 and Poisson ratio nu. */
+int jno_assemble_p2_elasticity(const double* d_points2, const int32_t* d_cells10, int64_t n_cells,
+                               const int32_t* d_rowptr, const int32_t* d_col, int64_t nnz,
+                               const double* d_E, double nu, const int32_t* d_order,
+                               const int32_t* h_color_ptr, int ncolors, double* d_vals,
+                               void* stream);
+
+/* A9 surface load: constant traction h_t[3] on the boundary facets lying on x[axis] == value,
+ * int_F t phi = |F|/3 t on the facet's edge nodes (_surf_elem_res, fem_native.py:1882-1960).
+ * Adds into d_b (3 dofs per node, node-major). */
+int jno_traction_p2(const double* d_points2, const int32_t* d_cells10, int64_t n_cells,
+                    const int32_t* d_inc_ptr, const int32_t* d_inc, int axis, double value,
+                    const double* h_t, double* d_b, void* stream);
+
+/* A18 for the compliance C = b.u (self-adjoint, lam = u): dC/dE_e = -u_e^T (K_e/E_e) u_e. */
+int jno_compliance_grad_p2(const double* d_points2, const int32_t* d_cells10, int64_t n_cells,
+                           double nu, const double* d_u, double* d_grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

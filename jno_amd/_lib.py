@@ -97,6 +97,13 @@ SIGNATURES = {
         [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, D, C.POINTER(JnoPcgStats), P, SZ, P],
     ),
     "jno_adjoint_grad_k_p1": (C.c_int, [P, P, I64, P, P, P, P, P]),
+    "jno_promote_p2_workspace_size": (SZ, [I64]),
+    "jno_promote_p2": (C.c_int, [P, P, I64, I64, P, P, P, P, C.POINTER(I64), P, SZ, P]),
+    "jno_color_workspace_size": (SZ, [I64]),
+    "jno_color_cells": (C.c_int, [P, I64, C.c_int, P, P, P, C.POINTER(I32), I32, C.POINTER(I32), P, SZ, P]),
+    "jno_assemble_p2_elasticity": (C.c_int, [P, P, I64, P, P, I64, P, D, P, C.POINTER(I32), C.c_int, P, P]),
+    "jno_traction_p2": (C.c_int, [P, P, I64, P, P, C.c_int, D, C.POINTER(D), P, P]),
+    "jno_compliance_grad_p2": (C.c_int, [P, P, I64, D, P, P, P]),
 }
 
 _lib = None

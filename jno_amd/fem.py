@@ -82,12 +82,19 @@ class CSR:
     def device(self):
         return self.vals.device
 
-    def mv(self, x: torch.Tensor) -> torch.Tensor:
-        x = x.to(torch.float64).contiguous()
+    def _spmv(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.detach().to(torch.float64).contiguous()
         y = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
         _lib.call("jno_spmv_f64", self.shape[0], _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.vals),
                   _lib.ptr(x), _lib.ptr(y), _lib.stream_ptr(self.device))
         return y
+
+    def mv(self, x: torch.Tensor) -> torch.Tensor:
+        """y = A x; differentiable in x (the operators assembled here are symmetric, so the
+        cotangent is A g)."""
+        if torch.is_grad_enabled() and x.requires_grad:
+            return _MatVec.apply(x, self)
+        return self._spmv(x)
 
     __matmul__ = mv
     __call__ = mv
@@ -142,6 +149,17 @@ class CSR:
         _lib.call("jno_drop_zeros_fill", n, _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.vals),
                   _lib.ptr(rowptr_o), _lib.ptr(col_o), _lib.ptr(vals_o), stream)
         return CSR(rowptr_o, col_o[: nnz.value], vals_o[: nnz.value], self.shape, self.dinv)
+
+
+class _MatVec(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, A: CSR):
+        ctx.A = A
+        return A._spmv(x)
+
+    @staticmethod
+    def backward(ctx, gy):
+        return ctx.A._spmv(gy), None
 
 
 def _as_k(k, T: int, device) -> torch.Tensor:

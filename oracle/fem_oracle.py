@@ -511,3 +511,43 @@ def mass_matrix_p1(points, cells, plan=None):
     n = points.shape[0]
     rowptr, col, vals = assemble(p1_mass(points, cells), cells, 1, n, plan)
     return csr_matrix(rowptr, col, vals, n)
+
+
+# ---------------------------------------------------------------------------------------------
+# P2 vector elasticity system (the 10-node, 3-dof-per-node configuration) and its compliance
+# gradient
+# ---------------------------------------------------------------------------------------------
+
+
+def plane_facets_p2(points, cells10, axis, value):
+    """P2 boundary facets (F,6) whose three vertices lie on the plane x[axis] == value."""
+    bf = boundary_facets(cells10)
+    on = np.all(points[bf[:, :3], axis] == value, axis=1)
+    return bf[on]
+
+
+def elasticity_p2_system(points, cells10, E, nu=0.3, traction=(0.0, 0.0, -1.0), clamp=(0, 0.0), load=(0, 1.0),
+                         plan=None):
+    """A u = b for E (lam tr eps tr eps + 2 mu eps:eps) with u = 0 on the clamp plane (all three
+    components) and the constant traction on the load plane (jno/utils/solver/fem_native.py:1882-1960
+    surface term; weak form of tests/test_topology_optimisation.py:59-66 with E per element)."""
+    n = points.shape[0] * 3
+    rowptr, col, vals = assemble(elasticity_p2(points, cells10, E, nu), cells10, 3, n, plan)
+    A = csr_matrix(rowptr, col, vals, n)
+    b = np.zeros(n)
+    tris = plane_facets_p2(points, cells10, *load)
+    contrib = facet_traction_p2(points, tris, traction)
+    for c in range(3):
+        np.add.at(b, tris.reshape(-1) * 3 + c, contrib[:, :, c].reshape(-1))
+    nodes = boundary_nodes(points, cells10)
+    nodes = nodes[points[nodes, clamp[0]] == clamp[1]]
+    dofs, dv = dirichlet_pairs(nodes, 3)
+    A, b = dirichlet_eliminate(A, b, dofs, dv)
+    return A, b, dofs
+
+
+def compliance_grad_p2(points, cells10, E, nu, u):
+    """dC/dE_e = -u_e^T (K_e/E_e) u_e for the compliance C = b.u (self-adjoint: lam = u)."""
+    Ke = elasticity_p2(points, cells10, np.ones(cells10.shape[0]), nu)
+    ue = np.asarray(u, float)[cell_dofs(cells10, 3)]
+    return -np.einsum("ti,tij,tj->t", ue, Ke, ue)

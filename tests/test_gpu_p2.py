@@ -1,0 +1,182 @@
+"""Parity of the HIP P2 elasticity path with the oracle: promotion (bit-exact), pattern (bit-exact),
+coloring validity, MFMA element matrices + colored scatter (1e-12), traction load, solves (1e-9) and
+the compliance gradient (oracle and central FD)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8])
+def test_promotion_bit_exact(dev, n):
+    import jno_amd as J
+    from jno_amd.elasticity import promote_p2
+
+    m2 = promote_p2(J.kuhn_box(n))
+    p2, c2 = O.promote_p2(*O.kuhn_box(n))
+    assert np.array_equal(_np(m2.cells), c2)
+    assert np.array_equal(_np(m2.points), p2)
+
+
+def test_promotion_non_unit_box(dev):
+    import jno_amd as J
+    from jno_amd.elasticity import promote_p2
+
+    lo, hi = (-0.3, 1.0, 2.0), (0.7, 1.9, 2.5)
+    m2 = promote_p2(J.Shape.box(*lo, *hi).structured((2, 3, 4)).domain())
+    p2, c2 = O.promote_p2(*O.kuhn_box(2, 3, 4, lo, hi))
+    assert np.array_equal(_np(m2.cells), c2) and np.array_equal(_np(m2.points), p2)
+
+
+def test_p2_boundary_and_pattern(dev):
+    import jno_amd as J
+    from jno_amd.elasticity import promote_p2
+
+    n = 2
+    m2 = promote_p2(J.kuhn_box(n))
+    p2, c2 = O.promote_p2(*O.kuhn_box(n))
+    assert np.array_equal(np.flatnonzero(_np(m2.boundary_mask())), O.boundary_nodes(p2, c2))
+    assert np.array_equal(np.flatnonzero(_np(m2.boundary_mask("left"))), O.boundary_nodes(p2, c2, "left"))
+    plan = m2.plan(3)
+    rp, col, _, _ = O.pattern_struct(c2, 3)
+    assert np.array_equal(_np(plan.rowptr), rp) and np.array_equal(_np(plan.col), col)
+    g = np.load(os.path.join(GOLDEN, "p2v3_pattern_n1.npz"))
+    plan1 = promote_p2(J.kuhn_box(1)).plan(3)
+    assert np.array_equal(_np(plan1.rowptr), g["rowptr"]) and np.array_equal(_np(plan1.col), g["col"])
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_p2_pattern_sha256(dev, n):
+    import hashlib
+
+    import jno_amd as J
+    from jno_amd.elasticity import promote_p2
+
+    meta = json.load(open(os.path.join(GOLDEN, "pattern_sha256.json")))[f"p2v3_n{n}"]
+    m2 = promote_p2(J.kuhn_box(n))
+    sha = lambda t: hashlib.sha256(np.ascontiguousarray(_np(t), dtype="<i4").tobytes()).hexdigest()  # noqa: E731
+    assert sha(m2.cells) == meta["cells_sha256"]
+    plan = m2.plan(3)
+    assert plan.nnz == meta["nnz"]
+    assert sha(plan.rowptr) == meta["rowptr_sha256"] and sha(plan.col) == meta["col_sha256"]
+
+
+def test_coloring_is_conflict_free(dev):
+    import jno_amd as J
+    from jno_amd.elasticity import ElasticitySystem
+
+    s = ElasticitySystem(J.kuhn_box(4))
+    cells = _np(s.mesh.cells)
+    order = _np(s.coloring.order)
+    seen = np.zeros(cells.shape[0], int)
+    for a, b in s.coloring.groups():
+        grp = order[a:b]
+        seen[grp] += 1
+        nodes = cells[grp].reshape(-1)
+        assert len(np.unique(nodes)) == len(nodes), "two cells of one color share a node"
+    assert np.all(seen == 1)
+    assert s.coloring.ncolors <= 128
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_elasticity_assembly(dev, n):
+    import jno_amd as J
+    from jno_amd.elasticity import ElasticitySystem
+
+    p2, c2 = O.promote_p2(*O.kuhn_box(n))
+    E = np.exp(0.3 * np.random.default_rng(0).standard_normal(c2.shape[0]))
+    s = ElasticitySystem(J.kuhn_box(n))
+    A, b = s.evaluate({"E": torch.tensor(E, device="cuda")})
+    Ao, bo, d = O.elasticity_p2_system(p2, c2, E)
+    Ao.sort_indices()
+    assert np.array_equal(_np(A.col), Ao.indices)
+    assert np.abs(_np(A.vals) - Ao.data).max() <= 1e-12 * np.abs(Ao.data).max()
+    assert np.abs(_np(b) - bo).max() <= 1e-15
+    D = _np(A.dense())
+    assert np.abs(D - D.T).max() <= 1e-13 * np.abs(D).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_elasticity_solve_and_compliance_gradient(dev, precision):
+    import jno_amd as J
+    from jno_amd import precond, solve
+    from jno_amd.elasticity import ElasticitySystem, compliance
+
+    n = 4
+    p2, c2 = O.promote_p2(*O.kuhn_box(n))
+    T = c2.shape[0]
+    E = np.exp(0.3 * np.random.default_rng(1).standard_normal(T))
+    s = ElasticitySystem(J.kuhn_box(n))
+    sv = solve.cg(tol=1e-10, precision=precision)
+    Et = torch.tensor(E, device="cuda", requires_grad=True)
+    Cval, u = compliance(s, Et, sv)
+    Cval.backward()
+    Ao, bo, d = O.elasticity_p2_system(p2, c2, E)
+    uo = O.direct_solve(Ao, bo)
+    assert _rel(_np(u), uo) <= 1e-9, sv.last_stats
+    assert abs(float(Cval) - bo @ uo) <= 1e-9 * abs(bo @ uo)
+    go = O.compliance_grad_p2(p2, c2, E, 0.3, uo)
+    assert _rel(_np(Et.grad), go) <= 1e-8
+    # central FD with defect correction on seeded elements
+    fine = solve.cg(tol=1e-13)
+    rng = np.random.default_rng(5)
+    for el in rng.choice(T, 4, replace=False):
+        h = 1e-6 * E[el]
+        Cs = []
+        for sg in (1, -1):
+            EE = torch.tensor(E, device="cuda")
+            EE[el] += sg * h
+            A2, b2 = s.evaluate({"E": EE})
+            du = fine(A2, b2 - A2.mv(u.detach()), M=precond.jacobi())
+            Cs.append(float(torch.dot(b2, u.detach() + du)))
+        fd = (Cs[0] - Cs[1]) / (2 * h)
+        g = float(Et.grad[el])
+        assert abs(fd - g) <= 1e-6 * abs(g), (el, fd, g)
+
+
+def test_c5_32_elasticity(dev):
+    import jno_amd as J
+    from jno_amd import precond, solve
+    from jno_amd.elasticity import ElasticitySystem
+
+    n = 32
+    s = ElasticitySystem(J.kuhn_box(n))
+    m = s.mesh
+    assert m.n_nodes == 274_625 and m.n_cells == 196_608
+    assert s.ndof == 823_875
+    assert s.plan.nnz == 69_108_489
+    assert int((s.dmask > 0).sum()) == 3 * 65**2
+    E = torch.tensor(np.exp(0.3 * np.random.default_rng(0).standard_normal(m.n_cells)), device="cuda")
+    A, b = s.evaluate({"E": E})
+    assert abs(float(b.sum()) + 1.0) < 1e-12  # unit traction on the unit face
+    mixed = solve.cg(tol=1e-10, precision="mixed")
+    u = mixed(A, b, M=precond.jacobi())
+    assert mixed.last_stats["relres"] <= 1e-10
+    f64 = solve.cg(tol=1e-10)
+    u64 = f64(A, b, M=precond.jacobi())
+    assert f64.last_stats["relres"] <= 1e-10
+    err = _rel(_np(u), _np(u64))
+    print(f"\nC5 32^3 P2: mixed {mixed.last_stats['iterations']} it ({mixed.last_stats['refinements']} repl), "
+          f"fp64 {f64.last_stats['iterations']} it, mixed vs fp64 {err:.2e}, format {mixed.last_stats['operator']}")
+    assert err <= 1e-8
+    g = s.compliance_grad(u)
+    assert bool((g <= 0).all())
+    # energy identity: C = b.u = u^T A u; sum_e E_e dC/dE_e = -u^T A u on the free dofs
+    C = float(torch.dot(b, u))
+    assert abs(float(torch.dot(u, A.mv(u))) - C) <= 1e-8 * abs(C)
+    assert abs(float(torch.dot(E, g)) + C) <= 1e-8 * abs(C)
