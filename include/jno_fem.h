@@ -198,6 +198,27 @@ int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, d
                   jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
+ * A20  batched solves.  Replaces jax.vmap over the cg slot (LinearSolver traits vmap="native",
+ * jno/utils/solver/solver_api.py:275-280; tests/test_fem_solver_slots.py:123-132), generalised to
+ * systems of different sizes and operators: every item runs its own Jacobi-PCG recurrence (FP32
+ * inner iterations, FP64 residual replacement, stop on the true FP64 residual) and its own device
+ * state machine, so items do not wait for each other.  Items may share one operator (many right
+ * hand sides).  steps_per_graph > 0 captures that many (iteration or replacement) steps in a
+ * hipGraph and replays it; 0 launches the kernels directly.  h_stats receives one entry per item.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct jno_batch_item {
+  jno_op op;           /* needs vals32 and vals64                          */
+  const double* dinv;  /* Jacobi inverse diagonal (FP64)                   */
+  const double* b;     /* right-hand side                                  */
+  double* x;           /* solution (x0 = 0)                                */
+} jno_batch_item;
+
+size_t jno_batch_workspace_size(int n_items, const jno_batch_item* h_items);
+int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, double tol, double atol,
+                        int64_t maxiter, double replace_factor, int steps_per_graph,
+                        jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------
  * A18  adjoint contraction.  Replaces the VJP chain of custom_linear_solve(symmetric=True) ->
  * BCOO data -> Dirichlet mask -> segment_sum -> jacfwd (jno/utils/solver/fem_native.py:2326-2345,
  * jno/utils/solver/fem_utils.py:3332-3336):  d_grad[e] = -|T_e| (G_e P lam_e).(G_e u_e) where P zeroes

@@ -51,6 +51,10 @@ class JnoPcgStats(C.Structure):
     ]
 
 
+class JnoBatchItem(C.Structure):
+    _fields_ = [("op", JnoOp), ("dinv", C.c_void_p), ("b", C.c_void_p), ("x", C.c_void_p)]
+
+
 class JnoError(RuntimeError):
     """A failed call into libjno_fem (carries the status code)."""
 
@@ -97,6 +101,11 @@ SIGNATURES = {
         [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, D, C.POINTER(JnoPcgStats), P, SZ, P],
     ),
     "jno_adjoint_grad_k_p1": (C.c_int, [P, P, I64, P, P, P, P, P]),
+    "jno_batch_workspace_size": (SZ, [C.c_int, C.POINTER(JnoBatchItem)]),
+    "jno_batch_pcg_mixed": (
+        C.c_int,
+        [C.c_int, C.POINTER(JnoBatchItem), D, D, I64, D, C.c_int, C.POINTER(JnoPcgStats), P, SZ, P],
+    ),
     "jno_promote_p2_workspace_size": (SZ, [I64]),
     "jno_promote_p2": (C.c_int, [P, P, I64, I64, P, P, P, P, C.POINTER(I64), P, SZ, P]),
     "jno_color_workspace_size": (SZ, [I64]),

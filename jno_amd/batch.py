@@ -1,0 +1,78 @@
+"""Batched Jacobi-PCG: many systems of different sizes, or many right-hand sides on one operator,
+in one device-driven solve (FP32 inner iterations, FP64 residual replacement, true-residual stop).
+
+The reference batches with ``jax.vmap`` over the cg slot (LinearSolver traits ``vmap="native"``,
+jno/utils/solver/solver_api.py:275-280; tests/test_fem_solver_slots.py:123-132), which runs every
+member until the slowest converges and needs one operator shape.  Here every system keeps its own
+recurrence and state on the device, so systems of different meshes share one launch sequence
+without waiting for each other.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from .fem import CSR
+from .operator import SolverOperator
+
+
+class BatchSolver:
+    def __init__(self, tol: float = 1e-10, atol: float = 0.0, maxiter: int = 20_000, replace_factor: float = 0.1,
+                 steps_per_graph: int = 8, format: str = "auto"):
+        self.tol, self.atol, self.maxiter = float(tol), float(atol), int(maxiter)
+        self.replace_factor = float(replace_factor)
+        self.steps_per_graph = int(steps_per_graph)
+        self.format = format
+        self.last_stats: List[dict] = []
+
+    def _op(self, A: CSR) -> SolverOperator:
+        key = ("solver_op", self.format, True)
+        op = A._cache.get(key)
+        if op is None:
+            op = SolverOperator(A, fmt=self.format, fp32=True)
+            A._cache[key] = op
+        return op
+
+    def __call__(self, systems: Sequence[Tuple[CSR, torch.Tensor]]) -> List[torch.Tensor]:
+        """Solve A_i x_i = b_i for every (A_i, b_i); an operator object may repeat (shared)."""
+        if not systems:
+            return []
+        items = (_lib.JnoBatchItem * len(systems))()
+        xs, keep = [], []
+        dev = systems[0][0].device
+        for i, (A, b) in enumerate(systems):
+            _lib.require_gpu(A.vals, f"A[{i}]")
+            b = torch.as_tensor(b).reshape(-1).to(device=A.device, dtype=torch.float64).contiguous()
+            if b.shape[0] != A.shape[0]:
+                raise ValueError(f"system {i}: b has {b.shape[0]} entries for {A.shape[0]} rows")
+            op = self._op(A)
+            dinv = A.diag_inv()
+            x = torch.empty_like(b)
+            items[i].op = op.op
+            items[i].dinv = dinv.data_ptr()
+            items[i].b = b.data_ptr()
+            items[i].x = x.data_ptr()
+            xs.append(x)
+            keep += [b, dinv, op]
+        lib = _lib.load()
+        wsb = lib.jno_batch_workspace_size(len(systems), items)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        stats = (_lib.JnoPcgStats * len(systems))()
+        rc = lib.jno_batch_pcg_mixed(len(systems), items, self.tol, self.atol, self.maxiter, self.replace_factor,
+                                     self.steps_per_graph, stats, _lib.ptr(ws), wsb, _lib.stream_ptr(dev))
+        self.last_stats = [
+            {"iterations": int(s.iterations), "refinements": int(s.refinements), "relres": float(s.relres),
+             "seconds": float(s.seconds)} for s in stats
+        ]
+        if rc not in (_lib.JNO_OK, _lib.JNO_ENOTCONV, _lib.JNO_ENONFINITE):
+            _lib.check(rc, "jno_batch_pcg_mixed")
+        bad = [i for i, s in enumerate(self.last_stats) if not (s["relres"] <= 1e-4)]
+        if bad:
+            raise RuntimeError(
+                f"jno.solve.cg did not solve {len(bad)} of {len(systems)} batched systems (first: {bad[0]}, "
+                f"relative residual {self.last_stats[bad[0]]['relres']:.1e})"
+            )
+        return xs

@@ -30,10 +30,14 @@ __device__ __forceinline__ TetGeom tet_geom(const double* __restrict__ pts, int4
     e2[i] = p2[i] - p0[i];
     e3[i] = p3[i] - p0[i];
   }
-  double c23[3] = {e2[1] * e3[2] - e2[2] * e3[1], e2[2] * e3[0] - e2[0] * e3[2], e2[0] * e3[1] - e2[1] * e3[0]};
-  double c31[3] = {e3[1] * e1[2] - e3[2] * e1[1], e3[2] * e1[0] - e3[0] * e1[2], e3[0] * e1[1] - e3[1] * e1[0]};
-  double c12[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-  const double det = e1[0] * c23[0] + e1[1] * c23[1] + e1[2] * c23[2];
+  // cross products rounded term by term: a zero component of an axis-aligned edge stays an exact
+  // zero, so couplings that vanish in exact arithmetic (the non-axis edges of a Kuhn lattice)
+  // come out exactly 0.0 at any spacing
+  double c23[3], c31[3], c12[3];
+  cross_rn(e2, e3, c23);
+  cross_rn(e3, e1, c31);
+  cross_rn(e1, e2, c12);
+  const double det = __dadd_rn(__dadd_rn(__dmul_rn(e1[0], c23[0]), __dmul_rn(e1[1], c23[1])), __dmul_rn(e1[2], c23[2]));
   const double inv = 1.0 / det;
   TetGeom t;
 #pragma unroll
@@ -77,7 +81,10 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
         const double coef = kcoef[e] * t.vol;
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
-          const double v = coef * (t.g[a][0] * t.g[bb][0] + t.g[a][1] * t.g[bb][1] + t.g[a][2] * t.g[bb][2]);
+          // rounded term by term (no FMA contraction) so that K[r][c] and K[c][r] are bitwise equal
+          const double dot = __dadd_rn(__dadd_rn(__dmul_rn(t.g[a][0], t.g[bb][0]), __dmul_rn(t.g[a][1], t.g[bb][1])),
+                                       __dmul_rn(t.g[a][2], t.g[bb][2]));
+          const double v = __dmul_rn(coef, dot);
 #pragma unroll
           for (int s = 0; s < MAXR; ++s) acc[s] += (cols[s] == nb[bb]) ? v : 0.0;
         }

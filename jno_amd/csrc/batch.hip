@@ -1,0 +1,530 @@
+// Batched Jacobi-PCG: many independent systems (uneven sizes, each with its own operator, or one
+// shared operator with many right-hand sides) solved together with FP32 inner iterations and FP64
+// residual replacement.
+//
+// Every system ("item") runs its own recurrence and its own small state machine on the device.
+// A step is the fixed kernel pair (KA, KB) over one flattened table of (item, row block) entries:
+//   KA  reduces the item's partial sums left by the previous KB and decides the item's step:
+//         ITER     p = z + beta p_old, deferred x update, q = A32 p, partial p.q
+//         REPLACE  (the FP32 residual dropped by replace_factor^2, met the tolerance or the cap)
+//                  x64 += xs + alpha p_old, xs = 0
+//         DONE     (after a REPLACE whose true FP64 residual met the tolerance, or at the cap)
+//   KB  ITER     r -= alpha q (alpha from the reduced p.q), partials r.z, r.r
+//       REPLACE  r64 = b - A64 x64, rs = (float) r64, partials ||r64||^2, r.z, r.r, b.b
+// Each entry reduces its item's partials itself in a fixed order, so no value crosses workgroups
+// inside a kernel (no atomics or cache-flushing fences on the step path) and all entries of an
+// item take the same decision.  Per-item fields are written by the item's first entry only, into
+// fields that kernel does not read.  The step arguments are constant, so a chunk of steps is
+// captured once in a hipGraph and replayed; items do not wait for each other.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "sparse.h"
+
+namespace jno {
+
+constexpr int BT_NT = 256;
+constexpr int BT_RPB = 1024;   // rows per table entry
+constexpr int BT_GRID = 2048;  // grid-stride workgroups per launch
+
+enum { PH_ITER = 0, PH_REPLACE = 1, PH_DONE = 2 };
+enum { BS_RUNNING = 0, BS_CONVERGED = 1, BS_MAXITER = 2, BS_NONFINITE = 3 };
+
+struct BItem {
+  int64_t n, off;
+  int32_t format, ndiag;
+  int32_t offs[JNO_DIA_MAX];
+  const int32_t* rowptr;
+  const int32_t* col;
+  const float* v32;
+  const double* v64;
+  const double* b;
+  double* x;
+  int32_t e0, ne;  // entry range in the table
+};
+
+struct BState {
+  // written by KA's first entry, read by KB
+  int32_t phase_b;
+  int32_t done;       // also read by KA as a fast exit (its own decision agrees)
+  double gamma_cur;   // gamma_k
+  // written by KA's first entry when it closes a REPLACE, read by later KAs
+  double atol2, rr_replace, rr_true, bb, rr_last;
+  int64_t refinements;
+  int32_t status;
+  // written by KB's first entry, read by KA
+  int32_t last_kb;    // kind of the previous KB (its partials)
+  int64_t k;
+  double alpha, gamma_prev;
+};
+
+struct BArgs {
+  const BItem* items;
+  BState* st;
+  const int2* table;  // (item, local block)
+  int32_t n_entries;
+  float* rs;
+  float* p[2];
+  float* q;
+  float* xs;
+  float* dinv;
+  double* PA;         // [n_entries]       KA: p.q
+  double* PB;         // [4][n_entries]    KB: ITER (r.z, r.r) / REPLACE (rr_true, r.z, r.r, b.b)
+  int32_t* active;
+  double tol2, atol2_abs, repl2;
+  int64_t maxiter;
+};
+
+template <typename T, typename V>
+__device__ __forceinline__ T mat_row(const BItem& it, int64_t i, const V* vals, const T* x, bool dia) {
+  if (dia) {
+    T s = (T)vals[i] * x[i];
+    for (int d = 1; d < it.ndiag; ++d) {
+      const int64_t o = it.offs[d];
+      const V* v = vals + (int64_t)d * it.n;
+      if (i + o < it.n) s += (T)v[i] * x[i + o];
+      if (i - o >= 0) s += (T)v[i - o] * x[i - o];
+    }
+    return s;
+  }
+  T s = 0;
+  for (int j = it.rowptr[i]; j < it.rowptr[i + 1]; ++j) s += (T)vals[j] * x[it.col[j]];
+  return s;
+}
+
+// item total of one partial array, fixed order; every thread gets it
+__device__ __forceinline__ double item_sum(const double* __restrict__ P, const BItem& it, double* sm) {
+  double s = 0.0;
+  for (int e = threadIdx.x; e < it.ne; e += BT_NT) s += P[it.e0 + e];
+  return block_sum<BT_NT>(s, sm);
+}
+
+__device__ __forceinline__ void put(double v, double* __restrict__ P, int entry, double* sm) {
+  v = block_sum<BT_NT>(v, sm);
+  if (threadIdx.x == 0) P[entry] = v;
+}
+
+template <bool DIA>
+__global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
+  __shared__ double sm[BT_NT / 64];
+  const int64_t NE = a.n_entries;
+  for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
+    const int2 te = a.table[entry];
+    const BItem& it = a.items[te.x];
+    if ((it.format == JNO_FMT_DIA) != DIA) continue;
+    BState* S = a.st + te.x;
+    if (S->done) continue;  // entry-uniform
+    const bool first = te.y == 0;
+    const int64_t k = S->k;
+    // ---- decide this step from the previous KB's partials
+    int phase;
+    double gamma, rr;
+    if (S->last_kb == PH_ITER) {
+      gamma = item_sum(a.PB, it, sm);
+      rr = item_sum(a.PB + NE, it, sm);
+      const bool bad = !isfinite(gamma) || !isfinite(rr);
+      phase = (bad || rr <= S->rr_replace || !(rr > S->atol2) || k >= a.maxiter) ? PH_REPLACE : PH_ITER;
+      if (bad && first && threadIdx.x == 0) S->status = BS_NONFINITE;
+    } else {  // the previous KB formed the true residual
+      const double rt = item_sum(a.PB, it, sm);
+      gamma = item_sum(a.PB + NE, it, sm);
+      rr = item_sum(a.PB + 2 * NE, it, sm);
+      const double bb = item_sum(a.PB + 3 * NE, it, sm);
+      const double atol2 = fmax(a.tol2 * bb, a.atol2_abs);
+      const bool bad = !isfinite(rt) || S->status == BS_NONFINITE;
+      if (!(rt > atol2)) {
+        phase = PH_DONE;
+      } else if (bad || k >= a.maxiter) {
+        phase = PH_DONE;
+      } else {
+        phase = PH_ITER;
+      }
+      if (first && threadIdx.x == 0) {
+        S->atol2 = atol2;
+        S->rr_replace = a.repl2 * rr;
+        S->rr_true = rt;
+        S->bb = bb;
+        S->rr_last = rr;
+        S->refinements = S->refinements + 1;
+        if (phase == PH_DONE) {
+          S->status = !(rt > atol2) ? (S->status == BS_NONFINITE ? BS_NONFINITE : BS_CONVERGED)
+                                    : (bad ? BS_NONFINITE : BS_MAXITER);
+          S->done = 1;
+          atomicSub(a.active, 1);
+        }
+      }
+    }
+    if (first && threadIdx.x == 0) {
+      S->phase_b = phase;
+      S->gamma_cur = gamma;
+    }
+    if (phase == PH_DONE) continue;
+    const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
+    const float* pold = a.p[k & 1] + it.off;
+    float* xs = a.xs + it.off;
+    const double alpha_prev = S->alpha;
+    const float ap = (float)alpha_prev;
+    if (phase == PH_REPLACE) {
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
+        float d = xs[i];
+        if (k > 0 && alpha_prev != 0.0) d += ap * pold[i];
+        it.x[i] += (double)d;
+        xs[i] = 0.0f;
+      }
+      continue;
+    }
+    float* pnew = a.p[(k + 1) & 1] + it.off;
+    const float beta = (k == 0) ? 0.0f : (float)(gamma / S->gamma_prev);
+    const float* r = a.rs + it.off;
+    const float* dv = a.dinv + it.off;
+    float* q = a.q + it.off;
+    double part = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
+      const float po = pold[i];
+      const float pi = dv[i] * r[i] + beta * po;
+      float s;
+      if (DIA) {
+        s = it.v32[i] * pi;
+        for (int d = 1; d < it.ndiag; ++d) {
+          const int64_t o = it.offs[d];
+          const float* v = it.v32 + (int64_t)d * it.n;
+          if (i + o < it.n) s += v[i] * (dv[i + o] * r[i + o] + beta * pold[i + o]);
+          if (i - o >= 0) s += v[i - o] * (dv[i - o] * r[i - o] + beta * pold[i - o]);
+        }
+      } else {
+        s = 0.0f;
+        for (int j = it.rowptr[i]; j < it.rowptr[i + 1]; ++j) {
+          const int c = it.col[j];
+          s += it.v32[j] * (dv[c] * r[c] + beta * pold[c]);
+        }
+      }
+      pnew[i] = pi;
+      q[i] = s;
+      if (k > 0) xs[i] += ap * po;
+      part += (double)pi * (double)s;
+    }
+    put(part, a.PA, entry, sm);
+  }
+}
+
+template <bool DIA>
+__global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
+  __shared__ double sm[BT_NT / 64];
+  const int64_t NE = a.n_entries;
+  for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
+    const int2 te = a.table[entry];
+    const BItem& it = a.items[te.x];
+    if ((it.format == JNO_FMT_DIA) != DIA) continue;
+    BState* S = a.st + te.x;
+    const int phase = S->phase_b;
+    if (phase == PH_DONE) continue;
+    const bool first = te.y == 0;
+    const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
+    float* r = a.rs + it.off;
+    const float* dv = a.dinv + it.off;
+    if (phase == PH_ITER) {
+      const double pq = item_sum(a.PA, it, sm);
+      const double gamma = S->gamma_cur;
+      const double alpha = gamma / pq;
+      const float al = (float)alpha;
+      const float* q = a.q + it.off;
+      double pg = 0, pr = 0;
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
+        const float ri = r[i] - al * q[i];
+        r[i] = ri;
+        pg += (double)ri * (double)(dv[i] * ri);
+        pr += (double)ri * (double)ri;
+      }
+      put(pg, a.PB, entry, sm);
+      put(pr, a.PB + NE, entry, sm);
+      if (first && threadIdx.x == 0) {
+        S->alpha = alpha;
+        S->gamma_prev = gamma;
+        S->k = S->k + 1;
+        S->last_kb = PH_ITER;
+      }
+    } else {  // PH_REPLACE: true residual in FP64
+      double prt = 0, pg = 0, pr = 0, pb = 0;
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
+        const double bi = it.b[i];
+        const double ri = bi - mat_row<double, double>(it, i, it.v64, it.x, DIA);
+        const float rf = (float)ri;
+        r[i] = rf;
+        prt += ri * ri;
+        pb += bi * bi;
+        pg += (double)rf * (double)(dv[i] * rf);
+        pr += (double)rf * (double)rf;
+      }
+      put(prt, a.PB, entry, sm);
+      put(pg, a.PB + NE, entry, sm);
+      put(pr, a.PB + 2 * NE, entry, sm);
+      put(pb, a.PB + 3 * NE, entry, sm);
+      if (first && threadIdx.x == 0) {
+        S->alpha = 0.0;  // the deferred x update was applied by KA
+        S->last_kb = PH_REPLACE;
+      }
+    }
+  }
+}
+
+__global__ void batch_init_k(BArgs a, int n_items) {
+  // per entry: zero xs, p, x
+  for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
+    const int2 te = a.table[entry];
+    const BItem& it = a.items[te.x];
+    const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+      it.x[i] = 0.0;
+      a.xs[it.off + i] = 0.0f;
+      a.p[0][it.off + i] = 0.0f;
+      a.p[1][it.off + i] = 0.0f;
+    }
+  }
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n_items) {
+    BState s = {};
+    s.phase_b = PH_REPLACE;  // the first KB forms r = b, the norms and the tolerances
+    s.last_kb = PH_REPLACE;
+    s.status = BS_RUNNING;
+    s.refinements = -1;
+    a.st[g] = s;
+  }
+}
+
+__global__ void batch_dinv_k(const BItem* items, const double* const* dinvs, const int2* table, int n_entries,
+                             float* dinv32) {
+  for (int entry = blockIdx.x; entry < n_entries; entry += gridDim.x) {
+    const int2 te = table[entry];
+    const BItem& it = items[te.x];
+    const double* d = dinvs[te.x];
+    const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) dinv32[it.off + i] = (float)d[i];
+  }
+}
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct BLayout {
+  BItem* items;
+  BState* st;
+  int2* table;
+  const double** dinvs;
+  double* PA;
+  double* PB;
+  int32_t* active;
+  float *rs, *p0, *p1, *q, *xs, *dinv;
+  size_t bytes;
+};
+
+static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, char* base) {
+  BLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t b) -> char* {
+    char* p = base ? base + off : nullptr;
+    off += a256(b);
+    return p;
+  };
+  L.items = (BItem*)take(sizeof(BItem) * n_items);
+  L.st = (BState*)take(sizeof(BState) * n_items);
+  L.table = (int2*)take(sizeof(int2) * n_entries);
+  L.dinvs = (const double**)take(sizeof(double*) * n_items);
+  L.PA = (double*)take(sizeof(double) * n_entries);
+  L.PB = (double*)take(sizeof(double) * 4 * n_entries);
+  L.active = (int32_t*)take(256);
+  L.rs = (float*)take(4 * ntot);
+  L.p0 = (float*)take(4 * ntot);
+  L.p1 = (float*)take(4 * ntot);
+  L.q = (float*)take(4 * ntot);
+  L.xs = (float*)take(4 * ntot);
+  L.dinv = (float*)take(4 * ntot);
+  L.bytes = off;
+  return L;
+}
+
+static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot) {
+  int64_t ne = 0, nt = 0;
+  for (int i = 0; i < n; ++i) {
+    ne += cdiv(items[i].op.n, BT_RPB);
+    nt += items[i].op.n;
+  }
+  if (ntot) *ntot = nt;
+  return ne;
+}
+
+}  // namespace jno
+
+using namespace jno;
+
+extern "C" size_t jno_batch_workspace_size(int n_items, const jno_batch_item* h_items) {
+  if (n_items <= 0 || !h_items) return 0;
+  int64_t ntot;
+  const int64_t ne = count_entries(n_items, h_items, &ntot);
+  return blayout(n_items, ne, ntot, nullptr).bytes;
+}
+
+extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, double tol, double atol, int64_t maxiter,
+                                   double replace_factor, int steps_per_graph, jno_pcg_stats* h_stats, void* d_ws,
+                                   size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(n_items > 0 && h_items && h_stats, JNO_EINVAL, "jno_batch_pcg_mixed: empty batch");
+  JNO_REQUIRE(replace_factor > 0.0 && replace_factor < 1.0, JNO_EINVAL, "jno_batch_pcg_mixed: replace_factor in (0,1)");
+  JNO_REQUIRE(maxiter >= 0 && steps_per_graph >= 0, JNO_EINVAL, "jno_batch_pcg_mixed: bad maxiter/steps");
+  int64_t ntot;
+  const int64_t ne = count_entries(n_items, h_items, &ntot);
+  JNO_REQUIRE(ne < INT32_MAX, JNO_EINVAL, "jno_batch_pcg_mixed: batch too large");
+  JNO_REQUIRE(ws_bytes >= jno_batch_workspace_size(n_items, h_items), JNO_EWORKSPACE,
+              "jno_batch_pcg_mixed: workspace too small");
+  hipStream_t s = as_stream(stream);
+  BLayout L = blayout(n_items, ne, ntot, static_cast<char*>(d_ws));
+  // host staging of the item descriptors and the entry table
+  std::vector<BItem> items(n_items);
+  std::vector<int2> table(ne);
+  std::vector<const double*> dinvs(n_items);
+  int64_t off = 0, e = 0;
+  bool any_dia = false, any_csr = false;
+  for (int i = 0; i < n_items; ++i) {
+    const jno_batch_item& h = h_items[i];
+    JNO_REQUIRE(h.op.vals32 && h.op.vals64 && h.dinv && h.b && h.x, JNO_EINVAL,
+                "jno_batch_pcg_mixed: item %d lacks FP32/FP64 values or vectors", i);
+    JNO_REQUIRE(h.op.format == JNO_FMT_DIA || (h.op.format == JNO_FMT_CSR && h.op.rowptr && h.op.col), JNO_EINVAL,
+                "jno_batch_pcg_mixed: item %d has a bad operator", i);
+    JNO_REQUIRE(h.op.n > 0, JNO_EINVAL, "jno_batch_pcg_mixed: item %d is empty", i);
+    BItem& it = items[i];
+    it.n = h.op.n;
+    it.off = off;
+    it.format = h.op.format;
+    it.ndiag = h.op.ndiag;
+    for (int d = 0; d < JNO_DIA_MAX; ++d) it.offs[d] = h.op.offsets[d];
+    it.rowptr = h.op.rowptr;
+    it.col = h.op.col;
+    it.v32 = h.op.vals32;
+    it.v64 = h.op.vals64;
+    it.b = h.b;
+    it.x = h.x;
+    it.e0 = (int32_t)e;
+    it.ne = (int32_t)cdiv(it.n, BT_RPB);
+    for (int j = 0; j < it.ne; ++j) table[e++] = make_int2(i, j);
+    dinvs[i] = h.dinv;
+    off += it.n;
+    any_dia |= it.format == JNO_FMT_DIA;
+    any_csr |= it.format == JNO_FMT_CSR;
+  }
+  JNO_HIP(hipMemcpyAsync(L.items, items.data(), sizeof(BItem) * n_items, hipMemcpyHostToDevice, s));
+  JNO_HIP(hipMemcpyAsync(L.table, table.data(), sizeof(int2) * ne, hipMemcpyHostToDevice, s));
+  JNO_HIP(hipMemcpyAsync(L.dinvs, dinvs.data(), sizeof(double*) * n_items, hipMemcpyHostToDevice, s));
+  const int32_t h_active = n_items;
+  JNO_HIP(hipMemcpyAsync(L.active, &h_active, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  BArgs a;
+  a.items = L.items;
+  a.st = L.st;
+  a.table = L.table;
+  a.n_entries = (int32_t)ne;
+  a.rs = L.rs;
+  a.p[0] = L.p0;
+  a.p[1] = L.p1;
+  a.q = L.q;
+  a.xs = L.xs;
+  a.dinv = L.dinv;
+  a.PA = L.PA;
+  a.PB = L.PB;
+  a.active = L.active;
+  a.tol2 = tol * tol;
+  a.atol2_abs = atol * atol;
+  a.repl2 = replace_factor * replace_factor;
+  a.maxiter = maxiter;
+  const int grid = (int)std::min<int64_t>(ne, BT_GRID);
+  batch_init_k<<<(unsigned)std::max<int64_t>(grid, cdiv(n_items, BT_NT)), BT_NT, 0, s>>>(a, n_items);
+  JNO_LAUNCH_CHECK();
+  batch_dinv_k<<<grid, BT_NT, 0, s>>>(L.items, L.dinvs, L.table, (int)ne, L.dinv);
+  JNO_LAUNCH_CHECK();
+  JNO_HIP(hipStreamSynchronize(s));  // host staging vectors go out of scope after this call
+
+  auto launch_kb = [&](hipStream_t st) -> int {
+    if (any_dia) batch_kb<true><<<grid, BT_NT, 0, st>>>(a);
+    if (any_csr) batch_kb<false><<<grid, BT_NT, 0, st>>>(a);
+    JNO_LAUNCH_CHECK();
+    return JNO_OK;
+  };
+  auto launch_step = [&](hipStream_t st) -> int {
+    if (any_dia) batch_ka<true><<<grid, BT_NT, 0, st>>>(a);
+    if (any_csr) batch_ka<false><<<grid, BT_NT, 0, st>>>(a);
+    JNO_LAUNCH_CHECK();
+    return launch_kb(st);
+  };
+
+  int rc = launch_kb(s);  // the first true residual r = b
+  if (rc) return rc;
+  int32_t* h_act = nullptr;
+  JNO_HIP(hipHostMalloc((void**)&h_act, 2 * sizeof(int32_t), hipHostMallocDefault));
+  hipEvent_t ev[2];
+  JNO_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  JNO_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  const int chunk = steps_per_graph > 0 ? steps_per_graph : 8;
+  if (steps_per_graph > 0) {
+    // capture one chunk of steps on a private stream (default priority), replay it on `s`
+    JNO_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    JNO_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk && rc == JNO_OK; ++i) rc = launch_step(cap);
+    hipError_t ce = hipStreamEndCapture(cap, &graph);
+    if (rc) return rc;
+    JNO_HIP(ce);
+    JNO_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  int slot = 0, pending = 0;
+  int64_t launched = 0;
+  // every item is DONE after at most maxiter iterations plus one replacement per iteration
+  const int64_t max_steps = 2 * (maxiter + 2) + 4 * (int64_t)chunk;
+  for (;;) {
+    if (exec) {
+      JNO_HIP(hipGraphLaunch(exec, s));
+    } else {
+      for (int i = 0; i < chunk; ++i) {
+        rc = launch_step(s);
+        if (rc) return rc;
+      }
+    }
+    launched += chunk;
+    JNO_HIP(hipMemcpyAsync(&h_act[slot], L.active, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipEventRecord(ev[slot], s));
+    if (pending) {
+      JNO_HIP(hipEventSynchronize(ev[slot ^ 1]));
+      if (h_act[slot ^ 1] <= 0) break;
+    }
+    if (launched > max_steps) break;
+    pending = 1;
+    slot ^= 1;
+  }
+  JNO_HIP(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::vector<BState> st(n_items);
+  JNO_HIP(hipMemcpy(st.data(), L.st, sizeof(BState) * n_items, hipMemcpyDeviceToHost));
+  if (exec) {
+    JNO_HIP(hipGraphExecDestroy(exec));
+    JNO_HIP(hipGraphDestroy(graph));
+    JNO_HIP(hipStreamDestroy(cap));
+  }
+  JNO_HIP(hipEventDestroy(ev[0]));
+  JNO_HIP(hipEventDestroy(ev[1]));
+  JNO_HIP(hipHostFree(h_act));
+  int worst = JNO_OK;
+  for (int i = 0; i < n_items; ++i) {
+    const double bn = sqrt(st[i].bb);
+    h_stats[i].iterations = st[i].k;
+    h_stats[i].refinements = st[i].refinements;
+    h_stats[i].relres = bn > 0 ? sqrt(st[i].rr_true) / bn : sqrt(st[i].rr_true);
+    h_stats[i].relres_recursive = bn > 0 ? sqrt(st[i].rr_last) / bn : sqrt(st[i].rr_last);
+    h_stats[i].seconds = secs;
+    if (st[i].status == BS_NONFINITE) worst = JNO_ENONFINITE;
+    else if (st[i].status != BS_CONVERGED && worst == JNO_OK) worst = JNO_ENOTCONV;
+  }
+  if (worst == JNO_ENONFINITE) set_error("jno_batch_pcg_mixed: a non-finite value in at least one system");
+  if (worst == JNO_ENOTCONV) set_error("jno_batch_pcg_mixed: at least one system did not converge in %lld iterations",
+                                       (long long)maxiter);
+  return worst;
+}

@@ -74,6 +74,13 @@ __device__ __forceinline__ T block_sum(T v, T* smem /* >= NT/64 */) {
   return r;
 }
 
+// a x b with every product and difference rounded separately (no FMA contraction)
+__device__ __forceinline__ void cross_rn(const double* a, const double* b, double* c) {
+  c[0] = __dsub_rn(__dmul_rn(a[1], b[2]), __dmul_rn(a[2], b[1]));
+  c[1] = __dsub_rn(__dmul_rn(a[2], b[0]), __dmul_rn(a[0], b[2]));
+  c[2] = __dsub_rn(__dmul_rn(a[0], b[1]), __dmul_rn(a[1], b[0]));
+}
+
 // Exclusive scan of int32 counts into int32 offsets (n+1 entries, last = total).  Three-phase,
 // workspace: cdiv(n, 1024) + 1 int64.  Writes the total to *d_total (int64, device).
 int exclusive_scan_i32(const int32_t* d_in, int32_t* d_out, int64_t n, int64_t* d_total, void* d_ws,

@@ -202,13 +202,12 @@ __device__ __forceinline__ Geo geo_of(const double* __restrict__ pts, const int3
   for (int v = 0; v < 3; ++v)
 #pragma unroll
     for (int i = 0; i < 3; ++i) e[v][i] = pts[3 * (int64_t)c[v + 1] + i] - p0[i];
-  const double c23[3] = {e[1][1] * e[2][2] - e[1][2] * e[2][1], e[1][2] * e[2][0] - e[1][0] * e[2][2],
-                         e[1][0] * e[2][1] - e[1][1] * e[2][0]};
-  const double c31[3] = {e[2][1] * e[0][2] - e[2][2] * e[0][1], e[2][2] * e[0][0] - e[2][0] * e[0][2],
-                         e[2][0] * e[0][1] - e[2][1] * e[0][0]};
-  const double c12[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
-                         e[0][0] * e[1][1] - e[0][1] * e[1][0]};
-  const double det = e[0][0] * c23[0] + e[0][1] * c23[1] + e[0][2] * c23[2];
+  double c23[3], c31[3], c12[3];
+  cross_rn(e[1], e[2], c23);
+  cross_rn(e[2], e[0], c31);
+  cross_rn(e[0], e[1], c12);
+  const double det =
+      __dadd_rn(__dadd_rn(__dmul_rn(e[0][0], c23[0]), __dmul_rn(e[0][1], c23[1])), __dmul_rn(e[0][2], c23[2]));
   Geo g;
   const double inv = 1.0 / det;
 #pragma unroll
@@ -232,8 +231,11 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // Elasticity element matrices on MFMA (v_mfma_f64_16x16x4f64), one wavefront per element.
 //   K[(a,i),(b,j)] = lam X^T Xc [(a,i),(b,j)] + mu X^T Xc [(a,j),(b,i)] + mu delta_ij (H^T Hc)[a,b]
 // with X[q,(a,i)] = G_q[a,i], Xc = c_q X, H[(q,k),a] = G_q[a,k], c_q = w_q |det J| E_e.
-// MFMA operand layout (16x16x4 f64): A[r][k] in lane r + 16k, B[k][c] in lane c + 16k,
-// D[4*(lane/16)+t][lane%16] in element t of the lane's double4.
+// MFMA operand layout (16x16x4 f64): A[r][k] in lane r + 16k, B[k][c] in lane c + 16k, and
+// D[4t + lane/16][lane%16] in element t of the lane's double4 (one row per output group, four
+// groups interleaved by the 16-lane input block).
+// K1 = X^T X is formed without c_q and scaled afterwards, so it is bitwise symmetric; the final
+// combination is rounded term by term, so K itself is bitwise symmetric like the reference's.
 // ---------------------------------------------------------------------------------------------
 constexpr int EL_WAVES = 4;
 
@@ -250,37 +252,35 @@ __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
   const int32_t* c = cells10 + (int64_t)e * 10;
   const Geo g = geo_of(pts, c);
   const int r16 = lane % 16, k4 = lane / 16;
+  // all four quad weights are 1/24, so c_q = w |det J| E_e is one scalar per element
   const double cq = QW * g.adet * E[e];
-  // ---- X^T Xc: four 16x16 blocks, K = 4 quad points
-  double xa[2], xb[2];
+  // ---- K1 = X^T X: four 16x16 blocks, K = 4 quad points
+  double xa[2];
 #pragma unroll
   for (int blk = 0; blk < 2; ++blk) {
-    const int rc = blk * 16 + r16;  // (a,i) index for A rows / B cols
-    const double v = rc < 30 ? gphys(g, k4, rc / 3, rc % 3) : 0.0;
-    xa[blk] = v;
-    xb[blk] = cq * v;
+    const int rc = blk * 16 + r16;  // (a,i) index of A rows / B cols
+    xa[blk] = rc < 30 ? gphys(g, k4, rc / 3, rc % 3) : 0.0;
   }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) {
       dbl4 acc = {0.0, 0.0, 0.0, 0.0};
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[bi], xb[bj], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[bi], xa[bj], acc, 0, 0, 0);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) sK[wid][bi * 16 + 4 * k4 + t][bj * 16 + r16] = acc[t];
+      for (int t = 0; t < 4; ++t) sK[wid][bi * 16 + 4 * t + k4][bj * 16 + r16] = acc[t];
     }
-  // ---- H^T Hc: 10x10 (padded 16x16), K = 12 = (q, k) pairs in three steps of 4
+  // ---- S = H^T H: 10x10 (padded 16x16), K = 12 = (q, k) pairs in three steps of 4
   {
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
       const int m = 4 * s + k4, q = m / 3, kk = m % 3;
       const double gv = r16 < 10 ? gphys(g, q, r16, kk) : 0.0;
-      const double cqs = QW * g.adet * E[e];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(gv, cqs * gv, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(gv, gv, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) sS[wid][4 * k4 + t][r16] = acc[t];
+    for (int t = 0; t < 4; ++t) sS[wid][4 * t + k4][r16] = acc[t];
   }
   // the LDS tiles written above are read by other lanes of this wavefront below
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -302,9 +302,9 @@ __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
       double* out = vals + rowptr[3 * na + i] + 3 * lo;
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        double kv = lam * sK[wid][3 * a + i][3 * b + j] + mu * sK[wid][3 * a + j][3 * b + i];
-        if (i == j) kv += mu * sS[wid][a][b];
-        out[j] += kv;
+        double kv = __dadd_rn(__dmul_rn(lam, sK[wid][3 * a + i][3 * b + j]), __dmul_rn(mu, sK[wid][3 * a + j][3 * b + i]));
+        if (i == j) kv = __dadd_rn(kv, __dmul_rn(mu, sS[wid][a][b]));
+        out[j] += __dmul_rn(cq, kv);
       }
     }
   }

@@ -6,12 +6,17 @@
 //       never a separate pass),  x += alpha_{k-1} p_{k-1} (the x update is deferred one iteration
 //       and fused here),  q = A p_k,  partial p.q
 //   K2  alpha_k = gamma_k / p.q,  r -= alpha_k q,  partials r.z and r.r
-// Scalars never leave the device: each kernel's last workgroup (ticket counter + fences) reduces
-// the per-workgroup partials in a fixed order, so every run gives bitwise identical scalars, and
-// writes them into a small state block read by the next kernel.  Convergence, the iteration cap
-// and non-finite scalars are decided on the device; launches after the decision exit at once.
-// The host enqueues fixed chunks of iterations and polls a pinned copy of the state one chunk
-// behind, so the queue never drains while it checks.
+// No value crosses workgroups inside a kernel: each workgroup writes its partial sums, and every
+// workgroup of the next kernel reduces all of them itself, in one fixed order, before it starts.
+// So the scalars are bitwise identical across workgroups and runs, and there are no atomics or
+// cache-flushing fences on the iteration path.  Per-iteration scalars that must persist (k,
+// gamma_{k-1}, alpha_{k-1}, stop flags) are written by workgroup 0 into fields the same kernel
+// never reads.
+// Convergence, the iteration cap and non-finite scalars are decided on the device; launches after
+// the decision exit at once.  The host enqueues fixed chunks of iterations and polls a pinned copy
+// of the state one chunk behind, so the queue never drains while it checks.
+// Workgroups own contiguous row ranges, numbered so that each XCD (workgroups are dealt to the 8
+// XCDs round-robin) gets one contiguous slab: gathered neighbour rows then mostly hit its own L2.
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -22,7 +27,8 @@
 namespace jno {
 
 constexpr int PCG_NT = 256;
-constexpr int PCG_MAX_BLOCKS = 2048;
+constexpr int PCG_MAX_BLOCKS = 1024;
+constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
 
@@ -33,22 +39,29 @@ struct PcgState {
   int32_t pad;
   int64_t k;       // iterations done (written by K2 block 0)
   int64_t k_stop;  // iteration at which K1 stopped
-  double gamma_next;  // gamma_k = r_k . z_k    (K2 last block / prep)
-  double rr;          // r_k . r_k              (K2 last block / prep)
-  double gamma_cur;   // copy of gamma_k for K2 (K1 block 0)
-  double gamma_prev;  // gamma_{k-1}            (K2 block 0)
-  double alpha;       // alpha_{k-1}            (K2 block 0)
-  double pq;          // p_k . A p_k            (K1 last block)
+  double rr_stop;     // recursive r.r at the stop
+  double gamma_cur;   // gamma_k for K2 (K1 block 0)
+  double gamma_prev;  // gamma_{k-1}       (K2 block 0)
+  double alpha;       // alpha_{k-1}       (K2 block 0)
   double atol2;       // stop when r.r <= atol2
   double rr_replace;  // FP32 inner: replace the residual when r.r <= rr_replace
-  double rr_true;     // ||b - A x||^2 (prep)
-  double bb;          // b . b (prep)
+  double rr_true;     // ||b - A x||^2 (finalize)
+  double bb;          // b . b (finalize)
+  double rr0;         // r.r after the last residual preparation (finalize)
   int64_t maxiter;
+};
+
+// partial-sum areas (per workgroup): A = K1's p.q; B = K2's (r.z, r.r); P = prep's (rr_true, b.b)
+struct Partials {
+  double* A;  // [G]
+  double* B;  // [2][G]
+  double* P;  // [2][G]
 };
 
 template <typename T, typename V>
 struct PcgArgs {
   int64_t n;
+  int64_t rows_per_block;
   const int32_t* rowptr;
   const int32_t* col;
   const V* vals;
@@ -61,67 +74,26 @@ struct PcgArgs {
   T* q;
   T* x;
   PcgState* st;
-  double* partials;    // 2 * PCG_MAX_BLOCKS
-  unsigned* counter;
+  Partials part;
 };
 
-// Block-reduce two doubles; the last block to finish reduces all blocks' partials in a fixed order
-// and returns true (with the totals in *t0, *t1) on thread 0 of that block.
-__device__ __forceinline__ bool reduce2_last(double v0, double v1, double* partials, unsigned* counter, double* t0,
-                                             double* t1) {
-  __shared__ double sm0[PCG_NT / 64], sm1[PCG_NT / 64];
-  __shared__ bool s_last;
-  v0 = wave_sum(v0);
-  v1 = wave_sum(v1);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    sm0[wid] = v0;
-    sm1[wid] = v1;
-  }
-  __syncthreads();
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0) {
-    double a = 0, b = 0;
-    for (int w = 0; w < PCG_NT / 64; ++w) {
-      a += sm0[w];
-      b += sm1[w];
-    }
-    partials[blockIdx.x] = a;
-    partials[PCG_MAX_BLOCKS + blockIdx.x] = b;
-    __threadfence();
-    const unsigned ticket = atomicAdd(counter, 1u);
-    s_last = (ticket == (unsigned)nb - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;  // block-uniform
-  __threadfence();
-  double a = 0, b = 0;
-  for (int i = threadIdx.x; i < nb; i += PCG_NT) {
-    a += partials[i];
-    b += partials[PCG_MAX_BLOCKS + i];
-  }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  __syncthreads();
-  if (lane == 0) {
-    sm0[wid] = a;
-    sm1[wid] = b;
-  }
-  __syncthreads();
-  bool res = false;
-  if (threadIdx.x == 0) {
-    double s0 = 0, s1 = 0;
-    for (int w = 0; w < PCG_NT / 64; ++w) {
-      s0 += sm0[w];
-      s1 += sm1[w];
-    }
-    *t0 = s0;
-    *t1 = s1;
-    *counter = 0u;
-    res = true;
-  }
-  __syncthreads();  // the shared scratch may be reused by a following reduction
-  return res;
+// contiguous row range of this workgroup, XCD-aware (gridDim.x is a multiple of 8 or < 8)
+__device__ __forceinline__ int64_t block_first_row(int64_t rows_per_block) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int logical = (G % N_XCD == 0) ? (b % N_XCD) * (G / N_XCD) + b / N_XCD : b;
+  return (int64_t)logical * rows_per_block;
+}
+
+// sum of part[0..G) in a fixed order; every thread gets the total
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int G, double* sm) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < G; i += PCG_NT) s += part[i];
+  return block_sum<PCG_NT>(s, sm);
+}
+
+__device__ __forceinline__ void store_partial(double v, double* __restrict__ dst, double* sm) {
+  v = block_sum<PCG_NT>(v, sm);
+  if (threadIdx.x == 0) dst[blockIdx.x] = v;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -135,25 +107,29 @@ __device__ __forceinline__ T pnew_at(const T* __restrict__ dinv, const T* __rest
 
 template <int L, bool DIA, typename T, typename V>
 __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
+  __shared__ double sm[PCG_NT / 64];
   PcgState* st = a.st;
   if (st->done) return;
+  const int G = gridDim.x;
+  const double gamma = reduce_partials(a.part.B, G, sm);
+  const double rr = reduce_partials(a.part.B + G, G, sm);
   const int64_t k = st->k;
-  const double gamma = st->gamma_next, rr = st->rr;
   const double alpha_prev = st->alpha;
   const bool stop_conv = !(rr > st->atol2);
   const bool stop_cap = k >= st->maxiter;
   const bool stop_repl = rr <= st->rr_replace;
   const bool bad = !isfinite(gamma) || !isfinite(rr);
   const T ap = (T)alpha_prev;
+  const int64_t r0 = block_first_row(a.rows_per_block);
+  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
   if (stop_conv || stop_cap || stop_repl || bad) {
     // finish the deferred x update and stop
-    if (k > 0 && alpha_prev != 0.0) {
-      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
-        a.x[i] += ap * a.p_old[i];
-    }
+    if (k > 0 && alpha_prev != 0.0)
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) a.x[i] += ap * a.p_old[i];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done1 = 1;
       st->k_stop = k;
+      st->rr_stop = rr;
       st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
     }
     return;
@@ -162,7 +138,7 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
   const T beta = (T)beta_d;
   double part = 0.0;
   if (DIA) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) {
       const T po = a.p_old[i];
       const T pi = a.dinv[i] * a.r[i] + beta * po;
       T s = (T)a.vals[i] * pi;
@@ -179,8 +155,7 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
     }
   } else {
     const int sub = threadIdx.x % L;
-    const int64_t step = (int64_t)gridDim.x * blockDim.x / L;
-    for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L; row < a.n; row += step) {
+    for (int64_t row = r0 + threadIdx.x / L; row < r1; row += PCG_NT / L) {
       const int b = a.rowptr[row], e = a.rowptr[row + 1];
       T s = 0;
       for (int j = b + sub; j < e; j += L) s += (T)a.vals[j] * pnew_at(a.dinv, a.r, a.p_old, beta, a.col[j]);
@@ -197,8 +172,7 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
-  double t0, t1;
-  if (reduce2_last(part, 0.0, a.partials, a.counter, &t0, &t1)) st->pq = t0;
+  store_partial(part, a.part.A, sm);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -206,16 +180,21 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
 // ---------------------------------------------------------------------------------------------
 template <typename T, typename V>
 __global__ void __launch_bounds__(PCG_NT) pcg_k2(PcgArgs<T, V> a) {
+  __shared__ double sm[PCG_NT / 64];
   PcgState* st = a.st;
   if (st->done1) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
     return;
   }
+  const int G = gridDim.x;
+  const double pq = reduce_partials(a.part.A, G, sm);
   const double gamma = st->gamma_cur;
-  const double alpha = gamma / st->pq;
+  const double alpha = gamma / pq;
   const T al = (T)alpha;
+  const int64_t r0 = block_first_row(a.rows_per_block);
+  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
   double pg = 0.0, pr = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) {
     const T ri = a.r[i] - al * a.q[i];
     a.r[i] = ri;
     const T zi = a.dinv[i] * ri;
@@ -227,24 +206,25 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k2(PcgArgs<T, V> a) {
     st->gamma_prev = gamma;
     st->k = st->k + 1;
   }
-  double t0, t1;
-  if (reduce2_last(pg, pr, a.partials, a.counter + 1, &t0, &t1)) {
-    st->gamma_next = t0;
-    st->rr = t1;
-  }
+  store_partial(pg, a.part.B, sm);
+  store_partial(pr, a.part.B + G, sm);
 }
 
 // ---------------------------------------------------------------------------------------------
-// residual preparation: r64 = b - Ax (Ax nullable = 0), rs = (T) r64, optional x-accumulation,
-// totals rr_true = r64.r64, gamma = rs.dinv.rs, rr = rs.rs, bb = b.b
+// residual preparation: r64 = b - Ax (Ax nullable = 0), rs = (T) r64; partials of
+// gamma = rs.dinv.rs, rr = rs.rs (into B, for the next K1) and of ||r64||^2, b.b (into P)
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, const double* __restrict__ b,
-                                                          const double* __restrict__ Ax, const T* __restrict__ dinv,
-                                                          T* __restrict__ rs, PcgState* st, double* partials,
-                                                          unsigned* counter, int write_rs) {
+__global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, int64_t rows_per_block,
+                                                          const double* __restrict__ b, const double* __restrict__ Ax,
+                                                          const T* __restrict__ dinv, T* __restrict__ rs, Partials part,
+                                                          int write_rs) {
+  __shared__ double sm[PCG_NT / 64];
+  const int G = gridDim.x;
+  const int64_t r0 = block_first_row(rows_per_block);
+  const int64_t r1 = min(n, r0 + rows_per_block);
   double prt = 0, pg = 0, pr = 0, pb = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) {
     const double bi = b[i];
     const double r = Ax ? bi - Ax[i] : bi;
     prt += r * r;
@@ -254,16 +234,32 @@ __global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, const doubl
     pg += (double)ri * (double)(dinv[i] * ri);
     pr += (double)ri * (double)ri;
   }
-  double t0, t1;
-  if (reduce2_last(prt, pb, partials, counter, &t0, &t1)) {
-    st->rr_true = t0;
-    st->bb = t1;
-  }
-  // second pair through the other counter slot (separate tickets, separate partial area)
-  if (reduce2_last(pg, pr, partials + 2 * PCG_MAX_BLOCKS, counter + 1, &t0, &t1)) {
-    st->gamma_next = t0;
-    st->rr = t1;
-  }
+  store_partial(pg, part.B, sm);
+  store_partial(pr, part.B + G, sm);
+  store_partial(prt, part.P, sm);
+  store_partial(pb, part.P + G, sm);
+}
+
+// one workgroup: reduce the prep partials into the state and set the tolerances
+__global__ void __launch_bounds__(PCG_NT) finalize_k(PcgState* st, Partials part, int G, double tol, double atol,
+                                                     int64_t maxiter, double replace2, int first, int restart) {
+  __shared__ double sm[PCG_NT / 64];
+  const double rt = reduce_partials(part.P, G, sm);
+  const double bb = reduce_partials(part.P + G, G, sm);
+  const double rr = reduce_partials(part.B + G, G, sm);
+  if (threadIdx.x != 0) return;
+  st->rr_true = rt;
+  st->bb = bb;
+  st->rr0 = rr;
+  if (!restart) return;
+  st->atol2 = fmax(tol * tol * bb, atol * atol);
+  st->maxiter = maxiter;
+  st->done = 0;
+  st->done1 = 0;
+  st->status = ST_RUNNING;
+  st->alpha = 0.0;  // any deferred x update was applied at the stop
+  if (first) st->k = 0;
+  st->rr_replace = replace2 > 0.0 ? replace2 * rr : -1.0;
 }
 
 template <typename T>
@@ -284,23 +280,6 @@ __global__ void cast_f32_vec_k(int64_t n, const double* __restrict__ a, float* _
     b[i] = (float)a[i];
 }
 
-__global__ void state_init_k(PcgState* st, double tol, double atol, int64_t maxiter, double replace2, int first) {
-  // runs after residual_prep: atol2 = max(tol^2 b.b, atol^2)
-  const double atol2 = fmax(tol * tol * st->bb, atol * atol);
-  st->atol2 = atol2;
-  st->maxiter = maxiter;
-  st->done = 0;
-  st->done1 = 0;
-  st->status = ST_RUNNING;
-  st->alpha = 0.0;
-  if (first) st->k = 0;
-  st->rr_replace = replace2 > 0.0 ? replace2 * st->rr : -1.0;
-}
-
-__global__ void counters_zero_k(unsigned* c, int n) {
-  if (threadIdx.x < n) c[threadIdx.x] = 0u;
-}
-
 // ---------------------------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------------------------
@@ -308,8 +287,7 @@ static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Ws {
   PcgState* st;
-  unsigned* counters;
-  double* partials;  // 4 * PCG_MAX_BLOCKS
+  Partials part;
   void* r;
   void* p0;
   void* p1;
@@ -329,8 +307,9 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
   };
   Ws t;
   t.st = (PcgState*)take(sizeof(PcgState));
-  t.counters = (unsigned*)take(16 * sizeof(unsigned));
-  t.partials = (double*)take(4 * PCG_MAX_BLOCKS * sizeof(double));
+  t.part.A = (double*)take(PCG_MAX_BLOCKS * sizeof(double));
+  t.part.B = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
+  t.part.P = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.r = take(n * vb);
   t.p0 = take(n * vb);
   t.p1 = take(n * vb);
@@ -340,6 +319,17 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
   t.ax = (double*)take(n * 8);
   if (w) *w = t;
   return off;
+}
+
+// grid of the iteration kernels: a multiple of 8 (XCD slabs), at most PCG_MAX_BLOCKS, and the
+// rows per workgroup
+static void pcg_grid(int64_t n, int lanes, int* G, int64_t* rpb) {
+  int64_t want = cdiv(n * lanes, PCG_NT * 2);
+  if (want < 1) want = 1;
+  if (want > PCG_MAX_BLOCKS) want = PCG_MAX_BLOCKS;
+  if (want >= N_XCD) want = (want / N_XCD) * N_XCD;
+  *G = (int)want;
+  *rpb = cdiv(n, want);
 }
 
 struct HostPoll {
@@ -402,11 +392,9 @@ static KernelProf g_prof;
 
 // Runs K1/K2 chunks until the device reports a stop; returns the final state in *out.
 template <typename T, typename V>
-static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, hipStream_t s,
+static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid, hipStream_t s,
                      PcgState* out) {
   JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
-  const int64_t work = op->format == JNO_FMT_DIA ? a.n : a.n * op->lanes;
-  const int grid = grid_for(work, PCG_NT, PCG_MAX_BLOCKS);
   int64_t k = k0;
   int slot = 0, pending = 0;
   size_t nev = 0;
@@ -490,9 +478,11 @@ static int check_op(const jno_op* op, int need32) {
 }
 
 template <typename T, typename V>
-static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T* r, T* q, T* x, const Ws& w) {
+static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T* r, T* q, T* x, const Ws& w,
+                               int64_t rpb) {
   PcgArgs<T, V> a;
   a.n = op->n;
+  a.rows_per_block = rpb;
   a.rowptr = op->rowptr;
   a.col = op->col;
   a.vals = vals;
@@ -503,8 +493,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.q = q;
   a.x = x;
   a.st = w.st;
-  a.partials = w.partials;
-  a.counter = w.counters;
+  a.part = w.part;
   a.p_old = nullptr;
   a.p_new = nullptr;
   return a;
@@ -552,30 +541,32 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   Ws w;
   ws_layout(n, 0, static_cast<char*>(d_ws), &w);
   const auto t0 = std::chrono::steady_clock::now();
-  const int g = grid_for(n, PCG_NT, PCG_MAX_BLOCKS);
-  counters_zero_k<<<1, 64, 0, s>>>(w.counters, 16);
+  int G;
+  int64_t rpb;
+  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, &G, &rpb);
+  const int gz = grid_for(n, PCG_NT, 2048);
   double* r = (double*)w.r;
   double* pb[2] = {(double*)w.p0, (double*)w.p1};
-  zero_k<double><<<g, PCG_NT, 0, s>>>(pb[0], n);
-  zero_k<double><<<g, PCG_NT, 0, s>>>(pb[1], n);
+  zero_k<double><<<gz, PCG_NT, 0, s>>>(pb[0], n);
+  zero_k<double><<<gz, PCG_NT, 0, s>>>(pb[1], n);
   if (use_x0) {
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
   } else {
-    zero_k<double><<<g, PCG_NT, 0, s>>>(d_x, n);
+    zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
   }
-  residual_prep_k<double><<<g, PCG_NT, 0, s>>>(n, d_b, use_x0 ? w.ax : nullptr, d_dinv, r, w.st, w.partials,
-                                               w.counters + 2, 1);
-  state_init_k<<<1, 1, 0, s>>>(w.st, tol, atol, maxiter, 0.0, 1);
+  residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, r, w.part, 1);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 1, 1);
   JNO_LAUNCH_CHECK();
-  PcgArgs<double, double> a = make_args<double, double>(h_op, h_op->vals64, d_dinv, r, (double*)w.q, d_x, w);
+  PcgArgs<double, double> a = make_args<double, double>(h_op, h_op->vals64, d_dinv, r, (double*)w.q, d_x, w, rpb);
   PcgState f;
-  rc = run_inner<double, double>(h_op, a, pb, 0, g_chunk, s, &f);
+  rc = run_inner<double, double>(h_op, a, pb, 0, g_chunk, G, s, &f);
   if (rc) return rc;
   // true residual ||b - A x|| / ||b||  (the eager guard of _maybe_residual_check)
   rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
   if (rc) return rc;
-  residual_prep_k<double><<<g, PCG_NT, 0, s>>>(n, d_b, w.ax, d_dinv, r, w.st, w.partials, w.counters + 2, 0);
+  residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, r, w.part, 0);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 0, 0);
   JNO_LAUNCH_CHECK();
   PcgState e;
   rc = read_state(w.st, &e, s);
@@ -586,7 +577,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
     h_stats->iterations = f.k_stop;
     h_stats->refinements = 0;
     h_stats->relres = bn > 0 ? sqrt(e.rr_true) / bn : sqrt(e.rr_true);
-    h_stats->relres_recursive = bn > 0 ? sqrt(f.rr) / bn : sqrt(f.rr);
+    h_stats->relres_recursive = bn > 0 ? sqrt(f.rr_stop) / bn : sqrt(f.rr_stop);
     h_stats->seconds = std::chrono::duration<double>(t1 - t0).count();
   }
   return finish_status(f, h_stats, f.k_stop);
@@ -607,40 +598,44 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   Ws w;
   ws_layout(n, 1, static_cast<char*>(d_ws), &w);
   const auto t0 = std::chrono::steady_clock::now();
-  const int g = grid_for(n, PCG_NT, PCG_MAX_BLOCKS);
-  counters_zero_k<<<1, 64, 0, s>>>(w.counters, 16);
+  int G;
+  int64_t rpb;
+  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, &G, &rpb);
+  const int gz = grid_for(n, PCG_NT, 2048);
   float* rs = (float*)w.r;
   float* pb[2] = {(float*)w.p0, (float*)w.p1};
   float* xs = (float*)w.xs;
-  zero_k<float><<<g, PCG_NT, 0, s>>>(pb[0], n);
-  zero_k<float><<<g, PCG_NT, 0, s>>>(pb[1], n);
-  zero_k<float><<<g, PCG_NT, 0, s>>>(xs, n);
-  cast_f32_vec_k<<<g, PCG_NT, 0, s>>>(n, d_dinv, w.dinv32);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[0], n);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[1], n);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
+  cast_f32_vec_k<<<gz, PCG_NT, 0, s>>>(n, d_dinv, w.dinv32);
   if (use_x0) {
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
   } else {
-    zero_k<double><<<g, PCG_NT, 0, s>>>(d_x, n);
+    zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
   }
   const double repl2 = replace_factor * replace_factor;
-  residual_prep_k<float><<<g, PCG_NT, 0, s>>>(n, d_b, use_x0 ? w.ax : nullptr, w.dinv32, rs, w.st, w.partials,
-                                              w.counters + 2, 1);
-  state_init_k<<<1, 1, 0, s>>>(w.st, tol, atol, maxiter, repl2, 1);
+  residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, w.dinv32, rs, w.part, 1);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, repl2, 1, 1);
   JNO_LAUNCH_CHECK();
-  PcgArgs<float, float> a = make_args<float, float>(h_op, h_op->vals32, w.dinv32, rs, (float*)w.q, xs, w);
+  PcgArgs<float, float> a = make_args<float, float>(h_op, h_op->vals32, w.dinv32, rs, (float*)w.q, xs, w, rpb);
   int64_t k = 0, refinements = 0;
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;
   for (;;) {
-    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, s, &f);
+    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f);
     if (rc) return rc;
     k = f.k_stop;
     // FP64 refinement: x += xs, r = b - A x, rs = (float) r
-    accumulate_x_k<<<g, PCG_NT, 0, s>>>(n, d_x, xs);
+    accumulate_x_k<<<gz, PCG_NT, 0, s>>>(n, d_x, xs);
     JNO_LAUNCH_CHECK();
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
-    residual_prep_k<float><<<g, PCG_NT, 0, s>>>(n, d_b, w.ax, w.dinv32, rs, w.st, w.partials, w.counters + 2, 1);
+    residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, w.dinv32, rs, w.part, 1);
+    // sets the state for a restart too (alpha = 0: the deferred x update of the next K1 is void,
+    // it was applied at the stop; p_{k-1} and gamma_{k-1} are kept)
+    finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, repl2, 0, 1);
     JNO_LAUNCH_CHECK();
     rc = read_state(w.st, &e, s);
     if (rc) return rc;
@@ -656,10 +651,6 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
       if (!std::isfinite(rr_true)) f.status = ST_NONFINITE;
       break;
     }
-    // resume the Krylov recurrence on the replaced residual: alpha = 0 so the deferred x update of
-    // the next K1 is void (it was applied at the stop); p_{k-1} and gamma_{k-1} are kept.
-    state_init_k<<<1, 1, 0, s>>>(w.st, tol, atol, maxiter, repl2, 0);
-    JNO_LAUNCH_CHECK();
   }
   const auto t1 = std::chrono::steady_clock::now();
   const double bn = sqrt(bb);
@@ -667,7 +658,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
     h_stats->iterations = k;
     h_stats->refinements = refinements;
     h_stats->relres = bn > 0 ? sqrt(rr_true) / bn : sqrt(rr_true);
-    h_stats->relres_recursive = bn > 0 ? sqrt(e.rr) / bn : sqrt(e.rr);
+    h_stats->relres_recursive = bn > 0 ? sqrt(e.rr0) / bn : sqrt(e.rr0);
     h_stats->seconds = std::chrono::duration<double>(t1 - t0).count();
   }
   return finish_status(f, h_stats, k);

@@ -1,0 +1,131 @@
+"""Batched solves: uneven meshes each with its own operator, and many right-hand sides on one
+operator, against the oracle's direct solves (<= 1e-9) and the single-system solver."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _systems(ns, seed0=100, fmt_csr=()):
+    import jno_amd as J
+
+    out = []
+    for i, n in enumerate(ns):
+        m = J.kuhn_box(n)
+        s = J.fem_poisson(m)
+        k = O.k_lognormal(m.n_cells, seed0 + i)
+        A, b = s.evaluate({"k": torch.tensor(k, device="cuda")})
+        out.append((n, k, A, b))
+    return out
+
+
+@pytest.mark.parametrize("graph", [0, 4])
+def test_uneven_batch_matches_oracle(dev, graph):
+    from jno_amd.batch import BatchSolver
+
+    ns = [3, 5, 8, 12, 4, 9]
+    sy = _systems(ns)
+    bs = BatchSolver(tol=1e-10, steps_per_graph=graph)
+    xs = bs([(A, b) for _, _, A, b in sy])
+    for (n, k, A, b), x, st in zip(sy, xs, bs.last_stats):
+        p, t = O.kuhn_box(n)
+        Ao, bo, d = O.poisson_p1_system(p, t, k)
+        xd = O.direct_solve(Ao, bo)
+        assert _rel(_np(x), xd) <= 1e-9, (n, st)
+        assert st["relres"] <= 1e-10, st
+    # larger systems need more iterations; each system kept its own count
+    its = [s["iterations"] for s in bs.last_stats]
+    assert its[3] > its[0]
+
+
+def test_batch_with_csr_items_and_shared_operator(dev):
+    from jno_amd.batch import BatchSolver
+    from jno_amd.operator import SolverOperator
+
+    sy = _systems([6, 7])
+    n, k, A, b = sy[0]
+    # force a CSR layout on the second system
+    A2 = sy[1][2]
+    A2._cache[("solver_op", "auto", True)] = SolverOperator(A2, fmt="csr", lanes=4)
+    rng = np.random.default_rng(0)
+    rhs = [torch.tensor(rng.standard_normal(A.shape[0]), device="cuda") for _ in range(5)]
+    bs = BatchSolver(tol=1e-10, steps_per_graph=8)
+    xs = bs([(A, r) for r in rhs] + [(A2, sy[1][3])])
+    p, t = O.kuhn_box(n)
+    Ao, bo, d = O.poisson_p1_system(p, t, k)
+    for r, x in zip(rhs, xs[:5]):
+        assert _rel(_np(x), O.direct_solve(Ao, _np(r))) <= 1e-9
+    p2, t2 = O.kuhn_box(7)
+    Ao2, bo2, _ = O.poisson_p1_system(p2, t2, sy[1][1])
+    assert _rel(_np(xs[5]), O.direct_solve(Ao2, bo2)) <= 1e-9
+
+
+def test_batch_zero_rhs_and_cap(dev):
+    from jno_amd.batch import BatchSolver
+
+    sy = _systems([5, 6])
+    A, b = sy[0][2], sy[0][3]
+    bs = BatchSolver(tol=1e-10)
+    xs = bs([(A, torch.zeros_like(b)), (sy[1][2], sy[1][3])])
+    assert not _np(xs[0]).any() and bs.last_stats[0]["iterations"] == 0
+    with pytest.raises(RuntimeError, match="did not solve"):
+        BatchSolver(tol=1e-10, maxiter=2)([(A, b)])
+
+
+def test_c4_batch_256_uneven_and_64_rhs(dev):
+    import time
+
+    import jno_amd as J
+    from jno_amd import precond, solve
+    from jno_amd.batch import BatchSolver
+
+    rng = np.random.default_rng(4)
+    ns = rng.integers(24, 65, size=256)
+    systems = []
+    for i, n in enumerate(ns):
+        m = J.kuhn_box(int(n))
+        s = J.fem_poisson(m)
+        k = torch.tensor(O.k_lognormal(m.n_cells, 1000 + i), device="cuda")
+        systems.append(s.evaluate({"k": k}))
+    ndof = sum(A.shape[0] for A, _ in systems)
+    bs = BatchSolver(tol=1e-10, steps_per_graph=8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs = bs(systems)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert all(s["relres"] <= 1e-10 for s in bs.last_stats)
+    # spot-check against the single-system mixed solver
+    single = solve.cg(tol=1e-10, precision="mixed")
+    for i in (0, int(np.argmax(ns)), int(np.argmin(ns))):
+        A, b = systems[i]
+        assert _rel(_np(xs[i]), _np(single(A, b, M=precond.jacobi()))) <= 1e-9
+    its = [s["iterations"] for s in bs.last_stats]
+    print(f"\nC4 batch: 256 systems, {ndof} dofs, {dt:.3f} s, iterations {min(its)}..{max(its)}")
+    # 64 N(0,1) right-hand sides on one 96^3 operator
+    m = J.kuhn_box(96)
+    s = J.fem_poisson(m)
+    A, b = s.evaluate({"k": torch.tensor(O.k_lognormal(m.n_cells, 7), device="cuda")})
+    g = torch.Generator(device="cpu").manual_seed(0)
+    rhs = [torch.randn(A.shape[0], generator=g, dtype=torch.float64).to("cuda") for _ in range(64)]
+    for r in rhs:
+        r[s.dmask.bool()] = 0.0
+    t0 = time.perf_counter()
+    xs = bs([(A, r) for r in rhs])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert all(st["relres"] <= 1e-10 for st in bs.last_stats)
+    for j in (0, 63):
+        assert _rel(_np(xs[j]), _np(single(A, rhs[j], M=precond.jacobi()))) <= 1e-9
+    print(f"C4 64 rhs on 96^3: {dt:.3f} s")
