@@ -150,7 +150,8 @@ typedef struct jno_op {
 size_t jno_op_analyze_workspace_size(void);
 int jno_op_analyze(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
                    int32_t* h_ndiag, int32_t* h_offsets, void* d_ws, size_t ws_bytes, void* stream);
-/* DIA fill from CSR: d_dia64 f64[ndiag*n] (nullable), d_dia32 f32[ndiag*n] (nullable). */
+/* DIA fill from CSR: d_dia64 f64[ndiag*ld] (nullable), d_dia32 f32[ndiag*ld] (nullable), where
+ * diagonal d starts at d*ld and ld = n rounded up to a multiple of 64 (aligned vector loads). */
 int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
                     const double* d_vals, int ndiag, const int32_t* h_offsets, double* d_dia64,
                     float* d_dia32, void* stream);
@@ -181,8 +182,9 @@ size_t jno_pcg_workspace_size(int64_t n, int mixed);
  * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
 int jno_pcg_set_chunk(int chunk);
 /* Live kernel timing of the PCG iteration kernels (no reference counterpart: measurement hook).
- * When enabled, HIP events are recorded on the solve's stream around every iteration kernel and
- * the summed durations of the iterations that did work are accumulated; enabling resets them.
+ * When enabled, HIP events are recorded on the solve's stream around the two iteration kernels of
+ * every 8th iteration and the summed durations of the sampled iterations that did work are
+ * accumulated (counts in *_count); enabling resets them.
  * K1 = SpMV + p/x update + p.q, K2 = r update + r.z, r.r. */
 int jno_pcg_profile(int enable);
 int jno_pcg_profile_read(double* h_k1_ms, int64_t* h_k1_count, double* h_k2_ms, int64_t* h_k2_count);

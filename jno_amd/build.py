@@ -26,6 +26,9 @@ HIPFLAGS = [
     "--offload-arch=gfx950",
     "-Wall",
     "-Wno-unused-result",
+    # products and sums are rounded as written: bitwise-symmetric element matrices, exact zeros
+    # of axis-aligned geometry, and numpy-identical lattice coordinates depend on it
+    "-ffp-contract=off",
     # packed-FP32 instructions are kept out of all device code
     "-Xclang",
     "-target-feature",

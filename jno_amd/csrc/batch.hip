@@ -49,7 +49,7 @@ struct BItem {
 struct BState {
   // written by KA's first entry, read by KB
   int32_t phase_b;
-  int32_t done;       // also read by KA as a fast exit (its own decision agrees)
+  int32_t done;       // written by KB's first entry (once KA decided DONE), read by KA
   double gamma_cur;   // gamma_k
   // written by KA's first entry when it closes a REPLACE, read by later KAs
   double atol2, rr_replace, rr_true, bb, rr_last;
@@ -84,7 +84,7 @@ __device__ __forceinline__ T mat_row(const BItem& it, int64_t i, const V* vals, 
     T s = (T)vals[i] * x[i];
     for (int d = 1; d < it.ndiag; ++d) {
       const int64_t o = it.offs[d];
-      const V* v = vals + (int64_t)d * it.n;
+      const V* v = vals + (int64_t)d * dia_ld(it.n);
       if (i + o < it.n) s += (T)v[i] * x[i + o];
       if (i - o >= 0) s += (T)v[i - o] * x[i - o];
     }
@@ -152,7 +152,6 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
         if (phase == PH_DONE) {
           S->status = !(rt > atol2) ? (S->status == BS_NONFINITE ? BS_NONFINITE : BS_CONVERGED)
                                     : (bad ? BS_NONFINITE : BS_MAXITER);
-          S->done = 1;
           atomicSub(a.active, 1);
         }
       }
@@ -190,7 +189,7 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
         s = it.v32[i] * pi;
         for (int d = 1; d < it.ndiag; ++d) {
           const int64_t o = it.offs[d];
-          const float* v = it.v32 + (int64_t)d * it.n;
+          const float* v = it.v32 + (int64_t)d * dia_ld(it.n);
           if (i + o < it.n) s += v[i] * (dv[i + o] * r[i + o] + beta * pold[i + o]);
           if (i - o >= 0) s += v[i - o] * (dv[i - o] * r[i - o] + beta * pold[i - o]);
         }
@@ -220,8 +219,12 @@ __global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
     if ((it.format == JNO_FMT_DIA) != DIA) continue;
     BState* S = a.st + te.x;
     const int phase = S->phase_b;
-    if (phase == PH_DONE) continue;
     const bool first = te.y == 0;
+    if (phase == PH_DONE) {
+      // KA never writes `done`, so its fast exit reads a value that is stable during KA
+      if (first && threadIdx.x == 0) S->done = 1;
+      continue;
+    }
     const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
     float* r = a.rs + it.off;
     const float* dv = a.dinv + it.off;

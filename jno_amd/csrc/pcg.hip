@@ -105,6 +105,86 @@ __device__ __forceinline__ T pnew_at(const T* __restrict__ dinv, const T* __rest
   return dinv[j] * r[j] + beta * p_old[j];
 }
 
+// VN consecutive values as one 16-byte load (global_load_dwordx4; the address only needs the
+// element alignment: gfx950 serves unaligned vector accesses)
+template <typename T>
+struct V16 {
+  static constexpr int N = 16 / sizeof(T);
+  T v[N];
+};
+
+template <typename T>
+__device__ __forceinline__ V16<T> ld16(const T* p) {
+  V16<T> r;
+  __builtin_memcpy(&r, p, 16);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void st16(T* p, const V16<T>& r) {
+  __builtin_memcpy(p, &r, 16);
+}
+
+// DIA row tile [i0, i0+VN): p_new = z + beta p_old on the tile, q = A p_new, deferred x update;
+// returns the tile's p.q.  `interior` tiles need no bounds checks on neighbour rows.
+template <typename T, typename V>
+__device__ __forceinline__ double dia_tile(const PcgArgs<T, V>& a, int64_t i0, T beta, T ap, bool upd_x) {
+  constexpr int VN = V16<T>::N;
+  const int64_t ld = dia_ld(a.n);
+  const V16<T> dv = ld16(a.dinv + i0), rv = ld16(a.r + i0), po = ld16(a.p_old + i0);
+  V16<T> pi, s;
+#pragma unroll
+  for (int j = 0; j < VN; ++j) pi.v[j] = dv.v[j] * rv.v[j] + beta * po.v[j];
+  {
+    const V16<V> d0 = ld16(a.vals + i0);
+#pragma unroll
+    for (int j = 0; j < VN; ++j) s.v[j] = (T)d0.v[j] * pi.v[j];
+  }
+  for (int d = 1; d < a.ndiag; ++d) {
+    const int64_t o = a.off.o[d];
+    const V* vd = a.vals + (int64_t)d * ld;
+    const V16<V> vu = ld16(vd + i0), vl = ld16(vd + i0 - o);
+    const V16<T> du = ld16(a.dinv + i0 + o), ru = ld16(a.r + i0 + o), pu = ld16(a.p_old + i0 + o);
+    const V16<T> dl = ld16(a.dinv + i0 - o), rl = ld16(a.r + i0 - o), pl = ld16(a.p_old + i0 - o);
+#pragma unroll
+    for (int j = 0; j < VN; ++j) {
+      s.v[j] += (T)vu.v[j] * (du.v[j] * ru.v[j] + beta * pu.v[j]);
+      s.v[j] += (T)vl.v[j] * (dl.v[j] * rl.v[j] + beta * pl.v[j]);
+    }
+  }
+  st16(a.p_new + i0, pi);
+  st16(a.q + i0, s);
+  if (upd_x) {
+    V16<T> xv = ld16(a.x + i0);
+#pragma unroll
+    for (int j = 0; j < VN; ++j) xv.v[j] += ap * po.v[j];
+    st16(a.x + i0, xv);
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int j = 0; j < VN; ++j) part += (double)pi.v[j] * (double)s.v[j];
+  return part;
+}
+
+// one DIA row with bounds checks (tiles at the ends of the matrix)
+template <typename T, typename V>
+__device__ __forceinline__ double dia_row(const PcgArgs<T, V>& a, int64_t i, T beta, T ap, bool upd_x) {
+  const int64_t ld = dia_ld(a.n);
+  const T po = a.p_old[i];
+  const T pi = a.dinv[i] * a.r[i] + beta * po;
+  T s = (T)a.vals[i] * pi;
+  for (int d = 1; d < a.ndiag; ++d) {
+    const int64_t o = a.off.o[d];
+    const V* v = a.vals + (int64_t)d * ld;
+    if (i + o < a.n) s += (T)v[i] * pnew_at(a.dinv, a.r, a.p_old, beta, i + o);
+    if (i - o >= 0) s += (T)v[i - o] * pnew_at(a.dinv, a.r, a.p_old, beta, i - o);
+  }
+  a.p_new[i] = pi;
+  a.q[i] = s;
+  if (upd_x) a.x[i] += ap * po;
+  return (double)pi * (double)s;
+}
+
 template <int L, bool DIA, typename T, typename V>
 __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
   __shared__ double sm[PCG_NT / 64];
@@ -138,20 +218,16 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
   const T beta = (T)beta_d;
   double part = 0.0;
   if (DIA) {
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) {
-      const T po = a.p_old[i];
-      const T pi = a.dinv[i] * a.r[i] + beta * po;
-      T s = (T)a.vals[i] * pi;
-      for (int d = 1; d < a.ndiag; ++d) {
-        const int64_t o = a.off.o[d];
-        const V* v = a.vals + (int64_t)d * a.n;
-        if (i + o < a.n) s += (T)v[i] * pnew_at(a.dinv, a.r, a.p_old, beta, i + o);
-        if (i - o >= 0) s += (T)v[i - o] * pnew_at(a.dinv, a.r, a.p_old, beta, i - o);
+    // each thread owns VN consecutive rows; the block's first row is VN-aligned
+    constexpr int VN = V16<T>::N;
+    int64_t omax = 0;
+    for (int d = 1; d < a.ndiag; ++d) omax = max(omax, (int64_t)a.off.o[d]);
+    for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * PCG_NT) {
+      if (i0 + VN <= r1 && i0 - omax >= 0 && i0 + VN - 1 + omax < a.n) {
+        part += dia_tile<T, V>(a, i0, beta, ap, k > 0);
+      } else {
+        for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += dia_row<T, V>(a, i, beta, ap, k > 0);
       }
-      a.p_new[i] = pi;
-      a.q[i] = s;
-      if (k > 0) a.x[i] += ap * po;
-      part += (double)pi * (double)s;
     }
   } else {
     const int sub = threadIdx.x % L;
@@ -194,12 +270,27 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k2(PcgArgs<T, V> a) {
   const int64_t r0 = block_first_row(a.rows_per_block);
   const int64_t r1 = min(a.n, r0 + a.rows_per_block);
   double pg = 0.0, pr = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) {
-    const T ri = a.r[i] - al * a.q[i];
-    a.r[i] = ri;
-    const T zi = a.dinv[i] * ri;
-    pg += (double)ri * (double)zi;
-    pr += (double)ri * (double)ri;
+  constexpr int VN = V16<T>::N;
+  for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * PCG_NT) {
+    if (i0 + VN <= r1) {
+      V16<T> rv = ld16(a.r + i0);
+      const V16<T> qv = ld16(a.q + i0), dv = ld16(a.dinv + i0);
+#pragma unroll
+      for (int j = 0; j < VN; ++j) {
+        const T ri = rv.v[j] - al * qv.v[j];
+        rv.v[j] = ri;
+        pg += (double)ri * (double)(dv.v[j] * ri);
+        pr += (double)ri * (double)ri;
+      }
+      st16(a.r + i0, rv);
+    } else {
+      for (int64_t i = i0; i < r1; ++i) {
+        const T ri = a.r[i] - al * a.q[i];
+        a.r[i] = ri;
+        pg += (double)ri * (double)(a.dinv[i] * ri);
+        pr += (double)ri * (double)ri;
+      }
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->alpha = alpha;
@@ -329,7 +420,8 @@ static void pcg_grid(int64_t n, int lanes, int* G, int64_t* rpb) {
   if (want > PCG_MAX_BLOCKS) want = PCG_MAX_BLOCKS;
   if (want >= N_XCD) want = (want / N_XCD) * N_XCD;
   *G = (int)want;
-  *rpb = cdiv(n, want);
+  // a multiple of 4 rows so every workgroup's first row is 16-byte aligned for FP32 and FP64
+  *rpb = cdiv(cdiv(n, want), 4) * 4;
 }
 
 struct HostPoll {
@@ -372,10 +464,12 @@ static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t
   return JNO_OK;
 }
 
-// Optional live kernel timing: HIP events recorded on the solve's stream between consecutive
-// K1/K2 launches (E0 K1 E1 K2 E2 K1 E3 ...), summed over the iterations that did work.
+// Optional live kernel timing: on every stride-th iteration, HIP events are recorded on the
+// solve's stream around its K1 and K2 (E0 K1 E1 K2 E2); the sampled iterations that did work are
+// summed.
 struct KernelProf {
   bool on = false;
+  int stride = 8;  // time every stride-th iteration (events perturb the queue very little then)
   std::vector<hipEvent_t> ev;
   double k1_ms = 0, k2_ms = 0;
   int64_t k1_n = 0, k2_n = 0;
@@ -398,18 +492,17 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   int64_t k = k0;
   int slot = 0, pending = 0;
   size_t nev = 0;
-  if (g_prof.on) {
-    if (g_prof.ensure(1) != JNO_OK) return JNO_EHIP;
-    JNO_HIP(hipEventRecord(g_prof.ev[nev++], s));
-  }
+  std::vector<int64_t> sampled;  // iteration index of each (E0, E1, E2) event triple
   for (;;) {
     for (int i = 0; i < chunk; ++i, ++k) {
       a.p_old = pbuf[k & 1];
       a.p_new = pbuf[(k + 1) & 1];
-      if (g_prof.on) {
-        if (g_prof.ensure(nev + 2) != JNO_OK) return JNO_EHIP;
-        int rc = launch_iter<T, V>(op, a, grid, s, g_prof.ev[nev], g_prof.ev[nev + 1]);
-        nev += 2;
+      if (g_prof.on && k % g_prof.stride == 0) {
+        if (g_prof.ensure(nev + 3) != JNO_OK) return JNO_EHIP;
+        JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
+        int rc = launch_iter<T, V>(op, a, grid, s, g_prof.ev[nev + 1], g_prof.ev[nev + 2]);
+        nev += 3;
+        sampled.push_back(k);
         if (rc) return rc;
       } else {
         int rc = launch_iter<T, V>(op, a, grid, s, nullptr, nullptr);
@@ -430,11 +523,11 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   JNO_HIP(hipStreamSynchronize(s));
   *out = g_poll.h[0];
   if (g_prof.on) {
-    const int64_t work_iters = out->k_stop - k0;
-    for (int64_t i = 0; i < work_iters && (size_t)(2 * i + 2) < nev; ++i) {
+    for (size_t t = 0; t < sampled.size(); ++t) {
+      if (sampled[t] >= out->k_stop) break;  // launches after the stop did no work
       float t1 = 0, t2 = 0;
-      JNO_HIP(hipEventElapsedTime(&t1, g_prof.ev[2 * i], g_prof.ev[2 * i + 1]));
-      JNO_HIP(hipEventElapsedTime(&t2, g_prof.ev[2 * i + 1], g_prof.ev[2 * i + 2]));
+      JNO_HIP(hipEventElapsedTime(&t1, g_prof.ev[3 * t], g_prof.ev[3 * t + 1]));
+      JNO_HIP(hipEventElapsedTime(&t2, g_prof.ev[3 * t + 1], g_prof.ev[3 * t + 2]));
       g_prof.k1_ms += t1;
       g_prof.k2_ms += t2;
       g_prof.k1_n += 1;

@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) dia_spmv_k(int64_t n, int ndiag, DiaOffse
     X s = (X)dia[i] * x[i];
     for (int d = 1; d < ndiag; ++d) {
       const int64_t o = off.o[d];
-      const V* v = dia + (int64_t)d * n;
+      const V* v = dia + (int64_t)d * dia_ld(n);
       if (i + o < n) s += (X)v[i] * x[i + o];
       if (i - o >= 0) s += (X)v[i - o] * x[i - o];
     }
@@ -232,8 +232,8 @@ __global__ void dia_fill_k(int64_t n, const int32_t* __restrict__ rowptr, const 
 #pragma unroll
     for (int d = 0; d < JNO_DIA_MAX; ++d) {
       if (d < ndiag) {
-        if (d64) d64[(int64_t)d * n + r] = acc[d];
-        if (d32) d32[(int64_t)d * n + r] = (float)acc[d];
+        if (d64) d64[(int64_t)d * dia_ld(n) + r] = acc[d];
+        if (d32) d32[(int64_t)d * dia_ld(n) + r] = (float)acc[d];
       }
     }
   }

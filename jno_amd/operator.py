@@ -58,8 +58,9 @@ class SolverOperator:
 
     def _build_dia(self, nd, offs, fp32):
         n, A = self.n, self.A
-        d64 = torch.empty(nd * n, dtype=torch.float64, device=self.device)
-        d32 = torch.empty(nd * n, dtype=torch.float32, device=self.device) if fp32 else None
+        ld = (n + 63) // 64 * 64  # diagonals padded to 64 rows (aligned vector loads)
+        d64 = torch.empty(nd * ld, dtype=torch.float64, device=self.device)
+        d32 = torch.empty(nd * ld, dtype=torch.float32, device=self.device) if fp32 else None
         offs_c = (C.c_int32 * _lib.JNO_DIA_MAX)(*offs)
         _lib.call("jno_op_fill_dia", n, _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), nd, offs_c, _lib.ptr(d64),
                   _lib.ptr(d32), _lib.stream_ptr(self.device))

@@ -82,7 +82,8 @@ def test_c2_128_pattern_and_mixed_solve(dev):
     # the FP64 GPU PCG on the same operator
     f64 = solve.cg(tol=1e-10, precision="fp64")
     u64 = f64(A, b, M=precond.jacobi())
-    assert f64.last_stats["relres"] <= 1e-10
+    # jax cg semantics: the recursive residual meets rtol; the true one is reported beside it
+    assert f64.last_stats["relres_recursive"] <= 1e-10 and f64.last_stats["relres"] <= 2e-10
     # FP64 CPU restatement of the reference solve (jax cg semantics, rtol 1e-10)
     cs = _cpu_system(n, k_host)
     assert np.abs(cs.vals - _np(A.vals)).max() <= 1e-12 * np.abs(cs.vals).max()
@@ -134,6 +135,14 @@ def test_c3_64_gradient(dev):
     # formed from the increments, L+ - L- = e0^T M (d+ - d-) + (d+^T M d+ - d-^T M d-)/2, so the
     # 1e-6 relative change of one element out of 1.57M is not lost to cancellation in L itself.
     fine = solve.cg(tol=1e-13, precision="fp64")
+    # the per-element comparison uses a gradient from tight solves, so solver error does not
+    # masquerade as FD error on elements whose gradient is small
+    kt2 = torch.tensor(k, device=dev, requires_grad=True)
+    ustar = fine(As, bs, M=precond.jacobi())
+    u = adjoint.solve(s, kt2, fine)
+    e = u - ustar
+    (0.5 * torch.dot(e, M.mv(e))).backward()
+    g = _np(kt2.grad)
     u0 = u.detach()
     e0 = u0 - ustar
     rng = np.random.default_rng(2024)
