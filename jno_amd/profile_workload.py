@@ -1,0 +1,72 @@
+"""A bounded, repeatable workload for profiler runs (rocprofv3 kernel traces and PMC counters).
+
+    python -m jno_amd.profile_workload --case c2 --iters 40 [--precision mixed|fp64] [--copy-mib 1024]
+
+Builds the configuration's system (128^3 P1 Poisson for c2, 64^3 for c3, 32^3 P2 elasticity for
+c5), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
+finally copies a buffer of --copy-mib MiB with a device-to-device copy kernel: the copy's
+FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels' counters.
+"""
+from __future__ import annotations
+
+import argparse
+import time
+
+import numpy as np
+import torch
+
+
+def build(case: str, dev):
+    import jno_amd as J
+
+    if case in ("c2", "c3"):
+        n = 128 if case == "c2" else 64
+        m = J.kuhn_box(n, device=dev)
+        s = J.fem_poisson(m)
+        k = torch.tensor(np.exp(0.5 * np.random.default_rng(0).standard_normal(m.n_cells)), device=dev)
+        return s.evaluate({"k": k})
+    if case == "c5":
+        from jno_amd.elasticity import ElasticitySystem
+
+        s = ElasticitySystem(J.kuhn_box(32, device=dev))
+        E = torch.tensor(np.exp(0.3 * np.random.default_rng(0).standard_normal(s.mesh.n_cells)), device=dev)
+        return s.evaluate({"E": E})
+    raise ValueError(f"unknown case {case!r}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--case", default="c2")
+    ap.add_argument("--iters", type=int, default=40)
+    ap.add_argument("--precision", default="mixed", choices=("mixed", "fp64"))
+    ap.add_argument("--format", default="auto")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    args = ap.parse_args()
+    from jno_amd import precond
+    from jno_amd.solve import LinearSolver
+
+    dev = torch.device("cuda:0")
+    A, b = build(args.case, dev)
+    # replace_factor tiny: no residual replacement inside the bounded run
+    sv = LinearSolver("cg", 1e-30, 0.0, args.iters, precision=args.precision, format=args.format,
+                      replace_factor=1e-30, guard=False)
+    sv(A, b, M=precond.jacobi())  # warm (layout, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sv(A, b, M=precond.jacobi())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"{args.case} {args.precision}: {sv.last_stats['iterations']} iterations in {dt * 1e3:.3f} ms "
+          f"({dt / max(sv.last_stats['iterations'], 1) * 1e6:.1f} us/iteration), operator {sv.last_stats['operator']}")
+    nbytes = args.copy_mib << 20
+    src = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {nbytes} bytes read + {nbytes} bytes written")
+
+
+if __name__ == "__main__":
+    main()

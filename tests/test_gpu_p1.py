@@ -301,7 +301,7 @@ def test_adjoint_gradient_vs_oracle_and_fd(dev):
             kk = k.copy()
             kk[el] += sg * h
             A2, b2 = s.evaluate({"k": torch.tensor(kk, device="cuda")})
-            ds.append(solve.cg(tol=1e-13)(A2, b2 - A2.mv(u0), M=precond.jacobi()))
+            ds.append(solve.cg(tol=1e-12)(A2, b2 - A2.mv(u0), M=precond.jacobi()))
         dp, dm = ds
         dL = torch.dot(e0, M.mv(dp - dm)) + 0.5 * (torch.dot(dp, M.mv(dp)) - torch.dot(dm, M.mv(dm)))
         fd = float(dL) / (2 * h)
