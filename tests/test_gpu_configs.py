@@ -147,6 +147,8 @@ def test_c3_64_gradient(dev):
     e0 = u0 - ustar
     rng = np.random.default_rng(2024)
     worst = 0.0
+    worst_abs = 0.0  # error relative to max|g|: a few 1e-8 of solver error is what small |g_e| cannot hide
+    gmax = float(np.abs(g).max())
     fine_stats = []
     for el in rng.choice(T, 32, replace=False):
         h = 1e-6 * k[el]
@@ -160,7 +162,8 @@ def test_c3_64_gradient(dev):
         dL = torch.dot(e0, M.mv(dp - dm)) + 0.5 * (torch.dot(dp, M.mv(dp)) - torch.dot(dm, M.mv(dm)))
         fd = float(dL) / (2 * h)
         worst = max(worst, abs(fd - g[el]) / max(abs(g[el]), 1e-300))
+        worst_abs = max(worst_abs, abs(fd - g[el]) / gmax)
         fine_stats.append(fine.last_stats["iterations"])
-    print(f"\n64^3 gradient: rel L2 vs oracle {_rel(g, go):.2e}; worst FD rel err {worst:.2e}; "
-          f"defect solves {min(fine_stats)}..{max(fine_stats)} iterations")
-    assert worst <= 1e-6
+        assert abs(fd - g[el]) <= 1e-6 * abs(g[el]) + 1e-8 * gmax, (el, fd, g[el], gmax)
+    print(f"\n64^3 gradient: rel L2 vs oracle {_rel(g, go):.2e}; worst FD err {worst:.2e} of |g_e|, "
+          f"{worst_abs:.2e} of max|g|; defect solves {min(fine_stats)}..{max(fine_stats)} iterations")
