@@ -181,6 +181,13 @@ size_t jno_pcg_workspace_size(int64_t n, int mixed);
 /* Iterations enqueued per host poll of the device-side convergence state (default 8).  No
  * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
 int jno_pcg_set_chunk(int chunk);
+/* Launch shape of the iteration kernels, per iteration precision (no reference counterpart; set
+ * by the autotuner; process-wide): precision = 0 for the FP64 iterations of jno_pcg_f64, 1 for
+ * the FP32 inner iterations of jno_pcg_mixed; grid_cap = maximum workgroups (multiple of 8,
+ * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for one row per thread;
+ * block = threads per workgroup (256/512/1024).  jno_pcg_get_tuning reads the current shape. */
+int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block);
+int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block);
 /* Live kernel timing of the PCG iteration kernels (no reference counterpart: measurement hook).
  * When enabled, HIP events are recorded on the solve's stream around the two iteration kernels of
  * every 8th iteration and the summed durations of the sampled iterations that did work are

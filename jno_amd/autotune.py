@@ -49,3 +49,44 @@ def choose_format(A):
         choice = ("csr", best[1])
     _CACHE[key] = choice
     return choice
+
+
+# Launch shapes of the PCG iteration kernels tried by tune_launch: (block, grid_cap, k1_vector)
+CANDIDATE_SHAPES = tuple((blk, cap, vec) for blk in (256, 512, 1024) for cap in (256, 512, 1024) for vec in (0, 1))
+_LAUNCH_CACHE: dict = {}
+
+
+def tune_launch(A, b, precision: str = "mixed", iters: int = 100, shapes=CANDIDATE_SHAPES):
+    """Time ``iters`` bounded PCG iterations on ``(A, b)`` for each launch shape and install the
+    fastest one for ``precision`` (process-wide, through ``jno_pcg_set_tuning``).  Returns
+    ``(block, grid_cap, k1_vector)``; cached per pattern signature and precision."""
+    import time
+
+    from . import precond
+    from .solve import LinearSolver, get_tuning, set_tuning
+
+    fmt, lanes = choose_format(A)
+    key = (A.shape[0], A.nnz, fmt, lanes, precision)
+    if key in _LAUNCH_CACHE:
+        set_tuning(precision, *_LAUNCH_CACHE[key])
+        return _LAUNCH_CACHE[key]
+    saved = get_tuning(precision)
+    sv = LinearSolver("cg", 1e-30, 0.0, iters, precision=precision, replace_factor=1e-30, guard=False)
+    M = precond.jacobi()
+    best = None
+    try:
+        for shape in shapes:
+            set_tuning(precision, *shape)
+            sv(A, b, M=M)  # warm
+            torch.cuda.synchronize(A.device)
+            t0 = time.perf_counter()
+            sv(A, b, M=M)
+            torch.cuda.synchronize(A.device)
+            t = time.perf_counter() - t0
+            if best is None or t < best[0]:
+                best = (t, shape)
+    finally:
+        set_tuning(precision, *saved)
+    _LAUNCH_CACHE[key] = best[1]
+    set_tuning(precision, *best[1])
+    return best[1]

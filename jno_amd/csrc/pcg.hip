@@ -27,7 +27,18 @@
 namespace jno {
 
 constexpr int PCG_NT = 256;
-constexpr int PCG_MAX_BLOCKS = 1024;
+constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup partials
+
+// Launch shape of the iteration kernels, one per iteration precision (set by the autotuner through
+// jno_pcg_set_tuning).  The defaults are the fastest shapes of a sweep over block x grid cap x DIA
+// tile variant on a 2.1M-row 4-diagonal operator: FP32 iterations 1024 threads x 256 workgroups
+// with one row per thread, FP64 iterations 512 x 1024 with 16-byte vector tiles.
+struct Tuning {
+  int grid_cap;  // workgroups of the iteration kernels (a multiple of 8)
+  int vec;       // DIA K1: VN-row vector tiles (1) or one row per thread (0)
+  int block;     // threads per workgroup (256, 512, 1024)
+};
+static Tuning g_tune[2] = {{1024, 1, 512}, {256, 0, 1024}};  // [0] FP64 iterations, [1] FP32 iterations
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -66,6 +77,8 @@ struct PcgArgs {
   const int32_t* col;
   const V* vals;
   int ndiag;
+  int vec;    // DIA K1 tile variant
+  int block;  // threads per workgroup of the launch (host side only)
   DiaOffsets off;
   const T* dinv;
   T* r;
@@ -85,14 +98,16 @@ __device__ __forceinline__ int64_t block_first_row(int64_t rows_per_block) {
 }
 
 // sum of part[0..G) in a fixed order; every thread gets the total
+template <int NT = PCG_NT>
 __device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int G, double* sm) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < G; i += PCG_NT) s += part[i];
-  return block_sum<PCG_NT>(s, sm);
+  for (int i = threadIdx.x; i < G; i += NT) s += part[i];
+  return block_sum<NT>(s, sm);
 }
 
+template <int NT = PCG_NT>
 __device__ __forceinline__ void store_partial(double v, double* __restrict__ dst, double* sm) {
-  v = block_sum<PCG_NT>(v, sm);
+  v = block_sum<NT>(v, sm);
   if (threadIdx.x == 0) dst[blockIdx.x] = v;
 }
 
@@ -185,14 +200,14 @@ __device__ __forceinline__ double dia_row(const PcgArgs<T, V>& a, int64_t i, T b
   return (double)pi * (double)s;
 }
 
-template <int L, bool DIA, typename T, typename V>
-__global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
-  __shared__ double sm[PCG_NT / 64];
+template <int NT, int L, bool DIA, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
   PcgState* st = a.st;
   if (st->done) return;
   const int G = gridDim.x;
-  const double gamma = reduce_partials(a.part.B, G, sm);
-  const double rr = reduce_partials(a.part.B + G, G, sm);
+  const double gamma = reduce_partials<NT>(a.part.B, G, sm);
+  const double rr = reduce_partials<NT>(a.part.B + G, G, sm);
   const int64_t k = st->k;
   const double alpha_prev = st->alpha;
   const bool stop_conv = !(rr > st->atol2);
@@ -205,7 +220,7 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
   if (stop_conv || stop_cap || stop_repl || bad) {
     // finish the deferred x update and stop
     if (k > 0 && alpha_prev != 0.0)
-      for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) a.x[i] += ap * a.p_old[i];
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) a.x[i] += ap * a.p_old[i];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->done1 = 1;
       st->k_stop = k;
@@ -217,21 +232,23 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
   const double beta_d = (k == 0) ? 0.0 : gamma / st->gamma_prev;
   const T beta = (T)beta_d;
   double part = 0.0;
-  if (DIA) {
+  if (DIA && a.vec) {
     // each thread owns VN consecutive rows; the block's first row is VN-aligned
     constexpr int VN = V16<T>::N;
     int64_t omax = 0;
     for (int d = 1; d < a.ndiag; ++d) omax = max(omax, (int64_t)a.off.o[d]);
-    for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * PCG_NT) {
+    for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * NT) {
       if (i0 + VN <= r1 && i0 - omax >= 0 && i0 + VN - 1 + omax < a.n) {
         part += dia_tile<T, V>(a, i0, beta, ap, k > 0);
       } else {
         for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += dia_row<T, V>(a, i, beta, ap, k > 0);
       }
     }
+  } else if (DIA) {
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += dia_row<T, V>(a, i, beta, ap, k > 0);
   } else {
     const int sub = threadIdx.x % L;
-    for (int64_t row = r0 + threadIdx.x / L; row < r1; row += PCG_NT / L) {
+    for (int64_t row = r0 + threadIdx.x / L; row < r1; row += NT / L) {
       const int b = a.rowptr[row], e = a.rowptr[row + 1];
       T s = 0;
       for (int j = b + sub; j < e; j += L) s += (T)a.vals[j] * pnew_at(a.dinv, a.r, a.p_old, beta, a.col[j]);
@@ -248,33 +265,43 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k1(PcgArgs<T, V> a) {
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
-  store_partial(part, a.part.A, sm);
+  store_partial<NT>(part, a.part.A, sm);
 }
 
 // ---------------------------------------------------------------------------------------------
 // K2: r -= alpha q, partials r.z and r.r
 // ---------------------------------------------------------------------------------------------
-template <typename T, typename V>
-__global__ void __launch_bounds__(PCG_NT) pcg_k2(PcgArgs<T, V> a) {
-  __shared__ double sm[PCG_NT / 64];
+template <int NT, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
   PcgState* st = a.st;
   if (st->done1) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
     return;
   }
   const int G = gridDim.x;
-  const double pq = reduce_partials(a.part.A, G, sm);
+  const int64_t r0 = block_first_row(a.rows_per_block);
+  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
+  constexpr int VN = V16<T>::N;
+  // the first tile's loads are issued before the partial-sum reduction, so they overlap it
+  const int64_t f0 = r0 + (int64_t)threadIdx.x * VN;
+  const bool pre = f0 + VN <= r1;
+  V16<T> rv0, qv0, dv0;
+  if (pre) {
+    rv0 = ld16(a.r + f0);
+    qv0 = ld16(a.q + f0);
+    dv0 = ld16(a.dinv + f0);
+  }
+  const double pq = reduce_partials<NT>(a.part.A, G, sm);
   const double gamma = st->gamma_cur;
   const double alpha = gamma / pq;
   const T al = (T)alpha;
-  const int64_t r0 = block_first_row(a.rows_per_block);
-  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
   double pg = 0.0, pr = 0.0;
-  constexpr int VN = V16<T>::N;
-  for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * PCG_NT) {
+  for (int64_t i0 = f0; i0 < r1; i0 += (int64_t)VN * NT) {
     if (i0 + VN <= r1) {
-      V16<T> rv = ld16(a.r + i0);
-      const V16<T> qv = ld16(a.q + i0), dv = ld16(a.dinv + i0);
+      const bool first = pre && i0 == f0;
+      V16<T> rv = first ? rv0 : ld16(a.r + i0);
+      const V16<T> qv = first ? qv0 : ld16(a.q + i0), dv = first ? dv0 : ld16(a.dinv + i0);
 #pragma unroll
       for (int j = 0; j < VN; ++j) {
         const T ri = rv.v[j] - al * qv.v[j];
@@ -297,8 +324,8 @@ __global__ void __launch_bounds__(PCG_NT) pcg_k2(PcgArgs<T, V> a) {
     st->gamma_prev = gamma;
     st->k = st->k + 1;
   }
-  store_partial(pg, a.part.B, sm);
-  store_partial(pr, a.part.B + G, sm);
+  store_partial<NT>(pg, a.part.B, sm);
+  store_partial<NT>(pr, a.part.B + G, sm);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -414,10 +441,10 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
 
 // grid of the iteration kernels: a multiple of 8 (XCD slabs), at most PCG_MAX_BLOCKS, and the
 // rows per workgroup
-static void pcg_grid(int64_t n, int lanes, int* G, int64_t* rpb) {
-  int64_t want = cdiv(n * lanes, PCG_NT * 2);
+static void pcg_grid(int64_t n, int lanes, const Tuning& tu, int* G, int64_t* rpb) {
+  int64_t want = cdiv(n * lanes, (int64_t)tu.block * 2);
   if (want < 1) want = 1;
-  if (want > PCG_MAX_BLOCKS) want = PCG_MAX_BLOCKS;
+  if (want > tu.grid_cap) want = tu.grid_cap;
   if (want >= N_XCD) want = (want / N_XCD) * N_XCD;
   *G = (int)want;
   // a multiple of 4 rows so every workgroup's first row is 16-byte aligned for FP32 and FP64
@@ -439,29 +466,39 @@ struct HostPoll {
 };
 static thread_local HostPoll g_poll;
 
-template <typename T, typename V>
-static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t after_k1,
-                       hipEvent_t after_k2) {
+template <int NT, typename T, typename V>
+static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t after_k1,
+                          hipEvent_t after_k2) {
   if (op->format == JNO_FMT_DIA) {
-    pcg_k1<1, true, T, V><<<grid, PCG_NT, 0, s>>>(a);
+    pcg_k1<NT, 1, true, T, V><<<grid, NT, 0, s>>>(a);
   } else {
     switch (op->lanes) {
-      case 1: pcg_k1<1, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
-      case 2: pcg_k1<2, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
-      case 4: pcg_k1<4, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
-      case 8: pcg_k1<8, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
-      case 16: pcg_k1<16, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
-      case 32: pcg_k1<32, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
-      case 64: pcg_k1<64, false, T, V><<<grid, PCG_NT, 0, s>>>(a); break;
+      case 1: pcg_k1<NT, 1, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 2: pcg_k1<NT, 2, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 4: pcg_k1<NT, 4, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 8: pcg_k1<NT, 8, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 16: pcg_k1<NT, 16, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 32: pcg_k1<NT, 32, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 64: pcg_k1<NT, 64, false, T, V><<<grid, NT, 0, s>>>(a); break;
       default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: lanes per row must be a power of two in 1..64");
     }
   }
   JNO_LAUNCH_CHECK();
   if (after_k1) JNO_HIP(hipEventRecord(after_k1, s));
-  pcg_k2<T, V><<<grid, PCG_NT, 0, s>>>(a);
+  pcg_k2<NT, T, V><<<grid, NT, 0, s>>>(a);
   JNO_LAUNCH_CHECK();
   if (after_k2) JNO_HIP(hipEventRecord(after_k2, s));
   return JNO_OK;
+}
+
+template <typename T, typename V>
+static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t after_k1,
+                       hipEvent_t after_k2) {
+  switch (a.block) {
+    case 256: return launch_iter_nt<256, T, V>(op, a, grid, s, after_k1, after_k2);
+    case 512: return launch_iter_nt<512, T, V>(op, a, grid, s, after_k1, after_k2);
+    default: return launch_iter_nt<1024, T, V>(op, a, grid, s, after_k1, after_k2);
+  }
 }
 
 // Optional live kernel timing: on every stride-th iteration, HIP events are recorded on the
@@ -572,7 +609,7 @@ static int check_op(const jno_op* op, int need32) {
 
 template <typename T, typename V>
 static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T* r, T* q, T* x, const Ws& w,
-                               int64_t rpb) {
+                               int64_t rpb, const Tuning& tu) {
   PcgArgs<T, V> a;
   a.n = op->n;
   a.rows_per_block = rpb;
@@ -580,6 +617,8 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.col = op->col;
   a.vals = vals;
   a.ndiag = op->ndiag;
+  a.vec = tu.vec;
+  a.block = tu.block;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
   a.dinv = dinv;
   a.r = r;
@@ -615,6 +654,26 @@ extern "C" int jno_pcg_profile_read(double* h_k1_ms, int64_t* h_k1_n, double* h_
   return JNO_OK;
 }
 
+extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block) {
+  JNO_REQUIRE(precision == 0 || precision == 1, JNO_EINVAL,
+              "jno_pcg_set_tuning: precision must be 0 (FP64 iterations) or 1 (FP32 iterations)");
+  JNO_REQUIRE(grid_cap >= 8 && grid_cap <= PCG_MAX_BLOCKS && grid_cap % 8 == 0, JNO_EINVAL,
+              "jno_pcg_set_tuning: grid_cap must be a multiple of 8 in 8..%d", PCG_MAX_BLOCKS);
+  JNO_REQUIRE(block == 256 || block == 512 || block == 1024, JNO_EINVAL,
+              "jno_pcg_set_tuning: block must be 256, 512 or 1024");
+  g_tune[precision] = Tuning{grid_cap, k1_vector ? 1 : 0, block};
+  return JNO_OK;
+}
+
+extern "C" int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block) {
+  JNO_REQUIRE(precision == 0 || precision == 1, JNO_EINVAL,
+              "jno_pcg_get_tuning: precision must be 0 (FP64 iterations) or 1 (FP32 iterations)");
+  if (grid_cap) *grid_cap = g_tune[precision].grid_cap;
+  if (k1_vector) *k1_vector = g_tune[precision].vec;
+  if (block) *block = g_tune[precision].block;
+  return JNO_OK;
+}
+
 extern "C" int jno_pcg_set_chunk(int chunk) {
   JNO_REQUIRE(chunk >= 1 && chunk <= 1024, JNO_EINVAL, "jno_pcg_set_chunk: chunk must be in 1..1024");
   g_chunk = chunk;
@@ -636,7 +695,8 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   const auto t0 = std::chrono::steady_clock::now();
   int G;
   int64_t rpb;
-  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, &G, &rpb);
+  const Tuning tu = g_tune[0];
+  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   double* r = (double*)w.r;
   double* pb[2] = {(double*)w.p0, (double*)w.p1};
@@ -651,7 +711,8 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, r, w.part, 1);
   finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 1, 1);
   JNO_LAUNCH_CHECK();
-  PcgArgs<double, double> a = make_args<double, double>(h_op, h_op->vals64, d_dinv, r, (double*)w.q, d_x, w, rpb);
+  PcgArgs<double, double> a =
+      make_args<double, double>(h_op, h_op->vals64, d_dinv, r, (double*)w.q, d_x, w, rpb, tu);
   PcgState f;
   rc = run_inner<double, double>(h_op, a, pb, 0, g_chunk, G, s, &f);
   if (rc) return rc;
@@ -693,7 +754,8 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   const auto t0 = std::chrono::steady_clock::now();
   int G;
   int64_t rpb;
-  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, &G, &rpb);
+  const Tuning tu = g_tune[1];
+  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   float* rs = (float*)w.r;
   float* pb[2] = {(float*)w.p0, (float*)w.p1};
@@ -712,7 +774,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, w.dinv32, rs, w.part, 1);
   finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, repl2, 1, 1);
   JNO_LAUNCH_CHECK();
-  PcgArgs<float, float> a = make_args<float, float>(h_op, h_op->vals32, w.dinv32, rs, (float*)w.q, xs, w, rpb);
+  PcgArgs<float, float> a = make_args<float, float>(h_op, h_op->vals32, w.dinv32, rs, (float*)w.q, xs, w, rpb, tu);
   int64_t k = 0, refinements = 0;
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;

@@ -122,3 +122,22 @@ def cg(*, tol: float = 1e-8, atol: float = 0.0, maxiter: Optional[int] = 20_000,
 def set_chunk(chunk: int):
     """Iterations enqueued per host poll of the device-side convergence state."""
     _lib.call("jno_pcg_set_chunk", int(chunk))
+
+
+def _prec_index(precision: str) -> int:
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    return 1 if precision == "mixed" else 0
+
+
+def set_tuning(precision: str, block: int, grid_cap: int, k1_vector: int):
+    """Launch shape of the PCG iteration kernels for ``precision`` ('fp64' iterations or the FP32
+    inner iterations of 'mixed'): threads per workgroup, maximum workgroups, DIA tile variant."""
+    _lib.call("jno_pcg_set_tuning", _prec_index(precision), int(grid_cap), int(k1_vector), int(block))
+
+
+def get_tuning(precision: str):
+    """Current ``(block, grid_cap, k1_vector)`` for ``precision``."""
+    cap, vec, blk = C.c_int32(), C.c_int32(), C.c_int32()
+    _lib.call("jno_pcg_get_tuning", _prec_index(precision), C.byref(cap), C.byref(vec), C.byref(blk))
+    return int(blk.value), int(cap.value), int(vec.value)
