@@ -130,40 +130,49 @@ def test_c3_64_gradient(dev):
     lam, _, _, _ = cs.pcg(1e-13)
     go = O.adjoint_grad_k_p1(p, t, uo, lam, d)
     assert _rel(g, go) <= 1e-6, _rel(g, go)
-    # central FD at h = 1e-6 k_e on 32 seeded elements.  The perturbed states are u + d with
-    # A(k +- h) d = b - A(k +- h) u solved tightly (defect correction), and the loss difference is
+    # central FD at h = 1e-6 k_e on 32 seeded elements.  The perturbed states are u0 + d with
+    # A(k +- h) d = b - A(k +- h) u0 solved tightly (defect correction), and the loss difference is
     # formed from the increments, L+ - L- = e0^T M (d+ - d-) + (d+^T M d+ - d-^T M d-)/2, so the
     # 1e-6 relative change of one element out of 1.57M is not lost to cancellation in L itself.
+    # A is linear in k, so A(k +- h) u0 = A(k) u0 +- h B_e u0 with B_e u0 formed once from
+    # (A(k + k_e 1_e) - A(k)) u0 / k_e: the rounding of the assembled entries (eps |A_ij|, about
+    # 1e-17, against the h K_e change of about 1e-9, and not free of the constant null space of
+    # K_e) would otherwise put a noise of about 1e-6 / (h / k_e) of g_e into the difference.
     fine = solve.cg(tol=1e-12, precision="fp64")
-    # the per-element comparison uses a gradient from tight solves, so solver error does not
-    # masquerade as FD error on elements whose gradient is small
-    kt2 = torch.tensor(k, device=dev, requires_grad=True)
-    ustar = fine(As, bs, M=precond.jacobi())
-    u = adjoint.solve(s, kt2, fine)
-    e = u - ustar
-    (0.5 * torch.dot(e, M.mv(e))).backward()
-    g = _np(kt2.grad)
-    u0 = u.detach()
+
+    def tight(A_, b_):
+        # one defect-correction pass: the error left by rtol 1e-12 (up to cond(A) * 1e-12) would
+        # otherwise show as FD error on elements whose gradient is small
+        x_ = fine(A_, b_, M=precond.jacobi())
+        return x_ + fine(A_, b_ - A_.mv(x_), M=precond.jacobi())
+
+    # the per-element comparison uses the same contraction (grad_k) on tightly solved states
+    A0, b0 = s.evaluate({"k": kt.detach()})
+    ustar = tight(As, bs)
+    u0 = tight(A0, b0)
     e0 = u0 - ustar
+    rhs = M.mv(e0)
+    rhs[s.dmask.bool()] = 0.0
+    g = _np(s.grad_k(u0, tight(A0, rhs)))
+    r0 = b0 - A0.mv(u0)
+    Au0 = A0.mv(u0)
     rng = np.random.default_rng(2024)
-    worst = 0.0
-    worst_abs = 0.0  # error relative to max|g|: a few 1e-8 of solver error is what small |g_e| cannot hide
-    gmax = float(np.abs(g).max())
-    fine_stats = []
+    errs = []
     for el in rng.choice(T, 32, replace=False):
         h = 1e-6 * k[el]
+        kk = kt.detach().clone()
+        kk[el] *= 2.0
+        Bu = (s.evaluate({"k": kk})[0].mv(u0) - Au0) / k[el]
         ds = []
         for sg in (1, -1):
             kk = kt.detach().clone()
             kk[el] += sg * h
-            A2, b2 = s.evaluate({"k": kk})
-            ds.append(fine(A2, b2 - A2.mv(u0), M=precond.jacobi()))
+            A2, _ = s.evaluate({"k": kk})
+            ds.append(tight(A2, r0 - (sg * h) * Bu))
         dp, dm = ds
         dL = torch.dot(e0, M.mv(dp - dm)) + 0.5 * (torch.dot(dp, M.mv(dp)) - torch.dot(dm, M.mv(dm)))
         fd = float(dL) / (2 * h)
-        worst = max(worst, abs(fd - g[el]) / max(abs(g[el]), 1e-300))
-        worst_abs = max(worst_abs, abs(fd - g[el]) / gmax)
-        fine_stats.append(fine.last_stats["iterations"])
-        assert abs(fd - g[el]) <= 1e-6 * abs(g[el]) + 1e-8 * gmax, (el, fd, g[el], gmax)
-    print(f"\n64^3 gradient: rel L2 vs oracle {_rel(g, go):.2e}; worst FD err {worst:.2e} of |g_e|, "
-          f"{worst_abs:.2e} of max|g|; defect solves {min(fine_stats)}..{max(fine_stats)} iterations")
+        errs.append(abs(fd - g[el]) / abs(g[el]))
+        assert abs(fd - g[el]) <= 1e-6 * abs(g[el]), (el, fd, g[el])
+    print(f"\n64^3 gradient: rel L2 vs oracle {_rel(g, go):.2e}; FD rel err max {max(errs):.2e}, "
+          f"median {float(np.median(errs)):.2e} over {len(errs)} elements")
