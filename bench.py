@@ -47,8 +47,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the path would shard by instances or by mesh "
-              "slabs; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: this path runs on one GPU (it would shard by batch instances or by mesh slabs "
+              "with a halo exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -91,8 +91,8 @@ def main():
         A, b, u = step(mixed)
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / args.steps
-    k1_ms, k1_n, k2_ms, k2_n = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
-    lib.jno_pcg_profile_read(C.byref(k1_ms), C.byref(k1_n), C.byref(k2_ms), C.byref(k2_n))
+    prof_ms, prof_n = (C.c_double * 3)(), C.c_int64()
+    lib.jno_pcg_profile_read(prof_ms, C.byref(prof_n))
     lib.jno_pcg_profile(0)
     st_mixed = dict(mixed.last_stats)
     op = mixed.operator(A)
@@ -118,23 +118,42 @@ def main():
     st_fp64 = dict(fp64.last_stats)
     agree = float(torch.linalg.norm(u - u64) / torch.linalg.norm(u64))
 
-    # dominant kernel: K1 of the FP32 inner loop (SpMV + p/x update + p.q)
-    k1_avg_ms = k1_ms.value / max(k1_n.value, 1)
-    k2_avg_ms = k2_ms.value / max(k2_n.value, 1)
-    if op.format == "dia":
-        # per row: ndiag stored FP32 values, reads r, dinv, p_old, x and writes p_new, q, x (FP32)
-        k1_bytes_row = 4 * op.ndiag + 4 * 4 + 3 * 4
-        k1_bytes = N * k1_bytes_row
-    else:
-        nnz = op.nnz_stored
-        k1_bytes = nnz * (4 + 4) + 4 * (N + 1) + N * (4 * 4 + 3 * 4)
-    achieved = k1_bytes / (k1_avg_ms * 1e-3) / 1e9
-    k2_bytes = N * (3 * 4 + 4)
-    k2_gbs = k2_bytes / (k2_avg_ms * 1e-3) / 1e9
-    iters = st_mixed["iterations"]
-    # the survey's byte model of one FP32-inner iteration on P_eager CSR (nnz*(4+4) + 4(N+1) + 13*4N)
-    nnz_eager = op.nnz_stored if op.format == "csr" else None
+    # the plain FP64 PCG on the reference's own layout: CSR of the zero-dropped pattern (compress_eager)
+    log("[bench] plain FP64 PCG on CSR")
+    fp64csr = solve.cg(tol=1e-10, precision="fp64", format="csr")
+    fp64csr(A, b, M=precond.jacobi())  # warm
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    fp64csr(A, b, M=precond.jacobi())
+    torch.cuda.synchronize()
+    t_fp64csr = time.perf_counter() - ta
+    st_fp64csr = dict(fp64csr.last_stats)
 
+    # per-kernel live timing (HIP events on the solve's stream, every 8th iteration sampled) and the
+    # algorithmic bytes each kernel must move per launch (FP32 inner iterations, 4-byte values)
+    blk, cap, vec, mode = solve.get_tuning("mixed")
+    nsamp = max(prof_n.value, 1)
+    ms = [prof_ms[j] / nsamp for j in range(3)]
+    if op.format == "dia":
+        spmv_mat = N * 4 * op.ndiag  # main + stored upper diagonals (the lower ones are the same values)
+    else:
+        spmv_mat = op.nnz_stored * (4 + 4) + 4 * (N + 1)
+    if mode == 1:  # K0 (p, x update), KS (q = A p, p.q), K2 (r update, r.z, r.r)
+        kernels = [("pcg_k0", ms[0], N * (4 * 4 + 2 * 4)),
+                   (f"pcg_ks<{op.format}, f32>", ms[1], spmv_mat + N * (4 + 4)),
+                   ("pcg_k2", ms[2], N * (3 * 4 + 4))]
+    elif mode == 2:  # KA (all vector updates), KB (w = A u, u.w)
+        kernels = [("pcg_ka", ms[0], N * (6 * 4 + 5 * 4)),
+                   (f"pcg_kb<{op.format}, f32>", ms[1], spmv_mat + N * (4 + 4))]
+    else:  # K1 (p recompute for gathered neighbours, x update, q = A p), K2
+        kernels = [(f"pcg_k1<{op.format}, f32>", ms[1], spmv_mat + N * (4 * 4 + 3 * 4)),
+                   ("pcg_k2", ms[2], N * (3 * 4 + 4))]
+    kernel_rows = {name: {"avg_ms": round(t, 5), "bytes": b, "gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}
+                   for name, t, b in kernels}
+    dom_name, dom_ms, dom_bytes = max(kernels, key=lambda kv: kv[1])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = committed_traffic(dom_name, f"p1-{n}")
+    iters = st_mixed["iterations"]
     out = {
         "metric": "P1 Poisson 128³ time-to-1e-10 + HBM GB/s; fwd+adjoint grads/s vs FP64 plain",
         "value": round(t_step, 6),
@@ -155,19 +174,22 @@ def main():
             "mixed_refinements": st_mixed["refinements"], "mixed_relres_true": st_mixed["relres"],
             "fp64_s": round(t_fp64, 6), "fp64_iterations": st_fp64["iterations"], "fp64_relres_true": st_fp64["relres"],
             "speedup_mixed_vs_fp64": round(t_fp64 / max(st_mixed["seconds"], 1e-12), 3),
+            "fp64_csr_s": round(t_fp64csr, 6), "fp64_csr_iterations": st_fp64csr["iterations"],
+            "fp64_csr_operator": st_fp64csr["operator"],
+            "speedup_mixed_vs_fp64_csr": round(t_fp64csr / max(st_mixed["seconds"], 1e-12), 3),
             "mixed_vs_fp64_rel_l2": agree,
             "dof_iterations_per_s": N * iters / max(st_mixed["seconds"], 1e-12),
             "assemble_s": round(t_assemble, 6), "layout_s": round(t_layout, 6), "plan_s": round(t_plan, 6),
             "operator": op.describe() | {"nnz_struct": plan.nnz},
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"pcg_k1<{op.format}, f32>", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": k1_bytes, "avg_launch_ms": k1_avg_ms, "launches": k1_n.value,
-            "k2_gbs": round(k2_gbs, 1), "k2_avg_ms": k2_avg_ms,
+            "bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms, "sampled_iterations": prof_n.value,
+            "launch_shape": {"block": blk, "grid_cap": cap, "spmv_variant": vec, "mode": mode},
+            "kernels": kernel_rows,
         },
     }
-    del nnz_eager
 
     if not args.no_grad:
         log("[bench] forward + adjoint gradient on 64^3")
@@ -177,6 +199,23 @@ def main():
         log("[bench] CPU baseline (C + OpenMP FP64 restatement, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_seconds, st_fp64["iterations"])
     print(json.dumps(out), flush=True)
+
+
+def committed_traffic(kernel, workload):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter on a copy kernel of known size), or
+    None when no summary covers them."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if not os.path.isdir(path):
+        return None
+    for fn in sorted(os.listdir(path)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(path, fn)):
+            parts = line.strip().strip("`").split()
+            if len(parts) == 4 and parts[0] == "TRAFFIC" and parts[1] == kernel.replace(" ", "") and parts[2] == workload:
+                return int(parts[3])
+    return None
 
 
 def grad_rate(J, torch, dev, n=64, reps=3):

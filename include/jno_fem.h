@@ -184,17 +184,20 @@ int jno_pcg_set_chunk(int chunk);
 /* Launch shape of the iteration kernels, per iteration precision (no reference counterpart; set
  * by the autotuner; process-wide): precision = 0 for the FP64 iterations of jno_pcg_f64, 1 for
  * the FP32 inner iterations of jno_pcg_mixed; grid_cap = maximum workgroups (multiple of 8,
- * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for one row per thread;
- * block = threads per workgroup (256/512/1024).  jno_pcg_get_tuning reads the current shape. */
-int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block);
-int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block);
+ * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for rows per thread;
+ * block = threads per workgroup (256/512/1024); split = 1 for three kernels per iteration (p
+ * materialized, then a plain SpMV), 0 for two (p recomputed inside the SpMV kernel).
+ * jno_pcg_get_tuning reads the current shape. */
+int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split);
+int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split);
 /* Live kernel timing of the PCG iteration kernels (no reference counterpart: measurement hook).
- * When enabled, HIP events are recorded on the solve's stream around the two iteration kernels of
- * every 8th iteration and the summed durations of the sampled iterations that did work are
- * accumulated (counts in *_count); enabling resets them.
- * K1 = SpMV + p/x update + p.q, K2 = r update + r.z, r.r. */
+ * When enabled, HIP events are recorded on the solve's stream around the kernels of every 8th
+ * iteration, and the durations of the sampled iterations that did work are summed into
+ * h_ms[0] (K0: p update, split iterations only), h_ms[1] (the SpMV kernel: K1 = p recompute +
+ * SpMV + x update + p.q, or KS = SpMV + p.q) and h_ms[2] (K2: r update, r.z, r.r); *h_count is
+ * the number of sampled iterations.  Enabling resets the sums. */
 int jno_pcg_profile(int enable);
-int jno_pcg_profile_read(double* h_k1_ms, int64_t* h_k1_count, double* h_k2_ms, int64_t* h_k2_count);
+int jno_pcg_profile_read(double* h_ms, int64_t* h_count);
 int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
                 double tol, double atol, int64_t maxiter, jno_pcg_stats* h_stats, void* d_ws,
                 size_t ws_bytes, void* stream);

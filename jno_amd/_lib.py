@@ -93,10 +93,10 @@ SIGNATURES = {
     "jno_op_spmv_f32": (C.c_int, [C.POINTER(JnoOp), P, P, P]),
     "jno_pcg_workspace_size": (SZ, [I64, C.c_int]),
     "jno_pcg_set_chunk": (C.c_int, [C.c_int]),
-    "jno_pcg_set_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
-    "jno_pcg_get_tuning": (C.c_int, [C.c_int, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    "jno_pcg_set_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "jno_pcg_get_tuning": (C.c_int, [C.c_int, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "jno_pcg_profile": (C.c_int, [C.c_int]),
-    "jno_pcg_profile_read": (C.c_int, [C.POINTER(D), C.POINTER(I64), C.POINTER(D), C.POINTER(I64)]),
+    "jno_pcg_profile_read": (C.c_int, [C.POINTER(D), C.POINTER(I64)]),
     "jno_pcg_f64": (C.c_int, [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, C.POINTER(JnoPcgStats), P, SZ, P]),
     "jno_pcg_mixed": (
         C.c_int,

@@ -51,8 +51,14 @@ def choose_format(A):
     return choice
 
 
-# Launch shapes of the PCG iteration kernels tried by tune_launch: (block, grid_cap, k1_vector)
-CANDIDATE_SHAPES = tuple((blk, cap, vec) for blk in (256, 512, 1024) for cap in (256, 512, 1024) for vec in (0, 1))
+# Launch shapes of the PCG iteration kernels tried by tune_launch: (block, grid_cap, k1_vector, split)
+CANDIDATE_SHAPES = tuple(
+    (blk, cap, vec, split)
+    for blk in (256, 512, 1024)
+    for cap in (256, 512, 1024)
+    for vec in (0, 1)
+    for split in (0, 1)
+)
 _LAUNCH_CACHE: dict = {}
 
 

@@ -1,11 +1,14 @@
 // Jacobi-preconditioned CG: plain FP64 (jax.scipy.sparse.linalg.cg semantics as called by
 // jno.solve.cg, jno/solve.py:191-206) and FP32 inner iterations with FP64 residual replacement.
 //
-// One iteration is two kernels:
+// One iteration is two or three kernels (the iteration mode is a launch-tuning choice):
 //   K1  p_k = z_k + beta p_{k-1}  (z = dinv r, recomputed for every gathered neighbour, so p is
 //       never a separate pass),  x += alpha_{k-1} p_{k-1} (the x update is deferred one iteration
 //       and fused here),  q = A p_k,  partial p.q
 //   K2  alpha_k = gamma_k / p.q,  r -= alpha_k q,  partials r.z and r.r
+// or K0 (p_k and the deferred x update, own rows only), KS (q = A p_k, partial p.q), K2: more
+// launches but a plain SpMV that gathers one vector instead of three, the faster choice on the
+// 128^3 Kuhn lattice; or the single-reduction iteration KA KB further below.
 // No value crosses workgroups inside a kernel: each workgroup writes its partial sums, and every
 // workgroup of the next kernel reduces all of them itself, in one fixed order, before it starts.
 // So the scalars are bitwise identical across workgroups and runs, and there are no atomics or
@@ -30,15 +33,17 @@ constexpr int PCG_NT = 256;
 constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup partials
 
 // Launch shape of the iteration kernels, one per iteration precision (set by the autotuner through
-// jno_pcg_set_tuning).  The defaults are the fastest shapes of a sweep over block x grid cap x DIA
-// tile variant on a 2.1M-row 4-diagonal operator: FP32 iterations 1024 threads x 256 workgroups
-// with one row per thread, FP64 iterations 512 x 1024 with 16-byte vector tiles.
+// jno_pcg_set_tuning).  The defaults are the fastest shapes of sweeps over workgroup size x grid
+// cap x SpMV tile variant x iteration mode on a 2.1M-row 4-diagonal operator (128^3 Kuhn P1):
+// 1024 threads x 512 workgroups, p materialized (K0 KS K2); FP32 iterations take two rows per
+// thread in KS, FP64 iterations 16-byte vector tiles.
 struct Tuning {
   int grid_cap;  // workgroups of the iteration kernels (a multiple of 8)
-  int vec;       // DIA K1: VN-row vector tiles (1) or one row per thread (0)
+  int vec;       // DIA SpMV kernel: 0 two rows per thread, 1 VN-row 16-byte tiles, 2 four rows
   int block;     // threads per workgroup (256, 512, 1024)
+  int split;     // iteration mode: 0 K1 K2, 1 K0 KS K2 (p materialized), 2 KA KB (single reduction)
 };
-static Tuning g_tune[2] = {{1024, 1, 512}, {256, 0, 1024}};  // [0] FP64 iterations, [1] FP32 iterations
+static Tuning g_tune[2] = {{512, 1, 1024, 1}, {512, 0, 1024, 1}};  // [0] FP64 iterations, [1] FP32 iterations
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -60,13 +65,19 @@ struct PcgState {
   double bb;          // b . b (finalize)
   double rr0;         // r.r after the last residual preparation (finalize)
   int64_t maxiter;
+  // single-reduction iterations: KA stages k+1, gamma_k, alpha_k here, KB copies them to k,
+  // gamma_prev, alpha (KA reads those, so it cannot write them itself)
+  int64_t k_next;
+  double gamma_next;
+  double alpha_next;
 };
 
 // partial-sum areas (per workgroup): A = K1's p.q; B = K2's (r.z, r.r); P = prep's (rr_true, b.b)
 struct Partials {
-  double* A;  // [G]
-  double* B;  // [2][G]
-  double* P;  // [2][G]
+  double* A;   // [G]
+  double* B;   // [2][G]
+  double* P;   // [2][G]
+  double* B2;  // [2][G]: single-reduction iterations read B and write B2 on even k, and swap on odd k
 };
 
 template <typename T, typename V>
@@ -79,6 +90,7 @@ struct PcgArgs {
   int ndiag;
   int vec;    // DIA K1 tile variant
   int block;  // threads per workgroup of the launch (host side only)
+  int split;  // three kernels per iteration (host side only)
   DiaOffsets off;
   const T* dinv;
   T* r;
@@ -86,6 +98,8 @@ struct PcgArgs {
   T* p_new;
   T* q;
   T* x;
+  T* s;  // single-reduction iterations: s = A p
+  T* u;  // single-reduction iterations: u = dinv r
   PcgState* st;
   Partials part;
 };
@@ -142,10 +156,11 @@ __device__ __forceinline__ void st16(T* p, const V16<T>& r) {
 
 // DIA row tile [i0, i0+VN): p_new = z + beta p_old on the tile, q = A p_new, deferred x update;
 // returns the tile's p.q.  `interior` tiles need no bounds checks on neighbour rows.
-template <typename T, typename V>
+template <int ND, typename T, typename V>
 __device__ __forceinline__ double dia_tile(const PcgArgs<T, V>& a, int64_t i0, T beta, T ap, bool upd_x) {
   constexpr int VN = V16<T>::N;
   const int64_t ld = dia_ld(a.n);
+  const int nd = ND > 0 ? ND : a.ndiag;
   const V16<T> dv = ld16(a.dinv + i0), rv = ld16(a.r + i0), po = ld16(a.p_old + i0);
   V16<T> pi, s;
 #pragma unroll
@@ -155,7 +170,8 @@ __device__ __forceinline__ double dia_tile(const PcgArgs<T, V>& a, int64_t i0, T
 #pragma unroll
     for (int j = 0; j < VN; ++j) s.v[j] = (T)d0.v[j] * pi.v[j];
   }
-  for (int d = 1; d < a.ndiag; ++d) {
+#pragma unroll
+  for (int d = 1; d < nd; ++d) {
     const int64_t o = a.off.o[d];
     const V* vd = a.vals + (int64_t)d * ld;
     const V16<V> vu = ld16(vd + i0), vl = ld16(vd + i0 - o);
@@ -200,25 +216,77 @@ __device__ __forceinline__ double dia_row(const PcgArgs<T, V>& a, int64_t i, T b
   return (double)pi * (double)s;
 }
 
-template <int NT, int L, bool DIA, typename T, typename V>
-__global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+// R interior DIA rows i, i + NT, ..., i + (R-1) NT of one thread (all neighbours in range) with a
+// compile-time diagonal count: every load of the R rows is independent of every other, so they are
+// all in flight before the first use (a runtime diagonal loop waits on each diagonal's loads).
+template <int ND, int R, int NT, typename T, typename V>
+__device__ __forceinline__ double dia_rows(const PcgArgs<T, V>& a, int64_t i, T beta, T ap, bool upd_x) {
+  const int64_t ld = dia_ld(a.n);
+  T dv[R], rv[R], po[R], xv[R];
+  V d0[R], vu[R][ND - 1], vl[R][ND - 1];
+  T du[R][ND - 1], ru[R][ND - 1], pu[R][ND - 1], dl[R][ND - 1], rl[R][ND - 1], pl[R][ND - 1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t row = i + (int64_t)r * NT;
+    dv[r] = a.dinv[row];
+    rv[r] = a.r[row];
+    po[r] = a.p_old[row];
+    d0[r] = a.vals[row];
+    xv[r] = upd_x ? a.x[row] : T(0);
+#pragma unroll
+    for (int d = 1; d < ND; ++d) {
+      const int64_t o = a.off.o[d];
+      vu[r][d - 1] = a.vals[(int64_t)d * ld + row];
+      vl[r][d - 1] = a.vals[(int64_t)d * ld + row - o];
+      du[r][d - 1] = a.dinv[row + o];
+      ru[r][d - 1] = a.r[row + o];
+      pu[r][d - 1] = a.p_old[row + o];
+      dl[r][d - 1] = a.dinv[row - o];
+      rl[r][d - 1] = a.r[row - o];
+      pl[r][d - 1] = a.p_old[row - o];
+    }
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t row = i + (int64_t)r * NT;
+    const T pi = dv[r] * rv[r] + beta * po[r];
+    T s = (T)d0[r] * pi;
+#pragma unroll
+    for (int d = 0; d < ND - 1; ++d) {
+      s += (T)vu[r][d] * (du[r][d] * ru[r][d] + beta * pu[r][d]);
+      s += (T)vl[r][d] * (dl[r][d] * rl[r][d] + beta * pl[r][d]);
+    }
+    a.p_new[row] = pi;
+    a.q[row] = s;
+    if (upd_x) a.x[row] = xv[r] + ap * po[r];
+    part += (double)pi * (double)s;
+  }
+  return part;
+}
+
+// ND: compile-time DIA diagonal count (0 = runtime a.ndiag); L: lanes per CSR row
+// Start of an iteration (K1, or K0 of the split iteration): reduce the r.z and r.r partials of the
+// previous K2, take the stop decision of jax's cg loop (r.r <= atol2, k >= maxiter) plus the
+// residual-replacement and non-finite stops, and form beta.  On a stop the deferred x update is
+// finished, workgroup 0 records the stop in the state and false is returned.
+template <int NT, typename T, typename V>
+__device__ __forceinline__ bool iter_head(const PcgArgs<T, V>& a, double* sm, double& gamma, T& beta, T& ap,
+                                          int64_t& k, int64_t& r0, int64_t& r1) {
   PcgState* st = a.st;
-  if (st->done) return;
   const int G = gridDim.x;
-  const double gamma = reduce_partials<NT>(a.part.B, G, sm);
+  gamma = reduce_partials<NT>(a.part.B, G, sm);
   const double rr = reduce_partials<NT>(a.part.B + G, G, sm);
-  const int64_t k = st->k;
+  k = st->k;
   const double alpha_prev = st->alpha;
   const bool stop_conv = !(rr > st->atol2);
   const bool stop_cap = k >= st->maxiter;
   const bool stop_repl = rr <= st->rr_replace;
   const bool bad = !isfinite(gamma) || !isfinite(rr);
-  const T ap = (T)alpha_prev;
-  const int64_t r0 = block_first_row(a.rows_per_block);
-  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
+  ap = (T)alpha_prev;
+  r0 = block_first_row(a.rows_per_block);
+  r1 = min(a.n, r0 + a.rows_per_block);
   if (stop_conv || stop_cap || stop_repl || bad) {
-    // finish the deferred x update and stop
     if (k > 0 && alpha_prev != 0.0)
       for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) a.x[i] += ap * a.p_old[i];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -227,23 +295,50 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
       st->rr_stop = rr;
       st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
     }
-    return;
+    return false;
   }
-  const double beta_d = (k == 0) ? 0.0 : gamma / st->gamma_prev;
-  const T beta = (T)beta_d;
+  beta = (T)((k == 0) ? 0.0 : gamma / st->gamma_prev);
+  return true;
+}
+
+template <int NT, int L, bool DIA, int ND, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
+  PcgState* st = a.st;
+  if (st->done) return;
+  double gamma;
+  T beta, ap;
+  int64_t k, r0, r1;
+  if (!iter_head<NT, T, V>(a, sm, gamma, beta, ap, k, r0, r1)) return;
   double part = 0.0;
-  if (DIA && a.vec) {
+  if (DIA && a.vec == 1) {
     // each thread owns VN consecutive rows; the block's first row is VN-aligned
     constexpr int VN = V16<T>::N;
     int64_t omax = 0;
     for (int d = 1; d < a.ndiag; ++d) omax = max(omax, (int64_t)a.off.o[d]);
     for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * NT) {
       if (i0 + VN <= r1 && i0 - omax >= 0 && i0 + VN - 1 + omax < a.n) {
-        part += dia_tile<T, V>(a, i0, beta, ap, k > 0);
+        part += dia_tile<ND, T, V>(a, i0, beta, ap, k > 0);
       } else {
         for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += dia_row<T, V>(a, i, beta, ap, k > 0);
       }
     }
+  } else if (DIA && ND > 0) {
+    // R rows per thread and step, NT apart (coalesced), all loads issued before the first use
+    constexpr int R = 2;
+    int64_t omax = 0;
+#pragma unroll
+    for (int d = 1; d < (ND > 0 ? ND : 1); ++d) omax = max(omax, (int64_t)a.off.o[d]);
+    int64_t i = r0 + threadIdx.x;
+    for (; i + (int64_t)(R - 1) * NT < r1; i += (int64_t)R * NT) {
+      if (i - omax >= 0 && i + (int64_t)(R - 1) * NT + omax < a.n) {
+        part += dia_rows<(ND > 0 ? ND : 1), R, NT, T, V>(a, i, beta, ap, k > 0);
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) part += dia_row<T, V>(a, i + (int64_t)r * NT, beta, ap, k > 0);
+      }
+    }
+    for (; i < r1; i += NT) part += dia_row<T, V>(a, i, beta, ap, k > 0);
   } else if (DIA) {
     for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += dia_row<T, V>(a, i, beta, ap, k > 0);
   } else {
@@ -271,6 +366,11 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
 // ---------------------------------------------------------------------------------------------
 // K2: r -= alpha q, partials r.z and r.r
 // ---------------------------------------------------------------------------------------------
+// Rows [r0, r1) of a workgroup as VN-row tiles: thread t owns tiles t, t + NT, t + 2 NT, ...;
+// U of them are loaded together (issued before any use), rows past the last full tile (fewer than
+// VN, only at the end of the matrix) go one per thread.
+constexpr int TILE_U = 2;
+
 template <int NT, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
   __shared__ double sm[NT / 64];
@@ -280,43 +380,62 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
     return;
   }
   const int G = gridDim.x;
+  constexpr int VN = V16<T>::N;
+  constexpr int U = TILE_U;
   const int64_t r0 = block_first_row(a.rows_per_block);
   const int64_t r1 = min(a.n, r0 + a.rows_per_block);
-  constexpr int VN = V16<T>::N;
-  // the first tile's loads are issued before the partial-sum reduction, so they overlap it
-  const int64_t f0 = r0 + (int64_t)threadIdx.x * VN;
-  const bool pre = f0 + VN <= r1;
-  V16<T> rv0, qv0, dv0;
-  if (pre) {
-    rv0 = ld16(a.r + f0);
-    qv0 = ld16(a.q + f0);
-    dv0 = ld16(a.dinv + f0);
+  const int64_t r1v = r0 < r1 ? r0 + ((r1 - r0) / VN) * VN : r0;
+  int64_t t = r0 + (int64_t)threadIdx.x * VN;
+  // the first step's loads are issued before the partial-sum reduction, so they overlap it
+  V16<T> rv[U], qv[U], dv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i0 = t + (int64_t)u * VN * NT;
+    if (i0 < r1v) {
+      rv[u] = ld16(a.r + i0);
+      qv[u] = ld16(a.q + i0);
+      dv[u] = ld16(a.dinv + i0);
+    }
   }
   const double pq = reduce_partials<NT>(a.part.A, G, sm);
   const double gamma = st->gamma_cur;
   const double alpha = gamma / pq;
   const T al = (T)alpha;
   double pg = 0.0, pr = 0.0;
-  for (int64_t i0 = f0; i0 < r1; i0 += (int64_t)VN * NT) {
-    if (i0 + VN <= r1) {
-      const bool first = pre && i0 == f0;
-      V16<T> rv = first ? rv0 : ld16(a.r + i0);
-      const V16<T> qv = first ? qv0 : ld16(a.q + i0), dv = first ? dv0 : ld16(a.dinv + i0);
+  for (;;) {
 #pragma unroll
-      for (int j = 0; j < VN; ++j) {
-        const T ri = rv.v[j] - al * qv.v[j];
-        rv.v[j] = ri;
-        pg += (double)ri * (double)(dv.v[j] * ri);
-        pr += (double)ri * (double)ri;
+    for (int u = 0; u < U; ++u) {
+      const int64_t i0 = t + (int64_t)u * VN * NT;
+      if (i0 < r1v) {
+#pragma unroll
+        for (int j = 0; j < VN; ++j) {
+          const T ri = rv[u].v[j] - al * qv[u].v[j];
+          rv[u].v[j] = ri;
+          pg += (double)ri * (double)(dv[u].v[j] * ri);
+          pr += (double)ri * (double)ri;
+        }
+        st16(a.r + i0, rv[u]);
       }
-      st16(a.r + i0, rv);
-    } else {
-      for (int64_t i = i0; i < r1; ++i) {
-        const T ri = a.r[i] - al * a.q[i];
-        a.r[i] = ri;
-        pg += (double)ri * (double)(a.dinv[i] * ri);
-        pr += (double)ri * (double)ri;
+    }
+    t += (int64_t)U * VN * NT;
+    if (t >= r1v) break;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i0 = t + (int64_t)u * VN * NT;
+      if (i0 < r1v) {
+        rv[u] = ld16(a.r + i0);
+        qv[u] = ld16(a.q + i0);
+        dv[u] = ld16(a.dinv + i0);
       }
+    }
+  }
+  {
+    const int64_t i = r1v + threadIdx.x;
+    if (i < r1) {
+      const T ri = a.r[i] - al * a.q[i];
+      a.r[i] = ri;
+      pg += (double)ri * (double)(a.dinv[i] * ri);
+      pr += (double)ri * (double)ri;
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -329,6 +448,337 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Split iteration (three kernels): K0 materializes p, KS is a plain SpMV on it, K2 as above.
+// Fewer bytes move per row in the SpMV (one neighbour vector instead of r, dinv and p_old), at the
+// cost of a third launch and of writing and re-reading p once.
+// ---------------------------------------------------------------------------------------------
+template <int NT, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_k0(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
+  PcgState* st = a.st;
+  if (st->done) return;
+  constexpr int VN = V16<T>::N;
+  constexpr int U = TILE_U;
+  const bool upd_x = st->k > 0;
+  const int64_t b0 = block_first_row(a.rows_per_block);
+  const int64_t b1 = min(a.n, b0 + a.rows_per_block);
+  const int64_t r1v = b0 < b1 ? b0 + ((b1 - b0) / VN) * VN : b0;
+  int64_t t = b0 + (int64_t)threadIdx.x * VN;
+  // the first step's loads are issued before the partial-sum reduction, so they overlap it
+  V16<T> dv[U], rv[U], po[U], xv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t i0 = t + (int64_t)u * VN * NT;
+    if (i0 < r1v) {
+      dv[u] = ld16(a.dinv + i0);
+      rv[u] = ld16(a.r + i0);
+      po[u] = ld16(a.p_old + i0);
+      if (upd_x) xv[u] = ld16(a.x + i0);
+    }
+  }
+  double gamma;
+  T beta, ap;
+  int64_t k, r0, r1;
+  if (!iter_head<NT, T, V>(a, sm, gamma, beta, ap, k, r0, r1)) return;
+  for (;;) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i0 = t + (int64_t)u * VN * NT;
+      if (i0 < r1v) {
+        V16<T> pi;
+#pragma unroll
+        for (int j = 0; j < VN; ++j) pi.v[j] = dv[u].v[j] * rv[u].v[j] + beta * po[u].v[j];
+        st16(a.p_new + i0, pi);
+        if (upd_x) {
+#pragma unroll
+          for (int j = 0; j < VN; ++j) xv[u].v[j] += ap * po[u].v[j];
+          st16(a.x + i0, xv[u]);
+        }
+      }
+    }
+    t += (int64_t)U * VN * NT;
+    if (t >= r1v) break;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i0 = t + (int64_t)u * VN * NT;
+      if (i0 < r1v) {
+        dv[u] = ld16(a.dinv + i0);
+        rv[u] = ld16(a.r + i0);
+        po[u] = ld16(a.p_old + i0);
+        if (upd_x) xv[u] = ld16(a.x + i0);
+      }
+    }
+  }
+  {
+    const int64_t i = r1v + threadIdx.x;
+    if (i < r1) {
+      const T p0 = a.p_old[i];
+      a.p_new[i] = a.dinv[i] * a.r[i] + beta * p0;
+      if (upd_x) a.x[i] += ap * p0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
+}
+
+// one DIA row of q = A p with bounds checks
+template <typename T, typename V>
+__device__ __forceinline__ double spmv_dia_row(const PcgArgs<T, V>& a, int64_t i) {
+  const int64_t ld = dia_ld(a.n);
+  const T* p = a.p_new;
+  const T pi = p[i];
+  T s = (T)a.vals[i] * pi;
+  for (int d = 1; d < a.ndiag; ++d) {
+    const int64_t o = a.off.o[d];
+    const V* v = a.vals + (int64_t)d * ld;
+    if (i + o < a.n) s += (T)v[i] * p[i + o];
+    if (i - o >= 0) s += (T)v[i - o] * p[i - o];
+  }
+  a.q[i] = s;
+  return (double)pi * (double)s;
+}
+
+// R interior DIA rows i, i + NT, ... of q = A p, all loads in flight before the first use
+template <int ND, int R, int NT, typename T, typename V>
+__device__ __forceinline__ double spmv_dia_rows(const PcgArgs<T, V>& a, int64_t i) {
+  const int64_t ld = dia_ld(a.n);
+  const T* p = a.p_new;
+  T pv[R], pu[R][ND - 1], pl[R][ND - 1];
+  V d0[R], vu[R][ND - 1], vl[R][ND - 1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t row = i + (int64_t)r * NT;
+    pv[r] = p[row];
+    d0[r] = a.vals[row];
+#pragma unroll
+    for (int d = 1; d < ND; ++d) {
+      const int64_t o = a.off.o[d];
+      vu[r][d - 1] = a.vals[(int64_t)d * ld + row];
+      vl[r][d - 1] = a.vals[(int64_t)d * ld + row - o];
+      pu[r][d - 1] = p[row + o];
+      pl[r][d - 1] = p[row - o];
+    }
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T s = (T)d0[r] * pv[r];
+#pragma unroll
+    for (int d = 0; d < ND - 1; ++d) {
+      s += (T)vu[r][d] * pu[r][d];
+      s += (T)vl[r][d] * pl[r][d];
+    }
+    a.q[i + (int64_t)r * NT] = s;
+    part += (double)pv[r] * (double)s;
+  }
+  return part;
+}
+
+// rows [r0, r1) of q = A p, R rows per thread and step (NT apart, coalesced)
+template <int ND, int R, int NT, typename T, typename V>
+__device__ __forceinline__ double spmv_dia_scalar(const PcgArgs<T, V>& a, int64_t r0, int64_t r1, int64_t omax) {
+  double part = 0.0;
+  int64_t i = r0 + threadIdx.x;
+  for (; i + (int64_t)(R - 1) * NT < r1; i += (int64_t)R * NT) {
+    if (i - omax >= 0 && i + (int64_t)(R - 1) * NT + omax < a.n) {
+      part += spmv_dia_rows<ND, R, NT, T, V>(a, i);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) part += spmv_dia_row<T, V>(a, i + (int64_t)r * NT);
+    }
+  }
+  for (; i < r1; i += NT) part += spmv_dia_row<T, V>(a, i);
+  return part;
+}
+
+// VN-row tile [i0, i0 + VN) of q = A p with 16-byte loads (interior tiles only)
+template <int ND, typename T, typename V>
+__device__ __forceinline__ double spmv_dia_tile(const PcgArgs<T, V>& a, int64_t i0) {
+  constexpr int VN = V16<T>::N;
+  const int64_t ld = dia_ld(a.n);
+  const int nd = ND > 0 ? ND : a.ndiag;
+  const T* p = a.p_new;
+  const V16<T> pv = ld16(p + i0);
+  V16<T> s;
+  {
+    const V16<V> d0 = ld16(a.vals + i0);
+#pragma unroll
+    for (int j = 0; j < VN; ++j) s.v[j] = (T)d0.v[j] * pv.v[j];
+  }
+#pragma unroll
+  for (int d = 1; d < nd; ++d) {
+    const int64_t o = a.off.o[d];
+    const V* vd = a.vals + (int64_t)d * ld;
+    const V16<V> vu = ld16(vd + i0), vl = ld16(vd + i0 - o);
+    const V16<T> pu = ld16(p + i0 + o), pl = ld16(p + i0 - o);
+#pragma unroll
+    for (int j = 0; j < VN; ++j) {
+      s.v[j] += (T)vu.v[j] * pu.v[j];
+      s.v[j] += (T)vl.v[j] * pl.v[j];
+    }
+  }
+  st16(a.q + i0, s);
+  double part = 0.0;
+#pragma unroll
+  for (int j = 0; j < VN; ++j) part += (double)pv.v[j] * (double)s.v[j];
+  return part;
+}
+
+// KS: q = A p (p materialized by K0), partial p.q.  With CG set it is KB of the single-reduction
+// iteration (launched with p_new = u, q = w): w = A u, partial u.w, and workgroup 0 moves the
+// scalars KA staged into the fields KA reads.
+template <int NT, int L, bool DIA, int ND, bool CG, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_ks(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
+  PcgState* st = a.st;
+  if (st->done1) {
+    if (CG && blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+    return;
+  }
+  if (CG && blockIdx.x == 0 && threadIdx.x == 0) {
+    st->k = st->k_next;
+    st->gamma_prev = st->gamma_next;
+    st->alpha = st->alpha_next;
+  }
+  const int64_t r0 = block_first_row(a.rows_per_block);
+  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
+  const T* p = a.p_new;
+  double part = 0.0;
+  if (DIA) {
+    int64_t omax = 0;
+    for (int d = 1; d < a.ndiag; ++d) omax = max(omax, (int64_t)a.off.o[d]);
+    if (a.vec == 1) {
+      constexpr int VN = V16<T>::N;
+      for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * NT) {
+        if (i0 + VN <= r1 && i0 - omax >= 0 && i0 + VN - 1 + omax < a.n)
+          part += spmv_dia_tile<ND, T, V>(a, i0);
+        else
+          for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += spmv_dia_row<T, V>(a, i);
+      }
+    } else if (ND > 0) {
+      part = a.vec == 2 ? spmv_dia_scalar<(ND > 0 ? ND : 1), 4, NT, T, V>(a, r0, r1, omax)
+                        : spmv_dia_scalar<(ND > 0 ? ND : 1), 2, NT, T, V>(a, r0, r1, omax);
+    } else {
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += spmv_dia_row<T, V>(a, i);
+    }
+  } else {
+    const int sub = threadIdx.x % L;
+    for (int64_t row = r0 + threadIdx.x / L; row < r1; row += NT / L) {
+      const int b = a.rowptr[row], e = a.rowptr[row + 1];
+      T s = 0;
+      for (int j = b + sub; j < e; j += L) s += (T)a.vals[j] * p[a.col[j]];
+#pragma unroll
+      for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+      if (sub == 0) {
+        a.q[row] = s;
+        part += (double)p[row] * (double)s;
+      }
+    }
+  }
+  store_partial<NT>(part, a.part.A, sm);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Single-reduction iteration (Chronopoulos & Gear, J. Comput. Appl. Math. 25 (1989) 153-168):
+// with u = M r, w = A u, gamma = r.u and delta = u.w reduced together,
+//   beta_k = gamma_k / gamma_{k-1},  alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//   p = u + beta p,  s = w + beta s  (s = A p by recursion),  x += alpha p,  r -= alpha s
+// so the SpMV acts on u, which is ready as soon as r is updated: two kernels per iteration, KA
+// (all vector updates, u, partials r.u and r.r) and KB (w = A u, partial u.w), and one reduction.
+// It is the same Krylov iteration as CG in exact arithmetic; the rounding differs, which is why
+// it serves the FP32 inner iterations (closed by FP64 residual replacement), not the plain FP64
+// solve.
+// ---------------------------------------------------------------------------------------------
+template <int NT, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_ka(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
+  PcgState* st = a.st;
+  if (st->done) return;
+  const int G = gridDim.x;
+  const int64_t k = st->k;
+  // KA both reads and writes r.u / r.r partials: the previous KA's in one buffer, its own in the
+  // other, chosen by the parity of k (the same for every workgroup)
+  const double* b_in = (k & 1) ? a.part.B2 : a.part.B;
+  double* b_out = (k & 1) ? a.part.B : a.part.B2;
+  const double gamma = reduce_partials<NT>(b_in, G, sm);
+  const double rr = reduce_partials<NT>(b_in + G, G, sm);
+  const double delta = reduce_partials<NT>(a.part.A, G, sm);
+  const bool stop_conv = !(rr > st->atol2);
+  const bool stop_cap = k >= st->maxiter;
+  const bool stop_repl = rr <= st->rr_replace;
+  const bool bad_in = !isfinite(gamma) || !isfinite(rr) || !isfinite(delta);
+  double beta = 0.0, den = delta;
+  if (k > 0) {
+    beta = gamma / st->gamma_prev;
+    den = delta - beta * gamma / st->alpha;
+  }
+  const double alpha = gamma / den;
+  const bool bad = bad_in || (!stop_conv && !isfinite(alpha));
+  if (stop_conv || stop_cap || stop_repl || bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done1 = 1;
+      st->k_stop = k;
+      st->rr_stop = rr;
+      st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
+    }
+    return;
+  }
+  const T be = (T)beta, al = (T)alpha;
+  const int64_t r0 = block_first_row(a.rows_per_block);
+  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
+  constexpr int VN = V16<T>::N;
+  double pg = 0.0, pr = 0.0;
+  for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * NT) {
+    if (i0 + VN <= r1) {
+      V16<T> rv = ld16(a.r + i0), pv = ld16(a.p_new + i0), sv = ld16(a.s + i0), xv = ld16(a.x + i0);
+      const V16<T> dv = ld16(a.dinv + i0), wv = ld16(a.q + i0);
+      V16<T> uv;
+#pragma unroll
+      for (int j = 0; j < VN; ++j) {
+        const T u = dv.v[j] * rv.v[j];
+        const T p = u + be * pv.v[j];
+        const T s = wv.v[j] + be * sv.v[j];
+        xv.v[j] += al * p;
+        const T r = rv.v[j] - al * s;
+        const T un = dv.v[j] * r;
+        pv.v[j] = p;
+        sv.v[j] = s;
+        rv.v[j] = r;
+        uv.v[j] = un;
+        pg += (double)r * (double)un;
+        pr += (double)r * (double)r;
+      }
+      st16(a.p_new + i0, pv);
+      st16(a.s + i0, sv);
+      st16(a.x + i0, xv);
+      st16(a.r + i0, rv);
+      st16(a.u + i0, uv);
+    } else {
+      for (int64_t i = i0; i < r1; ++i) {
+        const T d = a.dinv[i];
+        const T p = d * a.r[i] + be * a.p_new[i];
+        const T s = a.q[i] + be * a.s[i];
+        a.x[i] += al * p;
+        const T r = a.r[i] - al * s;
+        const T un = d * r;
+        a.p_new[i] = p;
+        a.s[i] = s;
+        a.r[i] = r;
+        a.u[i] = un;
+        pg += (double)r * (double)un;
+        pr += (double)r * (double)r;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->k_next = k + 1;
+    st->gamma_next = gamma;
+    st->alpha_next = alpha;
+  }
+  store_partial<NT>(pg, b_out, sm);
+  store_partial<NT>(pr, b_out + G, sm);
+}
+
+// ---------------------------------------------------------------------------------------------
 // residual preparation: r64 = b - Ax (Ax nullable = 0), rs = (T) r64; partials of
 // gamma = rs.dinv.rs, rr = rs.rs (into B, for the next K1) and of ||r64||^2, b.b (into P)
 // ---------------------------------------------------------------------------------------------
@@ -336,7 +786,7 @@ template <typename T>
 __global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, int64_t rows_per_block,
                                                           const double* __restrict__ b, const double* __restrict__ Ax,
                                                           const T* __restrict__ dinv, T* __restrict__ rs, Partials part,
-                                                          int write_rs) {
+                                                          int write_rs, T* __restrict__ u_out) {
   __shared__ double sm[PCG_NT / 64];
   const int G = gridDim.x;
   const int64_t r0 = block_first_row(rows_per_block);
@@ -349,7 +799,9 @@ __global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, int64_t row
     pb += bi * bi;
     const T ri = (T)r;
     if (write_rs) rs[i] = ri;
-    pg += (double)ri * (double)(dinv[i] * ri);
+    const T ui = dinv[i] * ri;
+    if (u_out) u_out[i] = ui;
+    pg += (double)ri * (double)ui;
     pr += (double)ri * (double)ri;
   }
   store_partial(pg, part.B, sm);
@@ -360,7 +812,8 @@ __global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, int64_t row
 
 // one workgroup: reduce the prep partials into the state and set the tolerances
 __global__ void __launch_bounds__(PCG_NT) finalize_k(PcgState* st, Partials part, int G, double tol, double atol,
-                                                     int64_t maxiter, double replace2, int first, int restart) {
+                                                     int64_t maxiter, double replace2, int first, int restart,
+                                                     int single_reduction) {
   __shared__ double sm[PCG_NT / 64];
   const double rt = reduce_partials(part.P, G, sm);
   const double bb = reduce_partials(part.P + G, G, sm);
@@ -375,8 +828,15 @@ __global__ void __launch_bounds__(PCG_NT) finalize_k(PcgState* st, Partials part
   st->done = 0;
   st->done1 = 0;
   st->status = ST_RUNNING;
-  st->alpha = 0.0;  // any deferred x update was applied at the stop
-  if (first) st->k = 0;
+  // two-reduction iterations: any deferred x update was applied at the stop; single-reduction
+  // iterations update x at once and need alpha_{k-1} in the next alpha
+  if (!single_reduction) st->alpha = 0.0;
+  if (first) {
+    st->k = 0;
+    st->k_next = 0;
+    st->gamma_next = 0.0;
+    st->alpha_next = 0.0;
+  }
   st->rr_replace = replace2 > 0.0 ? replace2 * rr : -1.0;
 }
 
@@ -410,6 +870,7 @@ struct Ws {
   void* p0;
   void* p1;
   void* q;
+  void* s;       // single-reduction iterations: A p
   void* xs;      // FP32 accumulator (mixed)
   float* dinv32; // mixed
   double* ax;    // FP64 A x
@@ -428,10 +889,12 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
   t.part.A = (double*)take(PCG_MAX_BLOCKS * sizeof(double));
   t.part.B = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.part.P = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
+  t.part.B2 = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.r = take(n * vb);
   t.p0 = take(n * vb);
   t.p1 = take(n * vb);
   t.q = take(n * vb);
+  t.s = take(n * vb);
   t.xs = mixed ? take(n * 4) : nullptr;
   t.dinv32 = mixed ? (float*)take(n * 4) : nullptr;
   t.ax = (double*)take(n * 8);
@@ -466,38 +929,105 @@ struct HostPoll {
 };
 static thread_local HostPoll g_poll;
 
-template <int NT, typename T, typename V>
-static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t after_k1,
-                          hipEvent_t after_k2) {
+// Iteration modes (Tuning::split): two-reduction CG as K1 K2 (p recomputed inside the SpMV
+// kernel) or K0 KS K2 (p materialized), or single-reduction CG as KA KB.
+enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2 };
+
+// The SpMV-carrying kernel of one iteration: K1 (MODE_FUSED), KS (MODE_SPLIT) or KB
+// (MODE_SINGLE), by format, DIA diagonal count (4: the 3-D Kuhn lattice; 3: the 2-D triangle
+// lattice) and CSR lanes per row
+template <int NT, int MODE, typename T, typename V>
+static int launch_spmv_kernel(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+#define JNO_K(L_, DIA_, ND_)                                                          \
+  do {                                                                                \
+    if (MODE == MODE_FUSED)                                                           \
+      pcg_k1<NT, L_, DIA_, ND_, T, V><<<grid, NT, 0, s>>>(a);                         \
+    else                                                                              \
+      pcg_ks<NT, L_, DIA_, ND_, MODE == MODE_SINGLE, T, V><<<grid, NT, 0, s>>>(a);    \
+  } while (0)
   if (op->format == JNO_FMT_DIA) {
-    pcg_k1<NT, 1, true, T, V><<<grid, NT, 0, s>>>(a);
+    switch (op->ndiag) {
+      case 4: JNO_K(1, true, 4); break;
+      case 3: JNO_K(1, true, 3); break;
+      default: JNO_K(1, true, 0); break;
+    }
   } else {
     switch (op->lanes) {
-      case 1: pcg_k1<NT, 1, false, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 2: pcg_k1<NT, 2, false, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 4: pcg_k1<NT, 4, false, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 8: pcg_k1<NT, 8, false, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 16: pcg_k1<NT, 16, false, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 32: pcg_k1<NT, 32, false, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 64: pcg_k1<NT, 64, false, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 1: JNO_K(1, false, 0); break;
+      case 2: JNO_K(2, false, 0); break;
+      case 4: JNO_K(4, false, 0); break;
+      case 8: JNO_K(8, false, 0); break;
+      case 16: JNO_K(16, false, 0); break;
+      case 32: JNO_K(32, false, 0); break;
+      case 64: JNO_K(64, false, 0); break;
       default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: lanes per row must be a power of two in 1..64");
     }
   }
+#undef JNO_K
   JNO_LAUNCH_CHECK();
-  if (after_k1) JNO_HIP(hipEventRecord(after_k1, s));
+  return JNO_OK;
+}
+
+// KB of the single-reduction iteration: w (in q) = A u, partial u.w (the SpMV kernel reads p_new)
+template <typename T, typename V>
+static PcgArgs<T, V> kb_args(const PcgArgs<T, V>& a) {
+  PcgArgs<T, V> b = a;
+  b.p_new = a.u;
+  return b;
+}
+
+template <int NT, typename T, typename V>
+static int launch_kb_nt(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  return launch_spmv_kernel<NT, MODE_SINGLE, T, V>(op, kb_args(a), grid, s);
+}
+
+template <typename T, typename V>
+static int launch_kb(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  switch (a.block) {
+    case 256: return launch_kb_nt<256, T, V>(op, a, grid, s);
+    case 512: return launch_kb_nt<512, T, V>(op, a, grid, s);
+    default: return launch_kb_nt<1024, T, V>(op, a, grid, s);
+  }
+}
+
+// One iteration: K1 K2 | K0 KS K2 | KA KB.  ev (optional) receives three events: after K0 or KA
+// (at the start for K1 K2), after the SpMV kernel, after K2 (right after KB for KA KB).
+template <int NT, typename T, typename V>
+static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
+  int rc;
+  if (a.split == MODE_SINGLE) {
+    pcg_ka<NT, T, V><<<grid, NT, 0, s>>>(a);
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    rc = launch_kb_nt<NT, T, V>(op, a, grid, s);
+    if (rc) return rc;
+    if (ev) JNO_HIP(hipEventRecord(ev[1], s));
+    if (ev) JNO_HIP(hipEventRecord(ev[2], s));
+    return JNO_OK;
+  }
+  if (a.split == MODE_SPLIT) {
+    pcg_k0<NT, T, V><<<grid, NT, 0, s>>>(a);
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    rc = launch_spmv_kernel<NT, MODE_SPLIT, T, V>(op, a, grid, s);
+  } else {
+    if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    rc = launch_spmv_kernel<NT, MODE_FUSED, T, V>(op, a, grid, s);
+  }
+  if (rc) return rc;
+  if (ev) JNO_HIP(hipEventRecord(ev[1], s));
   pcg_k2<NT, T, V><<<grid, NT, 0, s>>>(a);
   JNO_LAUNCH_CHECK();
-  if (after_k2) JNO_HIP(hipEventRecord(after_k2, s));
+  if (ev) JNO_HIP(hipEventRecord(ev[2], s));
   return JNO_OK;
 }
 
 template <typename T, typename V>
-static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t after_k1,
-                       hipEvent_t after_k2) {
+static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
   switch (a.block) {
-    case 256: return launch_iter_nt<256, T, V>(op, a, grid, s, after_k1, after_k2);
-    case 512: return launch_iter_nt<512, T, V>(op, a, grid, s, after_k1, after_k2);
-    default: return launch_iter_nt<1024, T, V>(op, a, grid, s, after_k1, after_k2);
+    case 256: return launch_iter_nt<256, T, V>(op, a, grid, s, ev);
+    case 512: return launch_iter_nt<512, T, V>(op, a, grid, s, ev);
+    default: return launch_iter_nt<1024, T, V>(op, a, grid, s, ev);
   }
 }
 
@@ -508,12 +1038,14 @@ struct KernelProf {
   bool on = false;
   int stride = 8;  // time every stride-th iteration (events perturb the queue very little then)
   std::vector<hipEvent_t> ev;
-  double k1_ms = 0, k2_ms = 0;
-  int64_t k1_n = 0, k2_n = 0;
+  double ms[3] = {0, 0, 0};  // K0 (split iterations only), SpMV kernel (K1 or KS), K2
+  int64_t n = 0;             // sampled iterations
   int ensure(size_t n) {
     while (ev.size() < n) {
       hipEvent_t e;
-      JNO_HIP(hipEventCreate(&e));
+      // timing events only: without the system-scope fence a record does not flush the caches
+      // between the kernels it brackets
+      JNO_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       ev.push_back(e);
     }
     return JNO_OK;
@@ -529,20 +1061,20 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   int64_t k = k0;
   int slot = 0, pending = 0;
   size_t nev = 0;
-  std::vector<int64_t> sampled;  // iteration index of each (E0, E1, E2) event triple
+  std::vector<int64_t> sampled;  // iteration index of each (start, K0, SpMV, K2) event quadruple
   for (;;) {
     for (int i = 0; i < chunk; ++i, ++k) {
       a.p_old = pbuf[k & 1];
       a.p_new = pbuf[(k + 1) & 1];
       if (g_prof.on && k % g_prof.stride == 0) {
-        if (g_prof.ensure(nev + 3) != JNO_OK) return JNO_EHIP;
+        if (g_prof.ensure(nev + 4) != JNO_OK) return JNO_EHIP;
         JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
-        int rc = launch_iter<T, V>(op, a, grid, s, g_prof.ev[nev + 1], g_prof.ev[nev + 2]);
-        nev += 3;
+        int rc = launch_iter<T, V>(op, a, grid, s, &g_prof.ev[nev + 1]);
+        nev += 4;
         sampled.push_back(k);
         if (rc) return rc;
       } else {
-        int rc = launch_iter<T, V>(op, a, grid, s, nullptr, nullptr);
+        int rc = launch_iter<T, V>(op, a, grid, s, nullptr);
         if (rc) return rc;
       }
     }
@@ -562,13 +1094,12 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   if (g_prof.on) {
     for (size_t t = 0; t < sampled.size(); ++t) {
       if (sampled[t] >= out->k_stop) break;  // launches after the stop did no work
-      float t1 = 0, t2 = 0;
-      JNO_HIP(hipEventElapsedTime(&t1, g_prof.ev[3 * t], g_prof.ev[3 * t + 1]));
-      JNO_HIP(hipEventElapsedTime(&t2, g_prof.ev[3 * t + 1], g_prof.ev[3 * t + 2]));
-      g_prof.k1_ms += t1;
-      g_prof.k2_ms += t2;
-      g_prof.k1_n += 1;
-      g_prof.k2_n += 1;
+      for (int j = 0; j < 3; ++j) {
+        float ms = 0;
+        JNO_HIP(hipEventElapsedTime(&ms, g_prof.ev[4 * t + j], g_prof.ev[4 * t + j + 1]));
+        g_prof.ms[j] += ms;
+      }
+      g_prof.n += 1;
     }
   }
   return JNO_OK;
@@ -619,6 +1150,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.ndiag = op->ndiag;
   a.vec = tu.vec;
   a.block = tu.block;
+  a.split = tu.split;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
   a.dinv = dinv;
   a.r = r;
@@ -628,6 +1160,8 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.part = w.part;
   a.p_old = nullptr;
   a.p_new = nullptr;
+  a.s = (T*)w.s;
+  a.u = (T*)w.p1;  // single-reduction iterations keep p in place in p0; p1 holds u
   return a;
 }
 
@@ -641,36 +1175,40 @@ extern "C" size_t jno_pcg_workspace_size(int64_t n, int mixed) { return ws_layou
 
 extern "C" int jno_pcg_profile(int enable) {
   g_prof.on = enable != 0;
-  g_prof.k1_ms = g_prof.k2_ms = 0;
-  g_prof.k1_n = g_prof.k2_n = 0;
+  g_prof.ms[0] = g_prof.ms[1] = g_prof.ms[2] = 0;
+  g_prof.n = 0;
   return JNO_OK;
 }
 
-extern "C" int jno_pcg_profile_read(double* h_k1_ms, int64_t* h_k1_n, double* h_k2_ms, int64_t* h_k2_n) {
-  if (h_k1_ms) *h_k1_ms = g_prof.k1_ms;
-  if (h_k1_n) *h_k1_n = g_prof.k1_n;
-  if (h_k2_ms) *h_k2_ms = g_prof.k2_ms;
-  if (h_k2_n) *h_k2_n = g_prof.k2_n;
+extern "C" int jno_pcg_profile_read(double* h_ms, int64_t* h_count) {
+  if (h_ms)
+    for (int j = 0; j < 3; ++j) h_ms[j] = g_prof.ms[j];
+  if (h_count) *h_count = g_prof.n;
   return JNO_OK;
 }
 
-extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block) {
+extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split) {
   JNO_REQUIRE(precision == 0 || precision == 1, JNO_EINVAL,
               "jno_pcg_set_tuning: precision must be 0 (FP64 iterations) or 1 (FP32 iterations)");
   JNO_REQUIRE(grid_cap >= 8 && grid_cap <= PCG_MAX_BLOCKS && grid_cap % 8 == 0, JNO_EINVAL,
               "jno_pcg_set_tuning: grid_cap must be a multiple of 8 in 8..%d", PCG_MAX_BLOCKS);
   JNO_REQUIRE(block == 256 || block == 512 || block == 1024, JNO_EINVAL,
               "jno_pcg_set_tuning: block must be 256, 512 or 1024");
-  g_tune[precision] = Tuning{grid_cap, k1_vector ? 1 : 0, block};
+  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_SINGLE, JNO_EINVAL,
+              "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2) or 2 (single reduction, KA KB)");
+  JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 2, JNO_EINVAL,
+              "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles) or 2 (4 rows)");
+  g_tune[precision] = Tuning{grid_cap, k1_vector, block, split};
   return JNO_OK;
 }
 
-extern "C" int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block) {
+extern "C" int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split) {
   JNO_REQUIRE(precision == 0 || precision == 1, JNO_EINVAL,
               "jno_pcg_get_tuning: precision must be 0 (FP64 iterations) or 1 (FP32 iterations)");
   if (grid_cap) *grid_cap = g_tune[precision].grid_cap;
   if (k1_vector) *k1_vector = g_tune[precision].vec;
   if (block) *block = g_tune[precision].block;
+  if (split) *split = g_tune[precision].split;
   return JNO_OK;
 }
 
@@ -699,28 +1237,35 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   double* r = (double*)w.r;
-  double* pb[2] = {(double*)w.p0, (double*)w.p1};
-  zero_k<double><<<gz, PCG_NT, 0, s>>>(pb[0], n);
-  zero_k<double><<<gz, PCG_NT, 0, s>>>(pb[1], n);
+  const bool single = tu.split == MODE_SINGLE;
+  double* pb[2] = {(double*)w.p0, single ? (double*)w.p0 : (double*)w.p1};
+  zero_k<double><<<gz, PCG_NT, 0, s>>>((double*)w.p0, n);
+  zero_k<double><<<gz, PCG_NT, 0, s>>>((double*)w.p1, n);
+  zero_k<double><<<gz, PCG_NT, 0, s>>>((double*)w.s, n);
   if (use_x0) {
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
   } else {
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
   }
-  residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, r, w.part, 1);
-  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 1, 1);
+  residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, r, w.part, 1,
+                                                single ? (double*)w.p1 : nullptr);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 1, 1, single);
   JNO_LAUNCH_CHECK();
   PcgArgs<double, double> a =
       make_args<double, double>(h_op, h_op->vals64, d_dinv, r, (double*)w.q, d_x, w, rpb, tu);
+  if (single) {
+    rc = launch_kb<double, double>(h_op, a, G, s);  // w = A u, u.w for the first KA
+    if (rc) return rc;
+  }
   PcgState f;
   rc = run_inner<double, double>(h_op, a, pb, 0, g_chunk, G, s, &f);
   if (rc) return rc;
   // true residual ||b - A x|| / ||b||  (the eager guard of _maybe_residual_check)
   rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
   if (rc) return rc;
-  residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, r, w.part, 0);
-  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 0, 0);
+  residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, r, w.part, 0, nullptr);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 0, 0, single);
   JNO_LAUNCH_CHECK();
   PcgState e;
   rc = read_state(w.st, &e, s);
@@ -758,10 +1303,13 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   float* rs = (float*)w.r;
-  float* pb[2] = {(float*)w.p0, (float*)w.p1};
+  const bool single = tu.split == MODE_SINGLE;
+  float* pb[2] = {(float*)w.p0, single ? (float*)w.p0 : (float*)w.p1};
   float* xs = (float*)w.xs;
-  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[0], n);
-  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[1], n);
+  float* u_out = single ? (float*)w.p1 : nullptr;
+  zero_k<float><<<gz, PCG_NT, 0, s>>>((float*)w.p0, n);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>((float*)w.p1, n);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>((float*)w.s, n);
   zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
   cast_f32_vec_k<<<gz, PCG_NT, 0, s>>>(n, d_dinv, w.dinv32);
   if (use_x0) {
@@ -771,10 +1319,15 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
   }
   const double repl2 = replace_factor * replace_factor;
-  residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, w.dinv32, rs, w.part, 1);
-  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, repl2, 1, 1);
+  residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, w.dinv32, rs, w.part, 1,
+                                               u_out);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, repl2, 1, 1, single);
   JNO_LAUNCH_CHECK();
   PcgArgs<float, float> a = make_args<float, float>(h_op, h_op->vals32, w.dinv32, rs, (float*)w.q, xs, w, rpb, tu);
+  if (single) {
+    rc = launch_kb<float, float>(h_op, a, G, s);  // w = A u, u.w for the first KA
+    if (rc) return rc;
+  }
   int64_t k = 0, refinements = 0;
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;
@@ -787,11 +1340,19 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
     JNO_LAUNCH_CHECK();
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
-    residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, w.dinv32, rs, w.part, 1);
-    // sets the state for a restart too (alpha = 0: the deferred x update of the next K1 is void,
-    // it was applied at the stop; p_{k-1} and gamma_{k-1} are kept)
-    finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, repl2, 0, 1);
+    // the single-reduction KA of iteration k reads the r.u / r.r partials of buffer (k odd ? B2 : B)
+    Partials pp = w.part;
+    if (single && (k & 1)) pp.B = w.part.B2;
+    residual_prep_k<float><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, w.dinv32, rs, pp, 1, u_out);
+    // sets the state for a restart too (two-reduction iterations: alpha = 0, the deferred x update
+    // of the next K1 is void, it was applied at the stop; p_{k-1} and gamma_{k-1} are kept; the
+    // single-reduction iteration keeps p, s = A p, gamma_{k-1}, alpha_{k-1} and needs w = A u)
+    finalize_k<<<1, PCG_NT, 0, s>>>(w.st, pp, G, tol, atol, maxiter, repl2, 0, 1, single);
     JNO_LAUNCH_CHECK();
+    if (single) {
+      rc = launch_kb<float, float>(h_op, a, G, s);
+      if (rc) return rc;
+    }
     rc = read_state(w.st, &e, s);
     if (rc) return rc;
     ++refinements;

@@ -6,6 +6,7 @@ Builds the configuration's system (128^3 P1 Poisson for c2, 64^3 for c3, 32^3 P2
 c5), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
 finally copies a buffer of --copy-mib MiB with a device-to-device copy kernel: the copy's
 FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels' counters.
+--case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver.
 """
 from __future__ import annotations
 
@@ -34,6 +35,20 @@ def build(case: str, dev):
     raise ValueError(f"unknown case {case!r}")
 
 
+def build_batch(dev, count=256, seed=4):
+    """``count`` P1 Poisson systems on n^3 Kuhn boxes, n uniform in [24, 64] from ``seed``, each
+    with its own lognormal k (default_rng(1000 + i))."""
+    import jno_amd as J
+
+    ns = np.random.default_rng(seed).integers(24, 65, size=count)
+    systems = []
+    for i, n in enumerate(ns):
+        m = J.kuhn_box(int(n), device=dev)
+        k = np.exp(0.5 * np.random.default_rng(1000 + i).standard_normal(m.n_cells))
+        systems.append(J.fem_poisson(m).evaluate({"k": torch.tensor(k, device=dev)}))
+    return systems
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", default="c2")
@@ -46,6 +61,20 @@ def main():
     from jno_amd.solve import LinearSolver
 
     dev = torch.device("cuda:0")
+    if args.case == "c4":
+        from jno_amd.batch import BatchSolver
+
+        systems = build_batch(dev)
+        bs = BatchSolver(tol=1e-10, steps_per_graph=8)
+        bs(systems)  # warm (graphs, code objects)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bs(systems)
+        torch.cuda.synchronize()
+        its = [st["iterations"] for st in bs.last_stats]
+        print(f"c4 batch: {len(systems)} systems, {sum(A.shape[0] for A, _ in systems)} rows, "
+              f"{(time.perf_counter() - t0) * 1e3:.1f} ms, iterations {min(its)}..{max(its)}")
+        return
     A, b = build(args.case, dev)
     # replace_factor tiny: no residual replacement inside the bounded run
     sv = LinearSolver("cg", 1e-30, 0.0, args.iters, precision=args.precision, format=args.format,

@@ -130,14 +130,16 @@ def _prec_index(precision: str) -> int:
     return 1 if precision == "mixed" else 0
 
 
-def set_tuning(precision: str, block: int, grid_cap: int, k1_vector: int):
+def set_tuning(precision: str, block: int, grid_cap: int, k1_vector: int, split: int = 0):
     """Launch shape of the PCG iteration kernels for ``precision`` ('fp64' iterations or the FP32
-    inner iterations of 'mixed'): threads per workgroup, maximum workgroups, DIA tile variant."""
-    _lib.call("jno_pcg_set_tuning", _prec_index(precision), int(grid_cap), int(k1_vector), int(block))
+    inner iterations of 'mixed'): threads per workgroup, maximum workgroups, DIA tile variant, and
+    two (``split=0``) or three (``split=1``: p materialized before a plain SpMV) kernels per
+    iteration."""
+    _lib.call("jno_pcg_set_tuning", _prec_index(precision), int(grid_cap), int(k1_vector), int(block), int(split))
 
 
 def get_tuning(precision: str):
-    """Current ``(block, grid_cap, k1_vector)`` for ``precision``."""
-    cap, vec, blk = C.c_int32(), C.c_int32(), C.c_int32()
-    _lib.call("jno_pcg_get_tuning", _prec_index(precision), C.byref(cap), C.byref(vec), C.byref(blk))
-    return int(blk.value), int(cap.value), int(vec.value)
+    """Current ``(block, grid_cap, k1_vector, split)`` for ``precision``."""
+    cap, vec, blk, spl = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    _lib.call("jno_pcg_get_tuning", _prec_index(precision), C.byref(cap), C.byref(vec), C.byref(blk), C.byref(spl))
+    return int(blk.value), int(cap.value), int(vec.value), int(spl.value)
