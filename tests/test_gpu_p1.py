@@ -259,6 +259,36 @@ def test_operator_layouts_agree(dev):
         assert _rel(_np(op.mv(x.float())).astype(np.float64), ref) < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_iteration_modes_and_launch_shapes(dev, precision):
+    """Every launch shape of the PCG iteration kernels solves to the oracle's direct solution:
+    iteration mode K1 K2 / K0 KS K2 / KA KB (single reduction), SpMV variant (2 rows per thread,
+    16-byte tiles, 4 rows per thread), several workgroup sizes and grid caps, DIA and CSR."""
+    from jno_amd import precond, solve
+
+    n = 11  # non-dyadic, ragged row ranges per workgroup
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 3)
+    m, s, A, b = _system(n, k)
+    Ao, bo, _ = O.poisson_p1_system(p, t, k)
+    uo = O.direct_solve(Ao, bo)
+    saved = solve.get_tuning(precision)
+    try:
+        for fmt in ("dia", "csr"):
+            for mode in (0, 1, 2):
+                for vec in (0, 1, 2):
+                    for blk, cap in ((256, 8), (1024, 64), (512, 4096)):
+                        solve.set_tuning(precision, blk, cap, vec, mode)
+                        sv = solve.cg(tol=1e-10, precision=precision, format=fmt)
+                        u = sv(A, b, M=precond.jacobi())
+                        st = sv.last_stats
+                        # jax semantics stop FP64 on the recursive residual; mixed stops on the true one
+                        assert st["relres"] <= (1e-10 if precision == "mixed" else 2e-10), (fmt, mode, vec, blk, cap, st)
+                        assert _rel(_np(u), uo) <= SOL_TOL, (fmt, mode, vec, blk, cap, st)
+    finally:
+        solve.set_tuning(precision, *saved)
+
+
 def test_adjoint_gradient_vs_oracle_and_fd(dev):
     import jno_amd as J
     from jno_amd import adjoint, precond, solve
