@@ -125,9 +125,15 @@ int jno_drop_zeros_fill(int64_t n_rows, const int32_t* d_rowptr, const int32_t* 
  *   JNO_FMT_DIA  : symmetric diagonal storage (main + upper diagonals; lower read by symmetry),
  *                  chosen when the operator has at most 2*JNO_DIA_MAX-1 distinct diagonals and
  *                  is structurally symmetric.  Index-free, fully coalesced.
- * jno_op_analyze inspects a CSR and reports whether DIA applies; jno_op_fill_dia lays it out.
+ *   JNO_FMT_BSR3 : 3x3-block CSR for node-interleaved vector fields (vec = 3, e.g. P2
+ *                  elasticity): rowptr/col index block rows and block columns (nodes); the 9
+ *                  entries of block row R are stored entry-major, entry e = 3i+j of its block b at
+ *                  9*rowptr[R] + e*len_R + b, so a lane per block reads each entry coalesced;
+ *                  `lanes` = lanes per block row, nnz = 9 * blocks.  Stored zeros are kept.
+ * jno_op_analyze inspects a CSR and reports whether DIA applies; jno_op_fill_dia lays it out;
+ * jno_op_bsr3_count / jno_op_fill_bsr3 do the same for BSR3.
  * ------------------------------------------------------------------------------------------- */
-enum { JNO_FMT_CSR = 0, JNO_FMT_DIA = 1 };
+enum { JNO_FMT_CSR = 0, JNO_FMT_DIA = 1, JNO_FMT_BSR3 = 2 };
 #define JNO_DIA_MAX 16
 
 typedef struct jno_op {
@@ -155,6 +161,16 @@ int jno_op_analyze(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col
 int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
                     const double* d_vals, int ndiag, const int32_t* h_offsets, double* d_dia64,
                     float* d_dia32, void* stream);
+/* BSR3 applicability of a CSR: *h_nnzb = number of 3x3 blocks when every block row's three rows
+ * hold the columns 3c+0, 3c+1, 3c+2 of the same ascending block columns c (the vec = 3 pattern of
+ * jno_pattern_count/fill), else -1.  jno_op_fill_bsr3 writes the block rowptr (n_rows/3 + 1), the
+ * block columns (nnzb) and the values (9*nnzb; FP64 and FP32 copies, each nullable). */
+size_t jno_op_bsr3_workspace_size(void);
+int jno_op_bsr3_count(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, int64_t* h_nnzb,
+                      void* d_ws, size_t ws_bytes, void* stream);
+int jno_op_fill_bsr3(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                     int32_t* d_browptr, int32_t* d_bcol, double* d_bvals64, float* d_bvals32,
+                     void* stream);
 /* FP32 copy of CSR values. */
 int jno_cast_f32(int64_t n, const double* d_in, float* d_out, void* stream);
 /* y = A x with a solver operator, FP64 values. */

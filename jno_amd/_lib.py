@@ -23,6 +23,7 @@ JNO_ECAPACITY = -6
 
 JNO_FMT_CSR = 0
 JNO_FMT_DIA = 1
+JNO_FMT_BSR3 = 2
 JNO_DIA_MAX = 16
 
 
@@ -88,6 +89,9 @@ SIGNATURES = {
     "jno_op_analyze_workspace_size": (SZ, []),
     "jno_op_analyze": (C.c_int, [I64, P, P, P, C.POINTER(I32), C.POINTER(I32), P, SZ, P]),
     "jno_op_fill_dia": (C.c_int, [I64, P, P, P, C.c_int, C.POINTER(I32), P, P, P]),
+    "jno_op_bsr3_workspace_size": (SZ, []),
+    "jno_op_bsr3_count": (C.c_int, [I64, P, P, C.POINTER(I64), P, SZ, P]),
+    "jno_op_fill_bsr3": (C.c_int, [I64, P, P, P, P, P, P, P, P]),
     "jno_cast_f32": (C.c_int, [I64, P, P, P]),
     "jno_op_spmv_f64": (C.c_int, [C.POINTER(JnoOp), P, P, P]),
     "jno_op_spmv_f32": (C.c_int, [C.POINTER(JnoOp), P, P, P]),

@@ -1,5 +1,5 @@
-"""Autotuner for the solver-operator layout: DIA when it applies, else CSR with the number of
-wavefront lanes per row that times fastest.  Decisions are cached per pattern signature
+"""Autotuner for the solver-operator layout: DIA when it applies, else BSR3 or CSR with the number
+of wavefront lanes per (block) row that times fastest.  Decisions are cached per pattern signature
 (rows, nnz, and the diagonal set), so a re-assembled operator on the same mesh reuses them."""
 from __future__ import annotations
 
@@ -7,7 +7,7 @@ import torch
 
 _CACHE: dict = {}
 
-CANDIDATE_LANES = (1, 2, 4, 8, 16)
+CANDIDATE_LANES = (1, 2, 4, 8, 16, 32)
 
 
 def _time_spmv(op, reps=20) -> float:
@@ -24,29 +24,39 @@ def _time_spmv(op, reps=20) -> float:
     return t0.elapsed_time(t1) / reps
 
 
-def choose_format(A):
-    """(format, lanes) for the operator ``A`` (a :class:`jno_amd.fem.CSR`)."""
+def choose_format(A, allow=("dia", "bsr3", "csr")):
+    """(format, lanes) for the operator ``A`` (a :class:`jno_amd.fem.CSR`): DIA when it applies
+    with a reasonable fill, else the fastest of BSR3 (when it applies) and CSR over the candidate
+    lanes per (block) row, timed on the device."""
     from .operator import SolverOperator
 
     nd, offs = SolverOperator.analyze(A)
-    key = (A.shape[0], A.nnz, nd, tuple(offs))
+    key = (A.shape[0], A.nnz, nd, tuple(offs), tuple(allow))
     if key in _CACHE:
         return _CACHE[key]
     n = A.shape[0]
     choice = None
-    if nd > 0:
+    if nd > 0 and "dia" in allow:
         # DIA streams 2*ndiag-1 diagonals of n values; accept it unless the fill ratio is poor
         stored = n * (2 * nd - 1)
         if A.nnz >= 0.35 * stored or stored <= 2 * A.nnz:
             choice = ("dia", 1)
     if choice is None:
         best = None
-        for L in CANDIDATE_LANES:
-            op = SolverOperator(A, fmt="csr", lanes=L)
-            t = _time_spmv(op)
-            if best is None or t < best[0]:
-                best = (t, L)
-        choice = ("csr", best[1])
+        fmts = []
+        if "bsr3" in allow and SolverOperator.bsr3_blocks(A) >= 0:
+            fmts.append("bsr3")
+        if "csr" in allow:
+            fmts.append("csr")
+        for fmt in fmts:
+            op = SolverOperator(A, fmt=fmt, lanes=1)  # one layout; lanes only change the launch
+            for L in CANDIDATE_LANES:
+                op.op.lanes = op.lanes = L
+                t = _time_spmv(op)
+                if best is None or t < best[0]:
+                    best = (t, fmt, L)
+            del op
+        choice = (best[1], best[2])
     _CACHE[key] = choice
     return choice
 

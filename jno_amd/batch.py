@@ -32,7 +32,12 @@ class BatchSolver:
         key = ("solver_op", self.format, True)
         op = A._cache.get(key)
         if op is None:
-            op = SolverOperator(A, fmt=self.format, fp32=True)
+            fmt, lanes = self.format, None
+            if fmt == "auto":  # the batch kernels stream DIA and CSR items
+                from .autotune import choose_format
+
+                fmt, lanes = choose_format(A, allow=("dia", "csr"))
+            op = SolverOperator(A, fmt=fmt, lanes=lanes, fp32=True)
             A._cache[key] = op
         return op
 

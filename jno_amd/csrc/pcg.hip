@@ -677,6 +677,39 @@ __global__ void __launch_bounds__(NT) pcg_ks(PcgArgs<T, V> a) {
   store_partial<NT>(part, a.part.A, sm);
 }
 
+// KS (or KB with CG set) on a BSR3 operator: the block rows [r0/3, r1/3) of the workgroup (rows
+// per workgroup are a multiple of 12), L lanes per block row
+template <int NT, int L, bool CG, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
+  __shared__ double sm[NT / 64];
+  PcgState* st = a.st;
+  if (st->done1) {
+    if (CG && blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+    return;
+  }
+  if (CG && blockIdx.x == 0 && threadIdx.x == 0) {
+    st->k = st->k_next;
+    st->gamma_prev = st->gamma_next;
+    st->alpha = st->alpha_next;
+  }
+  const int64_t r0 = block_first_row(a.rows_per_block);
+  const int64_t r1 = min(a.n, r0 + a.rows_per_block);
+  const T* p = a.p_new;
+  const int sub = threadIdx.x % L;
+  double part = 0.0;
+  for (int64_t R = r0 / 3 + threadIdx.x / L; R < r1 / 3; R += NT / L) {
+    T s0, s1, s2;
+    bsr3_row<L, V, T>(a.rowptr, a.col, a.vals, p, R, sub, s0, s1, s2);
+    if (sub == 0) {
+      a.q[3 * R] = s0;
+      a.q[3 * R + 1] = s1;
+      a.q[3 * R + 2] = s2;
+      part += (double)p[3 * R] * (double)s0 + (double)p[3 * R + 1] * (double)s1 + (double)p[3 * R + 2] * (double)s2;
+    }
+  }
+  store_partial<NT>(part, a.part.A, sm);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Single-reduction iteration (Chronopoulos & Gear, J. Comput. Appl. Math. 25 (1989) 153-168):
 // with u = M r, w = A u, gamma = r.u and delta = u.w reduced together,
@@ -903,15 +936,20 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
 }
 
 // grid of the iteration kernels: a multiple of 8 (XCD slabs), at most PCG_MAX_BLOCKS, and the
-// rows per workgroup
-static void pcg_grid(int64_t n, int lanes, const Tuning& tu, int* G, int64_t* rpb) {
-  int64_t want = cdiv(n * lanes, (int64_t)tu.block * 2);
+// rows per workgroup: a multiple of 4 so every workgroup's first row is 16-byte aligned for FP32
+// and FP64, and of 12 for BSR3 (whole 3-row blocks)
+static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
+  const int64_t n = op->n;
+  const int64_t lane_rows = op->format == JNO_FMT_DIA    ? n
+                            : op->format == JNO_FMT_BSR3 ? (n / 3) * op->lanes
+                                                         : n * op->lanes;
+  int64_t want = cdiv(lane_rows, (int64_t)tu.block * 2);
   if (want < 1) want = 1;
   if (want > tu.grid_cap) want = tu.grid_cap;
   if (want >= N_XCD) want = (want / N_XCD) * N_XCD;
   *G = (int)want;
-  // a multiple of 4 rows so every workgroup's first row is 16-byte aligned for FP32 and FP64
-  *rpb = cdiv(cdiv(n, want), 4) * 4;
+  const int64_t mult = op->format == JNO_FMT_BSR3 ? 12 : 4;
+  *rpb = cdiv(cdiv(n, want), mult) * mult;
 }
 
 struct HostPoll {
@@ -938,6 +976,22 @@ enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2 };
 // lattice) and CSR lanes per row
 template <int NT, int MODE, typename T, typename V>
 static int launch_spmv_kernel(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  if (op->format == JNO_FMT_BSR3) {
+    JNO_REQUIRE(MODE != MODE_FUSED, JNO_EINVAL, "pcg: BSR3 operators run the split or single-reduction iteration");
+    constexpr bool CG = MODE == MODE_SINGLE;
+    switch (op->lanes) {
+      case 1: pcg_ks_bsr3<NT, 1, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 2: pcg_ks_bsr3<NT, 2, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 4: pcg_ks_bsr3<NT, 4, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 8: pcg_ks_bsr3<NT, 8, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 16: pcg_ks_bsr3<NT, 16, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 32: pcg_ks_bsr3<NT, 32, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 64: pcg_ks_bsr3<NT, 64, CG, T, V><<<grid, NT, 0, s>>>(a); break;
+      default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: lanes per block row must be a power of two in 1..64");
+    }
+    JNO_LAUNCH_CHECK();
+    return JNO_OK;
+  }
 #define JNO_K(L_, DIA_, ND_)                                                          \
   do {                                                                                \
     if (MODE == MODE_FUSED)                                                           \
@@ -1128,10 +1182,12 @@ static int finish_status(const PcgState& f, jno_pcg_stats* stats, int64_t iters)
 
 static int check_op(const jno_op* op, int need32) {
   JNO_REQUIRE(op, JNO_EINVAL, "pcg: null operator");
-  JNO_REQUIRE(op->format == JNO_FMT_CSR || op->format == JNO_FMT_DIA, JNO_EINVAL, "pcg: unknown operator format");
+  JNO_REQUIRE(op->format == JNO_FMT_CSR || op->format == JNO_FMT_DIA || op->format == JNO_FMT_BSR3, JNO_EINVAL,
+              "pcg: unknown operator format");
   JNO_REQUIRE(op->vals64, JNO_EINVAL, "pcg: operator has no FP64 values");
   JNO_REQUIRE(!need32 || op->vals32, JNO_EINVAL, "pcg: mixed precision needs the operator's FP32 values");
-  if (op->format == JNO_FMT_CSR) JNO_REQUIRE(op->rowptr && op->col, JNO_EINVAL, "pcg: CSR operator without indices");
+  if (op->format != JNO_FMT_DIA) JNO_REQUIRE(op->rowptr && op->col, JNO_EINVAL, "pcg: CSR operator without indices");
+  if (op->format == JNO_FMT_BSR3) JNO_REQUIRE(op->n % 3 == 0, JNO_EINVAL, "pcg: a BSR3 operator has 3k rows");
   if (op->format == JNO_FMT_DIA)
     JNO_REQUIRE(op->ndiag >= 1 && op->ndiag <= JNO_DIA_MAX && op->offsets[0] == 0, JNO_EINVAL,
                 "pcg: bad DIA operator");
@@ -1150,7 +1206,8 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.ndiag = op->ndiag;
   a.vec = tu.vec;
   a.block = tu.block;
-  a.split = tu.split;
+  // the fused K1 recomputes p for gathered neighbours; BSR3 only has the plain SpMV kernel
+  a.split = (op->format == JNO_FMT_BSR3 && tu.split == MODE_FUSED) ? MODE_SPLIT : tu.split;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
   a.dinv = dinv;
   a.r = r;
@@ -1234,7 +1291,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   int G;
   int64_t rpb;
   const Tuning tu = g_tune[0];
-  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, tu, &G, &rpb);
+  pcg_grid(h_op, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   double* r = (double*)w.r;
   const bool single = tu.split == MODE_SINGLE;
@@ -1300,7 +1357,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   int G;
   int64_t rpb;
   const Tuning tu = g_tune[1];
-  pcg_grid(n, h_op->format == JNO_FMT_DIA ? 1 : h_op->lanes, tu, &G, &rpb);
+  pcg_grid(h_op, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   float* rs = (float*)w.r;
   const bool single = tu.split == MODE_SINGLE;

@@ -25,6 +25,63 @@ __global__ void __launch_bounds__(256) csr_spmv_k(int64_t n, const int32_t* __re
   }
 }
 
+template <int L, typename V, typename X>
+__global__ void __launch_bounds__(256) bsr3_spmv_k(int64_t nb, const int32_t* __restrict__ browptr,
+                                                   const int32_t* __restrict__ bcol, const V* __restrict__ bvals,
+                                                   const X* __restrict__ x, X* __restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x % L;
+  for (int64_t R = gid / L; R < nb; R += (int64_t)gridDim.x * blockDim.x / L) {
+    X s0, s1, s2;
+    bsr3_row<L, V, X>(browptr, bcol, bvals, x, R, sub, s0, s1, s2);
+    if (sub == 0) {
+      y[3 * R] = s0;
+      y[3 * R + 1] = s1;
+      y[3 * R + 2] = s2;
+    }
+  }
+}
+
+// BSR3 applicability: every block row's three rows have the same length 3*len, and row 3R+i holds
+// columns 3c_b + j (j = 0, 1, 2) for its blocks b in one ascending order
+__global__ void bsr3_check_k(int64_t nb, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                             int* __restrict__ bad) {
+  for (int64_t R = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; R < nb; R += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = 3 * R;
+    const int b0 = rowptr[r], len3 = rowptr[r + 1] - b0;
+    bool ok = len3 % 3 == 0 && b0 % 9 == 0;
+    for (int i = 1; i < 3 && ok; ++i) ok = rowptr[r + i + 1] - rowptr[r + i] == len3;
+    for (int i = 0; i < 3 && ok; ++i) {
+      const int ri = rowptr[r + i];
+      for (int m = 0; m < len3 && ok; ++m) {
+        const int cb = col[b0 + (m / 3) * 3];
+        ok = cb % 3 == 0 && col[ri + m] == cb + m % 3;
+      }
+    }
+    if (!ok) atomicOr(bad, 1);
+  }
+}
+
+__global__ void bsr3_fill_k(int64_t nb, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ vals, int32_t* __restrict__ browptr,
+                            int32_t* __restrict__ bcol, double* __restrict__ v64, float* __restrict__ v32) {
+  for (int64_t R = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; R < nb; R += (int64_t)gridDim.x * blockDim.x) {
+    const int b0 = rowptr[3 * R], len = (rowptr[3 * R + 1] - b0) / 3, bs = b0 / 9;
+    browptr[R] = bs;
+    if (R == nb - 1) browptr[nb] = rowptr[3 * nb] / 9;
+    for (int b = 0; b < len; ++b) {
+      bcol[bs + b] = col[b0 + 3 * b] / 3;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const double v = vals[rowptr[3 * R + i] + 3 * b + j];
+          const int64_t at = 9 * (int64_t)bs + (int64_t)(3 * i + j) * len + b;
+          if (v64) v64[at] = v;
+          if (v32) v32[at] = (float)v;
+        }
+    }
+  }
+}
+
 template <typename V, typename X>
 __global__ void __launch_bounds__(256) dia_spmv_k(int64_t n, int ndiag, DiaOffsets off, const V* __restrict__ dia,
                                                   const X* __restrict__ x, X* __restrict__ y) {
@@ -49,6 +106,21 @@ int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s) {
     DiaOffsets off;
     for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = op->offsets[d];
     dia_spmv_k<V, X><<<grid_for(n, 256, 16384), 256, 0, s>>>(n, op->ndiag, off, vals, x, y);
+  } else if (op->format == JNO_FMT_BSR3) {
+    JNO_REQUIRE(n % 3 == 0, JNO_EINVAL, "spmv: a BSR3 operator has a multiple of 3 rows");
+    const int64_t nb = n / 3;
+    const int L = op->lanes;
+    const int g = grid_for(nb * L, 256, 16384);
+    switch (L) {
+      case 1: bsr3_spmv_k<1, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      case 2: bsr3_spmv_k<2, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      case 4: bsr3_spmv_k<4, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      case 8: bsr3_spmv_k<8, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      case 16: bsr3_spmv_k<16, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      case 32: bsr3_spmv_k<32, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      case 64: bsr3_spmv_k<64, V, X><<<g, 256, 0, s>>>(nb, op->rowptr, op->col, vals, x, y); break;
+      default: JNO_REQUIRE(false, JNO_EINVAL, "spmv: lanes per block row must be a power of two in 1..64");
+    }
   } else {
     const int L = op->lanes;
     const int g = grid_for(n * L, 256, 16384);
@@ -396,6 +468,41 @@ extern "C" int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const in
   if (n_rows == 0) return JNO_OK;
   dia_fill_k<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, d_rowptr, d_col, d_vals, ndiag, off,
                                                                     d_dia64, d_dia32);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" size_t jno_op_bsr3_workspace_size(void) { return 256; }
+
+extern "C" int jno_op_bsr3_count(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, int64_t* h_nnzb,
+                                 void* d_ws, size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(h_nnzb, JNO_EINVAL, "jno_op_bsr3_count: null h_nnzb");
+  *h_nnzb = -1;
+  if (n_rows <= 0 || n_rows % 3 != 0) return JNO_OK;
+  JNO_REQUIRE(d_rowptr && d_col && d_ws, JNO_EINVAL, "jno_op_bsr3_count: null pointer");
+  JNO_REQUIRE(ws_bytes >= jno_op_bsr3_workspace_size(), JNO_EWORKSPACE, "jno_op_bsr3_count: workspace too small");
+  hipStream_t s = as_stream(stream);
+  int* bad = static_cast<int*>(d_ws);
+  JNO_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  const int64_t nb = n_rows / 3;
+  bsr3_check_k<<<grid_for(nb, 256), 256, 0, s>>>(nb, d_rowptr, d_col, bad);
+  JNO_LAUNCH_CHECK();
+  int h[2] = {0, 0};
+  JNO_HIP(hipMemcpyAsync(&h[0], bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipMemcpyAsync(&h[1], d_rowptr + n_rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  if (!h[0] && h[1] % 9 == 0) *h_nnzb = h[1] / 9;
+  return JNO_OK;
+}
+
+extern "C" int jno_op_fill_bsr3(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                                int32_t* d_browptr, int32_t* d_bcol, double* d_bvals64, float* d_bvals32,
+                                void* stream) {
+  JNO_REQUIRE(n_rows > 0 && n_rows % 3 == 0, JNO_EINVAL, "jno_op_fill_bsr3: rows must be a positive multiple of 3");
+  JNO_REQUIRE(d_rowptr && d_col && d_vals && d_browptr && d_bcol, JNO_EINVAL, "jno_op_fill_bsr3: null pointer");
+  const int64_t nb = n_rows / 3;
+  bsr3_fill_k<<<grid_for(nb, 256), 256, 0, as_stream(stream)>>>(nb, d_rowptr, d_col, d_vals, d_browptr, d_bcol,
+                                                                 d_bvals64, d_bvals32);
   JNO_LAUNCH_CHECK();
   return JNO_OK;
 }

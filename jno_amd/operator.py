@@ -1,10 +1,14 @@
 """Solver operators: the device layout the PCG kernels stream, derived from an assembled CSR.
 
-Two layouts, picked per operator (``format="auto"`` lets :mod:`jno_amd.autotune` time them):
+Three layouts, picked per operator (``format="auto"`` lets :mod:`jno_amd.autotune` time them):
   * ``dia``  symmetric diagonal storage -- the main and upper diagonals of the nonzero pattern,
              the lower ones read through symmetry.  Index-free and fully coalesced; it applies when
              the nonzero entries lie on at most 31 distinct diagonals and the matrix is bitwise
              symmetric (structured meshes, e.g. the Kuhn lattice).
+  * ``bsr3`` 3x3-block CSR for node-interleaved vector fields (vec = 3: P2 elasticity): one
+             column index per 9 values, each block row's entries stored entry-major so a lane per
+             block reads them coalesced; ``lanes`` lanes per block row.  The stored pattern is the
+             structural one (zeros inside a block are kept).
   * ``csr``  CSR with ``lanes`` lanes of a wavefront per row (1..64), exact zeros dropped first
              (``compress_eager``, jno/utils/solver/fem_utils.py:4063-4091).
 Both carry FP64 values (plain PCG, FP64 residuals) and FP32 values (mixed-precision inner loop).
@@ -28,8 +32,8 @@ class SolverOperator:
         self._keep = []  # tensors referenced by the C struct
         self.op = _lib.JnoOp()
         fmt = fmt.lower()
-        if fmt not in ("auto", "dia", "csr"):
-            raise ValueError("format must be 'auto', 'dia' or 'csr'")
+        if fmt not in ("auto", "dia", "csr", "bsr3"):
+            raise ValueError("format must be 'auto', 'dia', 'bsr3' or 'csr'")
         if fmt == "auto":
             from .autotune import choose_format
 
@@ -40,6 +44,11 @@ class SolverOperator:
             if nd < 0:
                 raise ValueError("this operator has no symmetric DIA layout (too many diagonals or not symmetric)")
             self._build_dia(nd, offs, fp32)
+        elif fmt == "bsr3":
+            nb = self.bsr3_blocks(A)
+            if nb < 0:
+                raise ValueError("this operator has no BSR3 layout (not a node pattern times full 3x3 blocks)")
+            self._build_bsr3(nb, lanes or 16, fp32)
         else:
             self._build_csr(A.drop_zeros(), lanes or 4, fp32)
         self.format = fmt
@@ -55,6 +64,41 @@ class SolverOperator:
         _lib.call("jno_op_analyze", A.shape[0], _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), C.byref(nd),
                   offs, _lib.ptr(ws), wsb, _lib.stream_ptr(A.device))
         return int(nd.value), [int(offs[i]) for i in range(max(nd.value, 0))]
+
+    @staticmethod
+    def bsr3_blocks(A: CSR) -> int:
+        """Number of 3x3 blocks of the BSR3 layout of ``A``, -1 when it does not apply."""
+        lib = _lib.load()
+        wsb = lib.jno_op_bsr3_workspace_size()
+        ws = torch.empty(wsb, dtype=torch.uint8, device=A.device)
+        nb = C.c_int64(-1)
+        _lib.call("jno_op_bsr3_count", A.shape[0], _lib.ptr(A.rowptr), _lib.ptr(A.col), C.byref(nb), _lib.ptr(ws), wsb,
+                  _lib.stream_ptr(A.device))
+        return int(nb.value)
+
+    def _build_bsr3(self, nb, lanes, fp32):
+        if lanes not in (1, 2, 4, 8, 16, 32, 64):
+            raise ValueError("lanes per block row must be a power of two in 1..64")
+        n, A = self.n, self.A
+        browptr = torch.empty(n // 3 + 1, dtype=torch.int32, device=self.device)
+        bcol = torch.empty(max(nb, 1), dtype=torch.int32, device=self.device)
+        v64 = torch.empty(max(9 * nb, 1), dtype=torch.float64, device=self.device)
+        v32 = torch.empty(max(9 * nb, 1), dtype=torch.float32, device=self.device) if fp32 else None
+        _lib.call("jno_op_fill_bsr3", n, _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), _lib.ptr(browptr),
+                  _lib.ptr(bcol), _lib.ptr(v64), _lib.ptr(v32), _lib.stream_ptr(self.device))
+        self._keep += [browptr, bcol, v64, v32]
+        op = self.op
+        op.format = _lib.JNO_FMT_BSR3
+        op.lanes = lanes
+        op.n = n
+        op.nnz = 9 * nb
+        op.rowptr = browptr.data_ptr()
+        op.col = bcol.data_ptr()
+        op.vals64 = v64.data_ptr()
+        op.vals32 = v32.data_ptr() if v32 is not None else None
+        op.ndiag = 0
+        self.lanes = lanes
+        self.nnz_stored = 9 * nb
 
     def _build_dia(self, nd, offs, fp32):
         n, A = self.n, self.A
@@ -114,4 +158,4 @@ class SolverOperator:
     def describe(self) -> dict:
         if self.format == "dia":
             return {"format": "dia", "ndiag": self.ndiag, "offsets": self.offsets}
-        return {"format": "csr", "lanes": self.lanes, "nnz": self.nnz_stored}
+        return {"format": self.format, "lanes": self.lanes, "nnz": self.nnz_stored}
