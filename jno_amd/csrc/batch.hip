@@ -3,12 +3,13 @@
 // residual replacement.
 //
 // Every system ("item") runs its own recurrence and its own small state machine on the device.
-// A step is the fixed kernel pair (KA, KB) over one flattened table of (item, row block) entries:
+// A step is the fixed kernel triple (KA, KS, KB) over one flattened table of (item, row block) entries:
 //   KA  reduces the item's partial sums left by the previous KB and decides the item's step:
-//         ITER     p = z + beta p_old, deferred x update, q = A32 p, partial p.q
+//         ITER     p = z + beta p_old, deferred x update
 //         REPLACE  (the FP32 residual dropped by replace_factor^2, met the tolerance or the cap)
 //                  x64 += xs + alpha p_old, xs = 0
 //         DONE     (after a REPLACE whose true FP64 residual met the tolerance, or at the cap)
+//   KS  ITER     q = A32 p, partial p.q
 //   KB  ITER     r -= alpha q (alpha from the reduced p.q), partials r.z, r.r
 //       REPLACE  r64 = b - A64 x64, rs = (float) r64, partials ||r64||^2, r.z, r.r, b.b
 // Each entry reduces its item's partials itself in a fixed order, so no value crosses workgroups
@@ -78,8 +79,44 @@ struct BArgs {
   int64_t maxiter;
 };
 
+// The scalar fields of an item, held in registers for an entry's whole pass (the kernels store to
+// global memory, so fields read through a reference would be reloaded after every store); the
+// diagonal offsets stay in memory (a register copy of the array would be indexed dynamically and
+// spill to scratch)
+struct ItemView {
+  int64_t n, off;
+  int32_t format, ndiag;
+  const int32_t* offs;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const float* v32;
+  const double* v64;
+  const double* b;
+  double* x;
+  int32_t e0, ne;
+};
+
+__device__ __forceinline__ ItemView item_view(const BItem* items, int i) {
+  const BItem& s = items[i];
+  ItemView v;
+  v.n = s.n;
+  v.off = s.off;
+  v.format = s.format;
+  v.ndiag = s.ndiag;
+  v.offs = s.offs;
+  v.rowptr = s.rowptr;
+  v.col = s.col;
+  v.v32 = s.v32;
+  v.v64 = s.v64;
+  v.b = s.b;
+  v.x = s.x;
+  v.e0 = s.e0;
+  v.ne = s.ne;
+  return v;
+}
+
 template <typename T, typename V>
-__device__ __forceinline__ T mat_row(const BItem& it, int64_t i, const V* vals, const T* x, bool dia) {
+__device__ __forceinline__ T mat_row(const ItemView& it, int64_t i, const V* vals, const T* x, bool dia) {
   if (dia) {
     T s = (T)vals[i] * x[i];
     for (int d = 1; d < it.ndiag; ++d) {
@@ -96,7 +133,7 @@ __device__ __forceinline__ T mat_row(const BItem& it, int64_t i, const V* vals, 
 }
 
 // item total of one partial array, fixed order; every thread gets it
-__device__ __forceinline__ double item_sum(const double* __restrict__ P, const BItem& it, double* sm) {
+__device__ __forceinline__ double item_sum(const double* __restrict__ P, const ItemView& it, double* sm) {
   double s = 0.0;
   for (int e = threadIdx.x; e < it.ne; e += BT_NT) s += P[it.e0 + e];
   return block_sum<BT_NT>(s, sm);
@@ -113,7 +150,7 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
   const int64_t NE = a.n_entries;
   for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
     const int2 te = a.table[entry];
-    const BItem& it = a.items[te.x];
+    const ItemView it = item_view(a.items, te.x);
     if ((it.format == JNO_FMT_DIA) != DIA) continue;
     BState* S = a.st + te.x;
     if (S->done) continue;  // entry-uniform
@@ -179,31 +216,93 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
     const float beta = (k == 0) ? 0.0f : (float)(gamma / S->gamma_prev);
     const float* r = a.rs + it.off;
     const float* dv = a.dinv + it.off;
+    // p = z + beta p_old and the deferred x update, one 4-row tile per thread (16-byte accesses)
+    static_assert(BT_RPB == 4 * BT_NT, "one 4-row tile per thread");
+    const int64_t t0 = r0 + 4 * (int64_t)threadIdx.x;
+    if (t0 + 4 <= r1) {
+      const float4 dq = *reinterpret_cast<const float4*>(dv + t0);
+      const float4 rq = *reinterpret_cast<const float4*>(r + t0);
+      const float4 po = *reinterpret_cast<const float4*>(pold + t0);
+      float4 pi;
+      pi.x = dq.x * rq.x + beta * po.x;
+      pi.y = dq.y * rq.y + beta * po.y;
+      pi.z = dq.z * rq.z + beta * po.z;
+      pi.w = dq.w * rq.w + beta * po.w;
+      *reinterpret_cast<float4*>(pnew + t0) = pi;
+      if (k > 0) {
+        float4 xq = *reinterpret_cast<const float4*>(xs + t0);
+        xq.x += ap * po.x;
+        xq.y += ap * po.y;
+        xq.z += ap * po.z;
+        xq.w += ap * po.w;
+        *reinterpret_cast<float4*>(xs + t0) = xq;
+      }
+    } else {
+      for (int64_t i = t0; i < min(r1, t0 + 4); ++i) {
+        const float po = pold[i];
+        pnew[i] = dv[i] * r[i] + beta * po;
+        if (k > 0) xs[i] += ap * po;
+      }
+    }
+  }
+}
+
+// KS: q = A32 p for the items KA set to ITER, partial p.q
+template <bool DIA>
+__global__ void __launch_bounds__(BT_NT) batch_ks(BArgs a) {
+  __shared__ double sm[BT_NT / 64];
+  for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
+    const int2 te = a.table[entry];
+    const ItemView it = item_view(a.items, te.x);
+    if ((it.format == JNO_FMT_DIA) != DIA) continue;
+    const BState* S = a.st + te.x;
+    if (S->phase_b != PH_ITER) continue;  // entry-uniform
+    const int64_t k = S->k;
+    const float* p = a.p[(k + 1) & 1] + it.off;
     float* q = a.q + it.off;
+    const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
     double part = 0.0;
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
-      const float po = pold[i];
-      const float pi = dv[i] * r[i] + beta * po;
-      float s;
-      if (DIA) {
-        s = it.v32[i] * pi;
-        for (int d = 1; d < it.ndiag; ++d) {
-          const int64_t o = it.offs[d];
-          const float* v = it.v32 + (int64_t)d * dia_ld(it.n);
-          if (i + o < it.n) s += v[i] * (dv[i + o] * r[i + o] + beta * pold[i + o]);
-          if (i - o >= 0) s += v[i - o] * (dv[i - o] * r[i - o] + beta * pold[i - o]);
+    int64_t i0 = r0 + threadIdx.x;
+    if (DIA && it.ndiag == 4) {
+      // the 3-D Kuhn lattice: BR rows per thread with all 14 BR loads in flight before the first use
+      constexpr int BR = 2;
+      const int64_t oo[3] = {it.offs[1], it.offs[2], it.offs[3]};
+      const int64_t omax = max(max(oo[0], oo[1]), oo[2]);
+      const int64_t ld = dia_ld(it.n);
+      for (; i0 + (BR - 1) * BT_NT < r1 && i0 - omax >= 0 && i0 + (BR - 1) * BT_NT + omax < it.n; i0 += BR * BT_NT) {
+        float pv[BR], d0[BR], vu[BR][3], vl[BR][3], pu[BR][3], pl[BR][3];
+#pragma unroll
+        for (int t = 0; t < BR; ++t) {
+          const int64_t i = i0 + t * BT_NT;
+          pv[t] = p[i];
+          d0[t] = it.v32[i];
+#pragma unroll
+          for (int d = 1; d < 4; ++d) {
+            const int64_t o = oo[d - 1];
+            const float* v = it.v32 + (int64_t)d * ld;
+            vu[t][d - 1] = v[i];
+            vl[t][d - 1] = v[i - o];
+            pu[t][d - 1] = p[i + o];
+            pl[t][d - 1] = p[i - o];
+          }
         }
-      } else {
-        s = 0.0f;
-        for (int j = it.rowptr[i]; j < it.rowptr[i + 1]; ++j) {
-          const int c = it.col[j];
-          s += it.v32[j] * (dv[c] * r[c] + beta * pold[c]);
+#pragma unroll
+        for (int t = 0; t < BR; ++t) {
+          float sacc = d0[t] * pv[t];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            sacc += vu[t][d] * pu[t][d];
+            sacc += vl[t][d] * pl[t][d];
+          }
+          q[i0 + t * BT_NT] = sacc;
+          part += (double)pv[t] * (double)sacc;
         }
       }
-      pnew[i] = pi;
+    }
+    for (int64_t i = i0; i < r1; i += BT_NT) {
+      const float s = mat_row<float, float>(it, i, it.v32, p, DIA);
       q[i] = s;
-      if (k > 0) xs[i] += ap * po;
-      part += (double)pi * (double)s;
+      part += (double)p[i] * (double)s;
     }
     put(part, a.PA, entry, sm);
   }
@@ -215,7 +314,7 @@ __global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
   const int64_t NE = a.n_entries;
   for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
     const int2 te = a.table[entry];
-    const BItem& it = a.items[te.x];
+    const ItemView it = item_view(a.items, te.x);
     if ((it.format == JNO_FMT_DIA) != DIA) continue;
     BState* S = a.st + te.x;
     const int phase = S->phase_b;
@@ -235,11 +334,33 @@ __global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
       const float al = (float)alpha;
       const float* q = a.q + it.off;
       double pg = 0, pr = 0;
-      for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
-        const float ri = r[i] - al * q[i];
-        r[i] = ri;
-        pg += (double)ri * (double)(dv[i] * ri);
-        pr += (double)ri * (double)ri;
+      // an entry is at most BT_RPB = 4 * BT_NT rows: one 4-row tile per thread, 16-byte accesses
+      static_assert(BT_RPB == 4 * BT_NT, "one 4-row tile per thread");
+      const int64_t t0 = r0 + 4 * (int64_t)threadIdx.x;
+      if (t0 + 4 <= r1) {
+        float4 rv = *reinterpret_cast<const float4*>(r + t0);
+        const float4 qv = *reinterpret_cast<const float4*>(q + t0);
+        const float4 dq = *reinterpret_cast<const float4*>(dv + t0);
+        rv.x = rv.x - al * qv.x;
+        rv.y = rv.y - al * qv.y;
+        rv.z = rv.z - al * qv.z;
+        rv.w = rv.w - al * qv.w;
+        *reinterpret_cast<float4*>(r + t0) = rv;
+        pg += (double)rv.x * (double)(dq.x * rv.x);
+        pg += (double)rv.y * (double)(dq.y * rv.y);
+        pg += (double)rv.z * (double)(dq.z * rv.z);
+        pg += (double)rv.w * (double)(dq.w * rv.w);
+        pr += (double)rv.x * (double)rv.x;
+        pr += (double)rv.y * (double)rv.y;
+        pr += (double)rv.z * (double)rv.z;
+        pr += (double)rv.w * (double)rv.w;
+      } else {
+        for (int64_t i = t0; i < min(r1, t0 + 4); ++i) {
+          const float ri = r[i] - al * q[i];
+          r[i] = ri;
+          pg += (double)ri * (double)(dv[i] * ri);
+          pr += (double)ri * (double)ri;
+        }
       }
       put(pg, a.PB, entry, sm);
       put(pr, a.PB + NE, entry, sm);
@@ -347,11 +468,14 @@ static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, char* base)
   return L;
 }
 
+// each item's rows start 64-aligned in the concatenated FP32 vectors (16-byte vector tiles)
+static int64_t padded_rows(int64_t n) { return (n + 63) & ~(int64_t)63; }
+
 static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot) {
   int64_t ne = 0, nt = 0;
   for (int i = 0; i < n; ++i) {
     ne += cdiv(items[i].op.n, BT_RPB);
-    nt += items[i].op.n;
+    nt += padded_rows(items[i].op.n);
   }
   if (ntot) *ntot = nt;
   return ne;
@@ -410,7 +534,7 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
     it.ne = (int32_t)cdiv(it.n, BT_RPB);
     for (int j = 0; j < it.ne; ++j) table[e++] = make_int2(i, j);
     dinvs[i] = h.dinv;
-    off += it.n;
+    off += padded_rows(it.n);
     any_dia |= it.format == JNO_FMT_DIA;
     any_csr |= it.format == JNO_FMT_CSR;
   }
@@ -453,6 +577,8 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
   auto launch_step = [&](hipStream_t st) -> int {
     if (any_dia) batch_ka<true><<<grid, BT_NT, 0, st>>>(a);
     if (any_csr) batch_ka<false><<<grid, BT_NT, 0, st>>>(a);
+    if (any_dia) batch_ks<true><<<grid, BT_NT, 0, st>>>(a);
+    if (any_csr) batch_ks<false><<<grid, BT_NT, 0, st>>>(a);
     JNO_LAUNCH_CHECK();
     return launch_kb(st);
   };
