@@ -194,6 +194,10 @@ typedef struct jno_pcg_stats {
 } jno_pcg_stats;
 
 size_t jno_pcg_workspace_size(int64_t n, int mixed);
+/* The workspace for a given operator: with mixed = 1 and a DIA operator it includes room for the
+ * Jacobi-scaled FP32 copy of the operator that the scaled inner iteration (tuning mode 3) streams;
+ * a workspace of jno_pcg_workspace_size(n, 1) bytes runs the unscaled inner iteration instead. */
+size_t jno_pcg_workspace_size_op(const jno_op* h_op, int mixed);
 /* Iterations enqueued per host poll of the device-side convergence state (default 8).  No
  * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
 int jno_pcg_set_chunk(int chunk);
@@ -202,7 +206,9 @@ int jno_pcg_set_chunk(int chunk);
  * the FP32 inner iterations of jno_pcg_mixed; grid_cap = maximum workgroups (multiple of 8,
  * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for rows per thread;
  * block = threads per workgroup (256/512/1024); split = 1 for three kernels per iteration (p
- * materialized, then a plain SpMV), 0 for two (p recomputed inside the SpMV kernel).
+ * materialized, then a plain SpMV), 0 for two (p recomputed inside the SpMV kernel), 2 for the
+ * single-reduction iteration, 3 for the split iteration on the Jacobi-scaled FP32 operator (FP32
+ * inner iterations on DIA operators with room for it in the workspace; elsewhere it is mode 1).
  * jno_pcg_get_tuning reads the current shape. */
 int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split);
 int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split);

@@ -96,6 +96,7 @@ SIGNATURES = {
     "jno_op_spmv_f64": (C.c_int, [C.POINTER(JnoOp), P, P, P]),
     "jno_op_spmv_f32": (C.c_int, [C.POINTER(JnoOp), P, P, P]),
     "jno_pcg_workspace_size": (SZ, [I64, C.c_int]),
+    "jno_pcg_workspace_size_op": (SZ, [C.POINTER(JnoOp), C.c_int]),
     "jno_pcg_set_chunk": (C.c_int, [C.c_int]),
     "jno_pcg_set_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "jno_pcg_get_tuning": (C.c_int, [C.c_int, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),

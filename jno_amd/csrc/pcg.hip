@@ -33,17 +33,18 @@ constexpr int PCG_NT = 256;
 constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup partials
 
 // Launch shape of the iteration kernels, one per iteration precision (set by the autotuner through
-// jno_pcg_set_tuning).  The defaults are the fastest shapes of sweeps over workgroup size x grid
-// cap x SpMV tile variant x iteration mode on a 2.1M-row 4-diagonal operator (128^3 Kuhn P1):
-// 1024 threads x 512 workgroups, p materialized (K0 KS K2); FP32 iterations take two rows per
-// thread in KS, FP64 iterations 16-byte vector tiles.
+// jno_pcg_set_tuning).  The defaults are among the fastest shapes of sweeps over workgroup size x
+// grid cap x SpMV tile variant x iteration mode on a 2.1M-row 4-diagonal operator (128^3 Kuhn P1):
+// 1024 threads x 512 workgroups, p materialized (K0 KS K2), two rows per thread in KS; the FP32
+// inner iterations run on the Jacobi-scaled operator (mode 3).
 struct Tuning {
   int grid_cap;  // workgroups of the iteration kernels (a multiple of 8)
   int vec;       // DIA SpMV kernel: 0 two rows per thread, 1 VN-row 16-byte tiles, 2 four rows
   int block;     // threads per workgroup (256, 512, 1024)
-  int split;     // iteration mode: 0 K1 K2, 1 K0 KS K2 (p materialized), 2 KA KB (single reduction)
+  int split;     // iteration mode: 0 K1 K2, 1 K0 KS K2 (p materialized), 2 KA KB (single reduction),
+                 // 3 K0 KS K2 on the Jacobi-scaled operator (FP32 inner iterations on DIA)
 };
-static Tuning g_tune[2] = {{512, 1, 1024, 1}, {512, 0, 1024, 1}};  // [0] FP64 iterations, [1] FP32 iterations
+static Tuning g_tune[2] = {{512, 0, 1024, 1}, {512, 0, 1024, 3}};  // [0] FP64 iterations, [1] FP32 iterations
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -91,6 +92,7 @@ struct PcgArgs {
   int vec;    // DIA K1 tile variant
   int block;  // threads per workgroup of the launch (host side only)
   int split;  // three kernels per iteration (host side only)
+  int unit;   // Jacobi-scaled DIA operator: unit main diagonal, z = r (host side only)
   DiaOffsets off;
   const T* dinv;
   T* r;
@@ -123,6 +125,48 @@ template <int NT = PCG_NT>
 __device__ __forceinline__ void store_partial(double v, double* __restrict__ dst, double* sm) {
   v = block_sum<NT>(v, sm);
   if (threadIdx.x == 0) dst[blockIdx.x] = v;
+}
+
+// The iteration kernels' reductions, one barrier each (a kernel's fixed cost grows with the
+// barriers its waves pass).
+// Totals of the M consecutive partial arrays part[m*G .. m*G+G), identical in every thread: wave 0
+// sums them (lane l takes entries l, l+64, ... in order, then a fixed butterfly) and broadcasts them
+// through sm_in[M].
+template <int M>
+__device__ __forceinline__ void reduce_in(const double* __restrict__ part, int G, double* sm_in, double* out) {
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double s = 0.0;
+      for (int i = threadIdx.x; i < G; i += 64) s += part[(int64_t)m * G + i];
+      s = wave_sum(s);
+      if (threadIdx.x == 0) sm_in[m] = s;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) out[m] = sm_in[m];
+}
+
+// Workgroup totals of M per-thread values into dst[m*G + blockIdx.x]: each wave reduces, the wave
+// totals go through sm_out[M][NT/64], thread 0 adds them in wave order.
+template <int NT, int M>
+__device__ __forceinline__ void store_out(const double* v, double* __restrict__ dst, int G, double* sm_out) {
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const double s = wave_sum(v[m]);
+    if ((threadIdx.x & 63) == 0) sm_out[m * (NT / 64) + w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double t = 0.0;
+      for (int i = 0; i < NT / 64; ++i) t += sm_out[m * (NT / 64) + i];
+      dst[(int64_t)m * G + blockIdx.x] = t;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -275,8 +319,10 @@ __device__ __forceinline__ bool iter_head(const PcgArgs<T, V>& a, double* sm, do
                                           int64_t& k, int64_t& r0, int64_t& r1) {
   PcgState* st = a.st;
   const int G = gridDim.x;
-  gamma = reduce_partials<NT>(a.part.B, G, sm);
-  const double rr = reduce_partials<NT>(a.part.B + G, G, sm);
+  double in[2];
+  reduce_in<2>(a.part.B, G, sm, in);
+  gamma = in[0];
+  const double rr = in[1];
   k = st->k;
   const double alpha_prev = st->alpha;
   const bool stop_conv = !(rr > st->atol2);
@@ -303,7 +349,8 @@ __device__ __forceinline__ bool iter_head(const PcgArgs<T, V>& a, double* sm, do
 
 template <int NT, int L, bool DIA, int ND, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+  __shared__ double sm[4];                  // broadcast of reduced partial sums
+  __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
   PcgState* st = a.st;
   if (st->done) return;
   double gamma;
@@ -360,7 +407,7 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
-  store_partial<NT>(part, a.part.A, sm);
+  store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -371,9 +418,11 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
 // VN, only at the end of the matrix) go one per thread.
 constexpr int TILE_U = 2;
 
-template <int NT, typename T, typename V>
+// JAC: r.z with z = dinv r; without it z = r and both partials are r.r
+template <int NT, typename T, typename V, bool JAC = true>
 __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+  __shared__ double sm[4];                  // broadcast of reduced partial sums
+  __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
   PcgState* st = a.st;
   if (st->done1) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
@@ -394,10 +443,11 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
     if (i0 < r1v) {
       rv[u] = ld16(a.r + i0);
       qv[u] = ld16(a.q + i0);
-      dv[u] = ld16(a.dinv + i0);
+      if (JAC) dv[u] = ld16(a.dinv + i0);
     }
   }
-  const double pq = reduce_partials<NT>(a.part.A, G, sm);
+  double pq;
+  reduce_in<1>(a.part.A, G, sm, &pq);
   const double gamma = st->gamma_cur;
   const double alpha = gamma / pq;
   const T al = (T)alpha;
@@ -411,8 +461,9 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
         for (int j = 0; j < VN; ++j) {
           const T ri = rv[u].v[j] - al * qv[u].v[j];
           rv[u].v[j] = ri;
-          pg += (double)ri * (double)(dv[u].v[j] * ri);
-          pr += (double)ri * (double)ri;
+          const double rr = (double)ri * (double)ri;
+          pg += JAC ? (double)ri * (double)(dv[u].v[j] * ri) : rr;
+          pr += rr;
         }
         st16(a.r + i0, rv[u]);
       }
@@ -425,7 +476,7 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
       if (i0 < r1v) {
         rv[u] = ld16(a.r + i0);
         qv[u] = ld16(a.q + i0);
-        dv[u] = ld16(a.dinv + i0);
+        if (JAC) dv[u] = ld16(a.dinv + i0);
       }
     }
   }
@@ -434,8 +485,9 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
     if (i < r1) {
       const T ri = a.r[i] - al * a.q[i];
       a.r[i] = ri;
-      pg += (double)ri * (double)(a.dinv[i] * ri);
-      pr += (double)ri * (double)ri;
+      const double rr = (double)ri * (double)ri;
+      pg += JAC ? (double)ri * (double)(a.dinv[i] * ri) : rr;
+      pr += rr;
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -443,8 +495,8 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
     st->gamma_prev = gamma;
     st->k = st->k + 1;
   }
-  store_partial<NT>(pg, a.part.B, sm);
-  store_partial<NT>(pr, a.part.B + G, sm);
+  const double outv[2] = {pg, pr};
+  store_out<NT, 2>(outv, a.part.B, G, sm_out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -452,9 +504,11 @@ __global__ void __launch_bounds__(NT) pcg_k2(PcgArgs<T, V> a) {
 // Fewer bytes move per row in the SpMV (one neighbour vector instead of r, dinv and p_old), at the
 // cost of a third launch and of writing and re-reading p once.
 // ---------------------------------------------------------------------------------------------
-template <int NT, typename T, typename V>
+// JAC: z = dinv r (Jacobi); without it z = r (the Jacobi-scaled operator, identity preconditioner)
+template <int NT, typename T, typename V, bool JAC = true>
 __global__ void __launch_bounds__(NT) pcg_k0(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+  __shared__ double sm[4];                  // broadcast of reduced partial sums
+  __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
   PcgState* st = a.st;
   if (st->done) return;
   constexpr int VN = V16<T>::N;
@@ -470,7 +524,7 @@ __global__ void __launch_bounds__(NT) pcg_k0(PcgArgs<T, V> a) {
   for (int u = 0; u < U; ++u) {
     const int64_t i0 = t + (int64_t)u * VN * NT;
     if (i0 < r1v) {
-      dv[u] = ld16(a.dinv + i0);
+      if (JAC) dv[u] = ld16(a.dinv + i0);
       rv[u] = ld16(a.r + i0);
       po[u] = ld16(a.p_old + i0);
       if (upd_x) xv[u] = ld16(a.x + i0);
@@ -487,7 +541,7 @@ __global__ void __launch_bounds__(NT) pcg_k0(PcgArgs<T, V> a) {
       if (i0 < r1v) {
         V16<T> pi;
 #pragma unroll
-        for (int j = 0; j < VN; ++j) pi.v[j] = dv[u].v[j] * rv[u].v[j] + beta * po[u].v[j];
+        for (int j = 0; j < VN; ++j) pi.v[j] = (JAC ? dv[u].v[j] * rv[u].v[j] : rv[u].v[j]) + beta * po[u].v[j];
         st16(a.p_new + i0, pi);
         if (upd_x) {
 #pragma unroll
@@ -502,7 +556,7 @@ __global__ void __launch_bounds__(NT) pcg_k0(PcgArgs<T, V> a) {
     for (int u = 0; u < U; ++u) {
       const int64_t i0 = t + (int64_t)u * VN * NT;
       if (i0 < r1v) {
-        dv[u] = ld16(a.dinv + i0);
+        if (JAC) dv[u] = ld16(a.dinv + i0);
         rv[u] = ld16(a.r + i0);
         po[u] = ld16(a.p_old + i0);
         if (upd_x) xv[u] = ld16(a.x + i0);
@@ -513,20 +567,21 @@ __global__ void __launch_bounds__(NT) pcg_k0(PcgArgs<T, V> a) {
     const int64_t i = r1v + threadIdx.x;
     if (i < r1) {
       const T p0 = a.p_old[i];
-      a.p_new[i] = a.dinv[i] * a.r[i] + beta * p0;
+      a.p_new[i] = (JAC ? a.dinv[i] * a.r[i] : a.r[i]) + beta * p0;
       if (upd_x) a.x[i] += ap * p0;
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
 }
 
+// UNIT: the main diagonal is 1 and not read (the Jacobi-scaled operator)
 // one DIA row of q = A p with bounds checks
-template <typename T, typename V>
+template <typename T, typename V, bool UNIT = false>
 __device__ __forceinline__ double spmv_dia_row(const PcgArgs<T, V>& a, int64_t i) {
   const int64_t ld = dia_ld(a.n);
   const T* p = a.p_new;
   const T pi = p[i];
-  T s = (T)a.vals[i] * pi;
+  T s = UNIT ? pi : (T)a.vals[i] * pi;
   for (int d = 1; d < a.ndiag; ++d) {
     const int64_t o = a.off.o[d];
     const V* v = a.vals + (int64_t)d * ld;
@@ -538,7 +593,7 @@ __device__ __forceinline__ double spmv_dia_row(const PcgArgs<T, V>& a, int64_t i
 }
 
 // R interior DIA rows i, i + NT, ... of q = A p, all loads in flight before the first use
-template <int ND, int R, int NT, typename T, typename V>
+template <int ND, int R, int NT, typename T, typename V, bool UNIT = false>
 __device__ __forceinline__ double spmv_dia_rows(const PcgArgs<T, V>& a, int64_t i) {
   const int64_t ld = dia_ld(a.n);
   const T* p = a.p_new;
@@ -548,7 +603,7 @@ __device__ __forceinline__ double spmv_dia_rows(const PcgArgs<T, V>& a, int64_t 
   for (int r = 0; r < R; ++r) {
     const int64_t row = i + (int64_t)r * NT;
     pv[r] = p[row];
-    d0[r] = a.vals[row];
+    if (!UNIT) d0[r] = a.vals[row];
 #pragma unroll
     for (int d = 1; d < ND; ++d) {
       const int64_t o = a.off.o[d];
@@ -561,7 +616,7 @@ __device__ __forceinline__ double spmv_dia_rows(const PcgArgs<T, V>& a, int64_t 
   double part = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    T s = (T)d0[r] * pv[r];
+    T s = UNIT ? pv[r] : (T)d0[r] * pv[r];
 #pragma unroll
     for (int d = 0; d < ND - 1; ++d) {
       s += (T)vu[r][d] * pu[r][d];
@@ -574,24 +629,24 @@ __device__ __forceinline__ double spmv_dia_rows(const PcgArgs<T, V>& a, int64_t 
 }
 
 // rows [r0, r1) of q = A p, R rows per thread and step (NT apart, coalesced)
-template <int ND, int R, int NT, typename T, typename V>
+template <int ND, int R, int NT, typename T, typename V, bool UNIT = false>
 __device__ __forceinline__ double spmv_dia_scalar(const PcgArgs<T, V>& a, int64_t r0, int64_t r1, int64_t omax) {
   double part = 0.0;
   int64_t i = r0 + threadIdx.x;
   for (; i + (int64_t)(R - 1) * NT < r1; i += (int64_t)R * NT) {
     if (i - omax >= 0 && i + (int64_t)(R - 1) * NT + omax < a.n) {
-      part += spmv_dia_rows<ND, R, NT, T, V>(a, i);
+      part += spmv_dia_rows<ND, R, NT, T, V, UNIT>(a, i);
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) part += spmv_dia_row<T, V>(a, i + (int64_t)r * NT);
+      for (int r = 0; r < R; ++r) part += spmv_dia_row<T, V, UNIT>(a, i + (int64_t)r * NT);
     }
   }
-  for (; i < r1; i += NT) part += spmv_dia_row<T, V>(a, i);
+  for (; i < r1; i += NT) part += spmv_dia_row<T, V, UNIT>(a, i);
   return part;
 }
 
 // VN-row tile [i0, i0 + VN) of q = A p with 16-byte loads (interior tiles only)
-template <int ND, typename T, typename V>
+template <int ND, typename T, typename V, bool UNIT = false>
 __device__ __forceinline__ double spmv_dia_tile(const PcgArgs<T, V>& a, int64_t i0) {
   constexpr int VN = V16<T>::N;
   const int64_t ld = dia_ld(a.n);
@@ -599,7 +654,9 @@ __device__ __forceinline__ double spmv_dia_tile(const PcgArgs<T, V>& a, int64_t 
   const T* p = a.p_new;
   const V16<T> pv = ld16(p + i0);
   V16<T> s;
-  {
+  if (UNIT) {
+    s = pv;
+  } else {
     const V16<V> d0 = ld16(a.vals + i0);
 #pragma unroll
     for (int j = 0; j < VN; ++j) s.v[j] = (T)d0.v[j] * pv.v[j];
@@ -626,9 +683,10 @@ __device__ __forceinline__ double spmv_dia_tile(const PcgArgs<T, V>& a, int64_t 
 // KS: q = A p (p materialized by K0), partial p.q.  With CG set it is KB of the single-reduction
 // iteration (launched with p_new = u, q = w): w = A u, partial u.w, and workgroup 0 moves the
 // scalars KA staged into the fields KA reads.
-template <int NT, int L, bool DIA, int ND, bool CG, typename T, typename V>
+template <int NT, int L, bool DIA, int ND, bool CG, typename T, typename V, bool UNIT = false>
 __global__ void __launch_bounds__(NT) pcg_ks(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+  __shared__ double sm[4];                  // broadcast of reduced partial sums
+  __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
   PcgState* st = a.st;
   if (st->done1) {
     if (CG && blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
@@ -650,15 +708,15 @@ __global__ void __launch_bounds__(NT) pcg_ks(PcgArgs<T, V> a) {
       constexpr int VN = V16<T>::N;
       for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * NT) {
         if (i0 + VN <= r1 && i0 - omax >= 0 && i0 + VN - 1 + omax < a.n)
-          part += spmv_dia_tile<ND, T, V>(a, i0);
+          part += spmv_dia_tile<ND, T, V, UNIT>(a, i0);
         else
-          for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += spmv_dia_row<T, V>(a, i);
+          for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += spmv_dia_row<T, V, UNIT>(a, i);
       }
     } else if (ND > 0) {
-      part = a.vec == 2 ? spmv_dia_scalar<(ND > 0 ? ND : 1), 4, NT, T, V>(a, r0, r1, omax)
-                        : spmv_dia_scalar<(ND > 0 ? ND : 1), 2, NT, T, V>(a, r0, r1, omax);
+      part = a.vec == 2 ? spmv_dia_scalar<(ND > 0 ? ND : 1), 4, NT, T, V, UNIT>(a, r0, r1, omax)
+                        : spmv_dia_scalar<(ND > 0 ? ND : 1), 2, NT, T, V, UNIT>(a, r0, r1, omax);
     } else {
-      for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += spmv_dia_row<T, V>(a, i);
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += spmv_dia_row<T, V, UNIT>(a, i);
     }
   } else {
     const int sub = threadIdx.x % L;
@@ -674,14 +732,15 @@ __global__ void __launch_bounds__(NT) pcg_ks(PcgArgs<T, V> a) {
       }
     }
   }
-  store_partial<NT>(part, a.part.A, sm);
+  store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);
 }
 
 // KS (or KB with CG set) on a BSR3 operator: the block rows [r0/3, r1/3) of the workgroup (rows
 // per workgroup are a multiple of 12), L lanes per block row
 template <int NT, int L, bool CG, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+  __shared__ double sm[4];                  // broadcast of reduced partial sums
+  __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
   PcgState* st = a.st;
   if (st->done1) {
     if (CG && blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
@@ -707,7 +766,7 @@ __global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
       part += (double)p[3 * R] * (double)s0 + (double)p[3 * R + 1] * (double)s1 + (double)p[3 * R + 2] * (double)s2;
     }
   }
-  store_partial<NT>(part, a.part.A, sm);
+  store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -723,7 +782,8 @@ __global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
 // ---------------------------------------------------------------------------------------------
 template <int NT, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_ka(PcgArgs<T, V> a) {
-  __shared__ double sm[NT / 64];
+  __shared__ double sm[4];                  // broadcast of reduced partial sums
+  __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
   PcgState* st = a.st;
   if (st->done) return;
   const int G = gridDim.x;
@@ -732,9 +792,10 @@ __global__ void __launch_bounds__(NT) pcg_ka(PcgArgs<T, V> a) {
   // other, chosen by the parity of k (the same for every workgroup)
   const double* b_in = (k & 1) ? a.part.B2 : a.part.B;
   double* b_out = (k & 1) ? a.part.B : a.part.B2;
-  const double gamma = reduce_partials<NT>(b_in, G, sm);
-  const double rr = reduce_partials<NT>(b_in + G, G, sm);
-  const double delta = reduce_partials<NT>(a.part.A, G, sm);
+  double in2[2], delta;
+  reduce_in<2>(b_in, G, sm, in2);
+  reduce_in<1>(a.part.A, G, sm + 2, &delta);
+  const double gamma = in2[0], rr = in2[1];
   const bool stop_conv = !(rr > st->atol2);
   const bool stop_cap = k >= st->maxiter;
   const bool stop_repl = rr <= st->rr_replace;
@@ -807,8 +868,8 @@ __global__ void __launch_bounds__(NT) pcg_ka(PcgArgs<T, V> a) {
     st->gamma_next = gamma;
     st->alpha_next = alpha;
   }
-  store_partial<NT>(pg, b_out, sm);
-  store_partial<NT>(pr, b_out + G, sm);
+  const double outv[2] = {pg, pr};
+  store_out<NT, 2>(outv, b_out, G, sm_out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -892,6 +953,67 @@ __global__ void cast_f32_vec_k(int64_t n, const double* __restrict__ a, float* _
 }
 
 // ---------------------------------------------------------------------------------------------
+// Jacobi-scaled FP32 inner operator: with S = diag(sqrt(dinv)), Jacobi-PCG on A x = b is plain CG
+// on (S A S) y = S b with x = S y, iterate for iterate.  S A S has a unit main diagonal, so the
+// inner iterations stream neither dinv nor the main diagonal (52 instead of 64 bytes per row and
+// iteration in FP32).
+// ---------------------------------------------------------------------------------------------
+// sdia[d][i] = (float)(A[i, i+o_d] * s_i * s_{i+o_d}) for the stored upper diagonals d >= 1; *bad is
+// set when some (float)(A_ii dinv_i) != 1 (then the unscaled inner iteration is used)
+__global__ void scale_dia_k(int64_t n, int ndiag, DiaOffsets off, const double* __restrict__ dia,
+                            const double* __restrict__ dinv, float* __restrict__ sdia, int* __restrict__ bad) {
+  const int64_t ld = dia_ld(n);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if ((float)(dia[i] * dinv[i]) != 1.0f) atomicOr(bad, 1);
+    const double si = sqrt(dinv[i]);
+    for (int d = 1; d < ndiag; ++d) {
+      const int64_t o = off.o[d];
+      sdia[(int64_t)d * ld + i] = i + o < n ? (float)(dia[(int64_t)d * ld + i] * si * sqrt(dinv[i + o])) : 0.0f;
+    }
+  }
+}
+
+// r64 = b - Ax (Ax nullable = 0), rs = (float)(sqrt(dinv) r64); partials of rs.rs (both B slots)
+// and of ||r64||^2, b.b (P)
+__global__ void __launch_bounds__(PCG_NT) residual_prep_scaled_k(int64_t n, int64_t rows_per_block,
+                                                                 const double* __restrict__ b,
+                                                                 const double* __restrict__ Ax,
+                                                                 const double* __restrict__ dinv,
+                                                                 float* __restrict__ rs, Partials part) {
+  __shared__ double sm[PCG_NT / 64];
+  const int G = gridDim.x;
+  const int64_t r0 = block_first_row(rows_per_block);
+  const int64_t r1 = min(n, r0 + rows_per_block);
+  double prt = 0, pr = 0, pb = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += PCG_NT) {
+    const double bi = b[i];
+    const double r = Ax ? bi - Ax[i] : bi;
+    prt += r * r;
+    pb += bi * bi;
+    const float ri = (float)(r * sqrt(dinv[i]));
+    rs[i] = ri;
+    pr += (double)ri * (double)ri;
+  }
+  store_partial(pr, part.B, sm);
+  store_partial(pr, part.B + G, sm);
+  store_partial(prt, part.P, sm);
+  store_partial(pb, part.P + G, sm);
+}
+
+// x64 += sqrt(dinv) xs, xs = 0
+__global__ void accumulate_scaled_k(int64_t n, double* __restrict__ x, float* __restrict__ xs,
+                                    const double* __restrict__ dinv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += sqrt(dinv[i]) * (double)xs[i];
+    xs[i] = 0.0f;
+  }
+}
+
+__global__ void set_replace_k(PcgState* st, double rr_replace) {
+  if (threadIdx.x == 0) st->rr_replace = rr_replace;
+}
+
+// ---------------------------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------------------------
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -907,9 +1029,11 @@ struct Ws {
   void* xs;      // FP32 accumulator (mixed)
   float* dinv32; // mixed
   double* ax;    // FP64 A x
+  float* sdia;   // mixed on DIA: the Jacobi-scaled FP32 operator (nullable: no room)
 };
 
-static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
+// sdia_elems: FP32 entries of the Jacobi-scaled operator (0: none)
+static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_elems = 0) {
   const size_t vb = mixed ? 4 : 8;
   size_t off = 0;
   auto take = [&](size_t bytes) -> void* {
@@ -931,8 +1055,14 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w) {
   t.xs = mixed ? take(n * 4) : nullptr;
   t.dinv32 = mixed ? (float*)take(n * 4) : nullptr;
   t.ax = (double*)take(n * 8);
+  t.sdia = sdia_elems > 0 ? (float*)take(sdia_elems * 4) : nullptr;
   if (w) *w = t;
   return off;
+}
+
+// FP32 entries of the Jacobi-scaled copy of a DIA operator (its stored diagonals)
+static int64_t scaled_elems(const jno_op* op) {
+  return op && op->format == JNO_FMT_DIA ? (int64_t)op->ndiag * dia_ld(op->n) : 0;
 }
 
 // grid of the iteration kernels: a multiple of 8 (XCD slabs), at most PCG_MAX_BLOCKS, and the
@@ -969,7 +1099,9 @@ static thread_local HostPoll g_poll;
 
 // Iteration modes (Tuning::split): two-reduction CG as K1 K2 (p recomputed inside the SpMV
 // kernel) or K0 KS K2 (p materialized), or single-reduction CG as KA KB.
-enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2 };
+// MODE_SCALED: the split iteration on the Jacobi-scaled FP32 operator (FP32 inner iterations on a
+// DIA operator; the split iteration elsewhere)
+enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3 };
 
 // The SpMV-carrying kernel of one iteration: K1 (MODE_FUSED), KS (MODE_SPLIT) or KB
 // (MODE_SINGLE), by format, DIA diagonal count (4: the 3-D Kuhn lattice; 3: the 2-D triangle
@@ -1049,6 +1181,24 @@ static int launch_kb(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStre
 template <int NT, typename T, typename V>
 static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
   int rc;
+  if (a.unit) {
+    // Jacobi-scaled DIA operator (split iteration only): K0 and K2 without dinv, KS with the unit
+    // main diagonal
+    pcg_k0<NT, T, V, false><<<grid, NT, 0, s>>>(a);
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    switch (op->ndiag) {
+      case 4: pcg_ks<NT, 1, true, 4, false, T, V, true><<<grid, NT, 0, s>>>(a); break;
+      case 3: pcg_ks<NT, 1, true, 3, false, T, V, true><<<grid, NT, 0, s>>>(a); break;
+      default: pcg_ks<NT, 1, true, 0, false, T, V, true><<<grid, NT, 0, s>>>(a); break;
+    }
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[1], s));
+    pcg_k2<NT, T, V, false><<<grid, NT, 0, s>>>(a);
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[2], s));
+    return JNO_OK;
+  }
   if (a.split == MODE_SINGLE) {
     pcg_ka<NT, T, V><<<grid, NT, 0, s>>>(a);
     JNO_LAUNCH_CHECK();
@@ -1206,8 +1356,10 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.ndiag = op->ndiag;
   a.vec = tu.vec;
   a.block = tu.block;
-  // the fused K1 recomputes p for gathered neighbours; BSR3 only has the plain SpMV kernel
-  a.split = (op->format == JNO_FMT_BSR3 && tu.split == MODE_FUSED) ? MODE_SPLIT : tu.split;
+  // the fused K1 recomputes p for gathered neighbours; BSR3 only has the plain SpMV kernel.  The
+  // driver sets `unit` when it runs the scaled operator; otherwise MODE_SCALED is the split mode.
+  a.split = ((op->format == JNO_FMT_BSR3 && tu.split == MODE_FUSED) || tu.split == MODE_SCALED) ? MODE_SPLIT
+                                                                                                 : tu.split;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
   a.dinv = dinv;
   a.r = r;
@@ -1217,6 +1369,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.part = w.part;
   a.p_old = nullptr;
   a.p_new = nullptr;
+  a.unit = 0;
   a.s = (T*)w.s;
   a.u = (T*)w.p1;  // single-reduction iterations keep p in place in p0; p1 holds u
   return a;
@@ -1229,6 +1382,11 @@ static int g_chunk = 8;
 using namespace jno;
 
 extern "C" size_t jno_pcg_workspace_size(int64_t n, int mixed) { return ws_layout(n, mixed ? 1 : 0, nullptr, nullptr); }
+
+extern "C" size_t jno_pcg_workspace_size_op(const jno_op* h_op, int mixed) {
+  if (!h_op) return 0;
+  return ws_layout(h_op->n, mixed ? 1 : 0, nullptr, nullptr, mixed ? scaled_elems(h_op) : 0);
+}
 
 extern "C" int jno_pcg_profile(int enable) {
   g_prof.on = enable != 0;
@@ -1251,8 +1409,9 @@ extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, in
               "jno_pcg_set_tuning: grid_cap must be a multiple of 8 in 8..%d", PCG_MAX_BLOCKS);
   JNO_REQUIRE(block == 256 || block == 512 || block == 1024, JNO_EINVAL,
               "jno_pcg_set_tuning: block must be 256, 512 or 1024");
-  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_SINGLE, JNO_EINVAL,
-              "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2) or 2 (single reduction, KA KB)");
+  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_SCALED, JNO_EINVAL,
+              "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2), 2 (single reduction, KA KB) or 3 (K0 KS K2 "
+              "on the Jacobi-scaled operator)");
   JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 2, JNO_EINVAL,
               "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles) or 2 (4 rows)");
   g_tune[precision] = Tuning{grid_cap, k1_vector, block, split};
@@ -1339,6 +1498,97 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   return finish_status(f, h_stats, f.k_stop);
 }
 
+// FP32 inner iterations on the Jacobi-scaled DIA operator (MODE_SCALED).  Returns JNO_OK with
+// *used = false, having changed nothing the caller relies on, when the operator's scaled main
+// diagonal is not exactly 1 in FP32 (a zero-diagonal guard row): the caller then runs the
+// unscaled inner iteration.  The inner loop stops only on residual replacement or the cap; the
+// FP64 true residual decides convergence, and when the remaining reduction is smaller than a
+// replacement cycle the next cycle is aimed at it (with margin), so the last cycle does not
+// overshoot the tolerance by a full cycle.
+static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
+                        double tol, double atol, int64_t maxiter, double repl2, jno_pcg_stats* h_stats, const Ws& w,
+                        int G, int64_t rpb, const Tuning& tu, hipStream_t s, bool* used) {
+  *used = false;
+  const int64_t n = h_op->n;
+  const int gz = grid_for(n, PCG_NT, 2048);
+  const auto t0 = std::chrono::steady_clock::now();
+  DiaOffsets off;
+  for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = h_op->offsets[d];
+  int* bad = reinterpret_cast<int*>(w.part.P);  // scratch flag; P is rewritten by the preparation
+  JNO_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  scale_dia_k<<<gz, PCG_NT, 0, s>>>(n, h_op->ndiag, off, h_op->vals64, d_dinv, w.sdia, bad);
+  JNO_LAUNCH_CHECK();
+  int h_bad = 0;
+  JNO_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  if (h_bad) return JNO_OK;
+  *used = true;
+  float* rs = (float*)w.r;
+  float* pb[2] = {(float*)w.p0, (float*)w.p1};
+  float* xs = (float*)w.xs;
+  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[0], n);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[1], n);
+  zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
+  int rc;
+  if (use_x0) {
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    if (rc) return rc;
+  } else {
+    zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
+  }
+  residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, rs, w.part);
+  // inner tolerance 0: the FP64 residual below decides convergence
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 1, 1, 0);
+  JNO_LAUNCH_CHECK();
+  PcgArgs<float, float> a = make_args<float, float>(h_op, w.sdia, nullptr, rs, (float*)w.q, xs, w, rpb, tu);
+  a.unit = 1;
+  a.split = MODE_SPLIT;
+  int64_t k = 0, refinements = 0;
+  PcgState f, e;
+  double rr_true = 0.0, bb = 0.0;
+  for (;;) {
+    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f);
+    if (rc) return rc;
+    k = f.k_stop;
+    accumulate_scaled_k<<<gz, PCG_NT, 0, s>>>(n, d_x, xs, d_dinv);
+    JNO_LAUNCH_CHECK();
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    if (rc) return rc;
+    residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, rs, w.part);
+    finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 0, 1, 0);
+    JNO_LAUNCH_CHECK();
+    rc = read_state(w.st, &e, s);
+    if (rc) return rc;
+    ++refinements;
+    rr_true = e.rr_true;
+    bb = e.bb;
+    const double atol2 = fmax(tol * tol * bb, atol * atol);
+    if (!(rr_true > atol2)) {
+      f.status = ST_CONVERGED;
+      break;
+    }
+    if (f.status == ST_MAXITER || f.status == ST_NONFINITE || !std::isfinite(rr_true)) {
+      if (!std::isfinite(rr_true)) f.status = ST_NONFINITE;
+      break;
+    }
+    const double need = atol2 / rr_true;  // remaining reduction of r.r
+    if (need > repl2) {
+      set_replace_k<<<1, 64, 0, s>>>(w.st, 0.5 * need * e.rr0);
+      JNO_LAUNCH_CHECK();
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  const double bn = sqrt(bb);
+  if (h_stats) {
+    h_stats->iterations = k;
+    h_stats->refinements = refinements;
+    h_stats->relres = bn > 0 ? sqrt(rr_true) / bn : sqrt(rr_true);
+    h_stats->relres_recursive = h_stats->relres;  // the last replacement set r to the true residual
+    h_stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+  return finish_status(f, h_stats, k);
+}
+
 extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
                              double tol, double atol, int64_t maxiter, double replace_factor,
                              jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream) {
@@ -1351,13 +1601,21 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
               "jno_pcg_mixed: replace_factor must be in (0, 1)");
   JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_mixed: maxiter must be >= 0");
   hipStream_t s = as_stream(stream);
+  const Tuning tu = g_tune[1];
+  const int64_t se = scaled_elems(h_op);
+  const bool room = se > 0 && ws_bytes >= ws_layout(n, 1, nullptr, nullptr, se);
   Ws w;
-  ws_layout(n, 1, static_cast<char*>(d_ws), &w);
-  const auto t0 = std::chrono::steady_clock::now();
+  ws_layout(n, 1, static_cast<char*>(d_ws), &w, room ? se : 0);
   int G;
   int64_t rpb;
-  const Tuning tu = g_tune[1];
   pcg_grid(h_op, tu, &G, &rpb);
+  if (room && tu.split == MODE_SCALED) {
+    bool used = false;
+    rc = mixed_scaled(h_op, d_dinv, d_b, d_x, use_x0, tol, atol, maxiter, replace_factor * replace_factor, h_stats,
+                      w, G, rpb, tu, s, &used);
+    if (rc || used) return rc;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
   const int gz = grid_for(n, PCG_NT, 2048);
   float* rs = (float*)w.r;
   const bool single = tu.split == MODE_SINGLE;

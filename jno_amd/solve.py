@@ -83,7 +83,7 @@ class LinearSolver:
             use_x0 = 1
         mixed = self.precision == "mixed"
         lib = _lib.load()
-        wsb = lib.jno_pcg_workspace_size(n, 1 if mixed else 0)
+        wsb = lib.jno_pcg_workspace_size_op(C.byref(op.op), 1 if mixed else 0)
         ws = torch.empty(wsb, dtype=torch.uint8, device=A.device)
         stats = _lib.JnoPcgStats()
         stream = _lib.stream_ptr(A.device)

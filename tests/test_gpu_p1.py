@@ -262,8 +262,9 @@ def test_operator_layouts_agree(dev):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_iteration_modes_and_launch_shapes(dev, precision):
     """Every launch shape of the PCG iteration kernels solves to the oracle's direct solution:
-    iteration mode K1 K2 / K0 KS K2 / KA KB (single reduction), SpMV variant (2 rows per thread,
-    16-byte tiles, 4 rows per thread), several workgroup sizes and grid caps, DIA and CSR."""
+    iteration mode K1 K2 / K0 KS K2 / KA KB (single reduction) / K0 KS K2 on the Jacobi-scaled
+    FP32 operator, SpMV variant (2 rows per thread, 16-byte tiles, 4 rows per thread), several
+    workgroup sizes and grid caps, DIA and CSR."""
     from jno_amd import precond, solve
 
     n = 11  # non-dyadic, ragged row ranges per workgroup
@@ -275,7 +276,7 @@ def test_iteration_modes_and_launch_shapes(dev, precision):
     saved = solve.get_tuning(precision)
     try:
         for fmt in ("dia", "csr"):
-            for mode in (0, 1, 2):
+            for mode in (0, 1, 2, 3):
                 for vec in (0, 1, 2):
                     for blk, cap in ((256, 8), (1024, 64), (512, 4096)):
                         solve.set_tuning(precision, blk, cap, vec, mode)
