@@ -138,7 +138,11 @@ def main():
         spmv_mat = N * 4 * op.ndiag  # main + stored upper diagonals (the lower ones are the same values)
     else:
         spmv_mat = op.nnz_stored * (4 + 4) + 4 * (N + 1)
-    if mode == 1:  # K0 (p, x update), KS (q = A p, p.q), K2 (r update, r.z, r.r)
+    if mode == 3 and op.format == "dia":  # K0 KS K2 on the Jacobi-scaled operator: no dinv, unit diagonal
+        kernels = [("pcg_k0<scaled>", ms[0], N * (3 * 4 + 2 * 4)),
+                   ("pcg_ks<dia, f32, unit>", ms[1], N * 4 * (op.ndiag - 1) + N * (4 + 4)),
+                   ("pcg_k2<scaled>", ms[2], N * (2 * 4 + 4))]
+    elif mode in (1, 3):  # K0 (p, x update), KS (q = A p, p.q), K2 (r update, r.z, r.r)
         kernels = [("pcg_k0", ms[0], N * (4 * 4 + 2 * 4)),
                    (f"pcg_ks<{op.format}, f32>", ms[1], spmv_mat + N * (4 + 4)),
                    ("pcg_k2", ms[2], N * (3 * 4 + 4))]

@@ -135,12 +135,28 @@ __device__ __forceinline__ void store_partial(double v, double* __restrict__ dst
 template <int M>
 __device__ __forceinline__ void reduce_in(const double* __restrict__ part, int G, double* sm_in, double* out) {
   if (threadIdx.x < 64) {
+    double s[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) s[m] = 0.0;
+    // 8 entries per lane and array in flight at once (one round trip for G <= 512)
+    for (int base = 0; base < G; base += 64 * 8) {
+      double v[M][8];
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int i = base + u * 64 + (int)threadIdx.x;
+          v[m][u] = i < G ? part[(int64_t)m * G + i] : 0.0;
+        }
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[m] += v[m][u];
+    }
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      double s = 0.0;
-      for (int i = threadIdx.x; i < G; i += 64) s += part[(int64_t)m * G + i];
-      s = wave_sum(s);
-      if (threadIdx.x == 0) sm_in[m] = s;
+      const double t = wave_sum(s[m]);
+      if (threadIdx.x == 0) sm_in[m] = t;
     }
   }
   __syncthreads();
@@ -149,7 +165,7 @@ __device__ __forceinline__ void reduce_in(const double* __restrict__ part, int G
 }
 
 // Workgroup totals of M per-thread values into dst[m*G + blockIdx.x]: each wave reduces, the wave
-// totals go through sm_out[M][NT/64], thread 0 adds them in wave order.
+// totals go through sm_out[M][NT/64], and wave 0 adds them (a fixed butterfly).
 template <int NT, int M>
 __device__ __forceinline__ void store_out(const double* v, double* __restrict__ dst, int G, double* sm_out) {
   const int w = threadIdx.x >> 6;
@@ -159,12 +175,11 @@ __device__ __forceinline__ void store_out(const double* v, double* __restrict__ 
     if ((threadIdx.x & 63) == 0) sm_out[m * (NT / 64) + w] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      double t = 0.0;
-      for (int i = 0; i < NT / 64; ++i) t += sm_out[m * (NT / 64) + i];
-      dst[(int64_t)m * G + blockIdx.x] = t;
+      const double t = wave_sum(threadIdx.x < NT / 64 ? sm_out[m * (NT / 64) + threadIdx.x] : 0.0);
+      if (threadIdx.x == 0) dst[(int64_t)m * G + blockIdx.x] = t;
     }
   }
 }
