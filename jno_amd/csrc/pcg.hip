@@ -208,6 +208,12 @@ __device__ __forceinline__ V16<T> ld16(const T* p) {
   return r;
 }
 
+// Plain (write-back) stores of the iteration vectors: non-temporal stores measured no faster here.
+template <typename T>
+__device__ __forceinline__ void sto(T* p, T v) {
+  *p = v;
+}
+
 template <typename T>
 __device__ __forceinline__ void st16(T* p, const V16<T>& r) {
   __builtin_memcpy(p, &r, 16);
@@ -269,8 +275,8 @@ __device__ __forceinline__ double dia_row(const PcgArgs<T, V>& a, int64_t i, T b
     if (i + o < a.n) s += (T)v[i] * pnew_at(a.dinv, a.r, a.p_old, beta, i + o);
     if (i - o >= 0) s += (T)v[i - o] * pnew_at(a.dinv, a.r, a.p_old, beta, i - o);
   }
-  a.p_new[i] = pi;
-  a.q[i] = s;
+  sto(&a.p_new[i], pi);
+  sto(&a.q[i], s);
   if (upd_x) a.x[i] += ap * po;
   return (double)pi * (double)s;
 }
@@ -316,9 +322,9 @@ __device__ __forceinline__ double dia_rows(const PcgArgs<T, V>& a, int64_t i, T 
       s += (T)vu[r][d] * (du[r][d] * ru[r][d] + beta * pu[r][d]);
       s += (T)vl[r][d] * (dl[r][d] * rl[r][d] + beta * pl[r][d]);
     }
-    a.p_new[row] = pi;
-    a.q[row] = s;
-    if (upd_x) a.x[row] = xv[r] + ap * po[r];
+    sto(&a.p_new[row], pi);
+    sto(&a.q[row], s);
+    if (upd_x) sto(&a.x[row], xv[r] + ap * po[r]);
     part += (double)pi * (double)s;
   }
   return part;
@@ -414,9 +420,9 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
       if (sub == 0) {
         const T po = a.p_old[row];
         const T pi = a.dinv[row] * a.r[row] + beta * po;
-        a.p_new[row] = pi;
-        a.q[row] = s;
-        if (k > 0) a.x[row] += ap * po;
+        sto(&a.p_new[row], pi);
+        sto(&a.q[row], s);
+        if (k > 0) sto(&a.x[row], a.x[row] + ap * po);
         part += (double)pi * (double)s;
       }
     }
@@ -603,7 +609,7 @@ __device__ __forceinline__ double spmv_dia_row(const PcgArgs<T, V>& a, int64_t i
     if (i + o < a.n) s += (T)v[i] * p[i + o];
     if (i - o >= 0) s += (T)v[i - o] * p[i - o];
   }
-  a.q[i] = s;
+  sto(&a.q[i], s);
   return (double)pi * (double)s;
 }
 
@@ -637,7 +643,7 @@ __device__ __forceinline__ double spmv_dia_rows(const PcgArgs<T, V>& a, int64_t 
       s += (T)vu[r][d] * pu[r][d];
       s += (T)vl[r][d] * pl[r][d];
     }
-    a.q[i + (int64_t)r * NT] = s;
+    sto(&a.q[i + (int64_t)r * NT], s);
     part += (double)pv[r] * (double)s;
   }
   return part;
@@ -742,7 +748,7 @@ __global__ void __launch_bounds__(NT) pcg_ks(PcgArgs<T, V> a) {
 #pragma unroll
       for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
       if (sub == 0) {
-        a.q[row] = s;
+        sto(&a.q[row], s);
         part += (double)p[row] * (double)s;
       }
     }
@@ -775,9 +781,9 @@ __global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
     T s0, s1, s2;
     bsr3_row<L, V, T>(a.rowptr, a.col, a.vals, p, R, sub, s0, s1, s2);
     if (sub == 0) {
-      a.q[3 * R] = s0;
-      a.q[3 * R + 1] = s1;
-      a.q[3 * R + 2] = s2;
+      sto(&a.q[3 * R], s0);
+      sto(&a.q[3 * R + 1], s1);
+      sto(&a.q[3 * R + 2], s2);
       part += (double)p[3 * R] * (double)s0 + (double)p[3 * R + 1] * (double)s1 + (double)p[3 * R + 2] * (double)s2;
     }
   }
