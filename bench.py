@@ -138,7 +138,10 @@ def main():
         spmv_mat = N * 4 * op.ndiag  # main + stored upper diagonals (the lower ones are the same values)
     else:
         spmv_mat = op.nnz_stored * (4 + 4) + 4 * (N + 1)
-    if mode == 3 and op.format == "dia":  # K0 KS K2 on the Jacobi-scaled operator: no dinv, unit diagonal
+    if mode == 4 and op.format == "dia":  # K1 K2 on the Jacobi-scaled operator: no dinv, unit diagonal
+        kernels = [("pcg_k1<dia, f32, scaled>", ms[1], N * 4 * (op.ndiag - 1) + N * (3 * 4 + 3 * 4)),
+                   ("pcg_k2<scaled>", ms[2], N * (2 * 4 + 4))]
+    elif mode == 3 and op.format == "dia":  # K0 KS K2 on the Jacobi-scaled operator: no dinv, unit diagonal
         kernels = [("pcg_k0<scaled>", ms[0], N * (3 * 4 + 2 * 4)),
                    ("pcg_ks<dia, f32, unit>", ms[1], N * 4 * (op.ndiag - 1) + N * (4 + 4)),
                    ("pcg_k2<scaled>", ms[2], N * (2 * 4 + 4))]
@@ -149,7 +152,7 @@ def main():
     elif mode == 2:  # KA (all vector updates), KB (w = A u, u.w)
         kernels = [("pcg_ka", ms[0], N * (6 * 4 + 5 * 4)),
                    (f"pcg_kb<{op.format}, f32>", ms[1], spmv_mat + N * (4 + 4))]
-    else:  # K1 (p recompute for gathered neighbours, x update, q = A p), K2
+    else:  # modes 0 and 4 off DIA: K1 (p recompute for gathered neighbours, x update, q = A p), K2
         kernels = [(f"pcg_k1<{op.format}, f32>", ms[1], spmv_mat + N * (4 * 4 + 3 * 4)),
                    ("pcg_k2", ms[2], N * (3 * 4 + 4))]
     kernel_rows = {name: {"avg_ms": round(t, 5), "bytes": b, "gbs": round(b / (t * 1e-3) / 1e9, 1) if t > 0 else None}

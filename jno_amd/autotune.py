@@ -67,7 +67,7 @@ CANDIDATE_SHAPES = tuple(
     for blk in (256, 512, 1024)
     for cap in (256, 512, 1024)
     for vec in (0, 1)
-    for split in (0, 1, 3)
+    for split in (0, 1, 3, 4)
 )
 _LAUNCH_CACHE: dict = {}
 

@@ -35,16 +35,16 @@ constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup pa
 // Launch shape of the iteration kernels, one per iteration precision (set by the autotuner through
 // jno_pcg_set_tuning).  The defaults are among the fastest shapes of sweeps over workgroup size x
 // grid cap x SpMV tile variant x iteration mode on a 2.1M-row 4-diagonal operator (128^3 Kuhn P1):
-// 1024 threads x 512 workgroups, p materialized (K0 KS K2), two rows per thread in KS; the FP32
-// inner iterations run on the Jacobi-scaled operator (mode 3).
+// 1024 threads x 512 workgroups, two rows per thread in the SpMV kernel; FP64 iterations with p
+// materialized (K0 KS K2), FP32 inner iterations as K1 K2 on the Jacobi-scaled operator (mode 4).
 struct Tuning {
   int grid_cap;  // workgroups of the iteration kernels (a multiple of 8)
   int vec;       // DIA SpMV kernel: 0 two rows per thread, 1 VN-row 16-byte tiles, 2 four rows
   int block;     // threads per workgroup (256, 512, 1024)
   int split;     // iteration mode: 0 K1 K2, 1 K0 KS K2 (p materialized), 2 KA KB (single reduction),
-                 // 3 K0 KS K2 on the Jacobi-scaled operator (FP32 inner iterations on DIA)
+                 // 3 K0 KS K2 and 4 K1 K2 on the Jacobi-scaled operator (FP32 inner iterations on DIA)
 };
-static Tuning g_tune[2] = {{512, 0, 1024, 1}, {512, 0, 1024, 3}};  // [0] FP64 iterations, [1] FP32 iterations
+static Tuning g_tune[2] = {{512, 0, 1024, 1}, {512, 0, 1024, 4}};  // [0] FP64 iterations, [1] FP32 iterations
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -221,31 +221,40 @@ __device__ __forceinline__ void st16(T* p, const V16<T>& r) {
 
 // DIA row tile [i0, i0+VN): p_new = z + beta p_old on the tile, q = A p_new, deferred x update;
 // returns the tile's p.q.  `interior` tiles need no bounds checks on neighbour rows.
-template <int ND, typename T, typename V>
+// JAC = false: the Jacobi-scaled operator (z = r, unit main diagonal not read).
+template <int ND, typename T, typename V, bool JAC = true>
 __device__ __forceinline__ double dia_tile(const PcgArgs<T, V>& a, int64_t i0, T beta, T ap, bool upd_x) {
   constexpr int VN = V16<T>::N;
   const int64_t ld = dia_ld(a.n);
   const int nd = ND > 0 ? ND : a.ndiag;
-  const V16<T> dv = ld16(a.dinv + i0), rv = ld16(a.r + i0), po = ld16(a.p_old + i0);
-  V16<T> pi, s;
+  const V16<T> rv = ld16(a.r + i0), po = ld16(a.p_old + i0);
+  V16<T> dv, pi, s;
+  if (JAC) dv = ld16(a.dinv + i0);
 #pragma unroll
-  for (int j = 0; j < VN; ++j) pi.v[j] = dv.v[j] * rv.v[j] + beta * po.v[j];
-  {
+  for (int j = 0; j < VN; ++j) pi.v[j] = (JAC ? dv.v[j] * rv.v[j] : rv.v[j]) + beta * po.v[j];
+  if (JAC) {
     const V16<V> d0 = ld16(a.vals + i0);
 #pragma unroll
     for (int j = 0; j < VN; ++j) s.v[j] = (T)d0.v[j] * pi.v[j];
+  } else {
+    s = pi;
   }
 #pragma unroll
   for (int d = 1; d < nd; ++d) {
     const int64_t o = a.off.o[d];
     const V* vd = a.vals + (int64_t)d * ld;
     const V16<V> vu = ld16(vd + i0), vl = ld16(vd + i0 - o);
-    const V16<T> du = ld16(a.dinv + i0 + o), ru = ld16(a.r + i0 + o), pu = ld16(a.p_old + i0 + o);
-    const V16<T> dl = ld16(a.dinv + i0 - o), rl = ld16(a.r + i0 - o), pl = ld16(a.p_old + i0 - o);
+    const V16<T> ru = ld16(a.r + i0 + o), pu = ld16(a.p_old + i0 + o);
+    const V16<T> rl = ld16(a.r + i0 - o), pl = ld16(a.p_old + i0 - o);
+    V16<T> du, dl;
+    if (JAC) {
+      du = ld16(a.dinv + i0 + o);
+      dl = ld16(a.dinv + i0 - o);
+    }
 #pragma unroll
     for (int j = 0; j < VN; ++j) {
-      s.v[j] += (T)vu.v[j] * (du.v[j] * ru.v[j] + beta * pu.v[j]);
-      s.v[j] += (T)vl.v[j] * (dl.v[j] * rl.v[j] + beta * pl.v[j]);
+      s.v[j] += (T)vu.v[j] * ((JAC ? du.v[j] * ru.v[j] : ru.v[j]) + beta * pu.v[j]);
+      s.v[j] += (T)vl.v[j] * ((JAC ? dl.v[j] * rl.v[j] : rl.v[j]) + beta * pl.v[j]);
     }
   }
   st16(a.p_new + i0, pi);
@@ -263,17 +272,17 @@ __device__ __forceinline__ double dia_tile(const PcgArgs<T, V>& a, int64_t i0, T
 }
 
 // one DIA row with bounds checks (tiles at the ends of the matrix)
-template <typename T, typename V>
+template <typename T, typename V, bool JAC = true>
 __device__ __forceinline__ double dia_row(const PcgArgs<T, V>& a, int64_t i, T beta, T ap, bool upd_x) {
   const int64_t ld = dia_ld(a.n);
   const T po = a.p_old[i];
-  const T pi = a.dinv[i] * a.r[i] + beta * po;
-  T s = (T)a.vals[i] * pi;
+  const T pi = (JAC ? a.dinv[i] * a.r[i] : a.r[i]) + beta * po;
+  T s = JAC ? (T)a.vals[i] * pi : pi;
   for (int d = 1; d < a.ndiag; ++d) {
     const int64_t o = a.off.o[d];
     const V* v = a.vals + (int64_t)d * ld;
-    if (i + o < a.n) s += (T)v[i] * pnew_at(a.dinv, a.r, a.p_old, beta, i + o);
-    if (i - o >= 0) s += (T)v[i - o] * pnew_at(a.dinv, a.r, a.p_old, beta, i - o);
+    if (i + o < a.n) s += (T)v[i] * ((JAC ? a.dinv[i + o] * a.r[i + o] : a.r[i + o]) + beta * a.p_old[i + o]);
+    if (i - o >= 0) s += (T)v[i - o] * ((JAC ? a.dinv[i - o] * a.r[i - o] : a.r[i - o]) + beta * a.p_old[i - o]);
   }
   sto(&a.p_new[i], pi);
   sto(&a.q[i], s);
@@ -284,7 +293,7 @@ __device__ __forceinline__ double dia_row(const PcgArgs<T, V>& a, int64_t i, T b
 // R interior DIA rows i, i + NT, ..., i + (R-1) NT of one thread (all neighbours in range) with a
 // compile-time diagonal count: every load of the R rows is independent of every other, so they are
 // all in flight before the first use (a runtime diagonal loop waits on each diagonal's loads).
-template <int ND, int R, int NT, typename T, typename V>
+template <int ND, int R, int NT, typename T, typename V, bool JAC = true>
 __device__ __forceinline__ double dia_rows(const PcgArgs<T, V>& a, int64_t i, T beta, T ap, bool upd_x) {
   const int64_t ld = dia_ld(a.n);
   T dv[R], rv[R], po[R], xv[R];
@@ -293,20 +302,20 @@ __device__ __forceinline__ double dia_rows(const PcgArgs<T, V>& a, int64_t i, T 
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int64_t row = i + (int64_t)r * NT;
-    dv[r] = a.dinv[row];
+    if (JAC) dv[r] = a.dinv[row];
     rv[r] = a.r[row];
     po[r] = a.p_old[row];
-    d0[r] = a.vals[row];
+    if (JAC) d0[r] = a.vals[row];
     xv[r] = upd_x ? a.x[row] : T(0);
 #pragma unroll
     for (int d = 1; d < ND; ++d) {
       const int64_t o = a.off.o[d];
       vu[r][d - 1] = a.vals[(int64_t)d * ld + row];
       vl[r][d - 1] = a.vals[(int64_t)d * ld + row - o];
-      du[r][d - 1] = a.dinv[row + o];
+      if (JAC) du[r][d - 1] = a.dinv[row + o];
       ru[r][d - 1] = a.r[row + o];
       pu[r][d - 1] = a.p_old[row + o];
-      dl[r][d - 1] = a.dinv[row - o];
+      if (JAC) dl[r][d - 1] = a.dinv[row - o];
       rl[r][d - 1] = a.r[row - o];
       pl[r][d - 1] = a.p_old[row - o];
     }
@@ -315,12 +324,12 @@ __device__ __forceinline__ double dia_rows(const PcgArgs<T, V>& a, int64_t i, T 
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int64_t row = i + (int64_t)r * NT;
-    const T pi = dv[r] * rv[r] + beta * po[r];
-    T s = (T)d0[r] * pi;
+    const T pi = (JAC ? dv[r] * rv[r] : rv[r]) + beta * po[r];
+    T s = JAC ? (T)d0[r] * pi : pi;
 #pragma unroll
     for (int d = 0; d < ND - 1; ++d) {
-      s += (T)vu[r][d] * (du[r][d] * ru[r][d] + beta * pu[r][d]);
-      s += (T)vl[r][d] * (dl[r][d] * rl[r][d] + beta * pl[r][d]);
+      s += (T)vu[r][d] * ((JAC ? du[r][d] * ru[r][d] : ru[r][d]) + beta * pu[r][d]);
+      s += (T)vl[r][d] * ((JAC ? dl[r][d] * rl[r][d] : rl[r][d]) + beta * pl[r][d]);
     }
     sto(&a.p_new[row], pi);
     sto(&a.q[row], s);
@@ -368,7 +377,8 @@ __device__ __forceinline__ bool iter_head(const PcgArgs<T, V>& a, double* sm, do
   return true;
 }
 
-template <int NT, int L, bool DIA, int ND, typename T, typename V>
+// JAC = false (DIA only): the Jacobi-scaled operator, z = r and a unit main diagonal
+template <int NT, int L, bool DIA, int ND, typename T, typename V, bool JAC = true>
 __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
   __shared__ double sm[4];                  // broadcast of reduced partial sums
   __shared__ double sm_out[2 * (NT / 64)];  // wave totals of this kernel's partial sums
@@ -386,9 +396,9 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
     for (int d = 1; d < a.ndiag; ++d) omax = max(omax, (int64_t)a.off.o[d]);
     for (int64_t i0 = r0 + (int64_t)threadIdx.x * VN; i0 < r1; i0 += (int64_t)VN * NT) {
       if (i0 + VN <= r1 && i0 - omax >= 0 && i0 + VN - 1 + omax < a.n) {
-        part += dia_tile<ND, T, V>(a, i0, beta, ap, k > 0);
+        part += dia_tile<ND, T, V, JAC>(a, i0, beta, ap, k > 0);
       } else {
-        for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += dia_row<T, V>(a, i, beta, ap, k > 0);
+        for (int64_t i = i0; i < min(r1, i0 + VN); ++i) part += dia_row<T, V, JAC>(a, i, beta, ap, k > 0);
       }
     }
   } else if (DIA && ND > 0) {
@@ -400,15 +410,15 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
     int64_t i = r0 + threadIdx.x;
     for (; i + (int64_t)(R - 1) * NT < r1; i += (int64_t)R * NT) {
       if (i - omax >= 0 && i + (int64_t)(R - 1) * NT + omax < a.n) {
-        part += dia_rows<(ND > 0 ? ND : 1), R, NT, T, V>(a, i, beta, ap, k > 0);
+        part += dia_rows<(ND > 0 ? ND : 1), R, NT, T, V, JAC>(a, i, beta, ap, k > 0);
       } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) part += dia_row<T, V>(a, i + (int64_t)r * NT, beta, ap, k > 0);
+        for (int r = 0; r < R; ++r) part += dia_row<T, V, JAC>(a, i + (int64_t)r * NT, beta, ap, k > 0);
       }
     }
-    for (; i < r1; i += NT) part += dia_row<T, V>(a, i, beta, ap, k > 0);
+    for (; i < r1; i += NT) part += dia_row<T, V, JAC>(a, i, beta, ap, k > 0);
   } else if (DIA) {
-    for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += dia_row<T, V>(a, i, beta, ap, k > 0);
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) part += dia_row<T, V, JAC>(a, i, beta, ap, k > 0);
   } else {
     const int sub = threadIdx.x % L;
     for (int64_t row = r0 + threadIdx.x / L; row < r1; row += NT / L) {
@@ -1120,9 +1130,9 @@ static thread_local HostPoll g_poll;
 
 // Iteration modes (Tuning::split): two-reduction CG as K1 K2 (p recomputed inside the SpMV
 // kernel) or K0 KS K2 (p materialized), or single-reduction CG as KA KB.
-// MODE_SCALED: the split iteration on the Jacobi-scaled FP32 operator (FP32 inner iterations on a
-// DIA operator; the split iteration elsewhere)
-enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3 };
+// MODE_SCALED / MODE_SCALED_FUSED: the split / fused iteration on the Jacobi-scaled FP32 operator
+// (FP32 inner iterations on a DIA operator; the unscaled split / fused iteration elsewhere)
+enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SCALED_FUSED = 4 };
 
 // The SpMV-carrying kernel of one iteration: K1 (MODE_FUSED), KS (MODE_SPLIT) or KB
 // (MODE_SINGLE), by format, DIA diagonal count (4: the 3-D Kuhn lattice; 3: the 2-D triangle
@@ -1202,9 +1212,24 @@ static int launch_kb(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStre
 template <int NT, typename T, typename V>
 static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
   int rc;
+  if (a.unit && a.split == MODE_FUSED) {
+    // Jacobi-scaled DIA operator, two kernels: K1 and K2 without dinv, unit main diagonal
+    if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    switch (op->ndiag) {
+      case 4: pcg_k1<NT, 1, true, 4, T, V, false><<<grid, NT, 0, s>>>(a); break;
+      case 3: pcg_k1<NT, 1, true, 3, T, V, false><<<grid, NT, 0, s>>>(a); break;
+      default: pcg_k1<NT, 1, true, 0, T, V, false><<<grid, NT, 0, s>>>(a); break;
+    }
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[1], s));
+    pcg_k2<NT, T, V, false><<<grid, NT, 0, s>>>(a);
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[2], s));
+    return JNO_OK;
+  }
   if (a.unit) {
-    // Jacobi-scaled DIA operator (split iteration only): K0 and K2 without dinv, KS with the unit
-    // main diagonal
+    // Jacobi-scaled DIA operator, split iteration: K0 and K2 without dinv, KS with the unit main
+    // diagonal
     pcg_k0<NT, T, V, false><<<grid, NT, 0, s>>>(a);
     JNO_LAUNCH_CHECK();
     if (ev) JNO_HIP(hipEventRecord(ev[0], s));
@@ -1379,8 +1404,9 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.block = tu.block;
   // the fused K1 recomputes p for gathered neighbours; BSR3 only has the plain SpMV kernel.  The
   // driver sets `unit` when it runs the scaled operator; otherwise MODE_SCALED is the split mode.
-  a.split = ((op->format == JNO_FMT_BSR3 && tu.split == MODE_FUSED) || tu.split == MODE_SCALED) ? MODE_SPLIT
-                                                                                                 : tu.split;
+  int mode = tu.split == MODE_SCALED ? MODE_SPLIT : tu.split == MODE_SCALED_FUSED ? MODE_FUSED : tu.split;
+  if (op->format == JNO_FMT_BSR3 && mode == MODE_FUSED) mode = MODE_SPLIT;
+  a.split = mode;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
   a.dinv = dinv;
   a.r = r;
@@ -1430,9 +1456,9 @@ extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, in
               "jno_pcg_set_tuning: grid_cap must be a multiple of 8 in 8..%d", PCG_MAX_BLOCKS);
   JNO_REQUIRE(block == 256 || block == 512 || block == 1024, JNO_EINVAL,
               "jno_pcg_set_tuning: block must be 256, 512 or 1024");
-  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_SCALED, JNO_EINVAL,
-              "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2), 2 (single reduction, KA KB) or 3 (K0 KS K2 "
-              "on the Jacobi-scaled operator)");
+  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_SCALED_FUSED, JNO_EINVAL,
+              "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2), 2 (single reduction, KA KB), 3 (K0 KS K2 "
+              "on the Jacobi-scaled operator) or 4 (K1 K2 on the Jacobi-scaled operator)");
   JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 2, JNO_EINVAL,
               "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles) or 2 (4 rows)");
   g_tune[precision] = Tuning{grid_cap, k1_vector, block, split};
@@ -1563,7 +1589,7 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   JNO_LAUNCH_CHECK();
   PcgArgs<float, float> a = make_args<float, float>(h_op, w.sdia, nullptr, rs, (float*)w.q, xs, w, rpb, tu);
   a.unit = 1;
-  a.split = MODE_SPLIT;
+  a.split = tu.split == MODE_SCALED_FUSED ? MODE_FUSED : MODE_SPLIT;
   int64_t k = 0, refinements = 0;
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;
@@ -1630,7 +1656,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   int G;
   int64_t rpb;
   pcg_grid(h_op, tu, &G, &rpb);
-  if (room && tu.split == MODE_SCALED) {
+  if (room && (tu.split == MODE_SCALED || tu.split == MODE_SCALED_FUSED)) {
     bool used = false;
     rc = mixed_scaled(h_op, d_dinv, d_b, d_x, use_x0, tol, atol, maxiter, replace_factor * replace_factor, h_stats,
                       w, G, rpb, tu, s, &used);

@@ -276,7 +276,7 @@ def test_iteration_modes_and_launch_shapes(dev, precision):
     saved = solve.get_tuning(precision)
     try:
         for fmt in ("dia", "csr"):
-            for mode in (0, 1, 2, 3):
+            for mode in (0, 1, 2, 3, 4):
                 for vec in (0, 1, 2):
                     for blk, cap in ((256, 8), (1024, 64), (512, 4096)):
                         solve.set_tuning(precision, blk, cap, vec, mode)
