@@ -25,6 +25,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+REPLACE_FACTOR = 0.01
 
 
 def log(msg):
@@ -69,7 +70,10 @@ def main():
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
 
-    mixed = solve.cg(tol=1e-10, precision="mixed")
+    # FP64 residual replacement after a 100x drop of the FP32 residual: on this P1 Poisson operator
+    # it takes 5 instead of 10 replacements at the same 645 inner iterations (the library default,
+    # 0.1, is the safer choice for harder operators such as P2 elasticity)
+    mixed = solve.cg(tol=1e-10, precision="mixed", replace_factor=REPLACE_FACTOR)
     fp64 = solve.cg(tol=1e-10, precision="fp64")
 
     def step(solver):
@@ -236,7 +240,7 @@ def grad_rate(J, torch, dev, n=64, reps=3):
     M = system.mass()
     res = {}
     for prec in ("mixed", "fp64"):
-        sv = solve.cg(tol=1e-10, precision=prec)
+        sv = solve.cg(tol=1e-10, precision=prec, replace_factor=REPLACE_FACTOR)
         from jno_amd import precond
 
         Ast, bst = system.evaluate({"k": ks})

@@ -126,6 +126,15 @@ def test_c4_batch_256_uneven_and_64_rhs(dev):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     assert all(st["relres"] <= 1e-10 for st in bs.last_stats)
+    # N(0,1) right-hand sides are rough: at rtol 1e-10 a solution carries up to cond(A) * 1e-10 of
+    # error, more than 1e-9 here.  The bar is therefore relative to an FP64 solve at rtol 1e-13: the
+    # batched solutions must be as accurate as the single-system solver's at the same tolerance.
+    tight = solve.cg(tol=1e-13, precision="fp64")
+    errs = []
     for j in (0, 63):
-        assert _rel(_np(xs[j]), _np(single(A, rhs[j], M=precond.jacobi()))) <= 1e-9
-    print(f"C4 64 rhs on 96^3: {dt:.3f} s")
+        ut = _np(tight(A, rhs[j], M=precond.jacobi()))
+        e_batch = _rel(_np(xs[j]), ut)
+        e_single = _rel(_np(single(A, rhs[j], M=precond.jacobi())), ut)
+        errs.append((e_batch, e_single))
+        assert e_batch <= max(1e-9, 3.0 * e_single), (j, e_batch, e_single)
+    print(f"C4 64 rhs on 96^3: {dt:.3f} s; error vs rtol-1e-13 FP64 (batched, single) {errs}")
