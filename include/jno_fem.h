@@ -161,6 +161,13 @@ int jno_op_analyze(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col
 int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
                     const double* d_vals, int ndiag, const int32_t* h_offsets, double* d_dia64,
                     float* d_dia32, void* stream);
+/* jno_op_fill_dia plus a check that the layout reproduces the CSR: *h_ok = 1 when every stored
+ * entry off the given diagonals is exactly 0 and every lower entry equals its stored upper mirror
+ * bitwise (so the offsets of an earlier operator on the same pattern can be reused without a new
+ * jno_op_analyze), else 0.  d_dia64 is required; d_ws: 256 bytes. */
+int jno_op_fill_dia_checked(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                            const double* d_vals, int ndiag, const int32_t* h_offsets, double* d_dia64,
+                            float* d_dia32, int* h_ok, void* d_ws, size_t ws_bytes, void* stream);
 /* BSR3 applicability of a CSR: *h_nnzb = number of 3x3 blocks when every block row's three rows
  * hold the columns 3c+0, 3c+1, 3c+2 of the same ascending block columns c (the vec = 3 pattern of
  * jno_pattern_count/fill), else -1.  jno_op_fill_bsr3 writes the block rowptr (n_rows/3 + 1), the

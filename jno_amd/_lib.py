@@ -89,6 +89,7 @@ SIGNATURES = {
     "jno_op_analyze_workspace_size": (SZ, []),
     "jno_op_analyze": (C.c_int, [I64, P, P, P, C.POINTER(I32), C.POINTER(I32), P, SZ, P]),
     "jno_op_fill_dia": (C.c_int, [I64, P, P, P, C.c_int, C.POINTER(I32), P, P, P]),
+    "jno_op_fill_dia_checked": (C.c_int, [I64, P, P, P, C.c_int, C.POINTER(I32), P, P, C.POINTER(C.c_int), P, SZ, P]),
     "jno_op_bsr3_workspace_size": (SZ, []),
     "jno_op_bsr3_count": (C.c_int, [I64, P, P, C.POINTER(I64), P, SZ, P]),
     "jno_op_fill_bsr3": (C.c_int, [I64, P, P, P, P, P, P, P, P]),

@@ -311,6 +311,34 @@ __global__ void dia_fill_k(int64_t n, const int32_t* __restrict__ rowptr, const 
   }
 }
 
+// After dia_fill_k: every stored entry off the stored diagonal set must be exactly 0, and every
+// lower entry (r, c), c < r, must equal bitwise the stored upper value A[c, r] = dia[d][c].
+// Sets *bad otherwise (the layout would not reproduce A).
+__global__ void dia_check_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ vals, int ndiag, DiaOffsets off,
+                            const double* __restrict__ d64, int* __restrict__ bad) {
+  const int64_t ld = dia_ld(n);
+  bool ok = true;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      const int c = col[j];
+      const int o = c - (int)r;
+      const int a = o < 0 ? -o : o;
+      int dd = -1;
+#pragma unroll
+      for (int d = 0; d < JNO_DIA_MAX; ++d)
+        if (d < ndiag && off.o[d] == a) dd = d;
+      const double v = vals[j];
+      if (dd < 0) {
+        ok = ok && v == 0.0;
+      } else if (o < 0) {
+        ok = ok && __double_as_longlong(v) == __double_as_longlong(d64[(int64_t)dd * ld + c]);
+      }
+    }
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
 }  // namespace jno
 
 using namespace jno;
@@ -469,6 +497,32 @@ extern "C" int jno_op_fill_dia(int64_t n_rows, const int32_t* d_rowptr, const in
   dia_fill_k<<<grid_for(n_rows, 256), 256, 0, as_stream(stream)>>>(n_rows, d_rowptr, d_col, d_vals, ndiag, off,
                                                                     d_dia64, d_dia32);
   JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_op_fill_dia_checked(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                                       const double* d_vals, int ndiag, const int32_t* h_offsets, double* d_dia64,
+                                       float* d_dia32, int* h_ok, void* d_ws, size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(h_ok && d_dia64 && d_ws, JNO_EINVAL, "jno_op_fill_dia_checked: null pointer");
+  JNO_REQUIRE(ws_bytes >= 256, JNO_EWORKSPACE, "jno_op_fill_dia_checked: workspace too small (256 bytes)");
+  *h_ok = 0;
+  int rc = jno_op_fill_dia(n_rows, d_rowptr, d_col, d_vals, ndiag, h_offsets, d_dia64, d_dia32, stream);
+  if (rc) return rc;
+  if (n_rows == 0) {
+    *h_ok = 1;
+    return JNO_OK;
+  }
+  hipStream_t s = as_stream(stream);
+  DiaOffsets off;
+  for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = d < ndiag ? h_offsets[d] : 0;
+  int* bad = static_cast<int*>(d_ws);
+  JNO_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  dia_check_k<<<grid_for(n_rows, 256), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_vals, ndiag, off, d_dia64, bad);
+  JNO_LAUNCH_CHECK();
+  int h_bad = 1;
+  JNO_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *h_ok = h_bad ? 0 : 1;
   return JNO_OK;
 }
 

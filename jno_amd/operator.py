@@ -24,6 +24,15 @@ from . import _lib
 from .fem import CSR
 
 
+# DIA offsets of earlier operators, per pattern (the plan's index arrays): a re-assembled operator
+# on the same plan reuses them through the checked fill instead of a new analysis
+_DIA_OFFSETS: dict = {}
+
+
+def _pattern_key(A: CSR):
+    return (A.rowptr.data_ptr(), A.col.data_ptr(), A.shape[0], A.nnz)
+
+
 class SolverOperator:
     def __init__(self, A: CSR, fmt: str = "auto", lanes: Optional[int] = None, fp32: bool = True):
         self.A = A
@@ -34,16 +43,23 @@ class SolverOperator:
         fmt = fmt.lower()
         if fmt not in ("auto", "dia", "csr", "bsr3"):
             raise ValueError("format must be 'auto', 'dia', 'bsr3' or 'csr'")
+        self.ndiag = -1
+        hit = _DIA_OFFSETS.get(_pattern_key(A)) if fmt in ("auto", "dia") else None
+        # under "auto", only offsets that the format choice itself picked are reused
+        if hit is not None and (fmt == "dia" or hit[2]) and self._build_dia_checked(hit[0], hit[1], fp32):
+            self.format = "dia"
+            return
+        chosen = fmt == "auto"
         if fmt == "auto":
             from .autotune import choose_format
 
             fmt, lanes = choose_format(A)
-        self.ndiag = -1
         if fmt == "dia":
             nd, offs = self.analyze(A)
             if nd < 0:
                 raise ValueError("this operator has no symmetric DIA layout (too many diagonals or not symmetric)")
             self._build_dia(nd, offs, fp32)
+            _DIA_OFFSETS[_pattern_key(A)] = (nd, offs, chosen)
         elif fmt == "bsr3":
             nb = self.bsr3_blocks(A)
             if nb < 0:
@@ -100,6 +116,23 @@ class SolverOperator:
         self.lanes = lanes
         self.nnz_stored = 9 * nb
 
+    def _build_dia_checked(self, nd, offs, fp32) -> bool:
+        """DIA layout with earlier offsets, verified on the device to reproduce A; False (nothing kept)
+        when it does not."""
+        n, A = self.n, self.A
+        ld = (n + 63) // 64 * 64
+        d64 = torch.empty(nd * ld, dtype=torch.float64, device=self.device)
+        d32 = torch.empty(nd * ld, dtype=torch.float32, device=self.device) if fp32 else None
+        ws = torch.empty(256, dtype=torch.uint8, device=self.device)
+        offs_c = (C.c_int32 * _lib.JNO_DIA_MAX)(*offs)
+        ok = C.c_int(0)
+        _lib.call("jno_op_fill_dia_checked", n, _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), nd, offs_c,
+                  _lib.ptr(d64), _lib.ptr(d32), C.byref(ok), _lib.ptr(ws), 256, _lib.stream_ptr(self.device))
+        if not ok.value:
+            return False
+        self._set_dia(nd, offs, d64, d32)
+        return True
+
     def _build_dia(self, nd, offs, fp32):
         n, A = self.n, self.A
         ld = (n + 63) // 64 * 64  # diagonals padded to 64 rows (aligned vector loads)
@@ -108,6 +141,10 @@ class SolverOperator:
         offs_c = (C.c_int32 * _lib.JNO_DIA_MAX)(*offs)
         _lib.call("jno_op_fill_dia", n, _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(A.vals), nd, offs_c, _lib.ptr(d64),
                   _lib.ptr(d32), _lib.stream_ptr(self.device))
+        self._set_dia(nd, offs, d64, d32)
+
+    def _set_dia(self, nd, offs, d64, d32):
+        n, A = self.n, self.A
         self._keep += [d64, d32]
         op = self.op
         op.format = _lib.JNO_FMT_DIA

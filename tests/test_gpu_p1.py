@@ -259,6 +259,54 @@ def test_operator_layouts_agree(dev):
         assert _rel(_np(op.mv(x.float())).astype(np.float64), ref) < 1e-6
 
 
+def test_dia_offsets_reused_with_checked_fill(dev, monkeypatch):
+    """A re-assembly on the same plan reuses the DIA offsets without a new analysis; values that the
+    cached layout cannot reproduce (an asymmetric entry, an entry off the stored diagonals) fail the
+    device check and the full analysis runs."""
+    from jno_amd.fem import CSR
+    from jno_amd.operator import SolverOperator
+
+    n = 9
+    m, s, A, b = _system(n, O.k_lognormal(O.kuhn_box(n)[1].shape[0], 3))
+    op1 = SolverOperator(A, fmt="dia")
+    A2, _ = s.evaluate({"k": torch.tensor(O.k_lognormal(m.n_cells, 4), device="cuda")})
+    assert A2.rowptr.data_ptr() == A.rowptr.data_ptr() and A2.col.data_ptr() == A.col.data_ptr()
+    calls = []
+    analyze = SolverOperator.analyze
+    monkeypatch.setattr(SolverOperator, "analyze", staticmethod(lambda M: calls.append(1) or analyze(M)))
+    op2 = SolverOperator(A2, fmt="dia")
+    assert not calls and op2.format == "dia" and op2.offsets == op1.offsets
+    x = torch.randn(A.shape[0], dtype=torch.float64, device="cuda")
+    assert _rel(_np(op2.mv(x)), _np(A2.mv(x))) < 1e-14
+    assert _rel(_np(op2.mv(x.float())).astype(np.float64), _np(A2.mv(x))) < 1e-6
+
+    rp, col, v = _np(A2.rowptr), _np(A2.col), _np(A2.vals).copy()
+    rows = np.repeat(np.arange(A2.shape[0]), np.diff(rp))
+
+    def mirror(j):
+        r, c = rows[j], col[j]
+        return rp[c] + int(np.nonzero(col[rp[c]:rp[c + 1]] == r)[0][0])
+
+    # an asymmetric lower entry: no symmetric layout, the format choice falls back to CSR
+    j = int(np.nonzero((col < rows) & (v != 0.0))[0][0])
+    va = v.copy()
+    va[j] *= 1.5
+    A3 = CSR(A2.rowptr, A2.col, torch.tensor(va, device="cuda"), A2.shape)
+    op3 = SolverOperator(A3, fmt="auto")
+    assert calls and op3.format == "csr"
+    assert _rel(_np(op3.mv(x)), _np(A3.mv(x))) < 1e-14
+    # a symmetric pair on a structural-zero diagonal: DIA applies again, with more diagonals
+    calls.clear()
+    off_set = (col > rows) & (v == 0.0) & ~np.isin(col - rows, op1.offsets)
+    j = int(np.nonzero(off_set)[0][0])
+    vz = v.copy()
+    vz[j] = vz[mirror(j)] = -0.125
+    A4 = CSR(A2.rowptr, A2.col, torch.tensor(vz, device="cuda"), A2.shape)
+    op4 = SolverOperator(A4, fmt="dia")
+    assert calls and op4.ndiag == 5
+    assert _rel(_np(op4.mv(x)), _np(A4.mv(x))) < 1e-14
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_iteration_modes_and_launch_shapes(dev, precision):
     """Every launch shape of the PCG iteration kernels solves to the oracle's direct solution:
