@@ -142,8 +142,12 @@ def main():
         spmv_mat = N * 4 * op.ndiag  # main + stored upper diagonals (the lower ones are the same values)
     else:
         spmv_mat = op.nnz_stored * (4 + 4) + 4 * (N + 1)
-    if mode == 4 and op.format == "dia":  # K1 K2 on the Jacobi-scaled operator: no dinv, unit diagonal
-        kernels = [("pcg_k1<dia, f32, scaled>", ms[1], N * 4 * (op.ndiag - 1) + N * (3 * 4 + 3 * 4)),
+    if mode == 5 and op.format == "dia":  # one kernel per iteration on the Jacobi-scaled operator
+        # reads r, q, p, x and the stored upper diagonals; writes r, p, q, x
+        kernels = [("pcg_one<dia, f32, scaled>", ms[1], N * 4 * (op.ndiag - 1) + N * (4 * 4 + 4 * 4))]
+    elif mode == 4 and op.format == "dia":  # K1 K2 on the Jacobi-scaled operator: no dinv, unit diagonal
+        k1 = "pcg_k1_lds<dia, f32, scaled>" if vec == 3 else "pcg_k1<dia, f32, scaled>"  # vec 3: p staged in LDS
+        kernels = [(k1, ms[1], N * 4 * (op.ndiag - 1) + N * (3 * 4 + 3 * 4)),
                    ("pcg_k2<scaled>", ms[2], N * (2 * 4 + 4))]
     elif mode == 3 and op.format == "dia":  # K0 KS K2 on the Jacobi-scaled operator: no dinv, unit diagonal
         kernels = [("pcg_k0<scaled>", ms[0], N * (3 * 4 + 2 * 4)),

@@ -211,11 +211,13 @@ int jno_pcg_set_chunk(int chunk);
 /* Launch shape of the iteration kernels, per iteration precision (no reference counterpart; set
  * by the autotuner; process-wide): precision = 0 for the FP64 iterations of jno_pcg_f64, 1 for
  * the FP32 inner iterations of jno_pcg_mixed; grid_cap = maximum workgroups (multiple of 8,
- * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for rows per thread;
+ * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for rows per thread,
+ * 2 for four rows per thread, 3 for p staged in LDS (mode 4 only; elsewhere it is 0);
  * block = threads per workgroup (256/512/1024); split = 1 for three kernels per iteration (p
  * materialized, then a plain SpMV), 0 for two (p recomputed inside the SpMV kernel), 2 for the
- * single-reduction iteration, 3 for the split iteration on the Jacobi-scaled FP32 operator (FP32
- * inner iterations on DIA operators with room for it in the workspace; elsewhere it is mode 1).
+ * single-reduction iteration, 3, 4 and 5 for the split, the two-kernel and the one-kernel
+ * iteration on the Jacobi-scaled FP32 operator (FP32 inner iterations on DIA operators with room
+ * for it in the workspace; elsewhere they are modes 1, 0 and 0).
  * jno_pcg_get_tuning reads the current shape. */
 int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split);
 int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split);

@@ -66,8 +66,10 @@ CANDIDATE_SHAPES = tuple(
     (blk, cap, vec, split)
     for blk in (256, 512, 1024)
     for cap in (256, 512, 1024)
-    for vec in (0, 1)
-    for split in (0, 1, 3, 4)
+    for vec in (0, 1, 3)
+    for split in (0, 1, 3, 4, 5)
+    # LDS staging (vec 3) is a variant of mode 4; the one-kernel mode 5 has a single kernel shape
+    if (vec != 3 or split == 4) and (split != 5 or vec == 0)
 )
 _LAUNCH_CACHE: dict = {}
 

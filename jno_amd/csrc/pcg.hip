@@ -35,16 +35,19 @@ constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup pa
 // Launch shape of the iteration kernels, one per iteration precision (set by the autotuner through
 // jno_pcg_set_tuning).  The defaults are among the fastest shapes of sweeps over workgroup size x
 // grid cap x SpMV tile variant x iteration mode on a 2.1M-row 4-diagonal operator (128^3 Kuhn P1):
-// 1024 threads x 512 workgroups, two rows per thread in the SpMV kernel; FP64 iterations with p
-// materialized (K0 KS K2), FP32 inner iterations as K1 K2 on the Jacobi-scaled operator (mode 4).
+// FP64 iterations with p materialized (K0 KS K2), 1024 threads x 512 workgroups, two rows per
+// thread in the SpMV kernel; FP32 inner iterations as one kernel per iteration on the Jacobi-scaled
+// operator (mode 5), 512 threads x 512 workgroups (two per CU).
 struct Tuning {
   int grid_cap;  // workgroups of the iteration kernels (a multiple of 8)
-  int vec;       // DIA SpMV kernel: 0 two rows per thread, 1 VN-row 16-byte tiles, 2 four rows
+  int vec;       // DIA SpMV kernel: 0 two rows per thread, 1 VN-row 16-byte tiles, 2 four rows,
+                 // 3 p staged in LDS (mode 4 K1; the others take 0)
   int block;     // threads per workgroup (256, 512, 1024)
   int split;     // iteration mode: 0 K1 K2, 1 K0 KS K2 (p materialized), 2 KA KB (single reduction),
-                 // 3 K0 KS K2 and 4 K1 K2 on the Jacobi-scaled operator (FP32 inner iterations on DIA)
+                 // 3 K0 KS K2, 4 K1 K2 and 5 one kernel on the Jacobi-scaled operator (FP32 inner
+                 // iterations on DIA)
 };
-static Tuning g_tune[2] = {{512, 0, 1024, 1}, {512, 0, 1024, 4}};  // [0] FP64 iterations, [1] FP32 iterations
+static Tuning g_tune[2] = {{512, 0, 1024, 1}, {512, 0, 512, 5}};  // [0] FP64 iterations, [1] FP32 iterations
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -71,6 +74,12 @@ struct PcgState {
   int64_t k_next;
   double gamma_next;
   double alpha_next;
+  // one-kernel iterations: the launch with index k_special starts (special = 1) or resumes after a
+  // residual replacement (2) with alpha = 0; g_hist[k & 1] = the exact r.r read by launch k
+  int64_t k_special;
+  int32_t special;
+  int32_t pad2;
+  double g_hist[2];
 };
 
 // partial-sum areas (per workgroup): A = K1's p.q; B = K2's (r.z, r.r); P = prep's (rr_true, b.b)
@@ -79,6 +88,7 @@ struct Partials {
   double* B;   // [2][G]
   double* P;   // [2][G]
   double* B2;  // [2][G]: single-reduction iterations read B and write B2 on even k, and swap on odd k
+  double* C;   // [2][4][G]: one-kernel iterations, (r.r, p.q, q.r, q.q) by launch parity
 };
 
 template <typename T, typename V>
@@ -102,6 +112,11 @@ struct PcgArgs {
   T* x;
   T* s;  // single-reduction iterations: s = A p
   T* u;  // single-reduction iterations: u = dinv r
+  // one-kernel iterations: launch index and the other halves of the r and q pairs (r and q are read,
+  // r_out and q_out written; the halves swap every launch)
+  int64_t kk;
+  T* r_out;
+  T* q_out;
   PcgState* st;
   Partials part;
 };
@@ -439,6 +454,276 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
   store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// K1 on the Jacobi-scaled DIA operator with p staged in LDS (k1_vector = 3).  The workgroup's rows
+// go in tiles [t0, t1): first p_new = r + beta p_old is formed once per row for the tile and a halo
+// of h rows on either side (h = the largest offset <= LDS_HALO) into LDS, the own rows also to
+// memory with the deferred x update; after one barrier q = A p_new takes the near neighbours
+// (offsets <= h) from LDS and recomputes only the far ones from r and p_old.  Per row that is about
+// 13 global loads instead of 21, for the same bytes from HBM.
+// ---------------------------------------------------------------------------------------------
+constexpr int LDS_HALO = 256;
+constexpr int LDS_STAGE_BYTES = 32768;
+
+template <int NT, int ND, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
+  constexpr int TT = LDS_STAGE_BYTES / (int)sizeof(T) - 2 * LDS_HALO;  // rows per tile
+  constexpr int U1 = 4;                                                 // staged rows per thread in flight
+  constexpr int U2 = 2;                                                 // SpMV rows per thread in flight
+  __shared__ double sm[4];
+  __shared__ double sm_out[2 * (NT / 64)];
+  __shared__ T pt[TT + 2 * LDS_HALO];
+  PcgState* st = a.st;
+  if (st->done) return;
+  double gamma;
+  T beta, ap;
+  int64_t k, r0l, r1l;
+  if (!iter_head<NT, T, V>(a, sm, gamma, beta, ap, k, r0l, r1l)) return;
+  // row indices fit 32 bits (CSR columns are int32): 32-bit offsets keep the addresses in one
+  // register each
+  const bool upd_x = k > 0;
+  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)r1l;
+  const V* vals = a.vals;
+  const int ld = (int)dia_ld(a.n);
+  int o[ND];
+  int h = 0;
+#pragma unroll
+  for (int d = 1; d < ND; ++d) {
+    o[d] = a.off.o[d];
+    if (o[d] <= LDS_HALO) h = max(h, o[d]);
+  }
+  double part = 0.0;
+  for (int t0 = r0; t0 < r1; t0 += TT) {
+    const int t1 = min(r1, t0 + TT), s0 = t0 - h, s1 = t1 + h;
+    for (int j0 = s0 + (int)threadIdx.x; j0 < s1; j0 += U1 * NT) {
+      T rv[U1], pv[U1], xv[U1];
+#pragma unroll
+      for (int u = 0; u < U1; ++u) {
+        const int j = j0 + u * NT;
+        const bool in = j < s1 && j >= 0 && j < n;
+        const bool own = in && j >= t0 && j < t1;
+        rv[u] = in ? a.r[j] : T(0);
+        pv[u] = in ? a.p_old[j] : T(0);
+        xv[u] = own && upd_x ? a.x[j] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U1; ++u) {
+        const int j = j0 + u * NT;
+        if (j < s1) {
+          const T pn = rv[u] + beta * pv[u];
+          pt[j - s0] = pn;
+          if (j >= t0 && j < t1) {
+            sto(&a.p_new[j], pn);
+            if (upd_x) sto(&a.x[j], xv[u] + ap * pv[u]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int i0 = t0 + (int)threadIdx.x; i0 < t1; i0 += U2 * NT) {
+      V vu[U2][ND], vl[U2][ND];
+      T xu[U2][ND], xl[U2][ND];
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int i = i0 + u * NT;
+        const bool ok = i < t1;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+          const V* vd = vals + d * ld;
+          vu[u][d] = ok ? vd[i] : V(0);
+          vl[u][d] = ok && i - o[d] >= 0 ? vd[i - o[d]] : V(0);
+          if (o[d] > h) {
+            const bool up = ok && i + o[d] < n, lo = ok && i - o[d] >= 0;
+            xu[u][d] = up ? a.r[i + o[d]] + beta * a.p_old[i + o[d]] : T(0);
+            xl[u][d] = lo ? a.r[i - o[d]] + beta * a.p_old[i - o[d]] : T(0);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int i = i0 + u * NT;
+        if (i < t1) {
+          const T pi = pt[i - s0];
+          T s = pi;
+#pragma unroll
+          for (int d = 1; d < ND; ++d) {
+            if (o[d] <= h) {
+              xu[u][d] = pt[i + o[d] - s0];
+              xl[u][d] = pt[i - o[d] - s0];
+            }
+            s += (T)vu[u][d] * xu[u][d];
+            s += (T)vl[u][d] * xl[u][d];
+          }
+          sto(&a.q[i], s);
+          part += (double)pi * (double)s;
+        }
+      }
+    }
+    if (t1 < r1) __syncthreads();  // the next tile rewrites pt
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->gamma_cur = gamma;
+  store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// One kernel per iteration on the Jacobi-scaled DIA operator (iteration mode 5).  Launch k reads
+// r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1} and the partial sums of launch k-1: gamma_{k-1} = r.r
+// (exact), p.q, q.r, q.q.  With alpha = gamma_{k-1} / p.q the next residual norm follows without
+// another reduction, gamma_k = gamma_{k-1} - 2 alpha q.r + alpha^2 q.q (from the exact
+// gamma_{k-1}, so its rounding does not accumulate over iterations), and beta = gamma_k /
+// gamma_{k-1}.  Then, per row: r_k = r_{k-1} - alpha q_{k-1}, x += alpha p_{k-1},
+// p_k = r_k + beta p_{k-1} (staged in LDS as in pcg_k1_lds, far neighbours recomputed from r, q and
+// p), q_k = A p_k, and the partials of r_k.r_k, p_k.q_k, q_k.r_k, q_k.q_k.  r, q and p each
+// alternate between two buffers.  A stop leaves x_{k-1} and r_{k-1} consistent (no deferred
+// update).  After a residual replacement the driver reruns launch k-1 with the true residual and
+// alpha = 0, beta = gamma_true / gamma_{k-2} on p_{k-2}: the search direction survives the
+// replacement.  One launch per iteration moves 44 instead of 48 bytes per row and pays one launch
+// and one reduction instead of two.
+// ---------------------------------------------------------------------------------------------
+constexpr int ONE_STAGE_BYTES = 65536;
+
+template <int NT, int ND, typename T, typename V>
+__global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
+  constexpr int TT = ONE_STAGE_BYTES / (2 * (int)sizeof(T)) - LDS_HALO;  // rows per tile
+  constexpr int U1 = 4;
+  constexpr int U2 = 2;
+  __shared__ double sm[4];
+  __shared__ double sm_out[4 * (NT / 64)];
+  __shared__ T pt[TT + 2 * LDS_HALO];  // p_k of the tile and its halo
+  __shared__ T rt[TT];                 // r_k of the tile
+  PcgState* st = a.st;
+  if (st->done) return;
+  const int G = gridDim.x;
+  const int64_t k = a.kk;
+  double in[4];
+  reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
+  const double rr = in[0];
+  const int special = k == st->k_special ? st->special : 0;
+  double alpha = 0.0, beta = 0.0;
+  if (special == 2) {
+    beta = rr / st->g_hist[k & 1];
+  } else if (special == 0) {
+    alpha = rr / in[1];
+    beta = (rr - 2.0 * alpha * in[2] + alpha * alpha * in[3]) / rr;
+  }
+  const bool stop_conv = special != 2 && !(rr > st->atol2);
+  const bool stop_cap = k >= st->maxiter;
+  const bool stop_repl = special == 0 && rr <= st->rr_replace;
+  const bool bad = !isfinite(rr) || (!stop_conv && (!isfinite(alpha) || !isfinite(beta)));
+  if (stop_conv || stop_cap || stop_repl || bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->k_stop = k;
+      st->rr_stop = rr;
+      st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
+      st->done1 = 1;
+      st->done = 1;  // every workgroup of this launch takes the same decision
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && special != 2) st->g_hist[k & 1] = rr;
+  const T al = (T)alpha, be = (T)beta;
+  const bool upd = special == 0;  // alpha != 0: r and x updates, q read
+  const int64_t r0l = block_first_row(a.rows_per_block);
+  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + a.rows_per_block);
+  const V* vals = a.vals;
+  const int ld = (int)dia_ld(a.n);
+  int o[ND > 1 ? ND : 2];
+  int h = 0;
+#pragma unroll
+  for (int d = 1; d < ND; ++d) {
+    o[d] = a.off.o[d];
+    if (o[d] <= LDS_HALO) h = max(h, o[d]);
+  }
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t0 = r0; t0 < r1; t0 += TT) {
+    const int t1 = min(r1, t0 + TT), s0 = t0 - h, s1 = t1 + h;
+    for (int j0 = s0 + (int)threadIdx.x; j0 < s1; j0 += U1 * NT) {
+      T rv[U1], qv[U1], pv[U1], xv[U1];
+#pragma unroll
+      for (int u = 0; u < U1; ++u) {
+        const int j = j0 + u * NT;
+        const bool in = j < s1 && j >= 0 && j < n;
+        const bool own = in && j >= t0 && j < t1;
+        rv[u] = in ? a.r[j] : T(0);
+        qv[u] = in && upd ? a.q[j] : T(0);
+        pv[u] = in ? a.p_old[j] : T(0);
+        xv[u] = own && upd ? a.x[j] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U1; ++u) {
+        const int j = j0 + u * NT;
+        if (j < s1) {
+          const T rn = rv[u] - al * qv[u];
+          const T pn = rn + be * pv[u];
+          pt[j - s0] = pn;
+          if (j >= t0 && j < t1) {
+            rt[j - t0] = rn;
+            sto(&a.r_out[j], rn);
+            sto(&a.p_new[j], pn);
+            if (upd) sto(&a.x[j], xv[u] + al * pv[u]);
+            part[0] += (double)rn * (double)rn;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int i0 = t0 + (int)threadIdx.x; i0 < t1; i0 += U2 * NT) {
+      V vu[U2][ND > 1 ? ND : 2], vl[U2][ND > 1 ? ND : 2];
+      T xu[U2][ND > 1 ? ND : 2], xl[U2][ND > 1 ? ND : 2];
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int i = i0 + u * NT;
+        const bool ok = i < t1;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+          const V* vd = vals + d * ld;
+          vu[u][d] = ok ? vd[i] : V(0);
+          vl[u][d] = ok && i - o[d] >= 0 ? vd[i - o[d]] : V(0);
+          if (o[d] > h) {
+            const bool up = ok && i + o[d] < n, lo = ok && i - o[d] >= 0;
+            const T ru = up ? a.r[i + o[d]] : T(0), pu = up ? a.p_old[i + o[d]] : T(0);
+            const T qu = up && upd ? a.q[i + o[d]] : T(0);
+            const T rl = lo ? a.r[i - o[d]] : T(0), pl = lo ? a.p_old[i - o[d]] : T(0);
+            const T ql = lo && upd ? a.q[i - o[d]] : T(0);
+            xu[u][d] = (ru - al * qu) + be * pu;
+            xl[u][d] = (rl - al * ql) + be * pl;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int i = i0 + u * NT;
+        if (i < t1) {
+          const T pi = pt[i - s0];
+          T s = pi;
+#pragma unroll
+          for (int d = 1; d < ND; ++d) {
+            if (o[d] <= h) {
+              xu[u][d] = pt[i + o[d] - s0];
+              xl[u][d] = pt[i - o[d] - s0];
+            }
+            s += (T)vu[u][d] * xu[u][d];
+            s += (T)vl[u][d] * xl[u][d];
+          }
+          sto(&a.q_out[i], s);
+          part[1] += (double)pi * (double)s;
+          part[2] += (double)s * (double)rt[i - t0];
+          part[3] += (double)s * (double)s;
+        }
+      }
+    }
+    if (t1 < r1) __syncthreads();  // the next tile rewrites pt and rt
+  }
+  store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+}
+
+__global__ void set_special_k(PcgState* st, int64_t k, int kind) {
+  if (threadIdx.x == 0) {
+    st->k_special = k;
+    st->special = kind;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1004,13 +1289,14 @@ __global__ void scale_dia_k(int64_t n, int ndiag, DiaOffsets off, const double* 
   }
 }
 
-// r64 = b - Ax (Ax nullable = 0), rs = (float)(sqrt(dinv) r64); partials of rs.rs (both B slots)
-// and of ||r64||^2, b.b (P)
+// r64 = b - Ax (Ax nullable = 0), rs = (float)(sqrt(dinv) r64); partials of rs.rs (both B slots,
+// and c_rr when given: the r.r slot a one-kernel launch reads) and of ||r64||^2, b.b (P)
 __global__ void __launch_bounds__(PCG_NT) residual_prep_scaled_k(int64_t n, int64_t rows_per_block,
                                                                  const double* __restrict__ b,
                                                                  const double* __restrict__ Ax,
                                                                  const double* __restrict__ dinv,
-                                                                 float* __restrict__ rs, Partials part) {
+                                                                 float* __restrict__ rs, Partials part,
+                                                                 double* __restrict__ c_rr) {
   __shared__ double sm[PCG_NT / 64];
   const int G = gridDim.x;
   const int64_t r0 = block_first_row(rows_per_block);
@@ -1029,6 +1315,7 @@ __global__ void __launch_bounds__(PCG_NT) residual_prep_scaled_k(int64_t n, int6
   store_partial(pr, part.B + G, sm);
   store_partial(prt, part.P, sm);
   store_partial(pb, part.P + G, sm);
+  if (c_rr) store_partial(pr, c_rr, sm);
 }
 
 // x64 += sqrt(dinv) xs, xs = 0
@@ -1058,6 +1345,7 @@ struct Ws {
   void* q;
   void* s;       // single-reduction iterations: A p
   void* xs;      // FP32 accumulator (mixed)
+  void* q2;      // one-kernel iterations: the second q buffer (scaled FP32 operator only)
   float* dinv32; // mixed
   double* ax;    // FP64 A x
   float* sdia;   // mixed on DIA: the Jacobi-scaled FP32 operator (nullable: no room)
@@ -1078,6 +1366,7 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_el
   t.part.B = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.part.P = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.part.B2 = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
+  t.part.C = (double*)take(8 * PCG_MAX_BLOCKS * sizeof(double));
   t.r = take(n * vb);
   t.p0 = take(n * vb);
   t.p1 = take(n * vb);
@@ -1087,6 +1376,7 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_el
   t.dinv32 = mixed ? (float*)take(n * 4) : nullptr;
   t.ax = (double*)take(n * 8);
   t.sdia = sdia_elems > 0 ? (float*)take(sdia_elems * 4) : nullptr;
+  t.q2 = sdia_elems > 0 ? take(n * 4) : nullptr;
   if (w) *w = t;
   return off;
 }
@@ -1130,9 +1420,10 @@ static thread_local HostPoll g_poll;
 
 // Iteration modes (Tuning::split): two-reduction CG as K1 K2 (p recomputed inside the SpMV
 // kernel) or K0 KS K2 (p materialized), or single-reduction CG as KA KB.
-// MODE_SCALED / MODE_SCALED_FUSED: the split / fused iteration on the Jacobi-scaled FP32 operator
-// (FP32 inner iterations on a DIA operator; the unscaled split / fused iteration elsewhere)
-enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SCALED_FUSED = 4 };
+// MODE_SCALED / MODE_SCALED_FUSED / MODE_ONE: the split / fused / one-kernel iteration on the
+// Jacobi-scaled FP32 operator (FP32 inner iterations on a DIA operator; the unscaled split / fused /
+// fused iteration elsewhere)
+enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SCALED_FUSED = 4, MODE_ONE = 5 };
 
 // The SpMV-carrying kernel of one iteration: K1 (MODE_FUSED), KS (MODE_SPLIT) or KB
 // (MODE_SINGLE), by format, DIA diagonal count (4: the 3-D Kuhn lattice; 3: the 2-D triangle
@@ -1212,12 +1503,37 @@ static int launch_kb(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStre
 template <int NT, typename T, typename V>
 static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
   int rc;
+  if (a.unit && a.split == MODE_ONE) {
+    if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    switch (op->ndiag) {
+      case 4: pcg_one<NT, 4, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 3: pcg_one<NT, 3, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 2: pcg_one<NT, 2, T, V><<<grid, NT, 0, s>>>(a); break;
+      case 1: pcg_one<NT, 1, T, V><<<grid, NT, 0, s>>>(a); break;
+      default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: the one-kernel iteration takes 1..4 stored diagonals");
+    }
+    JNO_LAUNCH_CHECK();
+    if (ev) JNO_HIP(hipEventRecord(ev[1], s));
+    if (ev) JNO_HIP(hipEventRecord(ev[2], s));
+    return JNO_OK;
+  }
   if (a.unit && a.split == MODE_FUSED) {
     // Jacobi-scaled DIA operator, two kernels: K1 and K2 without dinv, unit main diagonal
     if (ev) JNO_HIP(hipEventRecord(ev[0], s));
+    const bool lds = a.vec == 3;
     switch (op->ndiag) {
-      case 4: pcg_k1<NT, 1, true, 4, T, V, false><<<grid, NT, 0, s>>>(a); break;
-      case 3: pcg_k1<NT, 1, true, 3, T, V, false><<<grid, NT, 0, s>>>(a); break;
+      case 4:
+        if (lds)
+          pcg_k1_lds<NT, 4, T, V><<<grid, NT, 0, s>>>(a);
+        else
+          pcg_k1<NT, 1, true, 4, T, V, false><<<grid, NT, 0, s>>>(a);
+        break;
+      case 3:
+        if (lds)
+          pcg_k1_lds<NT, 3, T, V><<<grid, NT, 0, s>>>(a);
+        else
+          pcg_k1<NT, 1, true, 3, T, V, false><<<grid, NT, 0, s>>>(a);
+        break;
       default: pcg_k1<NT, 1, true, 0, T, V, false><<<grid, NT, 0, s>>>(a); break;
     }
     JNO_LAUNCH_CHECK();
@@ -1306,7 +1622,7 @@ static KernelProf g_prof;
 // Runs K1/K2 chunks until the device reports a stop; returns the final state in *out.
 template <typename T, typename V>
 static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid, hipStream_t s,
-                     PcgState* out) {
+                     PcgState* out, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr) {
   JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
   int64_t k = k0;
   int slot = 0, pending = 0;
@@ -1316,6 +1632,13 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
     for (int i = 0; i < chunk; ++i, ++k) {
       a.p_old = pbuf[k & 1];
       a.p_new = pbuf[(k + 1) & 1];
+      if (rbuf) {  // one-kernel iterations: launch index and the r and q halves
+        a.kk = k;
+        a.r = rbuf[k & 1];
+        a.r_out = rbuf[(k + 1) & 1];
+        a.q = qbuf[k & 1];
+        a.q_out = qbuf[(k + 1) & 1];
+      }
       if (g_prof.on && k % g_prof.stride == 0) {
         if (g_prof.ensure(nev + 4) != JNO_OK) return JNO_EHIP;
         JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
@@ -1404,7 +1727,9 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.block = tu.block;
   // the fused K1 recomputes p for gathered neighbours; BSR3 only has the plain SpMV kernel.  The
   // driver sets `unit` when it runs the scaled operator; otherwise MODE_SCALED is the split mode.
-  int mode = tu.split == MODE_SCALED ? MODE_SPLIT : tu.split == MODE_SCALED_FUSED ? MODE_FUSED : tu.split;
+  int mode = tu.split == MODE_SCALED                                 ? MODE_SPLIT
+             : (tu.split == MODE_SCALED_FUSED || tu.split == MODE_ONE) ? MODE_FUSED
+                                                                       : tu.split;
   if (op->format == JNO_FMT_BSR3 && mode == MODE_FUSED) mode = MODE_SPLIT;
   a.split = mode;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
@@ -1419,6 +1744,9 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.unit = 0;
   a.s = (T*)w.s;
   a.u = (T*)w.p1;  // single-reduction iterations keep p in place in p0; p1 holds u
+  a.kk = 0;
+  a.r_out = nullptr;
+  a.q_out = nullptr;
   return a;
 }
 
@@ -1456,11 +1784,13 @@ extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, in
               "jno_pcg_set_tuning: grid_cap must be a multiple of 8 in 8..%d", PCG_MAX_BLOCKS);
   JNO_REQUIRE(block == 256 || block == 512 || block == 1024, JNO_EINVAL,
               "jno_pcg_set_tuning: block must be 256, 512 or 1024");
-  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_SCALED_FUSED, JNO_EINVAL,
+  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_ONE, JNO_EINVAL,
               "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2), 2 (single reduction, KA KB), 3 (K0 KS K2 "
-              "on the Jacobi-scaled operator) or 4 (K1 K2 on the Jacobi-scaled operator)");
-  JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 2, JNO_EINVAL,
-              "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles) or 2 (4 rows)");
+              "on the Jacobi-scaled operator), 4 (K1 K2 on the Jacobi-scaled operator) or 5 (one kernel per "
+              "iteration on the Jacobi-scaled operator)");
+  JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 3, JNO_EINVAL,
+              "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles), 2 (4 rows) or 3 "
+              "(p staged in LDS, iteration mode 4)");
   g_tune[precision] = Tuning{grid_cap, k1_vector, block, split};
   return JNO_OK;
 }
@@ -1570,12 +1900,21 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   JNO_HIP(hipStreamSynchronize(s));
   if (h_bad) return JNO_OK;
   *used = true;
+  // one kernel per iteration: r, q (and p) alternate between two buffers by launch parity
+  const bool one = tu.split == MODE_ONE && h_op->ndiag <= 4 && w.q2;
   float* rs = (float*)w.r;
   float* pb[2] = {(float*)w.p0, (float*)w.p1};
+  float* rb[2] = {rs, (float*)w.s};
+  float* qb[2] = {(float*)w.q, (float*)w.q2};
   float* xs = (float*)w.xs;
   zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[0], n);
   zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[1], n);
   zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
+  if (one) {
+    zero_k<float><<<gz, PCG_NT, 0, s>>>(rb[1], n);
+    zero_k<float><<<gz, PCG_NT, 0, s>>>(qb[0], n);
+    zero_k<float><<<gz, PCG_NT, 0, s>>>(qb[1], n);
+  }
   int rc;
   if (use_x0) {
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
@@ -1583,26 +1922,37 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   } else {
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
   }
-  residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, rs, w.part);
+  residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, rs, w.part,
+                                               one ? w.part.C : nullptr);
   // inner tolerance 0: the FP64 residual below decides convergence
   finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 1, 1, 0);
   JNO_LAUNCH_CHECK();
+  if (one) {
+    set_special_k<<<1, 64, 0, s>>>(w.st, 0, 1);
+    JNO_LAUNCH_CHECK();
+  }
   PcgArgs<float, float> a = make_args<float, float>(h_op, w.sdia, nullptr, rs, (float*)w.q, xs, w, rpb, tu);
   a.unit = 1;
-  a.split = tu.split == MODE_SCALED_FUSED ? MODE_FUSED : MODE_SPLIT;
-  int64_t k = 0, refinements = 0;
+  a.split = one ? MODE_ONE : tu.split == MODE_SCALED_FUSED || tu.split == MODE_ONE ? MODE_FUSED : MODE_SPLIT;
+  int64_t k = 0, refinements = 0, resumes = 0;
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;
   for (;;) {
-    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f);
+    const int64_t k_start = k;
+    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr);
     if (rc) return rc;
     k = f.k_stop;
     accumulate_scaled_k<<<gz, PCG_NT, 0, s>>>(n, d_x, xs, d_dinv);
     JNO_LAUNCH_CHECK();
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
-    residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, rs, w.part);
-    finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 0, 1, 0);
+    // one-kernel iterations resume by rerunning launch k-1 (from p_{k-2}, which launch k did not
+    // overwrite): its r input half takes the true residual, its r.r slot the new partials
+    const int64_t kr = k > 0 ? k - 1 : 0;
+    residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, one ? rb[kr & 1] : rs, w.part,
+                                                 one ? w.part.C + (kr & 1) * 4 * (int64_t)G : nullptr);
+    finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, one ? maxiter - resumes - 1 : maxiter, repl2, 0, 1,
+                                    0);
     JNO_LAUNCH_CHECK();
     rc = read_state(w.st, &e, s);
     if (rc) return rc;
@@ -1618,12 +1968,26 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
       if (!std::isfinite(rr_true)) f.status = ST_NONFINITE;
       break;
     }
+    if (f.k_stop == k_start && f.status == ST_CONVERGED) {
+      // the FP32 residual is exactly zero while the FP64 one is not below the tolerance: no
+      // further progress is possible
+      f.status = ST_MAXITER;
+      break;
+    }
     const double need = atol2 / rr_true;  // remaining reduction of r.r
     if (need > repl2) {
       set_replace_k<<<1, 64, 0, s>>>(w.st, 0.5 * need * e.rr0);
       JNO_LAUNCH_CHECK();
     }
+    if (one) {
+      // a stop at launch 0 (nothing to resume from) starts over
+      set_special_k<<<1, 64, 0, s>>>(w.st, kr, k > 0 ? 2 : 1);
+      JNO_LAUNCH_CHECK();
+      if (k > 0) ++resumes;
+      k = kr;
+    }
   }
+  k += resumes;  // launches that did work
   const auto t1 = std::chrono::steady_clock::now();
   const double bn = sqrt(bb);
   if (h_stats) {
@@ -1656,7 +2020,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   int G;
   int64_t rpb;
   pcg_grid(h_op, tu, &G, &rpb);
-  if (room && (tu.split == MODE_SCALED || tu.split == MODE_SCALED_FUSED)) {
+  if (room && (tu.split == MODE_SCALED || tu.split == MODE_SCALED_FUSED || tu.split == MODE_ONE)) {
     bool used = false;
     rc = mixed_scaled(h_op, d_dinv, d_b, d_x, use_x0, tol, atol, maxiter, replace_factor * replace_factor, h_stats,
                       w, G, rpb, tu, s, &used);

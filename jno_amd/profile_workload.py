@@ -1,6 +1,7 @@
 """A bounded, repeatable workload for profiler runs (rocprofv3 kernel traces and PMC counters).
 
     python -m jno_amd.profile_workload --case c2 --iters 40 [--precision mixed|fp64] [--copy-mib 1024]
+        [--tuning block,grid_cap,k1_vector,mode]
 
 Builds the configuration's system (128^3 P1 Poisson for c2, 64^3 for c3, 32^3 P2 elasticity for
 c5), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
@@ -56,6 +57,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=("mixed", "fp64"))
     ap.add_argument("--format", default="auto")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--tuning", default=None,
+                    help="launch shape block,grid_cap,k1_vector,mode of the iterations (default: the library's)")
     args = ap.parse_args()
     from jno_amd import precond
     from jno_amd.solve import LinearSolver
@@ -75,6 +78,11 @@ def main():
         print(f"c4 batch: {len(systems)} systems, {sum(A.shape[0] for A, _ in systems)} rows, "
               f"{(time.perf_counter() - t0) * 1e3:.1f} ms, iterations {min(its)}..{max(its)}")
         return
+    if args.tuning:
+        from jno_amd.solve import set_tuning
+
+        blk, cap, vec, mode = (int(v) for v in args.tuning.split(","))
+        set_tuning(args.precision, blk, cap, vec, mode)
     A, b = build(args.case, dev)
     # replace_factor tiny: no residual replacement inside the bounded run
     sv = LinearSolver("cg", 1e-30, 0.0, args.iters, precision=args.precision, format=args.format,

@@ -310,9 +310,9 @@ def test_dia_offsets_reused_with_checked_fill(dev, monkeypatch):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_iteration_modes_and_launch_shapes(dev, precision):
     """Every launch shape of the PCG iteration kernels solves to the oracle's direct solution:
-    iteration mode K1 K2 / K0 KS K2 / KA KB (single reduction) / K0 KS K2 on the Jacobi-scaled
-    FP32 operator, SpMV variant (2 rows per thread, 16-byte tiles, 4 rows per thread), several
-    workgroup sizes and grid caps, DIA and CSR."""
+    iteration mode K1 K2 / K0 KS K2 / KA KB (single reduction) / K0 KS K2, K1 K2 and one kernel on
+    the Jacobi-scaled FP32 operator, SpMV variant (2 rows per thread, 16-byte tiles, 4 rows per thread,
+    p staged in LDS), several workgroup sizes and grid caps, DIA and CSR."""
     from jno_amd import precond, solve
 
     n = 11  # non-dyadic, ragged row ranges per workgroup
@@ -324,8 +324,8 @@ def test_iteration_modes_and_launch_shapes(dev, precision):
     saved = solve.get_tuning(precision)
     try:
         for fmt in ("dia", "csr"):
-            for mode in (0, 1, 2, 3, 4):
-                for vec in (0, 1, 2):
+            for mode in (0, 1, 2, 3, 4, 5):
+                for vec in (0, 1, 2, 3):
                     for blk, cap in ((256, 8), (1024, 64), (512, 4096)):
                         solve.set_tuning(precision, blk, cap, vec, mode)
                         sv = solve.cg(tol=1e-10, precision=precision, format=fmt)
@@ -336,6 +336,40 @@ def test_iteration_modes_and_launch_shapes(dev, precision):
                         assert _rel(_np(u), uo) <= SOL_TOL, (fmt, mode, vec, blk, cap, st)
     finally:
         solve.set_tuning(precision, *saved)
+
+
+@pytest.mark.parametrize("n", [17, 40])
+def test_lds_staged_k1(dev, n):
+    """The LDS-staged kernels on the Jacobi-scaled operator (K1 of mode 4 with k1_vector 3, and the
+    one-kernel iteration, mode 5): offsets beyond the LDS halo (256 rows) recomputed from the
+    iteration vectors (n = 17: offset 324), several LDS tiles per workgroup (n = 40 with 8
+    workgroups: 8616 rows each), residual replacements (replace_factor 0.1 and 0.01).  Reference: an
+    FP64 solve at rtol 1e-13."""
+    from jno_amd import precond, solve
+
+    k = O.k_lognormal(6 * n**3, 5)
+    m, s, A, b = _system(n, k)
+    ref = _np(solve.cg(tol=1e-13, precision="fp64")(A, b, M=precond.jacobi()))
+    saved = solve.get_tuning("mixed")
+    its = {}
+    try:
+        for vec, mode in ((3, 4), (0, 5)):
+            for blk, cap in ((1024, 8), (256, 64), (1024, 512)):
+                for rf in (0.1, 0.01):
+                    solve.set_tuning("mixed", blk, cap, vec, mode)
+                    sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=rf)
+                    u = sv(A, b, M=precond.jacobi())
+                    st = sv.last_stats
+                    assert st["relres"] <= 1e-10, (mode, blk, cap, rf, st)
+                    assert _rel(_np(u), ref) <= SOL_TOL, (mode, blk, cap, rf, st)
+                    assert st["refinements"] >= 2, st
+                    its.setdefault((mode, rf), []).append(st["iterations"])
+    finally:
+        solve.set_tuning("mixed", *saved)
+    # the one-kernel iteration keeps the search direction across replacements: about the iteration
+    # count of the two-kernel one (plus one rerun launch per replacement)
+    for rf in (0.1, 0.01):
+        assert max(its[(5, rf)]) <= 1.1 * min(its[(4, rf)]) + 12, its
 
 
 def test_adjoint_gradient_vs_oracle_and_fd(dev):

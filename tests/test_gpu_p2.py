@@ -152,7 +152,7 @@ def test_bsr3_solves_every_iteration_mode(dev, precision):
     uo = O.direct_solve(Ao, bo)
     saved = solve.get_tuning(precision)
     try:
-        for mode in (0, 1, 2, 3, 4):
+        for mode in (0, 1, 2, 3, 4, 5):
             for blk, cap in ((256, 8), (1024, 512)):
                 solve.set_tuning(precision, blk, cap, 0, mode)
                 sv = solve.cg(tol=1e-10, precision=precision, format="bsr3")
