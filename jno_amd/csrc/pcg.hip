@@ -22,6 +22,7 @@
 // XCDs round-robin) gets one contiguous slab: gathered neighbour rows then mostly hit its own L2.
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -457,21 +458,40 @@ __global__ void __launch_bounds__(NT) pcg_k1(PcgArgs<T, V> a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// K1 on the Jacobi-scaled DIA operator with p staged in LDS (k1_vector = 3).  The workgroup's rows
-// go in tiles [t0, t1): first p_new = r + beta p_old is formed once per row for the tile and a halo
-// of h rows on either side (h = the largest offset <= LDS_HALO) into LDS, the own rows also to
-// memory with the deferred x update; after one barrier q = A p_new takes the near neighbours
-// (offsets <= h) from LDS and recomputes only the far ones from r and p_old.  Per row that is about
-// 13 global loads instead of 21, for the same bytes from HBM.
+// LDS-staged kernels on the Jacobi-scaled DIA operator (unit main diagonal, z = r).  A workgroup's
+// rows go in tiles [t0, t1): p_new is formed once per row for the tile and a halo of h rows on
+// either side into LDS (h = the largest "near" offset), the own rows also to memory; after one
+// barrier q = A p_new takes the near neighbours from LDS and recomputes only the far ones (offsets
+// > LDS_HALO) from the iteration vectors.  NN = the number of near off-diagonals: the stored offsets
+// ascend, so diagonals 1..NN are near and the rest far, fixed at compile time.
+// Every global load is unconditional, from an index clamped to row 0 where it does not apply
+// (ldc): a batch's loads then issue back to back and one wait covers them all, where guarded loads
+// become branches and the compiler waits after each.  A value loaded that way is finite and is only
+// ever multiplied by a zero coefficient: the stored upper diagonal is 0 past the last row, and the
+// lower coefficient before the first row is selected to 0 (ldz), the one select that remains.
 // ---------------------------------------------------------------------------------------------
 constexpr int LDS_HALO = 256;
 constexpr int LDS_STAGE_BYTES = 32768;
 
-template <int NT, int ND, typename T, typename V>
+template <typename T>
+__device__ __forceinline__ T ldc(const T* __restrict__ p, int i, bool ok) {
+  return p[ok ? i : 0];
+}
+
+template <typename T>
+__device__ __forceinline__ T ldz(const T* __restrict__ p, int i, bool ok) {
+  const T v = p[ok ? i : 0];
+  return ok ? v : T(0);
+}
+
+// K1 with p staged in LDS (mode 4, k1_vector = 3): p_new = r + beta p_old, the deferred x update
+// and q = A p_new, partial p.q.  Per row about 13 global loads instead of 21, for the same bytes.
+template <int NT, int ND, int NN, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
   constexpr int TT = LDS_STAGE_BYTES / (int)sizeof(T) - 2 * LDS_HALO;  // rows per tile
   constexpr int U1 = 4;                                                 // staged rows per thread in flight
   constexpr int U2 = 2;                                                 // SpMV rows per thread in flight
+  constexpr int NDA = ND > 1 ? ND : 2;
   __shared__ double sm[4];
   __shared__ double sm_out[2 * (NT / 64)];
   __shared__ T pt[TT + 2 * LDS_HALO];
@@ -481,19 +501,16 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
   T beta, ap;
   int64_t k, r0l, r1l;
   if (!iter_head<NT, T, V>(a, sm, gamma, beta, ap, k, r0l, r1l)) return;
-  // row indices fit 32 bits (CSR columns are int32): 32-bit offsets keep the addresses in one
-  // register each
+  // row indices fit 32 bits (CSR columns are int32): 32-bit offsets keep each address in one
+  // register
   const bool upd_x = k > 0;
   const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)r1l;
   const V* vals = a.vals;
   const int ld = (int)dia_ld(a.n);
-  int o[ND];
-  int h = 0;
+  int o[NDA];
 #pragma unroll
-  for (int d = 1; d < ND; ++d) {
-    o[d] = a.off.o[d];
-    if (o[d] <= LDS_HALO) h = max(h, o[d]);
-  }
+  for (int d = 1; d < ND; ++d) o[d] = a.off.o[d];
+  const int h = NN > 0 ? o[NN] : 0;
   double part = 0.0;
   for (int t0 = r0; t0 < r1; t0 += TT) {
     const int t1 = min(r1, t0 + TT), s0 = t0 - h, s1 = t1 + h;
@@ -503,10 +520,9 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
       for (int u = 0; u < U1; ++u) {
         const int j = j0 + u * NT;
         const bool in = j < s1 && j >= 0 && j < n;
-        const bool own = in && j >= t0 && j < t1;
-        rv[u] = in ? a.r[j] : T(0);
-        pv[u] = in ? a.p_old[j] : T(0);
-        xv[u] = own && upd_x ? a.x[j] : T(0);
+        rv[u] = ldc(a.r, j, in);
+        pv[u] = ldc(a.p_old, j, in);
+        xv[u] = ldc(a.x, j, in);
       }
 #pragma unroll
       for (int u = 0; u < U1; ++u) {
@@ -523,8 +539,8 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
     }
     __syncthreads();
     for (int i0 = t0 + (int)threadIdx.x; i0 < t1; i0 += U2 * NT) {
-      V vu[U2][ND], vl[U2][ND];
-      T xu[U2][ND], xl[U2][ND];
+      V vu[U2][NDA], vl[U2][NDA];
+      T xu[U2][NDA], xl[U2][NDA];
 #pragma unroll
       for (int u = 0; u < U2; ++u) {
         const int i = i0 + u * NT;
@@ -532,12 +548,12 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
 #pragma unroll
         for (int d = 1; d < ND; ++d) {
           const V* vd = vals + d * ld;
-          vu[u][d] = ok ? vd[i] : V(0);
-          vl[u][d] = ok && i - o[d] >= 0 ? vd[i - o[d]] : V(0);
-          if (o[d] > h) {
+          vu[u][d] = ldc(vd, i, ok);
+          vl[u][d] = ldz(vd, i - o[d], ok && i - o[d] >= 0);
+          if (d > NN) {
             const bool up = ok && i + o[d] < n, lo = ok && i - o[d] >= 0;
-            xu[u][d] = up ? a.r[i + o[d]] + beta * a.p_old[i + o[d]] : T(0);
-            xl[u][d] = lo ? a.r[i - o[d]] + beta * a.p_old[i - o[d]] : T(0);
+            xu[u][d] = ldc(a.r, i + o[d], up) + beta * ldc(a.p_old, i + o[d], up);
+            xl[u][d] = ldc(a.r, i - o[d], lo) + beta * ldc(a.p_old, i - o[d], lo);
           }
         }
       }
@@ -549,7 +565,7 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
           T s = pi;
 #pragma unroll
           for (int d = 1; d < ND; ++d) {
-            if (o[d] <= h) {
+            if (d <= NN) {
               xu[u][d] = pt[i + o[d] - s0];
               xl[u][d] = pt[i - o[d] - s0];
             }
@@ -569,26 +585,31 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
 
 // ---------------------------------------------------------------------------------------------
 // One kernel per iteration on the Jacobi-scaled DIA operator (iteration mode 5).  Launch k reads
-// r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1} and the partial sums of launch k-1: gamma_{k-1} = r.r
+// r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1}, x and the partial sums of launch k-1: gamma_{k-1} = r.r
 // (exact), p.q, q.r, q.q.  With alpha = gamma_{k-1} / p.q the next residual norm follows without
 // another reduction, gamma_k = gamma_{k-1} - 2 alpha q.r + alpha^2 q.q (from the exact
 // gamma_{k-1}, so its rounding does not accumulate over iterations), and beta = gamma_k /
 // gamma_{k-1}.  Then, per row: r_k = r_{k-1} - alpha q_{k-1}, x += alpha p_{k-1},
-// p_k = r_k + beta p_{k-1} (staged in LDS as in pcg_k1_lds, far neighbours recomputed from r, q and
-// p), q_k = A p_k, and the partials of r_k.r_k, p_k.q_k, q_k.r_k, q_k.q_k.  r, q and p each
-// alternate between two buffers.  A stop leaves x_{k-1} and r_{k-1} consistent (no deferred
-// update).  After a residual replacement the driver reruns launch k-1 with the true residual and
-// alpha = 0, beta = gamma_true / gamma_{k-2} on p_{k-2}: the search direction survives the
-// replacement.  One launch per iteration moves 44 instead of 48 bytes per row and pays one launch
-// and one reduction instead of two.
+// p_k = r_k + beta p_{k-1} (staged in LDS as above, far neighbours recomputed from r, q and p),
+// q_k = A p_k, and the partials of r_k.r_k, p_k.q_k, q_k.r_k, q_k.q_k.  r, q and p each alternate
+// between two buffers (launch k reads one and writes the other); x is updated in place.  A stop
+// leaves x_{k-1} and r_{k-1} consistent (no deferred update).  After a residual replacement the
+// driver reruns launch k-1 with the true residual and alpha = 0, beta = gamma_true / gamma_{k-2} on
+// p_{k-2}: the search direction survives the replacement.  One launch per iteration moves 44
+// instead of 48 bytes per row and pays one launch and one reduction instead of two.  The first
+// batch of loads is issued before the reduction of the partial sums (none of them depends on the
+// scalars), and each staged batch's loads are in flight while the previous one is stored.
+// (Interleaving r, p, x and q per row, to fetch a far neighbour in one load, measured slower: the
+// partially written lines.)
 // ---------------------------------------------------------------------------------------------
 constexpr int ONE_STAGE_BYTES = 65536;
 
-template <int NT, int ND, typename T, typename V>
+template <int NT, int ND, int NN, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   constexpr int TT = ONE_STAGE_BYTES / (2 * (int)sizeof(T)) - LDS_HALO;  // rows per tile
   constexpr int U1 = 4;
   constexpr int U2 = 2;
+  constexpr int NDA = ND > 1 ? ND : 2;
   __shared__ double sm[4];
   __shared__ double sm_out[4 * (NT / 64)];
   __shared__ T pt[TT + 2 * LDS_HALO];  // p_k of the tile and its halo
@@ -597,6 +618,28 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   if (st->done) return;
   const int G = gridDim.x;
   const int64_t k = a.kk;
+  const int64_t r0l = block_first_row(a.rows_per_block);
+  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + a.rows_per_block);
+  const V* vals = a.vals;
+  const int ld = (int)dia_ld(a.n);
+  int o[NDA];
+#pragma unroll
+  for (int d = 1; d < ND; ++d) o[d] = a.off.o[d];
+  const int h = NN > 0 ? o[NN] : 0;
+  T rv[U1], qv[U1], pv[U1], xv[U1];
+  auto load1 = [&](int j0, int s1) {
+#pragma unroll
+    for (int u = 0; u < U1; ++u) {
+      const int j = j0 + u * NT;
+      const bool in = j < s1 && j >= 0 && j < n;
+      rv[u] = ldc(a.r, j, in);
+      qv[u] = ldc(a.q, j, in);
+      pv[u] = ldc(a.p_old, j, in);
+      xv[u] = ldc(a.x, j, in);
+    }
+  };
+  // the first staged batch of the first tile, loaded before the reduction
+  load1(r0 - h + (int)threadIdx.x, min(r1, r0 + TT) + h);
   double in[4];
   reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
   const double rr = in[0];
@@ -623,55 +666,41 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0 && special != 2) st->g_hist[k & 1] = rr;
+  // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const T al = (T)alpha, be = (T)beta;
-  const bool upd = special == 0;  // alpha != 0: r and x updates, q read
-  const int64_t r0l = block_first_row(a.rows_per_block);
-  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + a.rows_per_block);
-  const V* vals = a.vals;
-  const int ld = (int)dia_ld(a.n);
-  int o[ND > 1 ? ND : 2];
-  int h = 0;
-#pragma unroll
-  for (int d = 1; d < ND; ++d) {
-    o[d] = a.off.o[d];
-    if (o[d] <= LDS_HALO) h = max(h, o[d]);
-  }
+  const bool upd = special == 0;
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   for (int t0 = r0; t0 < r1; t0 += TT) {
     const int t1 = min(r1, t0 + TT), s0 = t0 - h, s1 = t1 + h;
+    if (t0 > r0) load1(s0 + (int)threadIdx.x, s1);
     for (int j0 = s0 + (int)threadIdx.x; j0 < s1; j0 += U1 * NT) {
-      T rv[U1], qv[U1], pv[U1], xv[U1];
+      T rn[U1], pn[U1], xn[U1];
 #pragma unroll
       for (int u = 0; u < U1; ++u) {
-        const int j = j0 + u * NT;
-        const bool in = j < s1 && j >= 0 && j < n;
-        const bool own = in && j >= t0 && j < t1;
-        rv[u] = in ? a.r[j] : T(0);
-        qv[u] = in && upd ? a.q[j] : T(0);
-        pv[u] = in ? a.p_old[j] : T(0);
-        xv[u] = own && upd ? a.x[j] : T(0);
+        rn[u] = rv[u] - al * qv[u];
+        pn[u] = rn[u] + be * pv[u];
+        xn[u] = xv[u] + al * pv[u];
       }
+      load1(j0 + U1 * NT, s1);  // the next batch in flight meanwhile (past the end: row 0, unused)
 #pragma unroll
       for (int u = 0; u < U1; ++u) {
         const int j = j0 + u * NT;
         if (j < s1) {
-          const T rn = rv[u] - al * qv[u];
-          const T pn = rn + be * pv[u];
-          pt[j - s0] = pn;
+          pt[j - s0] = pn[u];
           if (j >= t0 && j < t1) {
-            rt[j - t0] = rn;
-            sto(&a.r_out[j], rn);
-            sto(&a.p_new[j], pn);
-            if (upd) sto(&a.x[j], xv[u] + al * pv[u]);
-            part[0] += (double)rn * (double)rn;
+            rt[j - t0] = rn[u];
+            sto(&a.r_out[j], rn[u]);
+            sto(&a.p_new[j], pn[u]);
+            if (upd) sto(&a.x[j], xn[u]);
+            part[0] += (double)rn[u] * (double)rn[u];
           }
         }
       }
     }
     __syncthreads();
     for (int i0 = t0 + (int)threadIdx.x; i0 < t1; i0 += U2 * NT) {
-      V vu[U2][ND > 1 ? ND : 2], vl[U2][ND > 1 ? ND : 2];
-      T xu[U2][ND > 1 ? ND : 2], xl[U2][ND > 1 ? ND : 2];
+      V vu[U2][NDA], vl[U2][NDA];
+      T xu[U2][NDA], xl[U2][NDA];
 #pragma unroll
       for (int u = 0; u < U2; ++u) {
         const int i = i0 + u * NT;
@@ -679,16 +708,12 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
 #pragma unroll
         for (int d = 1; d < ND; ++d) {
           const V* vd = vals + d * ld;
-          vu[u][d] = ok ? vd[i] : V(0);
-          vl[u][d] = ok && i - o[d] >= 0 ? vd[i - o[d]] : V(0);
-          if (o[d] > h) {
+          vu[u][d] = ldc(vd, i, ok);
+          vl[u][d] = ldz(vd, i - o[d], ok && i - o[d] >= 0);
+          if (d > NN) {
             const bool up = ok && i + o[d] < n, lo = ok && i - o[d] >= 0;
-            const T ru = up ? a.r[i + o[d]] : T(0), pu = up ? a.p_old[i + o[d]] : T(0);
-            const T qu = up && upd ? a.q[i + o[d]] : T(0);
-            const T rl = lo ? a.r[i - o[d]] : T(0), pl = lo ? a.p_old[i - o[d]] : T(0);
-            const T ql = lo && upd ? a.q[i - o[d]] : T(0);
-            xu[u][d] = (ru - al * qu) + be * pu;
-            xl[u][d] = (rl - al * ql) + be * pl;
+            xu[u][d] = (ldc(a.r, i + o[d], up) - al * ldc(a.q, i + o[d], up)) + be * ldc(a.p_old, i + o[d], up);
+            xl[u][d] = (ldc(a.r, i - o[d], lo) - al * ldc(a.q, i - o[d], lo)) + be * ldc(a.p_old, i - o[d], lo);
           }
         }
       }
@@ -700,7 +725,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
           T s = pi;
 #pragma unroll
           for (int d = 1; d < ND; ++d) {
-            if (o[d] <= h) {
+            if (d <= NN) {
               xu[u][d] = pt[i + o[d] - s0];
               xl[u][d] = pt[i - o[d] - s0];
             }
@@ -1289,14 +1314,15 @@ __global__ void scale_dia_k(int64_t n, int ndiag, DiaOffsets off, const double* 
   }
 }
 
-// r64 = b - Ax (Ax nullable = 0), rs = (float)(sqrt(dinv) r64); partials of rs.rs (both B slots,
-// and c_rr when given: the r.r slot a one-kernel launch reads) and of ||r64||^2, b.b (P)
+// r64 = b - Ax (Ax nullable = 0), rs[i * rs_stride] = (float)(sqrt(dinv) r64); partials of rs.rs
+// (both B slots, and c_rr when given: the r.r slot a one-kernel launch reads) and of ||r64||^2, b.b
+// (P)
 __global__ void __launch_bounds__(PCG_NT) residual_prep_scaled_k(int64_t n, int64_t rows_per_block,
                                                                  const double* __restrict__ b,
                                                                  const double* __restrict__ Ax,
                                                                  const double* __restrict__ dinv,
-                                                                 float* __restrict__ rs, Partials part,
-                                                                 double* __restrict__ c_rr) {
+                                                                 float* __restrict__ rs, int rs_stride,
+                                                                 Partials part, double* __restrict__ c_rr) {
   __shared__ double sm[PCG_NT / 64];
   const int G = gridDim.x;
   const int64_t r0 = block_first_row(rows_per_block);
@@ -1308,7 +1334,7 @@ __global__ void __launch_bounds__(PCG_NT) residual_prep_scaled_k(int64_t n, int6
     prt += r * r;
     pb += bi * bi;
     const float ri = (float)(r * sqrt(dinv[i]));
-    rs[i] = ri;
+    rs[i * rs_stride] = ri;
     pr += (double)ri * (double)ri;
   }
   store_partial(pr, part.B, sm);
@@ -1500,19 +1526,62 @@ static int launch_kb(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStre
 
 // One iteration: K1 K2 | K0 KS K2 | KA KB.  ev (optional) receives three events: after K0 or KA
 // (at the start for K1 K2), after the SpMV kernel, after K2 (right after KB for KA KB).
+// Near off-diagonals of a DIA operator for the LDS-staged kernels: the count of stored offsets
+// (after the main diagonal) up to LDS_HALO, or -1 when the offsets do not ascend.
+static int near_count(const jno_op* op) {
+  int nn = 0;
+  for (int d = 1; d < op->ndiag; ++d) {
+    if (op->offsets[d] <= op->offsets[d - 1]) return -1;
+    if (op->offsets[d] <= LDS_HALO) nn = d;
+  }
+  return nn;
+}
+
+static bool lds_ok(const jno_op* op) {
+  return op->format == JNO_FMT_DIA && op->ndiag >= 1 && op->ndiag <= 4 && near_count(op) >= 0;
+}
+
+// pcg_one (one = true) or pcg_k1_lds by diagonal count and near count (FP32 iterations only)
+template <int NT, typename T, typename V>
+static int launch_lds(bool one, const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  JNO_REQUIRE(lds_ok(op), JNO_EINVAL, "pcg: the LDS-staged kernels take 1..4 ascending DIA diagonals");
+  if constexpr (std::is_same<T, float>::value) {
+    const int key = op->ndiag * 8 + near_count(op);
+#define JNO_LDS_CASE(ND_, NN_)                                     \
+  case ND_ * 8 + NN_:                                              \
+    if (one)                                                       \
+      pcg_one<NT, ND_, NN_, T, V><<<grid, NT, 0, s>>>(a);          \
+    else                                                           \
+      pcg_k1_lds<NT, ND_, NN_, T, V><<<grid, NT, 0, s>>>(a);       \
+    break;
+    switch (key) {
+      JNO_LDS_CASE(1, 0)
+      JNO_LDS_CASE(2, 0)
+      JNO_LDS_CASE(2, 1)
+      JNO_LDS_CASE(3, 0)
+      JNO_LDS_CASE(3, 1)
+      JNO_LDS_CASE(3, 2)
+      JNO_LDS_CASE(4, 0)
+      JNO_LDS_CASE(4, 1)
+      JNO_LDS_CASE(4, 2)
+      JNO_LDS_CASE(4, 3)
+      default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: no LDS-staged kernel for this diagonal set");
+    }
+#undef JNO_LDS_CASE
+    JNO_LAUNCH_CHECK();
+    return JNO_OK;
+  } else {
+    JNO_REQUIRE(false, JNO_EINVAL, "pcg: the LDS-staged kernels run FP32 iterations");
+  }
+}
+
 template <int NT, typename T, typename V>
 static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
   int rc;
   if (a.unit && a.split == MODE_ONE) {
     if (ev) JNO_HIP(hipEventRecord(ev[0], s));
-    switch (op->ndiag) {
-      case 4: pcg_one<NT, 4, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 3: pcg_one<NT, 3, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 2: pcg_one<NT, 2, T, V><<<grid, NT, 0, s>>>(a); break;
-      case 1: pcg_one<NT, 1, T, V><<<grid, NT, 0, s>>>(a); break;
-      default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: the one-kernel iteration takes 1..4 stored diagonals");
-    }
-    JNO_LAUNCH_CHECK();
+    rc = launch_lds<NT, T, V>(true, op, a, grid, s);
+    if (rc) return rc;
     if (ev) JNO_HIP(hipEventRecord(ev[1], s));
     if (ev) JNO_HIP(hipEventRecord(ev[2], s));
     return JNO_OK;
@@ -1520,21 +1589,15 @@ static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStrea
   if (a.unit && a.split == MODE_FUSED) {
     // Jacobi-scaled DIA operator, two kernels: K1 and K2 without dinv, unit main diagonal
     if (ev) JNO_HIP(hipEventRecord(ev[0], s));
-    const bool lds = a.vec == 3;
-    switch (op->ndiag) {
-      case 4:
-        if (lds)
-          pcg_k1_lds<NT, 4, T, V><<<grid, NT, 0, s>>>(a);
-        else
-          pcg_k1<NT, 1, true, 4, T, V, false><<<grid, NT, 0, s>>>(a);
-        break;
-      case 3:
-        if (lds)
-          pcg_k1_lds<NT, 3, T, V><<<grid, NT, 0, s>>>(a);
-        else
-          pcg_k1<NT, 1, true, 3, T, V, false><<<grid, NT, 0, s>>>(a);
-        break;
-      default: pcg_k1<NT, 1, true, 0, T, V, false><<<grid, NT, 0, s>>>(a); break;
+    if (a.vec == 3 && lds_ok(op)) {
+      rc = launch_lds<NT, T, V>(false, op, a, grid, s);
+      if (rc) return rc;
+    } else {
+      switch (op->ndiag) {
+        case 4: pcg_k1<NT, 1, true, 4, T, V, false><<<grid, NT, 0, s>>>(a); break;
+        case 3: pcg_k1<NT, 1, true, 3, T, V, false><<<grid, NT, 0, s>>>(a); break;
+        default: pcg_k1<NT, 1, true, 0, T, V, false><<<grid, NT, 0, s>>>(a); break;
+      }
     }
     JNO_LAUNCH_CHECK();
     if (ev) JNO_HIP(hipEventRecord(ev[1], s));
@@ -1901,7 +1964,7 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   if (h_bad) return JNO_OK;
   *used = true;
   // one kernel per iteration: r, q (and p) alternate between two buffers by launch parity
-  const bool one = tu.split == MODE_ONE && h_op->ndiag <= 4 && w.q2;
+  const bool one = tu.split == MODE_ONE && lds_ok(h_op) && w.q2;
   float* rs = (float*)w.r;
   float* pb[2] = {(float*)w.p0, (float*)w.p1};
   float* rb[2] = {rs, (float*)w.s};
@@ -1922,8 +1985,7 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   } else {
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
   }
-  residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, rs, w.part,
-                                               one ? w.part.C : nullptr);
+  residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, rs, 1, w.part, one ? w.part.C : nullptr);
   // inner tolerance 0: the FP64 residual below decides convergence
   finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 1, 1, 0);
   JNO_LAUNCH_CHECK();
@@ -1942,15 +2004,15 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
     rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr);
     if (rc) return rc;
     k = f.k_stop;
+    // one-kernel iterations resume by rerunning launch k-1 (from p_{k-2}, which launch k did not
+    // overwrite): its r input half takes the true residual, its r.r slot the new partials
+    const int64_t kr = k > 0 ? k - 1 : 0;
     accumulate_scaled_k<<<gz, PCG_NT, 0, s>>>(n, d_x, xs, d_dinv);
     JNO_LAUNCH_CHECK();
     rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
     if (rc) return rc;
-    // one-kernel iterations resume by rerunning launch k-1 (from p_{k-2}, which launch k did not
-    // overwrite): its r input half takes the true residual, its r.r slot the new partials
-    const int64_t kr = k > 0 ? k - 1 : 0;
-    residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, one ? rb[kr & 1] : rs, w.part,
-                                                 one ? w.part.C + (kr & 1) * 4 * (int64_t)G : nullptr);
+    residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, one ? rb[kr & 1] : rs, 1,
+                                                 w.part, one ? w.part.C + (kr & 1) * 4 * (int64_t)G : nullptr);
     finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, one ? maxiter - resumes - 1 : maxiter, repl2, 0, 1,
                                     0);
     JNO_LAUNCH_CHECK();
