@@ -638,8 +638,11 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
       xv[u] = ldc(a.x, j, in);
     }
   };
+  // tiles of equal length tl <= TT
+  const int nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
+  const int tl = nt > 0 ? (r1 - r0 + nt - 1) / nt : TT;
   // the first staged batch of the first tile, loaded before the reduction
-  load1(r0 - h + (int)threadIdx.x, min(r1, r0 + TT) + h);
+  load1(r0 - h + (int)threadIdx.x, min(r1, r0 + tl) + h);
   double in[4];
   reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
   const double rr = in[0];
@@ -670,8 +673,8 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   const T al = (T)alpha, be = (T)beta;
   const bool upd = special == 0;
   double part[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int t0 = r0; t0 < r1; t0 += TT) {
-    const int t1 = min(r1, t0 + TT), s0 = t0 - h, s1 = t1 + h;
+  for (int t0 = r0; t0 < r1; t0 += tl) {
+    const int t1 = min(r1, t0 + tl), s0 = t0 - h, s1 = t1 + h;
     if (t0 > r0) load1(s0 + (int)threadIdx.x, s1);
     for (int j0 = s0 + (int)threadIdx.x; j0 < s1; j0 += U1 * NT) {
       T rn[U1], pn[U1], xn[U1];
