@@ -6,7 +6,8 @@
 // entries in ascending element order (the plan sorted them), recomputes that element's geometry
 // and adds the element-matrix row into register accumulators matched to the row's CSR columns.
 // Every value is written exactly once; the summation order per entry is fixed, so results are
-// bit-reproducible run to run.
+// bit-reproducible run to run.  (Half a wavefront per row, one lane per incident element with the
+// contributions added in the same order through LDS, measured twice as slow.)
 #include "common.h"
 
 namespace jno {
@@ -51,6 +52,14 @@ __device__ __forceinline__ TetGeom tet_geom(const double* __restrict__ pts, int4
   return t;
 }
 
+// row a of the gradients and vertex a of the element, by selects (a runtime index into a register
+// array would go through scratch memory)
+__device__ __forceinline__ void pick_row(const TetGeom& t, int4 c, int a, double ga[3], int& na) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ga[i] = a == 0 ? t.g[0][i] : a == 1 ? t.g[1][i] : a == 2 ? t.g[2][i] : t.g[3][i];
+  na = a == 0 ? c.x : a == 1 ? c.y : a == 2 ? c.z : c.w;
+}
+
 template <int MAXR>
 __global__ void __launch_bounds__(256) assemble_p1_k(
     int kind, const double* __restrict__ pts, const int4* __restrict__ cells, int64_t n_nodes,
@@ -77,20 +86,23 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
       const int4 c = cells[e];
       const TetGeom t = tet_geom(pts, c);
       const int nb[4] = {c.x, c.y, c.z, c.w};
+      double ga[3];
+      int na;
+      pick_row(t, c, a, ga, na);
       if (kind == 0) {
         const double coef = kcoef[e] * t.vol;
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
           // rounded term by term (no FMA contraction) so that K[r][c] and K[c][r] are bitwise equal
-          const double dot = __dadd_rn(__dadd_rn(__dmul_rn(t.g[a][0], t.g[bb][0]), __dmul_rn(t.g[a][1], t.g[bb][1])),
-                                       __dmul_rn(t.g[a][2], t.g[bb][2]));
+          const double dot = __dadd_rn(__dadd_rn(__dmul_rn(ga[0], t.g[bb][0]), __dmul_rn(ga[1], t.g[bb][1])),
+                                       __dmul_rn(ga[2], t.g[bb][2]));
           const double v = __dmul_rn(coef, dot);
 #pragma unroll
           for (int s = 0; s < MAXR; ++s) acc[s] += (cols[s] == nb[bb]) ? v : 0.0;
         }
         if (f_nodal) {
           const double fs = f_nodal[c.x] + f_nodal[c.y] + f_nodal[c.z] + f_nodal[c.w];
-          bl += t.vol * (f_nodal[nb[a]] + fs) / 20.0;
+          bl += t.vol * (f_nodal[na] + fs) / 20.0;
         } else {
           bl += f_const * t.vol / 4.0;
         }

@@ -140,6 +140,29 @@ def test_assembly_nonzero_dirichlet_and_nodal_load(dev):
     assert np.abs(_np(b) - bo).max() <= VAL_TOL * np.abs(bo).max()
 
 
+def test_assembly_fan_mesh(dev):
+    """Assembly on a mesh off the lattice: a fan whose centre has 40 incident tets of both
+    orientations (ring of 20 points in z = 0, apexes above and below), against the oracle."""
+    import jno_amd as J
+
+    ring = [(np.cos(2 * np.pi * i / 20), np.sin(2 * np.pi * i / 20), 0.0) for i in range(20)]
+    pts = np.array([(0.0, 0.0, 0.0)] + ring + [(0.0, 0.0, 1.0), (0.0, 0.0, -1.0)])
+    cells = []
+    for i in range(20):
+        a, b = 1 + i, 1 + (i + 1) % 20
+        cells += [(0, a, b, 21), (0, b, a, 22)]
+    cells = np.array(cells, dtype=np.int32)
+    fan = J.Mesh(points=torch.tensor(pts, device="cuda"), cells=torch.tensor(cells, device="cuda"))
+    k = O.k_lognormal(cells.shape[0], 22)
+    A, b = J.fem_poisson(fan, dirichlet=None).evaluate({"k": torch.tensor(k, device="cuda")})
+    N = pts.shape[0]
+    Ao = O.csr_matrix(*O.assemble(O.p1_stiffness(pts, cells, k), cells, 1, N), N)
+    Ao.sort_indices()
+    assert np.array_equal(_np(A.col), Ao.indices) and np.array_equal(_np(A.rowptr), Ao.indptr)
+    assert np.abs(_np(A.vals) - Ao.data).max() <= VAL_TOL * np.abs(Ao.data).max()
+    assert np.abs(_np(b) - O.p1_load(pts, cells, 1.0, N)).max() <= VAL_TOL * np.abs(_np(b)).max()
+
+
 def test_mass_matrix(dev):
     n = 5
     p, t = O.kuhn_box(n)
