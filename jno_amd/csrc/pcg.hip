@@ -606,14 +606,17 @@ constexpr int ONE_STAGE_BYTES = 65536;
 
 template <int NT, int ND, int NN, typename T, typename V>
 __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
-  constexpr int TT = ONE_STAGE_BYTES / (2 * (int)sizeof(T)) - LDS_HALO;  // rows per tile
+  constexpr int NF = ND > 1 ? ND - 1 - NN : 0;  // far off-diagonals
+  // rows per tile: pt holds TT + 2 halo rows, rt TT rows, pf two ranges of TT rows per far diagonal
+  constexpr int TT = (ONE_STAGE_BYTES / (int)sizeof(T) - 2 * LDS_HALO) / (2 + 2 * NF);
   constexpr int U1 = 4;
   constexpr int U2 = 2;
   constexpr int NDA = ND > 1 ? ND : 2;
   __shared__ double sm[4];
   __shared__ double sm_out[4 * (NT / 64)];
-  __shared__ T pt[TT + 2 * LDS_HALO];  // p_k of the tile and its halo
-  __shared__ T rt[TT];                 // r_k of the tile
+  __shared__ T pt[TT + 2 * LDS_HALO];         // p_k of the tile and its halo
+  __shared__ T rt[TT];                        // r_k of the tile
+  __shared__ T pf[NF > 0 ? 2 * NF : 1][NF > 0 ? TT : 1];  // p_k of the far neighbours (i + o_d, i - o_d)
   PcgState* st = a.st;
   if (st->done) return;
   const int G = gridDim.x;
@@ -700,6 +703,28 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
         }
       }
     }
+    // p_k of the far neighbour rows (offsets > LDS_HALO), staged in the same pass: their owners
+    // read the same rows at about the same time, so these loads hit in L2 (read later, in the
+    // SpMV, they had been evicted and came from memory again)
+#pragma unroll
+    for (int f = 0; f < 2 * NF; ++f) {
+      const int d = NN + 1 + f / 2;
+      const int sh = (f & 1) ? -o[d] : o[d];
+      for (int j0 = t0 + (int)threadIdx.x; j0 < t1; j0 += U1 * NT) {
+        T fr[U1], fq[U1], fp[U1];
+#pragma unroll
+        for (int u = 0; u < U1; ++u) {
+          const int j = j0 + u * NT + sh;
+          const bool in = j0 + u * NT < t1 && j >= 0 && j < n;
+          fr[u] = ldc(a.r, j, in);
+          fq[u] = ldc(a.q, j, in);
+          fp[u] = ldc(a.p_old, j, in);
+        }
+#pragma unroll
+        for (int u = 0; u < U1; ++u)
+          if (j0 + u * NT < t1) pf[f][j0 + u * NT - t0] = (fr[u] - al * fq[u]) + be * fp[u];
+      }
+    }
     __syncthreads();
     for (int i0 = t0 + (int)threadIdx.x; i0 < t1; i0 += U2 * NT) {
       V vu[U2][NDA], vl[U2][NDA];
@@ -713,11 +738,6 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
           const V* vd = vals + d * ld;
           vu[u][d] = ldc(vd, i, ok);
           vl[u][d] = ldz(vd, i - o[d], ok && i - o[d] >= 0);
-          if (d > NN) {
-            const bool up = ok && i + o[d] < n, lo = ok && i - o[d] >= 0;
-            xu[u][d] = (ldc(a.r, i + o[d], up) - al * ldc(a.q, i + o[d], up)) + be * ldc(a.p_old, i + o[d], up);
-            xl[u][d] = (ldc(a.r, i - o[d], lo) - al * ldc(a.q, i - o[d], lo)) + be * ldc(a.p_old, i - o[d], lo);
-          }
         }
       }
 #pragma unroll
@@ -731,6 +751,9 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
             if (d <= NN) {
               xu[u][d] = pt[i + o[d] - s0];
               xl[u][d] = pt[i - o[d] - s0];
+            } else {
+              xu[u][d] = pf[2 * (d - NN - 1)][i - t0];
+              xl[u][d] = pf[2 * (d - NN - 1) + 1][i - t0];
             }
             s += (T)vu[u][d] * xu[u][d];
             s += (T)vl[u][d] * xl[u][d];
