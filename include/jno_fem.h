@@ -94,6 +94,21 @@ int jno_assemble_p1(int kind, const double* d_points, const int32_t* d_cells, in
                     double f_const, const double* d_f_nodal, const uint8_t* d_dmask,
                     const double* d_g, double* d_vals, double* d_b, double* d_dinv, void* stream);
 
+/* jno_assemble_p1 that also writes the symmetric DIA layout of the result (no reference
+ * counterpart: it fuses jno_op_fill_dia into the assembly for an offset set found earlier on the
+ * same plan).  d_dia64 (and d_dia32 when not null) get ndiag diagonals of ld = n rounded up to 64
+ * rows, val[d][i] = A[i, i + h_offsets[d]].  *h_ok = 1 when every nonzero entry lies on a stored
+ * diagonal (the layout reproduces A; the assembly is bitwise symmetric by construction), else 0
+ * (then the DIA buffers must not be used; the CSR outputs are valid either way).  d_ws: 256
+ * bytes of device scratch.  Synchronizes the stream to read the flag. */
+int jno_assemble_p1_dia(int kind, const double* d_points, const int32_t* d_cells, int64_t n_cells,
+                        int64_t n_nodes, const int32_t* d_inc_ptr, const int32_t* d_inc,
+                        const int32_t* d_rowptr, const int32_t* d_col, int max_row, const double* d_k,
+                        double f_const, const double* d_f_nodal, const uint8_t* d_dmask,
+                        const double* d_g, double* d_vals, double* d_b, double* d_dinv, int ndiag,
+                        const int32_t* h_offsets, double* d_dia64, float* d_dia32, int* h_ok, void* d_ws,
+                        size_t ws_bytes, void* stream);
+
 /* A11 on any assembled CSR (P2 elasticity uses it): bcoo_zero_rows_cols + bcoo_set_unit_diag
  * (jno/utils/solver/fem_utils.py:3332-3359) with the lift b -= A e(g) and b[d] = g.  In place. */
 int jno_dirichlet_eliminate(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,

@@ -81,6 +81,8 @@ SIGNATURES = {
     "jno_boundary_mask": (C.c_int, [P, I64, C.c_int, I64, P, P, P, P]),
     "jno_mask_plane": (C.c_int, [P, I64, C.c_int, D, P, P, P]),
     "jno_assemble_p1": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, P]),
+    "jno_assemble_p1_dia": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, C.c_int,
+                                      C.POINTER(I32), P, P, C.POINTER(C.c_int), P, SZ, P]),
     "jno_dirichlet_eliminate": (C.c_int, [I64, P, P, P, P, P, P, P]),
     "jno_jacobi": (C.c_int, [I64, P, P, P, P, P]),
     "jno_spmv_f64": (C.c_int, [I64, P, P, P, P, P, P]),

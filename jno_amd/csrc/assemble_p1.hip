@@ -9,8 +9,23 @@
 // bit-reproducible run to run.  (Half a wavefront per row, one lane per incident element with the
 // contributions added in the same order through LDS, measured twice as slow.)
 #include "common.h"
+#include "sparse.h"
 
 namespace jno {
+
+// Optional symmetric-DIA output of the assembly (offsets known from an earlier analysis of the
+// same plan): the stored diagonals are written from the row's accumulators in the same pass, and
+// *bad is set when a nonzero entry lies off the stored diagonals (the layout does not apply).  The
+// lower diagonals need no check: the assembly is bitwise symmetric by construction (each entry
+// pair is summed over the same elements in the same order, and the Dirichlet elimination is
+// symmetric).
+struct DiaOut {
+  int ndiag;
+  DiaOffsets off;
+  double* d64;
+  float* d32;
+  int* bad;
+};
 
 struct TetGeom {
   double g[4][3];  // physical gradients of the barycentric basis
@@ -66,7 +81,7 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
     const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ col, const double* __restrict__ kcoef, double f_const,
     const double* __restrict__ f_nodal, const uint8_t* __restrict__ dmask, const double* __restrict__ g,
-    double* __restrict__ vals, double* __restrict__ b, double* __restrict__ dinv) {
+    double* __restrict__ vals, double* __restrict__ b, double* __restrict__ dinv, DiaOut dia) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_nodes;
        r += (int64_t)gridDim.x * blockDim.x) {
     const int rb = rowptr[r];
@@ -142,6 +157,29 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
     }
     if (b) b[r] = bl;
     if (dinv) dinv[r] = fabs(d) > 1e-30 ? 1.0 / d : 1.0;
+    if (dia.d64) {
+      const int64_t ld = dia_ld(n_nodes);
+      bool off_set = false;
+#pragma unroll
+      for (int s = 0; s < MAXR; ++s) {
+        if (s < len && acc[s] != 0.0) {
+          const int o = cols[s] - (int)r;
+          const int ao = o < 0 ? -o : o;
+          bool in_set = false;
+          for (int dd = 0; dd < dia.ndiag; ++dd) in_set = in_set || dia.off.o[dd] == ao;
+          off_set = off_set || !in_set;
+        }
+      }
+      if (off_set) atomicOr(dia.bad, 1);
+      for (int dd = 0; dd < dia.ndiag; ++dd) {
+        double v = 0.0;
+#pragma unroll
+        for (int s = 0; s < MAXR; ++s)
+          if (cols[s] - (int)r == dia.off.o[dd]) v += acc[s];
+        dia.d64[(int64_t)dd * ld + r] = v;
+        if (dia.d32) dia.d32[(int64_t)dd * ld + r] = (float)v;
+      }
+    }
   }
 }
 
@@ -173,9 +211,28 @@ template <int MAXR>
 static void launch_assemble(hipStream_t s, int kind, const double* pts, const int4* cells, int64_t N,
                             const int32_t* inc_ptr, const int32_t* inc, const int32_t* rowptr, const int32_t* col,
                             const double* k, double fc, const double* fn, const uint8_t* dm, const double* g,
-                            double* vals, double* b, double* dinv) {
+                            double* vals, double* b, double* dinv, const DiaOut& dia) {
   assemble_p1_k<MAXR><<<grid_for(N, 256, 8192), 256, 0, s>>>(kind, pts, cells, N, inc_ptr, inc, rowptr, col, k, fc,
-                                                             fn, dm, g, vals, b, dinv);
+                                                             fn, dm, g, vals, b, dinv, dia);
+}
+
+static int assemble_any(int kind, const double* d_points, const int32_t* d_cells, int64_t n_nodes,
+                        const int32_t* d_inc_ptr, const int32_t* d_inc, const int32_t* d_rowptr,
+                        const int32_t* d_col, int max_row, const double* d_k, double f_const,
+                        const double* d_f_nodal, const uint8_t* d_dmask, const double* d_g, double* d_vals,
+                        double* d_b, double* d_dinv, const DiaOut& dia, hipStream_t s) {
+  const int4* cells = reinterpret_cast<const int4*>(d_cells);
+  if (max_row <= 16)
+    launch_assemble<16>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
+                        d_dmask, d_g, d_vals, d_b, d_dinv, dia);
+  else if (max_row <= 32)
+    launch_assemble<32>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
+                        d_dmask, d_g, d_vals, d_b, d_dinv, dia);
+  else
+    launch_assemble<64>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
+                        d_dmask, d_g, d_vals, d_b, d_dinv, dia);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
 }
 
 }  // namespace jno
@@ -195,18 +252,48 @@ extern "C" int jno_assemble_p1(int kind, const double* d_points, const int32_t* 
               "jno_assemble_p1: null pointer");
   (void)n_cells;
   if (n_nodes == 0) return JNO_OK;
+  DiaOut none{};
+  return assemble_any(kind, d_points, d_cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, max_row, d_k, f_const,
+                      d_f_nodal, d_dmask, d_g, d_vals, d_b, d_dinv, none, as_stream(stream));
+}
+
+extern "C" int jno_assemble_p1_dia(int kind, const double* d_points, const int32_t* d_cells, int64_t n_cells,
+                                   int64_t n_nodes, const int32_t* d_inc_ptr, const int32_t* d_inc,
+                                   const int32_t* d_rowptr, const int32_t* d_col, int max_row, const double* d_k,
+                                   double f_const, const double* d_f_nodal, const uint8_t* d_dmask,
+                                   const double* d_g, double* d_vals, double* d_b, double* d_dinv, int ndiag,
+                                   const int32_t* h_offsets, double* d_dia64, float* d_dia32, int* h_ok, void* d_ws,
+                                   size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(kind == 0 || kind == 1, JNO_EINVAL, "jno_assemble_p1_dia: kind must be 0 (stiffness) or 1 (mass)");
+  JNO_REQUIRE(kind == 1 || d_k, JNO_EINVAL, "jno_assemble_p1_dia: stiffness needs the per-element coefficient k");
+  JNO_REQUIRE(max_row >= 0 && max_row <= 64, JNO_ECAPACITY,
+              "jno_assemble_p1_dia: a row has %d entries; the P1 row-pull kernel holds at most 64", max_row);
+  JNO_REQUIRE(d_points && d_cells && d_rowptr && d_col && d_vals && d_inc_ptr && d_inc && d_dia64 && h_ok && d_ws,
+              JNO_EINVAL, "jno_assemble_p1_dia: null pointer");
+  JNO_REQUIRE(ndiag >= 1 && ndiag <= JNO_DIA_MAX && h_offsets && h_offsets[0] == 0, JNO_EINVAL,
+              "jno_assemble_p1_dia: bad offsets (1..%d, the main diagonal first)", JNO_DIA_MAX);
+  JNO_REQUIRE(ws_bytes >= 256, JNO_EWORKSPACE, "jno_assemble_p1_dia: workspace too small (256 bytes)");
+  (void)n_cells;
+  *h_ok = 0;
+  if (n_nodes == 0) {
+    *h_ok = 1;
+    return JNO_OK;
+  }
   hipStream_t s = as_stream(stream);
-  const int4* cells = reinterpret_cast<const int4*>(d_cells);
-  if (max_row <= 16)
-    launch_assemble<16>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
-                        d_dmask, d_g, d_vals, d_b, d_dinv);
-  else if (max_row <= 32)
-    launch_assemble<32>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
-                        d_dmask, d_g, d_vals, d_b, d_dinv);
-  else
-    launch_assemble<64>(s, kind, d_points, cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, d_k, f_const, d_f_nodal,
-                        d_dmask, d_g, d_vals, d_b, d_dinv);
-  JNO_LAUNCH_CHECK();
+  DiaOut dia{};
+  dia.ndiag = ndiag;
+  for (int d = 0; d < JNO_DIA_MAX; ++d) dia.off.o[d] = d < ndiag ? h_offsets[d] : -1;
+  dia.d64 = d_dia64;
+  dia.d32 = d_dia32;
+  dia.bad = static_cast<int*>(d_ws);
+  JNO_HIP(hipMemsetAsync(dia.bad, 0, sizeof(int), s));
+  int rc = assemble_any(kind, d_points, d_cells, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col, max_row, d_k, f_const,
+                        d_f_nodal, d_dmask, d_g, d_vals, d_b, d_dinv, dia, s);
+  if (rc) return rc;
+  int h_bad = 1;
+  JNO_HIP(hipMemcpyAsync(&h_bad, dia.bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *h_ok = h_bad ? 0 : 1;
   return JNO_OK;
 }
 

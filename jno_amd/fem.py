@@ -223,12 +223,35 @@ class FemLinearSystem:
             A = out
             A._cache.clear()
         b = torch.empty(N, dtype=torch.float64, device=m.device)
+        from .operator import SolverOperator, chosen_dia_offsets
+
+        dia = chosen_dia_offsets(A)
+        if dia is None:
+            _lib.call(
+                "jno_assemble_p1", 0, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
+                _lib.ptr(plan.inc), _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.max_row, _lib.ptr(k), self.f_const,
+                _lib.ptr(self.f_nodal), _lib.ptr(self.dmask), _lib.ptr(self.g), _lib.ptr(A.vals), _lib.ptr(b),
+                _lib.ptr(A.dinv), _lib.stream_ptr(m.device),
+            )
+            return A, b
+        # the solver's DIA layout (offsets found on an earlier operator of this plan) in the same pass
+        nd, offs = dia
+        ld = (N + 63) // 64 * 64
+        d64 = torch.empty(nd * ld, dtype=torch.float64, device=m.device)
+        d32 = torch.empty(nd * ld, dtype=torch.float32, device=m.device)
+        ws = torch.empty(256, dtype=torch.uint8, device=m.device)
+        ok = C.c_int(0)
         _lib.call(
-            "jno_assemble_p1", 0, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
+            "jno_assemble_p1_dia", 0, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
             _lib.ptr(plan.inc), _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.max_row, _lib.ptr(k), self.f_const,
             _lib.ptr(self.f_nodal), _lib.ptr(self.dmask), _lib.ptr(self.g), _lib.ptr(A.vals), _lib.ptr(b),
-            _lib.ptr(A.dinv), _lib.stream_ptr(m.device),
+            _lib.ptr(A.dinv), nd, (C.c_int32 * _lib.JNO_DIA_MAX)(*offs), _lib.ptr(d64), _lib.ptr(d32),
+            C.byref(ok), _lib.ptr(ws), 256, _lib.stream_ptr(m.device),
         )
+        if ok.value:
+            op = SolverOperator.from_dia(A, nd, offs, d64, d32)
+            A._cache[("solver_op", "auto", True)] = op
+            A._cache[("solver_op", "auto", False)] = op
         return A, b
 
     def mass(self) -> CSR:

@@ -33,6 +33,13 @@ def _pattern_key(A: CSR):
     return (A.rowptr.data_ptr(), A.col.data_ptr(), A.shape[0], A.nnz)
 
 
+def chosen_dia_offsets(A: CSR):
+    """(ndiag, offsets) that the format choice picked for an earlier operator on A's pattern, or
+    None.  The assembly uses them to write the DIA layout in the same pass."""
+    hit = _DIA_OFFSETS.get(_pattern_key(A))
+    return (hit[0], hit[1]) if hit is not None and hit[2] else None
+
+
 class SolverOperator:
     def __init__(self, A: CSR, fmt: str = "auto", lanes: Optional[int] = None, fp32: bool = True):
         self.A = A
@@ -68,6 +75,19 @@ class SolverOperator:
         else:
             self._build_csr(A.drop_zeros(), lanes or 4, fp32)
         self.format = fmt
+
+    @classmethod
+    def from_dia(cls, A: CSR, nd, offs, d64, d32):
+        """Operator on DIA buffers that were filled elsewhere (the fused assembly)."""
+        op = cls.__new__(cls)
+        op.A = A
+        op.n = A.shape[0]
+        op.device = A.device
+        op._keep = []
+        op.op = _lib.JnoOp()
+        op._set_dia(nd, list(offs), d64, d32)
+        op.format = "dia"
+        return op
 
     @staticmethod
     def analyze(A: CSR):

@@ -330,6 +330,50 @@ def test_dia_offsets_reused_with_checked_fill(dev, monkeypatch):
     assert _rel(_np(op4.mv(x)), _np(A4.mv(x))) < 1e-14
 
 
+def test_assembly_writes_dia_layout(dev):
+    """Once a solve has picked DIA for a plan, re-assembly writes the layout in the same pass: the
+    diagonals equal bitwise those filled from the CSR.  Moving the mesh points off the lattice
+    (same plan) puts nonzeros off the stored diagonals: the fused check refuses the layout and the
+    solve still matches the oracle."""
+    import jno_amd as J
+    from jno_amd import precond, solve
+    from jno_amd.operator import SolverOperator
+
+    n = 10
+    p, t = O.kuhn_box(n)
+    m = J.kuhn_box(n)
+    s = J.fem_poisson(m)
+    k = torch.tensor(O.k_lognormal(t.shape[0], 31), device="cuda")
+    A0, b0 = s.evaluate({"k": k})
+    sv = solve.cg(tol=1e-10, precision="mixed")
+    sv(A0, b0, M=precond.jacobi())  # picks (and remembers) the DIA offsets of this plan
+    A, b = s.evaluate({"k": k * 1.5})
+    fused = A._cache.get(("solver_op", "auto", True))
+    assert fused is not None and fused.format == "dia"
+    ref = SolverOperator(A, fmt="dia")
+    ld = (A.shape[0] + 63) // 64 * 64
+    assert fused.offsets == ref.offsets
+    for f_buf, r_buf in ((fused._keep[0], ref._keep[0]), (fused._keep[1], ref._keep[1])):
+        fb = f_buf.reshape(fused.ndiag, ld)[:, : A.shape[0]].contiguous()
+        rb = r_buf.reshape(ref.ndiag, ld)[:, : A.shape[0]].contiguous()
+        assert np.array_equal(_np(fb).view(np.uint8), _np(rb).view(np.uint8))
+    u = sv(A, b, M=precond.jacobi())
+    Ao, bo, _ = O.poisson_p1_system(p, t, 1.5 * _np(k))
+    assert _rel(_np(u), O.direct_solve(Ao, bo)) <= SOL_TOL
+    # off the lattice: interior points moved by up to 0.2 h, same cells and plan
+    rng = np.random.default_rng(3)
+    interior = np.ones(p.shape[0], bool)
+    interior[O.boundary_nodes(p, t)] = False
+    pj = p.copy()
+    pj[interior] += rng.uniform(-0.2, 0.2, (interior.sum(), 3)) / n
+    m.points.copy_(torch.tensor(pj, device="cuda"))
+    A2, b2 = s.evaluate({"k": k})
+    assert ("solver_op", "auto", True) not in A2._cache
+    u2 = sv(A2, b2, M=precond.jacobi())
+    Ao2, bo2, _ = O.poisson_p1_system(pj, t, _np(k))
+    assert _rel(_np(u2), O.direct_solve(Ao2, bo2)) <= SOL_TOL
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_iteration_modes_and_launch_shapes(dev, precision):
     """Every launch shape of the PCG iteration kernels solves to the oracle's direct solution:
