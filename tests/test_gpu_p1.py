@@ -486,3 +486,44 @@ def test_adjoint_gradient_vs_oracle_and_fd(dev):
         dL = torch.dot(e0, M.mv(dp - dm)) + 0.5 * (torch.dot(dp, M.mv(dp)) - torch.dot(dm, M.mv(dm)))
         fd = float(dL) / (2 * h)
         assert abs(fd - grad[el]) <= 1e-6 * abs(grad[el]) + 1e-15, (el, fd, grad[el])
+
+
+class _Exporter:
+    """A foreign DLPack exporter (what a JAX array presents): only __dlpack__ and its device."""
+
+    def __init__(self, t):
+        self._t = t
+
+    def __dlpack__(self, *args, **kw):
+        return self._t.__dlpack__(*args, **kw)
+
+    def __dlpack_device__(self):
+        return self._t.__dlpack_device__()
+
+
+def test_linear_slot_over_dlpack(dev):
+    """The reference's linear= slot signature with everything passed by DLPack: the operator as
+    BCOO triplets with the Dirichlet unit diagonal appended as extra triplets (the reference's
+    parametric operator, jno/utils/solver/fem_utils.py:3332-3359), b as an exporter; the solution
+    matches the oracle and leaves through DLPack without a copy."""
+    from jno_amd import interop, precond, solve
+
+    n = 8
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 41)
+    m, s, A, b = _system(n, k)
+    data, idx = A.bcoo
+    mask = s.dmask.bool()
+    rows, cols = idx[:, 0].long(), idx[:, 1].long()
+    data2 = data.clone()
+    data2[(rows == cols) & mask[rows]] = 0.0  # constrained diagonals zeroed in place ...
+    dd = torch.nonzero(mask).flatten().to(torch.int32)
+    data3 = torch.cat([data2, torch.ones(dd.numel(), dtype=torch.float64, device="cuda")])
+    idx3 = torch.cat([idx, torch.stack([dd, dd], dim=1)])  # ... and the unit diagonal appended
+    for precision in ("fp64", "mixed"):
+        slot = interop.linear_slot(solve.cg(tol=1e-10, precision=precision))
+        x = slot((_Exporter(data3), _Exporter(idx3), A.shape), _Exporter(b), M=precond.jacobi())
+        Ao, bo, _ = O.poisson_p1_system(p, t, k)
+        assert _rel(_np(x), O.direct_solve(Ao, bo)) <= SOL_TOL, precision
+        back = torch.from_dlpack(_Exporter(x))
+        assert back.data_ptr() == x.data_ptr()
