@@ -220,7 +220,7 @@ size_t jno_pcg_workspace_size(int64_t n, int mixed);
  * Jacobi-scaled FP32 copy of the operator that the scaled inner iteration (tuning mode 3) streams;
  * a workspace of jno_pcg_workspace_size(n, 1) bytes runs the unscaled inner iteration instead. */
 size_t jno_pcg_workspace_size_op(const jno_op* h_op, int mixed);
-/* Iterations enqueued per host poll of the device-side convergence state (default 8).  No
+/* Iterations enqueued per host poll of the device-side convergence state (default 16).  No
  * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
 int jno_pcg_set_chunk(int chunk);
 /* Launch shape of the iteration kernels, per iteration precision (no reference counterpart; set
