@@ -12,6 +12,12 @@
 //   KS  ITER     q = A32 p, partial p.q
 //   KB  ITER     r -= alpha q (alpha from the reduced p.q), partials r.z, r.r
 //       REPLACE  r64 = b - A64 x64, rs = (float) r64, partials ||r64||^2, r.z, r.r, b.b
+// DIA items run on their Jacobi-scaled operator S A S (S = diag(sqrt(dinv)), unit main diagonal, as
+// the single-system mixed solver does): p = r + beta p and r.z = r.r stream no dinv and the SpMV no
+// main diagonal (48 instead of 60 bytes per row and step); x = S y and r = S (b - A x) are formed
+// in FP64 at the replacements, and the inner loop stops only to replace (the last cycle aimed at
+// the remaining reduction), so the FP64 true residual alone decides convergence.  An item whose
+// (float)(A_ii dinv_i) is not exactly 1 somewhere, and every CSR item, runs unscaled.
 // Each entry reduces its item's partials itself in a fixed order, so no value crosses workgroups
 // inside a kernel (no atomics or cache-flushing fences on the step path) and all entries of an
 // item take the same decision.  Per-item fields are written by the item's first entry only, into
@@ -27,9 +33,10 @@
 
 namespace jno {
 
-constexpr int BT_NT = 256;
-constexpr int BT_RPB = 1024;   // rows per table entry
-constexpr int BT_GRID = 2048;  // grid-stride workgroups per launch
+constexpr int BT_NT = 512;
+constexpr int BT_RPB = 2048;   // rows per table entry (every entry reduces its item's partials: fewer,
+                               // larger entries cut that; 1024 rows measured 9% slower at C4, 4096 4%)
+constexpr int BT_GRID = 8192;  // grid-stride workgroups per launch (2048: 8% slower at C4)
 
 enum { PH_ITER = 0, PH_REPLACE = 1, PH_DONE = 2 };
 enum { BS_RUNNING = 0, BS_CONVERGED = 1, BS_MAXITER = 2, BS_NONFINITE = 3 };
@@ -45,6 +52,9 @@ struct BItem {
   const double* b;
   double* x;
   int32_t e0, ne;  // entry range in the table
+  const float* s32;      // scaled operator: the ndiag - 1 stored upper diagonals of S A S
+  const double* dinv64;  // the item's FP64 Jacobi inverse diagonal
+  int32_t scaled, pad;
 };
 
 struct BState {
@@ -94,6 +104,9 @@ struct ItemView {
   const double* b;
   double* x;
   int32_t e0, ne;
+  const float* s32;
+  const double* dinv64;
+  bool scaled;
 };
 
 __device__ __forceinline__ ItemView item_view(const BItem* items, int i) {
@@ -112,7 +125,24 @@ __device__ __forceinline__ ItemView item_view(const BItem* items, int i) {
   v.x = s.x;
   v.e0 = s.e0;
   v.ne = s.ne;
+  v.s32 = s.s32;
+  v.dinv64 = s.dinv64;
+  v.scaled = s.scaled != 0;
   return v;
+}
+
+// row i of the scaled operator S A S (unit main diagonal; the stored upper diagonals of s32, the
+// lower ones through symmetry)
+__device__ __forceinline__ float scaled_row(const ItemView& it, int64_t i, const float* x) {
+  const int64_t ld = dia_ld(it.n);
+  float s = x[i];
+  for (int d = 1; d < it.ndiag; ++d) {
+    const int64_t o = it.offs[d];
+    const float* v = it.s32 + (int64_t)(d - 1) * ld;
+    if (i + o < it.n) s += v[i] * x[i + o];
+    if (i - o >= 0) s += v[i - o] * x[i - o];
+  }
+  return s;
 }
 
 template <typename T, typename V>
@@ -180,8 +210,11 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
         phase = PH_ITER;
       }
       if (first && threadIdx.x == 0) {
-        S->atol2 = atol2;
-        S->rr_replace = a.repl2 * rr;
+        // scaled items: the inner loop stops only to replace, the last cycle aimed at the
+        // remaining reduction (with margin) when that is smaller than a whole cycle
+        const double need = atol2 / rt;
+        S->atol2 = it.scaled ? 0.0 : atol2;
+        S->rr_replace = (it.scaled && need > a.repl2) ? 0.5 * need * rr : a.repl2 * rr;
         S->rr_true = rt;
         S->bb = bb;
         S->rr_last = rr;
@@ -207,7 +240,7 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
       for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
         float d = xs[i];
         if (k > 0 && alpha_prev != 0.0) d += ap * pold[i];
-        it.x[i] += (double)d;
+        it.x[i] += it.scaled ? sqrt(it.dinv64[i]) * (double)d : (double)d;  // x = S y when scaled
         xs[i] = 0.0f;
       }
       continue;
@@ -220,14 +253,21 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
     static_assert(BT_RPB == 4 * BT_NT, "one 4-row tile per thread");
     const int64_t t0 = r0 + 4 * (int64_t)threadIdx.x;
     if (t0 + 4 <= r1) {
-      const float4 dq = *reinterpret_cast<const float4*>(dv + t0);
+      const float4 dq = it.scaled ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : *reinterpret_cast<const float4*>(dv + t0);
       const float4 rq = *reinterpret_cast<const float4*>(r + t0);
       const float4 po = *reinterpret_cast<const float4*>(pold + t0);
       float4 pi;
-      pi.x = dq.x * rq.x + beta * po.x;
-      pi.y = dq.y * rq.y + beta * po.y;
-      pi.z = dq.z * rq.z + beta * po.z;
-      pi.w = dq.w * rq.w + beta * po.w;
+      if (it.scaled) {  // z = r
+        pi.x = rq.x + beta * po.x;
+        pi.y = rq.y + beta * po.y;
+        pi.z = rq.z + beta * po.z;
+        pi.w = rq.w + beta * po.w;
+      } else {
+        pi.x = dq.x * rq.x + beta * po.x;
+        pi.y = dq.y * rq.y + beta * po.y;
+        pi.z = dq.z * rq.z + beta * po.z;
+        pi.w = dq.w * rq.w + beta * po.w;
+      }
       *reinterpret_cast<float4*>(pnew + t0) = pi;
       if (k > 0) {
         float4 xq = *reinterpret_cast<const float4*>(xs + t0);
@@ -240,7 +280,7 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
     } else {
       for (int64_t i = t0; i < min(r1, t0 + 4); ++i) {
         const float po = pold[i];
-        pnew[i] = dv[i] * r[i] + beta * po;
+        pnew[i] = (it.scaled ? r[i] : dv[i] * r[i]) + beta * po;
         if (k > 0) xs[i] += ap * po;
       }
     }
@@ -263,7 +303,41 @@ __global__ void __launch_bounds__(BT_NT) batch_ks(BArgs a) {
     const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
     double part = 0.0;
     int64_t i0 = r0 + threadIdx.x;
-    if (DIA && it.ndiag == 4) {
+    if (DIA && it.scaled && it.ndiag == 4) {
+      // the 3-D Kuhn lattice on the scaled operator: unit diagonal, 3 stored upper diagonals
+      constexpr int BR = 2;
+      const int64_t oo[3] = {it.offs[1], it.offs[2], it.offs[3]};
+      const int64_t omax = max(max(oo[0], oo[1]), oo[2]);
+      const int64_t ld = dia_ld(it.n);
+      for (; i0 + (BR - 1) * BT_NT < r1 && i0 - omax >= 0 && i0 + (BR - 1) * BT_NT + omax < it.n; i0 += BR * BT_NT) {
+        float pv[BR], vu[BR][3], vl[BR][3], pu[BR][3], pl[BR][3];
+#pragma unroll
+        for (int t = 0; t < BR; ++t) {
+          const int64_t i = i0 + t * BT_NT;
+          pv[t] = p[i];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const int64_t o = oo[d];
+            const float* v = it.s32 + (int64_t)d * ld;
+            vu[t][d] = v[i];
+            vl[t][d] = v[i - o];
+            pu[t][d] = p[i + o];
+            pl[t][d] = p[i - o];
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < BR; ++t) {
+          float sacc = pv[t];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            sacc += vu[t][d] * pu[t][d];
+            sacc += vl[t][d] * pl[t][d];
+          }
+          q[i0 + t * BT_NT] = sacc;
+          part += (double)pv[t] * (double)sacc;
+        }
+      }
+    } else if (DIA && it.ndiag == 4) {
       // the 3-D Kuhn lattice: BR rows per thread with all 14 BR loads in flight before the first use
       constexpr int BR = 2;
       const int64_t oo[3] = {it.offs[1], it.offs[2], it.offs[3]};
@@ -300,7 +374,7 @@ __global__ void __launch_bounds__(BT_NT) batch_ks(BArgs a) {
       }
     }
     for (int64_t i = i0; i < r1; i += BT_NT) {
-      const float s = mat_row<float, float>(it, i, it.v32, p, DIA);
+      const float s = (DIA && it.scaled) ? scaled_row(it, i, p) : mat_row<float, float>(it, i, it.v32, p, DIA);
       q[i] = s;
       part += (double)p[i] * (double)s;
     }
@@ -340,7 +414,8 @@ __global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
       if (t0 + 4 <= r1) {
         float4 rv = *reinterpret_cast<const float4*>(r + t0);
         const float4 qv = *reinterpret_cast<const float4*>(q + t0);
-        const float4 dq = *reinterpret_cast<const float4*>(dv + t0);
+        // z = r on the scaled operator: r.z = r.r
+        const float4 dq = it.scaled ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : *reinterpret_cast<const float4*>(dv + t0);
         rv.x = rv.x - al * qv.x;
         rv.y = rv.y - al * qv.y;
         rv.z = rv.z - al * qv.z;
@@ -358,7 +433,7 @@ __global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
         for (int64_t i = t0; i < min(r1, t0 + 4); ++i) {
           const float ri = r[i] - al * q[i];
           r[i] = ri;
-          pg += (double)ri * (double)(dv[i] * ri);
+          pg += (double)ri * (double)(it.scaled ? ri : dv[i] * ri);
           pr += (double)ri * (double)ri;
         }
       }
@@ -375,11 +450,11 @@ __global__ void __launch_bounds__(BT_NT) batch_kb(BArgs a) {
       for (int64_t i = r0 + threadIdx.x; i < r1; i += BT_NT) {
         const double bi = it.b[i];
         const double ri = bi - mat_row<double, double>(it, i, it.v64, it.x, DIA);
-        const float rf = (float)ri;
+        const float rf = it.scaled ? (float)(ri * sqrt(it.dinv64[i])) : (float)ri;  // r = S r64 when scaled
         r[i] = rf;
         prt += ri * ri;
         pb += bi * bi;
-        pg += (double)rf * (double)(dv[i] * rf);
+        pg += (double)rf * (double)(it.scaled ? rf : dv[i] * rf);
         pr += (double)rf * (double)rf;
       }
       put(prt, a.PB, entry, sm);
@@ -429,6 +504,32 @@ __global__ void batch_dinv_k(const BItem* items, const double* const* dinvs, con
   }
 }
 
+// Scaled operator of the DIA items, per entry's rows: s32[d-1][i] = (float)(A[i, i+o_d] s_i s_{i+o_d})
+// (s = sqrt(dinv)) for the stored upper diagonals d >= 1; bad[item] is set where (float)(A_ii dinv_i)
+// is not exactly 1 (that item then runs unscaled)
+__global__ void batch_scale_k(const BItem* items, const double* const* dinvs, const int2* table, int n_entries,
+                              int* bad) {
+  for (int entry = blockIdx.x; entry < n_entries; entry += gridDim.x) {
+    const int2 te = table[entry];
+    const BItem& it = items[te.x];
+    if (it.format != JNO_FMT_DIA || !it.s32) continue;
+    const double* dv = dinvs[te.x];
+    const int64_t ld = dia_ld(it.n);
+    float* s32 = const_cast<float*>(it.s32);
+    const int64_t r0 = (int64_t)te.y * BT_RPB, r1 = min(it.n, r0 + BT_RPB);
+    bool ok = true;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+      ok = ok && (float)(it.v64[i] * dv[i]) == 1.0f;
+      const double si = sqrt(dv[i]);
+      for (int d = 1; d < it.ndiag; ++d) {
+        const int64_t o = it.offs[d];
+        s32[(int64_t)(d - 1) * ld + i] = i + o < it.n ? (float)(it.v64[(int64_t)d * ld + i] * si * sqrt(dv[i + o])) : 0.0f;
+      }
+    }
+    if (!ok) atomicOr(&bad[te.x], 1);
+  }
+}
+
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct BLayout {
@@ -439,11 +540,13 @@ struct BLayout {
   double* PA;
   double* PB;
   int32_t* active;
+  int32_t* bad;  // [n_items]: the item's operator cannot be Jacobi-scaled exactly
   float *rs, *p0, *p1, *q, *xs, *dinv;
+  float* sdia;   // scaled upper diagonals of the DIA items, one block per item
   size_t bytes;
 };
 
-static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, char* base) {
+static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, int64_t nsd, char* base) {
   BLayout L{};
   size_t off = 0;
   auto take = [&](size_t b) -> char* {
@@ -464,20 +567,27 @@ static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, char* base)
   L.q = (float*)take(4 * ntot);
   L.xs = (float*)take(4 * ntot);
   L.dinv = (float*)take(4 * ntot);
+  L.bad = (int32_t*)take(sizeof(int32_t) * n_items);
+  L.sdia = (float*)take(4 * nsd);
   L.bytes = off;
   return L;
 }
 
+// FP32 entries of the scaled copies of the DIA items' stored upper diagonals
+static int64_t scaled_rows(const jno_op& op) { return op.format == JNO_FMT_DIA ? (op.ndiag - 1) * dia_ld(op.n) : 0; }
+
 // each item's rows start 64-aligned in the concatenated FP32 vectors (16-byte vector tiles)
 static int64_t padded_rows(int64_t n) { return (n + 63) & ~(int64_t)63; }
 
-static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot) {
-  int64_t ne = 0, nt = 0;
+static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot, int64_t* nsd) {
+  int64_t ne = 0, nt = 0, ns = 0;
   for (int i = 0; i < n; ++i) {
     ne += cdiv(items[i].op.n, BT_RPB);
     nt += padded_rows(items[i].op.n);
+    ns += scaled_rows(items[i].op);
   }
   if (ntot) *ntot = nt;
+  if (nsd) *nsd = ns;
   return ne;
 }
 
@@ -487,9 +597,9 @@ using namespace jno;
 
 extern "C" size_t jno_batch_workspace_size(int n_items, const jno_batch_item* h_items) {
   if (n_items <= 0 || !h_items) return 0;
-  int64_t ntot;
-  const int64_t ne = count_entries(n_items, h_items, &ntot);
-  return blayout(n_items, ne, ntot, nullptr).bytes;
+  int64_t ntot, nsd;
+  const int64_t ne = count_entries(n_items, h_items, &ntot, &nsd);
+  return blayout(n_items, ne, ntot, nsd, nullptr).bytes;
 }
 
 extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, double tol, double atol, int64_t maxiter,
@@ -498,18 +608,18 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
   JNO_REQUIRE(n_items > 0 && h_items && h_stats, JNO_EINVAL, "jno_batch_pcg_mixed: empty batch");
   JNO_REQUIRE(replace_factor > 0.0 && replace_factor < 1.0, JNO_EINVAL, "jno_batch_pcg_mixed: replace_factor in (0,1)");
   JNO_REQUIRE(maxiter >= 0 && steps_per_graph >= 0, JNO_EINVAL, "jno_batch_pcg_mixed: bad maxiter/steps");
-  int64_t ntot;
-  const int64_t ne = count_entries(n_items, h_items, &ntot);
+  int64_t ntot, nsd;
+  const int64_t ne = count_entries(n_items, h_items, &ntot, &nsd);
   JNO_REQUIRE(ne < INT32_MAX, JNO_EINVAL, "jno_batch_pcg_mixed: batch too large");
   JNO_REQUIRE(ws_bytes >= jno_batch_workspace_size(n_items, h_items), JNO_EWORKSPACE,
               "jno_batch_pcg_mixed: workspace too small");
   hipStream_t s = as_stream(stream);
-  BLayout L = blayout(n_items, ne, ntot, static_cast<char*>(d_ws));
+  BLayout L = blayout(n_items, ne, ntot, nsd, static_cast<char*>(d_ws));
   // host staging of the item descriptors and the entry table
   std::vector<BItem> items(n_items);
   std::vector<int2> table(ne);
   std::vector<const double*> dinvs(n_items);
-  int64_t off = 0, e = 0;
+  int64_t off = 0, e = 0, soff = 0;
   bool any_dia = false, any_csr = false;
   for (int i = 0; i < n_items; ++i) {
     const jno_batch_item& h = h_items[i];
@@ -534,6 +644,10 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
     it.ne = (int32_t)cdiv(it.n, BT_RPB);
     for (int j = 0; j < it.ne; ++j) table[e++] = make_int2(i, j);
     dinvs[i] = h.dinv;
+    it.dinv64 = h.dinv;
+    it.s32 = it.format == JNO_FMT_DIA ? L.sdia + soff : nullptr;
+    it.scaled = 0;
+    soff += scaled_rows(h.op);
     off += padded_rows(it.n);
     any_dia |= it.format == JNO_FMT_DIA;
     any_csr |= it.format == JNO_FMT_CSR;
@@ -566,6 +680,17 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
   JNO_LAUNCH_CHECK();
   batch_dinv_k<<<grid, BT_NT, 0, s>>>(L.items, L.dinvs, L.table, (int)ne, L.dinv);
   JNO_LAUNCH_CHECK();
+  if (any_dia) {
+    // the DIA items' scaled operators; an item whose Jacobi scaling is not exact runs unscaled
+    JNO_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t) * n_items, s));
+    batch_scale_k<<<grid, BT_NT, 0, s>>>(L.items, L.dinvs, L.table, (int)ne, L.bad);
+    JNO_LAUNCH_CHECK();
+    std::vector<int32_t> bad(n_items);
+    JNO_HIP(hipMemcpyAsync(bad.data(), L.bad, sizeof(int32_t) * n_items, hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n_items; ++i) items[i].scaled = items[i].format == JNO_FMT_DIA && !bad[i];
+    JNO_HIP(hipMemcpyAsync(L.items, items.data(), sizeof(BItem) * n_items, hipMemcpyHostToDevice, s));
+  }
   JNO_HIP(hipStreamSynchronize(s));  // host staging vectors go out of scope after this call
 
   auto launch_kb = [&](hipStream_t st) -> int {
