@@ -1123,14 +1123,28 @@ __global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
   const T* p = a.p_new;
   const int sub = threadIdx.x % L;
   double part = 0.0;
-  for (int64_t R = r0 / 3 + threadIdx.x / L; R < r1 / 3; R += NT / L) {
-    T s0, s1, s2;
-    bsr3_row<L, V, T>(a.rowptr, a.col, a.vals, p, R, sub, s0, s1, s2);
+  // U block rows per lane group in flight (their loads issued together), NT / L groups apart: two
+  // for FP32 (C5: 7.6% faster than one), one for FP64 (its registers would cut the occupancy)
+  constexpr int U = sizeof(T) == 4 ? 2 : 1;
+  constexpr int64_t STEP = NT / L;
+  const int64_t R1 = r1 / 3;
+  for (int64_t R0 = r0 / 3 + threadIdx.x / L; R0 < R1; R0 += U * STEP) {
+    int64_t Rs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) Rs[u] = R0 + u * STEP < R1 ? R0 + u * STEP : -1;
+    T sr[U][3];
+    bsr3_rows<L, U, V, T>(a.rowptr, a.col, a.vals, p, Rs, sub, sr);
     if (sub == 0) {
-      sto(&a.q[3 * R], s0);
-      sto(&a.q[3 * R + 1], s1);
-      sto(&a.q[3 * R + 2], s2);
-      part += (double)p[3 * R] * (double)s0 + (double)p[3 * R + 1] * (double)s1 + (double)p[3 * R + 2] * (double)s2;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (Rs[u] < 0) continue;
+        const int64_t R = Rs[u];
+        sto(&a.q[3 * R], sr[u][0]);
+        sto(&a.q[3 * R + 1], sr[u][1]);
+        sto(&a.q[3 * R + 2], sr[u][2]);
+        part += (double)p[3 * R] * (double)sr[u][0] + (double)p[3 * R + 1] * (double)sr[u][1] +
+                (double)p[3 * R + 2] * (double)sr[u][2];
+      }
     }
   }
   store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);

@@ -31,13 +31,23 @@ __global__ void __launch_bounds__(256) bsr3_spmv_k(int64_t nb, const int32_t* __
                                                    const X* __restrict__ x, X* __restrict__ y) {
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int sub = threadIdx.x % L;
-  for (int64_t R = gid / L; R < nb; R += (int64_t)gridDim.x * blockDim.x / L) {
-    X s0, s1, s2;
-    bsr3_row<L, V, X>(browptr, bcol, bvals, x, R, sub, s0, s1, s2);
+  // U block rows per lane group in flight, as in the PCG's BSR3 kernel (two for FP32, one for FP64)
+  constexpr int U = sizeof(X) == 4 ? 2 : 1;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x / L;
+  for (int64_t R0 = gid / L; R0 < nb; R0 += U * step) {
+    int64_t Rs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) Rs[u] = R0 + u * step < nb ? R0 + u * step : -1;
+    X sr[U][3];
+    bsr3_rows<L, U, V, X>(browptr, bcol, bvals, x, Rs, sub, sr);
     if (sub == 0) {
-      y[3 * R] = s0;
-      y[3 * R + 1] = s1;
-      y[3 * R + 2] = s2;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (Rs[u] < 0) continue;
+        y[3 * Rs[u]] = sr[u][0];
+        y[3 * Rs[u] + 1] = sr[u][1];
+        y[3 * Rs[u] + 2] = sr[u][2];
+      }
     }
   }
 }
