@@ -108,3 +108,26 @@ def tune_launch(A, b, precision: str = "mixed", iters: int = 100, shapes=CANDIDA
     _LAUNCH_CACHE[key] = best[1]
     set_tuning(precision, *best[1])
     return best[1]
+
+
+def tune_batch(systems, chunks=(4, 8, 16), **solver_kw) -> int:
+    """Steps per hipGraph replay for :class:`jno_amd.batch.BatchSolver` on ``systems`` (a list of
+    ``(A, b)``): each candidate is timed on one whole batched solve (after a warm solve that captures
+    its graph), and the fastest is returned.  Longer chunks mean fewer host polls but more step
+    launches after the last item converged."""
+    import time
+
+    from .batch import BatchSolver
+
+    best = None
+    for c in chunks:
+        bs = BatchSolver(steps_per_graph=int(c), **solver_kw)
+        bs(systems)  # warm: graph capture, operator layouts
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bs(systems)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        if best is None or t < best[0]:
+            best = (t, int(c))
+    return best[1]

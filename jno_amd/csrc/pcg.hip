@@ -1394,10 +1394,6 @@ __global__ void accumulate_scaled_k(int64_t n, double* __restrict__ x, float* __
   }
 }
 
-__global__ void set_replace_k(PcgState* st, double rr_replace) {
-  if (threadIdx.x == 0) st->rr_replace = rr_replace;
-}
-
 // The outer step of the Jacobi-scaled mixed solve, on the device so that no host round trip sits
 // between replacement cycles.  After the FP64 residual of the accumulated x was prepared: reduce
 // ||r64||^2, b.b and the new FP32 r.r, then either stop (converged on the true residual; the inner

@@ -527,3 +527,45 @@ def test_linear_slot_over_dlpack(dev):
         assert _rel(_np(x), O.direct_solve(Ao, bo)) <= SOL_TOL, precision
         back = torch.from_dlpack(_Exporter(x))
         assert back.data_ptr() == x.data_ptr()
+
+
+def test_autotuner_choices(dev):
+    """The format choice (DIA on the lattice, CSR with a lane count off it), the launch-shape tuner
+    (installs the fastest of the candidates, and the solve still matches the oracle) and the
+    batch chunk tuner."""
+    from jno_amd import autotune, precond, solve
+    from jno_amd.batch import BatchSolver
+
+    n = 9
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 51)
+    m, s, A, b = _system(n, k)
+    assert autotune.choose_format(A) == ("dia", 1)
+    # off the lattice the non-axis couplings are nonzero: more diagonals, or CSR with tuned lanes
+    pj = p.copy()
+    interior = np.ones(p.shape[0], bool)
+    interior[O.boundary_nodes(p, t)] = False
+    pj[interior] += np.random.default_rng(5).uniform(-0.2, 0.2, (interior.sum(), 3)) / n
+    import jno_amd as J
+
+    mj = J.Mesh(points=torch.tensor(pj, device="cuda"), cells=m.cells)
+    Aj, bj = J.fem_poisson(mj).evaluate({"k": torch.tensor(k, device="cuda")})
+    fmt, lanes = autotune.choose_format(Aj, allow=("csr",))
+    assert fmt == "csr" and lanes in autotune.CANDIDATE_LANES
+    shapes = ((256, 64, 0, 5), (512, 128, 3, 4), (1024, 64, 0, 1))
+    saved = solve.get_tuning("mixed")
+    try:
+        best = autotune.tune_launch(A, b, precision="mixed", iters=40, shapes=shapes)
+        assert best in shapes and solve.get_tuning("mixed") == best
+        u = solve.cg(tol=1e-10, precision="mixed")(A, b, M=precond.jacobi())
+        Ao, bo, _ = O.poisson_p1_system(p, t, k)
+        assert _rel(_np(u), O.direct_solve(Ao, bo)) <= SOL_TOL
+    finally:
+        solve.set_tuning("mixed", *saved)
+    systems = [_system(nn, O.k_lognormal(6 * nn**3, 60 + nn))[2:] for nn in (5, 7, 6)]
+    c = autotune.tune_batch(systems, chunks=(2, 8), tol=1e-10)
+    assert c in (2, 8)
+    xs = BatchSolver(tol=1e-10, steps_per_graph=c)(systems)
+    for (Ai, bi), xi in zip(systems, xs):
+        # the solver stops on its own FP64 residual at 1e-10; recomputed here in another order
+        assert float(torch.linalg.norm(bi - Ai.mv(xi)) / torch.linalg.norm(bi)) <= 1.5e-10
