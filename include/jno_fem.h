@@ -329,6 +329,38 @@ int jno_traction_p2(const double* d_points2, const int32_t* d_cells10, int64_t n
 int jno_compliance_grad_p2(const double* d_points2, const int32_t* d_cells10, int64_t n_cells,
                            double nu, const double* d_u, double* d_grad, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Theta-method transient step of the linear block M u' + A u = c + f(t) on one CSR pattern.
+ * Replaces SemidiscreteTimeBlock.step's linear branch (jno/utils/solver/backend_blocks.py:329-351)
+ * with the static step operator of compose_transient_step_solvers / _add_step_operator
+ * (jno/utils/solver/solver_api.py:856-947) and the Dirichlet edit of _apply_dirichlet_transient
+ * (jno/utils/solver/fem_1d.py:939-949).
+ *
+ * jno_theta_operators: d_mass, d_stiff (nnz, same pattern; d_stiff already symmetrically
+ * eliminated) -> d_step = Mz + theta dt A, d_expl = Mz - (1-theta) dt A (nullable), d_step_dinv =
+ * 1/diag(d_step) with the |d| > 1e-30 guard, where Mz is d_mass with the rows and columns of the
+ * d_dmask dofs zeroed (d_dmask nullable: no constrained dofs).  dt > 0, 0 < theta <= 1.
+ * ------------------------------------------------------------------------------------------- */
+int jno_theta_operators(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                        const double* d_mass, const double* d_stiff, const uint8_t* d_dmask, double dt,
+                        double theta, double* d_step, double* d_expl, double* d_step_dinv, void* stream);
+
+/* Right-hand side of one step: d_rhs = E u + dt c + dt (theta f(t+dt) + (1-theta) f(t))
+ * (backend_blocks.py:342-344).  d_c, and the pair d_f_now/d_f_next, are nullable (terms dropped);
+ * max_row is the pattern's longest row.  d_rhs must not alias d_u.  The same call with d_c and the
+ * forcing null applies E alone (the transposed step of the adjoint march; E is symmetric). */
+int jno_theta_rhs(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, int max_row,
+                  const double* d_expl, const double* d_u, const double* d_c, const double* d_f_now,
+                  const double* d_f_next, double dt, double theta, double* d_rhs, void* stream);
+
+/* Adjoint of one step w.r.t. the element coefficient: d_w = theta u_next + (1-theta) u_now on free
+ * dofs and g on the d_dmask dofs (0 when d_g is null); dt * jno_adjoint_grad_k_p1(w, mu) is then the
+ * step's contribution to dL/dk for the step adjoint mu = K^-1 lam_next (no reference counterpart
+ * as a kernel: the reference differentiates the march with jax.lax.scan's reverse mode,
+ * backend_blocks.py:475-530). */
+int jno_theta_mix(int64_t n, const double* d_u_next, const double* d_u_now, const uint8_t* d_dmask,
+                  const double* d_g, double theta, double* d_w, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,10 +5,11 @@ Lagrange P1/P2 tetrahedral assembly into CSR, symmetric Dirichlet elimination an
 compute runs in the HIP library ``libjno_fem.so`` (C-ABI in include/jno_fem.h) called through
 ctypes over PyTorch-ROCm device tensors; there is no CPU fallback.
 """
-from . import interop, precond, solve  # noqa: F401
+from . import interop, precond, solve, transient  # noqa: F401
 from ._lib import JnoError, load  # noqa: F401
 from .fem import CSR, FemLinearSystem, Plan, fem_poisson  # noqa: F401
 from .mesh import Mesh, Shape, kuhn_box  # noqa: F401
+from .transient import SemidiscreteTimeBlock, theta_march  # noqa: F401
 
 __all__ = ["Shape", "Mesh", "kuhn_box", "Plan", "CSR", "FemLinearSystem", "fem_poisson", "solve", "precond",
-           "interop", "JnoError", "load"]
+           "interop", "transient", "SemidiscreteTimeBlock", "theta_march", "JnoError", "load"]

@@ -1,0 +1,125 @@
+// Theta-method step of the linear semidiscrete block  M u' + A u = c + f(t)  on the plan's CSR
+// pattern (M and A share it):
+//   (M + theta dt A) u_next = (M - (1 - theta) dt A) u + dt c + dt (theta f(t+dt) + (1-theta) f(t))
+// (SemidiscreteTimeBlock.step, jno/utils/solver/backend_blocks.py:329-351).  The Dirichlet edit of
+// the transient block is the reference's _apply_dirichlet_transient (jno/utils/solver/fem_1d.py:
+// 939-949): A and c carry the symmetric elimination already; M loses the constrained rows AND
+// columns (no diagonal), so a constrained row of the step reads theta dt u_next[d] =
+// -(1-theta) dt u[d] + dt g[d].
+//
+// The step operator K = Mz + theta dt A and the explicit operator E = Mz - (1-theta) dt A are formed
+// once per (dt, theta) in one pass over the values (the constant-operator path of
+// compose_transient_step_solvers, jno/utils/solver/solver_api.py:856-947), together with K's Jacobi
+// inverse diagonal.  Each step then costs one fused pass for its right-hand side plus the PCG solve
+// of K, which reuses the steady path's operator layouts and kernels unchanged.
+#include "common.h"
+
+namespace jno {
+
+// one thread per row: Mz masked by the constrained rows/columns, K and E rounded as written
+// (m + s*a, no contraction: K and E are bitwise symmetric when M and A are)
+__global__ void __launch_bounds__(256) theta_ops_k(int64_t n, const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const double* __restrict__ mv,
+                                                   const double* __restrict__ av, const uint8_t* __restrict__ dmask,
+                                                   double sk, double se, double* __restrict__ kv,
+                                                   double* __restrict__ ev, double* __restrict__ kdinv) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const bool keep_r = !(dmask && dmask[r]);
+    double d = 0.0;
+    for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+      const int c = col[j];
+      const bool keep = keep_r && !(dmask && dmask[c]);
+      const double m = keep ? mv[j] : 0.0;
+      const double a = av[j];
+      const double k = __dadd_rn(m, __dmul_rn(sk, a));
+      kv[j] = k;
+      if (ev) ev[j] = __dadd_rn(m, __dmul_rn(se, a));
+      if (c == r) d += k;
+    }
+    // 1/diag with the |d| > 1e-30 guard of the step's Jacobi (backend_blocks.py:350-351)
+    kdinv[r] = fabs(d) > 1e-30 ? 1.0 / d : 1.0;
+  }
+}
+
+// y = E x [+ dt c] [+ dt (theta f1 + (1-theta) f0)], L lanes per row
+template <int L>
+__global__ void __launch_bounds__(256) theta_rhs_k(int64_t n, const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const double* __restrict__ ev,
+                                                   const double* __restrict__ x, const double* __restrict__ c,
+                                                   const double* __restrict__ f0, const double* __restrict__ f1,
+                                                   double dt, double theta, double* __restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x % L;
+  for (int64_t r = gid / L; r < n; r += (int64_t)gridDim.x * blockDim.x / L) {
+    const int b = rowptr[r], e = rowptr[r + 1];
+    double s = 0.0;
+    for (int j = b + sub; j < e; j += L) s += ev[j] * x[col[j]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+    if (sub == 0) {
+      if (c) s += dt * c[r];
+      if (f1) s += dt * (theta * f1[r] + (1.0 - theta) * f0[r]);
+      y[r] = s;
+    }
+  }
+}
+
+// w = theta u1 + (1-theta) u0 on free rows, g on constrained rows (0 without g)
+__global__ void theta_mix_k(int64_t n, const double* __restrict__ u1, const double* __restrict__ u0,
+                            const uint8_t* __restrict__ dmask, const double* __restrict__ g, double theta,
+                            double* __restrict__ w) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = theta * u1[i] + (1.0 - theta) * u0[i];
+    w[i] = (dmask && dmask[i]) ? (g ? g[i] : 0.0) : v;
+  }
+}
+
+}  // namespace jno
+
+using namespace jno;
+
+extern "C" int jno_theta_operators(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                                   const double* d_mass, const double* d_stiff, const uint8_t* d_dmask, double dt,
+                                   double theta, double* d_step, double* d_expl, double* d_step_dinv, void* stream) {
+  JNO_REQUIRE(d_rowptr && d_col && d_mass && d_stiff && d_step && d_step_dinv, JNO_EINVAL,
+              "jno_theta_operators: null pointer");
+  JNO_REQUIRE(dt > 0.0 && theta > 0.0 && theta <= 1.0, JNO_EINVAL,
+              "jno_theta_operators: need dt > 0 and 0 < theta <= 1 (got dt=%g, theta=%g)", dt, theta);
+  if (n_rows == 0) return JNO_OK;
+  theta_ops_k<<<grid_for(n_rows, 256, 8192), 256, 0, as_stream(stream)>>>(
+      n_rows, d_rowptr, d_col, d_mass, d_stiff, d_dmask, theta * dt, -(1.0 - theta) * dt, d_step, d_expl, d_step_dinv);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_theta_rhs(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, int max_row,
+                             const double* d_expl, const double* d_u, const double* d_c, const double* d_f_now,
+                             const double* d_f_next, double dt, double theta, double* d_rhs, void* stream) {
+  JNO_REQUIRE(d_rowptr && d_col && d_expl && d_u && d_rhs, JNO_EINVAL, "jno_theta_rhs: null pointer");
+  JNO_REQUIRE((d_f_now == nullptr) == (d_f_next == nullptr), JNO_EINVAL,
+              "jno_theta_rhs: the forcing needs both f(t) and f(t+dt), or neither");
+  JNO_REQUIRE(d_u != d_rhs, JNO_EINVAL, "jno_theta_rhs: rhs must not alias u");
+  if (n_rows == 0) return JNO_OK;
+  hipStream_t s = as_stream(stream);
+  if (max_row <= 16) {
+    theta_rhs_k<4><<<grid_for(n_rows * 4, 256, 16384), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_expl, d_u, d_c,
+                                                                     d_f_now, d_f_next, dt, theta, d_rhs);
+  } else if (max_row <= 64) {
+    theta_rhs_k<8><<<grid_for(n_rows * 8, 256, 16384), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_expl, d_u, d_c,
+                                                                     d_f_now, d_f_next, dt, theta, d_rhs);
+  } else {
+    theta_rhs_k<16><<<grid_for(n_rows * 16, 256, 16384), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_expl, d_u, d_c,
+                                                                       d_f_now, d_f_next, dt, theta, d_rhs);
+  }
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_theta_mix(int64_t n, const double* d_u_next, const double* d_u_now, const uint8_t* d_dmask,
+                             const double* d_g, double theta, double* d_w, void* stream) {
+  JNO_REQUIRE(d_u_next && d_u_now && d_w, JNO_EINVAL, "jno_theta_mix: null pointer");
+  if (n == 0) return JNO_OK;
+  theta_mix_k<<<grid_for(n, 256, 8192), 256, 0, as_stream(stream)>>>(n, d_u_next, d_u_now, d_dmask, d_g, theta, d_w);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}

@@ -73,6 +73,9 @@ class CSR:
     shape: tuple
     dinv: Optional[torch.Tensor] = None  # Jacobi inverse diagonal, when assembly produced it
     _cache: dict = field(default_factory=dict, repr=False)
+    # operator family on a shared pattern ("" for the assembled operator; the transient step
+    # operator uses its own): layouts learnt on one family are not offered to another
+    family: str = ""
 
     @property
     def nnz(self) -> int:

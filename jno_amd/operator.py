@@ -24,20 +24,35 @@ from . import _lib
 from .fem import CSR
 
 
-# DIA offsets of earlier operators, per pattern (the plan's index arrays): a re-assembled operator
-# on the same plan reuses them through the checked fill instead of a new analysis
+# DIA offsets of earlier operators, per pattern (the plan's index arrays), fewest diagonals first:
+# a re-assembled operator on the same plan reuses them through the checked fill instead of a new
+# analysis.  One pattern can carry several sets (the stiffness has exact zeros on diagonals where
+# the mass or a transient step operator of the same plan does not).
 _DIA_OFFSETS: dict = {}
 
 
 def _pattern_key(A: CSR):
-    return (A.rowptr.data_ptr(), A.col.data_ptr(), A.shape[0], A.nnz)
+    return (A.rowptr.data_ptr(), A.col.data_ptr(), A.shape[0], A.nnz, A.family)
+
+
+def _remember_offsets(A: CSR, nd, offs, chosen):
+    sets = _DIA_OFFSETS.setdefault(_pattern_key(A), [])
+    for i, (nd0, offs0, chosen0) in enumerate(sets):
+        if offs0 == offs:
+            sets[i] = (nd0, offs0, chosen0 or chosen)
+            return
+    sets.append((nd, offs, chosen))
+    sets.sort(key=lambda e: e[0])
 
 
 def chosen_dia_offsets(A: CSR):
-    """(ndiag, offsets) that the format choice picked for an earlier operator on A's pattern, or
-    None.  The assembly uses them to write the DIA layout in the same pass."""
-    hit = _DIA_OFFSETS.get(_pattern_key(A))
-    return (hit[0], hit[1]) if hit is not None and hit[2] else None
+    """(ndiag, offsets) with the fewest diagonals that the format choice picked for an earlier
+    operator on A's pattern, or None.  The assembly uses them to write the DIA layout in the same
+    pass (and falls back when A has entries off them)."""
+    for nd, offs, chosen in _DIA_OFFSETS.get(_pattern_key(A), []):
+        if chosen:
+            return nd, offs
+    return None
 
 
 class SolverOperator:
@@ -51,11 +66,12 @@ class SolverOperator:
         if fmt not in ("auto", "dia", "csr", "bsr3"):
             raise ValueError("format must be 'auto', 'dia', 'bsr3' or 'csr'")
         self.ndiag = -1
-        hit = _DIA_OFFSETS.get(_pattern_key(A)) if fmt in ("auto", "dia") else None
+        sets = _DIA_OFFSETS.get(_pattern_key(A), []) if fmt in ("auto", "dia") else []
         # under "auto", only offsets that the format choice itself picked are reused
-        if hit is not None and (fmt == "dia" or hit[2]) and self._build_dia_checked(hit[0], hit[1], fp32):
-            self.format = "dia"
-            return
+        for nd, offs, chosen in sets:
+            if (fmt == "dia" or chosen) and self._build_dia_checked(nd, offs, fp32):
+                self.format = "dia"
+                return
         chosen = fmt == "auto"
         if fmt == "auto":
             from .autotune import choose_format
@@ -66,7 +82,7 @@ class SolverOperator:
             if nd < 0:
                 raise ValueError("this operator has no symmetric DIA layout (too many diagonals or not symmetric)")
             self._build_dia(nd, offs, fp32)
-            _DIA_OFFSETS[_pattern_key(A)] = (nd, offs, chosen)
+            _remember_offsets(A, nd, offs, chosen)
         elif fmt == "bsr3":
             nb = self.bsr3_blocks(A)
             if nb < 0:

@@ -7,7 +7,8 @@ Builds the configuration's system (128^3 P1 Poisson for c2, 64^3 for c3, 32^3 P2
 c5), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
 finally copies a buffer of --copy-mib MiB with a device-to-device copy kernel: the copy's
 FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels' counters.
---case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver.
+--case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver;
+--case heat marches --iters backward-Euler steps of the n^3 transient heat block (--n, default 128).
 """
 from __future__ import annotations
 
@@ -50,6 +51,32 @@ def build_batch(dev, count=256, seed=4):
     return systems
 
 
+def run_heat(dev, steps: int, precision: str, n: int = 128):
+    """3-D analogue of the reference's transient heat benchmark (benchmarks/fem_bench.py:241-259:
+    u_t v + k grad u . grad v on the unit square, IC sin sin, u = 0 on the boundary, T = 0.2 in 1500
+    backward-Euler steps): the unit cube at n^3 cells, lognormal k, IC sin sin sin, dt = 0.2/1500,
+    ``steps`` steps each solved to rtol 1e-10 with warm start."""
+    import jno_amd as J
+
+    m = J.kuhn_box(n, device=dev)
+    s = J.fem_poisson(m, f=0.0)
+    k = torch.tensor(np.exp(0.5 * np.random.default_rng(0).standard_normal(m.n_cells)), device=dev)
+    blk = J.SemidiscreteTimeBlock.from_system(s, {"k": k}, dt=0.2 / 1500)
+    p = m.points
+    u0 = (torch.sin(np.pi * p[:, 0]) * torch.sin(np.pi * p[:, 1]) * torch.sin(np.pi * p[:, 2])).contiguous()
+    sv = J.solve.cg(tol=1e-10, precision=precision)
+    blk.march(u0, 2, linear_solve=sv)  # warm (step operators, layout, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u = blk.march(u0, steps, linear_solve=sv)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    its = blk.last_iterations
+    print(f"heat {n}^3 {precision}: {steps} steps in {dt * 1e3:.1f} ms ({dt / steps * 1e3:.3f} ms/step), "
+          f"iterations/step {min(its)}..{max(its)} (mean {sum(its) / len(its):.1f}), |u_N| {float(u.norm()):.6e}, "
+          f"operator {sv.last_stats['operator']}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", default="c2")
@@ -57,6 +84,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=("mixed", "fp64"))
     ap.add_argument("--format", default="auto")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--n", type=int, default=128, help="cells per axis of the heat case")
     ap.add_argument("--tuning", default=None,
                     help="launch shape block,grid_cap,k1_vector,mode of the iterations (default: the library's)")
     args = ap.parse_args()
@@ -77,6 +105,9 @@ def main():
         its = [st["iterations"] for st in bs.last_stats]
         print(f"c4 batch: {len(systems)} systems, {sum(A.shape[0] for A, _ in systems)} rows, "
               f"{(time.perf_counter() - t0) * 1e3:.1f} ms, iterations {min(its)}..{max(its)}")
+        return
+    if args.case == "heat":
+        run_heat(dev, args.iters, args.precision, args.n)
         return
     if args.tuning:
         from jno_amd.solve import set_tuning

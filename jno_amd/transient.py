@@ -1,0 +1,215 @@
+"""Theta-method transient march of the linear semidiscrete block  M u' + A u = c + f(t).
+
+Numeric analogue of the reference's linear ``SemidiscreteTimeBlock``
+(jno/utils/solver/backend_blocks.py:150-351) as the native transient assembly builds it
+(jno/utils/solver/fem_native.py:3150-3169): ``A`` and ``c`` are the steady operator and load with the
+symmetric Dirichlet elimination, ``M`` is the consistent mass with the constrained rows and columns
+zeroed (``_apply_dirichlet_transient``, jno/utils/solver/fem_1d.py:939-949).  One step solves
+
+    (M + theta dt A) u_next = (M - (1 - theta) dt A) u + dt c + dt (theta f(t+dt) + (1-theta) f(t))
+
+with theta = 1 (backward Euler, the default) or 1/2 (trapezoidal).  Like the constant-operator path
+of ``compose_transient_step_solvers`` (jno/utils/solver/solver_api.py:881-947) the step operator and
+its Jacobi preconditioner are formed once per (dt, theta) -- here in one HIP pass that also forms
+the explicit operator -- and every step is one fused right-hand-side pass plus the Jacobi-PCG solve
+of the steady path (same operator layouts, same kernels), warm-started from the current state.
+
+``theta_march`` makes the march differentiable in the per-element coefficient ``k`` and the initial
+state: the backward sweep solves with the (symmetric) step operator and contracts each step's
+contribution on the device (the reference differentiates its ``lax.scan`` in reverse mode,
+backend_blocks.py:475-530).
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional, Sequence
+
+import torch
+
+from . import _lib
+from .fem import CSR, FemLinearSystem
+from .precond import Diagonal
+from .solve import LinearSolver, cg
+
+
+class SemidiscreteTimeBlock:
+    """Linear block ``M u' + A u = affine_bias + forcing_vector_fn(t)`` on one CSR pattern.
+
+    ``M`` is the raw consistent mass (the constrained rows/columns named by ``dmask`` are zeroed
+    when the step operators are formed), ``A``/``affine_bias`` the eliminated operator and load,
+    ``g`` the Dirichlet values (None: 0).  ``theta`` is the block's default scheme (the reference's
+    ``metadata["theta"]``), ``dt`` its assembled step.
+    """
+
+    def __init__(self, M: CSR, A: CSR, affine_bias: Optional[torch.Tensor], *, dmask=None, g=None,
+                 forcing_vector_fn: Optional[Callable] = None, state0: Optional[torch.Tensor] = None,
+                 t0: float = 0.0, t1: Optional[float] = None, dt: Optional[float] = None, theta: float = 1.0,
+                 max_row: Optional[int] = None):
+        if M.shape != A.shape or M.nnz != A.nnz:
+            raise ValueError("M and A must share one CSR pattern")
+        if not (torch.equal(M.rowptr, A.rowptr) and torch.equal(M.col, A.col)):
+            raise ValueError("M and A must share one CSR pattern")
+        _lib.require_gpu(A.vals, "A")
+        self.M, self.A = M, A
+        self.n = A.shape[0]
+        self.device = A.device
+        self.c = affine_bias
+        self.dmask, self.g = dmask, g
+        self.forcing_vector_fn = forcing_vector_fn
+        self.state0 = state0
+        self.t0, self.t1, self.dt = float(t0), t1, dt
+        self.theta = float(theta)
+        if max_row is None:
+            max_row = int((A.rowptr[1:] - A.rowptr[:-1]).max()) if self.n else 0
+        self.max_row = int(max_row)
+        self._ops: dict = {}
+
+    @classmethod
+    def from_system(cls, system: FemLinearSystem, args: dict, **kw) -> "SemidiscreteTimeBlock":
+        """The heat block of ``system`` at ``args``: ``u_t v + k grad u . grad v - f v`` with the
+        system's Dirichlet set, g and load."""
+        A, c = system.evaluate(args)
+        M = system.mass()
+        return cls(M, A, c, dmask=system.dmask, g=system.g, max_row=system.plan.max_row, **kw)
+
+    # -- step operators ----------------------------------------------------------------------
+
+    def step_operators(self, dt: float, theta: float):
+        """(K, E values): K = Mz + theta dt A as a CSR with its Jacobi inverse diagonal, and the
+        values of E = Mz - (1 - theta) dt A on the same pattern.  Cached per (dt, theta)."""
+        key = (float(dt), float(theta))
+        hit = self._ops.get(key)
+        if hit is not None:
+            return hit
+        A = self.A
+        kv = torch.empty_like(A.vals)
+        ev = torch.empty_like(A.vals)
+        kd = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        _lib.call("jno_theta_operators", self.n, _lib.ptr(A.rowptr), _lib.ptr(A.col), _lib.ptr(self.M.vals),
+                  _lib.ptr(A.vals), _lib.ptr(self.dmask), float(dt), float(theta), _lib.ptr(kv), _lib.ptr(ev),
+                  _lib.ptr(kd), _lib.stream_ptr(self.device))
+        K = CSR(A.rowptr, A.col, kv, A.shape, kd, family="theta-step")
+        self._ops[key] = (K, ev)
+        return K, ev
+
+    def _forcing(self, t, args):
+        if self.forcing_vector_fn is None:
+            return None
+        f = torch.as_tensor(self.forcing_vector_fn(t, args)).reshape(-1)
+        _lib.require_gpu(f, "forcing_vector_fn(t)")
+        if f.shape[0] != self.n:
+            raise ValueError(f"forcing_vector_fn returned {f.shape[0]} entries for {self.n} dofs")
+        return f.to(torch.float64).contiguous()
+
+    def rhs(self, u: torch.Tensor, t: float, dt: float, theta: float, args=None, out=None) -> torch.Tensor:
+        """E u + dt c + dt (theta f(t+dt) + (1-theta) f(t)) in one pass."""
+        _, ev = self.step_operators(dt, theta)
+        f0 = self._forcing(t, args)
+        f1 = self._forcing(t + dt, args) if f0 is not None else None
+        y = torch.empty(self.n, dtype=torch.float64, device=self.device) if out is None else out
+        _lib.call("jno_theta_rhs", self.n, _lib.ptr(self.A.rowptr), _lib.ptr(self.A.col), self.max_row, _lib.ptr(ev),
+                  _lib.ptr(u), _lib.ptr(self.c), _lib.ptr(f0), _lib.ptr(f1), float(dt), float(theta), _lib.ptr(y),
+                  _lib.stream_ptr(self.device))
+        return y
+
+    def apply_explicit(self, x: torch.Tensor, dt: float, theta: float) -> torch.Tensor:
+        """E x alone (the explicit operator is symmetric: also the transposed step)."""
+        _, ev = self.step_operators(dt, theta)
+        y = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        _lib.call("jno_theta_rhs", self.n, _lib.ptr(self.A.rowptr), _lib.ptr(self.A.col), self.max_row, _lib.ptr(ev),
+                  _lib.ptr(x), None, None, None, float(dt), float(theta), _lib.ptr(y), _lib.stream_ptr(self.device))
+        return y
+
+    # -- stepping ----------------------------------------------------------------------------
+
+    def step(self, u: torch.Tensor, t: float, dt: Optional[float] = None, args=None, theta: Optional[float] = None,
+             *, linear_solve: Optional[LinearSolver] = None) -> torch.Tensor:
+        """Advance ``u(t) -> u(t + dt)`` by one theta-step (``SemidiscreteTimeBlock.step``,
+        backend_blocks.py:216-351).  ``linear_solve`` is a solver slot ``fn(K, rhs, *, M, x0)``
+        (default ``jno_amd.solve.cg(tol=1e-10)``), given the step's Jacobi preconditioner and the
+        current state as warm start."""
+        dt = self.dt if dt is None else dt
+        if dt is None:
+            raise ValueError("step() needs dt (the block has no assembled step)")
+        theta = self.theta if theta is None else float(theta)
+        u = torch.as_tensor(u).reshape(-1)
+        _lib.require_gpu(u, "u")
+        u = u.to(torch.float64).contiguous()
+        if u.shape[0] != self.n:
+            raise ValueError(f"state has {u.shape[0]} entries for {self.n} dofs")
+        K, _ = self.step_operators(dt, theta)
+        b = self.rhs(u, t, dt, theta, args)
+        solver = linear_solve if linear_solve is not None else cg(tol=1e-10)
+        return solver(K, b, M=Diagonal(K.dinv), x0=u)
+
+    def march(self, u0: torch.Tensor, n_steps: int, dt: Optional[float] = None, args=None,
+              theta: Optional[float] = None, *, linear_solve: Optional[LinearSolver] = None,
+              keep: str = "final"):
+        """``n_steps`` steps from ``u0`` at t0.  keep="final" -> u_N; "all" -> list u_0..u_N."""
+        dt = self.dt if dt is None else dt
+        theta = self.theta if theta is None else float(theta)
+        u = torch.as_tensor(u0).reshape(-1).to(torch.float64).contiguous()
+        states = [u] if keep == "all" else None
+        t = self.t0
+        self.last_iterations = []
+        solver = linear_solve if linear_solve is not None else cg(tol=1e-10)
+        for _ in range(int(n_steps)):
+            u = self.step(u, t, dt, args, theta, linear_solve=solver)
+            self.last_iterations.append(solver.last_stats.get("iterations", -1))
+            t += dt
+            if states is not None:
+                states.append(u)
+        return states if states is not None else u
+
+
+class _ThetaMarch(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, k, u0, system: FemLinearSystem, dt, theta, n_steps, save, solver, forcing):
+        block = SemidiscreteTimeBlock.from_system(system, {system.param: k.detach()}, dt=dt, theta=theta,
+                                                  forcing_vector_fn=forcing)
+        states = block.march(u0.detach(), n_steps, linear_solve=solver, keep="all")
+        ctx.block, ctx.states, ctx.system, ctx.solver, ctx.save = block, states, system, solver, save
+        ctx.dt, ctx.theta = float(dt), float(theta)
+        return torch.stack([states[i] for i in save])
+
+    @staticmethod
+    def backward(ctx, gy):
+        block: SemidiscreteTimeBlock = ctx.block
+        system: FemLinearSystem = ctx.system
+        states, dt, theta = ctx.states, ctx.dt, ctx.theta
+        gy = gy.detach().to(torch.float64)
+        seed = {}
+        for j, i in enumerate(ctx.save):
+            seed[i] = seed[i] + gy[j] if i in seed else gy[j].clone()
+        K, _ = block.step_operators(dt, theta)
+        m = system.mesh
+        grad = torch.zeros(m.n_cells, dtype=torch.float64, device=m.device)
+        lam = torch.zeros(block.n, dtype=torch.float64, device=m.device)
+        w = torch.empty_like(lam)
+        # states after the last saved one do not reach the output
+        for n in range(max(seed) - 1, -1, -1):
+            if n + 1 in seed:
+                lam = lam + seed[n + 1]
+            mu = ctx.solver(K, lam.contiguous(), M=Diagonal(K.dinv))
+            _lib.call("jno_theta_mix", block.n, _lib.ptr(states[n + 1]), _lib.ptr(states[n]), _lib.ptr(block.dmask),
+                      _lib.ptr(block.g), theta, _lib.ptr(w), _lib.stream_ptr(m.device))
+            grad += dt * system.grad_k(w, mu)
+            lam = block.apply_explicit(mu, dt, theta)
+        if 0 in seed:
+            lam = lam + seed[0]
+        return grad, lam, None, None, None, None, None, None, None
+
+
+def theta_march(system: FemLinearSystem, k: torch.Tensor, u0: torch.Tensor, dt: float, n_steps: int,
+                theta: float = 1.0, save: Optional[Sequence[int]] = None, solver: Optional[LinearSolver] = None,
+                forcing: Optional[Callable] = None) -> torch.Tensor:
+    """States ``u_i`` for i in ``save`` (default: the last) of the heat march of ``system`` at the
+    per-element coefficient ``k``, stacked (len(save), n); differentiable in ``k`` and ``u0``."""
+    save = [int(n_steps)] if save is None else [int(i) for i in save]
+    if any(i < 0 or i > n_steps for i in save):
+        raise ValueError(f"save indices must lie in 0..{n_steps}")
+    solver = solver or cg(tol=1e-10)
+    k = torch.as_tensor(k, dtype=torch.float64, device=system.mesh.device)
+    if k.ndim == 0:
+        k = k.expand(system.mesh.n_cells)
+    u0 = torch.as_tensor(u0, dtype=torch.float64, device=system.mesh.device).reshape(-1)
+    return _ThetaMarch.apply(k, u0, system, float(dt), float(theta), int(n_steps), save, solver, forcing)

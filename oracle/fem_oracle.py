@@ -551,3 +551,79 @@ def compliance_grad_p2(points, cells10, E, nu, u):
     Ke = elasticity_p2(points, cells10, np.ones(cells10.shape[0]), nu)
     ue = np.asarray(u, float)[cell_dofs(cells10, 3)]
     return -np.einsum("ti,tij,tj->t", ue, Ke, ue)
+
+
+# ---------------------------------------------------------------------------------------------
+# Theta-method transient step of the linear block M u' + A u = c + f(t)
+# ---------------------------------------------------------------------------------------------
+
+
+def transient_dirichlet(M, dofs):
+    """The mass half of ``_apply_dirichlet_transient`` (jno/utils/solver/fem_1d.py:939-949):
+    the constrained rows AND columns of M are zeroed (no diagonal), the pattern kept.  A and c take
+    the steady symmetric elimination (``dirichlet_eliminate``)."""
+    M = M.tocsr(copy=True)
+    M.sort_indices()
+    isd = np.zeros(M.shape[0])
+    isd[np.asarray(dofs, np.int64)] = 1.0
+    rows = np.repeat(np.arange(M.shape[0]), np.diff(M.indptr))
+    M.data = M.data * (1.0 - isd[rows]) * (1.0 - isd[M.indices])
+    return M
+
+
+def theta_operators(Mz, A, dt, theta):
+    """Step operator K = Mz + theta dt A (``_add_step_operator``, jno/utils/solver/solver_api.py:
+    856-879) and explicit operator E = Mz - (1-theta) dt A, value by value on the shared pattern."""
+    Mz = Mz.tocsr()
+    A = A.tocsr()
+    Mz.sort_indices()
+    A.sort_indices()
+    if not (np.array_equal(Mz.indptr, A.indptr) and np.array_equal(Mz.indices, A.indices)):
+        raise ValueError("M and A must share one pattern")
+    K = sp.csr_matrix((Mz.data + (theta * dt) * A.data, A.indices, A.indptr), shape=A.shape)
+    E = sp.csr_matrix((Mz.data + (-(1.0 - theta) * dt) * A.data, A.indices, A.indptr), shape=A.shape)
+    return K, E
+
+
+def theta_rhs(Mz, A, c, u, dt, theta, f_now=None, f_next=None):
+    """rhs = M u - (1-theta) dt (A u) + dt c + dt (theta f(t+dt) + (1-theta) f(t))
+    (``SemidiscreteTimeBlock.step``, jno/utils/solver/backend_blocks.py:342-344)."""
+    rhs = Mz @ u - (1.0 - theta) * dt * (A @ u) + dt * np.asarray(c, float)
+    if f_next is not None:
+        rhs = rhs + dt * (theta * np.asarray(f_next, float) + (1.0 - theta) * np.asarray(f_now, float))
+    return rhs
+
+
+def theta_march(Mz, A, c, u0, dt, theta, n_steps, forcing=None, t0=0.0):
+    """States u_0..u_n of ``n_steps`` theta-steps, each solved directly (the exact step answer that
+    the reference's per-step Krylov solve at tol 1e-10 approximates, backend_blocks.py:329-351).
+    ``forcing(t)`` -> load increment vector, or None."""
+    K, _ = theta_operators(Mz, A, dt, theta)
+    lu = spla.splu(K.tocsc())
+    us = [np.array(u0, float)]
+    t = t0
+    for _ in range(n_steps):
+        fn = (forcing(t), forcing(t + dt)) if forcing is not None else (None, None)
+        us.append(lu.solve(theta_rhs(Mz, A, c, us[-1], dt, theta, *fn)))
+        t += dt
+    return us
+
+
+def theta_march_grad_k(points, cells, Mz, A, us, dt, theta, dofs, g, lam_final):
+    """dL/dk_e for L with dL/du_N = ``lam_final`` along the march ``us``: the reverse sweep
+    mu = K^-1 lam_{n+1}, dL/dk += dt * adjoint_grad_k(w_n, mu) with w_n = theta u_{n+1} +
+    (1-theta) u_n on free dofs and g on constrained ones, lam_n = E mu (K, E symmetric).  Returns
+    (dL/dk, dL/du_0).  The reference gets the same numbers from jax.lax.scan's reverse mode
+    (backend_blocks.py:475-530)."""
+    K, E = theta_operators(Mz, A, dt, theta)
+    lu = spla.splu(K.tocsc())
+    dofs = np.asarray(dofs, np.int64)
+    grad = np.zeros(cells.shape[0])
+    lam = np.array(lam_final, float)
+    for n in range(len(us) - 2, -1, -1):
+        mu = lu.solve(lam)
+        w = theta * us[n + 1] + (1.0 - theta) * us[n]
+        w[dofs] = g
+        grad += dt * adjoint_grad_k_p1(points, cells, w, mu, dofs)
+        lam = E @ mu
+    return grad, lam

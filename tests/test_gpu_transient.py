@@ -1,0 +1,224 @@
+"""Parity of the HIP theta-method transient march (step operators, fused right-hand side, warm-started
+PCG per step, reverse sweep for dL/dk) with the oracle's direct-solve march.
+
+Bars: 1e-12 relative for the step operators' values; 1e-8 relative L2 for marched states (each step
+solved to rtol 1e-10 against the oracle's direct solves); 1e-6 relative for gradients."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from oracle import fem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PI = np.pi
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _systems(n, k, f=1.0, g=0.0):
+    import jno_amd as J
+
+    m = J.kuhn_box(n)
+    p, t = O.kuhn_box(n)
+    sysg = J.fem_poisson(m, f=f if np.ndim(f) == 0 else torch.as_tensor(f, device="cuda"), g=g)
+    A, c, d = O.poisson_p1_system(p, t, k, f=f, g=g)
+    Mz = O.transient_dirichlet(O.mass_matrix_p1(p, t), d)
+    return m, sysg, p, t, A, c, d, Mz
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_step_operators_match_oracle(dev, theta):
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    n = 6
+    k = O.k_lognormal(6 * n**3, 7)
+    m, sysg, p, t, A, c, d, Mz = _systems(n, k, g=0.25)
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": torch.as_tensor(k, device="cuda")})
+    dt = 0.0123
+    K, ev = blk.step_operators(dt, theta)
+    Ko, Eo = O.theta_operators(Mz, A, dt, theta)
+    assert np.abs(_np(K.vals) - Ko.data).max() <= 1e-12 * np.abs(Ko.data).max()
+    assert np.abs(_np(ev) - Eo.data).max() <= 1e-12 * np.abs(Eo.data).max()
+    assert _rel(_np(K.dinv), O.jacobi_inv(Ko)) < 1e-12
+    # constrained rows: theta dt on the diagonal and nothing else
+    Kg = sp.csr_matrix((_np(K.vals), Ko.indices, Ko.indptr), shape=Ko.shape)
+    assert np.all(Kg.diagonal()[d] == theta * dt)
+    off = Kg[d].toarray()
+    off[np.arange(len(d)), d] = 0.0
+    assert np.all(off == 0.0)
+    # cached per (dt, theta)
+    assert blk.step_operators(dt, theta)[0] is K
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_march_matches_oracle(dev, theta):
+    import jno_amd as J
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    n = 10
+    k = O.k_lognormal(6 * n**3, 5)
+    m, sysg, p, t, A, c, d, Mz = _systems(n, k, f=1.0, g=0.3)
+    u0 = 0.2 * np.random.default_rng(1).standard_normal(p.shape[0])
+    dt, N = 0.01, 5
+    ref = O.theta_march(Mz, A, c, u0, dt, theta, N)
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": torch.as_tensor(k, device="cuda")}, dt=dt, theta=theta)
+    us = blk.march(torch.as_tensor(u0, device="cuda"), N, keep="all")
+    for i in (1, N):
+        assert _rel(_np(us[i]), ref[i]) < 1e-8, i
+    assert all(it > 0 for it in blk.last_iterations)
+    # the step operator runs on the index-free diagonal layout; the steady operator of the same
+    # plan keeps its own (fewer) diagonals afterwards
+    K, _ = blk.step_operators(dt, theta)
+    sol = J.solve.cg(tol=1e-10)
+    assert sol.operator(K).describe()["format"] == "dia"
+    A2, _ = sysg.evaluate({"k": torch.as_tensor(k, device="cuda")})
+    nd_k = sol.operator(K).ndiag
+    assert sol.operator(A2).ndiag < nd_k
+
+
+def test_time_varying_forcing_matches_oracle(dev):
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    n = 8
+    p, t = O.kuhn_box(n)
+    ss = np.sin(PI * p[:, 0]) * np.sin(PI * p[:, 1]) * np.sin(PI * p[:, 2])
+    k = np.ones(t.shape[0])
+    m, sysg, p, t, A, c, d, Mz = _systems(n, k, f=ss)
+    free = np.ones_like(ss)
+    free[d] = 0.0
+    inc = free * O.p1_load(p, t, ss, p.shape[0]) * 3 * PI**2
+    inc_d = torch.as_tensor(inc, device="cuda")
+    dt, N = 0.3 / 20, 20
+    ref = O.theta_march(Mz, A, c, np.zeros_like(ss), dt, 0.5, N, forcing=lambda tt: tt * inc)
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": torch.ones(t.shape[0], device="cuda", dtype=torch.float64)},
+                                            dt=dt, theta=0.5, forcing_vector_fn=lambda tt, args: tt * inc_d)
+    u = blk.march(torch.zeros(p.shape[0], device="cuda", dtype=torch.float64), N)
+    assert _rel(_np(u), ref[-1]) < 1e-8
+    # and the manufactured answer u = ss t within the mesh's discretisation error
+    assert _rel(_np(u), ss * 0.3) < 8e-2
+
+
+def test_heat_decay_of_the_fundamental_mode(dev):
+    # tests/test_fem_block_step.py:49-62 on the unit cube (bar 5e-2 as in the reference test)
+    import jno_amd as J
+
+    n = 12
+    m = J.kuhn_box(n)
+    sysg = J.fem_poisson(m, f=0.0)
+    p = _np(m.points)
+    u0 = np.sin(PI * p[:, 0]) * np.sin(PI * p[:, 1]) * np.sin(PI * p[:, 2])
+    T = 0.02
+    blk = J.SemidiscreteTimeBlock.from_system(sysg, {"k": 1.0}, dt=T / 10)
+    u = blk.march(torch.as_tensor(u0, device="cuda"), 10)
+    exact = np.exp(-3 * PI**2 * T) * u0
+    assert _rel(_np(u), exact) < 5e-2
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_march_gradient_matches_oracle_adjoint(dev, theta):
+    import jno_amd as J
+
+    n = 4
+    p, t = O.kuhn_box(n)
+    T = t.shape[0]
+    k = O.k_lognormal(T, 13)
+    g, dt, N = 0.3, 0.02, 4
+    m, sysg, p, t, A, c, d, Mz = _systems(n, k, f=1.0, g=g)
+    u0 = 0.1 * np.random.default_rng(4).standard_normal(p.shape[0])
+    w = np.random.default_rng(2).standard_normal(p.shape[0])
+    us = O.theta_march(Mz, A, c, u0, dt, theta, N)
+    gk_ref, gu_ref = O.theta_march_grad_k(p, t, Mz, A, us, dt, theta, d, g, w)
+
+    kt = torch.as_tensor(k, device="cuda").requires_grad_(True)
+    u0t = torch.as_tensor(u0, device="cuda").requires_grad_(True)
+    out = J.theta_march(sysg, kt, u0t, dt, N, theta=theta, solver=J.solve.cg(tol=1e-12))
+    assert _rel(_np(out[-1]), us[-1]) < 1e-9
+    (out[-1] @ torch.as_tensor(w, device="cuda")).backward()
+    assert _rel(_np(kt.grad), gk_ref) < 1e-6
+    assert _rel(_np(u0t.grad), gu_ref) < 1e-6
+
+
+def test_march_gradient_through_saved_states_matches_fd(dev):
+    # a loss over several saved states (the trajectory readout of the reference's
+    # SemidiscreteTimeBlock.solve(save_ts=...)), against GPU central differences
+    import jno_amd as J
+
+    n = 4
+    m = J.kuhn_box(n)
+    sysg = J.fem_poisson(m, f=1.0, g=0.1)
+    T = m.n_cells
+    k0 = torch.as_tensor(O.k_lognormal(T, 21), device="cuda")
+    u0 = torch.zeros(m.n_nodes, device="cuda", dtype=torch.float64)
+    sol = J.solve.cg(tol=1e-13)
+    wts = torch.as_tensor(np.random.default_rng(8).standard_normal((3, m.n_nodes)), device="cuda")
+
+    def loss(kk):
+        ys = J.theta_march(sysg, kk, u0, 0.05, 6, theta=0.5, save=[2, 4, 6], solver=sol)
+        return (ys * wts).sum()
+
+    kt = k0.clone().requires_grad_(True)
+    loss(kt).backward()
+    for e in (0, T // 3, T - 1):
+        h = 1e-6 * float(k0[e])
+        kp, km = k0.clone(), k0.clone()
+        kp[e] += h
+        km[e] -= h
+        fd = (float(loss(kp)) - float(loss(km))) / (2 * h)
+        assert abs(float(kt.grad[e]) - fd) <= 1e-5 * max(abs(fd), 1e-4), (e, float(kt.grad[e]), fd)
+
+
+def test_large_march_properties(dev):
+    # 64^3 cells (274,625 dofs): energy decay of the homogeneous backward-Euler march, per-step
+    # residual of the step solve, Dirichlet rows kept at g, mixed-precision steps agree with FP64
+    import jno_amd as J
+
+    n = 64
+    m = J.kuhn_box(n)
+    sysg = J.fem_poisson(m, f=0.0)
+    blk = J.SemidiscreteTimeBlock.from_system(sysg, {"k": 1.0}, dt=1e-3)
+    p = _np(m.points)
+    u0 = torch.as_tensor(np.sin(PI * p[:, 0]) * np.sin(PI * p[:, 1]) * np.sin(PI * p[:, 2]) + 0.1 * np.cos(7 * p[:, 0]),
+                         device="cuda")
+    u0[sysg.dmask.bool()] = 0.0
+    us = blk.march(u0, 3, keep="all")
+    K, _ = blk.step_operators(1e-3, 1.0)
+    Mz_u = [blk.apply_explicit(u, 1e-3, 1.0) for u in us]  # theta = 1: E = Mz
+    energy = [float(u @ mu) for u, mu in zip(us, Mz_u)]
+    assert all(energy[i + 1] < energy[i] for i in range(3)), energy
+    for i in range(3):
+        b = blk.rhs(us[i], 0.0, 1e-3, 1.0)
+        r = K.mv(us[i + 1]) - b
+        assert float(r.norm() / b.norm()) <= 2e-10
+        assert float(us[i + 1][sysg.dmask.bool()].abs().max()) == 0.0
+    um = blk.march(u0, 3, linear_solve=J.solve.cg(tol=1e-10, precision="mixed"))
+    assert float((um - us[-1]).norm() / us[-1].norm()) < 1e-8
+
+
+def test_transient_argument_errors(dev):
+    import jno_amd as J
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    m = J.kuhn_box(3)
+    sysg = J.fem_poisson(m)
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": 1.0})
+    with pytest.raises(ValueError, match="theta"):
+        blk.step_operators(0.01, 0.0)
+    with pytest.raises(ValueError, match="dt > 0"):
+        blk.step_operators(-0.01, 1.0)
+    with pytest.raises(ValueError):
+        blk.step(torch.zeros(m.n_nodes, device="cuda"), 0.0)  # no dt
+    with pytest.raises(ValueError):
+        blk.step(torch.zeros(m.n_nodes + 1, device="cuda"), 0.0, 0.01)
+    other = J.fem_poisson(J.kuhn_box(2)).mass()
+    with pytest.raises(ValueError):
+        SemidiscreteTimeBlock(other, blk.A, blk.c)
+    with pytest.raises(ValueError):
+        J.theta_march(sysg, 1.0, torch.zeros(m.n_nodes, device="cuda"), 0.01, 3, save=[4])
