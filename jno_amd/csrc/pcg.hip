@@ -603,12 +603,22 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
 // partially written lines.)
 // ---------------------------------------------------------------------------------------------
 constexpr int ONE_STAGE_BYTES = 65536;
+// FW (far window): the far off-diagonals lie within LDS_FAR_SPAN of each other (the mass-carrying
+// 8-diagonal operators of a Kuhn lattice: offsets (nx+1)(ny+1) + {0, 1, nx+1, nx+2}), so p_k of the
+// far neighbours is staged as two windows (above and below the tile) of TT + span rows each,
+// instead of one range of TT rows per far diagonal and direction: a quarter of the far loads.
+constexpr int LDS_FAR_SPAN = 256;
 
-template <int NT, int ND, int NN, typename T, typename V>
+template <int NT, int ND, int NN, typename T, typename V, bool FW = false>
 __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   constexpr int NF = ND > 1 ? ND - 1 - NN : 0;  // far off-diagonals
+  static_assert(!FW || NF >= 2, "the far window groups two or more far diagonals");
   // rows per tile: pt holds TT + 2 halo rows, rt TT rows, pf two ranges of TT rows per far diagonal
-  constexpr int TT = (ONE_STAGE_BYTES / (int)sizeof(T) - 2 * LDS_HALO) / (2 + 2 * NF);
+  // (or, with FW, two windows of TT + LDS_FAR_SPAN rows)
+  constexpr int TT = FW ? (ONE_STAGE_BYTES / (int)sizeof(T) - 2 * LDS_HALO - 2 * LDS_FAR_SPAN) / 4
+                        : (ONE_STAGE_BYTES / (int)sizeof(T) - 2 * LDS_HALO) / (2 + 2 * NF);
+  constexpr int NPF = FW ? 2 : NF > 0 ? 2 * NF : 1;
+  constexpr int LPF = FW ? TT + LDS_FAR_SPAN : NF > 0 ? TT : 1;
   constexpr int U1 = 4;
   constexpr int U2 = 2;
   constexpr int NDA = ND > 1 ? ND : 2;
@@ -616,7 +626,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   __shared__ double sm_out[4 * (NT / 64)];
   __shared__ T pt[TT + 2 * LDS_HALO];         // p_k of the tile and its halo
   __shared__ T rt[TT];                        // r_k of the tile
-  __shared__ T pf[NF > 0 ? 2 * NF : 1][NF > 0 ? TT : 1];  // p_k of the far neighbours (i + o_d, i - o_d)
+  __shared__ T pf[NPF][LPF];                  // p_k of the far neighbours (i + o_d, i - o_d)
   PcgState* st = a.st;
   if (st->done) return;
   const int G = gridDim.x;
@@ -706,8 +716,30 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
     // p_k of the far neighbour rows (offsets > LDS_HALO), staged in the same pass: their owners
     // read the same rows at about the same time, so these loads hit in L2 (read later, in the
     // SpMV, they had been evicted and came from memory again)
+    if constexpr (FW) {
+      const int flo = o[NN + 1], span = o[ND - 1] - flo;  // span <= LDS_FAR_SPAN (host-checked)
+      const int len = (t1 - t0) + span;
 #pragma unroll
-    for (int f = 0; f < 2 * NF; ++f) {
+      for (int w = 0; w < 2; ++w) {
+        const int base = w == 0 ? t0 + flo : t0 - flo - span;
+        for (int j0 = (int)threadIdx.x; j0 < len; j0 += U1 * NT) {
+          T fr[U1], fq[U1], fp[U1];
+#pragma unroll
+          for (int u = 0; u < U1; ++u) {
+            const int j = base + j0 + u * NT;
+            const bool in = j0 + u * NT < len && j >= 0 && j < n;
+            fr[u] = ldc(a.r, j, in);
+            fq[u] = ldc(a.q, j, in);
+            fp[u] = ldc(a.p_old, j, in);
+          }
+#pragma unroll
+          for (int u = 0; u < U1; ++u)
+            if (j0 + u * NT < len) pf[w][j0 + u * NT] = (fr[u] - al * fq[u]) + be * fp[u];
+        }
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < (FW ? 0 : 2 * NF); ++f) {
       const int d = NN + 1 + f / 2;
       const int sh = (f & 1) ? -o[d] : o[d];
       for (int j0 = t0 + (int)threadIdx.x; j0 < t1; j0 += U1 * NT) {
@@ -751,6 +783,10 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
             if (d <= NN) {
               xu[u][d] = pt[i + o[d] - s0];
               xl[u][d] = pt[i - o[d] - s0];
+            } else if constexpr (FW) {
+              const int flo = o[NN + 1], span = o[ND - 1] - flo;
+              xu[u][d] = pf[0][i + o[d] - t0 - flo];
+              xl[u][d] = pf[1][i - o[d] - t0 + flo + span];
             } else {
               xu[u][d] = pf[2 * (d - NN - 1)][i - t0];
               xl[u][d] = pf[2 * (d - NN - 1) + 1][i - t0];
@@ -1625,33 +1661,43 @@ static int near_count(const jno_op* op) {
 }
 
 static bool lds_ok(const jno_op* op) {
-  return op->format == JNO_FMT_DIA && op->ndiag >= 1 && op->ndiag <= 4 && near_count(op) >= 0;
+  if (op->format != JNO_FMT_DIA || op->ndiag < 1 || op->ndiag > 8) return false;
+  const int nn = near_count(op);
+  // instantiated diagonal sets: up to 4 diagonals (Poisson stiffness on Kuhn lattices and their
+  // sub-sets), and 8 diagonals with 3 near ones inside one far window (mass-carrying operators on
+  // the same lattices, e.g. the transient step operator, nx <= 254) or with 1 near one
+  if (nn < 0) return false;
+  if (op->ndiag <= 4) return true;
+  if (op->ndiag != 8) return false;
+  return nn == 1 || (nn == 3 && op->offsets[7] - op->offsets[4] <= LDS_FAR_SPAN);
 }
 
 // pcg_one (one = true) or pcg_k1_lds by diagonal count and near count (FP32 iterations only)
 template <int NT, typename T, typename V>
 static int launch_lds(bool one, const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
-  JNO_REQUIRE(lds_ok(op), JNO_EINVAL, "pcg: the LDS-staged kernels take 1..4 ascending DIA diagonals");
+  JNO_REQUIRE(lds_ok(op), JNO_EINVAL, "pcg: no LDS-staged kernel for this DIA diagonal set");
   if constexpr (std::is_same<T, float>::value) {
     const int key = op->ndiag * 8 + near_count(op);
-#define JNO_LDS_CASE(ND_, NN_)                                     \
+#define JNO_LDS_CASE(ND_, NN_, FW_)                                \
   case ND_ * 8 + NN_:                                              \
     if (one)                                                       \
-      pcg_one<NT, ND_, NN_, T, V><<<grid, NT, 0, s>>>(a);          \
+      pcg_one<NT, ND_, NN_, T, V, FW_><<<grid, NT, 0, s>>>(a);     \
     else                                                           \
       pcg_k1_lds<NT, ND_, NN_, T, V><<<grid, NT, 0, s>>>(a);       \
     break;
     switch (key) {
-      JNO_LDS_CASE(1, 0)
-      JNO_LDS_CASE(2, 0)
-      JNO_LDS_CASE(2, 1)
-      JNO_LDS_CASE(3, 0)
-      JNO_LDS_CASE(3, 1)
-      JNO_LDS_CASE(3, 2)
-      JNO_LDS_CASE(4, 0)
-      JNO_LDS_CASE(4, 1)
-      JNO_LDS_CASE(4, 2)
-      JNO_LDS_CASE(4, 3)
+      JNO_LDS_CASE(1, 0, false)
+      JNO_LDS_CASE(2, 0, false)
+      JNO_LDS_CASE(2, 1, false)
+      JNO_LDS_CASE(3, 0, false)
+      JNO_LDS_CASE(3, 1, false)
+      JNO_LDS_CASE(3, 2, false)
+      JNO_LDS_CASE(4, 0, false)
+      JNO_LDS_CASE(4, 1, false)
+      JNO_LDS_CASE(4, 2, false)
+      JNO_LDS_CASE(4, 3, false)
+      JNO_LDS_CASE(8, 1, false)
+      JNO_LDS_CASE(8, 3, true)
       default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: no LDS-staged kernel for this diagonal set");
     }
 #undef JNO_LDS_CASE

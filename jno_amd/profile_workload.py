@@ -51,7 +51,7 @@ def build_batch(dev, count=256, seed=4):
     return systems
 
 
-def run_heat(dev, steps: int, precision: str, n: int = 128):
+def run_heat(dev, steps: int, precision: str, n: int = 128, replace_factor: float = 0.1):
     """3-D analogue of the reference's transient heat benchmark (benchmarks/fem_bench.py:241-259:
     u_t v + k grad u . grad v on the unit square, IC sin sin, u = 0 on the boundary, T = 0.2 in 1500
     backward-Euler steps): the unit cube at n^3 cells, lognormal k, IC sin sin sin, dt = 0.2/1500,
@@ -64,7 +64,7 @@ def run_heat(dev, steps: int, precision: str, n: int = 128):
     blk = J.SemidiscreteTimeBlock.from_system(s, {"k": k}, dt=0.2 / 1500)
     p = m.points
     u0 = (torch.sin(np.pi * p[:, 0]) * torch.sin(np.pi * p[:, 1]) * torch.sin(np.pi * p[:, 2])).contiguous()
-    sv = J.solve.cg(tol=1e-10, precision=precision)
+    sv = J.solve.cg(tol=1e-10, precision=precision, replace_factor=replace_factor)
     blk.march(u0, 2, linear_solve=sv)  # warm (step operators, layout, code objects)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -72,7 +72,7 @@ def run_heat(dev, steps: int, precision: str, n: int = 128):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     its = blk.last_iterations
-    print(f"heat {n}^3 {precision}: {steps} steps in {dt * 1e3:.1f} ms ({dt / steps * 1e3:.3f} ms/step), "
+    print(f"heat {n}^3 {precision} rf {replace_factor:g}: {steps} steps in {dt * 1e3:.1f} ms ({dt / steps * 1e3:.3f} ms/step), "
           f"iterations/step {min(its)}..{max(its)} (mean {sum(its) / len(its):.1f}), |u_N| {float(u.norm()):.6e}, "
           f"operator {sv.last_stats['operator']}")
 
@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--format", default="auto")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--n", type=int, default=128, help="cells per axis of the heat case")
+    ap.add_argument("--replace-factor", type=float, default=0.1, help="residual replacement factor (heat, mixed)")
     ap.add_argument("--tuning", default=None,
                     help="launch shape block,grid_cap,k1_vector,mode of the iterations (default: the library's)")
     args = ap.parse_args()
@@ -106,14 +107,14 @@ def main():
         print(f"c4 batch: {len(systems)} systems, {sum(A.shape[0] for A, _ in systems)} rows, "
               f"{(time.perf_counter() - t0) * 1e3:.1f} ms, iterations {min(its)}..{max(its)}")
         return
-    if args.case == "heat":
-        run_heat(dev, args.iters, args.precision, args.n)
-        return
     if args.tuning:
         from jno_amd.solve import set_tuning
 
         blk, cap, vec, mode = (int(v) for v in args.tuning.split(","))
         set_tuning(args.precision, blk, cap, vec, mode)
+    if args.case == "heat":
+        run_heat(dev, args.iters, args.precision, args.n, args.replace_factor)
+        return
     A, b = build(args.case, dev)
     # replace_factor tiny: no residual replacement inside the bounded run
     sv = LinearSolver("cg", 1e-30, 0.0, args.iters, precision=args.precision, format=args.format,

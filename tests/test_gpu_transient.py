@@ -106,6 +106,31 @@ def test_time_varying_forcing_matches_oracle(dev):
     assert _rel(_np(u), ss * 0.3) < 8e-2
 
 
+@pytest.mark.parametrize("dims", [(10, 10, 10), (300, 3, 3), (255, 3, 3)])
+def test_mixed_march_on_eight_diagonal_layouts(dev, dims):
+    # the step operator carries the mass diagonals: 8 stored diagonals, with 3 near ones and the
+    # far ones inside one window (10^3), 1 near one (nx = 300), or 2 near ones (nx = 255, not an
+    # LDS-staged set: the generic kernels run).  (With 2 cells across, every coupling off the x
+    # lines touches a constrained node and the nonzero pattern is tridiagonal.)
+    import jno_amd as J
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    m = J.kuhn_box(*dims)
+    p, t = O.kuhn_box(*dims)
+    k = O.k_lognormal(t.shape[0], 9)
+    sysg = J.fem_poisson(m, f=1.0, g=0.2)
+    A, c, d = O.poisson_p1_system(p, t, k, f=1.0, g=0.2)
+    Mz = O.transient_dirichlet(O.mass_matrix_p1(p, t), d)
+    u0 = np.full(p.shape[0], 0.2)
+    dt, N = 2e-3, 4
+    ref = O.theta_march(Mz, A, c, u0, dt, 0.5, N)
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": torch.as_tensor(k, device="cuda")}, dt=dt, theta=0.5)
+    sol = J.solve.cg(tol=1e-10, precision="mixed", replace_factor=1e-3)
+    u = blk.march(torch.as_tensor(u0, device="cuda"), N, linear_solve=sol)
+    assert _rel(_np(u), ref[-1]) < 1e-8
+    assert sol.last_stats["operator"]["ndiag"] == 8
+
+
 def test_heat_decay_of_the_fundamental_mode(dev):
     # tests/test_fem_block_step.py:49-62 on the unit cube (bar 5e-2 as in the reference test)
     import jno_amd as J
