@@ -223,6 +223,11 @@ size_t jno_pcg_workspace_size_op(const jno_op* h_op, int mixed);
 /* Iterations enqueued per host poll of the device-side convergence state (default 16).  No
  * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
 int jno_pcg_set_chunk(int chunk);
+/* Paced chunks for the one-kernel FP32 inner loops (default on): the first chunks of each inner
+ * loop are small and later ones follow the observed convergence rate, so fewer launches are
+ * enqueued past a stop.  0 = fixed chunks of jno_pcg_set_chunk.  Process-wide, no reference
+ * counterpart (a launch-scheduling knob). */
+int jno_pcg_set_pacing(int enable);
 /* Launch shape of the iteration kernels, per iteration precision (no reference counterpart; set
  * by the autotuner; process-wide): precision = 0 for the FP64 iterations of jno_pcg_f64, 1 for
  * the FP32 inner iterations of jno_pcg_mixed; grid_cap = maximum workgroups (multiple of 8,

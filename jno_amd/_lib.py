@@ -101,6 +101,7 @@ SIGNATURES = {
     "jno_pcg_workspace_size": (SZ, [I64, C.c_int]),
     "jno_pcg_workspace_size_op": (SZ, [C.POINTER(JnoOp), C.c_int]),
     "jno_pcg_set_chunk": (C.c_int, [C.c_int]),
+    "jno_pcg_set_pacing": (C.c_int, [C.c_int]),
     "jno_pcg_set_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "jno_pcg_get_tuning": (C.c_int, [C.c_int, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "jno_pcg_profile": (C.c_int, [C.c_int]),

@@ -20,6 +20,7 @@
 // of the state one chunk behind, so the queue never drains while it checks.
 // Workgroups own contiguous row ranges, numbered so that each XCD (workgroups are dealt to the 8
 // XCDs round-robin) gets one contiguous slab: gathered neighbour rows then mostly hit its own L2.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <type_traits>
@@ -1816,16 +1817,33 @@ struct KernelProf {
 static KernelProf g_prof;
 
 // Runs K1/K2 chunks until the device reports a stop; returns the final state in *out.
+// Pacing of an inner loop's launches (one-kernel iterations).  Launches enqueued after the stop
+// exit at once but still cost a launch each (about 3 us), and the host learns of the stop one poll
+// late, so a fixed chunk wastes up to two chunks per stop.  The first two chunks are small (an
+// early stop, e.g. the outer step's, is seen after few launches), and later chunk sizes follow the
+// observed convergence rate of r.r: the launches still needed to reach the stop threshold, less
+// those already queued, plus a small margin.  log_rate carries the rate from one inner loop to the
+// next.
+struct InnerPace {
+  double log_rate = 0.0;  // ln(r.r) per iteration, 0 = unknown
+};
+constexpr int PACE_FIRST = 4;
+constexpr int PACE_MARGIN = 3;
+
 template <typename T, typename V>
 static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid, hipStream_t s,
-                     PcgState* out, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr) {
+                     PcgState* out, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr, InnerPace* pace = nullptr) {
   JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
   int64_t k = k0;
-  int slot = 0, pending = 0;
+  int slot = 0, pending = 0, round = 0;
+  int64_t upto[2] = {k0, k0};  // launches enqueued before each outstanding poll
+  int64_t k_ref = -1;          // first sampled launch of this inner loop and its r.r
+  double rr_ref = 0.0;
   size_t nev = 0;
   std::vector<int64_t> sampled;  // iteration index of each (start, K0, SpMV, K2) event quadruple
+  int n = pace ? std::min(chunk, PACE_FIRST) : chunk;
   for (;;) {
-    for (int i = 0; i < chunk; ++i, ++k) {
+    for (int i = 0; i < n; ++i, ++k) {
       a.p_old = pbuf[k & 1];
       a.p_new = pbuf[(k + 1) & 1];
       if (rbuf) {  // one-kernel iterations: launch index and the r and q halves
@@ -1849,10 +1867,35 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
     }
     JNO_HIP(hipMemcpyAsync(&g_poll.h[slot], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     JNO_HIP(hipEventRecord(g_poll.ev[slot], s));
+    upto[slot] = k;
+    ++round;
+    n = chunk;
     if (pending) {
       const int prev = slot ^ 1;
       JNO_HIP(hipEventSynchronize(g_poll.ev[prev]));
-      if (g_poll.h[prev].done1) break;
+      const PcgState& h = g_poll.h[prev];
+      if (h.done1) break;
+      if (pace && rbuf) {
+        // r.r read by the last launch before that poll (a resuming launch does not record it, but
+        // the first chunk holds more than one launch)
+        const int64_t kl = upto[prev] - 1;
+        const double rr = h.g_hist[kl & 1];
+        if (k_ref < 0) {
+          k_ref = kl;
+          rr_ref = rr;
+        } else if (kl > k_ref && rr > 0.0 && rr_ref > 0.0) {
+          pace->log_rate = std::log(rr / rr_ref) / (double)(kl - k_ref);
+        }
+        const double target = std::max(h.atol2, h.rr_replace);
+        if (pace->log_rate < -1e-3 && rr > 0.0 && target > 0.0) {
+          const double need = rr > target ? std::log(target / rr) / pace->log_rate : 0.0;
+          const int64_t queued = k - upto[prev];
+          const double want = std::ceil(need) - (double)queued + PACE_MARGIN;
+          n = (int)std::max(1.0, std::min((double)chunk, want));
+        }
+      }
+    } else if (pace) {
+      n = std::min(chunk, 2 * PACE_FIRST);  // the second chunk, enqueued before the first poll
     }
     pending = 1;
     slot ^= 1;
@@ -1947,6 +1990,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
 }
 
 static int g_chunk = 16;  // iterations enqueued per host poll (8: 1% slower at 128^3)
+static int g_pace = 1;    // paced chunks for one-kernel inner loops (InnerPace)
 
 }  // namespace jno
 
@@ -2004,6 +2048,11 @@ extern "C" int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, 
 extern "C" int jno_pcg_set_chunk(int chunk) {
   JNO_REQUIRE(chunk >= 1 && chunk <= 1024, JNO_EINVAL, "jno_pcg_set_chunk: chunk must be in 1..1024");
   g_chunk = chunk;
+  return JNO_OK;
+}
+
+extern "C" int jno_pcg_set_pacing(int enable) {
+  g_pace = enable ? 1 : 0;
   return JNO_OK;
 }
 
@@ -2135,9 +2184,11 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   int64_t k = 0, refinements = 0, resumes = 0;
   bool last_resume = false;
   PcgState f;
+  InnerPace pace;
   for (;;) {
     const int64_t k_start = k;
-    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr);
+    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr,
+                                 one && g_pace ? &pace : nullptr);
     if (rc) return rc;
     if (f.outer_done) break;
     k = f.k_stop;

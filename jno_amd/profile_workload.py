@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--format", default="auto")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--n", type=int, default=128, help="cells per axis of the heat case")
+    ap.add_argument("--no-pace", action="store_true", help="fixed chunks for the one-kernel inner loops")
     ap.add_argument("--replace-factor", type=float, default=0.1, help="residual replacement factor (heat, mixed)")
     ap.add_argument("--tuning", default=None,
                     help="launch shape block,grid_cap,k1_vector,mode of the iterations (default: the library's)")
@@ -107,6 +108,10 @@ def main():
         print(f"c4 batch: {len(systems)} systems, {sum(A.shape[0] for A, _ in systems)} rows, "
               f"{(time.perf_counter() - t0) * 1e3:.1f} ms, iterations {min(its)}..{max(its)}")
         return
+    if args.no_pace:
+        from jno_amd.solve import set_pacing
+
+        set_pacing(False)
     if args.tuning:
         from jno_amd.solve import set_tuning
 

@@ -124,6 +124,12 @@ def set_chunk(chunk: int):
     _lib.call("jno_pcg_set_chunk", int(chunk))
 
 
+def set_pacing(enable: bool):
+    """Paced chunks for the one-kernel FP32 inner loops (small first chunks, later ones sized by
+    the observed convergence rate); False = fixed chunks of ``set_chunk``."""
+    _lib.call("jno_pcg_set_pacing", 1 if enable else 0)
+
+
 def _prec_index(precision: str) -> int:
     if precision not in ("fp64", "mixed"):
         raise ValueError("precision must be 'fp64' or 'mixed'")
