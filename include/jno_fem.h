@@ -351,11 +351,11 @@ int jno_theta_operators(int64_t n_rows, const int32_t* d_rowptr, const int32_t* 
                         double theta, double* d_step, double* d_expl, double* d_step_dinv, void* stream);
 
 /* Right-hand side of one step: d_rhs = E u + dt c + dt (theta f(t+dt) + (1-theta) f(t))
- * (backend_blocks.py:342-344).  d_c, and the pair d_f_now/d_f_next, are nullable (terms dropped);
- * max_row is the pattern's longest row.  d_rhs must not alias d_u.  The same call with d_c and the
- * forcing null applies E alone (the transposed step of the adjoint march; E is symmetric). */
-int jno_theta_rhs(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, int max_row,
-                  const double* d_expl, const double* d_u, const double* d_c, const double* d_f_now,
+ * (backend_blocks.py:342-344), E given as an operator layout (jno_op, FP64 values: symmetric DIA
+ * or CSR) built from the d_expl values.  d_c, and the pair d_f_now/d_f_next, are nullable (terms
+ * dropped).  d_rhs must not alias d_u.  The same call with d_c and the forcing null applies E
+ * alone (the transposed step of the adjoint march; E is symmetric). */
+int jno_theta_rhs(const jno_op* h_expl, const double* d_u, const double* d_c, const double* d_f_now,
                   const double* d_f_next, double dt, double theta, double* d_rhs, void* stream);
 
 /* Adjoint of one step w.r.t. the element coefficient: d_w = theta u_next + (1-theta) u_now on free

