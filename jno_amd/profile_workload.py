@@ -51,7 +51,7 @@ def build_batch(dev, count=256, seed=4):
     return systems
 
 
-def run_heat(dev, steps: int, precision: str, n: int = 128, replace_factor: float = 0.1):
+def run_heat(dev, steps: int, precision: str, n: int = 128, replace_factor=None):
     """3-D analogue of the reference's transient heat benchmark (benchmarks/fem_bench.py:241-259:
     u_t v + k grad u . grad v on the unit square, IC sin sin, u = 0 on the boundary, T = 0.2 in 1500
     backward-Euler steps): the unit cube at n^3 cells, lognormal k, IC sin sin sin, dt = 0.2/1500,
@@ -64,7 +64,13 @@ def run_heat(dev, steps: int, precision: str, n: int = 128, replace_factor: floa
     blk = J.SemidiscreteTimeBlock.from_system(s, {"k": k}, dt=0.2 / 1500)
     p = m.points
     u0 = (torch.sin(np.pi * p[:, 0]) * torch.sin(np.pi * p[:, 1]) * torch.sin(np.pi * p[:, 2])).contiguous()
-    sv = J.solve.cg(tol=1e-10, precision=precision, replace_factor=replace_factor)
+    from jno_amd.transient import default_step_solver
+
+    if precision == "mixed" and replace_factor is None:
+        sv = default_step_solver()
+    else:
+        sv = J.solve.cg(tol=1e-10, precision=precision, replace_factor=replace_factor or 0.1)
+    replace_factor = sv.replace_factor
     blk.march(u0, 2, linear_solve=sv)  # warm (step operators, layout, code objects)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -86,7 +92,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--n", type=int, default=128, help="cells per axis of the heat case")
     ap.add_argument("--no-pace", action="store_true", help="fixed chunks for the one-kernel inner loops")
-    ap.add_argument("--replace-factor", type=float, default=0.1, help="residual replacement factor (heat, mixed)")
+    ap.add_argument("--replace-factor", type=float, default=None,
+                    help="residual replacement factor of the heat case (default: the step solver's)")
     ap.add_argument("--tuning", default=None,
                     help="launch shape block,grid_cap,k1_vector,mode of the iterations (default: the library's)")
     args = ap.parse_args()
