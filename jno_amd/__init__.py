@@ -7,9 +7,9 @@ ctypes over PyTorch-ROCm device tensors; there is no CPU fallback.
 """
 from . import interop, precond, solve, transient  # noqa: F401
 from ._lib import JnoError, load  # noqa: F401
-from .fem import CSR, FemLinearSystem, Plan, fem_poisson  # noqa: F401
+from .fem import BCOO, CSR, FemLinearSystem, Plan, fem_poisson  # noqa: F401
 from .mesh import Mesh, Shape, kuhn_box  # noqa: F401
 from .transient import SemidiscreteTimeBlock, theta_march  # noqa: F401
 
-__all__ = ["Shape", "Mesh", "kuhn_box", "Plan", "CSR", "FemLinearSystem", "fem_poisson", "solve", "precond",
+__all__ = ["Shape", "Mesh", "kuhn_box", "Plan", "CSR", "BCOO", "FemLinearSystem", "fem_poisson", "solve", "precond",
            "interop", "transient", "SemidiscreteTimeBlock", "theta_march", "JnoError", "load"]

@@ -137,6 +137,12 @@ class CSR:
         """(data, idx) with idx (nnz, 2) int32 sorted by (row, col), the BCOO view."""
         return self.vals, torch.stack([self.row_indices().to(torch.int32), self.col], dim=1)
 
+    def to_bcoo(self) -> BCOO:
+        """The reference's BCOO of this operator (exact zeros dropped, sorted row-major)."""
+        Az = self.drop_zeros()
+        data, idx = Az.bcoo
+        return BCOO(data, idx.contiguous(), self.shape)
+
     def drop_zeros(self) -> "CSR":
         """``compress_eager`` (jno/utils/solver/fem_utils.py:4063-4091): entries exactly 0.0 removed."""
         n = self.shape[0]
@@ -152,6 +158,27 @@ class CSR:
         _lib.call("jno_drop_zeros_fill", n, _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.vals),
                   _lib.ptr(rowptr_o), _lib.ptr(col_o), _lib.ptr(vals_o), stream)
         return CSR(rowptr_o, col_o[: nnz.value], vals_o[: nnz.value], self.shape, self.dinv)
+
+
+@dataclass
+class BCOO:
+    """The assembled operator as the reference stores it: a ``jax.experimental.sparse.BCOO`` after
+    ``compress_eager`` (jno/utils/solver/fem_utils.py:4063-4091; docs/fem.md:974-983): one entry per
+    (row, col), sorted row-major, exact zeros dropped (P_eager).  ``data`` (nse,) FP64 and
+    ``indices`` (nse, 2) int32 on the device, byte-for-byte the reference's index array."""
+
+    data: torch.Tensor
+    indices: torch.Tensor
+    shape: tuple
+
+    @property
+    def nse(self) -> int:
+        return int(self.data.shape[0])
+
+    def todense(self) -> torch.Tensor:
+        Dm = torch.zeros(self.shape, dtype=torch.float64, device=self.data.device)
+        Dm.index_put_((self.indices[:, 0].long(), self.indices[:, 1].long()), self.data, accumulate=True)
+        return Dm
 
 
 class _MatVec(torch.autograd.Function):
@@ -256,6 +283,12 @@ class FemLinearSystem:
             A._cache[("solver_op", "auto", True)] = op
             A._cache[("solver_op", "auto", False)] = op
         return A, b
+
+    def assembled(self, args: dict):
+        """(A, b) as the reference's non-parametric ``fem.A`` / ``fem.b`` hold them: A as a
+        :class:`BCOO` on P_eager (docs/fem.md:964-983), b the eliminated load."""
+        A, b = self.evaluate(args)
+        return A.to_bcoo(), b
 
     def mass(self) -> CSR:
         """Consistent P1 mass matrix (the u*v form) on the same pattern, no Dirichlet elimination."""

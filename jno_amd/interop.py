@@ -4,8 +4,9 @@ that hold JAX (or CuPy, NumPy) arrays.
 The reference's ``linear=`` slot calls ``fn(op, b, *, M=None, x0=None) -> x`` with a
 ``LinearOperator`` over a BCOO (jno/utils/solver/solver_api.py:243-297, :137-147; a bare callable
 drops in, tests/test_fem_solver_slots.py:94-108).  :func:`linear_slot` returns such a callable.
-Its operator may be a :class:`jno_amd.fem.CSR`, an object with ``bcoo`` = ``(data, idx)`` and
-``shape`` (the reference's LinearOperator), or a ``(data, idx, shape)`` tuple; its vectors may be
+Its operator may be a :class:`jno_amd.fem.CSR`, a BCOO (``data``, ``indices``, ``shape``: JAX's or
+:class:`jno_amd.fem.BCOO`), an object with ``bcoo`` = ``(data, idx)`` and ``shape`` (the reference's
+LinearOperator), or a ``(data, idx, shape)`` tuple; its vectors may be
 any DLPack exporter (``__dlpack__``).  The solution is a torch tensor, itself a DLPack exporter:
 ``jax.numpy.from_dlpack(x)`` on the JAX side.
 """
@@ -64,13 +65,16 @@ def to_csr(op, device: Optional[torch.device] = None) -> CSR:
     """A :class:`CSR` from what the ``linear=`` slot may receive (see the module docstring)."""
     if isinstance(op, CSR):
         return op
+    if hasattr(op, "indices") and hasattr(op, "data") and hasattr(op, "shape"):
+        # a BCOO itself (jax.experimental.sparse.BCOO or jno_amd.fem.BCOO)
+        return csr_from_bcoo(op.data, op.indices, op.shape, device)
     if hasattr(op, "bcoo"):
         data, idx = op.bcoo
         return csr_from_bcoo(data, idx, op.shape, device)
     if isinstance(op, (tuple, list)) and len(op) == 3:
         return csr_from_bcoo(*op, device=device)
-    raise TypeError("the operator must be a jno_amd CSR, an object with .bcoo = (data, idx) and .shape, "
-                    "or a (data, idx, shape) tuple")
+    raise TypeError("the operator must be a jno_amd CSR, a BCOO (.data, .indices, .shape), an object with "
+                    ".bcoo = (data, idx) and .shape, or a (data, idx, shape) tuple")
 
 
 def linear_slot(solver=None, device: Optional[torch.device] = None):

@@ -17,6 +17,9 @@ Pinning (what checks this restatement beyond itself; see tests/test_oracle_*.py)
   * elasticity_p2         rigid-body null space, symmetry, quadratic-field exactness
                           (tests/test_fem_higher_order.py:133-150 restated), P1-on-linear agreement
   * dirichlet_eliminate   identities of symmetric elimination (tests/test_fem_dirichlet_symmetry.py)
+  * bcoo_view             closed-form nse on Kuhn lattices; brute-force dense nonzero positions
+  * theta_* (transient)   amplification factor on discrete eigenmodes, steady limit, analytic decay,
+                          manufactured forced solutions; theta_march_grad_k by central FD
   * pcg                   scipy.sparse.linalg.spsolve (direct, independent of any iteration)
   * adjoint_grad_k        central finite differences with defect-correction solves
 """
@@ -406,6 +409,26 @@ def compress_eager(A):
     keep = np.abs(A.data) > 0.0
     rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
     return sp.csr_matrix((A.data[keep], (rows[keep], A.indices[keep])), shape=A.shape)
+
+
+def bcoo_view(A):
+    """(data, indices) of the BCOO that ``compress_eager`` returns for the assembled operator
+    (jno/utils/solver/fem_utils.py:4063-4091 after ``compress_plan``'s ``np.unique`` of the flattened
+    key row * stride + col, fem_utils.py:4049-4053): one entry per (row, col), sorted row-major,
+    exact zeros dropped, indices int32 (nse, 2)."""
+    A = compress_eager(A)
+    rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+    idx = np.stack([rows, A.indices], axis=1).astype(np.int32)
+    return A.data.copy(), idx
+
+
+def p_eager_nse_kuhn(n):
+    """Closed-form nse of the eliminated P1 Poisson operator on the n^3 Kuhn lattice with the whole
+    boundary constrained: the Kuhn P1 stiffness couples only axis neighbours (the face and body
+    diagonals are exact zeros), elimination leaves one diagonal entry per boundary row, and each
+    interior row keeps its diagonal and its interior axis neighbours."""
+    nb = (n + 1) ** 3 - (n - 1) ** 3
+    return nb + (n - 1) ** 3 + 6 * (n - 1) ** 2 * (n - 2)
 
 
 # ---------------------------------------------------------------------------------------------

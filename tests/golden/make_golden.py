@@ -8,7 +8,9 @@ Fixtures:
   p1_patterns.npz      CSR rowptr/col of the P1 Kuhn lattice pattern at n = 2, 4, 16
   p2v3_pattern_n1.npz  CSR of the P2 vector (3 comps) pattern at n = 1, plus its P2 mesh
   c1_solution.npz      16^3 P1 Poisson, k = 1 + 0.5 sin sin sin: b, dinv, u (scipy direct solve)
-  pattern_sha256.json  SHA-256 of rowptr and col (int32 little endian) and nnz for larger lattices
+  pattern_sha256.json  SHA-256 of rowptr and col (int32 little endian) and nnz for larger lattices;
+                       SHA-256 of the BCOO indices (nse x 2 int32, row-major) of the eliminated P1
+                       operator with exact zeros dropped (P_eager) at n = 32, 64
 """
 from __future__ import annotations
 
@@ -86,6 +88,12 @@ def main():
         rp, col = expand_vec(*node_pattern(c2, p2.shape[0]), 3)
         meta[f"p2v3_n{n}"] = {"nnz": int(len(col)), "rowptr_sha256": sha(rp), "col_sha256": sha(col),
                               "cells_sha256": sha(c2)}
+    for n in (32, 64):
+        # the BCOO of the assembled, eliminated operator (P_eager: exact zeros dropped), k = k_sin
+        p, t = O.kuhn_box(n)
+        A, _, _ = O.poisson_p1_system(p, t, O.k_sin(p, t))
+        _, idx = O.bcoo_view(A)
+        meta[f"p1_eager_n{n}"] = {"nse": int(idx.shape[0]), "indices_sha256": sha(idx)}
     with open(os.path.join(HERE, "pattern_sha256.json"), "w") as fh:
         json.dump(meta, fh, indent=1)
     print("golden fixtures written")

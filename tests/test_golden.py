@@ -58,3 +58,13 @@ def test_sha_fixture_counts():
     p, t = O.kuhn_box(4)
     p2, c2 = O.promote_p2(p, t)
     assert meta["p2v3_n4"]["cells_sha256"] == MG.sha(c2)
+
+
+def test_p_eager_sha_fixture_counts():
+    # the committed P_eager hashes carry the closed-form entry counts
+    meta = json.load(open(os.path.join(HERE, "pattern_sha256.json")))
+    for n in (32, 64):
+        assert meta[f"p1_eager_n{n}"]["nse"] == O.p_eager_nse_kuhn(n)
+    p, t = O.kuhn_box(32)
+    A, _, _ = O.poisson_p1_system(p, t, O.k_sin(p, t))
+    assert MG.sha(O.bcoo_view(A)[1]) == meta["p1_eager_n32"]["indices_sha256"]

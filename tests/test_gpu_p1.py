@@ -569,3 +569,52 @@ def test_autotuner_choices(dev):
     for (Ai, bi), xi in zip(systems, xs):
         # the solver stops on its own FP64 residual at 1e-10; recomputed here in another order
         assert float(torch.linalg.norm(bi - Ai.mv(xi)) / torch.linalg.norm(bi)) <= 1.5e-10
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 32])
+def test_p_eager_bcoo_bytes(dev, n):
+    # the reference's stored operator (compress_eager after the Dirichlet edit): the BCOO index array
+    # byte for byte, the closed-form entry count, the values to the assembly bar (dyadic spacings:
+    # there the vanishing couplings are exact zeros in any rounding)
+    import jno_amd as J
+
+    m = J.kuhn_box(n)
+    p, t = O.kuhn_box(n)
+    k = O.k_sin(p, t)
+    A, b = J.fem_poisson(m).assembled({"k": torch.as_tensor(k, device="cuda")})
+    Ao, bo, d = O.poisson_p1_system(p, t, k)
+    data_o, idx_o = O.bcoo_view(Ao)
+    assert isinstance(A, J.BCOO) and A.indices.dtype == torch.int32 and A.indices.shape == (A.nse, 2)
+    assert A.nse == idx_o.shape[0] == O.p_eager_nse_kuhn(n)
+    assert _np(A.indices).tobytes() == idx_o.tobytes()
+    assert np.abs(_np(A.data) - data_o).max() <= VAL_TOL * np.abs(data_o).max()
+    assert np.all(_np(A.data) != 0.0)
+    if n == 32:
+        import json
+
+        meta = json.load(open("tests/golden/pattern_sha256.json"))
+        import hashlib
+
+        assert hashlib.sha256(_np(A.indices).astype("<i4").tobytes()).hexdigest() == meta["p1_eager_n32"]["indices_sha256"]
+
+
+def test_p_eager_bcoo_golden_64_and_nonzero_g(dev):
+    # 64^3 against the committed hash; a nonzero g and a nodal load leave the pattern unchanged, and
+    # the BCOO solves in the linear= slot like the CSR
+    import hashlib
+    import json
+
+    import jno_amd as J
+
+    meta = json.load(open("tests/golden/pattern_sha256.json"))
+    m = J.kuhn_box(64)
+    p = _np(m.points)
+    k = torch.as_tensor(O.k_sin(p, _np(m.cells)), device="cuda")
+    f = torch.as_tensor(1.0 + p[:, 0], device="cuda")
+    A, b = J.fem_poisson(m, f=f, g=0.25).assembled({"k": k})
+    assert A.nse == meta["p1_eager_n64"]["nse"]
+    assert hashlib.sha256(_np(A.indices).astype("<i4").tobytes()).hexdigest() == meta["p1_eager_n64"]["indices_sha256"]
+    x = J.interop.linear_slot(J.solve.cg(tol=1e-10))(A, b, M=J.precond.jacobi())
+    A2, b2 = J.fem_poisson(m, f=f, g=0.25).evaluate({"k": k})
+    x2 = J.solve.cg(tol=1e-10)(A2, b2, M=J.precond.jacobi())
+    assert _rel(_np(x), _np(x2)) < 1e-9

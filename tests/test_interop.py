@@ -43,6 +43,11 @@ def test_csr_from_bcoo_sorts_and_sums_duplicates():
 
     B = to_csr(Op())
     assert torch.equal(B.rowptr, A.rowptr) and torch.equal(B.col, A.col) and torch.equal(B.vals, A.vals)
+    # and so does a BCOO itself (.data, .indices, .shape, as JAX's BCOO and jno_amd.BCOO carry them)
+    from jno_amd.fem import BCOO
+
+    Cm = to_csr(BCOO(torch.tensor(data), torch.tensor(idx), (n, n)))
+    assert torch.equal(Cm.rowptr, A.rowptr) and torch.equal(Cm.col, A.col) and torch.equal(Cm.vals, A.vals)
 
 
 def test_csr_from_bcoo_rejects_bad_input():

@@ -188,3 +188,23 @@ def test_c_restatement_matches_numpy_oracle():
     xo, ito = O.pcg(A, b, O.jacobi_inv(A), tol=1e-10)
     assert abs(it - ito) <= 1 and rel <= 1e-10 * 1.001
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
+
+
+@pytest.mark.parametrize("n", [2, 4, 8])
+def test_bcoo_view_closed_form_and_brute_force(n):
+    # P_eager of the eliminated operator: the closed-form nse of the Kuhn lattice, and the
+    # nonzero positions of the dense matrix in row-major order (brute force).  Dyadic spacings only:
+    # at n = 3 or 6 the oracle's inverse-Jacobian geometry leaves round-off (~1e-17) where the
+    # couplings vanish in exact arithmetic, so its P_eager carries extra entries there.
+    p, t = O.kuhn_box(n)
+    A, b, d = O.poisson_p1_system(p, t, O.k_lognormal(t.shape[0], n), g=0.5)
+    data, idx = O.bcoo_view(A)
+    assert idx.dtype == np.int32 and idx.shape == (O.p_eager_nse_kuhn(n), 2)
+    if n <= 4:
+        D = A.toarray()
+        rr, cc = np.nonzero(D)  # row-major order
+        assert np.array_equal(idx[:, 0], rr) and np.array_equal(idx[:, 1], cc)
+        assert np.array_equal(data, D[rr, cc])
+    assert np.all(data != 0.0)
+    key = idx[:, 0].astype(np.int64) * (n + 1) ** 3 + idx[:, 1]
+    assert np.all(np.diff(key) > 0)
