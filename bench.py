@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the bounded CPU PCG sample")
     return ap.parse_args()
 
@@ -214,6 +215,10 @@ def main():
         log("[bench] forward + adjoint gradient on 64^3")
         out["gradient"] = grad_rate(J, torch, dev)
 
+    if not args.no_transient:
+        log(f"[bench] transient heat march on {n}^3")
+        out["transient"] = heat_rate(J, torch, dev, n)
+
     if not args.no_cpu:
         log("[bench] CPU baseline (C + OpenMP FP64 restatement, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_seconds, st_fp64["iterations"])
@@ -274,6 +279,37 @@ def grad_rate(J, torch, dev, n=64, reps=3):
         res[f"{prec}_grad_norm"] = float(torch.linalg.norm(g))
     res["workload"] = f"P1 Poisson {n}^3 (T={T}), L = 1/2 (u-u*)^T M (u-u*), dL/dk for all elements"
     res["speedup_mixed_vs_fp64"] = round(res["fp64_s_per_grad"] / res["mixed_s_per_grad"], 3)
+    return res
+
+
+def heat_rate(J, torch, dev, n=128, steps=20):
+    """Backward-Euler steps of the transient heat block u_t v + k grad u . grad v on the unit cube
+    (the 3-D form of the reference's heat benchmark): lognormal k (seed 0), u0 = sin sin sin, u = 0
+    on the boundary, dt = 0.2/1500, each step solved to rtol 1e-10 with warm start."""
+    from jno_amd import solve
+    from jno_amd.transient import default_step_solver
+
+    mesh = J.kuhn_box(n, device=dev)
+    system = J.fem_poisson(mesh, f=0.0)
+    k = torch.tensor(np.exp(0.5 * np.random.default_rng(0).standard_normal(mesh.n_cells)), device=dev)
+    blk = J.SemidiscreteTimeBlock.from_system(system, {"k": k}, dt=0.2 / 1500)
+    p = mesh.points
+    u0 = (torch.sin(np.pi * p[:, 0]) * torch.sin(np.pi * p[:, 1]) * torch.sin(np.pi * p[:, 2])).contiguous()
+    res = {"workload": f"heat {n}^3 P1, theta = 1, dt = 0.2/1500, {steps} steps to rtol 1e-10 (warm start)"}
+    finals = {}
+    for name, sv in (("mixed", default_step_solver()), ("fp64", solve.cg(tol=1e-10))):
+        blk.march(u0, 2, linear_solve=sv)  # warm: step operators, layouts, code objects
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        finals[name] = blk.march(u0, steps, linear_solve=sv)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        its = blk.last_iterations
+        res[f"{name}_ms_per_step"] = round(dt * 1e3, 4)
+        res[f"{name}_iterations_per_step"] = round(sum(its) / len(its), 2)
+        res["step_operator"] = sv.last_stats["operator"]
+    res["mixed_vs_fp64_rel_l2"] = float((finals["mixed"] - finals["fp64"]).norm() / finals["fp64"].norm())
+    res["speedup_mixed_vs_fp64"] = round(res["fp64_ms_per_step"] / res["mixed_ms_per_step"], 3)
     return res
 
 
