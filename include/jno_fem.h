@@ -246,7 +246,9 @@ int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block,
  * iteration, and the durations of the sampled iterations that did work are summed into
  * h_ms[0] (K0: p update, split iterations only), h_ms[1] (the SpMV kernel: K1 = p recompute +
  * SpMV + x update + p.q, or KS = SpMV + p.q) and h_ms[2] (K2: r update, r.z, r.r); *h_count is
- * the number of sampled iterations.  Enabling resets the sums. */
+ * the number of sampled iterations.  One-kernel iterations (mode 5) are timed as windows of 4
+ * consecutive launches between two events, summed into h_ms[1] with 4 counted per window.
+ * Enabling resets the sums. */
 int jno_pcg_profile(int enable);
 int jno_pcg_profile_read(double* h_ms, int64_t* h_count);
 int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,

@@ -1796,7 +1796,10 @@ static int launch_iter(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t
 
 // Optional live kernel timing: on every stride-th iteration, HIP events are recorded on the
 // solve's stream around its K1 and K2 (E0 K1 E1 K2 E2); the sampled iterations that did work are
-// summed.
+// summed.  One-kernel iterations are timed as windows of PROF_WIN consecutive launches between two
+// events (the launches run back to back, so the window's length over PROF_WIN is the average
+// launch duration, with the events' own cost spread over the window).
+constexpr int PROF_WIN = 4;
 struct KernelProf {
   bool on = false;
   int stride = 8;  // time every stride-th iteration (events perturb the queue very little then)
@@ -1841,6 +1844,8 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   double rr_ref = 0.0;
   size_t nev = 0;
   std::vector<int64_t> sampled;  // iteration index of each (start, K0, SpMV, K2) event quadruple
+                                 // (one-kernel iterations: first launch of each event pair's window)
+  int64_t win = -1;              // first launch of the open timing window (one-kernel iterations)
   int n = pace ? std::min(chunk, PACE_FIRST) : chunk;
   for (;;) {
     for (int i = 0; i < n; ++i, ++k) {
@@ -1853,7 +1858,21 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
         a.q = qbuf[k & 1];
         a.q_out = qbuf[(k + 1) & 1];
       }
-      if (g_prof.on && k % g_prof.stride == 0) {
+      if (g_prof.on && rbuf) {
+        if (k % g_prof.stride == 0) {
+          if (g_prof.ensure(nev + 2) != JNO_OK) return JNO_EHIP;
+          JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
+          win = k;
+        }
+        int rc = launch_iter<T, V>(op, a, grid, s, nullptr);
+        if (rc) return rc;
+        if (win >= 0 && k == win + PROF_WIN - 1) {
+          JNO_HIP(hipEventRecord(g_prof.ev[nev + 1], s));
+          sampled.push_back(win);
+          nev += 2;
+          win = -1;
+        }
+      } else if (g_prof.on && k % g_prof.stride == 0) {
         if (g_prof.ensure(nev + 4) != JNO_OK) return JNO_EHIP;
         JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
         int rc = launch_iter<T, V>(op, a, grid, s, &g_prof.ev[nev + 1]);
@@ -1903,7 +1922,15 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   JNO_HIP(hipMemcpyAsync(&g_poll.h[0], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
   JNO_HIP(hipStreamSynchronize(s));
   *out = g_poll.h[0];
-  if (g_prof.on) {
+  if (g_prof.on && rbuf) {
+    for (size_t t = 0; t < sampled.size(); ++t) {
+      if (sampled[t] + PROF_WIN - 1 >= out->k_stop) break;  // every launch of the window did work
+      float ms = 0;
+      JNO_HIP(hipEventElapsedTime(&ms, g_prof.ev[2 * t], g_prof.ev[2 * t + 1]));
+      g_prof.ms[1] += ms;
+      g_prof.n += PROF_WIN;
+    }
+  } else if (g_prof.on) {
     for (size_t t = 0; t < sampled.size(); ++t) {
       if (sampled[t] >= out->k_stop) break;  // launches after the stop did no work
       for (int j = 0; j < 3; ++j) {
