@@ -65,11 +65,15 @@ def choose_format(A, allow=("dia", "bsr3", "csr")):
 CANDIDATE_SHAPES = tuple(
     (blk, cap, vec, split)
     for blk in (256, 512, 1024)
-    for cap in (256, 512, 1024)
-    for vec in (0, 1, 3)
+    for cap in (256, 512, 768, 1024)
+    for vec in (0, 1, 2, 3)
     for split in (0, 1, 3, 4, 5)
-    # LDS staging (vec 3) is a variant of mode 4; the one-kernel mode 5 has a single kernel shape
-    if (vec != 3 or split == 4) and (split != 5 or vec == 0)
+    # LDS staging (vec 3) is a variant of mode 4; vec 2 is tried only as the register-resident
+    # one-kernel variant of mode 5 (256 and 512 threads), and cap 768 only there
+    if (vec != 3 or split == 4)
+    and (vec != 2 or (split == 5 and blk <= 512))
+    and (cap != 768 or (split == 5 and vec == 2))
+    and (split != 5 or vec in (0, 2))
 )
 _LAUNCH_CACHE: dict = {}
 

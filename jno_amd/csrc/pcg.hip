@@ -43,13 +43,24 @@ constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup pa
 struct Tuning {
   int grid_cap;  // workgroups of the iteration kernels (a multiple of 8)
   int vec;       // DIA SpMV kernel: 0 two rows per thread, 1 VN-row 16-byte tiles, 2 four rows,
-                 // 3 p staged in LDS (mode 4 K1; the others take 0)
+                 // 3 p staged in LDS (mode 4 K1; the others take 0).  Mode 5: 0 pcg_one (LDS-staged),
+                 // 1 and 2 pcg_one_q (register-resident quads, far loads before / after the head)
   int block;     // threads per workgroup (256, 512, 1024)
   int split;     // iteration mode: 0 K1 K2, 1 K0 KS K2 (p materialized), 2 KA KB (single reduction),
                  // 3 K0 KS K2, 4 K1 K2 and 5 one kernel on the Jacobi-scaled operator (FP32 inner
                  // iterations on DIA)
 };
-static Tuning g_tune[2] = {{512, 0, 1024, 1}, {512, 0, 512, 5}};  // [0] FP64 iterations, [1] FP32 iterations
+// Iteration modes (Tuning::split): two-reduction CG as K1 K2 (p recomputed inside the SpMV
+// kernel) or K0 KS K2 (p materialized), or single-reduction CG as KA KB.
+// MODE_SCALED / MODE_SCALED_FUSED / MODE_ONE: the split / fused / one-kernel iteration on the
+// Jacobi-scaled FP32 operator (FP32 inner iterations on a DIA operator; the unscaled split / fused /
+// fused iteration elsewhere)
+enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SCALED_FUSED = 4, MODE_ONE = 5 };
+// FP32: the register-resident one-kernel iteration with late far loads (k1_vector 2), 256 threads x
+// 768 workgroups (three tiles of ~930 rows each at 128^3); operators it does not take run the
+// LDS-staged one (ONE_FALLBACK below)
+static Tuning g_tune[2] = {{512, 0, 1024, 1}, {768, 2, 256, 5}};  // [0] FP64 iterations, [1] FP32 iterations
+constexpr Tuning ONE_FALLBACK = {512, 0, 512, 5};
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -584,6 +595,41 @@ __global__ void __launch_bounds__(NT) pcg_k1_lds(PcgArgs<T, V> a) {
   store_out<NT, 1>(&part, a.part.A, gridDim.x, sm_out);
 }
 
+// The scalars of a one-kernel launch k from the reduced partials in = (r.r, p.q, q.r, q.q) of launch
+// k-1 (see pcg_one below), and its stop decision, identical in every workgroup.  Returns false when
+// the launch stops (block 0 records why and raises the stop flags).  special: 0 a plain step, 1 a
+// first launch (alpha = beta = 0), 2 a rerun after a residual replacement (alpha = 0, beta from the
+// recorded r.r two launches back).
+__device__ __forceinline__ bool one_decide(PcgState* st, int64_t k, const double* in, double& alpha, double& beta,
+                                           int& special) {
+  const double rr = in[0];
+  special = k == st->k_special ? st->special : 0;
+  alpha = 0.0;
+  beta = 0.0;
+  if (special == 2) {
+    beta = rr / st->g_hist[k & 1];
+  } else if (special == 0) {
+    alpha = rr / in[1];
+    beta = (rr - 2.0 * alpha * in[2] + alpha * alpha * in[3]) / rr;
+  }
+  const bool stop_conv = special != 2 && !(rr > st->atol2);
+  const bool stop_cap = k >= st->maxiter;
+  const bool stop_repl = special == 0 && rr <= st->rr_replace;
+  const bool bad = !isfinite(rr) || (!stop_conv && (!isfinite(alpha) || !isfinite(beta)));
+  if (stop_conv || stop_cap || stop_repl || bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->k_stop = k;
+      st->rr_stop = rr;
+      st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
+      st->done1 = 1;
+      st->done = 1;  // every workgroup of this launch takes the same decision
+    }
+    return false;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && special != 2) st->g_hist[k & 1] = rr;
+  return true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // One kernel per iteration on the Jacobi-scaled DIA operator (iteration mode 5).  Launch k reads
 // r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1}, x and the partial sums of launch k-1: gamma_{k-1} = r.r
@@ -659,30 +705,9 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   load1(r0 - h + (int)threadIdx.x, min(r1, r0 + tl) + h);
   double in[4];
   reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
-  const double rr = in[0];
-  const int special = k == st->k_special ? st->special : 0;
-  double alpha = 0.0, beta = 0.0;
-  if (special == 2) {
-    beta = rr / st->g_hist[k & 1];
-  } else if (special == 0) {
-    alpha = rr / in[1];
-    beta = (rr - 2.0 * alpha * in[2] + alpha * alpha * in[3]) / rr;
-  }
-  const bool stop_conv = special != 2 && !(rr > st->atol2);
-  const bool stop_cap = k >= st->maxiter;
-  const bool stop_repl = special == 0 && rr <= st->rr_replace;
-  const bool bad = !isfinite(rr) || (!stop_conv && (!isfinite(alpha) || !isfinite(beta)));
-  if (stop_conv || stop_cap || stop_repl || bad) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->k_stop = k;
-      st->rr_stop = rr;
-      st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
-      st->done1 = 1;
-      st->done = 1;  // every workgroup of this launch takes the same decision
-    }
-    return;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && special != 2) st->g_hist[k & 1] = rr;
+  double alpha, beta;
+  int special;
+  if (!one_decide(st, k, in, alpha, beta, special)) return;
   // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const T al = (T)alpha, be = (T)beta;
   const bool upd = special == 0;
@@ -805,6 +830,240 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
     if (t1 < r1) __syncthreads();  // the next tile rewrites pt and rt
   }
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The one-kernel iteration with register-resident rows (mode 5, k1_vector 1), for 4-diagonal
+// operators.  Same arithmetic and launch contract as pcg_one; the difference is how the memory
+// traffic is issued.  A thread owns a quad of 4 consecutive rows (16-byte loads and stores) and a
+// workgroup's tile is one quad per thread, TT = 4 NT rows.  Every global load of a tile is issued at
+// once: the partial sums of the previous launch first, then the quad's r, q, p, its upper and lower
+// off-diagonal values, r, q, p of its far neighbours (offsets above LDS_HALO, recomputed in
+// registers), and for the first 2 hq threads one row of r, q, p of the near halo (x follows the
+// head: it is needed only at the end).  The reduction
+// of the partials waits only for the loads issued before the tile's, so it overlaps the tile's round
+// trip; one round trip then covers the whole tile.  Only p_k of the tile and its halo goes through
+// LDS (the near neighbours of the SpMV); r_k and the far p_k stay in registers.
+// Bounds: the iteration vectors carry PCG_PAD zero rows on both sides (ws_layout, zero_pads_k).  A
+// quad that straddles row 0 or row n reads pad rows, which are 0 in every vector and enter only
+// through a zero coefficient (the stored upper diagonal is 0 past the last row, the lower one is
+// selected to 0 before the first); a quad wholly outside [0, n) loads rows 0..3 instead, finite
+// values that only meet zero coefficients or rows past n, whose q is selected to 0.  Pad rows of a
+// straddling quad are stored back as the 0 they compute to.
+// ---------------------------------------------------------------------------------------------
+constexpr int PCG_PAD_BYTES = 128;  // zero pad before and after each iteration vector
+constexpr int QRED_G = 1152;        // workgroups (partial sums per array) the head reduction takes
+
+// LF (k1_vector 2): the far neighbours' loads are issued after the head instead, and combined into
+// p_k right before the barrier: fewer registers live at the head (more waves per SIMD) for a second,
+// partly hidden round trip.
+template <int NT, int ND, int NN, bool LF = false>
+__global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
+  static_assert(ND == 4 && NN >= 2 && NN <= 3, "instantiated for 4-diagonal operators with 2 or 3 near diagonals");
+  constexpr int NF = ND - 1 - NN;  // far off-diagonals
+  constexpr int NFA = NF > 0 ? NF : 1;
+  constexpr int TT = 4 * NT;  // rows per tile
+  constexpr int NW = NT / 64;
+  constexpr int QRED_F = (4 * QRED_G + NT - 1) / NT;  // partial sums loaded per thread
+  using Q = V16<float>;
+  __shared__ double sm_w[4 * NW];
+  __shared__ double sm_out[4 * NW];
+  __shared__ __align__(16) float pt[TT + 2 * LDS_HALO];  // p_k of the tile and its near halo
+  PcgState* st = a.st;
+  if (st->done) return;
+  const int G = gridDim.x;
+  const int tid = (int)threadIdx.x;
+  const int64_t k = a.kk;
+  const int64_t r0l = block_first_row(a.rows_per_block);
+  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + a.rows_per_block);
+  const float* vals = a.vals;
+  const int ld = (int)dia_ld(a.n);
+  int o[ND];
+#pragma unroll
+  for (int d = 1; d < ND; ++d) o[d] = a.off.o[d];
+  const int hq = (o[NN] + 3) & ~3;  // near halo rows on either side, whole quads (<= LDS_HALO)
+  // equal tiles of at most TT rows, whole quads (r0 is a multiple of 4)
+  const int nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
+  const int tl = nt > 0 ? (((r1 - r0 + nt - 1) / nt + 3) & ~3) : TT;
+  // the head's partial-sum loads go out first: the 4 arrays of G partials as one flat range, entry
+  // f = tid + u NT in thread tid
+  const double* pin = a.part.C + (k & 1) * 4 * (int64_t)G;
+  const int G4 = 4 * G;
+  double pv_in[QRED_F];
+#pragma unroll
+  for (int u = 0; u < QRED_F; ++u) pv_in[u] = pin[min(u * NT + tid, G4 - 1)];
+  // the tile's loads
+  constexpr int UH = (2 * LDS_HALO + NT - 1) / NT;  // near halo rows per thread
+  Q rv, qv, pv, xv, vu[ND], vl[ND], fur[NFA], fuq[NFA], fup[NFA], fdr[NFA], fdq[NFA], fdp[NFA];
+  float hr[UH], hqv[UH], hp[UH];
+  int i = 0, jl[ND];
+  bool own = false, hin[UH];
+  auto load_far = [&](int ic) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int od = o[NN + 1 + f];
+      const int ju = ic + od < n ? ic + od : 0;
+      const int jd = ic - od >= -3 ? ic - od : 0;
+      fur[f] = ld16(a.r + ju);
+      fuq[f] = ld16(a.q + ju);
+      fup[f] = ld16(a.p_old + ju);
+      fdr[f] = ld16(a.r + jd);
+      fdq[f] = ld16(a.q + jd);
+      fdp[f] = ld16(a.p_old + jd);
+    }
+  };
+  auto load_tile = [&](int t0, int t1) {
+    i = t0 + 4 * tid;
+    own = i < t1;
+    const int ic = own ? i : 0;
+    rv = ld16(a.r + ic);
+    qv = ld16(a.q + ic);
+    pv = ld16(a.p_old + ic);
+#pragma unroll
+    for (int d = 1; d < ND; ++d) {
+      const float* vd = vals + (int64_t)d * ld;
+      vu[d] = ld16(vd + ic);
+      jl[d] = ic - o[d];
+      vl[d] = ld16(vd + (jl[d] >= -3 ? jl[d] : 0));
+    }
+    if (!LF) load_far(ic);
+    // halo rows [t0 - hq, t0) and [t1q, t1q + hq), t1q = the tile end in whole quads
+    const int t1q = t0 + ((t1 - t0 + 3) & ~3);
+#pragma unroll
+    for (int u = 0; u < UH; ++u) {
+      const int h = tid + u * NT;
+      const int row = h < hq ? t0 - hq + h : t1q + (h - hq);
+      hin[u] = h < 2 * hq && row >= 0 && row < n;
+      const int hc = hin[u] ? row : 0;
+      hr[u] = a.r[hc];
+      hqv[u] = a.q[hc];
+      hp[u] = a.p_old[hc];
+    }
+  };
+  if (nt > 0) load_tile(r0, min(r1, r0 + tl));
+  // head: reduce the partials of launch k-1 (fixed order: per thread, per wave, then the waves)
+  double in[4];
+  {
+    const int w = tid >> 6;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < QRED_F; ++u) {
+      const int f = u * NT + tid;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) s[m] += f >= m * G && f < (m + 1) * G ? pv_in[u] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double t = wave_sum(s[m]);
+      if ((tid & 63) == 0) sm_w[m * NW + w] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double t = 0.0;
+#pragma unroll
+      for (int v = 0; v < NW; ++v) t += sm_w[m * NW + v];
+      in[m] = t;
+    }
+  }
+  double alpha, beta;
+  int special;
+  if (!one_decide(st, k, in, alpha, beta, special)) return;
+  // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
+  const float al = (float)alpha, be = (float)beta;
+  const bool upd = special == 0;
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t0 = r0; t0 < r1; t0 += tl) {
+    const int t1 = min(r1, t0 + tl);
+    if (t0 > r0) {
+      __syncthreads();  // the previous tile's SpMV has read pt
+      load_tile(t0, t1);
+    }
+    if (LF) load_far(own ? i : 0);
+    if (upd) xv = ld16(a.x + (own ? i : 0));
+    const int t1q = t0 + ((t1 - t0 + 3) & ~3);
+    // r_k, p_k of the own quad; p_k of the halo rows and of the far neighbours
+    Q rn, pn;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      rn.v[e] = rv.v[e] - al * qv.v[e];
+      pn.v[e] = rn.v[e] + be * pv.v[e];
+    }
+    if (own) {
+      st16(a.r_out + i, rn);
+      st16(a.p_new + i, pn);
+      *reinterpret_cast<Q*>(&pt[hq + 4 * tid]) = pn;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) part[0] += (double)rn.v[e] * (double)rn.v[e];
+    }
+#pragma unroll
+    for (int u = 0; u < UH; ++u) {
+      const int h = tid + u * NT;
+      if (h < 2 * hq) pt[h < hq ? h : hq + (t1q - t0) + (h - hq)] = hin[u] ? (hr[u] - al * hqv[u]) + be * hp[u] : 0.0f;
+    }
+    Q fu[NFA], fd[NFA];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        fu[f].v[e] = (fur[f].v[e] - al * fuq[f].v[e]) + be * fup[f].v[e];
+        fd[f].v[e] = (fdr[f].v[e] - al * fdq[f].v[e]) + be * fdp[f].v[e];
+      }
+    __syncthreads();
+    if (own) {
+      Q qn;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int li = hq + 4 * tid + e;
+        float s = pn.v[e];
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+          float xu, xl;
+          if (d <= NN) {
+            xu = pt[li + o[d]];
+            xl = pt[li - o[d]];
+          } else {
+            xu = fu[d - NN - 1].v[e];
+            xl = fd[d - NN - 1].v[e];
+          }
+          const float cl = jl[d] + e >= 0 ? vl[d].v[e] : 0.0f;
+          s += vu[d].v[e] * xu;
+          s += cl * xl;
+        }
+        s = i + e < n ? s : 0.0f;
+        qn.v[e] = s;
+        part[1] += (double)pn.v[e] * (double)s;
+        part[2] += (double)s * (double)rn.v[e];
+        part[3] += (double)s * (double)s;
+      }
+      st16(a.q_out + i, qn);
+      if (upd) {
+        Q xn;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xn.v[e] = xv.v[e] + al * pv.v[e];
+        st16(a.x + i, xn);
+      }
+    }
+  }
+  store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+}
+
+// Zero the pads of up to 8 iteration vectors (PCG_PAD_BYTES before and after each of n elements).
+struct PadList {
+  float* v[8];
+  int count;
+};
+
+__global__ void zero_pads_k(int64_t n, PadList l) {
+  constexpr int P = PCG_PAD_BYTES / (int)sizeof(float);
+  const int t = (int)threadIdx.x;  // 2 P threads
+  for (int b = 0; b < l.count; ++b) {
+    float* v = l.v[b];
+    if (t < P)
+      v[t - P] = 0.0f;
+    else if (t < 2 * P)
+      v[n + t - P] = 0.0f;
+  }
 }
 
 __global__ void set_special_k(PcgState* st, int64_t k, int kind) {
@@ -1510,6 +1769,13 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_el
     off += a256(bytes);
     return p;
   };
+  // iteration vectors with PCG_PAD_BYTES of pad before and after (16-byte loads of quads that
+  // straddle either end stay inside the allocation)
+  auto takev = [&](size_t bytes) -> void* {
+    void* p = base ? base + off + PCG_PAD_BYTES : nullptr;
+    off += a256(bytes + 2 * PCG_PAD_BYTES);
+    return p;
+  };
   Ws t;
   t.st = (PcgState*)take(sizeof(PcgState));
   t.part.A = (double*)take(PCG_MAX_BLOCKS * sizeof(double));
@@ -1517,16 +1783,16 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_el
   t.part.P = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.part.B2 = (double*)take(2 * PCG_MAX_BLOCKS * sizeof(double));
   t.part.C = (double*)take(8 * PCG_MAX_BLOCKS * sizeof(double));
-  t.r = take(n * vb);
-  t.p0 = take(n * vb);
-  t.p1 = take(n * vb);
-  t.q = take(n * vb);
-  t.s = take(n * vb);
-  t.xs = mixed ? take(n * 4) : nullptr;
+  t.r = takev(n * vb);
+  t.p0 = takev(n * vb);
+  t.p1 = takev(n * vb);
+  t.q = takev(n * vb);
+  t.s = takev(n * vb);
+  t.xs = mixed ? takev(n * 4) : nullptr;
   t.dinv32 = mixed ? (float*)take(n * 4) : nullptr;
   t.ax = (double*)take(n * 8);
   t.sdia = sdia_elems > 0 ? (float*)take(sdia_elems * 4) : nullptr;
-  t.q2 = sdia_elems > 0 ? take(n * 4) : nullptr;
+  t.q2 = sdia_elems > 0 ? takev(n * 4) : nullptr;
   if (w) *w = t;
   return off;
 }
@@ -1539,8 +1805,28 @@ static int64_t scaled_elems(const jno_op* op) {
 // grid of the iteration kernels: a multiple of 8 (XCD slabs), at most PCG_MAX_BLOCKS, and the
 // rows per workgroup: a multiple of 4 so every workgroup's first row is 16-byte aligned for FP32
 // and FP64, and of 12 for BSR3 (whole 3-row blocks)
+static bool quad_ok(const jno_op* op);
+// The launch shape an FP32 solve of `op` runs: the register-resident one-kernel shapes (mode 5 with
+// k1_vector 1 or 2) apply to the 4-diagonal operators it is built for, at 256 or 512 threads; any
+// other operator (BSR3, CSR, 8-diagonal DIA) runs the previous default shape instead.
+static Tuning effective_tuning(const Tuning& tu, const jno_op* op) {
+  if (tu.split == MODE_ONE && (tu.vec == 1 || tu.vec == 2) && !(quad_ok(op) && tu.block <= 512)) return ONE_FALLBACK;
+  return tu;
+}
 static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
   const int64_t n = op->n;
+  if (tu.split == MODE_ONE && (tu.vec == 1 || tu.vec == 2)) {
+    // pcg_one_q: one tile of 4 block rows per workgroup where the grid cap allows (rounded up to
+    // whole XCD slabs), else several equal tiles per workgroup
+    int64_t want = cdiv(n, (int64_t)tu.block * 4);
+    if (want >= N_XCD) want = cdiv(want, (int64_t)N_XCD) * N_XCD;
+    const int64_t cap = std::min<int64_t>(tu.grid_cap, QRED_G);
+    if (want > cap) want = std::max<int64_t>(N_XCD, (cap / N_XCD) * N_XCD);
+    if (want < 1) want = 1;
+    *G = (int)want;
+    *rpb = cdiv(cdiv(n, want), (int64_t)4) * 4;
+    return;
+  }
   const int64_t lane_rows = op->format == JNO_FMT_DIA    ? n
                             : op->format == JNO_FMT_BSR3 ? (n / 3) * op->lanes
                                                          : n * op->lanes;
@@ -1568,12 +1854,6 @@ struct HostPoll {
 };
 static thread_local HostPoll g_poll;
 
-// Iteration modes (Tuning::split): two-reduction CG as K1 K2 (p recomputed inside the SpMV
-// kernel) or K0 KS K2 (p materialized), or single-reduction CG as KA KB.
-// MODE_SCALED / MODE_SCALED_FUSED / MODE_ONE: the split / fused / one-kernel iteration on the
-// Jacobi-scaled FP32 operator (FP32 inner iterations on a DIA operator; the unscaled split / fused /
-// fused iteration elsewhere)
-enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SCALED_FUSED = 4, MODE_ONE = 5 };
 
 // The SpMV-carrying kernel of one iteration: K1 (MODE_FUSED), KS (MODE_SPLIT) or KB
 // (MODE_SINGLE), by format, DIA diagonal count (4: the 3-D Kuhn lattice; 3: the 2-D triangle
@@ -1673,6 +1953,14 @@ static bool lds_ok(const jno_op* op) {
   return nn == 1 || (nn == 3 && op->offsets[7] - op->offsets[4] <= LDS_FAR_SPAN);
 }
 
+// the register-resident one-kernel iteration: 4 ascending diagonals with 2 or 3 near ones (Kuhn
+// lattices up to 254 cells along the first axis)
+static bool quad_ok(const jno_op* op) {
+  if (op->format != JNO_FMT_DIA || op->ndiag != 4) return false;
+  const int nn = near_count(op);
+  return nn == 2 || nn == 3;
+}
+
 // pcg_one (one = true) or pcg_k1_lds by diagonal count and near count (FP32 iterations only)
 template <int NT, typename T, typename V>
 static int launch_lds(bool one, const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
@@ -1709,12 +1997,35 @@ static int launch_lds(bool one, const jno_op* op, const PcgArgs<T, V>& a, int gr
   }
 }
 
+// pcg_one_q by near count (4-diagonal operators, FP32 iterations)
+template <int NT, typename T, typename V>
+static int launch_quad(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  JNO_REQUIRE(quad_ok(op), JNO_EINVAL, "pcg: no register-resident kernel for this DIA diagonal set");
+  JNO_REQUIRE(grid <= QRED_G, JNO_EINVAL, "pcg: %d workgroups exceed the head reduction of %d", grid, QRED_G);
+  if constexpr (std::is_same<T, float>::value && std::is_same<V, float>::value && NT <= 512) {
+    const int key = near_count(op) * 2 + (a.vec == 2 ? 1 : 0);
+    switch (key) {
+      case 4: pcg_one_q<NT, 4, 2><<<grid, NT, 0, s>>>(a); break;
+      case 5: pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a); break;
+      case 6: pcg_one_q<NT, 4, 3><<<grid, NT, 0, s>>>(a); break;
+      default: pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a); break;
+    }
+    JNO_LAUNCH_CHECK();
+    return JNO_OK;
+  } else {
+    JNO_REQUIRE(false, JNO_EINVAL, "pcg: the register-resident kernel runs FP32 iterations, 256 or 512 threads");
+  }
+}
+
 template <int NT, typename T, typename V>
 static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStream_t s, hipEvent_t* ev) {
   int rc;
   if (a.unit && a.split == MODE_ONE) {
     if (ev) JNO_HIP(hipEventRecord(ev[0], s));
-    rc = launch_lds<NT, T, V>(true, op, a, grid, s);
+    if ((a.vec == 1 || a.vec == 2) && NT <= 512 && quad_ok(op))
+      rc = launch_quad<NT, T, V>(op, a, grid, s);
+    else
+      rc = launch_lds<NT, T, V>(true, op, a, grid, s);
     if (rc) return rc;
     if (ev) JNO_HIP(hipEventRecord(ev[1], s));
     if (ev) JNO_HIP(hipEventRecord(ev[2], s));
@@ -2057,7 +2368,8 @@ extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, in
               "iteration on the Jacobi-scaled operator)");
   JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 3, JNO_EINVAL,
               "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles), 2 (4 rows) or 3 "
-              "(p staged in LDS, iteration mode 4)");
+              "(p staged in LDS, iteration mode 4); with mode 5: 0 the LDS-staged kernel, 1 and 2 the "
+              "register-resident one (far loads before / after the head)");
   g_tune[precision] = Tuning{grid_cap, k1_vector, block, split};
   return JNO_OK;
 }
@@ -2186,6 +2498,8 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
     zero_k<float><<<gz, PCG_NT, 0, s>>>(rb[1], n);
     zero_k<float><<<gz, PCG_NT, 0, s>>>(qb[0], n);
     zero_k<float><<<gz, PCG_NT, 0, s>>>(qb[1], n);
+    PadList pl = {{rb[0], rb[1], pb[0], pb[1], qb[0], qb[1], xs, nullptr}, 7};
+    zero_pads_k<<<1, 2 * PCG_PAD_BYTES / (int)sizeof(float), 0, s>>>(n, pl);
   }
   int rc;
   if (use_x0) {
@@ -2265,7 +2579,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
               "jno_pcg_mixed: replace_factor must be in (0, 1)");
   JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_mixed: maxiter must be >= 0");
   hipStream_t s = as_stream(stream);
-  const Tuning tu = g_tune[1];
+  const Tuning tu = effective_tuning(g_tune[1], h_op);
   const int64_t se = scaled_elems(h_op);
   const bool room = se > 0 && ws_bytes >= ws_layout(n, 1, nullptr, nullptr, se);
   Ws w;

@@ -420,23 +420,27 @@ def test_lds_staged_k1(dev, n):
     saved = solve.get_tuning("mixed")
     its = {}
     try:
-        for vec, mode in ((3, 4), (0, 5)):
-            for blk, cap in ((1024, 8), (256, 64), (1024, 512)):
+        # (3, 4): K1 with p staged in LDS; (0, 5): one kernel per iteration, LDS-staged; (1, 5) and
+        # (2, 5): one kernel per iteration with register-resident 4-row quads, far neighbours loaded
+        # before or after the head (256 and 512 threads; 1024 falls back to the LDS-staged kernel)
+        for vec, mode in ((3, 4), (0, 5), (1, 5), (2, 5)):
+            for blk, cap in ((1024, 8), (256, 64), (1024, 512), (512, 8), (256, 4096)):
                 for rf in (0.1, 0.01):
                     solve.set_tuning("mixed", blk, cap, vec, mode)
                     sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=rf)
                     u = sv(A, b, M=precond.jacobi())
                     st = sv.last_stats
-                    assert st["relres"] <= 1e-10, (mode, blk, cap, rf, st)
-                    assert _rel(_np(u), ref) <= SOL_TOL, (mode, blk, cap, rf, st)
+                    assert st["relres"] <= 1e-10, (vec, mode, blk, cap, rf, st)
+                    assert _rel(_np(u), ref) <= SOL_TOL, (vec, mode, blk, cap, rf, st)
                     assert st["refinements"] >= 2, st
-                    its.setdefault((mode, rf), []).append(st["iterations"])
+                    its.setdefault((vec, mode, rf), []).append(st["iterations"])
     finally:
         solve.set_tuning("mixed", *saved)
     # the one-kernel iteration keeps the search direction across replacements: about the iteration
     # count of the two-kernel one (plus one rerun launch per replacement)
     for rf in (0.1, 0.01):
-        assert max(its[(5, rf)]) <= 1.1 * min(its[(4, rf)]) + 12, its
+        for vec in (0, 1, 2):
+            assert max(its[(vec, 5, rf)]) <= 1.1 * min(its[(3, 4, rf)]) + 12, its
 
 
 def test_adjoint_gradient_vs_oracle_and_fd(dev):
