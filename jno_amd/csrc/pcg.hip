@@ -854,9 +854,10 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
 constexpr int PCG_PAD_BYTES = 128;  // zero pad before and after each iteration vector
 constexpr int QRED_G = 1152;        // workgroups (partial sums per array) the head reduction takes
 
-// LF (k1_vector 2): the far neighbours' loads are issued after the head instead, and combined into
-// p_k right before the barrier: fewer registers live at the head (more waves per SIMD) for a second,
-// partly hidden round trip.
+// LF (k1_vector 2 and 3): the far neighbours' loads (and x) are issued after the head instead, and
+// combined into p_k right before the barrier: fewer registers live at the head, for a second, partly
+// hidden round trip.  (Issuing the next tile's loads during the current tile's SpMV phase measured
+// slower: 211 VGPRs.)
 template <int NT, int ND, int NN, bool LF = false>
 __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   static_assert(ND == 4 && NN >= 2 && NN <= 3, "instantiated for 4-diagonal operators with 2 or 3 near diagonals");
@@ -865,6 +866,7 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   constexpr int TT = 4 * NT;  // rows per tile
   constexpr int NW = NT / 64;
   constexpr int QRED_F = (4 * QRED_G + NT - 1) / NT;  // partial sums loaded per thread
+  constexpr int UH = (2 * LDS_HALO + NT - 1) / NT;    // near halo rows per thread
   using Q = V16<float>;
   __shared__ double sm_w[4 * NW];
   __shared__ double sm_out[4 * NW];
@@ -892,55 +894,62 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   double pv_in[QRED_F];
 #pragma unroll
   for (int u = 0; u < QRED_F; ++u) pv_in[u] = pin[min(u * NT + tid, G4 - 1)];
-  // the tile's loads
-  constexpr int UH = (2 * LDS_HALO + NT - 1) / NT;  // near halo rows per thread
-  Q rv, qv, pv, xv, vu[ND], vl[ND], fur[NFA], fuq[NFA], fup[NFA], fdr[NFA], fdq[NFA], fdp[NFA];
-  float hr[UH], hqv[UH], hp[UH];
-  int i = 0, jl[ND];
-  bool own = false, hin[UH];
-  auto load_far = [&](int ic) {
+  // the registers one tile loads
+  struct Tile {
+    Q rv, qv, pv, xv, vu[ND], vl[ND], fur[NFA], fuq[NFA], fup[NFA], fdr[NFA], fdq[NFA], fdp[NFA];
+    float hr[UH], hqv[UH], hp[UH];
+    int i, jl[ND];
+    bool own, hin[UH];
+  };
+  auto load_far = [&](Tile& c) {
+    const int ic = c.own ? c.i : 0;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       const int od = o[NN + 1 + f];
       const int ju = ic + od < n ? ic + od : 0;
       const int jd = ic - od >= -3 ? ic - od : 0;
-      fur[f] = ld16(a.r + ju);
-      fuq[f] = ld16(a.q + ju);
-      fup[f] = ld16(a.p_old + ju);
-      fdr[f] = ld16(a.r + jd);
-      fdq[f] = ld16(a.q + jd);
-      fdp[f] = ld16(a.p_old + jd);
+      c.fur[f] = ld16(a.r + ju);
+      c.fuq[f] = ld16(a.q + ju);
+      c.fup[f] = ld16(a.p_old + ju);
+      c.fdr[f] = ld16(a.r + jd);
+      c.fdq[f] = ld16(a.q + jd);
+      c.fdp[f] = ld16(a.p_old + jd);
     }
   };
-  auto load_tile = [&](int t0, int t1) {
-    i = t0 + 4 * tid;
-    own = i < t1;
-    const int ic = own ? i : 0;
-    rv = ld16(a.r + ic);
-    qv = ld16(a.q + ic);
-    pv = ld16(a.p_old + ic);
+  auto load_tile = [&](Tile& c, int t0, int t1) {
+    c.i = t0 + 4 * tid;
+    c.own = c.i < t1;
+    const int ic = c.own ? c.i : 0;
+    c.rv = ld16(a.r + ic);
+    c.qv = ld16(a.q + ic);
+    c.pv = ld16(a.p_old + ic);
 #pragma unroll
     for (int d = 1; d < ND; ++d) {
       const float* vd = vals + (int64_t)d * ld;
-      vu[d] = ld16(vd + ic);
-      jl[d] = ic - o[d];
-      vl[d] = ld16(vd + (jl[d] >= -3 ? jl[d] : 0));
+      c.vu[d] = ld16(vd + ic);
+      c.jl[d] = ic - o[d];
+      c.vl[d] = ld16(vd + (c.jl[d] >= -3 ? c.jl[d] : 0));
     }
-    if (!LF) load_far(ic);
+    if (!LF) load_far(c);
     // halo rows [t0 - hq, t0) and [t1q, t1q + hq), t1q = the tile end in whole quads
     const int t1q = t0 + ((t1 - t0 + 3) & ~3);
 #pragma unroll
     for (int u = 0; u < UH; ++u) {
       const int h = tid + u * NT;
       const int row = h < hq ? t0 - hq + h : t1q + (h - hq);
-      hin[u] = h < 2 * hq && row >= 0 && row < n;
-      const int hc = hin[u] ? row : 0;
-      hr[u] = a.r[hc];
-      hqv[u] = a.q[hc];
-      hp[u] = a.p_old[hc];
+      c.hin[u] = h < 2 * hq && row >= 0 && row < n;
+      const int hc = c.hin[u] ? row : 0;
+      c.hr[u] = a.r[hc];
+      c.hqv[u] = a.q[hc];
+      c.hp[u] = a.p_old[hc];
     }
   };
-  if (nt > 0) load_tile(r0, min(r1, r0 + tl));
+  auto load_late = [&](Tile& c, bool upd) {
+    if (LF) load_far(c);
+    if (upd) c.xv = ld16(a.x + (c.own ? c.i : 0));
+  };
+  Tile c;
+  if (nt > 0) load_tile(c, r0, min(r1, r0 + tl));
   // head: reduce the partials of launch k-1 (fixed order: per thread, per wave, then the waves)
   double in[4];
   {
@@ -972,24 +981,25 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const float al = (float)alpha, be = (float)beta;
   const bool upd = special == 0;
+  if (nt > 0) load_late(c, upd);
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   for (int t0 = r0; t0 < r1; t0 += tl) {
     const int t1 = min(r1, t0 + tl);
     if (t0 > r0) {
       __syncthreads();  // the previous tile's SpMV has read pt
-      load_tile(t0, t1);
+      load_tile(c, t0, t1);
+      load_late(c, upd);
     }
-    if (LF) load_far(own ? i : 0);
-    if (upd) xv = ld16(a.x + (own ? i : 0));
     const int t1q = t0 + ((t1 - t0 + 3) & ~3);
+    const int i = c.i;
     // r_k, p_k of the own quad; p_k of the halo rows and of the far neighbours
     Q rn, pn;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      rn.v[e] = rv.v[e] - al * qv.v[e];
-      pn.v[e] = rn.v[e] + be * pv.v[e];
+      rn.v[e] = c.rv.v[e] - al * c.qv.v[e];
+      pn.v[e] = rn.v[e] + be * c.pv.v[e];
     }
-    if (own) {
+    if (c.own) {
       st16(a.r_out + i, rn);
       st16(a.p_new + i, pn);
       *reinterpret_cast<Q*>(&pt[hq + 4 * tid]) = pn;
@@ -999,36 +1009,44 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
 #pragma unroll
     for (int u = 0; u < UH; ++u) {
       const int h = tid + u * NT;
-      if (h < 2 * hq) pt[h < hq ? h : hq + (t1q - t0) + (h - hq)] = hin[u] ? (hr[u] - al * hqv[u]) + be * hp[u] : 0.0f;
+      if (h < 2 * hq)
+        pt[h < hq ? h : hq + (t1q - t0) + (h - hq)] = c.hin[u] ? (c.hr[u] - al * c.hqv[u]) + be * c.hp[u] : 0.0f;
     }
-    Q fu[NFA], fd[NFA];
+    // before the barrier: the far neighbours' p_k and their terms of q = A p_k (sp), and x; the SpMV
+    // phase then holds only the near diagonals' values and sp
+    Q sp;
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
+    for (int e = 0; e < 4; ++e) {
+      float s = pn.v[e];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        fu[f].v[e] = (fur[f].v[e] - al * fuq[f].v[e]) + be * fup[f].v[e];
-        fd[f].v[e] = (fdr[f].v[e] - al * fdq[f].v[e]) + be * fdp[f].v[e];
+      for (int d = NN + 1; d < ND; ++d) {
+        const int f = d - NN - 1;
+        const float fu = (c.fur[f].v[e] - al * c.fuq[f].v[e]) + be * c.fup[f].v[e];
+        const float fd = (c.fdr[f].v[e] - al * c.fdq[f].v[e]) + be * c.fdp[f].v[e];
+        const float cl = c.jl[d] + e >= 0 ? c.vl[d].v[e] : 0.0f;
+        s += c.vu[d].v[e] * fu;
+        s += cl * fd;
       }
+      sp.v[e] = s;
+    }
+    if (c.own && upd) {
+      Q xn;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xn.v[e] = c.xv.v[e] + al * c.pv.v[e];
+      st16(a.x + i, xn);
+    }
     __syncthreads();
-    if (own) {
+    if (c.own) {
       Q qn;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int li = hq + 4 * tid + e;
-        float s = pn.v[e];
+        float s = sp.v[e];
 #pragma unroll
-        for (int d = 1; d < ND; ++d) {
-          float xu, xl;
-          if (d <= NN) {
-            xu = pt[li + o[d]];
-            xl = pt[li - o[d]];
-          } else {
-            xu = fu[d - NN - 1].v[e];
-            xl = fd[d - NN - 1].v[e];
-          }
-          const float cl = jl[d] + e >= 0 ? vl[d].v[e] : 0.0f;
-          s += vu[d].v[e] * xu;
-          s += cl * xl;
+        for (int d = 1; d <= NN; ++d) {
+          const float cl = c.jl[d] + e >= 0 ? c.vl[d].v[e] : 0.0f;
+          s += c.vu[d].v[e] * pt[li + o[d]];
+          s += cl * pt[li - o[d]];
         }
         s = i + e < n ? s : 0.0f;
         qn.v[e] = s;
@@ -1037,12 +1055,6 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
         part[3] += (double)s * (double)s;
       }
       st16(a.q_out + i, qn);
-      if (upd) {
-        Q xn;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) xn.v[e] = xv.v[e] + al * pv.v[e];
-        st16(a.x + i, xn);
-      }
     }
   }
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
@@ -1806,6 +1818,18 @@ static int64_t scaled_elems(const jno_op* op) {
 // rows per workgroup: a multiple of 4 so every workgroup's first row is 16-byte aligned for FP32
 // and FP64, and of 12 for BSR3 (whole 3-row blocks)
 static bool quad_ok(const jno_op* op);
+
+// compute units of the current device (cached per device; 256 on MI355X)
+static int cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int v = 0;
+    cached[dev] = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0 ? v : 256;
+  }
+  return cached[dev];
+}
 // The launch shape an FP32 solve of `op` runs: the register-resident one-kernel shapes (mode 5 with
 // k1_vector 1 or 2) apply to the 4-diagonal operators it is built for, at 256 or 512 threads; any
 // other operator (BSR3, CSR, 8-diagonal DIA) runs the previous default shape instead.
@@ -1818,7 +1842,12 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
   if (tu.split == MODE_ONE && (tu.vec == 1 || tu.vec == 2)) {
     // pcg_one_q: one tile of 4 block rows per workgroup where the grid cap allows (rounded up to
     // whole XCD slabs), else several equal tiles per workgroup
+    // All workgroups are resident at once (at most 4 of 256 threads per CU); a grid of whole
+    // multiples of the CU count leaves no CU with an extra workgroup to run after the others (776
+    // workgroups at 128^3, 8 more than 3 per CU, measured 24.8 us against 22.8 at 768).
     int64_t want = cdiv(n, (int64_t)tu.block * 4);
+    const int64_t ncu = cu_count();
+    if (want >= ncu && ncu % N_XCD == 0) want = cdiv(want, ncu) * ncu;
     if (want >= N_XCD) want = cdiv(want, (int64_t)N_XCD) * N_XCD;
     const int64_t cap = std::min<int64_t>(tu.grid_cap, QRED_G);
     if (want > cap) want = std::max<int64_t>(N_XCD, (cap / N_XCD) * N_XCD);
@@ -2003,11 +2032,11 @@ static int launch_quad(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipSt
   JNO_REQUIRE(quad_ok(op), JNO_EINVAL, "pcg: no register-resident kernel for this DIA diagonal set");
   JNO_REQUIRE(grid <= QRED_G, JNO_EINVAL, "pcg: %d workgroups exceed the head reduction of %d", grid, QRED_G);
   if constexpr (std::is_same<T, float>::value && std::is_same<V, float>::value && NT <= 512) {
-    const int key = near_count(op) * 2 + (a.vec == 2 ? 1 : 0);
+    const int key = near_count(op) * 4 + a.vec;
     switch (key) {
-      case 4: pcg_one_q<NT, 4, 2><<<grid, NT, 0, s>>>(a); break;
-      case 5: pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a); break;
-      case 6: pcg_one_q<NT, 4, 3><<<grid, NT, 0, s>>>(a); break;
+      case 9: pcg_one_q<NT, 4, 2><<<grid, NT, 0, s>>>(a); break;
+      case 10: pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a); break;
+      case 13: pcg_one_q<NT, 4, 3><<<grid, NT, 0, s>>>(a); break;
       default: pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a); break;
     }
     JNO_LAUNCH_CHECK();
