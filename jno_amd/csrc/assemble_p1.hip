@@ -95,11 +95,9 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
     }
     double bl = 0.0;
     const int ib = inc_ptr[r], ie = inc_ptr[r + 1];
-    for (int q = ib; q < ie; ++q) {
-      const int ent = inc[q];
+    // one incident element's contributions to the row, in incidence order
+    auto accum = [&](int ent, const int4 c, const TetGeom& t) {
       const int e = ent >> 2, a = ent & 3;
-      const int4 c = cells[e];
-      const TetGeom t = tet_geom(pts, c);
       const int nb[4] = {c.x, c.y, c.z, c.w};
       double ga[3];
       int na;
@@ -130,6 +128,23 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
           for (int s = 0; s < MAXR; ++s) acc[s] += (cols[s] == nb[bb]) ? v : 0.0;
         }
       }
+    };
+    // two incident elements per step (rows of up to 16 entries; longer ones have no registers to
+    // spare): both elements' index, vertex and coordinate loads are in flight together (the gather
+    // chain incidence -> cell -> points is latency-bound); the sums keep the incidence order
+    int q = ib;
+    for (; MAXR <= 16 && q + 1 < ie; q += 2) {
+      const int ent0 = inc[q], ent1 = inc[q + 1];
+      const int4 c0 = cells[ent0 >> 2], c1 = cells[ent1 >> 2];
+      const TetGeom t0 = tet_geom(pts, c0);
+      const TetGeom t1 = tet_geom(pts, c1);
+      accum(ent0, c0, t0);
+      accum(ent1, c1, t1);
+    }
+    for (; q < ie; ++q) {
+      const int ent = inc[q];
+      const int4 c = cells[ent >> 2];
+      accum(ent, c, tet_geom(pts, c));
     }
     // symmetric Dirichlet elimination (_apply_dirichlet_symmetric, fem_1d.py:667-686)
     if (dmask) {
