@@ -32,6 +32,7 @@
 namespace jno {
 
 constexpr int PCG_NT = 256;
+
 constexpr int PCG_MAX_BLOCKS = 4096;  // workspace capacity for per-workgroup partials
 
 // Launch shape of the iteration kernels, one per iteration precision (set by the autotuner through
@@ -1431,17 +1432,41 @@ __global__ void __launch_bounds__(NT) pcg_ks_bsr3(PcgArgs<T, V> a) {
   const T* p = a.p_new;
   const int sub = threadIdx.x % L;
   double part = 0.0;
-  // U block rows per lane group in flight (their loads issued together), NT / L groups apart: two
-  // for FP32 (C5: 7.6% faster than one), one for FP64 (its registers would cut the occupancy)
+  // U block rows per lane group in flight (their loads issued together), NT / L groups apart, and CH
+  // blocks per lane of each: for FP32 two rows of one block per lane (C5: 7.6% faster than one row;
+  // four rows, or two blocks per lane, cut the occupancy and measured slower), for FP64 one row of two
+  // blocks per lane (C5 FP64 solve 594 -> 547 ms against one block: the rows of 42 and 65 blocks
+  // leave the serial loop).  The block-row pointers of the next U rows are loaded one batch ahead,
+  // so a batch waits on two dependent round trips (block columns and values, then the p gathers)
+  // instead of three (C5 mixed solve 379 -> 360 ms).
   constexpr int U = sizeof(T) == 4 ? 2 : 1;
+  constexpr int CH = sizeof(T) == 4 ? 1 : 2;
   constexpr int64_t STEP = NT / L;
   const int64_t R1 = r1 / 3;
-  for (int64_t R0 = r0 / 3 + threadIdx.x / L; R0 < R1; R0 += U * STEP) {
-    int64_t Rs[U];
+  int64_t R0 = r0 / 3 + threadIdx.x / L;
+  int pa[U], pb[U];
+  auto row_ptrs = [&](int64_t Rb) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) Rs[u] = R0 + u * STEP < R1 ? R0 + u * STEP : -1;
+    for (int u = 0; u < U; ++u) {
+      const int64_t R = Rb + u * STEP;
+      const int64_t Rc = R < R1 ? R : 0;
+      pa[u] = a.rowptr[Rc];
+      pb[u] = a.rowptr[Rc + 1];
+    }
+  };
+  row_ptrs(R0);
+  for (; R0 < R1; R0 += U * STEP) {
+    int64_t Rs[U];
+    int bs[U], len[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      Rs[u] = R0 + u * STEP < R1 ? R0 + u * STEP : -1;
+      bs[u] = pa[u];
+      len[u] = Rs[u] < 0 ? 0 : pb[u] - pa[u];
+    }
+    row_ptrs(R0 + U * STEP);
     T sr[U][3];
-    bsr3_rows<L, U, V, T>(a.rowptr, a.col, a.vals, p, Rs, sub, sr);
+    bsr3_rows_at<L, U, V, T, CH>(a.col, a.vals, p, bs, len, sub, sr);
     if (sub == 0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {

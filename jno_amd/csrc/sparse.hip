@@ -31,15 +31,17 @@ __global__ void __launch_bounds__(256) bsr3_spmv_k(int64_t nb, const int32_t* __
                                                    const X* __restrict__ x, X* __restrict__ y) {
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int sub = threadIdx.x % L;
-  // U block rows per lane group in flight, as in the PCG's BSR3 kernel (two for FP32, one for FP64)
+  // U block rows per lane group in flight and CH blocks per lane, as in the PCG's BSR3 kernel (two
+  // rows of one block for FP32, one row of two blocks for FP64)
   constexpr int U = sizeof(X) == 4 ? 2 : 1;
+  constexpr int CH = sizeof(X) == 4 ? 1 : 2;
   const int64_t step = (int64_t)gridDim.x * blockDim.x / L;
   for (int64_t R0 = gid / L; R0 < nb; R0 += U * step) {
     int64_t Rs[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) Rs[u] = R0 + u * step < nb ? R0 + u * step : -1;
     X sr[U][3];
-    bsr3_rows<L, U, V, X>(browptr, bcol, bvals, x, Rs, sub, sr);
+    bsr3_rows<L, U, V, X, CH>(browptr, bcol, bvals, x, Rs, sub, sr);
     if (sub == 0) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
