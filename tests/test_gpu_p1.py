@@ -405,6 +405,50 @@ def test_iteration_modes_and_launch_shapes(dev, precision):
         solve.set_tuning(precision, *saved)
 
 
+@pytest.mark.parametrize("dims,hi,ndiag", [((5, 9, 40), (5 / 8, 9 / 8, 5.0), 4),
+                                           ((30, 7, 3), (30 / 16, 7 / 16, 3 / 16), 4),
+                                           ((9, 20, 300), (9 / 32, 20 / 32, 300 / 32), 4),
+                                           ((5, 9, 40), (1.0, 1.0, 1.0), 4)])
+def test_one_kernel_non_cubic_boxes(dev, dims, hi, ndiag):
+    """The one-kernel FP32 iterations on non-cubic Kuhn boxes: the operator has 4 DIA offsets (1,
+    nz+1, (ny+1)(nz+1)), not those of a cube. The cases:
+    * 2 near diagonals and a far one (5 x 9 x 40: offsets 1, 41, 410);
+    * all 3 near (30 x 7 x 3: 1, 4, 32);
+    * the first far offset past the LDS halo (9 x 20 x 300: 1, 301, 6321). The register-resident
+      kernel does not take it, so its launch shape falls back to the LDS-staged kernel;
+    * unequal spacings (5 x 9 x 40 on the unit cube: 1, 41, 410). The Kuhn subdivision of a box
+      gives the 7-point stencil for any spacings, and the contraction-free assembly keeps the
+      other couplings exactly 0 (the oracle's differ from 0 by round-off, |a| < 2e-16 against
+      couplings of 0.03 to 2.6).
+    Each variant (mode 5 with k1_vector 0, 1, 2) and the default shape solve to the oracle's direct
+    solution."""
+    import jno_amd as J
+    from jno_amd import precond, solve
+
+    nx, ny, nz = dims
+    p, t = O.kuhn_box(nx, ny, nz, hi=hi)
+    k = O.k_lognormal(t.shape[0], 11)
+    m = J.kuhn_box(nx, ny, nz, hi=hi)
+    assert np.array_equal(_np(m.cells), t)
+    assert np.array_equal(_np(m.points), p)
+    A, b = J.fem_poisson(m).evaluate({"k": torch.as_tensor(k, device="cuda")})
+    Ao, bo, _ = O.poisson_p1_system(p, t, k)
+    uo = O.direct_solve(Ao, bo)
+    saved = solve.get_tuning("mixed")
+    try:
+        for shape in (None, (512, 512, 0, 5), (256, 768, 1, 5), (256, 768, 2, 5), (512, 64, 2, 5)):
+            if shape is not None:
+                solve.set_tuning("mixed", *shape)
+            sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=0.01)
+            u = sv(A, b, M=precond.jacobi())
+            st = sv.last_stats
+            assert st["operator"]["format"] == "dia" and st["operator"]["ndiag"] == ndiag, st
+            assert st["relres"] <= 1e-10, (dims, shape, st)
+            assert _rel(_np(u), uo) <= SOL_TOL, (dims, shape, st)
+    finally:
+        solve.set_tuning("mixed", *saved)
+
+
 @pytest.mark.parametrize("n", [17, 40])
 def test_lds_staged_k1(dev, n):
     """The LDS-staged kernels on the Jacobi-scaled operator (K1 of mode 4 with k1_vector 3, and the
