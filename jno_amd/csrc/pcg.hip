@@ -877,17 +877,44 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   const int G = gridDim.x;
   const int tid = (int)threadIdx.x;
   const int64_t k = a.kk;
-  const int64_t r0l = block_first_row(a.rows_per_block);
-  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + a.rows_per_block);
+  const int n = (int)a.n;
   const float* vals = a.vals;
   const int ld = (int)dia_ld(a.n);
   int o[ND];
 #pragma unroll
   for (int d = 1; d < ND; ++d) o[d] = a.off.o[d];
   const int hq = (o[NN] + 3) & ~3;  // near halo rows on either side, whole quads (<= LDS_HALO)
-  // equal tiles of at most TT rows, whole quads (r0 is a multiple of 4)
-  const int nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
-  const int tl = nt > 0 ? (((r1 - r0 + nt - 1) / nt + 3) & ~3) : TT;
+  // Tiles: nt per workgroup, of tl rows (whole quads, at most TT).  When the grid is whole XCD slabs
+  // and rows_per_block = nt tl (as the launch sizes it), the slab's tiles are dealt round-robin to
+  // the XCD's workgroups: tile t of workgroup l of the XCD starts at slab + (l + t G/8) tl, so the
+  // tiles in flight at one time form one contiguous band of the slab, which holds the far rows
+  // (offset (n+1)^2) of most of them.  Otherwise each workgroup takes its own contiguous range.
+  const int64_t rpb = a.rows_per_block;
+  const int ntq = (int)((rpb + TT - 1) / TT);
+  const bool ilv = G % N_XCD == 0 && rpb % ntq == 0 && (rpb / ntq) % 4 == 0;
+  int nt, tl, lim;
+  int64_t tbase, tstride;
+  if (ilv) {
+    nt = ntq;
+    tl = (int)(rpb / ntq);
+    const int per = G / N_XCD;
+    tbase = (int64_t)(blockIdx.x % N_XCD) * per * rpb + (int64_t)(blockIdx.x / N_XCD) * tl;
+    tstride = (int64_t)per * tl;
+    lim = n;
+  } else {
+    const int64_t r0l = block_first_row(rpb);
+    const int r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + rpb);
+    nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
+    tl = nt > 0 ? (((r1 - r0 + nt - 1) / nt + 3) & ~3) : TT;
+    tbase = r0;
+    tstride = tl;
+    lim = r1;
+  }
+  auto tile_at = [&](int t, int& t0, int& t1) {
+    const int64_t b = tbase + (int64_t)t * tstride;
+    t0 = (int)min(b, (int64_t)lim);
+    t1 = (int)min(b + tl, (int64_t)lim);
+  };
   // the head's partial-sum loads go out first: the 4 arrays of G partials as one flat range, entry
   // f = tid + u NT in thread tid
   const double* pin = a.part.C + (k & 1) * 4 * (int64_t)G;
@@ -950,7 +977,9 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
     if (upd) c.xv = ld16(a.x + (c.own ? c.i : 0));
   };
   Tile c;
-  if (nt > 0) load_tile(c, r0, min(r1, r0 + tl));
+  int ft0 = 0, ft1 = 0;
+  if (nt > 0) tile_at(0, ft0, ft1);
+  if (ft0 < ft1) load_tile(c, ft0, ft1);
   // head: reduce the partials of launch k-1 (fixed order: per thread, per wave, then the waves)
   double in[4];
   {
@@ -982,11 +1011,13 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const float al = (float)alpha, be = (float)beta;
   const bool upd = special == 0;
-  if (nt > 0) load_late(c, upd);
+  if (ft0 < ft1) load_late(c, upd);
   double part[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int t0 = r0; t0 < r1; t0 += tl) {
-    const int t1 = min(r1, t0 + tl);
-    if (t0 > r0) {
+  for (int t = 0; t < nt; ++t) {
+    int t0, t1;
+    tile_at(t, t0, t1);
+    if (t0 >= t1) break;  // tiles past the last row (the same for every thread)
+    if (t > 0) {
       __syncthreads();  // the previous tile's SpMV has read pt
       load_tile(c, t0, t1);
       load_late(c, upd);
@@ -1878,7 +1909,11 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
     if (want > cap) want = std::max<int64_t>(N_XCD, (cap / N_XCD) * N_XCD);
     if (want < 1) want = 1;
     *G = (int)want;
-    *rpb = cdiv(cdiv(n, want), (int64_t)4) * 4;
+    // rows per workgroup = nt tiles of tl rows exactly (tl a multiple of 4), so that the kernel can
+    // deal the tiles of an XCD slab round-robin
+    const int64_t r = cdiv(cdiv(n, want), (int64_t)4) * 4;
+    const int64_t nt = cdiv(r, (int64_t)tu.block * 4);
+    *rpb = nt * (cdiv(cdiv(r, nt), (int64_t)4) * 4);
     return;
   }
   const int64_t lane_rows = op->format == JNO_FMT_DIA    ? n
