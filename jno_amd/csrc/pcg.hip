@@ -679,8 +679,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   if (st->done) return;
   const int G = gridDim.x;
   const int64_t k = a.kk;
-  const int64_t r0l = block_first_row(a.rows_per_block);
-  const int n = (int)a.n, r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + a.rows_per_block);
+  const int n = (int)a.n;
   const V* vals = a.vals;
   const int ld = (int)dia_ld(a.n);
   int o[NDA];
@@ -699,11 +698,41 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
       xv[u] = ldc(a.x, j, in);
     }
   };
-  // tiles of equal length tl <= TT
-  const int nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
-  const int tl = nt > 0 ? (r1 - r0 + nt - 1) / nt : TT;
+  // Tiles of equal length tl <= TT, nt per workgroup.  When the grid is whole XCD slabs and
+  // rows_per_block = nt tl, the slab's tiles are dealt round-robin to the XCD's workgroups, as in
+  // pcg_one_q (the tiles in flight form one band that holds their far rows); otherwise each
+  // workgroup takes its own contiguous range.
+  const int64_t rpb = a.rows_per_block;
+  const int ntq = (int)((rpb + TT - 1) / TT);
+  int nt, tl, lim;
+  int64_t tbase, tstride;
+  if (G % N_XCD == 0 && rpb % ntq == 0) {
+    nt = ntq;
+    tl = (int)(rpb / ntq);
+    const int per = G / N_XCD;
+    tbase = (int64_t)(blockIdx.x % N_XCD) * per * rpb + (int64_t)(blockIdx.x / N_XCD) * tl;
+    tstride = (int64_t)per * tl;
+    lim = n;
+  } else {
+    const int64_t r0l = block_first_row(rpb);
+    const int r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + rpb);
+    nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
+    tl = nt > 0 ? (r1 - r0 + nt - 1) / nt : TT;
+    tbase = r0;
+    tstride = tl;
+    lim = r1;
+  }
+  auto tile_at = [&](int t, int& t0, int& t1) {
+    const int64_t b = tbase + (int64_t)t * tstride;
+    t0 = (int)min(b, (int64_t)lim);
+    t1 = (int)min(b + tl, (int64_t)lim);
+  };
   // the first staged batch of the first tile, loaded before the reduction
-  load1(r0 - h + (int)threadIdx.x, min(r1, r0 + tl) + h);
+  {
+    int f0, f1;
+    tile_at(0, f0, f1);
+    load1(f0 - h + (int)threadIdx.x, f1 + h);
+  }
   double in[4];
   reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
   double alpha, beta;
@@ -713,9 +742,12 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   const T al = (T)alpha, be = (T)beta;
   const bool upd = special == 0;
   double part[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int t0 = r0; t0 < r1; t0 += tl) {
-    const int t1 = min(r1, t0 + tl), s0 = t0 - h, s1 = t1 + h;
-    if (t0 > r0) load1(s0 + (int)threadIdx.x, s1);
+  for (int t = 0; t < nt; ++t) {
+    int t0, t1;
+    tile_at(t, t0, t1);
+    if (t0 >= t1) break;  // tiles past the last row (the same for every thread)
+    const int s0 = t0 - h, s1 = t1 + h;
+    if (t > 0) load1(s0 + (int)threadIdx.x, s1);
     for (int j0 = s0 + (int)threadIdx.x; j0 < s1; j0 += U1 * NT) {
       T rn[U1], pn[U1], xn[U1];
 #pragma unroll
@@ -828,7 +860,9 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
         }
       }
     }
-    if (t1 < r1) __syncthreads();  // the next tile rewrites pt and rt
+    int u0, u1;
+    tile_at(t + 1, u0, u1);
+    if (t + 1 < nt && u0 < u1) __syncthreads();  // the next tile rewrites pt and rt
   }
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
 }
@@ -1924,7 +1958,9 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
   if (want > tu.grid_cap) want = tu.grid_cap;
   if (want >= N_XCD) want = (want / N_XCD) * N_XCD;
   *G = (int)want;
-  const int64_t mult = op->format == JNO_FMT_BSR3 ? 12 : 4;
+  // a multiple of 48 for the one-kernel pcg_one, so that 1 to 4 equal tiles divide it (its tiles
+  // are then dealt round-robin within the XCD slabs)
+  const int64_t mult = op->format == JNO_FMT_BSR3 ? 12 : tu.split == MODE_ONE ? 48 : 4;
   *rpb = cdiv(cdiv(n, want), mult) * mult;
 }
 
