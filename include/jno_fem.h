@@ -237,7 +237,12 @@ int jno_pcg_set_pacing(int enable);
  * materialized, then a plain SpMV), 0 for two (p recomputed inside the SpMV kernel), 2 for the
  * single-reduction iteration, 3, 4 and 5 for the split, the two-kernel and the one-kernel
  * iteration on the Jacobi-scaled FP32 operator (FP32 inner iterations on DIA operators with room
- * for it in the workspace; elsewhere they are modes 1, 0 and 0).
+ * for it in the workspace; elsewhere they are modes 1, 0 and 0).  With split = 5, k1_vector 0
+ * selects the LDS-staged one-kernel iteration and 1 or 2 the register-resident one (4-row quads;
+ * the far neighbours loaded before / after the head reduction), which takes 4-diagonal operators
+ * with 2 or 3 diagonals within 256 rows at 256 or 512 threads; any other operator given that shape
+ * runs block 512, grid cap 512, k1_vector 0.  The FP32 default is split 5, k1_vector 2, block 256,
+ * grid cap 768; the FP64 default split 1, k1_vector 0, block 1024, grid cap 512.
  * jno_pcg_get_tuning reads the current shape. */
 int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split);
 int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split);

@@ -868,18 +868,19 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// The one-kernel iteration with register-resident rows (mode 5, k1_vector 1), for 4-diagonal
+// The one-kernel iteration with register-resident rows (mode 5, k1_vector 1 and 2), for 4-diagonal
 // operators.  Same arithmetic and launch contract as pcg_one; the difference is how the memory
 // traffic is issued.  A thread owns a quad of 4 consecutive rows (16-byte loads and stores) and a
 // workgroup's tile is one quad per thread, TT = 4 NT rows.  Every global load of a tile is issued at
 // once: the partial sums of the previous launch first, then the quad's r, q, p, its upper and lower
 // off-diagonal values, r, q, p of its far neighbours (offsets above LDS_HALO, recomputed in
 // registers), and for the first 2 hq threads one row of r, q, p of the near halo (x follows the
-// head: it is needed only at the end).  The reduction
-// of the partials waits only for the loads issued before the tile's, so it overlaps the tile's round
-// trip; one round trip then covers the whole tile.  Only p_k of the tile and its halo goes through
+// head: it is needed only at the end).  The reduction of the partials waits only for the loads
+// issued before the tile's, so it overlaps the tile's round trip; one round trip then covers the
+// whole tile.  Only p_k of the tile and its halo goes through
 // LDS (the near neighbours of the SpMV); r_k and the far p_k stay in registers.
-// Bounds: the iteration vectors carry PCG_PAD zero rows on both sides (ws_layout, zero_pads_k).  A
+// Bounds: the iteration vectors carry PCG_PAD_BYTES of zero pad on both sides (ws_layout,
+// zero_pads_k: 32 FP32 rows).  A
 // quad that straddles row 0 or row n reads pad rows, which are 0 in every vector and enter only
 // through a zero coefficient (the stored upper diagonal is 0 past the last row, the lower one is
 // selected to 0 before the first); a quad wholly outside [0, n) loads rows 0..3 instead, finite
@@ -889,7 +890,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
 constexpr int PCG_PAD_BYTES = 128;  // zero pad before and after each iteration vector
 constexpr int QRED_G = 1152;        // workgroups (partial sums per array) the head reduction takes
 
-// LF (k1_vector 2 and 3): the far neighbours' loads (and x) are issued after the head instead, and
+// LF (k1_vector 2): the far neighbours' loads (and x) are issued after the head instead, and
 // combined into p_k right before the barrier: fewer registers live at the head, for a second, partly
 // hidden round trip.  (Issuing the next tile's loads during the current tile's SpMV phase measured
 // slower: 211 VGPRs.)
