@@ -888,7 +888,10 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
 // straddling quad are stored back as the 0 they compute to.
 // ---------------------------------------------------------------------------------------------
 constexpr int PCG_PAD_BYTES = 128;  // zero pad before and after each iteration vector
-constexpr int QRED_G = 1152;        // workgroups (partial sums per array) the head reduction takes
+// workgroups (partial sums per array) the head reduction takes: every thread loads 4 QRED_G / NT of
+// them, so the bound is kept at the default grid (at 1152, the loads past 768 cost 0.7 us per
+// launch at 128^3)
+constexpr int QRED_G = 768;
 
 // LF (k1_vector 2): the far neighbours' loads (and x) are issued after the head instead, and
 // combined into p_k right before the barrier: fewer registers live at the head, for a second, partly
