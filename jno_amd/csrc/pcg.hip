@@ -911,7 +911,6 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   __shared__ double sm_out[4 * NW];
   __shared__ __align__(16) float pt[TT + 2 * LDS_HALO];  // p_k of the tile and its near halo
   PcgState* st = a.st;
-  if (st->done) return;
   const int G = gridDim.x;
   const int tid = (int)threadIdx.x;
   const int64_t k = a.kk;
@@ -1018,6 +1017,10 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   int ft0 = 0, ft1 = 0;
   if (nt > 0) tile_at(0, ft0, ft1);
   if (ft0 < ft1) load_tile(c, ft0, ft1);
+  // the stop flag is read after the first loads are issued, so its latency overlaps them (a launch
+  // past a stop exits with its loads unused: measured 0.35 us less per working launch, 1% less per
+  // 128^3 solve, against the extra traffic of the exiting ones)
+  if (st->done) return;
   // head: reduce the partials of launch k-1 (fixed order: per thread, per wave, then the waves)
   double in[4];
   {
