@@ -676,7 +676,6 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   __shared__ T rt[TT];                        // r_k of the tile
   __shared__ T pf[NPF][LPF];                  // p_k of the far neighbours (i + o_d, i - o_d)
   PcgState* st = a.st;
-  if (st->done) return;
   const int G = gridDim.x;
   const int64_t k = a.kk;
   const int n = (int)a.n;
@@ -733,6 +732,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
     tile_at(0, f0, f1);
     load1(f0 - h + (int)threadIdx.x, f1 + h);
   }
+  if (st->done) return;  // read after the first loads are issued, as in pcg_one_q
   double in[4];
   reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
   double alpha, beta;
