@@ -151,6 +151,24 @@ int jno_drop_zeros_fill(int64_t n_rows, const int32_t* d_rowptr, const int32_t* 
 enum { JNO_FMT_CSR = 0, JNO_FMT_DIA = 1, JNO_FMT_BSR3 = 2 };
 #define JNO_DIA_MAX 16
 
+/* Launch tuning of the PCG iteration kernels, carried per call (no reference counterpart: these
+ * are launch-granularity knobs of this engine, chosen by the autotuner).  A jno_op whose `tuning`
+ * is NULL runs the library defaults (jno_pcg_set_tuning / _set_chunk / _set_pacing set those).
+ * iter[0]: the FP64 iterations of jno_pcg_f64; iter[1]: the FP32 inner iterations of
+ * jno_pcg_mixed.  The fields have the meaning of jno_pcg_set_tuning's arguments. */
+typedef struct jno_launch_shape {
+  int32_t grid_cap;
+  int32_t k1_vector;
+  int32_t block;
+  int32_t split;
+} jno_launch_shape;
+
+typedef struct jno_tuning {
+  jno_launch_shape iter[2];
+  int32_t chunk;   /* iterations enqueued per host poll, 1..1024 */
+  int32_t pacing;  /* 1: paced chunks for the one-kernel FP32 inner loops, 0: fixed chunks */
+} jno_tuning;
+
 typedef struct jno_op {
   int32_t format;      /* JNO_FMT_*                                                     */
   int32_t lanes;       /* CSR: lanes per row                                            */
@@ -162,6 +180,7 @@ typedef struct jno_op {
   const float* vals32;    /* CSR FP32 values (or DIA FP32 values [ndiag][n]); nullable   */
   int32_t ndiag;       /* DIA: stored diagonals (main first, then ascending upper)       */
   int32_t offsets[JNO_DIA_MAX]; /* DIA: stored offsets (0 first)                        */
+  const jno_tuning* tuning;     /* nullable: launch tuning of the solves on this operator */
 } jno_op;
 
 /* Distinct column-row offsets (up to JNO_DIA_MAX upper ones) and symmetry of a CSR.
@@ -220,16 +239,17 @@ size_t jno_pcg_workspace_size(int64_t n, int mixed);
  * Jacobi-scaled FP32 copy of the operator that the scaled inner iteration (tuning mode 3) streams;
  * a workspace of jno_pcg_workspace_size(n, 1) bytes runs the unscaled inner iteration instead. */
 size_t jno_pcg_workspace_size_op(const jno_op* h_op, int mixed);
-/* Iterations enqueued per host poll of the device-side convergence state (default 16).  No
- * reference counterpart: a launch-granularity knob of this engine, set by the autotuner. */
+/* Default iterations enqueued per host poll of the device-side convergence state (16), for
+ * operators without their own jno_tuning.  No reference counterpart: a launch-granularity knob of
+ * this engine. */
 int jno_pcg_set_chunk(int chunk);
 /* Paced chunks for the one-kernel FP32 inner loops (default on): the first chunks of each inner
  * loop are small and later ones follow the observed convergence rate, so fewer launches are
- * enqueued past a stop.  0 = fixed chunks of jno_pcg_set_chunk.  Process-wide, no reference
- * counterpart (a launch-scheduling knob). */
+ * enqueued past a stop.  0 = fixed chunks of jno_pcg_set_chunk.  A library default for operators
+ * without their own jno_tuning; no reference counterpart (a launch-scheduling knob). */
 int jno_pcg_set_pacing(int enable);
-/* Launch shape of the iteration kernels, per iteration precision (no reference counterpart; set
- * by the autotuner; process-wide): precision = 0 for the FP64 iterations of jno_pcg_f64, 1 for
+/* Default launch shape of the iteration kernels, per iteration precision, for operators without
+ * their own jno_tuning (no reference counterpart): precision = 0 for the FP64 iterations of jno_pcg_f64, 1 for
  * the FP32 inner iterations of jno_pcg_mixed; grid_cap = maximum workgroups (multiple of 8,
  * <= 4096); k1_vector = 1 for 16-byte row tiles in the DIA SpMV kernel, 0 for rows per thread,
  * 2 for four rows per thread, 3 for p staged in LDS (mode 4 only; elsewhere it is 0);
@@ -246,8 +266,13 @@ int jno_pcg_set_pacing(int enable);
  * jno_pcg_get_tuning reads the current shape. */
 int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split);
 int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split);
-/* Live kernel timing of the PCG iteration kernels (no reference counterpart: measurement hook).
- * When enabled, HIP events are recorded on the solve's stream around the kernels of every 8th
+/* The library defaults as one jno_tuning (what a jno_op with tuning = NULL runs), and a check of a
+ * tuning struct (JNO_EINVAL with the reason when a field is out of range).  The defaults are
+ * copied under a lock at the start of each solve, so concurrent solves never see a torn shape. */
+int jno_pcg_default_tuning(jno_tuning* h_out);
+int jno_pcg_check_tuning(const jno_tuning* h_tuning);
+/* Live kernel timing of the PCG iteration kernels (no reference counterpart: measurement hook),
+ * per calling thread: it times the solves that thread runs.  When enabled, HIP events are recorded on the solve's stream around the kernels of every 8th
  * iteration, and the durations of the sampled iterations that did work are summed into
  * h_ms[0] (K0: p update, split iterations only), h_ms[1] (the SpMV kernel: K1 = p recompute +
  * SpMV + x update + p.q, or KS = SpMV + p.q) and h_ms[2] (K2: r update, r.z, r.r); *h_count is

@@ -27,6 +27,14 @@ JNO_FMT_BSR3 = 2
 JNO_DIA_MAX = 16
 
 
+class JnoLaunchShape(C.Structure):
+    _fields_ = [("grid_cap", C.c_int32), ("k1_vector", C.c_int32), ("block", C.c_int32), ("split", C.c_int32)]
+
+
+class JnoTuning(C.Structure):
+    _fields_ = [("iter", JnoLaunchShape * 2), ("chunk", C.c_int32), ("pacing", C.c_int32)]
+
+
 class JnoOp(C.Structure):
     _fields_ = [
         ("format", C.c_int32),
@@ -39,6 +47,7 @@ class JnoOp(C.Structure):
         ("vals32", C.c_void_p),
         ("ndiag", C.c_int32),
         ("offsets", C.c_int32 * JNO_DIA_MAX),
+        ("tuning", C.c_void_p),  # const jno_tuning* (NULL: the library defaults)
     ]
 
 
@@ -104,6 +113,8 @@ SIGNATURES = {
     "jno_pcg_set_pacing": (C.c_int, [C.c_int]),
     "jno_pcg_set_tuning": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "jno_pcg_get_tuning": (C.c_int, [C.c_int, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    "jno_pcg_default_tuning": (C.c_int, [C.POINTER(JnoTuning)]),
+    "jno_pcg_check_tuning": (C.c_int, [C.POINTER(JnoTuning)]),
     "jno_pcg_profile": (C.c_int, [C.c_int]),
     "jno_pcg_profile_read": (C.c_int, [C.POINTER(D), C.POINTER(I64)]),
     "jno_pcg_f64": (C.c_int, [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, C.POINTER(JnoPcgStats), P, SZ, P]),

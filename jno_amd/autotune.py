@@ -79,26 +79,26 @@ _LAUNCH_CACHE: dict = {}
 
 
 def tune_launch(A, b, precision: str = "mixed", iters: int = 100, shapes=CANDIDATE_SHAPES):
-    """Time ``iters`` bounded PCG iterations on ``(A, b)`` for each launch shape and install the
-    fastest one for ``precision`` (process-wide, through ``jno_pcg_set_tuning``).  Returns
-    ``(block, grid_cap, k1_vector)``; cached per pattern signature and precision."""
+    """Time ``iters`` bounded PCG iterations on ``(A, b)`` for each launch shape and attach the
+    fastest to A's solver operator for ``precision`` (``SolverOperator.tuning``, a per-call
+    :class:`jno_amd.solve.Tuning`: later solves on that operator run it; nothing process-wide
+    changes).  Returns ``(block, grid_cap, k1_vector, split)``; cached per pattern signature and
+    precision, so a re-assembled operator on the same mesh gets the shape without new timings."""
     import time
 
     from . import precond
-    from .solve import LinearSolver, get_tuning, set_tuning
+    from .solve import LinearSolver, Tuning
 
     fmt, lanes = choose_format(A)
     key = (A.shape[0], A.nnz, fmt, lanes, precision)
-    if key in _LAUNCH_CACHE:
-        set_tuning(precision, *_LAUNCH_CACHE[key])
-        return _LAUNCH_CACHE[key]
-    saved = get_tuning(precision)
-    sv = LinearSolver("cg", 1e-30, 0.0, iters, precision=precision, replace_factor=1e-30, guard=False)
-    M = precond.jacobi()
-    best = None
-    try:
+    op = LinearSolver("cg", 1e-10, 0.0, None, precision=precision).operator(A)
+    base = op.tuning if getattr(op, "tuning", None) is not None else Tuning.defaults()
+    if key not in _LAUNCH_CACHE:
+        M = precond.jacobi()
+        best = None
         for shape in shapes:
-            set_tuning(precision, *shape)
+            sv = LinearSolver("cg", 1e-30, 0.0, iters, precision=precision, replace_factor=1e-30, guard=False,
+                              tuning=base.with_shape(precision, *shape))
             sv(A, b, M=M)  # warm
             torch.cuda.synchronize(A.device)
             t0 = time.perf_counter()
@@ -106,12 +106,10 @@ def tune_launch(A, b, precision: str = "mixed", iters: int = 100, shapes=CANDIDA
             torch.cuda.synchronize(A.device)
             t = time.perf_counter() - t0
             if best is None or t < best[0]:
-                best = (t, shape)
-    finally:
-        set_tuning(precision, *saved)
-    _LAUNCH_CACHE[key] = best[1]
-    set_tuning(precision, *best[1])
-    return best[1]
+                best = (t, tuple(shape))
+        _LAUNCH_CACHE[key] = best[1]
+    op.tuning = base.with_shape(precision, *_LAUNCH_CACHE[key])
+    return _LAUNCH_CACHE[key]
 
 
 def tune_batch(systems, chunks=(4, 8, 16), **solver_kw) -> int:

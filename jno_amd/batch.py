@@ -27,17 +27,27 @@ class BatchSolver:
         self.steps_per_graph = int(steps_per_graph)
         self.format = format
         self.last_stats: List[dict] = []
+        self.last_status = 0
 
     def _op(self, A: CSR) -> SolverOperator:
-        key = ("solver_op", self.format, True)
+        # the batch kernels stream DIA and CSR items only: the batch keeps its own cache entry, and
+        # takes a single solver's cached layout (e.g. the DIA one of the fused assembly) only when it
+        # is one of those, so neither solver's layout depends on which ran first
+        key = ("batch_op", self.format)
         op = A._cache.get(key)
         if op is None:
-            fmt, lanes = self.format, None
-            if fmt == "auto":  # the batch kernels stream DIA and CSR items
-                from .autotune import choose_format
+            shared = A._cache.get(("solver_op", self.format, True))
+            if shared is not None and shared.format in ("dia", "csr"):
+                op = shared
+            else:
+                fmt, lanes = self.format, None
+                if fmt == "auto":
+                    from .autotune import choose_format
 
-                fmt, lanes = choose_format(A, allow=("dia", "csr"))
-            op = SolverOperator(A, fmt=fmt, lanes=lanes, fp32=True)
+                    fmt, lanes = choose_format(A, allow=("dia", "csr"))
+                elif fmt not in ("dia", "csr"):
+                    raise ValueError("the batched solver streams 'dia' or 'csr' operators")
+                op = SolverOperator(A, fmt=fmt, lanes=lanes, fp32=True)
             A._cache[key] = op
         return op
 
@@ -68,6 +78,7 @@ class BatchSolver:
         stats = (_lib.JnoPcgStats * len(systems))()
         rc = lib.jno_batch_pcg_mixed(len(systems), items, self.tol, self.atol, self.maxiter, self.replace_factor,
                                      self.steps_per_graph, stats, _lib.ptr(ws), wsb, _lib.stream_ptr(dev))
+        self.last_status = int(rc)
         self.last_stats = [
             {"iterations": int(s.iterations), "refinements": int(s.refinements), "relres": float(s.relres),
              "seconds": float(s.seconds)} for s in stats

@@ -201,14 +201,10 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
       rr = item_sum(a.PB + 2 * NE, it, sm);
       const double bb = item_sum(a.PB + 3 * NE, it, sm);
       const double atol2 = fmax(a.tol2 * bb, a.atol2_abs);
+      // a non-finite true residual (e.g. from a NaN in b) stops the item as non-finite: !(rt > atol2)
+      // alone would take a NaN for convergence
       const bool bad = !isfinite(rt) || S->status == BS_NONFINITE;
-      if (!(rt > atol2)) {
-        phase = PH_DONE;
-      } else if (bad || k >= a.maxiter) {
-        phase = PH_DONE;
-      } else {
-        phase = PH_ITER;
-      }
+      phase = (bad || !(rt > atol2) || k >= a.maxiter) ? PH_DONE : PH_ITER;
       if (first && threadIdx.x == 0) {
         // scaled items: the inner loop stops only to replace, the last cycle aimed at the
         // remaining reduction (with margin) when that is smaller than a whole cycle
@@ -220,8 +216,7 @@ __global__ void __launch_bounds__(BT_NT) batch_ka(BArgs a) {
         S->rr_last = rr;
         S->refinements = S->refinements + 1;
         if (phase == PH_DONE) {
-          S->status = !(rt > atol2) ? (S->status == BS_NONFINITE ? BS_NONFINITE : BS_CONVERGED)
-                                    : (bad ? BS_NONFINITE : BS_MAXITER);
+          S->status = bad ? BS_NONFINITE : !(rt > atol2) ? BS_CONVERGED : BS_MAXITER;
           atomicSub(a.active, 1);
         }
       }
@@ -591,6 +586,38 @@ static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot, 
   return ne;
 }
 
+// Pinned slots and events of the host's polls of the active-item count: created once per thread
+// and reused by every batched solve of that thread.
+struct BatchPoll {
+  int32_t* h = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool ok = false;
+  int init() {
+    if (ok) return JNO_OK;
+    JNO_HIP(hipHostMalloc((void**)&h, 2 * sizeof(int32_t), hipHostMallocDefault));
+    JNO_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    JNO_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    ok = true;
+    return JNO_OK;
+  }
+};
+static thread_local BatchPoll g_bpoll;
+
+// The per-call graph of a batched solve and its capture stream, released on every return path
+// (after the solve's stream has drained, so no queued replay still refers to them).
+struct BatchGraph {
+  hipStream_t s = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BatchGraph() {
+    if (exec || graph || cap) (void)hipStreamSynchronize(s);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
 }  // namespace jno
 
 using namespace jno;
@@ -708,27 +735,26 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
     return launch_kb(st);
   };
 
-  int rc = launch_kb(s);  // the first true residual r = b
+  int rc = g_bpoll.init();
   if (rc) return rc;
-  int32_t* h_act = nullptr;
-  JNO_HIP(hipHostMalloc((void**)&h_act, 2 * sizeof(int32_t), hipHostMallocDefault));
-  hipEvent_t ev[2];
-  JNO_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  JNO_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipStream_t cap = nullptr;
+  int32_t* h_act = g_bpoll.h;
+  hipEvent_t* ev = g_bpoll.ev;
+  rc = launch_kb(s);  // the first true residual r = b
+  if (rc) return rc;
+  BatchGraph bg;
+  bg.s = s;
   const int chunk = steps_per_graph > 0 ? steps_per_graph : 8;
   if (steps_per_graph > 0) {
     // capture one chunk of steps on a private stream (default priority), replay it on `s`
-    JNO_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    JNO_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk && rc == JNO_OK; ++i) rc = launch_step(cap);
-    hipError_t ce = hipStreamEndCapture(cap, &graph);
+    JNO_HIP(hipStreamCreateWithFlags(&bg.cap, hipStreamNonBlocking));
+    JNO_HIP(hipStreamBeginCapture(bg.cap, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk && rc == JNO_OK; ++i) rc = launch_step(bg.cap);
+    hipError_t ce = hipStreamEndCapture(bg.cap, &bg.graph);
     if (rc) return rc;
     JNO_HIP(ce);
-    JNO_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    JNO_HIP(hipGraphInstantiate(&bg.exec, bg.graph, nullptr, nullptr, 0));
   }
+  hipGraphExec_t exec = bg.exec;
   const auto t0 = std::chrono::steady_clock::now();
   int slot = 0, pending = 0;
   int64_t launched = 0;
@@ -757,15 +783,8 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
   JNO_HIP(hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::vector<BState> st(n_items);
-  JNO_HIP(hipMemcpy(st.data(), L.st, sizeof(BState) * n_items, hipMemcpyDeviceToHost));
-  if (exec) {
-    JNO_HIP(hipGraphExecDestroy(exec));
-    JNO_HIP(hipGraphDestroy(graph));
-    JNO_HIP(hipStreamDestroy(cap));
-  }
-  JNO_HIP(hipEventDestroy(ev[0]));
-  JNO_HIP(hipEventDestroy(ev[1]));
-  JNO_HIP(hipHostFree(h_act));
+  JNO_HIP(hipMemcpyAsync(st.data(), L.st, sizeof(BState) * n_items, hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
   int worst = JNO_OK;
   for (int i = 0; i < n_items; ++i) {
     const double bn = sqrt(st[i].bb);

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -60,8 +61,19 @@ enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SC
 // FP32: the register-resident one-kernel iteration with late far loads (k1_vector 2), 256 threads x
 // 768 workgroups (three tiles of ~930 rows each at 128^3); operators it does not take run the
 // LDS-staged one (ONE_FALLBACK below)
-static Tuning g_tune[2] = {{512, 0, 1024, 1}, {768, 2, 256, 5}};  // [0] FP64 iterations, [1] FP32 iterations
 constexpr Tuning ONE_FALLBACK = {512, 0, 512, 5};
+
+// The tuning one solve runs: the launch shapes per iteration precision ([0] FP64 iterations, [1]
+// FP32 inner iterations), the iterations per host poll and pacing.  Taken from the operator's own
+// jno_tuning when it has one, else copied from the library defaults under g_def_mu at the start of
+// the solve, so concurrent solves never share mutable launch state.
+struct CallTuning {
+  Tuning it[2];
+  int chunk;
+  int pace;
+};
+static std::mutex g_def_mu;
+static CallTuning g_def = {{{512, 0, 1024, 1}, {768, 2, 256, 5}}, 16, 1};
 constexpr int N_XCD = 8;
 
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
@@ -1986,6 +1998,41 @@ struct HostPoll {
 };
 static thread_local HostPoll g_poll;
 
+// How a solve reads the device-side state back.  Pinned (the default): into the thread's pinned
+// slots with an event each, one chunk behind the launches, so the queue never drains.  Sync (the
+// allocation-free entry points): into host memory of the solve itself, read synchronously, so
+// nothing is allocated or created; the queue drains at each poll.  The control flow (and hence the
+// launches and results) is the same in both.
+struct Poller {
+  bool sync = false;
+  PcgState local[2];
+  PcgState* slots() { return sync ? local : g_poll.h; }
+  int init() {
+    if (sync) return JNO_OK;
+    JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
+    return JNO_OK;
+  }
+  int post(int slot, const PcgState* d, hipStream_t s) {
+    JNO_HIP(hipMemcpyAsync(&slots()[slot], d, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    if (sync) {
+      JNO_HIP(hipStreamSynchronize(s));
+    } else {
+      JNO_HIP(hipEventRecord(g_poll.ev[slot], s));
+    }
+    return JNO_OK;
+  }
+  int wait(int slot) {
+    if (!sync) JNO_HIP(hipEventSynchronize(g_poll.ev[slot]));
+    return JNO_OK;
+  }
+  int read(const PcgState* d, hipStream_t s, PcgState* out) {
+    JNO_HIP(hipMemcpyAsync(&slots()[0], d, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipStreamSynchronize(s));
+    *out = slots()[0];
+    return JNO_OK;
+  }
+};
+
 
 // The SpMV-carrying kernel of one iteration: K1 (MODE_FUSED), KS (MODE_SPLIT) or KB
 // (MODE_SINGLE), by format, DIA diagonal count (4: the 3-D Kuhn lattice; 3: the 2-D triangle
@@ -2260,7 +2307,7 @@ struct KernelProf {
     return JNO_OK;
   }
 };
-static KernelProf g_prof;
+static thread_local KernelProf g_prof;  // per thread: times the solves of the thread that enabled it
 
 // Runs K1/K2 chunks until the device reports a stop; returns the final state in *out.
 // Pacing of an inner loop's launches (one-kernel iterations).  Launches enqueued after the stop
@@ -2279,9 +2326,12 @@ constexpr int PACE_FIRST = 4;
 constexpr int PACE_MARGIN = 1;
 
 template <typename T, typename V>
-static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid, hipStream_t s,
-                     PcgState* out, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr, InnerPace* pace = nullptr) {
-  JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
+static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid,
+                     hipStream_t s, PcgState* out, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr,
+                     InnerPace* pace = nullptr) {
+  int rc0 = poll.init();
+  if (rc0) return rc0;
+  const bool prof = g_prof.on && !poll.sync;  // the allocation-free path creates no events
   int64_t k = k0;
   int slot = 0, pending = 0, round = 0;
   int64_t upto[2] = {k0, k0};  // launches enqueued before each outstanding poll
@@ -2303,7 +2353,7 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
         a.q = qbuf[k & 1];
         a.q_out = qbuf[(k + 1) & 1];
       }
-      if (g_prof.on && rbuf) {
+      if (prof && rbuf) {
         if (k % g_prof.stride == 0) {
           if (g_prof.ensure(nev + 2) != JNO_OK) return JNO_EHIP;
           JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
@@ -2317,7 +2367,7 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
           nev += 2;
           win = -1;
         }
-      } else if (g_prof.on && k % g_prof.stride == 0) {
+      } else if (prof && k % g_prof.stride == 0) {
         if (g_prof.ensure(nev + 4) != JNO_OK) return JNO_EHIP;
         JNO_HIP(hipEventRecord(g_prof.ev[nev], s));
         int rc = launch_iter<T, V>(op, a, grid, s, &g_prof.ev[nev + 1]);
@@ -2329,15 +2379,16 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
         if (rc) return rc;
       }
     }
-    JNO_HIP(hipMemcpyAsync(&g_poll.h[slot], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    JNO_HIP(hipEventRecord(g_poll.ev[slot], s));
+    int rcp = poll.post(slot, a.st, s);
+    if (rcp) return rcp;
     upto[slot] = k;
     ++round;
     n = chunk;
     if (pending) {
       const int prev = slot ^ 1;
-      JNO_HIP(hipEventSynchronize(g_poll.ev[prev]));
-      const PcgState& h = g_poll.h[prev];
+      rcp = poll.wait(prev);
+      if (rcp) return rcp;
+      const PcgState& h = poll.slots()[prev];
       if (h.done1) break;
       if (pace && rbuf) {
         // r.r read by the last launch before that poll (a resuming launch does not record it, but
@@ -2364,10 +2415,9 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
     pending = 1;
     slot ^= 1;
   }
-  JNO_HIP(hipMemcpyAsync(&g_poll.h[0], a.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-  JNO_HIP(hipStreamSynchronize(s));
-  *out = g_poll.h[0];
-  if (g_prof.on && rbuf) {
+  int rcr = poll.read(a.st, s, out);
+  if (rcr) return rcr;
+  if (prof && rbuf) {
     for (size_t t = 0; t < sampled.size(); ++t) {
       if (sampled[t] + PROF_WIN - 1 >= out->k_stop) break;  // every launch of the window did work
       float ms = 0;
@@ -2375,7 +2425,7 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
       g_prof.ms[1] += ms;
       g_prof.n += PROF_WIN;
     }
-  } else if (g_prof.on) {
+  } else if (prof) {
     for (size_t t = 0; t < sampled.size(); ++t) {
       if (sampled[t] >= out->k_stop) break;  // launches after the stop did no work
       for (int j = 0; j < 3; ++j) {
@@ -2389,12 +2439,10 @@ static int run_inner(const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, 
   return JNO_OK;
 }
 
-static int read_state(PcgState* d, PcgState* out, hipStream_t s) {
-  JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
-  JNO_HIP(hipMemcpyAsync(&g_poll.h[0], d, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-  JNO_HIP(hipStreamSynchronize(s));
-  *out = g_poll.h[0];
-  return JNO_OK;
+static int read_state(Poller& poll, PcgState* d, PcgState* out, hipStream_t s) {
+  int rc = poll.init();
+  if (rc) return rc;
+  return poll.read(d, s, out);
 }
 
 static int finish_status(const PcgState& f, jno_pcg_stats* stats, int64_t iters) {
@@ -2461,8 +2509,52 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   return a;
 }
 
-static int g_chunk = 16;  // iterations enqueued per host poll (8: 1% slower at 128^3)
-static int g_pace = 1;    // paced chunks for one-kernel inner loops (InnerPace)
+// A launch shape is valid when it is one jno_pcg_set_tuning accepts (the message names the field).
+static int check_shape(const jno_launch_shape& t, const char* who) {
+  JNO_REQUIRE(t.grid_cap >= 8 && t.grid_cap <= PCG_MAX_BLOCKS && t.grid_cap % 8 == 0, JNO_EINVAL,
+              "%s: grid_cap must be a multiple of 8 in 8..%d", who, PCG_MAX_BLOCKS);
+  JNO_REQUIRE(t.block == 256 || t.block == 512 || t.block == 1024, JNO_EINVAL, "%s: block must be 256, 512 or 1024",
+              who);
+  JNO_REQUIRE(t.split >= MODE_FUSED && t.split <= MODE_ONE, JNO_EINVAL,
+              "%s: split must be 0 (K1 K2), 1 (K0 KS K2), 2 (single reduction, KA KB), 3 (K0 KS K2 on the "
+              "Jacobi-scaled operator), 4 (K1 K2 on the Jacobi-scaled operator) or 5 (one kernel per iteration on "
+              "the Jacobi-scaled operator)",
+              who);
+  JNO_REQUIRE(t.k1_vector >= 0 && t.k1_vector <= 3, JNO_EINVAL,
+              "%s: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles), 2 (4 rows) or 3 (p staged in LDS, "
+              "iteration mode 4); with mode 5: 0 the LDS-staged kernel, 1 and 2 the register-resident one (far "
+              "loads before / after the head)",
+              who);
+  return JNO_OK;
+}
+
+static int check_tuning(const jno_tuning* t, const char* who) {
+  for (int p = 0; p < 2; ++p) {
+    int rc = check_shape(t->iter[p], who);
+    if (rc) return rc;
+  }
+  JNO_REQUIRE(t->chunk >= 1 && t->chunk <= 1024, JNO_EINVAL, "%s: chunk must be in 1..1024", who);
+  JNO_REQUIRE(t->pacing == 0 || t->pacing == 1, JNO_EINVAL, "%s: pacing must be 0 or 1", who);
+  return JNO_OK;
+}
+
+// The tuning of one solve on `op` (its own, validated, or a copy of the defaults).
+static int call_tuning(const jno_op* op, CallTuning* out, const char* who) {
+  if (op->tuning) {
+    int rc = check_tuning(op->tuning, who);
+    if (rc) return rc;
+    for (int p = 0; p < 2; ++p) {
+      const jno_launch_shape& t = op->tuning->iter[p];
+      out->it[p] = Tuning{t.grid_cap, t.k1_vector, t.block, t.split};
+    }
+    out->chunk = op->tuning->chunk;
+    out->pace = op->tuning->pacing;
+    return JNO_OK;
+  }
+  std::lock_guard<std::mutex> lk(g_def_mu);
+  *out = g_def;
+  return JNO_OK;
+}
 
 }  // namespace jno
 
@@ -2492,41 +2584,54 @@ extern "C" int jno_pcg_profile_read(double* h_ms, int64_t* h_count) {
 extern "C" int jno_pcg_set_tuning(int precision, int grid_cap, int k1_vector, int block, int split) {
   JNO_REQUIRE(precision == 0 || precision == 1, JNO_EINVAL,
               "jno_pcg_set_tuning: precision must be 0 (FP64 iterations) or 1 (FP32 iterations)");
-  JNO_REQUIRE(grid_cap >= 8 && grid_cap <= PCG_MAX_BLOCKS && grid_cap % 8 == 0, JNO_EINVAL,
-              "jno_pcg_set_tuning: grid_cap must be a multiple of 8 in 8..%d", PCG_MAX_BLOCKS);
-  JNO_REQUIRE(block == 256 || block == 512 || block == 1024, JNO_EINVAL,
-              "jno_pcg_set_tuning: block must be 256, 512 or 1024");
-  JNO_REQUIRE(split >= MODE_FUSED && split <= MODE_ONE, JNO_EINVAL,
-              "jno_pcg_set_tuning: split must be 0 (K1 K2), 1 (K0 KS K2), 2 (single reduction, KA KB), 3 (K0 KS K2 "
-              "on the Jacobi-scaled operator), 4 (K1 K2 on the Jacobi-scaled operator) or 5 (one kernel per "
-              "iteration on the Jacobi-scaled operator)");
-  JNO_REQUIRE(k1_vector >= 0 && k1_vector <= 3, JNO_EINVAL,
-              "jno_pcg_set_tuning: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles), 2 (4 rows) or 3 "
-              "(p staged in LDS, iteration mode 4); with mode 5: 0 the LDS-staged kernel, 1 and 2 the "
-              "register-resident one (far loads before / after the head)");
-  g_tune[precision] = Tuning{grid_cap, k1_vector, block, split};
+  const jno_launch_shape t = {grid_cap, k1_vector, block, split};
+  int rc = check_shape(t, "jno_pcg_set_tuning");
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_def_mu);
+  g_def.it[precision] = Tuning{grid_cap, k1_vector, block, split};
   return JNO_OK;
 }
 
 extern "C" int jno_pcg_get_tuning(int precision, int* grid_cap, int* k1_vector, int* block, int* split) {
   JNO_REQUIRE(precision == 0 || precision == 1, JNO_EINVAL,
               "jno_pcg_get_tuning: precision must be 0 (FP64 iterations) or 1 (FP32 iterations)");
-  if (grid_cap) *grid_cap = g_tune[precision].grid_cap;
-  if (k1_vector) *k1_vector = g_tune[precision].vec;
-  if (block) *block = g_tune[precision].block;
-  if (split) *split = g_tune[precision].split;
+  std::lock_guard<std::mutex> lk(g_def_mu);
+  const Tuning& t = g_def.it[precision];
+  if (grid_cap) *grid_cap = t.grid_cap;
+  if (k1_vector) *k1_vector = t.vec;
+  if (block) *block = t.block;
+  if (split) *split = t.split;
   return JNO_OK;
 }
 
 extern "C" int jno_pcg_set_chunk(int chunk) {
   JNO_REQUIRE(chunk >= 1 && chunk <= 1024, JNO_EINVAL, "jno_pcg_set_chunk: chunk must be in 1..1024");
-  g_chunk = chunk;
+  std::lock_guard<std::mutex> lk(g_def_mu);
+  g_def.chunk = chunk;
   return JNO_OK;
 }
 
 extern "C" int jno_pcg_set_pacing(int enable) {
-  g_pace = enable ? 1 : 0;
+  std::lock_guard<std::mutex> lk(g_def_mu);
+  g_def.pace = enable ? 1 : 0;
   return JNO_OK;
+}
+
+extern "C" int jno_pcg_default_tuning(jno_tuning* h_out) {
+  JNO_REQUIRE(h_out, JNO_EINVAL, "jno_pcg_default_tuning: null output");
+  std::lock_guard<std::mutex> lk(g_def_mu);
+  for (int p = 0; p < 2; ++p) {
+    const Tuning& t = g_def.it[p];
+    h_out->iter[p] = jno_launch_shape{t.grid_cap, t.vec, t.block, t.split};
+  }
+  h_out->chunk = g_def.chunk;
+  h_out->pacing = g_def.pace;
+  return JNO_OK;
+}
+
+extern "C" int jno_pcg_check_tuning(const jno_tuning* h_tuning) {
+  JNO_REQUIRE(h_tuning, JNO_EINVAL, "jno_pcg_check_tuning: null tuning");
+  return check_tuning(h_tuning, "jno_pcg_check_tuning");
 }
 
 extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
@@ -2538,13 +2643,17 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   JNO_REQUIRE(d_dinv && d_b && d_x, JNO_EINVAL, "jno_pcg_f64: null vector");
   JNO_REQUIRE(ws_bytes >= jno_pcg_workspace_size(n, 0), JNO_EWORKSPACE, "jno_pcg_f64: workspace too small");
   JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_f64: maxiter must be >= 0");
+  CallTuning ct;
+  rc = call_tuning(h_op, &ct, "jno_pcg_f64");
+  if (rc) return rc;
+  Poller poll;
   hipStream_t s = as_stream(stream);
   Ws w;
   ws_layout(n, 0, static_cast<char*>(d_ws), &w);
   const auto t0 = std::chrono::steady_clock::now();
   int G;
   int64_t rpb;
-  const Tuning tu = g_tune[0];
+  const Tuning tu = ct.it[0];
   pcg_grid(h_op, tu, &G, &rpb);
   const int gz = grid_for(n, PCG_NT, 2048);
   double* r = (double*)w.r;
@@ -2570,7 +2679,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
     if (rc) return rc;
   }
   PcgState f;
-  rc = run_inner<double, double>(h_op, a, pb, 0, g_chunk, G, s, &f);
+  rc = run_inner<double, double>(poll, h_op, a, pb, 0, ct.chunk, G, s, &f);
   if (rc) return rc;
   // true residual ||b - A x|| / ||b||  (the eager guard of _maybe_residual_check)
   rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
@@ -2579,7 +2688,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 0, 0, single);
   JNO_LAUNCH_CHECK();
   PcgState e;
-  rc = read_state(w.st, &e, s);
+  rc = read_state(poll, w.st, &e, s);
   if (rc) return rc;
   const auto t1 = std::chrono::steady_clock::now();
   const double bn = sqrt(e.bb);
@@ -2593,6 +2702,20 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   return finish_status(f, h_stats, f.k_stop);
 }
 
+namespace jno {
+
+// Argument checks of jno_pcg_mixed and its allocation-free variant.
+int pcg_mixed_check(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int64_t maxiter,
+                    double replace_factor, const char* who) {
+  int rc = check_op(h_op, 1);
+  if (rc) return rc;
+  JNO_REQUIRE(d_dinv && d_b && d_x, JNO_EINVAL, "%s: null vector", who);
+  JNO_REQUIRE(replace_factor > 0.0 && replace_factor < 1.0, JNO_EINVAL, "%s: replace_factor must be in (0, 1)", who);
+  JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "%s: maxiter must be >= 0", who);
+  CallTuning ct;
+  return call_tuning(h_op, &ct, who);
+}
+
 // FP32 inner iterations on the Jacobi-scaled DIA operator (MODE_SCALED).  Returns JNO_OK with
 // *used = false, having changed nothing the caller relies on, when the operator's scaled main
 // diagonal is not exactly 1 in FP32 (a zero-diagonal guard row): the caller then runs the
@@ -2600,9 +2723,10 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
 // FP64 true residual decides convergence, and when the remaining reduction is smaller than a
 // replacement cycle the next cycle is aimed at it (with margin), so the last cycle does not
 // overshoot the tolerance by a full cycle.
-static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
-                        double tol, double atol, int64_t maxiter, double repl2, jno_pcg_stats* h_stats, const Ws& w,
-                        int G, int64_t rpb, const Tuning& tu, hipStream_t s, bool* used) {
+static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, const double* d_dinv,
+                        const double* d_b, double* d_x, int use_x0, double tol, double atol, int64_t maxiter,
+                        double repl2, jno_pcg_stats* h_stats, const Ws& w, int G, int64_t rpb, const Tuning& tu,
+                        hipStream_t s, bool* used) {
   *used = false;
   const int64_t n = h_op->n;
   const int gz = grid_for(n, PCG_NT, 2048);
@@ -2662,8 +2786,8 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   InnerPace pace;
   for (;;) {
     const int64_t k_start = k;
-    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr,
-                                 one && g_pace ? &pace : nullptr);
+    rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr,
+                                 one && ct.pace ? &pace : nullptr);
     if (rc) return rc;
     if (f.outer_done) break;
     k = f.k_stop;
@@ -2701,19 +2825,19 @@ static int mixed_scaled(const jno_op* h_op, const double* d_dinv, const double* 
   return finish_status(f, h_stats, k);
 }
 
-extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
-                             double tol, double atol, int64_t maxiter, double replace_factor,
-                             jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream) {
-  int rc = check_op(h_op, 1);
-  if (rc) return rc;
+// The FP32-inner / FP64-refinement solve behind jno_pcg_mixed and its allocation-free variant:
+// `sync_poll` reads the device state back synchronously (no pinned memory, no events).  The
+// arguments are checked by the callers.
+int pcg_mixed_impl(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0, double tol,
+                   double atol, int64_t maxiter, double replace_factor, jno_pcg_stats* h_stats, void* d_ws,
+                   size_t ws_bytes, hipStream_t s, bool sync_poll) {
   const int64_t n = h_op->n;
-  JNO_REQUIRE(d_dinv && d_b && d_x, JNO_EINVAL, "jno_pcg_mixed: null vector");
-  JNO_REQUIRE(ws_bytes >= jno_pcg_workspace_size(n, 1), JNO_EWORKSPACE, "jno_pcg_mixed: workspace too small");
-  JNO_REQUIRE(replace_factor > 0.0 && replace_factor < 1.0, JNO_EINVAL,
-              "jno_pcg_mixed: replace_factor must be in (0, 1)");
-  JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_mixed: maxiter must be >= 0");
-  hipStream_t s = as_stream(stream);
-  const Tuning tu = effective_tuning(g_tune[1], h_op);
+  CallTuning ct;
+  int rc = call_tuning(h_op, &ct, "jno_pcg_mixed");
+  if (rc) return rc;
+  Poller poll;
+  poll.sync = sync_poll;
+  const Tuning tu = effective_tuning(ct.it[1], h_op);
   const int64_t se = scaled_elems(h_op);
   const bool room = se > 0 && ws_bytes >= ws_layout(n, 1, nullptr, nullptr, se);
   Ws w;
@@ -2723,8 +2847,8 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   pcg_grid(h_op, tu, &G, &rpb);
   if (room && (tu.split == MODE_SCALED || tu.split == MODE_SCALED_FUSED || tu.split == MODE_ONE)) {
     bool used = false;
-    rc = mixed_scaled(h_op, d_dinv, d_b, d_x, use_x0, tol, atol, maxiter, replace_factor * replace_factor, h_stats,
-                      w, G, rpb, tu, s, &used);
+    rc = mixed_scaled(poll, ct, h_op, d_dinv, d_b, d_x, use_x0, tol, atol, maxiter, replace_factor * replace_factor,
+                      h_stats, w, G, rpb, tu, s, &used);
     if (rc || used) return rc;
   }
   const auto t0 = std::chrono::steady_clock::now();
@@ -2759,7 +2883,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;
   for (;;) {
-    rc = run_inner<float, float>(h_op, a, pb, k, g_chunk, G, s, &f);
+    rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f);
     if (rc) return rc;
     k = f.k_stop;
     // FP64 refinement: x += xs, r = b - A x, rs = (float) r
@@ -2780,7 +2904,7 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
       rc = launch_kb<float, float>(h_op, a, G, s);
       if (rc) return rc;
     }
-    rc = read_state(w.st, &e, s);
+    rc = read_state(poll, w.st, &e, s);
     if (rc) return rc;
     ++refinements;
     rr_true = e.rr_true;
@@ -2805,4 +2929,16 @@ extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const dou
     h_stats->seconds = std::chrono::duration<double>(t1 - t0).count();
   }
   return finish_status(f, h_stats, k);
+}
+
+}  // namespace jno
+
+extern "C" int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x, int use_x0,
+                             double tol, double atol, int64_t maxiter, double replace_factor,
+                             jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream) {
+  int rc = jno::pcg_mixed_check(h_op, d_dinv, d_b, d_x, maxiter, replace_factor, "jno_pcg_mixed");
+  if (rc) return rc;
+  JNO_REQUIRE(ws_bytes >= jno_pcg_workspace_size(h_op->n, 1), JNO_EWORKSPACE, "jno_pcg_mixed: workspace too small");
+  return jno::pcg_mixed_impl(h_op, d_dinv, d_b, d_x, use_x0, tol, atol, maxiter, replace_factor, h_stats, d_ws,
+                             ws_bytes, as_stream(stream), false);
 }

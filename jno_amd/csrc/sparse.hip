@@ -323,9 +323,11 @@ __global__ void dia_fill_k(int64_t n, const int32_t* __restrict__ rowptr, const 
   }
 }
 
-// After dia_fill_k: every stored entry off the stored diagonal set must be exactly 0, and every
-// lower entry (r, c), c < r, must equal bitwise the stored upper value A[c, r] = dia[d][c].
-// Sets *bad otherwise (the layout would not reproduce A).
+// After dia_fill_k: every stored entry off the stored diagonal set must be exactly 0, every lower
+// entry (r, c), c < r, must equal bitwise the stored upper value A[c, r] = dia[d][c], and every
+// nonzero upper entry (r, c), c > r, must have its lower twin (c, r) in the pattern (the layout reads
+// A[c, r] through symmetry; rows hold ascending columns).  Sets *bad otherwise (the layout would
+// not reproduce A).
 __global__ void dia_check_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                             const double* __restrict__ vals, int ndiag, DiaOffsets off,
                             const double* __restrict__ d64, int* __restrict__ bad) {
@@ -345,6 +347,14 @@ __global__ void dia_check_k(int64_t n, const int32_t* __restrict__ rowptr, const
         ok = ok && v == 0.0;
       } else if (o < 0) {
         ok = ok && __double_as_longlong(v) == __double_as_longlong(d64[(int64_t)dd * ld + c]);
+      } else if (o > 0 && v != 0.0) {
+        int lo = rowptr[c], hi = rowptr[c + 1];
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (col[mid] < (int)r) lo = mid + 1;
+          else hi = mid;
+        }
+        ok = ok && lo < rowptr[c + 1] && col[lo] == (int)r;
       }
     }
   }

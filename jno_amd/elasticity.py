@@ -91,6 +91,11 @@ class ElasticitySystem:
         _lib.call("jno_traction_p2", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, _lib.ptr(self.plan.inc_ptr),
                   _lib.ptr(self.plan.inc), axis, self.load_value, t, _lib.ptr(self._b0), _lib.stream_ptr(m.device))
 
+    def traction_load(self) -> torch.Tensor:
+        """The assembled traction load int_F t . phi_i dS (3 dofs per node, before the Dirichlet
+        elimination); it does not depend on E."""
+        return self._b0
+
     def evaluate(self, args: dict, out: Optional[CSR] = None):
         m, plan = self.mesh, self.plan
         E = torch.as_tensor(args[self.param], dtype=torch.float64, device=m.device)

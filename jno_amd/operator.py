@@ -16,6 +16,7 @@ Both carry FP64 values (plain PCG, FP64 residuals) and FP32 values (mixed-precis
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional
 
 import torch
@@ -24,19 +25,29 @@ from . import _lib
 from .fem import CSR
 
 
-# DIA offsets of earlier operators, per pattern (the plan's index arrays), fewest diagonals first:
-# a re-assembled operator on the same plan reuses them through the checked fill instead of a new
-# analysis.  One pattern can carry several sets (the stiffness has exact zeros on diagonals where
-# the mass or a transient step operator of the same plan does not).
+# DIA offsets of earlier operators, per pattern (the index tensors of a plan), fewest diagonals
+# first: a re-assembled operator on the same plan reuses them through the checked fill instead of a
+# new analysis.  One pattern can carry several sets (the stiffness has exact zeros on diagonals
+# where the mass or a transient step operator of the same plan does not).  Entries are keyed by the
+# identity of the live index tensors and dropped when those are freed, so a later pattern at a
+# reused address never sees them.
 _DIA_OFFSETS: dict = {}
 
 
 def _pattern_key(A: CSR):
-    return (A.rowptr.data_ptr(), A.col.data_ptr(), A.shape[0], A.nnz, A.family)
+    return (id(A.rowptr), id(A.col), A.shape[0], A.nnz, A.family)
+
+
+def _forget(key):
+    _DIA_OFFSETS.pop(key, None)
 
 
 def _remember_offsets(A: CSR, nd, offs, chosen):
-    sets = _DIA_OFFSETS.setdefault(_pattern_key(A), [])
+    key = _pattern_key(A)
+    if key not in _DIA_OFFSETS:
+        weakref.finalize(A.rowptr, _forget, key)
+        weakref.finalize(A.col, _forget, key)
+    sets = _DIA_OFFSETS.setdefault(key, [])
     for i, (nd0, offs0, chosen0) in enumerate(sets):
         if offs0 == offs:
             sets[i] = (nd0, offs0, chosen0 or chosen)
@@ -62,6 +73,7 @@ class SolverOperator:
         self.device = A.device
         self._keep = []  # tensors referenced by the C struct
         self.op = _lib.JnoOp()
+        self.tuning = None  # a jno_amd.solve.Tuning attached by the autotuner (None: library defaults)
         fmt = fmt.lower()
         if fmt not in ("auto", "dia", "csr", "bsr3"):
             raise ValueError("format must be 'auto', 'dia', 'bsr3' or 'csr'")
@@ -101,6 +113,7 @@ class SolverOperator:
         op.device = A.device
         op._keep = []
         op.op = _lib.JnoOp()
+        op.tuning = None
         op._set_dia(nd, list(offs), d64, d32)
         op.format = "dia"
         return op

@@ -10,6 +10,7 @@ on the true FP64 residual.
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass, replace
 from typing import Optional
 
 import torch
@@ -20,11 +21,46 @@ from .operator import SolverOperator
 from .precond import Diagonal
 
 
+@dataclass(frozen=True)
+class Tuning:
+    """Launch tuning of a solve (the C ``jno_tuning``), passed per call: ``(block, grid_cap, k1_vector,
+    split)`` for the FP64 iterations and for the FP32 inner iterations of mixed solves, iterations
+    per host poll, and paced chunks.  Hashable by value, as the reference's solver specs are
+    (jno/utils/solver/solver_api.py:261-308), so it can be part of a solver's key."""
+
+    fp64: tuple = (1024, 512, 0, 1)
+    mixed: tuple = (256, 768, 2, 5)
+    chunk: int = 16
+    pacing: bool = True
+
+    @staticmethod
+    def defaults() -> "Tuning":
+        """The library defaults (what a solve without its own tuning runs)."""
+        t = _lib.JnoTuning()
+        _lib.call("jno_pcg_default_tuning", C.byref(t))
+        shape = lambda s: (int(s.block), int(s.grid_cap), int(s.k1_vector), int(s.split))  # noqa: E731
+        return Tuning(shape(t.iter[0]), shape(t.iter[1]), int(t.chunk), bool(t.pacing))
+
+    def with_shape(self, precision: str, block: int, grid_cap: int, k1_vector: int, split: int = 0) -> "Tuning":
+        shape = (int(block), int(grid_cap), int(k1_vector), int(split))
+        return replace(self, **{"mixed" if _prec_index(precision) else "fp64": shape})
+
+    def to_c(self) -> "_lib.JnoTuning":
+        t = _lib.JnoTuning()
+        for i, (blk, cap, vec, spl) in enumerate((self.fp64, self.mixed)):
+            t.iter[i].block, t.iter[i].grid_cap, t.iter[i].k1_vector, t.iter[i].split = blk, cap, vec, spl
+        t.chunk = int(self.chunk)
+        t.pacing = 1 if self.pacing else 0
+        _lib.call("jno_pcg_check_tuning", C.byref(t))
+        return t
+
+
 class LinearSolver:
     """A configured solver: ``solver(A, b, *, M=None, x0=None) -> x`` (solver_api.py:243-297)."""
 
     def __init__(self, name: str, tol: float, atol: float, maxiter: Optional[int], precision: str = "fp64",
-                 format: str = "auto", replace_factor: float = 0.1, guard: bool = True):
+                 format: str = "auto", replace_factor: float = 0.1, guard: bool = True,
+                 tuning: Optional[Tuning] = None):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.name = name
@@ -34,7 +70,8 @@ class LinearSolver:
         self.format = format
         self.replace_factor = float(replace_factor)
         self.guard = guard
-        self.key = (name, self.tol, self.atol, self.maxiter, precision, format, self.replace_factor)
+        self.tuning = tuning
+        self.key = (name, self.tol, self.atol, self.maxiter, precision, format, self.replace_factor, tuning)
         self.last_stats: dict = {}
 
     def __eq__(self, other):
@@ -83,16 +120,26 @@ class LinearSolver:
             use_x0 = 1
         mixed = self.precision == "mixed"
         lib = _lib.load()
-        wsb = lib.jno_pcg_workspace_size_op(C.byref(op.op), 1 if mixed else 0)
+        # the tuning travels with this call: the solver's own, else the one the autotuner attached to
+        # the operator, else the library defaults (tuning = NULL)
+        opc = op.op
+        tun = self.tuning if self.tuning is not None else getattr(op, "tuning", None)
+        tc = None
+        if tun is not None:
+            tc = tun.to_c()
+            opc = _lib.JnoOp.from_buffer_copy(op.op)
+            opc.tuning = C.addressof(tc)
+        wsb = lib.jno_pcg_workspace_size_op(C.byref(opc), 1 if mixed else 0)
         ws = torch.empty(wsb, dtype=torch.uint8, device=A.device)
         stats = _lib.JnoPcgStats()
         stream = _lib.stream_ptr(A.device)
         if mixed:
-            rc = lib.jno_pcg_mixed(C.byref(op.op), _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), use_x0, self.tol,
+            rc = lib.jno_pcg_mixed(C.byref(opc), _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), use_x0, self.tol,
                                    self.atol, self.maxiter, self.replace_factor, C.byref(stats), _lib.ptr(ws), wsb, stream)
         else:
-            rc = lib.jno_pcg_f64(C.byref(op.op), _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), use_x0, self.tol, self.atol,
+            rc = lib.jno_pcg_f64(C.byref(opc), _lib.ptr(dinv), _lib.ptr(b), _lib.ptr(x), use_x0, self.tol, self.atol,
                                  self.maxiter, C.byref(stats), _lib.ptr(ws), wsb, stream)
+        del tc
         self.last_stats = {
             "iterations": int(stats.iterations),
             "refinements": int(stats.refinements),
@@ -114,13 +161,16 @@ class LinearSolver:
 
 
 def cg(*, tol: float = 1e-8, atol: float = 0.0, maxiter: Optional[int] = 20_000, precision: str = "fp64",
-       format: str = "auto", replace_factor: float = 0.1) -> LinearSolver:
-    """Jacobi-PCG slot: the analogue of ``jno.solve.cg(tol=, atol=, maxiter=)``."""
-    return LinearSolver("cg", tol, atol, maxiter, precision=precision, format=format, replace_factor=replace_factor)
+       format: str = "auto", replace_factor: float = 0.1, tuning: Optional[Tuning] = None) -> LinearSolver:
+    """Jacobi-PCG slot: the analogue of ``jno.solve.cg(tol=, atol=, maxiter=)``; ``tuning`` (a
+    :class:`Tuning`) fixes the launch shape of this solver's calls."""
+    return LinearSolver("cg", tol, atol, maxiter, precision=precision, format=format, replace_factor=replace_factor,
+                        tuning=tuning)
 
 
 def set_chunk(chunk: int):
-    """Iterations enqueued per host poll of the device-side convergence state."""
+    """Default iterations enqueued per host poll of the device-side convergence state (solves
+    without their own :class:`Tuning`)."""
     _lib.call("jno_pcg_set_chunk", int(chunk))
 
 
@@ -137,15 +187,14 @@ def _prec_index(precision: str) -> int:
 
 
 def set_tuning(precision: str, block: int, grid_cap: int, k1_vector: int, split: int = 0):
-    """Launch shape of the PCG iteration kernels for ``precision`` ('fp64' iterations or the FP32
-    inner iterations of 'mixed'): threads per workgroup, maximum workgroups, DIA tile variant, and
-    two (``split=0``) or three (``split=1``: p materialized before a plain SpMV) kernels per
-    iteration."""
+    """Default launch shape of the PCG iteration kernels for ``precision`` ('fp64' iterations or the
+    FP32 inner iterations of 'mixed'), for solves without their own :class:`Tuning`: threads per
+    workgroup, maximum workgroups, DIA tile variant, and the iteration mode (``split``)."""
     _lib.call("jno_pcg_set_tuning", _prec_index(precision), int(grid_cap), int(k1_vector), int(block), int(split))
 
 
 def get_tuning(precision: str):
-    """Current ``(block, grid_cap, k1_vector, split)`` for ``precision``."""
+    """Current default ``(block, grid_cap, k1_vector, split)`` for ``precision``."""
     cap, vec, blk, spl = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     _lib.call("jno_pcg_get_tuning", _prec_index(precision), C.byref(cap), C.byref(vec), C.byref(blk), C.byref(spl))
     return int(blk.value), int(cap.value), int(vec.value), int(spl.value)

@@ -138,3 +138,56 @@ def test_c4_batch_256_uneven_and_64_rhs(dev):
         errs.append((e_batch, e_single))
         assert e_batch <= max(1e-9, 3.0 * e_single), (j, e_batch, e_single)
     print(f"C4 64 rhs on 96^3: {dt:.3f} s; error vs rtol-1e-13 FP64 (batched, single) {errs}")
+
+
+def test_batch_nonfinite_rhs_is_reported(dev):
+    # a NaN in one right-hand side makes its true residual NaN: that item stops as non-finite (not
+    # as converged) and the call returns JNO_ENONFINITE; the other item still converges
+    from jno_amd import _lib
+    from jno_amd.batch import BatchSolver
+
+    sy = _systems([5, 6])
+    bnan = sy[0][3].clone()
+    bnan[7] = float("nan")
+    bs = BatchSolver(tol=1e-10)
+    with pytest.raises(RuntimeError, match="did not solve 1 of 2"):
+        bs([(sy[0][2], bnan), (sy[1][2], sy[1][3])])
+    assert bs.last_status == _lib.JNO_ENONFINITE
+    assert not np.isfinite(bs.last_stats[0]["relres"])
+    assert bs.last_stats[1]["relres"] <= 1e-10
+
+
+def test_batch_and_single_solver_on_one_p2_operator(dev):
+    # the single mixed solver caches a BSR3 layout for a vec-3 operator; the batch (DIA and CSR
+    # items only) keeps its own layout, in either call order, and both match the oracle
+    import jno_amd as J
+    from jno_amd import precond, solve
+    from jno_amd.batch import BatchSolver
+    from jno_amd.elasticity import ElasticitySystem
+
+    n = 2
+    p2, c2 = O.promote_p2(*O.kuhn_box(n))
+    E = np.exp(0.3 * np.random.default_rng(4).standard_normal(c2.shape[0]))
+    Ao, bo, _ = O.elasticity_p2_system(p2, c2, E)
+    uo = O.direct_solve(Ao, bo)
+    from jno_amd.autotune import choose_format
+    from jno_amd.operator import SolverOperator
+
+    s = ElasticitySystem(J.kuhn_box(n))
+    for order in ("single_first", "batch_first"):
+        A, b = s.evaluate({"E": torch.tensor(E, device="cuda")})
+        sv = solve.cg(tol=1e-10, precision="mixed")
+        bs = BatchSolver(tol=1e-10)
+        if order == "single_first":
+            # the single solver's automatic layout as at C5 size, where BSR3 times fastest
+            A._cache[("solver_op", "auto", True)] = SolverOperator(A, fmt="bsr3", lanes=16)
+            u = sv(A, b, M=precond.jacobi())
+            (x,) = bs([(A, b)])
+            assert sv.last_stats["operator"]["format"] == "bsr3", sv.last_stats
+        else:
+            (x,) = bs([(A, b)])
+            u = sv(A, b, M=precond.jacobi())
+            # the batch's layout did not leak into the single solver's automatic choice
+            assert sv.last_stats["operator"]["format"] == choose_format(A)[0], sv.last_stats
+        assert bs._op(A).format in ("dia", "csr")
+        assert _rel(_np(u), uo) <= 1e-9 and _rel(_np(x), uo) <= 1e-9, order

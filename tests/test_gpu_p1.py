@@ -388,21 +388,18 @@ def test_iteration_modes_and_launch_shapes(dev, precision):
     m, s, A, b = _system(n, k)
     Ao, bo, _ = O.poisson_p1_system(p, t, k)
     uo = O.direct_solve(Ao, bo)
-    saved = solve.get_tuning(precision)
-    try:
-        for fmt in ("dia", "csr"):
-            for mode in (0, 1, 2, 3, 4, 5):
-                for vec in (0, 1, 2, 3):
-                    for blk, cap in ((256, 8), (1024, 64), (512, 4096)):
-                        solve.set_tuning(precision, blk, cap, vec, mode)
-                        sv = solve.cg(tol=1e-10, precision=precision, format=fmt)
-                        u = sv(A, b, M=precond.jacobi())
-                        st = sv.last_stats
-                        # jax semantics stop FP64 on the recursive residual; mixed stops on the true one
-                        assert st["relres"] <= (1e-10 if precision == "mixed" else 2e-10), (fmt, mode, vec, blk, cap, st)
-                        assert _rel(_np(u), uo) <= SOL_TOL, (fmt, mode, vec, blk, cap, st)
-    finally:
-        solve.set_tuning(precision, *saved)
+    base = solve.Tuning.defaults()
+    for fmt in ("dia", "csr"):
+        for mode in (0, 1, 2, 3, 4, 5):
+            for vec in (0, 1, 2, 3):
+                for blk, cap in ((256, 8), (1024, 64), (512, 4096)):
+                    tun = base.with_shape(precision, blk, cap, vec, mode)
+                    sv = solve.cg(tol=1e-10, precision=precision, format=fmt, tuning=tun)
+                    u = sv(A, b, M=precond.jacobi())
+                    st = sv.last_stats
+                    # jax semantics stop FP64 on the recursive residual; mixed stops on the true one
+                    assert st["relres"] <= (1e-10 if precision == "mixed" else 2e-10), (fmt, mode, vec, blk, cap, st)
+                    assert _rel(_np(u), uo) <= SOL_TOL, (fmt, mode, vec, blk, cap, st)
 
 
 @pytest.mark.parametrize("dims,hi,ndiag", [((5, 9, 40), (5 / 8, 9 / 8, 5.0), 4),
@@ -434,19 +431,15 @@ def test_one_kernel_non_cubic_boxes(dev, dims, hi, ndiag):
     A, b = J.fem_poisson(m).evaluate({"k": torch.as_tensor(k, device="cuda")})
     Ao, bo, _ = O.poisson_p1_system(p, t, k)
     uo = O.direct_solve(Ao, bo)
-    saved = solve.get_tuning("mixed")
-    try:
-        for shape in (None, (512, 512, 0, 5), (256, 768, 1, 5), (256, 768, 2, 5), (512, 64, 2, 5)):
-            if shape is not None:
-                solve.set_tuning("mixed", *shape)
-            sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=0.01)
-            u = sv(A, b, M=precond.jacobi())
-            st = sv.last_stats
-            assert st["operator"]["format"] == "dia" and st["operator"]["ndiag"] == ndiag, st
-            assert st["relres"] <= 1e-10, (dims, shape, st)
-            assert _rel(_np(u), uo) <= SOL_TOL, (dims, shape, st)
-    finally:
-        solve.set_tuning("mixed", *saved)
+    base = solve.Tuning.defaults()
+    for shape in (None, (512, 512, 0, 5), (256, 768, 1, 5), (256, 768, 2, 5), (512, 64, 2, 5)):
+        tun = base.with_shape("mixed", *shape) if shape is not None else None
+        sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=0.01, tuning=tun)
+        u = sv(A, b, M=precond.jacobi())
+        st = sv.last_stats
+        assert st["operator"]["format"] == "dia" and st["operator"]["ndiag"] == ndiag, st
+        assert st["relres"] <= 1e-10, (dims, shape, st)
+        assert _rel(_np(u), uo) <= SOL_TOL, (dims, shape, st)
 
 
 @pytest.mark.parametrize("n", [17, 40])
@@ -461,25 +454,22 @@ def test_lds_staged_k1(dev, n):
     k = O.k_lognormal(6 * n**3, 5)
     m, s, A, b = _system(n, k)
     ref = _np(solve.cg(tol=1e-13, precision="fp64")(A, b, M=precond.jacobi()))
-    saved = solve.get_tuning("mixed")
+    base = solve.Tuning.defaults()
     its = {}
-    try:
-        # (3, 4): K1 with p staged in LDS; (0, 5): one kernel per iteration, LDS-staged; (1, 5) and
-        # (2, 5): one kernel per iteration with register-resident 4-row quads, far neighbours loaded
-        # before or after the head (256 and 512 threads; 1024 falls back to the LDS-staged kernel)
-        for vec, mode in ((3, 4), (0, 5), (1, 5), (2, 5)):
-            for blk, cap in ((1024, 8), (256, 64), (1024, 512), (512, 8), (256, 4096)):
-                for rf in (0.1, 0.01):
-                    solve.set_tuning("mixed", blk, cap, vec, mode)
-                    sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=rf)
-                    u = sv(A, b, M=precond.jacobi())
-                    st = sv.last_stats
-                    assert st["relres"] <= 1e-10, (vec, mode, blk, cap, rf, st)
-                    assert _rel(_np(u), ref) <= SOL_TOL, (vec, mode, blk, cap, rf, st)
-                    assert st["refinements"] >= 2, st
-                    its.setdefault((vec, mode, rf), []).append(st["iterations"])
-    finally:
-        solve.set_tuning("mixed", *saved)
+    # (3, 4): K1 with p staged in LDS; (0, 5): one kernel per iteration, LDS-staged; (1, 5) and
+    # (2, 5): one kernel per iteration with register-resident 4-row quads, far neighbours loaded
+    # before or after the head (256 and 512 threads; 1024 falls back to the LDS-staged kernel)
+    for vec, mode in ((3, 4), (0, 5), (1, 5), (2, 5)):
+        for blk, cap in ((1024, 8), (256, 64), (1024, 512), (512, 8), (256, 4096)):
+            for rf in (0.1, 0.01):
+                sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=rf,
+                              tuning=base.with_shape("mixed", blk, cap, vec, mode))
+                u = sv(A, b, M=precond.jacobi())
+                st = sv.last_stats
+                assert st["relres"] <= 1e-10, (vec, mode, blk, cap, rf, st)
+                assert _rel(_np(u), ref) <= SOL_TOL, (vec, mode, blk, cap, rf, st)
+                assert st["refinements"] >= 2, st
+                its.setdefault((vec, mode, rf), []).append(st["iterations"])
     # the one-kernel iteration keeps the search direction across replacements: about the iteration
     # count of the two-kernel one (plus one rerun launch per replacement)
     for rf in (0.1, 0.01):
@@ -601,15 +591,15 @@ def test_autotuner_choices(dev):
     fmt, lanes = autotune.choose_format(Aj, allow=("csr",))
     assert fmt == "csr" and lanes in autotune.CANDIDATE_LANES
     shapes = ((256, 64, 0, 5), (512, 128, 3, 4), (1024, 64, 0, 1))
-    saved = solve.get_tuning("mixed")
-    try:
-        best = autotune.tune_launch(A, b, precision="mixed", iters=40, shapes=shapes)
-        assert best in shapes and solve.get_tuning("mixed") == best
-        u = solve.cg(tol=1e-10, precision="mixed")(A, b, M=precond.jacobi())
-        Ao, bo, _ = O.poisson_p1_system(p, t, k)
-        assert _rel(_np(u), O.direct_solve(Ao, bo)) <= SOL_TOL
-    finally:
-        solve.set_tuning("mixed", *saved)
+    defaults = solve.get_tuning("mixed")
+    best = autotune.tune_launch(A, b, precision="mixed", iters=40, shapes=shapes)
+    # the shape is attached to this operator, not installed process-wide
+    assert best in shapes and solve.get_tuning("mixed") == defaults
+    op = solve.cg(tol=1e-10, precision="mixed").operator(A)
+    assert op.tuning is not None and op.tuning.mixed == best
+    u = solve.cg(tol=1e-10, precision="mixed")(A, b, M=precond.jacobi())
+    Ao, bo, _ = O.poisson_p1_system(p, t, k)
+    assert _rel(_np(u), O.direct_solve(Ao, bo)) <= SOL_TOL
     systems = [_system(nn, O.k_lognormal(6 * nn**3, 60 + nn))[2:] for nn in (5, 7, 6)]
     c = autotune.tune_batch(systems, chunks=(2, 8), tol=1e-10)
     assert c in (2, 8)
@@ -666,3 +656,72 @@ def test_p_eager_bcoo_golden_64_and_nonzero_g(dev):
     A2, b2 = J.fem_poisson(m, f=f, g=0.25).evaluate({"k": k})
     x2 = J.solve.cg(tol=1e-10)(A2, b2, M=J.precond.jacobi())
     assert _rel(_np(x), _np(x2)) < 1e-9
+
+
+def test_dia_checked_fill_rejects_a_missing_lower_twin(dev):
+    # a structurally unsymmetric CSR: the upper entry (0, 1) is stored, its lower twin (1, 0) is not;
+    # the DIA layout would read A[1, 0] through symmetry, so the checked fill must refuse it
+    import ctypes as C
+
+    from jno_amd import _lib
+
+    n = 4
+    rowptr = torch.tensor([0, 2, 3, 5, 7], dtype=torch.int32, device="cuda")
+    col = torch.tensor([0, 1, 1, 2, 3, 2, 3], dtype=torch.int32, device="cuda")
+    vals = torch.tensor([2.0, -1.0, 2.0, 2.0, -1.0, -1.0, 2.0], dtype=torch.float64, device="cuda")
+    ld = 64
+    d64 = torch.empty(2 * ld, dtype=torch.float64, device="cuda")
+    ws = torch.empty(256, dtype=torch.uint8, device="cuda")
+    offs = (C.c_int32 * _lib.JNO_DIA_MAX)(0, 1)
+    ok = C.c_int(1)
+    _lib.call("jno_op_fill_dia_checked", n, _lib.ptr(rowptr), _lib.ptr(col), _lib.ptr(vals), 2, offs, _lib.ptr(d64),
+              None, C.byref(ok), _lib.ptr(ws), 256, _lib.stream_ptr())
+    assert ok.value == 0
+    # with the twin present (and equal) the same offsets are accepted
+    rowptr2 = torch.tensor([0, 2, 4, 6, 8], dtype=torch.int32, device="cuda")
+    col2 = torch.tensor([0, 1, 0, 1, 2, 3, 2, 3], dtype=torch.int32, device="cuda")
+    vals2 = torch.tensor([2.0, -1.0, -1.0, 2.0, 2.0, -1.0, -1.0, 2.0], dtype=torch.float64, device="cuda")
+    _lib.call("jno_op_fill_dia_checked", n, _lib.ptr(rowptr2), _lib.ptr(col2), _lib.ptr(vals2), 2, offs,
+              _lib.ptr(d64), None, C.byref(ok), _lib.ptr(ws), 256, _lib.stream_ptr())
+    assert ok.value == 1
+
+
+def test_concurrent_solves_with_per_call_tuning(dev):
+    """Two threads solve different systems at the same time, each on its own stream with its own
+    launch shape (per-call tuning, no process-wide state): every result equals, bit for bit, the
+    same solve run alone, and the library defaults are untouched."""
+    import threading
+
+    from jno_amd import precond, solve
+
+    cases = []
+    for n, seed, prec, shape in ((12, 21, "mixed", (512, 512, 0, 5)), (9, 22, "mixed", (256, 768, 2, 5)),
+                                 (10, 23, "fp64", (1024, 64, 0, 1)), (8, 24, "fp64", (256, 128, 0, 0))):
+        m, s, A, b = _system(n, O.k_lognormal(6 * n**3, seed))
+        sv = solve.cg(tol=1e-10, precision=prec, tuning=solve.Tuning.defaults().with_shape(prec, *shape))
+        cases.append((A, b, sv))
+    defaults = (solve.get_tuning("fp64"), solve.get_tuning("mixed"))
+    ref = [_np(sv(A, b, M=precond.jacobi())) for A, b, sv in cases]
+    out = [[None] * 3 for _ in cases]
+    errors = []
+
+    def work(i):
+        try:
+            A, b, sv = cases[i]
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                for r in range(3):
+                    out[i][r] = _np(sv(A, b, M=precond.jacobi()))
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append((i, repr(e)))
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(cases))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for i, r in enumerate(ref):
+        for x in out[i]:
+            assert np.array_equal(x, r), i
+    assert (solve.get_tuning("fp64"), solve.get_tuning("mixed")) == defaults

@@ -150,19 +150,16 @@ def test_bsr3_solves_every_iteration_mode(dev, precision):
     A, b = s.evaluate({"E": torch.tensor(E, device="cuda")})
     Ao, bo, d = O.elasticity_p2_system(p2, c2, E)
     uo = O.direct_solve(Ao, bo)
-    saved = solve.get_tuning(precision)
-    try:
-        for mode in (0, 1, 2, 3, 4, 5):
-            for blk, cap in ((256, 8), (1024, 512)):
-                solve.set_tuning(precision, blk, cap, 0, mode)
-                sv = solve.cg(tol=1e-10, precision=precision, format="bsr3")
-                u = sv(A, b, M=precond.jacobi())
-                st = sv.last_stats
-                assert st["operator"]["format"] == "bsr3"
-                assert st["relres"] <= (1e-10 if precision == "mixed" else 2e-10), (mode, blk, cap, st)
-                assert _rel(_np(u), uo) <= 1e-9, (mode, blk, cap, st)
-    finally:
-        solve.set_tuning(precision, *saved)
+    base = solve.Tuning.defaults()
+    for mode in (0, 1, 2, 3, 4, 5):
+        for blk, cap in ((256, 8), (1024, 512)):
+            sv = solve.cg(tol=1e-10, precision=precision, format="bsr3",
+                          tuning=base.with_shape(precision, blk, cap, 0, mode))
+            u = sv(A, b, M=precond.jacobi())
+            st = sv.last_stats
+            assert st["operator"]["format"] == "bsr3"
+            assert st["relres"] <= (1e-10 if precision == "mixed" else 2e-10), (mode, blk, cap, st)
+            assert _rel(_np(u), uo) <= 1e-9, (mode, blk, cap, st)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -246,3 +243,53 @@ def test_c5_32_elasticity(dev):
     C = float(torch.dot(b, u))
     assert abs(float(torch.dot(u, A.mv(u))) - C) <= 1e-8 * abs(C)
     assert abs(float(torch.dot(E, g)) + C) <= 1e-8 * abs(C)
+
+
+# quadratic fields on the face x = 1 and their closed-form integrals over [0,1]^2 in (y, z)
+_FACE_FIELDS = (
+    (lambda x, y, z: 1.0 + 0 * x, 1.0),
+    (lambda x, y, z: z, 0.5),
+    (lambda x, y, z: y * y, 1.0 / 3.0),
+    (lambda x, y, z: y * z, 0.25),
+    (lambda x, y, z: x * z * z - 2.0 * y + 3.0, 1.0 / 3.0 - 1.0 + 3.0),
+)
+
+
+@pytest.mark.parametrize("n", [3, 32])
+def test_traction_load_integrates_quadratics_exactly(dev, n):
+    # P2 interpolates every quadratic exactly, so the consistent load satisfies
+    # sum_i b_i v(x_i) = t int_{x=1} v dS; a vertex/edge swap or a lumped split keeps the total force
+    # and fails this (surface term, jno/utils/solver/fem_native.py:1882-1960)
+    import jno_amd as J
+    from jno_amd.elasticity import ElasticitySystem
+
+    t = (0.3, -0.7, 1.1)
+    s = ElasticitySystem(J.kuhn_box(n), traction=t)
+    pts = _np(s.mesh.points)
+    b = _np(s.traction_load()).reshape(-1, 3)
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    for fn, exact in _FACE_FIELDS:
+        assert np.abs(b.T @ fn(x, y, z) - np.asarray(t) * exact).max() <= 1e-13
+    if n == 3:  # and node by node against the oracle's load
+        p2, c2 = O.promote_p2(*O.kuhn_box(n))
+        tris = O.plane_facets_p2(p2, c2, 0, 1.0)
+        contrib = O.facet_traction_p2(p2, tris, t)
+        bo = np.zeros((p2.shape[0], 3))
+        for c in range(3):
+            np.add.at(bo[:, c], tris.reshape(-1), contrib[:, :, c].reshape(-1))
+        assert np.abs(b - bo).max() <= 1e-15
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_vector_traction_recovers_manufactured_displacement(dev, n):
+    # tests/test_fem_3d.py:364-382 re-expressed for the P2 elasticity form: nu = 0, E = 1, clamp at
+    # x = 0 and t = (1, 0, 0) on x = 1 give u = (x, 0, 0) exactly (sigma = e_x e_x)
+    import jno_amd as J
+    from jno_amd import precond, solve
+    from jno_amd.elasticity import ElasticitySystem
+
+    s = ElasticitySystem(J.kuhn_box(n), nu=0.0, traction=(1.0, 0.0, 0.0))
+    A, b = s.evaluate({"E": torch.ones(s.mesh.n_cells, dtype=torch.float64, device="cuda")})
+    u = _np(solve.cg(tol=1e-13)(A, b, M=precond.jacobi())).reshape(-1, 3)
+    x = _np(s.mesh.points)[:, 0]
+    assert np.abs(u - np.stack([x, 0 * x, 0 * x], 1)).max() <= 1e-10

@@ -109,3 +109,102 @@ def test_p2_pattern_is_node_pattern_times_3x3(n):
     rp1, col1, _, _ = O.pattern_struct(c2, 1)
     assert np.array_equal(np.diff(rp)[::3], 3 * np.diff(rp1))
     assert np.array_equal(col[rp[0]: rp[1]] // 3, np.repeat(col1[rp1[0]: rp1[1]], 3))
+
+
+# ---------------------------------------------------------------------------------------------
+# The traction load, pinned independently of the edge-midpoint rule it is written with: every
+# quadratic field v is interpolated exactly by P2, so sum_i b_i v(x_i) = t . int_F v dS must hold
+# for the consistent load (surface term, jno/utils/solver/fem_native.py:1882-1960).
+# ---------------------------------------------------------------------------------------------
+
+# quadratic fields on the face x = 1 and their closed-form integrals over [0,1]^2 in (y, z)
+_FACE_FIELDS = (
+    (lambda x, y, z: 1.0 + 0 * x, 1.0),
+    (lambda x, y, z: z, 0.5),
+    (lambda x, y, z: y * y, 1.0 / 3.0),
+    (lambda x, y, z: y * z, 0.25),
+    (lambda x, y, z: x * z * z - 2.0 * y + 3.0, 1.0 / 3.0 - 1.0 + 3.0),
+)
+
+
+def _face_load(p2, c2, t, weights=None):
+    """Nodal load of the traction t on the face x = 1; ``weights`` (6,) replaces the oracle's
+    per-facet split (fractions of |F|) to plant a mistake."""
+    tris = O.plane_facets_p2(p2, c2, 0, 1.0)
+    contrib = O.facet_traction_p2(p2, tris, t)
+    if weights is not None:
+        p = p2[tris[:, :3]]
+        area = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+        contrib = (area[:, None] * np.asarray(weights)[None, :])[:, :, None] * np.asarray(t, float)[None, None, :]
+    b = np.zeros((p2.shape[0], 3))
+    for c in range(3):
+        np.add.at(b[:, c], tris.reshape(-1), contrib[:, :, c].reshape(-1))
+    return b
+
+
+def _face_identity_error(p2, b, t):
+    worst = 0.0
+    x, y, z = p2[:, 0], p2[:, 1], p2[:, 2]
+    for fn, exact in _FACE_FIELDS:
+        lhs = b.T @ fn(x, y, z)  # (3,): sum_i b_i[c] v(x_i)
+        worst = max(worst, float(np.abs(lhs - np.asarray(t) * exact).max()))
+    return worst
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_traction_load_integrates_quadratics_exactly(n):
+    p2, c2 = _mesh(n)
+    t = (0.3, -0.7, 1.1)
+    assert _face_identity_error(p2, _face_load(p2, c2, t), t) <= 1e-14
+
+
+def test_traction_identity_detects_a_wrong_split():
+    # planted in the oracle's split only: vertex and edge weights swapped, and a lumped |F|/6 per
+    # node; both keep the total force, and both fail the quadratic identity
+    p2, c2 = _mesh(2)
+    t = (0.0, 0.0, -1.0)
+    for w in ((1 / 3, 1 / 3, 1 / 3, 0, 0, 0), (1 / 6,) * 6):
+        b = _face_load(p2, c2, t, weights=w)
+        assert abs(b[:, 2].sum() + 1.0) < 1e-14  # the totals check alone passes
+        assert _face_identity_error(p2, b, t) > 1e-3
+
+
+def test_facet_traction_matches_an_independent_triangle_rule():
+    # one skew 6-node triangle in space: the split integrates every quadratic as the collapsed
+    # (Duffy) Gauss-Legendre rule does, 5 x 5 points, exact to degree 8 on the triangle
+    rng = np.random.default_rng(3)
+    V = rng.standard_normal((3, 3))
+    mids = np.array([0.5 * (V[1] + V[2]), 0.5 * (V[0] + V[2]), 0.5 * (V[0] + V[1])])  # edges (1,2),(0,2),(0,1)
+    pts = np.vstack([V, mids])
+    tri = np.arange(6)[None, :]
+    t = np.array([0.4, -1.3, 2.0])
+    w = O.facet_traction_p2(pts, tri, t)[0]  # (6, 3)
+    g, gw = np.polynomial.legendre.leggauss(5)
+    g, gw = 0.5 * (g + 1), 0.5 * gw
+    area = 0.5 * np.linalg.norm(np.cross(V[1] - V[0], V[2] - V[0]))
+    coeffs = rng.standard_normal((6, 10))
+    for c in coeffs:
+        def v(X):
+            x, y, z = X[..., 0], X[..., 1], X[..., 2]
+            return (c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * x + c[5] * y * y + c[6] * z * z
+                    + c[7] * x * y + c[8] * y * z + c[9] * x * z)
+
+        # Duffy: (s, r) in [0,1]^2 -> barycentric (1-s, s(1-r), s r), Jacobian s
+        S, R = np.meshgrid(g, g, indexing="ij")
+        W = gw[:, None] * gw[None, :] * S
+        X = (1 - S)[..., None] * V[0] + (S * (1 - R))[..., None] * V[1] + (S * R)[..., None] * V[2]
+        exact = 2.0 * area * float((W * v(X)).sum())
+        assert np.allclose(w.T @ v(pts), t * exact, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_vector_traction_recovers_manufactured_displacement(n):
+    # re-expresses tests/test_fem_3d.py:364-382 for the P2 elasticity form: with nu = 0 (lambda = 0)
+    # and E = 1, u = (x, 0, 0) has sigma = eps = e_x e_x, so it is clamped at x = 0, traction-free on
+    # the lateral faces and carries t = (1, 0, 0) on x = 1; P2 contains it, so the discrete solution
+    # equals it when (and only when) the load is the consistent one
+    p2, c2 = _mesh(n)
+    A, b, _ = O.elasticity_p2_system(p2, c2, np.ones(c2.shape[0]), nu=0.0, traction=(1.0, 0.0, 0.0))
+    u = O.direct_solve(A, b).reshape(-1, 3)
+    exact = np.stack([p2[:, 0], 0 * p2[:, 0], 0 * p2[:, 0]], 1)
+    assert np.abs(u - exact).max() <= 1e-12
