@@ -292,6 +292,20 @@ int jno_pcg_mixed(const jno_op* h_op, const double* d_dinv, const double* d_b, d
                   int use_x0, double tol, double atol, int64_t maxiter, double replace_factor,
                   jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream);
 
+/* Allocation-free jno_pcg_mixed.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * Same arguments, kernels and results as jno_pcg_mixed, without the workspace: the solve runs in a
+ * static device scratch of JNO_PCG_SCRATCH_BYTES per GPU (64 MiB: mixed solves on DIA operators of
+ * up to about 1.19M rows, e.g. a 96^3 Kuhn lattice, and on CSR/BSR3 operators whose mixed workspace
+ * jno_pcg_workspace_size_op(op, 1) fits), held from the first call on a GPU until the process ends,
+ * and reads the device state back synchronously (no pinned memory, no events).  A larger operator
+ * is rejected with JNO_ECAPACITY before any work is queued.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (threads or streams) race, so serialize them. */
+#define JNO_PCG_SCRATCH_BYTES ((size_t)64 << 20)
+int jno_pcg_mixed_noalloc(const jno_op* h_op, const double* d_dinv, const double* d_b, double* d_x,
+                          int use_x0, double tol, double atol, int64_t maxiter, double replace_factor,
+                          jno_pcg_stats* h_stats, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * A20  batched solves.  Replaces jax.vmap over the cg slot (LinearSolver traits vmap="native",
  * jno/utils/solver/solver_api.py:275-280; tests/test_fem_solver_slots.py:123-132), generalised to

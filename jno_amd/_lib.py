@@ -122,6 +122,10 @@ SIGNATURES = {
         C.c_int,
         [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, D, C.POINTER(JnoPcgStats), P, SZ, P],
     ),
+    "jno_pcg_mixed_noalloc": (
+        C.c_int,
+        [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, D, C.POINTER(JnoPcgStats), P],
+    ),
     "jno_adjoint_grad_k_p1": (C.c_int, [P, P, I64, P, P, P, P, P]),
     "jno_batch_workspace_size": (SZ, [C.c_int, C.POINTER(JnoBatchItem)]),
     "jno_batch_pcg_mixed": (
