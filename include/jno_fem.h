@@ -148,7 +148,7 @@ int jno_drop_zeros_fill(int64_t n_rows, const int32_t* d_rowptr, const int32_t* 
  * jno_op_analyze inspects a CSR and reports whether DIA applies; jno_op_fill_dia lays it out;
  * jno_op_bsr3_count / jno_op_fill_bsr3 do the same for BSR3.
  * ------------------------------------------------------------------------------------------- */
-enum { JNO_FMT_CSR = 0, JNO_FMT_DIA = 1, JNO_FMT_BSR3 = 2 };
+enum { JNO_FMT_CSR = 0, JNO_FMT_DIA = 1, JNO_FMT_BSR3 = 2, JNO_FMT_EBE = 3 };
 #define JNO_DIA_MAX 16
 
 /* Launch tuning of the PCG iteration kernels, carried per call (no reference counterpart: these
@@ -169,6 +169,40 @@ typedef struct jno_tuning {
   int32_t pacing;  /* 1: paced chunks for the one-kernel FP32 inner loops, 0: fixed chunks */
 } jno_tuning;
 
+/* Element-by-element (matrix-free) operator of a P2 vector (3 dofs per node) element form: the
+ * analogue of the reference's ElementOperator (jno/utils/solver/element_operator.py:1-178), which
+ * keeps one dense (n_local, n_local) block per cell plus the connectivity instead of a global sparse
+ * matrix.  Here a block is E_e * K_c with K_c the block of the cell's geometry class (cells whose
+ * vertex differences are bitwise equal share one: the 6 Kuhn tet shapes of a structured box), so
+ * the operator stores a class table, one scale per cell and the connectivity.  y = A x is the
+ * symmetrically eliminated operator: y = P K P x + (I - P) x with P the projector onto the free
+ * dofs.  Element slots are grouped by class in tiles of 16 (one MFMA tile; padding slots hold -1).
+ * All pointers are device memory. */
+typedef struct jno_ebe {
+  int64_t n_cells;              /* T                                                          */
+  int64_t n_nodes;              /* N; the operator has 3 N rows (dof = 3 node + comp)         */
+  int64_t n_tiles;              /* tiles of 16 element slots                                  */
+  int32_t n_classes;            /* geometry classes                                           */
+  int32_t pad;
+  const int32_t* cells;         /* i32[10*16 n_tiles]: each slot's cell nodes (basix P2 order), in
+                                   slot order (node 0 for padding)                             */
+  const int32_t* order;         /* i32[16 n_tiles]: cell of each slot, -1 for padding          */
+  const int32_t* tile_class;    /* i32[n_tiles]: class of each tile                            */
+  const uint32_t* slot_mask;    /* u32[16 n_tiles]: bit d = local dof d of the slot is constrained
+                                   (all bits for padding)                                      */
+  const double* table64;        /* f64[n_classes*1024]: K_c[r][k] (zero past 30) at
+                                   ((r/16)*8 + k/4)*64 + (k%4)*16 + r%16 (MFMA operand order)  */
+  const float* table32;         /* f32 copy (FP32 inner iterations)                            */
+  const double* scale64;        /* f64[16 n_tiles]: each slot's scale E_e (0 for padding)      */
+  const float* scale32;         /* f32[16 n_tiles]                                             */
+  const int32_t* inc_ptr;       /* i32[N+1]: node -> its incidence entries                     */
+  const int32_t* inc;           /* i32[10 T]: slot*10 + local node, ascending cell per node    */
+  const int32_t* qpos;          /* i32[10*16 n_tiles]: incidence position of (slot, local node),
+                                   -1 for padding (the inverse of inc)                         */
+  const uint8_t* dmask;         /* u8[3N]: constrained dofs                                    */
+  void* scratch;                /* f64[30 T]: product buffer of jno_op_spmv_* (nullable)       */
+} jno_ebe;
+
 typedef struct jno_op {
   int32_t format;      /* JNO_FMT_*                                                     */
   int32_t lanes;       /* CSR: lanes per row                                            */
@@ -181,7 +215,38 @@ typedef struct jno_op {
   int32_t ndiag;       /* DIA: stored diagonals (main first, then ascending upper)       */
   int32_t offsets[JNO_DIA_MAX]; /* DIA: stored offsets (0 first)                        */
   const jno_tuning* tuning;     /* nullable: launch tuning of the solves on this operator */
+  const jno_ebe* ebe;           /* JNO_FMT_EBE: the element data (vals64/vals32 = its tables) */
 } jno_op;
+
+/* Building an EBE operator (geometry classes are found on the host from the cells' Jacobians;
+ * the rest runs here).
+ * jno_p2_cell_jacobians: d_jac f64[9 T], the vertex differences x_{v+1} - x_0 of each cell (v-major),
+ *   exactly as the element kernels form them: cells with bitwise-equal rows share an element matrix.
+ * jno_ebe_class_blocks: K_c (E = 1) of the classes' representative cells d_rep[C] on FP64 MFMA, the
+ *   same arithmetic as jno_assemble_p2_elasticity's element matrices, into d_table64[C*1024] (and
+ *   the FP32 copy d_table32, nullable) in the table order of jno_ebe, zero past 30.
+ * jno_ebe_slots: each slot's cell nodes in slot order (d_slot_cells i32[10 n_slots]) and its
+ *   constrained-dof bits (d_dmask u8[3N]) into d_slot_mask.
+ * jno_ebe_scales: each slot's scale from the per-cell d_E (f64[T]) into d_s64 and d_s32 (slot order,
+ *   0 for padding), once per operator value.
+ * jno_ebe_incidence: d_inc_out[q] = d_slot_of[e]*10 + a for each plan incidence entry e*10 + a,
+ *   and its inverse d_qpos[slot*10 + a] = q (i32[10 n_slots], -1 for padding slots).
+ * jno_ebe_diagonal: the exact diagonal of the operator (the reference's ElementOperator.diag,
+ *   element_operator.py:150-157), or with invert = 1 its Jacobi inverse with the |d| > 1e-30 guard
+ *   (jno/precond.py:105-109).
+ * jno_ebe_dirichlet_rhs: b[d] = 0 on the constrained dofs (the elimination's b[d] = g with g = 0). */
+int jno_p2_cell_jacobians(const double* d_points2, const int32_t* d_cells10, int64_t n_cells, double* d_jac,
+                          void* stream);
+int jno_ebe_class_blocks(const double* d_points2, const int32_t* d_cells10, int n_classes, const int32_t* d_rep,
+                         double nu, double* d_table64, float* d_table32, void* stream);
+int jno_ebe_slots(int64_t n_slots, const int32_t* d_order, const int32_t* d_cells10, const uint8_t* d_dmask,
+                  int32_t* d_slot_cells, uint32_t* d_slot_mask, void* stream);
+int jno_ebe_scales(int64_t n_slots, const int32_t* d_order, const double* d_E, double* d_s64, float* d_s32,
+                   void* stream);
+int jno_ebe_incidence(int64_t n_entries, const int32_t* d_inc, const int32_t* d_slot_of, int64_t n_slots,
+                      int32_t* d_inc_out, int32_t* d_qpos, void* stream);
+int jno_ebe_diagonal(const jno_op* h_op, int invert, double* d_out, void* stream);
+int jno_ebe_dirichlet_rhs(int64_t n_rows, const uint8_t* d_dmask, double* d_b, void* stream);
 
 /* Distinct column-row offsets (up to JNO_DIA_MAX upper ones) and symmetry of a CSR.
  * *h_ndiag = number of distinct offsets >= 0 (0 is always counted), or -1 when there are more than
@@ -214,7 +279,7 @@ int jno_op_fill_bsr3(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_c
                      void* stream);
 /* FP32 copy of CSR values. */
 int jno_cast_f32(int64_t n, const double* d_in, float* d_out, void* stream);
-/* y = A x with a solver operator, FP64 values. */
+/* y = A x with a solver operator, FP64 values (an EBE operator needs its ebe->scratch). */
 int jno_op_spmv_f64(const jno_op* h_op, const double* d_x, double* d_y, void* stream);
 int jno_op_spmv_f32(const jno_op* h_op, const float* d_x, float* d_y, void* stream);
 

@@ -24,6 +24,7 @@ JNO_ECAPACITY = -6
 JNO_FMT_CSR = 0
 JNO_FMT_DIA = 1
 JNO_FMT_BSR3 = 2
+JNO_FMT_EBE = 3
 JNO_DIA_MAX = 16
 
 
@@ -33,6 +34,29 @@ class JnoLaunchShape(C.Structure):
 
 class JnoTuning(C.Structure):
     _fields_ = [("iter", JnoLaunchShape * 2), ("chunk", C.c_int32), ("pacing", C.c_int32)]
+
+
+class JnoEbe(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int64),
+        ("n_nodes", C.c_int64),
+        ("n_tiles", C.c_int64),
+        ("n_classes", C.c_int32),
+        ("pad", C.c_int32),
+        ("cells", C.c_void_p),
+        ("order", C.c_void_p),
+        ("tile_class", C.c_void_p),
+        ("slot_mask", C.c_void_p),
+        ("table64", C.c_void_p),
+        ("table32", C.c_void_p),
+        ("scale64", C.c_void_p),
+        ("scale32", C.c_void_p),
+        ("inc_ptr", C.c_void_p),
+        ("inc", C.c_void_p),
+        ("qpos", C.c_void_p),
+        ("dmask", C.c_void_p),
+        ("scratch", C.c_void_p),
+    ]
 
 
 class JnoOp(C.Structure):
@@ -48,6 +72,7 @@ class JnoOp(C.Structure):
         ("ndiag", C.c_int32),
         ("offsets", C.c_int32 * JNO_DIA_MAX),
         ("tuning", C.c_void_p),  # const jno_tuning* (NULL: the library defaults)
+        ("ebe", C.c_void_p),  # const jno_ebe* (JNO_FMT_EBE)
     ]
 
 
@@ -105,6 +130,13 @@ SIGNATURES = {
     "jno_op_bsr3_count": (C.c_int, [I64, P, P, C.POINTER(I64), P, SZ, P]),
     "jno_op_fill_bsr3": (C.c_int, [I64, P, P, P, P, P, P, P, P]),
     "jno_cast_f32": (C.c_int, [I64, P, P, P]),
+    "jno_p2_cell_jacobians": (C.c_int, [P, P, I64, P, P]),
+    "jno_ebe_class_blocks": (C.c_int, [P, P, C.c_int, P, D, P, P, P]),
+    "jno_ebe_slots": (C.c_int, [I64, P, P, P, P, P, P]),
+    "jno_ebe_scales": (C.c_int, [I64, P, P, P, P, P]),
+    "jno_ebe_incidence": (C.c_int, [I64, P, P, I64, P, P, P]),
+    "jno_ebe_diagonal": (C.c_int, [C.POINTER(JnoOp), C.c_int, P, P]),
+    "jno_ebe_dirichlet_rhs": (C.c_int, [I64, P, P, P]),
     "jno_op_spmv_f64": (C.c_int, [C.POINTER(JnoOp), P, P, P]),
     "jno_op_spmv_f32": (C.c_int, [C.POINTER(JnoOp), P, P, P]),
     "jno_pcg_workspace_size": (SZ, [I64, C.c_int]),

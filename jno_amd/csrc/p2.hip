@@ -239,21 +239,10 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------------------------
 constexpr int EL_WAVES = 4;
 
-__global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
-    const double* __restrict__ pts, const int32_t* __restrict__ cells10, const int32_t* __restrict__ order,
-    int32_t n_el, const double* __restrict__ E, double lam, double mu, const int32_t* __restrict__ rowptr,
-    const int32_t* __restrict__ col, double* __restrict__ vals) {
-  __shared__ double sK[EL_WAVES][32][33];
-  __shared__ double sS[EL_WAVES][16][17];
-  const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int idx = blockIdx.x * EL_WAVES + wid;
-  if (idx >= n_el) return;  // wave-uniform
-  const int e = order ? order[idx] : idx;
-  const int32_t* c = cells10 + (int64_t)e * 10;
-  const Geo g = geo_of(pts, c);
+// The MFMA tiles of one element, computed by one wavefront: sK = X^T X (30x30, padded 32x32) and
+// sS = H^T H (10x10, padded 16x16); the caller's wavefront reads them after a wave barrier.
+__device__ __forceinline__ void p2_elem_tiles(const Geo& g, int lane, double (*sK)[33], double (*sS)[17]) {
   const int r16 = lane % 16, k4 = lane / 16;
-  // all four quad weights are 1/24, so c_q = w |det J| E_e is one scalar per element
-  const double cq = QW * g.adet * E[e];
   // ---- K1 = X^T X: four 16x16 blocks, K = 4 quad points
   double xa[2];
 #pragma unroll
@@ -268,7 +257,7 @@ __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
       dbl4 acc = {0.0, 0.0, 0.0, 0.0};
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[bi], xa[bj], acc, 0, 0, 0);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) sK[wid][bi * 16 + 4 * t + k4][bj * 16 + r16] = acc[t];
+      for (int t = 0; t < 4; ++t) sK[bi * 16 + 4 * t + k4][bj * 16 + r16] = acc[t];
     }
   // ---- S = H^T H: 10x10 (padded 16x16), K = 12 = (q, k) pairs in three steps of 4
   {
@@ -280,11 +269,37 @@ __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(gv, gv, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) sS[wid][4 * t + k4][r16] = acc[t];
+    for (int t = 0; t < 4; ++t) sS[4 * t + k4][r16] = acc[t];
   }
-  // the LDS tiles written above are read by other lanes of this wavefront below
+  // the LDS tiles written above are read by other lanes of this wavefront
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   __builtin_amdgcn_wave_barrier();
+}
+
+// entry ((a,i),(b,j)) of the element matrix before the factor c_q, rounded term by term (bitwise
+// symmetric)
+__device__ __forceinline__ double p2_kv(const double (*sK)[33], const double (*sS)[17], double lam, double mu, int a,
+                                        int i, int b, int j) {
+  double kv = __dadd_rn(__dmul_rn(lam, sK[3 * a + i][3 * b + j]), __dmul_rn(mu, sK[3 * a + j][3 * b + i]));
+  if (i == j) kv = __dadd_rn(kv, __dmul_rn(mu, sS[a][b]));
+  return kv;
+}
+
+__global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
+    const double* __restrict__ pts, const int32_t* __restrict__ cells10, const int32_t* __restrict__ order,
+    int32_t n_el, const double* __restrict__ E, double lam, double mu, const int32_t* __restrict__ rowptr,
+    const int32_t* __restrict__ col, double* __restrict__ vals) {
+  __shared__ double sK[EL_WAVES][32][33];
+  __shared__ double sS[EL_WAVES][16][17];
+  const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int idx = blockIdx.x * EL_WAVES + wid;
+  if (idx >= n_el) return;  // wave-uniform
+  const int e = order ? order[idx] : idx;
+  const int32_t* c = cells10 + (int64_t)e * 10;
+  const Geo g = geo_of(pts, c);
+  // all four quad weights are 1/24, so c_q = w |det J| E_e is one scalar per element
+  const double cq = QW * g.adet * E[e];
+  p2_elem_tiles(g, lane, sK[wid], sS[wid]);
   // ---- colored scatter: 100 node pairs over 64 lanes, 3x3 block each
   for (int pr = lane; pr < 100; pr += 64) {
     const int a = pr / 10, b = pr % 10;
@@ -301,12 +316,50 @@ __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
     for (int i = 0; i < 3; ++i) {
       double* out = vals + rowptr[3 * na + i] + 3 * lo;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double kv = __dadd_rn(__dmul_rn(lam, sK[wid][3 * a + i][3 * b + j]), __dmul_rn(mu, sK[wid][3 * a + j][3 * b + i]));
-        if (i == j) kv = __dadd_rn(kv, __dmul_rn(mu, sS[wid][a][b]));
-        out[j] += __dmul_rn(cq, kv);
-      }
+      for (int j = 0; j < 3; ++j) out[j] += __dmul_rn(cq, p2_kv(sK[wid], sS[wid], lam, mu, a, i, b, j));
     }
+  }
+}
+
+// Dense element matrices (E = 1) of the listed cells, one wavefront per cell, as 32 x 32 values (zero
+// past 30) in the MFMA operand order of the element-by-element operator's SpMV: entry (r, k) at
+// ((r / 16) * 8 + k / 4) * 64 + (k % 4) * 16 + r % 16, so that each of the SpMV's 16 operand loads is
+// one contiguous 64-value run across the wavefront.  Same arithmetic as elast_p2_k's values
+// (c_q = w |det J| with E = 1).
+__global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_block_k(const double* __restrict__ pts,
+                                                                  const int32_t* __restrict__ cells10,
+                                                                  const int32_t* __restrict__ list, int32_t n_list,
+                                                                  double lam, double mu, double* __restrict__ t64,
+                                                                  float* __restrict__ t32) {
+  __shared__ double sK[EL_WAVES][32][33];
+  __shared__ double sS[EL_WAVES][16][17];
+  const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int idx = blockIdx.x * EL_WAVES + wid;
+  if (idx >= n_list) return;  // wave-uniform
+  const int32_t* c = cells10 + (int64_t)list[idx] * 10;
+  const Geo g = geo_of(pts, c);
+  const double cq = QW * g.adet;
+  p2_elem_tiles(g, lane, sK[wid], sS[wid]);
+  for (int w = lane; w < 32 * 32; w += 64) {
+    const int r = w / 32, k = w % 32;
+    double v = 0.0;
+    if (r < 30 && k < 30) v = __dmul_rn(cq, p2_kv(sK[wid], sS[wid], lam, mu, r / 3, r % 3, k / 3, k % 3));
+    const int o = ((r / 16) * 8 + k / 4) * 64 + (k % 4) * 16 + r % 16;
+    t64[(int64_t)idx * 1024 + o] = v;
+    if (t32) t32[(int64_t)idx * 1024 + o] = (float)v;
+  }
+}
+
+// vertex differences x_{v+1} - x_0 of each cell (the only geometry the element matrices depend on)
+__global__ void cell_jacobians_k(const double* __restrict__ pts, const int32_t* __restrict__ cells10, int64_t T,
+                                 double* __restrict__ jac) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < T; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* c = cells10 + e * 10;
+    const double* p0 = pts + 3 * (int64_t)c[0];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) jac[e * 9 + 3 * v + i] = pts[3 * (int64_t)c[v + 1] + i] - p0[i];
   }
 }
 
@@ -543,6 +596,29 @@ extern "C" int jno_compliance_grad_p2(const double* d_points2, const int32_t* d_
   const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
   compliance_grad_p2_k<<<grid_for(n_cells, 256, 8192), 256, 0, as_stream(stream)>>>(d_points2, d_cells10, n_cells, lam,
                                                                                     mu, d_u, d_grad);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_p2_cell_jacobians(const double* d_points2, const int32_t* d_cells10, int64_t n_cells, double* d_jac,
+                                     void* stream) {
+  JNO_REQUIRE(d_points2 && d_cells10 && d_jac, JNO_EINVAL, "jno_p2_cell_jacobians: null pointer");
+  if (n_cells == 0) return JNO_OK;
+  cell_jacobians_k<<<grid_for(n_cells, 256), 256, 0, as_stream(stream)>>>(d_points2, d_cells10, n_cells, d_jac);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_ebe_class_blocks(const double* d_points2, const int32_t* d_cells10, int n_classes,
+                                    const int32_t* d_rep, double nu, double* d_table64, float* d_table32,
+                                    void* stream) {
+  JNO_REQUIRE(nu > -1.0 && nu < 0.5, JNO_EINVAL, "jno_ebe_class_blocks: Poisson ratio must be in (-1, 0.5)");
+  JNO_REQUIRE(d_points2 && d_cells10 && d_rep && d_table64 && n_classes >= 0, JNO_EINVAL,
+              "jno_ebe_class_blocks: null pointer");
+  if (n_classes == 0) return JNO_OK;
+  const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
+  elast_p2_block_k<<<(unsigned)cdiv(n_classes, EL_WAVES), 64 * EL_WAVES, 0, as_stream(stream)>>>(
+      d_points2, d_cells10, d_rep, n_classes, lam, mu, d_table64, d_table32);
   JNO_LAUNCH_CHECK();
   return JNO_OK;
 }

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pcg_common.h"
 #include "sparse.h"
 
 namespace jno {
@@ -74,39 +75,6 @@ struct CallTuning {
 };
 static std::mutex g_def_mu;
 static CallTuning g_def = {{{512, 0, 1024, 1}, {768, 2, 256, 5}}, 16, 1};
-constexpr int N_XCD = 8;
-
-enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
-
-struct PcgState {
-  int32_t done;    // written by K2 (block 0), read by K1
-  int32_t done1;   // written by K1 (block 0), read by K2
-  int32_t status;  // why K1 stopped
-  int32_t pad;
-  int64_t k;       // iterations done (written by K2 block 0)
-  int64_t k_stop;  // iteration at which K1 stopped
-  double rr_stop;     // recursive r.r at the stop
-  double gamma_cur;   // gamma_k for K2 (K1 block 0)
-  double gamma_prev;  // gamma_{k-1}       (K2 block 0)
-  double alpha;       // alpha_{k-1}       (K2 block 0)
-  double atol2;       // stop when r.r <= atol2
-  double rr_replace;  // FP32 inner: replace the residual when r.r <= rr_replace
-  double rr_true;     // ||b - A x||^2 (finalize)
-  double bb;          // b . b (finalize)
-  double rr0;         // r.r after the last residual preparation (finalize)
-  int64_t maxiter;
-  // single-reduction iterations: KA stages k+1, gamma_k, alpha_k here, KB copies them to k,
-  // gamma_prev, alpha (KA reads those, so it cannot write them itself)
-  int64_t k_next;
-  double gamma_next;
-  double alpha_next;
-  // one-kernel iterations: the launch with index k_special starts (special = 1) or resumes after a
-  // residual replacement (2) with alpha = 0; g_hist[k & 1] = the exact r.r read by launch k
-  int64_t k_special;
-  int32_t special;
-  int32_t outer_done;  // scaled mixed solves: the outer step decided to stop (convergence, cap, ...)
-  double g_hist[2];
-};
 
 // partial-sum areas (per workgroup): A = K1's p.q; B = K2's (r.z, r.r); P = prep's (rr_true, b.b)
 struct Partials {
@@ -138,6 +106,7 @@ struct PcgArgs {
   T* x;
   T* s;  // single-reduction iterations: s = A p
   T* u;  // single-reduction iterations: u = dinv r
+  void* ext;  // format-specific scratch of the SpMV (the EBE product buffer)
   // one-kernel iterations: launch index and the other halves of the r and q pairs (r and q are read,
   // r_out and q_out written; the halves swap every launch)
   int64_t kk;
@@ -146,13 +115,6 @@ struct PcgArgs {
   PcgState* st;
   Partials part;
 };
-
-// contiguous row range of this workgroup, XCD-aware (gridDim.x is a multiple of 8 or < 8)
-__device__ __forceinline__ int64_t block_first_row(int64_t rows_per_block) {
-  const int G = gridDim.x, b = blockIdx.x;
-  const int logical = (G % N_XCD == 0) ? (b % N_XCD) * (G / N_XCD) + b / N_XCD : b;
-  return (int64_t)logical * rows_per_block;
-}
 
 // sum of part[0..G) in a fixed order; every thread gets the total
 template <int NT = PCG_NT>
@@ -203,26 +165,6 @@ __device__ __forceinline__ void reduce_in(const double* __restrict__ part, int G
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < M; ++m) out[m] = sm_in[m];
-}
-
-// Workgroup totals of M per-thread values into dst[m*G + blockIdx.x]: each wave reduces, the wave
-// totals go through sm_out[M][NT/64], and wave 0 adds them (a fixed butterfly).
-template <int NT, int M>
-__device__ __forceinline__ void store_out(const double* v, double* __restrict__ dst, int G, double* sm_out) {
-  const int w = threadIdx.x >> 6;
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    const double s = wave_sum(v[m]);
-    if ((threadIdx.x & 63) == 0) sm_out[m * (NT / 64) + w] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < 64) {
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const double t = wave_sum(threadIdx.x < NT / 64 ? sm_out[m * (NT / 64) + threadIdx.x] : 0.0);
-      if (threadIdx.x == 0) dst[(int64_t)m * G + blockIdx.x] = t;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1878,11 +1820,13 @@ struct Ws {
   void* q2;      // one-kernel iterations: the second q buffer (scaled FP32 operator only)
   float* dinv32; // mixed
   double* ax;    // FP64 A x
+  void* ext;     // the SpMV's own scratch (EBE product buffer; nullable)
   float* sdia;   // mixed on DIA: the Jacobi-scaled FP32 operator (nullable: no room)
 };
 
-// sdia_elems: FP32 entries of the Jacobi-scaled operator (0: none)
-static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_elems = 0) {
+// sdia_elems: FP32 entries of the Jacobi-scaled operator (0: none); ext_bytes: the SpMV's own
+// scratch (an EBE operator's product buffer)
+static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_elems = 0, size_t ext_bytes = 0) {
   const size_t vb = mixed ? 4 : 8;
   size_t off = 0;
   auto take = [&](size_t bytes) -> void* {
@@ -1912,6 +1856,7 @@ static size_t ws_layout(int64_t n, int mixed, char* base, Ws* w, int64_t sdia_el
   t.xs = mixed ? takev(n * 4) : nullptr;
   t.dinv32 = mixed ? (float*)take(n * 4) : nullptr;
   t.ax = (double*)take(n * 8);
+  t.ext = ext_bytes > 0 ? take(ext_bytes) : nullptr;
   t.sdia = sdia_elems > 0 ? (float*)take(sdia_elems * 4) : nullptr;
   t.q2 = sdia_elems > 0 ? takev(n * 4) : nullptr;
   if (w) *w = t;
@@ -1969,9 +1914,9 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
     *rpb = nt * (cdiv(cdiv(r, nt), (int64_t)4) * 4);
     return;
   }
-  const int64_t lane_rows = op->format == JNO_FMT_DIA    ? n
-                            : op->format == JNO_FMT_BSR3 ? (n / 3) * op->lanes
-                                                         : n * op->lanes;
+  const int64_t lane_rows = op->format == JNO_FMT_DIA || op->format == JNO_FMT_EBE ? n
+                            : op->format == JNO_FMT_BSR3                          ? (n / 3) * op->lanes
+                                                                                  : n * op->lanes;
   int64_t want = cdiv(lane_rows, (int64_t)tu.block * 2);
   if (want < 1) want = 1;
   if (want > tu.grid_cap) want = tu.grid_cap;
@@ -1979,7 +1924,7 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
   *G = (int)want;
   // a multiple of 48 for the one-kernel pcg_one, so that 1 to 4 equal tiles divide it (its tiles
   // are then dealt round-robin within the XCD slabs)
-  const int64_t mult = op->format == JNO_FMT_BSR3 ? 12 : tu.split == MODE_ONE ? 48 : 4;
+  const int64_t mult = op->format == JNO_FMT_BSR3 || op->format == JNO_FMT_EBE ? 12 : tu.split == MODE_ONE ? 48 : 4;
   *rpb = cdiv(cdiv(n, want), mult) * mult;
 }
 
@@ -2039,6 +1984,13 @@ struct Poller {
 // lattice) and CSR lanes per row
 template <int NT, int MODE, typename T, typename V>
 static int launch_spmv_kernel(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipStream_t s) {
+  if (op->format == JNO_FMT_EBE) {
+    JNO_REQUIRE(MODE != MODE_FUSED, JNO_EINVAL, "pcg: EBE operators run the split or single-reduction iteration");
+    PcgStateView sv;
+    sv.st = a.st;
+    return ebe_spmv_pcg<T>(op, a.p_new, a.q, static_cast<T*>(a.ext), a.part.A, sv, grid, a.rows_per_block, NT,
+                           MODE == MODE_SINGLE, s);
+  }
   if (op->format == JNO_FMT_BSR3) {
     JNO_REQUIRE(MODE != MODE_FUSED, JNO_EINVAL, "pcg: BSR3 operators run the split or single-reduction iteration");
     constexpr bool CG = MODE == MODE_SINGLE;
@@ -2460,11 +2412,14 @@ static int finish_status(const PcgState& f, jno_pcg_stats* stats, int64_t iters)
 
 static int check_op(const jno_op* op, int need32) {
   JNO_REQUIRE(op, JNO_EINVAL, "pcg: null operator");
-  JNO_REQUIRE(op->format == JNO_FMT_CSR || op->format == JNO_FMT_DIA || op->format == JNO_FMT_BSR3, JNO_EINVAL,
-              "pcg: unknown operator format");
+  JNO_REQUIRE(op->format == JNO_FMT_CSR || op->format == JNO_FMT_DIA || op->format == JNO_FMT_BSR3 ||
+                  op->format == JNO_FMT_EBE,
+              JNO_EINVAL, "pcg: unknown operator format");
   JNO_REQUIRE(op->vals64, JNO_EINVAL, "pcg: operator has no FP64 values");
   JNO_REQUIRE(!need32 || op->vals32, JNO_EINVAL, "pcg: mixed precision needs the operator's FP32 values");
-  if (op->format != JNO_FMT_DIA) JNO_REQUIRE(op->rowptr && op->col, JNO_EINVAL, "pcg: CSR operator without indices");
+  if (op->format == JNO_FMT_EBE) JNO_REQUIRE(op->ebe && op->n == 3 * op->ebe->n_nodes, JNO_EINVAL, "pcg: bad EBE operator");
+  if (op->format == JNO_FMT_CSR || op->format == JNO_FMT_BSR3)
+    JNO_REQUIRE(op->rowptr && op->col, JNO_EINVAL, "pcg: CSR operator without indices");
   if (op->format == JNO_FMT_BSR3) JNO_REQUIRE(op->n % 3 == 0, JNO_EINVAL, "pcg: a BSR3 operator has 3k rows");
   if (op->format == JNO_FMT_DIA)
     JNO_REQUIRE(op->ndiag >= 1 && op->ndiag <= JNO_DIA_MAX && op->offsets[0] == 0, JNO_EINVAL,
@@ -2489,7 +2444,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   int mode = tu.split == MODE_SCALED                                 ? MODE_SPLIT
              : (tu.split == MODE_SCALED_FUSED || tu.split == MODE_ONE) ? MODE_FUSED
                                                                        : tu.split;
-  if (op->format == JNO_FMT_BSR3 && mode == MODE_FUSED) mode = MODE_SPLIT;
+  if ((op->format == JNO_FMT_BSR3 || op->format == JNO_FMT_EBE) && mode == MODE_FUSED) mode = MODE_SPLIT;
   a.split = mode;
   for (int d = 0; d < JNO_DIA_MAX; ++d) a.off.o[d] = op->offsets[d];
   a.dinv = dinv;
@@ -2503,6 +2458,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.unit = 0;
   a.s = (T*)w.s;
   a.u = (T*)w.p1;  // single-reduction iterations keep p in place in p0; p1 holds u
+  a.ext = w.ext;
   a.kk = 0;
   a.r_out = nullptr;
   a.q_out = nullptr;
@@ -2564,7 +2520,7 @@ extern "C" size_t jno_pcg_workspace_size(int64_t n, int mixed) { return ws_layou
 
 extern "C" size_t jno_pcg_workspace_size_op(const jno_op* h_op, int mixed) {
   if (!h_op) return 0;
-  return ws_layout(h_op->n, mixed ? 1 : 0, nullptr, nullptr, mixed ? scaled_elems(h_op) : 0);
+  return ws_layout(h_op->n, mixed ? 1 : 0, nullptr, nullptr, mixed ? scaled_elems(h_op) : 0, ebe_scratch_bytes(h_op));
 }
 
 extern "C" int jno_pcg_profile(int enable) {
@@ -2641,7 +2597,8 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   if (rc) return rc;
   const int64_t n = h_op->n;
   JNO_REQUIRE(d_dinv && d_b && d_x, JNO_EINVAL, "jno_pcg_f64: null vector");
-  JNO_REQUIRE(ws_bytes >= jno_pcg_workspace_size(n, 0), JNO_EWORKSPACE, "jno_pcg_f64: workspace too small");
+  const size_t ext = ebe_scratch_bytes(h_op);
+  JNO_REQUIRE(ws_bytes >= ws_layout(n, 0, nullptr, nullptr, 0, ext), JNO_EWORKSPACE, "jno_pcg_f64: workspace too small");
   JNO_REQUIRE(maxiter >= 0, JNO_EINVAL, "jno_pcg_f64: maxiter must be >= 0");
   CallTuning ct;
   rc = call_tuning(h_op, &ct, "jno_pcg_f64");
@@ -2649,7 +2606,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   Poller poll;
   hipStream_t s = as_stream(stream);
   Ws w;
-  ws_layout(n, 0, static_cast<char*>(d_ws), &w);
+  ws_layout(n, 0, static_cast<char*>(d_ws), &w, 0, ext);
   const auto t0 = std::chrono::steady_clock::now();
   int G;
   int64_t rpb;
@@ -2663,7 +2620,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   zero_k<double><<<gz, PCG_NT, 0, s>>>((double*)w.p1, n);
   zero_k<double><<<gz, PCG_NT, 0, s>>>((double*)w.s, n);
   if (use_x0) {
-    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
     if (rc) return rc;
   } else {
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
@@ -2682,7 +2639,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
   rc = run_inner<double, double>(poll, h_op, a, pb, 0, ct.chunk, G, s, &f);
   if (rc) return rc;
   // true residual ||b - A x|| / ||b||  (the eager guard of _maybe_residual_check)
-  rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+  rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
   if (rc) return rc;
   residual_prep_k<double><<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, r, w.part, 0, nullptr);
   finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, maxiter, 0.0, 0, 0, single);
@@ -2761,7 +2718,7 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
   }
   int rc;
   if (use_x0) {
-    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
     if (rc) return rc;
   } else {
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
@@ -2797,7 +2754,7 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
     const int64_t kr = one && k > 0 ? k - 1 : k;
     accumulate_scaled_k<<<gz, PCG_NT, 0, s>>>(n, d_x, xs, d_dinv);
     JNO_LAUNCH_CHECK();
-    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
     if (rc) return rc;
     residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, w.ax, d_dinv, one ? rb[kr & 1] : rs, 1,
                                                  w.part, one ? w.part.C + (kr & 1) * 4 * (int64_t)G : nullptr);
@@ -2839,9 +2796,11 @@ int pcg_mixed_impl(const jno_op* h_op, const double* d_dinv, const double* d_b, 
   poll.sync = sync_poll;
   const Tuning tu = effective_tuning(ct.it[1], h_op);
   const int64_t se = scaled_elems(h_op);
-  const bool room = se > 0 && ws_bytes >= ws_layout(n, 1, nullptr, nullptr, se);
+  const size_t ext = ebe_scratch_bytes(h_op);
+  JNO_REQUIRE(ws_bytes >= ws_layout(n, 1, nullptr, nullptr, 0, ext), JNO_EWORKSPACE, "jno_pcg_mixed: workspace too small");
+  const bool room = se > 0 && ws_bytes >= ws_layout(n, 1, nullptr, nullptr, se, ext);
   Ws w;
-  ws_layout(n, 1, static_cast<char*>(d_ws), &w, room ? se : 0);
+  ws_layout(n, 1, static_cast<char*>(d_ws), &w, room ? se : 0, ext);
   int G;
   int64_t rpb;
   pcg_grid(h_op, tu, &G, &rpb);
@@ -2864,7 +2823,7 @@ int pcg_mixed_impl(const jno_op* h_op, const double* d_dinv, const double* d_b, 
   zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
   cast_f32_vec_k<<<gz, PCG_NT, 0, s>>>(n, d_dinv, w.dinv32);
   if (use_x0) {
-    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
     if (rc) return rc;
   } else {
     zero_k<double><<<gz, PCG_NT, 0, s>>>(d_x, n);
@@ -2889,7 +2848,7 @@ int pcg_mixed_impl(const jno_op* h_op, const double* d_dinv, const double* d_b, 
     // FP64 refinement: x += xs, r = b - A x, rs = (float) r
     accumulate_x_k<<<gz, PCG_NT, 0, s>>>(n, d_x, xs);
     JNO_LAUNCH_CHECK();
-    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s);
+    rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
     if (rc) return rc;
     // the single-reduction KA of iteration k reads the r.u / r.r partials of buffer (k odd ? B2 : B)
     Partials pp = w.part;

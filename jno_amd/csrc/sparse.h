@@ -12,8 +12,13 @@ struct DiaOffsets {
 // loads of a diagonal at a 4-aligned row are aligned
 __host__ __device__ __forceinline__ int64_t dia_ld(int64_t n) { return (n + 63) & ~(int64_t)63; }
 
+// y = A x with a solver operator; `ext` is the product buffer an EBE operator needs (nullable: its
+// ebe->scratch)
 template <typename V, typename X>
-int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s);
+int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s, void* ext = nullptr);
+// y = A x with an element-by-element operator (ebe.hip)
+template <typename T>
+int ebe_spmv(const jno_op* op, const T* x, T* y, void* yloc, hipStream_t s);
 
 // U block rows of y = A x in the BSR3 layout (3x3 blocks; the 9 entries of block row R stored
 // entry-major: entry e = 3i+j of block b at 9*browptr[R] + e*len + b, len = blocks in the row), given

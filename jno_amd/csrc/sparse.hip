@@ -1,6 +1,8 @@
 // CSR utilities: SpMV (lanes-per-row CSR and symmetric DIA), Jacobi diagonal, Dirichlet
 // elimination on an assembled CSR, exact-zero compaction, and the operator analysis that decides
 // whether the index-free symmetric diagonal layout applies.
+#include <type_traits>
+
 #include "common.h"
 #include "sparse.h"
 
@@ -110,10 +112,17 @@ __global__ void __launch_bounds__(256) dia_spmv_k(int64_t n, int ndiag, DiaOffse
 }
 
 template <typename V, typename X>
-int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s) {
+int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s, void* ext) {
   JNO_REQUIRE(op && vals && x && y, JNO_EINVAL, "spmv: null pointer (is the FP32 copy built?)");
   const int64_t n = op->n;
   if (n == 0) return JNO_OK;
+  if (op->format == JNO_FMT_EBE) {
+    if constexpr (std::is_same<V, X>::value) {
+      return ebe_spmv<X>(op, x, y, ext, s);
+    } else {
+      JNO_REQUIRE(false, JNO_EINVAL, "spmv: an EBE operator computes in its table's precision");
+    }
+  }
   if (op->format == JNO_FMT_DIA) {
     DiaOffsets off;
     for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = op->offsets[d];
@@ -151,8 +160,8 @@ int op_spmv(const jno_op* op, const V* vals, const X* x, X* y, hipStream_t s) {
   return JNO_OK;
 }
 
-template int op_spmv<double, double>(const jno_op*, const double*, const double*, double*, hipStream_t);
-template int op_spmv<float, float>(const jno_op*, const float*, const float*, float*, hipStream_t);
+template int op_spmv<double, double>(const jno_op*, const double*, const double*, double*, hipStream_t, void*);
+template int op_spmv<float, float>(const jno_op*, const float*, const float*, float*, hipStream_t, void*);
 
 // ---------------------------------------------------------------------------------------------
 // Jacobi, Dirichlet, cast

@@ -122,6 +122,24 @@ class ElasticitySystem:
                       _lib.ptr(b), _lib.ptr(self.dmask), None, stream)
         return A, b
 
+    def element_operator(self, args: dict):
+        """(A, b) with A the element-by-element operator (:class:`jno_amd.ebe.ElementOperator`, the
+        reference's ElementOperator, jno/utils/solver/element_operator.py:1-178) instead of the
+        assembled CSR: the same eliminated operator and load, no global matrix."""
+        from .ebe import ElementOperator
+
+        m = self.mesh
+        E = torch.as_tensor(args[self.param], dtype=torch.float64, device=m.device)
+        if E.ndim == 0:
+            E = E.expand(m.n_cells)
+        if E.shape != (m.n_cells,):
+            raise ValueError(f"{self.param} must have one value per cell ({m.n_cells},)")
+        A = ElementOperator(self, E.contiguous())
+        b = self._b0.clone()
+        if self.dmask is not None:
+            _lib.call("jno_ebe_dirichlet_rhs", self.ndof, _lib.ptr(self.dmask), _lib.ptr(b), _lib.stream_ptr(m.device))
+        return A, b
+
     def compliance_grad(self, u: torch.Tensor) -> torch.Tensor:
         """dC/dE_e = -u_e^T (K_e/E_e) u_e for C = b.u."""
         m = self.mesh

@@ -1,10 +1,10 @@
 """A bounded, repeatable workload for profiler runs (rocprofv3 kernel traces and PMC counters).
 
     python -m jno_amd.profile_workload --case c2 --iters 40 [--precision mixed|fp64] [--copy-mib 1024]
-        [--tuning block,grid_cap,k1_vector,mode]
+        [--tuning block,grid_cap,k1_vector,mode] [--format auto|dia|bsr3|csr|ebe]
 
 Builds the configuration's system (128^3 P1 Poisson for c2, 64^3 for c3, 32^3 P2 elasticity for
-c5), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
+c5; --format ebe: its element-by-element operator), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
 finally copies a buffer of --copy-mib MiB with a device-to-device copy kernel: the copy's
 FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels' counters.
 --case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver;
@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 
-def build(case: str, dev):
+def build(case: str, dev, fmt: str = "auto"):
     import jno_amd as J
 
     if case in ("c2", "c3"):
@@ -33,6 +33,8 @@ def build(case: str, dev):
 
         s = ElasticitySystem(J.kuhn_box(32, device=dev))
         E = torch.tensor(np.exp(0.3 * np.random.default_rng(0).standard_normal(s.mesh.n_cells)), device=dev)
+        if fmt == "ebe":  # the element-by-element operator instead of an assembled layout
+            return s.element_operator({"E": E})
         return s.evaluate({"E": E})
     raise ValueError(f"unknown case {case!r}")
 
@@ -127,7 +129,7 @@ def main():
     if args.case == "heat":
         run_heat(dev, args.iters, args.precision, args.n, args.replace_factor)
         return
-    A, b = build(args.case, dev)
+    A, b = build(args.case, dev, args.format)
     # replace_factor tiny: no residual replacement inside the bounded run
     sv = LinearSolver("cg", 1e-30, 0.0, args.iters, precision=args.precision, format=args.format,
                       replace_factor=1e-30, guard=False)

@@ -84,6 +84,8 @@ class LinearSolver:
         return f"jno_amd.solve.{self.name}(tol={self.tol}, precision={self.precision!r})"
 
     def operator(self, A: CSR) -> SolverOperator:
+        if getattr(A, "format", None) == "ebe":  # an element-by-element operator is its own layout
+            return A
         key = ("solver_op", self.format, self.precision == "mixed")
         op = A._cache.get(key)
         if op is None:
@@ -92,9 +94,12 @@ class LinearSolver:
         return op
 
     def __call__(self, A, b, *, M=None, x0=None):
-        if not isinstance(A, CSR):
-            raise TypeError("A must be an assembled jno_amd CSR operator (FemLinearSystem.evaluate)")
-        _lib.require_gpu(A.vals, "A")
+        from .ebe import ElementOperator
+
+        if not isinstance(A, (CSR, ElementOperator)):
+            raise TypeError("A must be an assembled jno_amd CSR operator (FemLinearSystem.evaluate) or an "
+                            "ElementOperator")
+        _lib.require_gpu(A.vals if isinstance(A, CSR) else A.E, "A")
         n = A.shape[0]
         b = torch.as_tensor(b).reshape(-1)
         _lib.require_gpu(b, "b")
