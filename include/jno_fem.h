@@ -64,6 +64,15 @@ int jno_pattern_fill(const int32_t* d_cells, int64_t n_cells, int nloc, int vec,
                      const int32_t* d_inc_ptr, const int32_t* d_inc, const int32_t* d_rowptr,
                      int32_t* d_col, void* stream);
 
+/* Content key of an index array (the plan cache keys patterns on the cells' content, as
+ * compress_plan memoizes on a digest of its index array, jno/utils/solver/fem_utils.py:4027-4060):
+ * h_hash[2] = two independent 64-bit sums of a position-dependent mix of every entry; and an
+ * equality check of two arrays (*h_equal = 1 when equal) that guards a cache hit.  d_ws: 256 bytes.
+ * Both synchronize the stream. */
+int jno_hash_i32(const int32_t* d_data, int64_t n, uint64_t* h_hash, void* d_ws, size_t ws_bytes, void* stream);
+int jno_equal_i32(const int32_t* d_a, const int32_t* d_b, int64_t n, int* h_equal, void* d_ws, size_t ws_bytes,
+                  void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * A10 boundary.  Replaces _boundary_facets (jno/_fem.py:2695-2773) + _boundary_node_ids
  * (jno/utils/solver/fem_native.py:2563-2636): d_mask[n] = 1 for every node of a facet used by

@@ -112,6 +112,8 @@ SIGNATURES = {
     "jno_pattern_workspace_size": (SZ, [I64, C.c_int, I64]),
     "jno_pattern_count": (C.c_int, [P, I64, C.c_int, C.c_int, I64, P, P, P, C.POINTER(I64), C.POINTER(I32), P, SZ, P]),
     "jno_pattern_fill": (C.c_int, [P, I64, C.c_int, C.c_int, I64, P, P, P, P, P]),
+    "jno_hash_i32": (C.c_int, [P, I64, C.POINTER(C.c_uint64), P, SZ, P]),
+    "jno_equal_i32": (C.c_int, [P, P, I64, C.POINTER(C.c_int), P, SZ, P]),
     "jno_boundary_mask": (C.c_int, [P, I64, C.c_int, I64, P, P, P, P]),
     "jno_mask_plane": (C.c_int, [P, I64, C.c_int, D, P, P, P]),
     "jno_assemble_p1": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, P]),

@@ -1,10 +1,13 @@
 // A7 CSR sparsity pattern build (the bit-exact index structure) and the node->element incidence
 // that the row-pull assembly reuses.
 //
-//   count:  node degree histogram -> scan -> incidence fill -> per-node (one wavefront each):
-//           rank-sort the incidence list (deterministic assembly order), gather every node of every
-//           incident element into LDS, count distinct ones -> dof-level row counts -> scan = rowptr
+//   count:  node degree histogram -> scan -> incidence fill -> per-node (one wavefront each): sort
+//           the incidence list (deterministic assembly order), gather every node of every incident
+//           element, sort them and count the distinct ones -> dof-level row counts -> scan = rowptr
 //   fill:   per node again: distinct neighbours in ascending order, expanded to dofs n*vec+c.
+// The sorts are bitonic networks in registers across the wavefront for nodes of up to 64 incident
+// entries and 256 candidates; larger nodes (unstructured fans) compare their candidates pairwise in
+// LDS.
 //
 // The result equals np.unique over the (row, col) keys of all element pairs, which is what
 // compress_plan (jno/utils/solver/fem_utils.py:4009-4060) returns.
@@ -69,6 +72,130 @@ __device__ __forceinline__ int mark_first(const int32_t* cand, int m, uint8_t* f
   return wave_sum_int(cnt);
 }
 
+// ---------------------------------------------------------------------------------------------
+// The sort-based path (nodes with at most 64 incident entries and 256 candidates: every node of a
+// P1 or P2 Kuhn lattice): one wavefront per node sorts in registers instead of comparing all pairs.
+// Values are held in a blocked layout, element i = K lane + r in register r of lane `lane`; a
+// bitonic network over 64 K elements (Batcher) sorts them ascending, INT_MAX padding last.
+// ---------------------------------------------------------------------------------------------
+constexpr int PAT_PAD = 0x7fffffff;
+
+template <int K>
+__device__ __forceinline__ void wave_bitonic(int (&v)[K], int lane) {
+  constexpr int M = 64 * K;
+#pragma unroll
+  for (int s = 2; s <= M; s <<= 1) {
+#pragma unroll
+    for (int d = s >> 1; d > 0; d >>= 1) {
+      if (d >= K) {
+        // the partner of element i is i ^ d, in lane lane ^ (d / K), same register
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+          const int i = lane * K + r;
+          const int pv = __shfl_xor(v[r], d / K, 64);
+          const bool up = (i & s) == 0, lower = (i & d) == 0;
+          v[r] = (up == lower) ? min(v[r], pv) : max(v[r], pv);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+          if (r & d) continue;
+          const int i = lane * K + r;
+          const bool up = (i & s) == 0;
+          const int a = v[r], b = v[r | d];
+          v[r] = up ? min(a, b) : max(a, b);
+          v[r | d] = up ? max(a, b) : min(a, b);
+        }
+      }
+    }
+  }
+}
+
+// The node's incidence entries, sorted ascending (assembly sums in ascending element order): lane l
+// holds entry l (PAT_PAD past deg); written back when `store`
+__device__ __forceinline__ int sorted_incidence(int32_t* __restrict__ inc, int beg, int deg, int lane, bool store) {
+  int v[1] = {lane < deg ? inc[beg + lane] : PAT_PAD};
+  if (store) {
+    wave_bitonic<1>(v, lane);
+    if (lane < deg) inc[beg + lane] = v[0];
+  }
+  return v[0];
+}
+
+// Sorted candidate neighbour nodes of a node (every node of every incident element), K per lane;
+// returns the count of distinct ones and sets first[r] for the first occurrence of each
+template <int K>
+__device__ __forceinline__ int sorted_candidates(const int32_t* __restrict__ cells, int nloc, int ent, int deg,
+                                                 int lane, int (&c)[K], bool (&first)[K]) {
+  const int m = deg * nloc;
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int i = lane * K + r;
+    const bool in = i < m;
+    const int q = in ? i / nloc : 0;  // q < deg <= 64 for the candidates; element 0 otherwise, unused
+    const int e = __shfl(ent, q, 64) / nloc;
+    const int32_t cv = cells[(int64_t)(in ? e : 0) * nloc + (in ? i - q * nloc : 0)];
+    c[r] = in ? cv : PAT_PAD;
+  }
+  wave_bitonic<K>(c, lane);
+  const int prev_last = __shfl_up(c[K - 1], 1, 64);
+  int cnt = 0;
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    const int prev = r > 0 ? c[r - 1] : (lane > 0 ? prev_last : -1);
+    first[r] = c[r] != PAT_PAD && c[r] != prev;
+    cnt += first[r] ? 1 : 0;
+  }
+  return cnt;
+}
+
+// exclusive prefix sum over the 64 lanes (every lane gets its own prefix), and the total
+__device__ __forceinline__ int wave_exclusive(int v, int lane, int* total) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+template <int K>
+__device__ __forceinline__ void count_sorted(const int32_t* __restrict__ cells, int nloc, int vec, int64_t node, int ent,
+                                             int deg, int lane, int32_t* __restrict__ rowcnt) {
+  int c[K];
+  bool first[K];
+  const int cnt = sorted_candidates<K>(cells, nloc, ent, deg, lane, c, first);
+  const int uniq = wave_sum_int(cnt);
+  if (lane < vec) rowcnt[node * vec + lane] = uniq * vec;
+}
+
+template <int K>
+__device__ __forceinline__ void fill_sorted(const int32_t* __restrict__ cells, int nloc, int vec, int64_t node, int ent,
+                                            int deg, int lane, const int32_t* __restrict__ rowptr,
+                                            int32_t* __restrict__ col) {
+  int c[K];
+  bool first[K];
+  const int cnt = sorted_candidates<K>(cells, nloc, ent, deg, lane, c, first);
+  int total;
+  int pos = wave_exclusive(cnt, lane, &total);
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    if (!first[r]) continue;
+    for (int cc = 0; cc < vec; ++cc) {
+      int32_t* out = col + rowptr[node * vec + cc] + (int64_t)pos * vec;
+      for (int c2 = 0; c2 < vec; ++c2) out[c2] = c[r] * vec + c2;
+    }
+    ++pos;
+  }
+}
+
+// the candidate registers per lane for m candidates (0: too many for the sort-based path)
+__device__ __forceinline__ int sort_width(int deg, int m) {
+  return deg > 64 ? 0 : m <= 64 ? 1 : m <= 128 ? 2 : m <= 256 ? 4 : 0;
+}
+
 __global__ void __launch_bounds__(PAT_NT) pattern_count_k(const int32_t* __restrict__ cells, int nloc, int vec,
                                                           int64_t n_nodes, const int32_t* __restrict__ inc_ptr,
                                                           int32_t* __restrict__ inc, int32_t* __restrict__ rowcnt,
@@ -83,11 +210,20 @@ __global__ void __launch_bounds__(PAT_NT) pattern_count_k(const int32_t* __restr
   const int64_t nwaves = (int64_t)gridDim.x * (PAT_NT / WAVE);
   for (int64_t node = blockIdx.x * (int64_t)(PAT_NT / WAVE) + wid; node < n_nodes; node += nwaves) {
     const int beg = inc_ptr[node], deg = inc_ptr[node + 1] - beg;
+    const int kw = sort_width(deg, deg * nloc);  // wavefront-uniform
+    if (kw) {
+      const int ent = sorted_incidence(inc, beg, deg, lane, true);
+      if (kw == 1) count_sorted<1>(cells, nloc, vec, node, ent, deg, lane, rowcnt);
+      else if (kw == 2) count_sorted<2>(cells, nloc, vec, node, ent, deg, lane, rowcnt);
+      else count_sorted<4>(cells, nloc, vec, node, ent, deg, lane, rowcnt);
+      continue;
+    }
     if (deg > PAT_MAX_DEG || deg * nloc > PAT_MAX_CAND) {
       if (lane == 0) atomicMax(status, deg);
       continue;
     }
-    // rank-sort the incidence entries (all distinct) so assembly sums in ascending element order
+    // larger nodes: rank-sort the incidence entries (all distinct) so assembly sums in ascending
+    // element order, and compare the candidates pairwise
     for (int i = lane; i < deg; i += WAVE) sinc[i] = inc[beg + i];
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -117,6 +253,15 @@ __global__ void __launch_bounds__(PAT_NT) pattern_fill_k(const int32_t* __restri
   const int64_t nwaves = (int64_t)gridDim.x * (PAT_NT / WAVE);
   for (int64_t node = blockIdx.x * (int64_t)(PAT_NT / WAVE) + wid; node < n_nodes; node += nwaves) {
     const int beg = inc_ptr[node], deg = inc_ptr[node + 1] - beg;
+    const int kw = sort_width(deg, deg * nloc);  // wavefront-uniform
+    if (kw) {
+      // the incidence was sorted by the count phase
+      const int ent = lane < deg ? inc[beg + lane] : PAT_PAD;
+      if (kw == 1) fill_sorted<1>(cells, nloc, vec, node, ent, deg, lane, rowptr, col);
+      else if (kw == 2) fill_sorted<2>(cells, nloc, vec, node, ent, deg, lane, rowptr, col);
+      else fill_sorted<4>(cells, nloc, vec, node, ent, deg, lane, rowptr, col);
+      continue;
+    }
     const int m = gather_candidates(cells, nloc, inc, beg, deg, cand, lane);
     const int uniq = mark_first(cand, m, flag, lane);
     // position of each first occurrence among the distinct values
@@ -143,6 +288,53 @@ __global__ void max_row_k(const int32_t* __restrict__ rowcnt, int64_t n, int32_t
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Content keys of index arrays (the plan cache keys a pattern on its cells' content, as compress_plan
+// memoizes on a digest of its index array, jno/utils/solver/fem_utils.py:4027-4060): two
+// independent 64-bit sums of a position-dependent mix of every entry (splitmix64's finalizer of
+// (i << 32 | v) under two seeds).  Integer sums are exact, so the key does not depend on the order
+// of the reduction; the equality check guards a hit against a collision.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) hash_i32_k(const int32_t* __restrict__ a, int64_t n,
+                                                   unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long sm[2][4];
+  unsigned long long h0 = 0, h1 = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t x = ((uint64_t)i << 32) | (uint32_t)a[i];
+    h0 += mix64(x + 0x9e3779b97f4a7c15ull);
+    h1 += mix64(x ^ 0xd1b54a32d192ed03ull);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    h0 += __shfl_xor(h0, o, 64);
+    h1 += __shfl_xor(h1, o, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sm[0][w] = h0;
+    sm[1][w] = h1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], sm[0][0] + sm[0][1] + sm[0][2] + sm[0][3]);
+    atomicAdd(&out[1], sm[1][0] + sm[1][1] + sm[1][2] + sm[1][3]);
+  }
+}
+
+__global__ void equal_i32_k(const int32_t* __restrict__ a, const int32_t* __restrict__ b, int64_t n,
+                            int* __restrict__ diff) {
+  bool d = false;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d = d || a[i] != b[i];
+  if (d) atomicOr(diff, 1);
 }
 
 // workspace layout: [deg/cursor i32 n_nodes+1][rowcnt i32 ndof+1][status i32 4][total i64 2][scan ws]
@@ -226,5 +418,40 @@ extern "C" int jno_pattern_fill(const int32_t* d_cells, int64_t n_cells, int nlo
   pattern_fill_k<<<grid_for(n_nodes, PAT_NT / WAVE, 16384), PAT_NT, 0, as_stream(stream)>>>(
       d_cells, nloc, vec, n_nodes, d_inc_ptr, d_inc, d_rowptr, d_col);
   JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_hash_i32(const int32_t* d_data, int64_t n, uint64_t* h_hash, void* d_ws, size_t ws_bytes,
+                            void* stream) {
+  JNO_REQUIRE(h_hash && d_ws && (d_data || n == 0) && n >= 0, JNO_EINVAL, "jno_hash_i32: null pointer");
+  JNO_REQUIRE(ws_bytes >= 256, JNO_EWORKSPACE, "jno_hash_i32: workspace too small (256 bytes)");
+  hipStream_t s = as_stream(stream);
+  unsigned long long* acc = static_cast<unsigned long long*>(d_ws);
+  fill_i32<<<1, 64, 0, s>>>(reinterpret_cast<int32_t*>(acc), 4, 0);
+  JNO_LAUNCH_CHECK();
+  if (n > 0) hash_i32_k<<<grid_for(n, 256, 2048), 256, 0, s>>>(d_data, n, acc);
+  JNO_LAUNCH_CHECK();
+  unsigned long long h[2];
+  JNO_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  h_hash[0] = h[0];
+  h_hash[1] = h[1];
+  return JNO_OK;
+}
+
+extern "C" int jno_equal_i32(const int32_t* d_a, const int32_t* d_b, int64_t n, int* h_equal, void* d_ws,
+                             size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(h_equal && d_ws && ((d_a && d_b) || n == 0) && n >= 0, JNO_EINVAL, "jno_equal_i32: null pointer");
+  JNO_REQUIRE(ws_bytes >= 256, JNO_EWORKSPACE, "jno_equal_i32: workspace too small (256 bytes)");
+  hipStream_t s = as_stream(stream);
+  int* diff = static_cast<int*>(d_ws);
+  fill_i32<<<1, 64, 0, s>>>(diff, 1, 0);
+  JNO_LAUNCH_CHECK();
+  if (n > 0) equal_i32_k<<<grid_for(n, 256, 4096), 256, 0, s>>>(d_a, d_b, n, diff);
+  JNO_LAUNCH_CHECK();
+  int h = 1;
+  JNO_HIP(hipMemcpyAsync(&h, diff, sizeof(int), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *h_equal = h ? 0 : 1;
   return JNO_OK;
 }

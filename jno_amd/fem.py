@@ -9,6 +9,8 @@ All work runs in the HIP library; this module only allocates and wires device bu
 from __future__ import annotations
 
 import ctypes as C
+import threading
+from collections import OrderedDict
 from dataclasses import dataclass, field
 from typing import Optional, Union
 
@@ -16,6 +18,61 @@ import torch
 
 from . import _lib
 from .mesh import Mesh
+
+# Plans keyed on the content of the cells (and the sizes and vec), like compress_plan's memo
+# (jno/utils/solver/fem_utils.py:4027-4060): a mesh rebuilt with the same connectivity (C4's many
+# boxes of one size, a re-created mesh) reuses the plan instead of building it again; a remeshed
+# domain changes the content and misses.  Least recently used plans are dropped past the budget;
+# the lock makes lookups safe from several threads.
+PLAN_CACHE_BYTES = 16 << 30
+PLAN_CACHE_MAX = 256
+_PLAN_CACHE: "OrderedDict" = OrderedDict()
+_PLAN_LOCK = threading.Lock()
+
+
+def _cells_key(mesh: Mesh, vec: int):
+    ws = torch.empty(256, dtype=torch.uint8, device=mesh.device)
+    h = (C.c_uint64 * 2)()
+    _lib.call("jno_hash_i32", _lib.ptr(mesh.cells), mesh.cells.numel(), h, _lib.ptr(ws), 256,
+              _lib.stream_ptr(mesh.device))
+    dev = mesh.device.index if mesh.device.index is not None else torch.cuda.current_device()
+    return (dev, int(h[0]), int(h[1]), mesh.n_cells, mesh.nloc, mesh.n_nodes, vec)
+
+
+def _same_cells(a: torch.Tensor, b: torch.Tensor) -> bool:
+    if a is b:
+        return True
+    if a.shape != b.shape:
+        return False
+    ws = torch.empty(256, dtype=torch.uint8, device=a.device)
+    eq = C.c_int(0)
+    _lib.call("jno_equal_i32", _lib.ptr(a), _lib.ptr(b), a.numel(), C.byref(eq), _lib.ptr(ws), 256,
+              _lib.stream_ptr(a.device))
+    return bool(eq.value)
+
+
+def plan_for(mesh: Mesh, vec: int = 1) -> "Plan":
+    """The plan of ``mesh`` for ``vec`` components, from the content-keyed cache or built."""
+    key = _cells_key(mesh, vec)
+    with _PLAN_LOCK:
+        hit = _PLAN_CACHE.get(key)
+        if hit is not None and _same_cells(hit[1], mesh.cells):
+            _PLAN_CACHE.move_to_end(key)
+            return hit[0]
+    plan = Plan.build(mesh, vec)
+    with _PLAN_LOCK:
+        _PLAN_CACHE[key] = (plan, mesh.cells)
+        _PLAN_CACHE.move_to_end(key)
+        total = sum(p.nbytes + c.numel() * 4 for p, c in _PLAN_CACHE.values())
+        while len(_PLAN_CACHE) > 1 and (len(_PLAN_CACHE) > PLAN_CACHE_MAX or total > PLAN_CACHE_BYTES):
+            _, (p, c) = _PLAN_CACHE.popitem(last=False)
+            total -= p.nbytes + c.numel() * 4
+    return plan
+
+
+def clear_plan_cache():
+    with _PLAN_LOCK:
+        _PLAN_CACHE.clear()
 
 
 @dataclass
@@ -57,6 +114,10 @@ class Plan:
             _lib.ptr(rowptr), _lib.ptr(col), stream,
         )
         return Plan(rowptr, col[: nnz.value], inc_ptr, inc, int(nnz.value), int(max_row.value), ndof, vec)
+
+    @property
+    def nbytes(self) -> int:
+        return int(sum(t.numel() * t.element_size() for t in (self.rowptr, self.col, self.inc_ptr, self.inc)))
 
 
 @dataclass

@@ -62,12 +62,13 @@ class Mesh:
         return v.mean(dim=1)
 
     def plan(self, vec: int = 1):
-        """The CSR pattern + incidence plan for ``vec`` components (built once, cached)."""
-        from .fem import Plan
+        """The CSR pattern + incidence plan for ``vec`` components: from the content-keyed plan cache
+        (``jno_amd.fem.plan_for``; a mesh with the same cells shares it), kept on the mesh."""
+        from .fem import plan_for
 
         key = ("plan", vec)
         if key not in self._cache:
-            self._cache[key] = Plan.build(self, vec)
+            self._cache[key] = plan_for(self, vec)
         return self._cache[key]
 
     def boundary_mask(self, region: str = "boundary") -> torch.Tensor:

@@ -725,3 +725,47 @@ def test_concurrent_solves_with_per_call_tuning(dev):
         for x in out[i]:
             assert np.array_equal(x, r), i
     assert (solve.get_tuning("fp64"), solve.get_tuning("mixed")) == defaults
+
+
+def test_plan_cache_is_keyed_on_the_cells_content(dev):
+    # compress_plan's memo (jno/utils/solver/fem_utils.py:4027-4060): the same connectivity gives the
+    # same plan, a different one misses; the points do not enter the key (the pattern depends on the
+    # cells only), and the fused DIA assembly still checks its layout on the new values
+    import ctypes as C
+
+    import jno_amd as J
+    from jno_amd import _lib, fem, precond, solve
+
+    fem.clear_plan_cache()
+    m1, m2, m3 = J.kuhn_box(6), J.kuhn_box(6), J.kuhn_box(7)
+    assert m1.cells is not m2.cells
+    assert m1.plan(1) is m2.plan(1) and m1.plan(1) is not m3.plan(1)
+    assert m1.plan(3) is not m1.plan(1)
+    # content hash and equality
+    ws = torch.empty(256, dtype=torch.uint8, device="cuda")
+    h1, h2 = (C.c_uint64 * 2)(), (C.c_uint64 * 2)()
+    c = m1.cells.clone()
+    _lib.call("jno_hash_i32", _lib.ptr(c), c.numel(), h1, _lib.ptr(ws), 256, _lib.stream_ptr())
+    c.view(-1)[[5, 9]] = c.view(-1)[[9, 5]]  # two entries swapped: same multiset, other content
+    _lib.call("jno_hash_i32", _lib.ptr(c), c.numel(), h2, _lib.ptr(ws), 256, _lib.stream_ptr())
+    assert tuple(h1) != tuple(h2)
+    eq = C.c_int(0)
+    _lib.call("jno_equal_i32", _lib.ptr(c), _lib.ptr(m1.cells), c.numel(), C.byref(eq), _lib.ptr(ws), 256,
+              _lib.stream_ptr())
+    assert eq.value == 0
+    # a perturbed mesh on the same cells shares the plan and still solves to the oracle
+    n = 6
+    p, t = O.kuhn_box(n)
+    pj = p.copy()
+    interior = np.ones(p.shape[0], bool)
+    interior[O.boundary_nodes(p, t)] = False
+    pj[interior] += np.random.default_rng(9).uniform(-0.2, 0.2, (interior.sum(), 3)) / n
+    k = O.k_lognormal(t.shape[0], 13)
+    sv = solve.cg(tol=1e-10, precision="mixed")
+    A1, b1 = J.fem_poisson(m1).evaluate({"k": torch.tensor(k, device="cuda")})
+    sv(A1, b1, M=precond.jacobi())  # the DIA offsets of this plan are learnt here
+    mj = J.Mesh(points=torch.tensor(pj, device="cuda"), cells=m2.cells.clone())
+    assert mj.plan(1) is m1.plan(1)
+    Aj, bj = J.fem_poisson(mj).evaluate({"k": torch.tensor(k, device="cuda")})
+    Ao, bo, _ = O.poisson_p1_system(pj, t, k)
+    assert _rel(_np(sv(Aj, bj, M=precond.jacobi())), O.direct_solve(Ao, bo)) <= SOL_TOL
