@@ -8,10 +8,13 @@ residual replacement, stopping on the true FP64 residual.  The CSR plan (pattern
 as the reference caches it.
 
 Also reported: the plain FP64 GPU PCG on the same operator layout, the forward+adjoint gradient
-rate on 64^3 (dL/dk for all 1,572,864 elements), the live HIP-event roofline of the dominant
-kernel, and the CPU baseline (the C + OpenMP FP64 restatement, a bounded sample on the host).
+rate on 64^3 (dL/dk for all 1,572,864 elements), the transient heat march, the C4 batch (256
+uneven systems; 64 right-hand sides on 96^3) and the C5 P2 elasticity solve on the
+element-by-element operator, each with its roofline fraction, the live HIP-event roofline of the
+dominant kernel, and the CPU baseline (the C + OpenMP FP64 restatement, a bounded sample on the
+host: one warm-up, then the median of 3).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-grad]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-grad] [--no-batch] [--no-p2]
 """
 from __future__ import annotations
 
@@ -42,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-transient", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-p2", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the bounded CPU PCG sample")
     return ap.parse_args()
 
@@ -222,6 +227,14 @@ def main():
         log(f"[bench] transient heat march on {n}^3")
         out["transient"] = heat_rate(J, torch, dev, n)
 
+    if not args.no_batch:
+        log("[bench] C4 batch: 256 uneven systems, 64 right-hand sides on 96^3")
+        out["batch"] = c4_rate(J, torch, dev)
+
+    if not args.no_p2:
+        log("[bench] C5 P2 elasticity 32^3 on the element-by-element operator")
+        out["p2_elasticity"] = c5_rate(J, torch, dev)
+
     if not args.no_cpu:
         log("[bench] CPU baseline (C + OpenMP FP64 restatement, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(n, args.seed, args.cpu_seconds, st_fp64["iterations"])
@@ -325,21 +338,130 @@ def cpu_baseline(n, seed, budget_s, full_iters):
     k = np.exp(0.5 * np.random.default_rng(seed).standard_normal(t.shape[0]))
     mask = np.any((p == 0.0) | (p == 1.0), axis=1).astype(np.uint8)
     s = cpu_fem.P1System(p, t, k, 1.0, mask)
-    # a short probe sizes the bounded sample to the budget
+    # warm-up (the construction's assembly and a short probe, which also sizes the bounded sample to
+    # the budget), then three timed repetitions of assembly + the PCG sample: medians
     _, it0, _, dt0 = s.pcg(1e-10, maxiter=10)
     per_it = dt0 / max(it0, 1)
-    sample = int(max(20, min(full_iters, budget_s / max(per_it, 1e-9))))
-    _, its, _, dts = s.pcg(1e-10, maxiter=sample)
-    per_it = dts / max(its, 1)
-    est = s.t_assemble + per_it * full_iters
+    sample = int(max(20, min(full_iters, budget_s / 3 / max(per_it, 1e-9))))
+    t_asm, t_it = [], []
+    for _ in range(3):
+        t_asm.append(s.assemble())
+        _, its, _, dts = s.pcg(1e-10, maxiter=sample)
+        t_it.append(dts / max(its, 1))
+    asm, per_it = float(np.median(t_asm)), float(np.median(t_it))
+    est = asm + per_it * full_iters
     return {
         "value": round(est, 3),
         "unit": "s per assemble+solve to rtol 1e-10 (estimated: assembly + per-iteration time x full iteration count)",
         "cores": cpu_fem.threads(),
         "kind": "port",
-        "sample": f"C+OpenMP FP64 restatement on the structural CSR (nnz {len(s.col)}): full assembly "
-                  f"({s.t_assemble:.3f} s, pattern {s.t_pattern:.3f} s untimed) + {its} of {full_iters} PCG iterations "
-                  f"({per_it * 1e3:.2f} ms/iteration)",
+        "sample": f"C+OpenMP FP64 restatement on the structural CSR (nnz {len(s.col)}), one warm-up then the median "
+                  f"of 3: assembly {asm:.3f} s (pattern {s.t_pattern:.3f} s untimed) + {its} of {full_iters} PCG "
+                  f"iterations at {per_it * 1e3:.2f} ms/iteration",
+    }
+
+
+def c5_rate(J, torch, dev, n=32):
+    """C5: P2 elasticity on the n^3 Kuhn box (E = exp(0.3 z), default_rng(0); nu = 0.3; clamp x = 0,
+    traction (0, 0, -1) on x = 1), mixed solve to rtol 1e-10 on the element-by-element operator, with
+    the assembled BSR3 layout beside it; the EBE SpMV (element pass + node gather) timed live."""
+    from jno_amd import _lib, precond, solve
+    from jno_amd.elasticity import ElasticitySystem
+
+    lib = _lib.load()
+    s = ElasticitySystem(J.kuhn_box(n, device=dev))
+    T, N = s.mesh.n_cells, s.mesh.n_nodes
+    E = torch.tensor(np.exp(0.3 * np.random.default_rng(0).standard_normal(T)), device=dev)
+    A, b = s.element_operator({"E": E})
+    sv = solve.cg(tol=1e-10, precision="mixed")
+    sv(A, b, M=precond.jacobi())  # warm: code objects, diagonal
+    torch.cuda.synchronize()
+    lib.jno_pcg_profile(1)
+    t0 = time.perf_counter()
+    u = sv(A, b, M=precond.jacobi())
+    torch.cuda.synchronize()
+    t_ebe = time.perf_counter() - t0
+    prof_ms, prof_n = (C.c_double * 3)(), C.c_int64()
+    lib.jno_pcg_profile_read(prof_ms, C.byref(prof_n))
+    lib.jno_pcg_profile(0)
+    st = dict(sv.last_stats)
+    g = A.geometry
+    slots, rows = g.n_slots, 3 * N
+    # algorithmic bytes of one FP32 SpMV: element pass (slot cells, bits, scale, incidence positions,
+    # x once, products written) + gather (node pointers, products, x, mask, y)
+    spmv_bytes = slots * (40 + 4 + 4 + 40) + rows * 4 + T * 30 * 4 + (N + 1) * 4 + T * 30 * 4 + rows * (4 + 1 + 4)
+    spmv_ms = prof_ms[1] / max(prof_n.value, 1)
+    Ab, bb = s.evaluate({"E": E})
+    svb = solve.cg(tol=1e-10, precision="mixed", format="bsr3")
+    svb(Ab, bb, M=precond.jacobi())  # warm: layout
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ub = svb(Ab, bb, M=precond.jacobi())
+    torch.cuda.synchronize()
+    t_bsr3 = time.perf_counter() - t0
+    gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
+    return {
+        "workload": f"P2 elasticity {n}^3 (T={T}, {rows} dofs), E=exp(0.3 z) seed 0, nu=0.3, mixed to rtol 1e-10",
+        "ebe_mixed_s": round(t_ebe, 6), "ebe_iterations": st["iterations"], "ebe_relres_true": st["relres"],
+        "bsr3_mixed_s": round(t_bsr3, 6), "bsr3_iterations": svb.last_stats["iterations"],
+        "speedup_ebe_vs_bsr3": round(t_bsr3 / t_ebe, 3),
+        "ebe_vs_bsr3_rel_l2": float(torch.linalg.norm(u - ub) / torch.linalg.norm(ub)),
+        "roofline": {"kernel": "ebe_elem_k<f32> + ebe_gather_k<f32> (one SpMV)", "avg_launch_ms": round(spmv_ms, 5),
+                     "bytes_per_launch": spmv_bytes, "achieved": round(gbs, 1) if gbs else None, "unit": "GB/s",
+                     "peak": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,
+                     "sampled_iterations": prof_n.value},
+    }
+
+
+def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
+    """C4: ``count`` P1 Poisson systems on n^3 Kuhn boxes, n uniform in [24, 64] (default_rng(seed)),
+    each with its own lognormal k (default_rng(1000 + i)), f = 1, solved together to rtol 1e-10; and
+    ``n_rhs`` N(0,1) right-hand sides on one n_rhs_mesh^3 operator.  Effective bandwidth: the batch
+    step's 48 algorithmic bytes per row (Jacobi-scaled DIA items: p update 16, SpMV 20, r update 12)
+    over every item's inner iterations, against the solve time."""
+    from jno_amd.batch import BatchSolver
+
+    ns = np.random.default_rng(seed).integers(24, 65, size=count)
+    systems = []
+    for i, nn in enumerate(ns):
+        m = J.kuhn_box(int(nn), device=dev)
+        k = np.exp(0.5 * np.random.default_rng(1000 + i).standard_normal(m.n_cells))
+        systems.append(J.fem_poisson(m).evaluate({"k": torch.tensor(k, device=dev)}))
+    bs = BatchSolver(tol=1e-10, steps_per_graph=8)
+    bs(systems)  # warm: layouts, graph capture, code objects
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs(systems)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    its = [s["iterations"] for s in bs.last_stats]
+    rows = [A.shape[0] for A, _ in systems]
+    moved = sum(48 * r * it for r, it in zip(rows, its))
+    m96 = J.kuhn_box(n_rhs_mesh, device=dev)
+    k96 = np.exp(0.5 * np.random.default_rng(7).standard_normal(m96.n_cells))
+    A96, _ = J.fem_poisson(m96).evaluate({"k": torch.tensor(k96, device=dev)})
+    rng = np.random.default_rng(8)
+    rhs = [(A96, torch.tensor(rng.standard_normal(A96.shape[0]), device=dev)) for _ in range(n_rhs)]
+    br = BatchSolver(tol=1e-10, steps_per_graph=8)
+    br(rhs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    br(rhs)
+    torch.cuda.synchronize()
+    t_rhs = time.perf_counter() - t0
+    its_r = [s["iterations"] for s in br.last_stats]
+    moved_r = sum(48 * A96.shape[0] * it for it in its_r)
+    gbs = moved / t_batch / 1e9
+    return {
+        "workload": f"{count} P1 Poisson systems, n^3 Kuhn boxes with n in [24, 64] (seed {seed}), {sum(rows)} rows; "
+                    f"plus {n_rhs} N(0,1) right-hand sides on {n_rhs_mesh}^3; mixed to rtol 1e-10",
+        "batch_s": round(t_batch, 6), "iterations_min_max": [min(its), max(its)],
+        "worst_relres": max(s["relres"] for s in bs.last_stats),
+        "rhs_s": round(t_rhs, 6), "rhs_iterations_min_max": [min(its_r), max(its_r)],
+        "roofline": {"kernel": "batch step (batch_ka + batch_ks + batch_kb), whole solve",
+                     "bytes": moved, "achieved": round(gbs, 1), "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "rhs_achieved": round(moved_r / t_rhs / 1e9, 1)},
     }
 
 

@@ -68,10 +68,19 @@ class P1System:
         k = np.ascontiguousarray(k, dtype=np.float64)
         dm = None if dmask is None else np.ascontiguousarray(dmask, dtype=np.uint8)
         gg = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
-        lib.oracle_p1_assemble(_p(self.points), _p(self.cells), N, _p(self.inc_ptr), _p(self.inc), _p(self.rowptr),
-                               _p(self.col), _p(k), float(f), _p(dm), _p(gg), _p(self.vals), _p(self.b), _p(self.dinv))
+        self._args = (k, float(f), dm, gg)
+        self.assemble()
         t2 = time.perf_counter()
         self.t_pattern, self.t_assemble = t1 - t0, t2 - t1
+
+    def assemble(self) -> float:
+        """Re-assemble A, b, dinv on the plan (the per-step work); returns the seconds it took."""
+        k, f, dm, gg = self._args
+        t0 = time.perf_counter()
+        load().oracle_p1_assemble(_p(self.points), _p(self.cells), self.n, _p(self.inc_ptr), _p(self.inc),
+                                  _p(self.rowptr), _p(self.col), _p(k), f, _p(dm), _p(gg), _p(self.vals), _p(self.b),
+                                  _p(self.dinv))
+        return time.perf_counter() - t0
 
     def pcg(self, tol=1e-10, atol=0.0, maxiter=20000):
         x = np.empty(self.n, np.float64)
