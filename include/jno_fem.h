@@ -118,6 +118,27 @@ int jno_assemble_p1_dia(int kind, const double* d_points, const int32_t* d_cells
                         const int32_t* h_offsets, double* d_dia64, float* d_dia32, int* h_ok, void* d_ws,
                         size_t ws_bytes, void* stream);
 
+/* The same assembly on a structured Kuhn box (the mesh jno_mesh_kuhn_box makes with nx, ny, nz >= 2,
+ * on the plan jno_pattern_count/fill built for it; no reference counterpart as a separate entry:
+ * it is jno_assemble_p1 specialized to that mesh).  Each row sums the same element rows in the
+ * same ascending element order without gathering the incidence, cells or points: the six cell
+ * types' |T| and term-by-term gradient dot products come from h_geo (jno_box_geometry, once per
+ * mesh), and only k is read per element.  On dyadic spacings the values equal jno_assemble_p1's
+ * bitwise; on others every cell of a type takes voxel 0's geometry (a last-bit difference).
+ * d_dia64 (nullable) / d_dia32: the DIA layout at h_offsets, which must hold every offset with a
+ * nonzero coupling (the caller decides that from h_geo: no flag, no stream synchronization).
+ * jno_box_geometry: h_geo[6*17] = per type |T| then the 4x4 dot products (d_ws 816 bytes).
+ * jno_mesh_is_kuhn_box: *h_ok = 1 when the points and cells are bitwise those of the box
+ * (d_ws 256 bytes).  Both synchronize the stream. */
+int jno_box_geometry(int nx, int ny, int nz, const double* d_points, double* h_geo, void* d_ws, size_t ws_bytes,
+                     void* stream);
+int jno_mesh_is_kuhn_box(int nx, int ny, int nz, const double* h_lo, const double* h_hi, const double* d_points,
+                         const int32_t* d_cells, int* h_ok, void* d_ws, size_t ws_bytes, void* stream);
+int jno_assemble_p1_box(int kind, int nx, int ny, int nz, const double* h_geo, const int32_t* d_rowptr,
+                        const double* d_k, double f_const, const double* d_f_nodal, const uint8_t* d_dmask,
+                        const double* d_g, double* d_vals, double* d_b, double* d_dinv, int ndiag,
+                        const int32_t* h_offsets, double* d_dia64, float* d_dia32, void* stream);
+
 /* A11 on any assembled CSR (P2 elasticity uses it): bcoo_zero_rows_cols + bcoo_set_unit_diag
  * (jno/utils/solver/fem_utils.py:3332-3359) with the lift b -= A e(g) and b[d] = g.  In place. */
 int jno_dirichlet_eliminate(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,

@@ -119,6 +119,11 @@ SIGNATURES = {
     "jno_assemble_p1": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, P]),
     "jno_assemble_p1_dia": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, C.c_int,
                                       C.POINTER(I32), P, P, C.POINTER(C.c_int), P, SZ, P]),
+    "jno_box_geometry": (C.c_int, [C.c_int, C.c_int, C.c_int, P, C.POINTER(D), P, SZ, P]),
+    "jno_mesh_is_kuhn_box": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(D), C.POINTER(D), P, P,
+                                       C.POINTER(C.c_int), P, SZ, P]),
+    "jno_assemble_p1_box": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(D), P, P, D, P, P, P, P, P, P,
+                                      C.c_int, C.POINTER(I32), P, P, P]),
     "jno_dirichlet_eliminate": (C.c_int, [I64, P, P, P, P, P, P, P]),
     "jno_jacobi": (C.c_int, [I64, P, P, P, P, P]),
     "jno_spmv_f64": (C.c_int, [I64, P, P, P, P, P, P]),

@@ -8,6 +8,8 @@
 // Every value is written exactly once; the summation order per entry is fixed, so results are
 // bit-reproducible run to run.  (Half a wavefront per row, one lane per incident element with the
 // contributions added in the same order through LDS, measured twice as slow.)
+#include <type_traits>
+
 #include "common.h"
 #include "sparse.h"
 
@@ -198,6 +200,350 @@ __global__ void __launch_bounds__(256) assemble_p1_k(
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Structured Kuhn box (the mesh of Shape.box(..).structured(n)): the row of node (i, j, k) sums the
+// same element rows as assemble_p1_k, in the same ascending element order, without gathering the
+// incidence, cells or points.  Every cell of type t (0..5) of a box has the vertex differences of
+// the type-t cell of voxel 0 -- bitwise on dyadic spacings; on others they differ in the last bits
+// from voxel to voxel, and voxel 0's stand for all -- so the six cells' |T| and gradient dot
+// products (formed by tet_geom and the same term-by-term dots) are computed once per mesh and the
+// kernel only reads k.  The 15 couplings of a node (offsets d in {-1,0,1}^3 whose nonzero
+// components share a sign) are accumulated into registers by compile-time stencil slot, ordered
+// like the CSR row (ascending node id = lexicographic d); a boundary row skips its missing slots.
+// ---------------------------------------------------------------------------------------------
+struct BoxGeom {
+  double vol[6];
+  double dot[6][4][4];
+};
+
+__host__ __device__ constexpr int kuhn_corner(int t, int a) {
+  constexpr int K[6][4] = {{0, 4, 6, 7}, {0, 4, 5, 7}, {0, 2, 6, 7}, {0, 2, 3, 7}, {0, 1, 5, 7}, {0, 1, 3, 7}};
+  return K[t][a];
+}
+
+// stencil slot of the offset (di, dj, dk): its rank among the 15 in lexicographic order
+__host__ __device__ constexpr int stencil_slot(int di, int dj, int dk) {
+  int r = 0;
+  for (int a = -1; a <= 1; ++a)
+    for (int b = -1; b <= 1; ++b)
+      for (int c = -1; c <= 1; ++c) {
+        const bool pos = a >= 0 && b >= 0 && c >= 0, neg = a <= 0 && b <= 0 && c <= 0;
+        if (!pos && !neg) continue;
+        if (a == di && b == dj && c == dk) return r;
+        ++r;
+      }
+  return -1;
+}
+__host__ __device__ constexpr int slot_di(int s) {
+  int r = 0;
+  for (int a = -1; a <= 1; ++a)
+    for (int b = -1; b <= 1; ++b)
+      for (int c = -1; c <= 1; ++c) {
+        const bool pos = a >= 0 && b >= 0 && c >= 0, neg = a <= 0 && b <= 0 && c <= 0;
+        if (!pos && !neg) continue;
+        if (r++ == s) return a * 9 + b * 3 + c;  // packed (a, b, c)
+      }
+  return 0;
+}
+
+// q-th incident (voxel corner c, type t, local vertex a) of a node, in ascending element order:
+// voxel (i - di, j - dj, k - dk) of corner c = (di, dj, dk) has the larger id the smaller c is, so
+// corners descend from 7, and types ascend within a voxel
+struct BoxInc {
+  int c, t, a;
+};
+__host__ __device__ constexpr BoxInc box_inc(int q) {
+  int n = 0;
+  for (int c = 7; c >= 0; --c)
+    for (int t = 0; t < 6; ++t)
+      for (int a = 0; a < 4; ++a)
+        if (kuhn_corner(t, a) == c) {
+          if (n == q) return BoxInc{c, t, a};
+          ++n;
+        }
+  return BoxInc{-1, -1, -1};
+}
+
+struct BoxArgs {
+  int nx, ny, nz;
+  BoxGeom geo;
+  const int32_t* rowptr;
+  const double* k;
+  double f_const;
+  const double* f_nodal;
+  const uint8_t* dmask;
+  const double* g;
+  double* vals;
+  double* b;
+  double* dinv;
+  int ndiag;
+  int dia_slot[JNO_DIA_MAX];  // stencil slot of each stored diagonal (-1: none, the diagonal is 0)
+  double* d64;
+  float* d32;
+};
+
+constexpr int BOX_BLOCK = 256;
+constexpr int BOX_GEO = 6 * 17;  // |T| of the six types, then their 4x4 dot products
+
+// the incidence Q of the row's node (i, j, kk): its element's contributions to the 15 stencil slots
+// and to the load.  An incidence outside the box (a boundary row) is masked: it reads a clamped k
+// and adds +0.0, which leaves every sum bitwise as the unmasked kernel's (sums start at +0.0, so no
+// -0.0 arises either way).
+// f(std::integral_constant<int, B>), ..., f(std::integral_constant<int, E - 1>): every index a
+// compile-time constant inside f
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// (di, dj, dk) of stencil slot S
+template <int S>
+struct SlotOffset {
+  static constexpr int pk = slot_di(S);
+  static constexpr int di = (pk + 13) / 9 - 1, dj = ((pk + 13) / 3) % 3 - 1, dk = (pk + 13) % 3 - 1;
+};
+
+template <int Q, int BB>
+struct IncTerm {  // vertex BB of incidence Q's element, seen from the row's node
+  static constexpr BoxInc L = box_inc(Q);
+  static constexpr int cb = kuhn_corner(L.t, BB);
+  static constexpr int di = ((cb >> 2) & 1) - ((L.c >> 2) & 1), dj = ((cb >> 1) & 1) - ((L.c >> 1) & 1),
+                       dk = (cb & 1) - (L.c & 1);
+  static constexpr int slot = stencil_slot(di, dj, dk);
+};
+
+// is the voxel of corner C (the row's node is that voxel's corner C) inside the box?
+template <int C>
+__device__ __forceinline__ bool box_inside(const BoxArgs& a, int i, int j, int kk) {
+  const int vi = i - ((C >> 2) & 1), vj = j - ((C >> 1) & 1), vk = kk - (C & 1);
+  return vi >= 0 && vi < a.nx && vj >= 0 && vj < a.ny && vk >= 0 && vk < a.nz;
+}
+
+// k of the types of corner C's voxel that hold the row's node (all six for corners 0 and 7, two for
+// the others), as 16-byte loads where the types are adjacent: a voxel's six k values are contiguous.
+// Loaded unconditionally (voxel 0 for a voxel outside the box) so that all of a row's loads are in
+// flight together.
+template <int C>
+__device__ __forceinline__ void box_k(const BoxArgs& a, int i, int j, int kk, bool in, double (&kc)[8][6]) {
+  const int vi = i - ((C >> 2) & 1), vj = j - ((C >> 1) & 1), vk = kk - (C & 1);
+  const double* kp = a.k + (in ? 6 * (((int64_t)vi * a.ny + vj) * a.nz + vk) : 0);
+  auto pair = [&](int t) {
+    const double2 v = *reinterpret_cast<const double2*>(kp + t);
+    kc[C][t] = v.x;
+    kc[C][t + 1] = v.y;
+  };
+  if constexpr (C == 0 || C == 7) {
+    pair(0);
+    pair(2);
+    pair(4);
+  } else if constexpr (C == 4) {
+    pair(0);
+  } else if constexpr (C == 2) {
+    pair(2);
+  } else if constexpr (C == 1) {
+    pair(4);
+  } else {  // corners 6 (types 0, 2), 5 (types 1, 4), 3 (types 3, 5)
+    constexpr int t0 = C == 6 ? 0 : C == 5 ? 1 : 3, t1 = C == 6 ? 2 : C == 5 ? 4 : 5;
+    kc[C][t0] = kp[t0];
+    kc[C][t1] = kp[t1];
+  }
+}
+
+template <int KIND, bool NODAL, int Q>
+__device__ __forceinline__ void box_accumulate(const BoxArgs& a, const double* __restrict__ sg, const double (&kc)[8][6],
+                                               const bool (&inc)[8], int64_t r, int64_t S1, int64_t S2,
+                                               double (&acc)[15], double& bl) {
+  if constexpr (Q < 24) {
+    constexpr BoxInc L = box_inc(Q);
+    const bool in = inc[L.c];
+    const double vol = sg[L.t];
+    const double* dot = sg + 6 + (L.t * 4 + L.a) * 4;
+    if constexpr (KIND == 0) {
+      const double coef = in ? __dmul_rn(kc[L.c][L.t], vol) : 0.0;
+      static_for<0, 4>([&](auto BB) { acc[IncTerm<Q, BB>::slot] += __dmul_rn(coef, dot[BB]); });
+      if constexpr (NODAL) {
+        double fs = 0.0;
+        static_for<0, 4>([&](auto BB) {
+          using T = IncTerm<Q, BB>;
+          fs += a.f_nodal[in ? r + T::di * S1 + T::dj * S2 + T::dk : r];
+        });
+        bl += in ? vol * (a.f_nodal[r] + fs) / 20.0 : 0.0;
+      } else {
+        bl += in ? a.f_const * vol / 4.0 : 0.0;
+      }
+    } else {
+      const double m = in ? vol / 20.0 : 0.0;
+      static_for<0, 4>([&](auto BB) { acc[IncTerm<Q, BB>::slot] += (BB == L.a) ? 2.0 * m : m; });
+    }
+    // the sums so far are complete here, and the geometry reads of the next incidences stay below
+    // (an empty asm that takes the sums as operands and clobbers memory): with all 96 reads hoisted
+    // to the top and the arithmetic sunk below them the kernel would hold ~300 registers
+    if constexpr (Q % 2 == 1)
+      __asm__ __volatile__(""
+                           : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
+                             "+v"(acc[6]), "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]),
+                             "+v"(acc[12]), "+v"(acc[13]), "+v"(acc[14]), "+v"(bl)
+                           :
+                           : "memory");
+    box_accumulate<KIND, NODAL, Q + 1>(a, sg, kc, inc, r, S1, S2, acc, bl);
+  }
+}
+
+// One row per thread, BOX_BLOCK consecutive rows per block.  The rows' CSR values are staged in
+// LDS and written as one coalesced span [rowptr[r0], rowptr[r0 + BOX_BLOCK]).
+template <int KIND, bool NODAL, bool DMASK>
+__global__ void __launch_bounds__(BOX_BLOCK) assemble_box_k(BoxArgs a) {
+  __shared__ double sg[BOX_GEO];
+  __shared__ double sv[15 * BOX_BLOCK];
+  const int nz1 = a.nz + 1, ny1 = a.ny + 1;
+  const int64_t S2 = nz1, S1 = (int64_t)ny1 * nz1;
+  const int64_t N = S1 * (a.nx + 1);
+  const int tid = threadIdx.x;
+  if (tid == 0) {  // compile-time indices: the table is read from the kernel arguments, not copied
+#pragma unroll
+    for (int t = 0; t < 6; ++t) sg[t] = a.geo.vol[t];
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sg[6 + (t * 4 + p) * 4 + q] = a.geo.dot[t][p][q];
+  }
+  const int64_t r0 = (int64_t)blockIdx.x * BOX_BLOCK;
+  const int64_t r = r0 + tid;
+  const int64_t base = a.rowptr[r0];
+  const int64_t span = a.rowptr[r0 + BOX_BLOCK < N ? r0 + BOX_BLOCK : N] - base;
+  __syncthreads();
+  if (r < N) {
+    const int kk = (int)(r % nz1);
+    const int64_t w = r / nz1;
+    const int j = (int)(w % ny1), i = (int)(w / ny1);
+    const int64_t pos0 = a.rowptr[r] - base;  // issued first: it is needed only at the end
+    double acc[15];
+#pragma unroll
+    for (int s = 0; s < 15; ++s) acc[s] = 0.0;
+    double bl = 0.0;
+    bool inc[8];
+    double kc[8][6];
+    static_for<0, 8>([&](auto C) { inc[C] = box_inside<C>(a, i, j, kk); });
+    if constexpr (KIND == 0) static_for<0, 8>([&](auto C) { box_k<C>(a, i, j, kk, inc[C], kc); });
+    box_accumulate<KIND, NODAL, 0>(a, sg, kc, inc, r, S1, S2, acc, bl);
+    // the slots that exist (the neighbour is inside the box)
+    bool ex[15];
+    static_for<0, 15>([&](auto S) {
+      using O = SlotOffset<S>;
+      ex[S] = i + O::di >= 0 && i + O::di <= a.nx && j + O::dj >= 0 && j + O::dj <= a.ny && kk + O::dk >= 0 &&
+              kk + O::dk <= a.nz;
+    });
+    // symmetric Dirichlet elimination (_apply_dirichlet_symmetric, fem_1d.py:667-686).  The mask (and
+    // g) of every neighbour is loaded unconditionally from a clamped index, so that the 15 loads are
+    // in flight together instead of one round trip per neighbour.
+    if constexpr (DMASK) {
+      bool dm[15];
+      static_for<0, 15>([&](auto S) {
+        using O = SlotOffset<S>;
+        dm[S] = a.dmask[ex[S] ? r + O::di * S1 + O::dj * S2 + O::dk : r] != 0;
+      });
+      double gv[15];
+      if (a.g) {
+        static_for<0, 15>([&](auto S) {
+          using O = SlotOffset<S>;
+          gv[S] = a.g[ex[S] ? r + O::di * S1 + O::dj * S2 + O::dk : r];
+        });
+      }
+      if (dm[7]) {
+        static_for<0, 15>([&](auto S) { acc[S] = S == 7 ? 1.0 : 0.0; });
+        bl = a.g ? gv[7] : 0.0;
+      } else {
+        double lift = 0.0;
+        static_for<0, 15>([&](auto S) {
+          if (ex[S] && dm[S]) {
+            if (a.g) lift += acc[S] * gv[S];
+            acc[S] = 0.0;
+          }
+        });
+        bl -= lift;
+      }
+    }
+    int64_t pos = pos0;
+#pragma unroll
+    for (int s = 0; s < 15; ++s)
+      if (ex[s]) sv[pos++] = acc[s];
+    if (a.b) a.b[r] = bl;
+    if (a.dinv) a.dinv[r] = fabs(acc[7]) > 1e-30 ? 1.0 / acc[7] : 1.0;
+    if (a.d64) {
+      const int64_t ld = dia_ld(N);
+      for (int dd = 0; dd < a.ndiag; ++dd) {
+        const int want = a.dia_slot[dd];
+        double v = 0.0;
+#pragma unroll
+        for (int s = 7; s < 15; ++s) v = (want == s) ? acc[s] : v;
+        a.d64[(int64_t)dd * ld + r] = v;
+        if (a.d32) a.d32[(int64_t)dd * ld + r] = (float)v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int64_t q = tid; q < span; q += BOX_BLOCK) a.vals[base + q] = sv[q];
+}
+
+// the six cell types of voxel 0: |T| and the term-by-term gradient dot products, as assemble_p1_k
+// forms them
+__global__ void box_geom_k(int ny, int nz, const double* __restrict__ pts, double* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t >= 6) return;
+  const int64_t S2 = nz + 1, S1 = (int64_t)(ny + 1) * (nz + 1);
+  int c[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int cc = kuhn_corner(t, a);
+    c[a] = (int)(((cc >> 2) & 1) * S1 + ((cc >> 1) & 1) * S2 + (cc & 1));
+  }
+  const TetGeom g = tet_geom(pts, make_int4(c[0], c[1], c[2], c[3]));
+  out[t] = g.vol;
+  for (int a = 0; a < 4; ++a)
+    for (int bb = 0; bb < 4; ++bb)
+      out[6 + (t * 4 + a) * 4 + bb] =
+          __dadd_rn(__dadd_rn(__dmul_rn(g.g[a][0], g.g[bb][0]), __dmul_rn(g.g[a][1], g.g[bb][1])),
+                    __dmul_rn(g.g[a][2], g.g[bb][2]));
+}
+
+// is (points, cells) the structured box jno_mesh_kuhn_box makes (bitwise)?  *bad set when not
+__global__ void box_check_k(int nx, int ny, int nz, double x0, double y0, double z0, double x1, double y1, double z1,
+                            const double* __restrict__ pts, const int4* __restrict__ cells, int* __restrict__ bad) {
+  const int64_t S2 = nz + 1, S1 = (int64_t)(ny + 1) * (nz + 1);
+  const int64_t N = S1 * (nx + 1), T = (int64_t)6 * nx * ny * nz;
+  bool ok = true;
+  for (int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; id < N || id < T;
+       id += (int64_t)gridDim.x * blockDim.x) {
+    if (id < N) {
+      const int k = (int)(id % S2);
+      const int64_t w = id / S2;
+      const int j = (int)(w % (ny + 1)), i = (int)(w / (ny + 1));
+      ok = ok && pts[3 * id] == linspace_at(x0, x1, nx, i) && pts[3 * id + 1] == linspace_at(y0, y1, ny, j) &&
+           pts[3 * id + 2] == linspace_at(z0, z1, nz, k);
+    }
+    if (id < T) {
+      const int typ = (int)(id % 6);
+      const int64_t v = id / 6;
+      const int k = (int)(v % nz);
+      const int64_t w = v / nz;
+      const int j = (int)(w % ny), i = (int)(w / ny);
+      const int4 c = cells[id];
+      const int cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int q = kuhn_corner(typ, a);
+        ok = ok && cc[a] == (int)((i + ((q >> 2) & 1)) * S1 + (j + ((q >> 1) & 1)) * S2 + (k + (q & 1)));
+      }
+    }
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
 __global__ void __launch_bounds__(256) adjoint_grad_p1_k(const double* __restrict__ pts, const int4* __restrict__ cells,
                                                          int64_t T, const double* __restrict__ u,
                                                          const double* __restrict__ lam,
@@ -319,6 +665,98 @@ extern "C" int jno_adjoint_grad_k_p1(const double* d_points, const int32_t* d_ce
   if (n_cells == 0) return JNO_OK;
   adjoint_grad_p1_k<<<grid_for(n_cells, 256, 8192), 256, 0, as_stream(stream)>>>(
       d_points, reinterpret_cast<const int4*>(d_cells), n_cells, d_u, d_lam, d_dmask, d_grad);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_box_geometry(int nx, int ny, int nz, const double* d_points, double* h_geo, void* d_ws,
+                                size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && d_points && h_geo && d_ws, JNO_EINVAL, "jno_box_geometry: bad argument");
+  JNO_REQUIRE(ws_bytes >= 6 * 17 * sizeof(double), JNO_EWORKSPACE, "jno_box_geometry: workspace too small (816 bytes)");
+  hipStream_t s = as_stream(stream);
+  box_geom_k<<<1, 64, 0, s>>>(ny, nz, d_points, static_cast<double*>(d_ws));
+  JNO_LAUNCH_CHECK();
+  JNO_HIP(hipMemcpyAsync(h_geo, d_ws, 6 * 17 * sizeof(double), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  return JNO_OK;
+}
+
+extern "C" int jno_mesh_is_kuhn_box(int nx, int ny, int nz, const double* h_lo, const double* h_hi,
+                                    const double* d_points, const int32_t* d_cells, int* h_ok, void* d_ws,
+                                    size_t ws_bytes, void* stream) {
+  JNO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && h_lo && h_hi && d_points && d_cells && h_ok && d_ws, JNO_EINVAL,
+              "jno_mesh_is_kuhn_box: bad argument");
+  JNO_REQUIRE(ws_bytes >= 256, JNO_EWORKSPACE, "jno_mesh_is_kuhn_box: workspace too small (256 bytes)");
+  hipStream_t s = as_stream(stream);
+  int* bad = static_cast<int*>(d_ws);
+  JNO_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  const int64_t N = (int64_t)(nx + 1) * (ny + 1) * (nz + 1), T = (int64_t)6 * nx * ny * nz;
+  box_check_k<<<grid_for(N > T ? N : T, 256, 8192), 256, 0, s>>>(nx, ny, nz, h_lo[0], h_lo[1], h_lo[2], h_hi[0],
+                                                                 h_hi[1], h_hi[2], d_points,
+                                                                 reinterpret_cast<const int4*>(d_cells), bad);
+  JNO_LAUNCH_CHECK();
+  int h_bad = 1;
+  JNO_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *h_ok = h_bad ? 0 : 1;
+  return JNO_OK;
+}
+
+extern "C" int jno_assemble_p1_box(int kind, int nx, int ny, int nz, const double* h_geo, const int32_t* d_rowptr,
+                                   const double* d_k, double f_const, const double* d_f_nodal, const uint8_t* d_dmask,
+                                   const double* d_g, double* d_vals, double* d_b, double* d_dinv, int ndiag,
+                                   const int32_t* h_offsets, double* d_dia64, float* d_dia32, void* stream) {
+  JNO_REQUIRE(kind == 0 || kind == 1, JNO_EINVAL, "jno_assemble_p1_box: kind must be 0 (stiffness) or 1 (mass)");
+  JNO_REQUIRE(kind == 1 || d_k, JNO_EINVAL, "jno_assemble_p1_box: stiffness needs the per-element coefficient k");
+  JNO_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, JNO_EINVAL, "jno_assemble_p1_box: the box needs at least 2 cells per axis");
+  JNO_REQUIRE(h_geo && d_rowptr && d_vals, JNO_EINVAL, "jno_assemble_p1_box: null pointer");
+  JNO_REQUIRE(!d_dia64 || (ndiag >= 1 && ndiag <= JNO_DIA_MAX && h_offsets && h_offsets[0] == 0), JNO_EINVAL,
+              "jno_assemble_p1_box: bad DIA offsets (1..%d, the main diagonal first)", JNO_DIA_MAX);
+  BoxArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  for (int t = 0; t < 6; ++t) {
+    a.geo.vol[t] = h_geo[t];
+    for (int p = 0; p < 4; ++p)
+      for (int q = 0; q < 4; ++q) a.geo.dot[t][p][q] = h_geo[6 + (t * 4 + p) * 4 + q];
+  }
+  a.rowptr = d_rowptr;
+  a.k = d_k;
+  a.f_const = f_const;
+  a.f_nodal = d_f_nodal;
+  a.dmask = d_dmask;
+  a.g = d_g;
+  a.vals = d_vals;
+  a.b = d_b;
+  a.dinv = d_dinv;
+  if (d_dia64) {
+    // the stencil slot (upper half: 7..14) of each stored diagonal's offset
+    const int64_t S2 = nz + 1, S1 = (int64_t)(ny + 1) * (nz + 1);
+    a.ndiag = ndiag;
+    for (int d = 0; d < JNO_DIA_MAX; ++d) {
+      a.dia_slot[d] = -1;
+      for (int sl = 7; d < ndiag && sl < 15; ++sl) {
+        const int pk = slot_di(sl);
+        if (((pk + 13) / 9 - 1) * S1 + (((pk + 13) / 3) % 3 - 1) * S2 + ((pk + 13) % 3 - 1) == h_offsets[d])
+          a.dia_slot[d] = sl;
+      }
+    }
+    a.d64 = d_dia64;
+    a.d32 = d_dia32;
+  }
+  const int64_t N = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  const int64_t blocks = (N + BOX_BLOCK - 1) / BOX_BLOCK;
+  hipStream_t s = as_stream(stream);
+  const bool dm = d_dmask != nullptr;
+  if (kind == 1)
+    assemble_box_k<1, false, false><<<blocks, BOX_BLOCK, 0, s>>>(a);
+  else if (d_f_nodal)
+    dm ? assemble_box_k<0, true, true><<<blocks, BOX_BLOCK, 0, s>>>(a)
+       : assemble_box_k<0, true, false><<<blocks, BOX_BLOCK, 0, s>>>(a);
+  else
+    dm ? assemble_box_k<0, false, true><<<blocks, BOX_BLOCK, 0, s>>>(a)
+       : assemble_box_k<0, false, false><<<blocks, BOX_BLOCK, 0, s>>>(a);
   JNO_LAUNCH_CHECK();
   return JNO_OK;
 }

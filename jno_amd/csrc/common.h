@@ -87,4 +87,13 @@ int exclusive_scan_i32(const int32_t* d_in, int32_t* d_out, int64_t n, int64_t* 
                        size_t ws_bytes, hipStream_t stream);
 size_t exclusive_scan_ws(int64_t n);
 
+// numpy.linspace(lo, hi, n+1)[i]: i*step + lo with step = (hi-lo)/n, last entry exactly hi.
+// The products and sums are rounded separately (no FMA contraction) so the value is the same
+// double numpy produces.
+__device__ __forceinline__ double linspace_at(double lo, double hi, int n, int i) {
+  if (i == n) return hi;
+  const double step = __ddiv_rn(__dsub_rn(hi, lo), (double)n);
+  return __dadd_rn(__dmul_rn((double)i, step), lo);
+}
+
 }  // namespace jno

@@ -8,15 +8,6 @@ namespace jno {
 __constant__ int8_t c_kuhn[6][4] = {{0, 4, 6, 7}, {0, 4, 5, 7}, {0, 2, 6, 7},
                                     {0, 2, 3, 7}, {0, 1, 5, 7}, {0, 1, 3, 7}};
 
-// numpy.linspace(lo, hi, n+1)[i]: i*step + lo with step = (hi-lo)/n, last entry exactly hi.
-// The products and sums are rounded separately (no FMA contraction) so the value is the same
-// double numpy produces.
-__device__ __forceinline__ double linspace_at(double lo, double hi, int n, int i) {
-  if (i == n) return hi;
-  double step = __ddiv_rn(__dsub_rn(hi, lo), (double)n);
-  return __dadd_rn(__dmul_rn((double)i, step), lo);
-}
-
 __global__ void kuhn_points(int nx, int ny, int nz, double x0, double y0, double z0, double x1,
                             double y1, double z1, double* __restrict__ pts) {
   const int64_t N = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);

@@ -253,6 +253,66 @@ class _MatVec(torch.autograd.Function):
         return ctx.A._spmv(gy), None
 
 
+# Kuhn box stencil (the kernel's compile-time table): the 15 offsets (di, dj, dk) of a node's
+# couplings in ascending node order, and the (corner, type, local vertex) of its incident cells
+_KUHN = ((0, 4, 6, 7), (0, 4, 5, 7), (0, 2, 6, 7), (0, 2, 3, 7), (0, 1, 5, 7), (0, 1, 3, 7))
+
+
+class BoxAssembly:
+    """What the structured-box assembly needs of a mesh (built once per mesh): the six cell types'
+    geometry table and, for the stiffness, the DIA offsets that can hold a nonzero coupling."""
+
+    def __init__(self, mesh: Mesh, geo):
+        nx, ny, nz = mesh.grid
+        self.grid = (nx, ny, nz)
+        self.geo = geo
+        S2, S1 = nz + 1, (ny + 1) * (nz + 1)
+        offs = {0}
+        for t in range(6):
+            for a in range(4):
+                c = _KUHN[t][a]
+                for bb in range(4):
+                    cb = _KUHN[t][bb]
+                    if geo[6 + (t * 4 + a) * 4 + bb] != 0.0:
+                        d = (((cb >> 2) & 1) - ((c >> 2) & 1), ((cb >> 1) & 1) - ((c >> 1) & 1), (cb & 1) - (c & 1))
+                        offs.add(abs(d[0] * S1 + d[1] * S2 + d[2]))
+        self.stiffness_offsets = sorted(offs)
+
+    def covers(self, offsets) -> bool:
+        return set(self.stiffness_offsets) <= set(offsets)
+
+    @staticmethod
+    def of(mesh: Mesh) -> Optional["BoxAssembly"]:
+        """The box assembly of ``mesh`` when it is (bitwise) the structured Kuhn box of its grid and
+        box with at least 2 cells per axis, else None.  Checked once (one stream synchronization)
+        and kept on the mesh with the points tensor's identity and version: moving the points with
+        torch (``mesh.points.copy_(..)``, a new tensor) checks again, and a mesh found off the lattice
+        stays on the generic assembly.  Points written behind torch's back (raw pointers) need
+        ``mesh.invalidate()``."""
+        key = ("box_assembly",)
+        pts = mesh.points
+        stamp = (id(pts), pts.data_ptr(), pts._version)
+        hit = mesh._cache.get(key)
+        if hit is not None and (hit[0] is None or hit[1] == stamp):
+            return hit[0]
+        res = None
+        if mesh.grid is not None and mesh.box is not None and mesh.nloc == 4 and min(mesh.grid) >= 2:
+            nx, ny, nz = mesh.grid
+            ws = torch.empty(1024, dtype=torch.uint8, device=mesh.device)
+            ok = C.c_int(0)
+            lo = (C.c_double * 3)(*mesh.box[0])
+            hi = (C.c_double * 3)(*mesh.box[1])
+            stream = _lib.stream_ptr(mesh.device)
+            _lib.call("jno_mesh_is_kuhn_box", nx, ny, nz, lo, hi, _lib.ptr(mesh.points), _lib.ptr(mesh.cells),
+                      C.byref(ok), _lib.ptr(ws), 1024, stream)
+            if ok.value:
+                geo = (C.c_double * (6 * 17))()
+                _lib.call("jno_box_geometry", nx, ny, nz, _lib.ptr(mesh.points), geo, _lib.ptr(ws), 1024, stream)
+                res = BoxAssembly(mesh, geo)
+        mesh._cache[key] = (res, stamp)
+        return res
+
+
 def _as_k(k, T: int, device) -> torch.Tensor:
     k = torch.as_tensor(k, dtype=torch.float64, device=device)
     if k.ndim == 0:
@@ -272,11 +332,13 @@ class FemLinearSystem:
     """
 
     def __init__(self, mesh: Mesh, f: Union[float, torch.Tensor] = 1.0, dirichlet: Optional[str] = "boundary",
-                 g: Union[float, torch.Tensor] = 0.0, param: str = "k"):
+                 g: Union[float, torch.Tensor] = 0.0, param: str = "k", structured: bool = True):
         if mesh.nloc != 4:
             raise ValueError("FemLinearSystem (P1 Poisson) needs a 4-node tet mesh")
         self.mesh = mesh
         self.param = param
+        # the structured-box kernel on a Kuhn box mesh (False: always the incidence-gather kernel)
+        self.structured = bool(structured)
         self.plan = mesh.plan(1)
         N = mesh.n_nodes
         dev = mesh.device
@@ -317,6 +379,28 @@ class FemLinearSystem:
         from .operator import SolverOperator, chosen_dia_offsets
 
         dia = chosen_dia_offsets(A)
+        box = BoxAssembly.of(m) if self.structured else None
+        if box is not None:
+            # the structured box: no incidence or geometry gathers, and the DIA layout (when the
+            # learnt offsets cover every nonzero coupling the geometry table allows) in the same pass
+            # with no flag to read back
+            use_dia = dia is not None and box.covers(dia[1])
+            nd, offs = dia if use_dia else (0, [0])
+            ld = (N + 63) // 64 * 64
+            d64 = torch.empty(nd * ld, dtype=torch.float64, device=m.device) if use_dia else None
+            d32 = torch.empty(nd * ld, dtype=torch.float32, device=m.device) if use_dia else None
+            nx, ny, nz = box.grid
+            _lib.call(
+                "jno_assemble_p1_box", 0, nx, ny, nz, box.geo, _lib.ptr(plan.rowptr), _lib.ptr(k), self.f_const,
+                _lib.ptr(self.f_nodal), _lib.ptr(self.dmask), _lib.ptr(self.g), _lib.ptr(A.vals), _lib.ptr(b),
+                _lib.ptr(A.dinv), nd, (C.c_int32 * _lib.JNO_DIA_MAX)(*offs), _lib.ptr(d64), _lib.ptr(d32),
+                _lib.stream_ptr(m.device),
+            )
+            if use_dia:
+                op = SolverOperator.from_dia(A, nd, offs, d64, d32)
+                A._cache[("solver_op", "auto", True)] = op
+                A._cache[("solver_op", "auto", False)] = op
+            return A, b
         if dia is None:
             _lib.call(
                 "jno_assemble_p1", 0, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
@@ -356,6 +440,13 @@ class FemLinearSystem:
         m, plan = self.mesh, self.plan
         N = m.n_nodes
         vals = torch.empty(plan.nnz, dtype=torch.float64, device=m.device)
+        box = BoxAssembly.of(m) if self.structured else None
+        if box is not None:
+            nx, ny, nz = box.grid
+            _lib.call("jno_assemble_p1_box", 1, nx, ny, nz, box.geo, _lib.ptr(plan.rowptr), None, 0.0, None, None,
+                      None, _lib.ptr(vals), None, None, 0, (C.c_int32 * _lib.JNO_DIA_MAX)(0), None, None,
+                      _lib.stream_ptr(m.device))
+            return CSR(plan.rowptr, plan.col, vals, (N, N))
         _lib.call(
             "jno_assemble_p1", 1, _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, N, _lib.ptr(plan.inc_ptr),
             _lib.ptr(plan.inc), _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.max_row, None, 0.0, None, None, None,
@@ -374,6 +465,6 @@ class FemLinearSystem:
         return grad
 
 
-def fem_poisson(mesh: Mesh, f=1.0, dirichlet="boundary", g=0.0, param="k") -> FemLinearSystem:
+def fem_poisson(mesh: Mesh, f=1.0, dirichlet="boundary", g=0.0, param="k", structured=True) -> FemLinearSystem:
     """Spelling of ``jno.fem([k*(grad u . grad v) - f*v, u(boundary) - g])`` with ``k`` a P0 parameter."""
-    return FemLinearSystem(mesh, f=f, dirichlet=dirichlet, g=g, param=param)
+    return FemLinearSystem(mesh, f=f, dirichlet=dirichlet, g=g, param=param, structured=structured)

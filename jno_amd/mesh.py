@@ -71,6 +71,11 @@ class Mesh:
             self._cache[key] = plan_for(self, vec)
         return self._cache[key]
 
+    def invalidate(self) -> None:
+        """Forget everything derived from the points and cells (plans, masks, the P2 promotion, the
+        structured-box check): call it after changing them in place."""
+        self._cache.clear()
+
     def boundary_mask(self, region: str = "boundary") -> torch.Tensor:
         """u8[N] mask of the nodes of ``region``: "boundary" = every node on a facet used by
         exactly one cell (``_boundary_facets``, jno/_fem.py:2695-2773); a named box face keeps the

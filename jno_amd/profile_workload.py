@@ -9,6 +9,9 @@ finally copies a buffer of --copy-mib MiB with a device-to-device copy kernel: t
 FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels' counters.
 --case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver;
 --case heat marches --iters backward-Euler steps of the n^3 transient heat block (--n, default 128).
+--case assemble re-assembles the n^3 P1 system (--n, default 128) --iters times on the structured-box
+kernel and then --iters times on the incidence-gather kernel, both writing the DIA layout in the
+same pass (one solve first picks the offsets).
 """
 from __future__ import annotations
 
@@ -85,6 +88,41 @@ def run_heat(dev, steps: int, precision: str, n: int = 128, replace_factor=None)
           f"operator {sv.last_stats['operator']}")
 
 
+def run_assemble(dev, iters: int, n: int = 128):
+    import jno_amd as J
+    from jno_amd import precond
+
+    m = J.kuhn_box(n, device=dev)
+    k = torch.tensor(np.exp(0.5 * np.random.default_rng(0).standard_normal(m.n_cells)), device=dev)
+    for structured in (True, False):
+        s = J.fem_poisson(m, structured=structured)
+        A, b = s.evaluate({"k": k})
+        J.solve.cg(tol=1e-6)(A, b, M=precond.jacobi())  # picks the DIA offsets of the plan
+        A, b = s.evaluate({"k": k})  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            A, b = s.evaluate({"k": k})
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / iters
+        fused = A._cache.get(("solver_op", "auto", True))
+        print(f"assemble {n}^3 {'structured box' if structured else 'incidence gather'}: {dt * 1e6:.1f} us per "
+              f"evaluate, nnz {A.nnz}, DIA fused {fused is not None and fused.format == 'dia'}")
+
+
+def calibration_copy(dev, mib: int):
+    """A device-to-device copy of ``mib`` MiB: its FETCH_SIZE / WRITE_SIZE per byte calibrate the
+    counters of the other kernels."""
+    nbytes = mib << 20
+    src = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {nbytes} bytes read + {nbytes} bytes written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", default="c2")
@@ -129,6 +167,10 @@ def main():
     if args.case == "heat":
         run_heat(dev, args.iters, args.precision, args.n, args.replace_factor)
         return
+    if args.case == "assemble":
+        run_assemble(dev, args.iters, args.n)
+        calibration_copy(dev, args.copy_mib)
+        return
     A, b = build(args.case, dev, args.format)
     # replace_factor tiny: no residual replacement inside the bounded run
     sv = LinearSolver("cg", 1e-30, 0.0, args.iters, precision=args.precision, format=args.format,
@@ -141,14 +183,7 @@ def main():
     dt = time.perf_counter() - t0
     print(f"{args.case} {args.precision}: {sv.last_stats['iterations']} iterations in {dt * 1e3:.3f} ms "
           f"({dt / max(sv.last_stats['iterations'], 1) * 1e6:.1f} us/iteration), operator {sv.last_stats['operator']}")
-    nbytes = args.copy_mib << 20
-    src = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    dst = torch.empty_like(src)
-    src.fill_(1)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    print(f"calibration copy: {nbytes} bytes read + {nbytes} bytes written")
+    calibration_copy(dev, args.copy_mib)
 
 
 if __name__ == "__main__":
