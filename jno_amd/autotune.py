@@ -68,12 +68,13 @@ CANDIDATE_SHAPES = tuple(
     for cap in (256, 512, 768, 1024)
     for vec in (0, 1, 2, 3)
     for split in (0, 1, 3, 4, 5)
-    # LDS staging (vec 3) is a variant of mode 4; vec 2 is tried only as the register-resident
-    # one-kernel variant of mode 5 (256 and 512 threads), and cap 768 only there
-    if (vec != 3 or split == 4)
+    # vec 3 is LDS staging in mode 4 and, in mode 5, the register-resident one-kernel iteration with
+    # pipelined tiles (256 and 512 threads); vec 2 is tried only as the register-resident one-kernel
+    # variant with late far loads, and cap 768 only with those two
+    if (vec != 3 or split == 4 or (split == 5 and blk <= 512))
     and (vec != 2 or (split == 5 and blk <= 512))
-    and (cap != 768 or (split == 5 and vec == 2))
-    and (split != 5 or vec in (0, 2))
+    and (cap != 768 or (split == 5 and vec in (2, 3)))
+    and (split != 5 or vec in (0, 2, 3))
 )
 _LAUNCH_CACHE: dict = {}
 

@@ -44,12 +44,39 @@ inline int grid_for(int64_t work, int block, int max_blocks = 4096) {
   return static_cast<int>(g);
 }
 
-// ---- wavefront reductions (wave64, DPP/permute via __shfl_xor) ---------------------------
+// ---- wavefront reductions (wave64; every lane of the wave active) --------------------------
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// v moved across lanes by the DPP control CTRL (rows outside ROW_MASK read 0), as two 32-bit moves
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// The FP64 wave sum by DPP row operations (VALU lane moves, no LDS round trips): pairs and quads
+// (quad_perm [1,0,3,2], [2,3,0,1]), half rows and rows (row_half_mirror, row_mirror), then rows 0+1
+// and 2+3 (row_bcast:15 into rows 1 and 3) and all four (row_bcast:31 into rows 2 and 3); lane 63
+// ends with the sum, which is broadcast.  A fixed order, so the result is deterministic.
+template <>
+__device__ __forceinline__ double wave_sum<double>(double v) {
+  v += dpp_move<0xB1, 0xF>(v);
+  v += dpp_move<0x4E, 0xF>(v);
+  v += dpp_move<0x141, 0xF>(v);
+  v += dpp_move<0x140, 0xF>(v);
+  v += dpp_move<0x142, 0xA>(v);
+  v += dpp_move<0x143, 0xC>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 __device__ __forceinline__ int wave_sum_int(int v) {

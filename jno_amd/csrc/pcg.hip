@@ -59,7 +59,7 @@ struct Tuning {
 // Jacobi-scaled FP32 operator (FP32 inner iterations on a DIA operator; the unscaled split / fused /
 // fused iteration elsewhere)
 enum { MODE_FUSED = 0, MODE_SPLIT = 1, MODE_SINGLE = 2, MODE_SCALED = 3, MODE_SCALED_FUSED = 4, MODE_ONE = 5 };
-// FP32: the register-resident one-kernel iteration with late far loads (k1_vector 2), 256 threads x
+// FP32: the register-resident one-kernel iteration with pipelined tiles (k1_vector 3), 256 threads x
 // 768 workgroups (three tiles of ~930 rows each at 128^3); operators it does not take run the
 // LDS-staged one (ONE_FALLBACK below)
 constexpr Tuning ONE_FALLBACK = {512, 0, 512, 5};
@@ -74,7 +74,7 @@ struct CallTuning {
   int pace;
 };
 static std::mutex g_def_mu;
-static CallTuning g_def = {{{512, 0, 1024, 1}, {768, 2, 256, 5}}, 16, 1};
+static CallTuning g_def = {{{512, 0, 1024, 1}, {768, 3, 256, 5}}, 16, 1};
 
 // partial-sum areas (per workgroup): A = K1's p.q; B = K2's (r.z, r.r); P = prep's (rr_true, b.b)
 struct Partials {
@@ -585,6 +585,55 @@ __device__ __forceinline__ bool one_decide(PcgState* st, int64_t k, const double
   return true;
 }
 
+// The state fields one_decide reads, loaded at the start of a launch so that their latency overlaps
+// the partial-sum loads (they are set before the launch: no launch writes a field it reads, g_hist
+// being read only by a resuming launch, which does not write it)
+struct DecideIn {
+  int64_t k_special, maxiter;
+  int32_t special;
+  double g_prev, atol2, rr_replace;
+};
+__device__ __forceinline__ DecideIn decide_in(const PcgState* st, int64_t k) {
+  DecideIn d;
+  d.k_special = st->k_special;
+  d.special = st->special;
+  d.g_prev = st->g_hist[k & 1];
+  d.atol2 = st->atol2;
+  d.maxiter = st->maxiter;
+  d.rr_replace = st->rr_replace;
+  return d;
+}
+// one_decide on prefetched fields (the same decision and the same state writes)
+__device__ __forceinline__ bool one_decide_pre(PcgState* st, const DecideIn& din, int64_t k, const double* in,
+                                               double& alpha, double& beta, int& special) {
+  const double rr = in[0];
+  special = k == din.k_special ? din.special : 0;
+  alpha = 0.0;
+  beta = 0.0;
+  if (special == 2) {
+    beta = rr / din.g_prev;
+  } else if (special == 0) {
+    alpha = rr / in[1];
+    beta = (rr - 2.0 * alpha * in[2] + alpha * alpha * in[3]) / rr;
+  }
+  const bool stop_conv = special != 2 && !(rr > din.atol2);
+  const bool stop_cap = k >= din.maxiter;
+  const bool stop_repl = special == 0 && rr <= din.rr_replace;
+  const bool bad = !isfinite(rr) || (!stop_conv && (!isfinite(alpha) || !isfinite(beta)));
+  if (stop_conv || stop_cap || stop_repl || bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->k_stop = k;
+      st->rr_stop = rr;
+      st->status = bad ? ST_NONFINITE : stop_conv ? ST_CONVERGED : stop_cap ? ST_MAXITER : ST_REPLACE;
+      st->done1 = 1;
+      st->done = 1;
+    }
+    return false;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && special != 2) st->g_hist[k & 1] = rr;
+  return true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // One kernel per iteration on the Jacobi-scaled DIA operator (iteration mode 5).  Launch k reads
 // r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1}, x and the partial sums of launch k-1: gamma_{k-1} = r.r
@@ -863,7 +912,10 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   using Q = V16<float>;
   __shared__ double sm_w[4 * NW];
   __shared__ double sm_out[4 * NW];
-  __shared__ __align__(16) float pt[TT + 2 * LDS_HALO];  // p_k of the tile and its near halo
+  // p_k of the tile and its near halo, two buffers used by alternate tiles: a tile's writes then
+  // need no barrier behind the previous tile's SpMV (the buffer they overwrite was last read two
+  // tiles back, before the barrier of the tile between)
+  __shared__ __align__(16) float ptb[2][TT + 2 * LDS_HALO];
   PcgState* st = a.st;
   const int G = gridDim.x;
   const int tid = (int)threadIdx.x;
@@ -970,7 +1022,9 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   Tile c;
   int ft0 = 0, ft1 = 0;
   if (nt > 0) tile_at(0, ft0, ft1);
-  if (ft0 < ft1) load_tile(c, ft0, ft1);
+  // issued unconditionally (a workgroup without rows loads row 0, unused): with the same loads in
+  // flight on every path, the head's wait for its partial sums does not also wait for these
+  load_tile(c, ft0 < ft1 ? ft0 : 0, ft0 < ft1 ? ft1 : 0);
   // the stop flag is read after the first loads are issued, so its latency overlaps them (a launch
   // past a stop exits with its loads unused: measured 0.35 us less per working launch, 1% less per
   // 128^3 solve, against the extra traffic of the exiting ones)
@@ -1013,10 +1067,10 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
     tile_at(t, t0, t1);
     if (t0 >= t1) break;  // tiles past the last row (the same for every thread)
     if (t > 0) {
-      __syncthreads();  // the previous tile's SpMV has read pt
       load_tile(c, t0, t1);
       load_late(c, upd);
     }
+    float* pt = ptb[t & 1];
     const int t1q = t0 + ((t1 - t0 + 3) & ~3);
     const int i = c.i;
     // r_k, p_k of the own quad; p_k of the halo rows and of the far neighbours
@@ -1084,6 +1138,280 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
       st16(a.q_out + i, qn);
     }
   }
+  store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The one-kernel iteration with register-resident rows and software-pipelined tiles (mode 5,
+// k1_vector 3), for 4-diagonal operators.  Same arithmetic, bounds handling, tiling and launch
+// contract as pcg_one_q; the difference is when each tile's loads are issued.  A tile's loads come
+// in three groups: A (r, q, p of the own quad and of the near halo), F (r, q, p of the far
+// neighbours and x) and B (the upper and lower off-diagonal values).  Phase 1 of a tile (r_k, p_k,
+// x, the far neighbours' p_k) needs A and F; phase 2 (q = A p_k from LDS, the partial sums) needs
+// B.  The next tile's A and F are issued right after phase 1 (before the barrier), so they are in
+// flight during the barrier and phase 2; the next tile's B is issued after phase 2, in flight during
+// the next phase 1.  A workgroup thus always has loads outstanding, instead of alternating a load
+// round trip with a compute phase on every tile.  The far-diagonal terms move to phase 2 (their
+// p_k, two quads, cross the barrier in registers), so phase 1 holds no diagonal values.
+// ---------------------------------------------------------------------------------------------
+template <int NT, int ND, int NN>
+__global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
+  static_assert(ND == 4 && NN >= 2 && NN <= 3, "instantiated for 4-diagonal operators with 2 or 3 near diagonals");
+  constexpr int NF = ND - 1 - NN;  // far off-diagonals
+  constexpr int NFA = NF > 0 ? NF : 1;
+  constexpr int TT = 4 * NT;  // rows per tile
+  constexpr int NW = NT / 64;
+  constexpr int QRED_H = (2 * QRED_G + NT - 1) / NT;  // pairs of partial sums loaded per thread
+  constexpr int UH = (2 * LDS_HALO + NT - 1) / NT;    // near halo rows per thread
+  using Q = V16<float>;
+  __shared__ double sm_w[4 * NW];
+  __shared__ double sm_out[4 * NW];
+  // p_k of the tile and its near halo, two buffers used by alternate tiles (the buffer a tile
+  // writes was last read two tiles back, before the barrier of the tile between)
+  __shared__ __align__(16) float ptb[2][TT + 2 * LDS_HALO];
+  PcgState* st = a.st;
+  const int G = gridDim.x;
+  const int tid = (int)threadIdx.x;
+  const int64_t k = a.kk;
+  const int n = (int)a.n;
+  const float* vals = a.vals;
+  const int ld = (int)dia_ld(a.n);
+  int o[ND];
+#pragma unroll
+  for (int d = 1; d < ND; ++d) o[d] = a.off.o[d];
+  const int hq = (o[NN] + 3) & ~3;  // near halo rows on either side, whole quads (<= LDS_HALO)
+  // tiles as in pcg_one_q: round-robin within whole XCD slabs when the launch is sized for it
+  const int64_t rpb = a.rows_per_block;
+  const int ntq = (int)((rpb + TT - 1) / TT);
+  const bool ilv = G % N_XCD == 0 && rpb % ntq == 0 && (rpb / ntq) % 4 == 0;
+  int nt, tl, lim;
+  int64_t tbase, tstride;
+  if (ilv) {
+    nt = ntq;
+    tl = (int)(rpb / ntq);
+    const int per = G / N_XCD;
+    tbase = (int64_t)(blockIdx.x % N_XCD) * per * rpb + (int64_t)(blockIdx.x / N_XCD) * tl;
+    tstride = (int64_t)per * tl;
+    lim = n;
+  } else {
+    const int64_t r0l = block_first_row(rpb);
+    const int r0 = (int)min(r0l, a.n), r1 = (int)min(a.n, r0l + rpb);
+    nt = r1 > r0 ? (r1 - r0 + TT - 1) / TT : 0;
+    tl = nt > 0 ? (((r1 - r0 + nt - 1) / nt + 3) & ~3) : TT;
+    tbase = r0;
+    tstride = tl;
+    lim = r1;
+  }
+  auto tile_at = [&](int t, int& t0, int& t1) {
+    const int64_t b = tbase + (int64_t)t * tstride;
+    t0 = (int)min(b, (int64_t)lim);
+    t1 = (int)min(b + tl, (int64_t)lim);
+  };
+  const double* pin = a.part.C + (k & 1) * 4 * (int64_t)G;
+  const int G4 = 4 * G;
+  // the partial sums as 16-byte pairs (G is even: a pair never spans two arrays), entries
+  // 2 (u NT + tid) and the next in thread tid
+  double2 pv_in[QRED_H];
+#pragma unroll
+  for (int u = 0; u < QRED_H; ++u) pv_in[u] = *reinterpret_cast<const double2*>(pin + min(2 * (u * NT + tid), G4 - 2));
+  const DecideIn din = decide_in(st, k);
+  // every wave's head loads are queued before any wave of the workgroup issues its tile's: the head
+  // waits for them, the tile's loads have time until the head is done
+  __syncthreads();
+  struct TA {  // group A: the own quad and the near halo rows
+    Q rv, qv, pv;
+    float hr[UH], hqv[UH], hp[UH];
+    int i, t0, t1;
+    bool own, hin[UH];
+  };
+  struct TF {  // group F: the far neighbours and x
+    Q fur[NFA], fuq[NFA], fup[NFA], fdr[NFA], fdq[NFA], fdp[NFA], xv;
+  };
+  struct TB {  // group B: the off-diagonal values
+    Q vu[ND], vl[ND];
+  };
+  auto load_a = [&](TA& c, int t0, int t1) {
+    c.t0 = t0;
+    c.t1 = t1;
+    c.i = t0 + 4 * tid;
+    c.own = c.i < t1;
+    const int ic = c.own ? c.i : 0;
+    c.rv = ld16(a.r + ic);
+    c.qv = ld16(a.q + ic);
+    c.pv = ld16(a.p_old + ic);
+    const int t1q = t0 + ((t1 - t0 + 3) & ~3);
+#pragma unroll
+    for (int u = 0; u < UH; ++u) {
+      const int h = tid + u * NT;
+      const int row = h < hq ? t0 - hq + h : t1q + (h - hq);
+      c.hin[u] = h < 2 * hq && row >= 0 && row < n;
+      const int hc = c.hin[u] ? row : 0;
+      c.hr[u] = a.r[hc];
+      c.hqv[u] = a.q[hc];
+      c.hp[u] = a.p_old[hc];
+    }
+  };
+  auto load_f = [&](TF& c, const TA& ca) {
+    const int ic = ca.own ? ca.i : 0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int od = o[NN + 1 + f];
+      const int ju = ic + od < n ? ic + od : 0;
+      const int jd = ic - od >= -3 ? ic - od : 0;
+      c.fur[f] = ld16(a.r + ju);
+      c.fuq[f] = ld16(a.q + ju);
+      c.fup[f] = ld16(a.p_old + ju);
+      c.fdr[f] = ld16(a.r + jd);
+      c.fdq[f] = ld16(a.q + jd);
+      c.fdp[f] = ld16(a.p_old + jd);
+    }
+    c.xv = ld16(a.x + ic);  // used only by an updating launch (every launch but the special ones)
+  };
+  auto load_b = [&](TB& c, const TA& ca) {
+    const int ic = ca.own ? ca.i : 0;
+#pragma unroll
+    for (int d = 1; d < ND; ++d) {
+      const float* vd = vals + (int64_t)d * ld;
+      const int jl = ic - o[d];
+      c.vu[d] = ld16(vd + ic);
+      c.vl[d] = ld16(vd + (jl >= -3 ? jl : 0));
+    }
+  };
+  TA ca;
+  TF cf;
+  TB cb;
+  ca.t0 = ca.t1 = 0;
+  int ft0 = 0, ft1 = 0;
+  if (nt > 0) tile_at(0, ft0, ft1);
+  const bool any = ft0 < ft1;
+  // issued unconditionally (a workgroup without rows loads row 0, unused): with the same loads in
+  // flight on every path, the head's wait for its partial sums does not also wait for these
+  load_a(ca, any ? ft0 : 0, any ? ft1 : 0);
+  load_b(cb, ca);
+  load_f(cf, ca);
+  // the stop flag is read after the first loads are issued, as in pcg_one_q
+  if (st->done) return;
+  double in[4];
+  {
+    const int w = tid >> 6;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < QRED_H; ++u) {
+      const int f = 2 * (u * NT + tid);
+      const double v = pv_in[u].x + pv_in[u].y;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) s[m] += f >= m * G && f < (m + 1) * G ? v : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double t = wave_sum(s[m]);
+      if ((tid & 63) == 0) sm_w[m * NW + w] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double t = 0.0;
+#pragma unroll
+      for (int v = 0; v < NW; ++v) t += sm_w[m * NW + v];
+      in[m] = t;
+    }
+  }
+  double alpha, beta;
+  int special;
+  if (!one_decide_pre(st, din, k, in, alpha, beta, special)) return;
+  const float al = (float)alpha, be = (float)beta;
+  const bool upd = special == 0;
+  double part[4] = {0.0, 0.0, 0.0, 0.0};
+  // one tile: phase 1, the next tile's A and F, the barrier, phase 2, the next tile's B; returns
+  // whether a next tile follows.  The loads are issued unconditionally (the last tile reloads its own
+  // rows, unused), so that every path has the same loads in flight and the waits before each use
+  // count exactly the loads issued after the one it needs.
+  auto step = [&](int t) -> bool {
+    const int t0 = ca.t0, t1 = ca.t1;
+    float* pt = ptb[t & 1];
+    const int t1q = t0 + ((t1 - t0 + 3) & ~3);
+    const int i = ca.i;
+    const bool own = ca.own;
+    // phase 1: r_k, p_k of the own quad, x; p_k of the halo rows and of the far neighbours
+    Q rn, pn;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      rn.v[e] = ca.rv.v[e] - al * ca.qv.v[e];
+      pn.v[e] = rn.v[e] + be * ca.pv.v[e];
+    }
+    if (own) {
+      st16(a.r_out + i, rn);
+      st16(a.p_new + i, pn);
+      *reinterpret_cast<Q*>(&pt[hq + 4 * tid]) = pn;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) part[0] += (double)rn.v[e] * (double)rn.v[e];
+    }
+#pragma unroll
+    for (int u = 0; u < UH; ++u) {
+      const int h = tid + u * NT;
+      if (h < 2 * hq)
+        pt[h < hq ? h : hq + (t1q - t0) + (h - hq)] =
+            ca.hin[u] ? (ca.hr[u] - al * ca.hqv[u]) + be * ca.hp[u] : 0.0f;
+    }
+    Q fu[NFA], fd[NFA];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        fu[f].v[e] = (cf.fur[f].v[e] - al * cf.fuq[f].v[e]) + be * cf.fup[f].v[e];
+        fd[f].v[e] = (cf.fdr[f].v[e] - al * cf.fdq[f].v[e]) + be * cf.fdp[f].v[e];
+      }
+    if (own && upd) {
+      Q xn;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xn.v[e] = cf.xv.v[e] + al * ca.pv.v[e];
+      st16(a.x + i, xn);
+    }
+    // the next tile's groups A and F go out now, in flight during the barrier and phase 2
+    int n0 = 0, n1 = 0;
+    if (t + 1 < nt) tile_at(t + 1, n0, n1);
+    const bool more = n0 < n1;
+    load_a(ca, more ? n0 : t0, more ? n1 : t1);
+    load_f(cf, ca);
+    __syncthreads();
+    // phase 2: q = A p_k (near diagonals from LDS, far ones from fu / fd), partial sums
+    if (own) {
+      Q qn;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int li = hq + 4 * tid + e;
+        float s = pn.v[e];
+#pragma unroll
+        for (int d = NN + 1; d < ND; ++d) {
+          const int f = d - NN - 1;
+          const float cl = i - o[d] + e >= 0 ? cb.vl[d].v[e] : 0.0f;
+          s += cb.vu[d].v[e] * fu[f].v[e];
+          s += cl * fd[f].v[e];
+        }
+#pragma unroll
+        for (int d = 1; d <= NN; ++d) {
+          const float cl = i - o[d] + e >= 0 ? cb.vl[d].v[e] : 0.0f;
+          s += cb.vu[d].v[e] * pt[li + o[d]];
+          s += cl * pt[li - o[d]];
+        }
+        s = i + e < n ? s : 0.0f;
+        qn.v[e] = s;
+        part[1] += (double)pn.v[e] * (double)s;
+        part[2] += (double)s * (double)rn.v[e];
+        part[3] += (double)s * (double)s;
+      }
+      st16(a.q_out + i, qn);
+    }
+    // the next tile's group B, in flight during its phase 1
+    load_b(cb, ca);
+    return more;
+  };
+  // the first tiles unrolled (straight-line code keeps the waits exact), any further ones in a loop
+  bool go = any;
+#pragma unroll
+  for (int t = 0; t < 3; ++t)
+    if (go) go = step(t);
+  for (int t = 3; go; ++t) go = step(t);
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
 }
 
@@ -1888,12 +2216,12 @@ static int cu_count() {
 // k1_vector 1 or 2) apply to the 4-diagonal operators it is built for, at 256 or 512 threads; any
 // other operator (BSR3, CSR, 8-diagonal DIA) runs the previous default shape instead.
 static Tuning effective_tuning(const Tuning& tu, const jno_op* op) {
-  if (tu.split == MODE_ONE && (tu.vec == 1 || tu.vec == 2) && !(quad_ok(op) && tu.block <= 512)) return ONE_FALLBACK;
+  if (tu.split == MODE_ONE && tu.vec >= 1 && !(quad_ok(op) && tu.block <= 512)) return ONE_FALLBACK;
   return tu;
 }
 static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
   const int64_t n = op->n;
-  if (tu.split == MODE_ONE && (tu.vec == 1 || tu.vec == 2)) {
+  if (tu.split == MODE_ONE && tu.vec >= 1) {
     // pcg_one_q: one tile of 4 block rows per workgroup where the grid cap allows (rounded up to
     // whole XCD slabs), else several equal tiles per workgroup
     // All workgroups are resident at once (at most 4 of 256 threads per CU); a grid of whole
@@ -2138,8 +2466,17 @@ static int launch_quad(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipSt
     switch (key) {
       case 9: pcg_one_q<NT, 4, 2><<<grid, NT, 0, s>>>(a); break;
       case 10: pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a); break;
+      // the pipelined kernel reads the partial sums as pairs (an even grid); an odd one runs LF
+      case 11:
+        if (grid % 2) pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a);
+        else pcg_one_p<NT, 4, 2><<<grid, NT, 0, s>>>(a);
+        break;
       case 13: pcg_one_q<NT, 4, 3><<<grid, NT, 0, s>>>(a); break;
-      default: pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a); break;
+      case 14: pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a); break;
+      default:
+        if (grid % 2) pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a);
+        else pcg_one_p<NT, 4, 3><<<grid, NT, 0, s>>>(a);
+        break;
     }
     JNO_LAUNCH_CHECK();
     return JNO_OK;
@@ -2153,7 +2490,7 @@ static int launch_iter_nt(const jno_op* op, PcgArgs<T, V>& a, int grid, hipStrea
   int rc;
   if (a.unit && a.split == MODE_ONE) {
     if (ev) JNO_HIP(hipEventRecord(ev[0], s));
-    if ((a.vec == 1 || a.vec == 2) && NT <= 512 && quad_ok(op))
+    if (a.vec >= 1 && NT <= 512 && quad_ok(op))
       rc = launch_quad<NT, T, V>(op, a, grid, s);
     else
       rc = launch_lds<NT, T, V>(true, op, a, grid, s);
@@ -2479,7 +2816,7 @@ static int check_shape(const jno_launch_shape& t, const char* who) {
   JNO_REQUIRE(t.k1_vector >= 0 && t.k1_vector <= 3, JNO_EINVAL,
               "%s: k1_vector must be 0 (2 rows per thread), 1 (16-byte tiles), 2 (4 rows) or 3 (p staged in LDS, "
               "iteration mode 4); with mode 5: 0 the LDS-staged kernel, 1 and 2 the register-resident one (far "
-              "loads before / after the head)",
+              "loads before / after the head), 3 the register-resident one with pipelined tiles",
               who);
   return JNO_OK;
 }

@@ -29,7 +29,7 @@ class Tuning:
     (jno/utils/solver/solver_api.py:261-308), so it can be part of a solver's key."""
 
     fp64: tuple = (1024, 512, 0, 1)
-    mixed: tuple = (256, 768, 2, 5)
+    mixed: tuple = (256, 768, 3, 5)
     chunk: int = 16
     pacing: bool = True
 

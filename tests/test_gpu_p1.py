@@ -432,7 +432,7 @@ def test_one_kernel_non_cubic_boxes(dev, dims, hi, ndiag):
     Ao, bo, _ = O.poisson_p1_system(p, t, k)
     uo = O.direct_solve(Ao, bo)
     base = solve.Tuning.defaults()
-    for shape in (None, (512, 512, 0, 5), (256, 768, 1, 5), (256, 768, 2, 5), (512, 64, 2, 5)):
+    for shape in (None, (512, 512, 0, 5), (256, 768, 1, 5), (256, 768, 2, 5), (256, 768, 3, 5), (512, 64, 2, 5), (512, 64, 3, 5)):
         tun = base.with_shape("mixed", *shape) if shape is not None else None
         sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=0.01, tuning=tun)
         u = sv(A, b, M=precond.jacobi())
@@ -458,8 +458,9 @@ def test_lds_staged_k1(dev, n):
     its = {}
     # (3, 4): K1 with p staged in LDS; (0, 5): one kernel per iteration, LDS-staged; (1, 5) and
     # (2, 5): one kernel per iteration with register-resident 4-row quads, far neighbours loaded
-    # before or after the head (256 and 512 threads; 1024 falls back to the LDS-staged kernel)
-    for vec, mode in ((3, 4), (0, 5), (1, 5), (2, 5)):
+    # before or after the head (256 and 512 threads; 1024 falls back to the LDS-staged kernel);
+    # (3, 5): the same with pipelined tiles (several tiles per workgroup at the small grid caps)
+    for vec, mode in ((3, 4), (0, 5), (1, 5), (2, 5), (3, 5)):
         for blk, cap in ((1024, 8), (256, 64), (1024, 512), (512, 8), (256, 4096)):
             for rf in (0.1, 0.01):
                 sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=rf,
