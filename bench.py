@@ -152,10 +152,12 @@ def main():
     else:
         spmv_mat = op.nnz_stored * (4 + 4) + 4 * (N + 1)
     if mode == 5 and op.format == "dia":  # one kernel per iteration on the Jacobi-scaled operator
-        # reads r, q, p, x and the stored upper diagonals; writes r, p, q, x.  k1_vector 1 and 2 run
-        # the register-resident variant on the 4-diagonal operators with 2 or 3 near diagonals
-        quad = vec in (1, 2) and blk <= 512 and op.ndiag == 4 and 2 <= sum(o <= 256 for o in op.offsets[1:]) <= 3
-        kname = "pcg_one_q<dia, f32, scaled>" if quad else "pcg_one<dia, f32, scaled>"
+        # reads r, q, p, x and the stored upper diagonals; writes r, p, q, x.  k1_vector 1, 2 and 3
+        # run the register-resident variants on the 4-diagonal operators with 2 or 3 near diagonals
+        # (3: pipelined tiles, on an even grid)
+        quad = vec in (1, 2, 3) and blk <= 512 and op.ndiag == 4 and 2 <= sum(o <= 256 for o in op.offsets[1:]) <= 3
+        kname = ("pcg_one_p<dia, f32, scaled>" if vec == 3 else "pcg_one_q<dia, f32, scaled>") if quad \
+            else "pcg_one<dia, f32, scaled>"
         kernels = [(kname, ms[1], N * 4 * (op.ndiag - 1) + N * (4 * 4 + 4 * 4))]
     elif mode == 4 and op.format == "dia":  # K1 K2 on the Jacobi-scaled operator: no dinv, unit diagonal
         k1 = "pcg_k1_lds<dia, f32, scaled>" if vec == 3 else "pcg_k1<dia, f32, scaled>"  # vec 3: p staged in LDS
