@@ -1421,6 +1421,18 @@ struct PadList {
   int count;
 };
 
+// Zero the vectors of the list whole (with their pads) when bit b of `whole` is set, else only
+// their pads: one launch for the set-up of a scaled FP32 solve
+__global__ void zero_list_k(int64_t n, PadList l, int whole) {
+  constexpr int P = PCG_PAD_BYTES / (int)sizeof(float);
+  for (int b = 0; b < l.count; ++b) {
+    float* v = l.v[b];
+    const int64_t lo = -P, hi = (whole >> b) & 1 ? n + P : P;  // [-P, P) are the two pads, shifted
+    for (int64_t j = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < hi; j += (int64_t)gridDim.x * blockDim.x)
+      v[(whole >> b) & 1 ? j : (j < 0 ? j : n + j)] = 0.0f;
+  }
+}
+
 __global__ void zero_pads_k(int64_t n, PadList l) {
   constexpr int P = PCG_PAD_BYTES / (int)sizeof(float);
   const int t = (int)threadIdx.x;  // 2 P threads
@@ -1973,9 +1985,11 @@ __global__ void __launch_bounds__(PCG_NT) residual_prep_k(int64_t n, int64_t row
 }
 
 // one workgroup: reduce the prep partials into the state and set the tolerances
+// check_scale (a scaled FP32 solve's first call): when scale_dia_k flagged the operator, stop before
+// the first iteration with ST_UNSCALED instead
 __global__ void __launch_bounds__(PCG_NT) finalize_k(PcgState* st, Partials part, int G, double tol, double atol,
                                                      int64_t maxiter, double replace2, int first, int restart,
-                                                     int single_reduction) {
+                                                     int single_reduction, int check_scale = 0) {
   __shared__ double sm[PCG_NT / 64];
   const double rt = reduce_partials(part.P, G, sm);
   const double bb = reduce_partials(part.P + G, G, sm);
@@ -2001,6 +2015,13 @@ __global__ void __launch_bounds__(PCG_NT) finalize_k(PcgState* st, Partials part
     st->outer_done = 0;
   }
   st->rr_replace = replace2 > 0.0 ? replace2 * rr : -1.0;
+  if (check_scale && st->scale_bad) {
+    st->done = 1;
+    st->done1 = 1;
+    st->status = ST_UNSCALED;
+    st->k_stop = 0;
+    st->outer_done = 1;
+  }
 }
 
 template <typename T>
@@ -3027,15 +3048,12 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
   const auto t0 = std::chrono::steady_clock::now();
   DiaOffsets off;
   for (int d = 0; d < JNO_DIA_MAX; ++d) off.o[d] = h_op->offsets[d];
-  int* bad = reinterpret_cast<int*>(w.part.P);  // scratch flag; P is rewritten by the preparation
+  // the check of the scaled main diagonal stays on the device: finalize_k stops the solve before its
+  // first iteration when scale_dia_k flagged it, and the host learns that at its first poll
+  int* bad = &w.st->scale_bad;
   JNO_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
   scale_dia_k<<<gz, PCG_NT, 0, s>>>(n, h_op->ndiag, off, h_op->vals64, d_dinv, w.sdia, bad);
   JNO_LAUNCH_CHECK();
-  int h_bad = 0;
-  JNO_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
-  JNO_HIP(hipStreamSynchronize(s));
-  if (h_bad) return JNO_OK;
-  *used = true;
   // one kernel per iteration: r, q (and p) alternate between two buffers by launch parity
   const bool one = tu.split == MODE_ONE && lds_ok(h_op) && w.q2;
   float* rs = (float*)w.r;
@@ -3043,15 +3061,13 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
   float* rb[2] = {rs, (float*)w.s};
   float* qb[2] = {(float*)w.q, (float*)w.q2};
   float* xs = (float*)w.xs;
-  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[0], n);
-  zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[1], n);
-  zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
-  if (one) {
-    zero_k<float><<<gz, PCG_NT, 0, s>>>(rb[1], n);
-    zero_k<float><<<gz, PCG_NT, 0, s>>>(qb[0], n);
-    zero_k<float><<<gz, PCG_NT, 0, s>>>(qb[1], n);
-    PadList pl = {{rb[0], rb[1], pb[0], pb[1], qb[0], qb[1], xs, nullptr}, 7};
-    zero_pads_k<<<1, 2 * PCG_PAD_BYTES / (int)sizeof(float), 0, s>>>(n, pl);
+  if (one) {  // p, x, the second r buffer and both q buffers whole, the first r buffer's pads
+    PadList pl = {{pb[0], pb[1], xs, rb[1], qb[0], qb[1], rb[0], nullptr}, 7};
+    zero_list_k<<<gz, PCG_NT, 0, s>>>(n, pl, 0x3F);
+  } else {
+    zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[0], n);
+    zero_k<float><<<gz, PCG_NT, 0, s>>>(pb[1], n);
+    zero_k<float><<<gz, PCG_NT, 0, s>>>(xs, n);
   }
   int rc;
   if (use_x0) {
@@ -3062,7 +3078,7 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
   }
   residual_prep_scaled_k<<<G, PCG_NT, 0, s>>>(n, rpb, d_b, use_x0 ? w.ax : nullptr, d_dinv, rs, 1, w.part, one ? w.part.C : nullptr);
   // inner tolerance 0: the FP64 residual below decides convergence
-  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 1, 1, 0);
+  finalize_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, 0.0, 0.0, maxiter, repl2, 1, 1, 0, 1);
   JNO_LAUNCH_CHECK();
   if (one) {
     set_special_k<<<1, 64, 0, s>>>(w.st, 0, 1);
@@ -3083,6 +3099,8 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
     rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr,
                                  one && ct.pace ? &pace : nullptr);
     if (rc) return rc;
+    if (f.status == ST_UNSCALED) return JNO_OK;  // *used stays false: nothing the caller relies on changed
+    *used = true;
     if (f.outer_done) break;
     k = f.k_stop;
     // one-kernel iterations resume by rerunning launch k-1 (from p_{k-2}, which launch k did not

@@ -7,13 +7,15 @@ namespace jno {
 
 constexpr int N_XCD = 8;
 
-enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4 };
+// ST_UNSCALED: a scaled FP32 solve found its operator's scaled main diagonal not exactly 1 and
+// stopped before its first iteration (the unscaled inner iteration runs instead)
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_NONFINITE = 3, ST_REPLACE = 4, ST_UNSCALED = 5 };
 
 struct PcgState {
   int32_t done;    // written by K2 (block 0), read by K1
   int32_t done1;   // written by K1 (block 0), read by K2
   int32_t status;  // why K1 stopped
-  int32_t pad;
+  int32_t scale_bad;  // scaled FP32 solves: set when the scaled main diagonal is not 1 (scale_dia_k)
   int64_t k;       // iterations done (written by K2 block 0)
   int64_t k_stop;  // iteration at which K1 stopped
   double rr_stop;     // recursive r.r at the stop

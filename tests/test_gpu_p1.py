@@ -770,3 +770,38 @@ def test_plan_cache_is_keyed_on_the_cells_content(dev):
     Aj, bj = J.fem_poisson(mj).evaluate({"k": torch.tensor(k, device="cuda")})
     Ao, bo, _ = O.poisson_p1_system(pj, t, k)
     assert _rel(_np(sv(Aj, bj, M=precond.jacobi())), O.direct_solve(Ao, bo)) <= SOL_TOL
+
+
+@pytest.mark.parametrize("fmt", ["dia", "csr"])
+def test_mixed_solve_with_a_zero_diagonal_row(dev, fmt):
+    """A row and column of zeros (b = 0 there): the Jacobi guard leaves the row unscaled
+    (precond.py:107, 1/d only for |d| > 1e-30), so the Jacobi-scaled FP32 iteration does not apply
+    (its scaled main diagonal would not be 1) and the solve falls back to the unscaled one, which
+    the device finds before the first iteration.  The solution equals the oracle's on the remaining
+    rows, 0 on the zero row, in both precisions."""
+    from jno_amd import precond, solve
+    from jno_amd.fem import CSR
+
+    n = 7
+    p, t = O.kuhn_box(n)
+    k = O.k_lognormal(t.shape[0], 41)
+    m, s, A, b = _system(n, k)
+    N = A.shape[0]
+    j = N // 2 + 3
+    rp, col, v = _np(A.rowptr), _np(A.col), _np(A.vals).copy()
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    v[(rows == j) | (col == j)] = 0.0
+    bj = _np(b).copy()
+    bj[j] = 0.0
+    Az = CSR(A.rowptr, A.col, torch.tensor(v, device="cuda"), A.shape)
+    bz = torch.tensor(bj, device="cuda")
+    keep = np.arange(N) != j
+    Ao = O.csr_matrix(rp, col, v, N)[keep][:, keep]
+    uo = np.zeros(N)
+    uo[keep] = O.direct_solve(Ao.tocsr(), bj[keep])
+    for prec in ("mixed", "fp64"):
+        sv = solve.cg(tol=1e-10, precision=prec, format=fmt)
+        u = _np(sv(Az, bz, M=precond.jacobi()))
+        assert sv.last_stats["relres"] <= 1e-10, (prec, sv.last_stats)
+        assert u[j] == 0.0
+        assert _rel(u, uo) <= SOL_TOL, (prec, fmt)
