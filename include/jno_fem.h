@@ -64,6 +64,16 @@ int jno_pattern_fill(const int32_t* d_cells, int64_t n_cells, int nloc, int vec,
                      const int32_t* d_inc_ptr, const int32_t* d_inc, const int32_t* d_rowptr,
                      int32_t* d_col, void* stream);
 
+/* The same plan for a structured Kuhn box (the mesh jno_mesh_kuhn_box makes, nx, ny, nz >= 2), in
+ * closed form: the rowptr, col, inc_ptr and inc jno_pattern_count/fill produce for that mesh, in
+ * one pass with no sort (no reference counterpart as a separate entry: compress_plan specialized
+ * to the lattice).  jno_pattern_kuhn_box_nnz gives nnz and the longest row (15) for the
+ * allocation; d_col: i32[nnz], d_inc: i32[24 nx ny nz].  The caller guarantees the mesh is the box
+ * (jno_mesh_is_kuhn_box). */
+int jno_pattern_kuhn_box_nnz(int nx, int ny, int nz, int64_t* h_nnz, int32_t* h_max_row);
+int jno_pattern_kuhn_box(int nx, int ny, int nz, int32_t* d_rowptr, int32_t* d_col, int32_t* d_inc_ptr,
+                         int32_t* d_inc, void* stream);
+
 /* Content key of an index array (the plan cache keys patterns on the cells' content, as
  * compress_plan memoizes on a digest of its index array, jno/utils/solver/fem_utils.py:4027-4060):
  * h_hash[2] = two independent 64-bit sums of a position-dependent mix of every entry; and an
@@ -83,6 +93,9 @@ int jno_boundary_mask(const int32_t* d_cells, int64_t n_cells, int nloc, int64_t
                       const int32_t* d_inc_ptr, const int32_t* d_inc, uint8_t* d_mask, void* stream);
 int jno_mask_plane(const double* d_points, int64_t n_nodes, int axis, double value,
                    const uint8_t* d_mask_in, uint8_t* d_mask_out, void* stream);
+/* jno_boundary_mask's result for a structured Kuhn box in closed form: the nodes on the box
+ * surface (the caller guarantees the mesh is the box). */
+int jno_mask_kuhn_box(int nx, int ny, int nz, uint8_t* d_mask, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
  * A4-A6, A8, A9, A11, A14, A19  P1 scalar assembly.  Replaces FemLinearSystem.evaluate

@@ -119,6 +119,9 @@ SIGNATURES = {
     "jno_assemble_p1": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, P]),
     "jno_assemble_p1_dia": (C.c_int, [C.c_int, P, P, I64, I64, P, P, P, P, C.c_int, P, D, P, P, P, P, P, P, C.c_int,
                                       C.POINTER(I32), P, P, C.POINTER(C.c_int), P, SZ, P]),
+    "jno_pattern_kuhn_box_nnz": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I32)]),
+    "jno_pattern_kuhn_box": (C.c_int, [C.c_int, C.c_int, C.c_int, P, P, P, P, P]),
+    "jno_mask_kuhn_box": (C.c_int, [C.c_int, C.c_int, C.c_int, P, P]),
     "jno_box_geometry": (C.c_int, [C.c_int, C.c_int, C.c_int, P, C.POINTER(D), P, SZ, P]),
     "jno_mesh_is_kuhn_box": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(D), C.POINTER(D), P, P,
                                        C.POINTER(C.c_int), P, SZ, P]),

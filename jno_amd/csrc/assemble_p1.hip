@@ -8,9 +8,8 @@
 // Every value is written exactly once; the summation order per entry is fixed, so results are
 // bit-reproducible run to run.  (Half a wavefront per row, one lane per incident element with the
 // contributions added in the same order through LDS, measured twice as slow.)
-#include <type_traits>
-
 #include "common.h"
+#include "kuhn.h"
 #include "sparse.h"
 
 namespace jno {
@@ -216,54 +215,6 @@ struct BoxGeom {
   double dot[6][4][4];
 };
 
-__host__ __device__ constexpr int kuhn_corner(int t, int a) {
-  constexpr int K[6][4] = {{0, 4, 6, 7}, {0, 4, 5, 7}, {0, 2, 6, 7}, {0, 2, 3, 7}, {0, 1, 5, 7}, {0, 1, 3, 7}};
-  return K[t][a];
-}
-
-// stencil slot of the offset (di, dj, dk): its rank among the 15 in lexicographic order
-__host__ __device__ constexpr int stencil_slot(int di, int dj, int dk) {
-  int r = 0;
-  for (int a = -1; a <= 1; ++a)
-    for (int b = -1; b <= 1; ++b)
-      for (int c = -1; c <= 1; ++c) {
-        const bool pos = a >= 0 && b >= 0 && c >= 0, neg = a <= 0 && b <= 0 && c <= 0;
-        if (!pos && !neg) continue;
-        if (a == di && b == dj && c == dk) return r;
-        ++r;
-      }
-  return -1;
-}
-__host__ __device__ constexpr int slot_di(int s) {
-  int r = 0;
-  for (int a = -1; a <= 1; ++a)
-    for (int b = -1; b <= 1; ++b)
-      for (int c = -1; c <= 1; ++c) {
-        const bool pos = a >= 0 && b >= 0 && c >= 0, neg = a <= 0 && b <= 0 && c <= 0;
-        if (!pos && !neg) continue;
-        if (r++ == s) return a * 9 + b * 3 + c;  // packed (a, b, c)
-      }
-  return 0;
-}
-
-// q-th incident (voxel corner c, type t, local vertex a) of a node, in ascending element order:
-// voxel (i - di, j - dj, k - dk) of corner c = (di, dj, dk) has the larger id the smaller c is, so
-// corners descend from 7, and types ascend within a voxel
-struct BoxInc {
-  int c, t, a;
-};
-__host__ __device__ constexpr BoxInc box_inc(int q) {
-  int n = 0;
-  for (int c = 7; c >= 0; --c)
-    for (int t = 0; t < 6; ++t)
-      for (int a = 0; a < 4; ++a)
-        if (kuhn_corner(t, a) == c) {
-          if (n == q) return BoxInc{c, t, a};
-          ++n;
-        }
-  return BoxInc{-1, -1, -1};
-}
-
 struct BoxArgs {
   int nx, ny, nz;
   BoxGeom geo;
@@ -284,36 +235,6 @@ struct BoxArgs {
 
 constexpr int BOX_BLOCK = 256;
 constexpr int BOX_GEO = 6 * 17;  // |T| of the six types, then their 4x4 dot products
-
-// the incidence Q of the row's node (i, j, kk): its element's contributions to the 15 stencil slots
-// and to the load.  An incidence outside the box (a boundary row) is masked: it reads a clamped k
-// and adds +0.0, which leaves every sum bitwise as the unmasked kernel's (sums start at +0.0, so no
-// -0.0 arises either way).
-// f(std::integral_constant<int, B>), ..., f(std::integral_constant<int, E - 1>): every index a
-// compile-time constant inside f
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
-// (di, dj, dk) of stencil slot S
-template <int S>
-struct SlotOffset {
-  static constexpr int pk = slot_di(S);
-  static constexpr int di = (pk + 13) / 9 - 1, dj = ((pk + 13) / 3) % 3 - 1, dk = (pk + 13) % 3 - 1;
-};
-
-template <int Q, int BB>
-struct IncTerm {  // vertex BB of incidence Q's element, seen from the row's node
-  static constexpr BoxInc L = box_inc(Q);
-  static constexpr int cb = kuhn_corner(L.t, BB);
-  static constexpr int di = ((cb >> 2) & 1) - ((L.c >> 2) & 1), dj = ((cb >> 1) & 1) - ((L.c >> 1) & 1),
-                       dk = (cb & 1) - (L.c & 1);
-  static constexpr int slot = stencil_slot(di, dj, dk);
-};
 
 // is the voxel of corner C (the row's node is that voxel's corner C) inside the box?
 template <int C>
@@ -352,6 +273,10 @@ __device__ __forceinline__ void box_k(const BoxArgs& a, int i, int j, int kk, bo
   }
 }
 
+// the incidence Q of the row's node: its element's contributions to the 15 stencil slots and to the
+// load.  An incidence outside the box (a boundary row) is masked: it reads a clamped k and adds
+// +0.0, which leaves every sum bitwise as the unmasked kernel's (sums start at +0.0, so no -0.0
+// arises either way).
 template <int KIND, bool NODAL, int Q>
 __device__ __forceinline__ void box_accumulate(const BoxArgs& a, const double* __restrict__ sg, const double (&kc)[8][6],
                                                const bool (&inc)[8], int64_t r, int64_t S1, int64_t S2,

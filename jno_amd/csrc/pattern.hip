@@ -12,6 +12,7 @@
 // The result equals np.unique over the (row, col) keys of all element pairs, which is what
 // compress_plan (jno/utils/solver/fem_utils.py:4009-4060) returns.
 #include "common.h"
+#include "kuhn.h"
 
 namespace jno {
 
@@ -453,5 +454,123 @@ extern "C" int jno_equal_i32(const int32_t* d_a, const int32_t* d_b, int64_t n, 
   JNO_HIP(hipMemcpyAsync(&h, diff, sizeof(int), hipMemcpyDeviceToHost, s));
   JNO_HIP(hipStreamSynchronize(s));
   *h_equal = h ? 0 : 1;
+  return JNO_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The plan of a structured Kuhn box in closed form: the same rowptr / col / inc_ptr / inc the
+// sort-based builder produces for that mesh, with no sort and no atomics.  Node (i, j, k) couples
+// to the 15 stencil neighbours that lie in the box (every such pair shares a tet: corners c1 and
+// c2 = c1 + d of a voxel lie on a common monotone path), in ascending id = slot order; its incident
+// (element, local vertex) entries are the box_inc order, ascending by element.  A row's offsets are
+// counts of the earlier rows: the rows where a given slot (or voxel corner) exists form a sub-box,
+// and the lexicographically earlier points of a sub-box have a closed form.
+// ---------------------------------------------------------------------------------------------
+namespace jno {
+
+// points (a, b, c) of the sub-box [a0, a1] x [b0, b1] x [c0, c1] that precede (i, j, k) in
+// lexicographic order (empty ranges count nothing)
+__device__ __forceinline__ int64_t lex_before(int i, int j, int k, int a0, int a1, int b0, int b1, int c0, int c1) {
+  const int64_t na = max(0, a1 - a0 + 1), nb = max(0, b1 - b0 + 1), nc = max(0, c1 - c0 + 1);
+  int64_t cnt = (int64_t)min(max(i - a0, 0), (int)na) * nb * nc;
+  if (i >= a0 && i <= a1) {
+    cnt += (int64_t)min(max(j - b0, 0), (int)nb) * nc;
+    if (j >= b0 && j <= b1) cnt += min(max(k - c0, 0), (int)nc);
+  }
+  return cnt;
+}
+
+__global__ void pattern_box_k(int nx, int ny, int nz, int32_t* __restrict__ rowptr, int32_t* __restrict__ col,
+                              int32_t* __restrict__ inc_ptr, int32_t* __restrict__ inc) {
+  const int64_t S2 = nz + 1, S1 = (int64_t)(ny + 1) * (nz + 1);
+  const int64_t N = S1 * (nx + 1);
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= N; r += (int64_t)gridDim.x * blockDim.x) {
+    // r = N: the point after the last row ((nx+1, 0, 0)), whose counts are the totals
+    const int k = r < N ? (int)(r % S2) : 0;
+    const int j = r < N ? (int)((r / S2) % (ny + 1)) : 0;
+    const int i = r < N ? (int)(r / S1) : nx + 1;
+    int64_t rp = 0;
+    static_for<0, 15>([&](auto S) {
+      using O = SlotOffset<S>;
+      rp += lex_before(i, j, k, max(0, -O::di), nx - max(0, O::di), max(0, -O::dj), ny - max(0, O::dj),
+                       max(0, -O::dk), nz - max(0, O::dk));
+    });
+    int64_t ip = 0;
+    static_for<0, 8>([&](auto C) {
+      constexpr int ci = (C >> 2) & 1, cj = (C >> 1) & 1, ck = C & 1;
+      constexpr int mult = C == 0 || C == 7 ? 6 : 2;  // the types holding corner C
+      ip += mult * lex_before(i, j, k, ci, nx - 1 + ci, cj, ny - 1 + cj, ck, nz - 1 + ck);
+    });
+    rowptr[r] = (int32_t)rp;
+    inc_ptr[r] = (int32_t)ip;
+    if (r == N) continue;
+    static_for<0, 15>([&](auto S) {
+      using O = SlotOffset<S>;
+      if (i + O::di >= 0 && i + O::di <= nx && j + O::dj >= 0 && j + O::dj <= ny && k + O::dk >= 0 && k + O::dk <= nz)
+        col[rp++] = (int32_t)(r + O::di * S1 + O::dj * S2 + O::dk);
+    });
+    static_for<0, 24>([&](auto Q) {
+      constexpr BoxInc L = box_inc(Q);
+      const int vi = i - ((L.c >> 2) & 1), vj = j - ((L.c >> 1) & 1), vk = k - (L.c & 1);
+      if (vi >= 0 && vi < nx && vj >= 0 && vj < ny && vk >= 0 && vk < nz)
+        inc[ip++] = (int32_t)(4 * (6 * (((int64_t)vi * ny + vj) * nz + vk) + L.t) + L.a);
+    });
+  }
+}
+
+// the nodes on the box surface (every node of a facet used by one cell, for the Kuhn box)
+__global__ void mask_box_k(int nx, int ny, int nz, uint8_t* __restrict__ mask) {
+  const int64_t S2 = nz + 1, S1 = (int64_t)(ny + 1) * (nz + 1);
+  const int64_t N = S1 * (nx + 1);
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(r % S2), j = (int)((r / S2) % (ny + 1)), i = (int)(r / S1);
+    mask[r] = (i == 0 || i == nx || j == 0 || j == ny || k == 0 || k == nz) ? 1 : 0;
+  }
+}
+
+}  // namespace jno
+
+// nnz of the structured box's pattern: per stencil slot, the nodes whose neighbour lies in the box
+static int64_t box_nnz(int nx, int ny, int nz) {
+  int64_t nnz = 0;
+  for (int a = -1; a <= 1; ++a)
+    for (int b = -1; b <= 1; ++b)
+      for (int c = -1; c <= 1; ++c) {
+        const bool pos = a >= 0 && b >= 0 && c >= 0, neg = a <= 0 && b <= 0 && c <= 0;
+        if (pos || neg) nnz += (int64_t)(nx + 1 - (a != 0)) * (ny + 1 - (b != 0)) * (nz + 1 - (c != 0));
+      }
+  return nnz;
+}
+
+extern "C" int jno_pattern_kuhn_box_nnz(int nx, int ny, int nz, int64_t* h_nnz, int32_t* h_max_row) {
+  JNO_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2 && h_nnz && h_max_row, JNO_EINVAL,
+              "jno_pattern_kuhn_box_nnz: the box needs at least 2 cells per axis");
+  const int64_t nnz = box_nnz(nx, ny, nz);
+  JNO_REQUIRE(nnz < INT32_MAX && (int64_t)24 * nx * ny * nz < INT32_MAX, JNO_ECAPACITY,
+              "jno_pattern_kuhn_box_nnz: the pattern exceeds 32-bit indices");
+  *h_nnz = nnz;
+  *h_max_row = 15;  // an interior node couples to all 15 stencil neighbours
+  return JNO_OK;
+}
+
+extern "C" int jno_pattern_kuhn_box(int nx, int ny, int nz, int32_t* d_rowptr, int32_t* d_col, int32_t* d_inc_ptr,
+                                    int32_t* d_inc, void* stream) {
+  int64_t nnz;
+  int32_t mr;
+  int rc = jno_pattern_kuhn_box_nnz(nx, ny, nz, &nnz, &mr);
+  if (rc) return rc;
+  JNO_REQUIRE(d_rowptr && d_col && d_inc_ptr && d_inc, JNO_EINVAL, "jno_pattern_kuhn_box: null pointer");
+  const int64_t N = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  jno::pattern_box_k<<<jno::grid_for(N + 1, 256, 16384), 256, 0, jno::as_stream(stream)>>>(nx, ny, nz, d_rowptr, d_col,
+                                                                                          d_inc_ptr, d_inc);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_mask_kuhn_box(int nx, int ny, int nz, uint8_t* d_mask, void* stream) {
+  JNO_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && d_mask, JNO_EINVAL, "jno_mask_kuhn_box: bad argument");
+  const int64_t N = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+  jno::mask_box_k<<<jno::grid_for(N, 256, 8192), 256, 0, jno::as_stream(stream)>>>(nx, ny, nz, d_mask);
+  JNO_LAUNCH_CHECK();
   return JNO_OK;
 }

@@ -51,22 +51,37 @@ def _same_cells(a: torch.Tensor, b: torch.Tensor) -> bool:
     return bool(eq.value)
 
 
+def _cache_bytes() -> int:
+    plans = {id(p): p.nbytes for p, _ in _PLAN_CACHE.values()}
+    return sum(plans.values()) + sum(c.numel() * 4 for _, c in _PLAN_CACHE.values() if c is not None)
+
+
 def plan_for(mesh: Mesh, vec: int = 1) -> "Plan":
-    """The plan of ``mesh`` for ``vec`` components, from the content-keyed cache or built."""
-    key = _cells_key(mesh, vec)
+    """The plan of ``mesh`` for ``vec`` components, from the cache or built.  A structured Kuhn box
+    (checked once per mesh, :meth:`BoxAssembly.of`) gets the closed-form plan keyed on its grid;
+    any other mesh the sort-based one keyed on the content of its cells."""
+    box = BoxAssembly.of(mesh) if vec == 1 and mesh.nloc == 4 and mesh.grid is not None else None
+    if box is not None:
+        key = ("kuhn_box", box.grid, str(mesh.device))
+        check = None
+    else:
+        key = _cells_key(mesh, vec)
+        check = mesh.cells
     with _PLAN_LOCK:
         hit = _PLAN_CACHE.get(key)
-        if hit is not None and _same_cells(hit[1], mesh.cells):
+        if hit is not None and (check is None or _same_cells(hit[1], check)):
             _PLAN_CACHE.move_to_end(key)
             return hit[0]
-    plan = Plan.build(mesh, vec)
+    plan = Plan.build_kuhn_box(mesh, box.grid) if box is not None else Plan.build(mesh, vec)
+    # a box plan is also filed under the content of its cells, so that a mesh with the same
+    # connectivity and no lattice (moved points, or built from the same tensors) shares it
+    entries = [(key, (plan, check))] + ([(_cells_key(mesh, vec), (plan, mesh.cells))] if box is not None else [])
     with _PLAN_LOCK:
-        _PLAN_CACHE[key] = (plan, mesh.cells)
-        _PLAN_CACHE.move_to_end(key)
-        total = sum(p.nbytes + c.numel() * 4 for p, c in _PLAN_CACHE.values())
-        while len(_PLAN_CACHE) > 1 and (len(_PLAN_CACHE) > PLAN_CACHE_MAX or total > PLAN_CACHE_BYTES):
-            _, (p, c) = _PLAN_CACHE.popitem(last=False)
-            total -= p.nbytes + c.numel() * 4
+        for k_, e in entries:
+            _PLAN_CACHE[k_] = e
+            _PLAN_CACHE.move_to_end(k_)
+        while len(_PLAN_CACHE) > 2 and (len(_PLAN_CACHE) > PLAN_CACHE_MAX or _cache_bytes() > PLAN_CACHE_BYTES):
+            _PLAN_CACHE.popitem(last=False)
     return plan
 
 
@@ -114,6 +129,23 @@ class Plan:
             _lib.ptr(rowptr), _lib.ptr(col), stream,
         )
         return Plan(rowptr, col[: nnz.value], inc_ptr, inc, int(nnz.value), int(max_row.value), ndof, vec)
+
+    @staticmethod
+    def build_kuhn_box(mesh: Mesh, grid) -> "Plan":
+        """The same plan for the structured Kuhn box ``grid`` in closed form (one pass, no sort)."""
+        dev = mesh.device
+        nx, ny, nz = grid
+        N, T = mesh.n_nodes, mesh.n_cells
+        nnz = C.c_int64(0)
+        max_row = C.c_int32(0)
+        _lib.call("jno_pattern_kuhn_box_nnz", nx, ny, nz, C.byref(nnz), C.byref(max_row))
+        rowptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        inc_ptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        col = torch.empty(nnz.value, dtype=torch.int32, device=dev)
+        inc = torch.empty(4 * T, dtype=torch.int32, device=dev)
+        _lib.call("jno_pattern_kuhn_box", nx, ny, nz, _lib.ptr(rowptr), _lib.ptr(col), _lib.ptr(inc_ptr),
+                  _lib.ptr(inc), _lib.stream_ptr(dev))
+        return Plan(rowptr, col, inc_ptr, inc, int(nnz.value), int(max_row.value), N, 1)
 
     @property
     def nbytes(self) -> int:

@@ -84,10 +84,17 @@ class Mesh:
         key = ("bmask", region)
         if key in self._cache:
             return self._cache[key]
-        plan = self.plan(1)
         stream = _lib.stream_ptr(self.device)
         allb = self._cache.get(("bmask", "boundary"))
+        from .fem import BoxAssembly
+
+        box = BoxAssembly.of(self) if allb is None and self.grid is not None and self.nloc == 4 else None
+        if box is not None:  # the structured Kuhn box: its surface nodes in closed form
+            allb = torch.empty(self.n_nodes, dtype=torch.uint8, device=self.device)
+            _lib.call("jno_mask_kuhn_box", *box.grid, _lib.ptr(allb), stream)
+            self._cache[("bmask", "boundary")] = allb
         if allb is None:
+            plan = self.plan(1)
             allb = torch.empty(self.n_nodes, dtype=torch.uint8, device=self.device)
             _lib.call(
                 "jno_boundary_mask",

@@ -185,3 +185,28 @@ def test_box_entry_rejects_bad_arguments(dev):
     assert lib.jno_assemble_p1_box(1, 1, 4, 4, geo, *args) == _lib.JNO_EINVAL
     ws = torch.empty(64, dtype=torch.uint8, device="cuda")
     assert lib.jno_box_geometry(4, 4, 4, _lib.ptr(m.points), geo, _lib.ptr(ws), 64, st) == _lib.JNO_EWORKSPACE
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 4, 5), (7, 2, 3), (16, 16, 16), (33, 17, 9)])
+def test_box_plan_and_mask_equal_the_sorted_ones(dev, dims):
+    """The closed-form plan of a Kuhn box (rowptr, col, the incidence and its offsets) and its
+    surface mask equal, element for element, what the sort-based builder and the boundary-facet
+    mask produce for the same mesh; the pattern also equals the oracle's."""
+    import jno_amd as J
+    from jno_amd.fem import Plan, plan_for
+
+    m = J.kuhn_box(*dims)
+    plan = plan_for(m, 1)
+    ref = Plan.build(m, 1)
+    assert (plan.nnz, plan.max_row, plan.ndof) == (ref.nnz, ref.max_row, ref.ndof)
+    for name in ("rowptr", "col", "inc_ptr", "inc"):
+        assert torch.equal(getattr(plan, name), getattr(ref, name)), name
+    if np.prod(dims) <= 300:
+        p, t = O.kuhn_box(*dims)
+        rp, col, _, nse = O.pattern_struct(t, 1)
+        assert plan.nnz == nse and np.array_equal(_np(plan.rowptr), rp) and np.array_equal(_np(plan.col), col)
+    mask = m.boundary_mask()
+    m2 = J.Mesh(points=m.points.clone(), cells=m.cells.clone())  # no grid: the facet-based mask
+    assert torch.equal(mask, m2.boundary_mask())
+    for face in ("left", "top", "back"):
+        assert torch.equal(m.boundary_mask(face), J.Mesh(points=m.points, cells=m.cells, box=m.box).boundary_mask(face))
