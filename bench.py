@@ -418,9 +418,13 @@ def c5_rate(J, torch, dev, n=32):
 def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
     """C4: ``count`` P1 Poisson systems on n^3 Kuhn boxes, n uniform in [24, 64] (default_rng(seed)),
     each with its own lognormal k (default_rng(1000 + i)), f = 1, solved together to rtol 1e-10; and
-    ``n_rhs`` N(0,1) right-hand sides on one n_rhs_mesh^3 operator.  Effective bandwidth: the batch
-    step's 48 algorithmic bytes per row (Jacobi-scaled DIA items: p update 16, SpMV 20, r update 12)
-    over every item's inner iterations, against the solve time."""
+    ``n_rhs`` N(0,1) right-hand sides on one n_rhs_mesh^3 operator.  Effective bandwidth: the
+    algorithmic bytes of every launch an item took part in, against the solve time: per row 44 B
+    per one-kernel iteration (r, q, p, xs read and written: 32; the 3 stored upper diagonals of the
+    Jacobi-scaled operator: 12), and per residual replacement 32 B for folding xs into x64 (xs, x64
+    read and written, dinv read) plus 60 B for the FP64 true residual (b, x64, 4 FP64 diagonals,
+    dinv read; r written).  Each item runs iterations + replacements + 1 iteration launches (the
+    resumed launch after each replacement and after the start)."""
     from jno_amd.batch import BatchSolver
 
     ns = np.random.default_rng(seed).integers(24, 65, size=count)
@@ -437,8 +441,13 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     its = [s["iterations"] for s in bs.last_stats]
+    refs = [s["refinements"] for s in bs.last_stats]
     rows = [A.shape[0] for A, _ in systems]
-    moved = sum(48 * r * it for r, it in zip(rows, its))
+
+    def batch_bytes(r, it, rf):
+        return r * (44 * (it + rf + 1) + (32 + 60) * (rf + 1))
+
+    moved = sum(batch_bytes(r, it, rf) for r, it, rf in zip(rows, its, refs))
     m96 = J.kuhn_box(n_rhs_mesh, device=dev)
     k96 = np.exp(0.5 * np.random.default_rng(7).standard_normal(m96.n_cells))
     A96, _ = J.fem_poisson(m96).evaluate({"k": torch.tensor(k96, device=dev)})
@@ -452,7 +461,7 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
     torch.cuda.synchronize()
     t_rhs = time.perf_counter() - t0
     its_r = [s["iterations"] for s in br.last_stats]
-    moved_r = sum(48 * A96.shape[0] * it for it in its_r)
+    moved_r = sum(batch_bytes(A96.shape[0], it, s["refinements"]) for it, s in zip(its_r, br.last_stats))
     gbs = moved / t_batch / 1e9
     return {
         "workload": f"{count} P1 Poisson systems, n^3 Kuhn boxes with n in [24, 64] (seed {seed}), {sum(rows)} rows; "
@@ -460,7 +469,8 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
         "batch_s": round(t_batch, 6), "iterations_min_max": [min(its), max(its)],
         "worst_relres": max(s["relres"] for s in bs.last_stats),
         "rhs_s": round(t_rhs, 6), "rhs_iterations_min_max": [min(its_r), max(its_r)],
-        "roofline": {"kernel": "batch step (batch_ka + batch_ks + batch_kb), whole solve",
+        "one_kernel": bs.last_one_kernel,
+        "roofline": {"kernel": "batch_one_k (one kernel per step), whole solve",
                      "bytes": moved, "achieved": round(gbs, 1), "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "rhs_achieved": round(moved_r / t_rhs / 1e9, 1)},

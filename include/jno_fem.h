@@ -435,6 +435,27 @@ int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, double tol, 
                         int64_t maxiter, double replace_factor, int steps_per_graph,
                         jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream);
 
+/* Launch shape of a batched solve, per call (no reference counterpart: the reference's vmap has no
+ * launch shape).  one_kernel = 1 runs one kernel per step (the one-kernel iteration of the single
+ * solver, carried to every item) when every item is a Jacobi-scaled 4-diagonal DIA operator whose
+ * second offset is at most 128 (3-D Kuhn lattices up to 127 cells along the first axis); otherwise,
+ * and with one_kernel = 0, the three-kernel step runs.  rows_per_entry (one-kernel step: a
+ * multiple of 2048, at most 65536) and grid (a multiple of 8, at most 65536) shape its table of
+ * (item, row range) entries; 0 takes the default (4096 rows, 4096 workgroups).  ran_one_kernel is
+ * an output: 1 when the call ran the one-kernel step. */
+typedef struct jno_batch_tuning {
+  int32_t one_kernel;
+  int32_t rows_per_entry;
+  int32_t grid;
+  int32_t ran_one_kernel;
+} jno_batch_tuning;
+/* jno_batch_pcg_mixed with a launch shape (h_tuning NULL: the defaults, as jno_batch_pcg_mixed).
+ * Same workspace size. */
+int jno_batch_pcg_mixed_tuned(int n_items, const jno_batch_item* h_items, double tol, double atol,
+                              int64_t maxiter, double replace_factor, int steps_per_graph,
+                              jno_batch_tuning* h_tuning, jno_pcg_stats* h_stats, void* d_ws,
+                              size_t ws_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------------
  * A18  adjoint contraction.  Replaces the VJP chain of custom_linear_solve(symmetric=True) ->
  * BCOO data -> Dirichlet mask -> segment_sum -> jacfwd (jno/utils/solver/fem_native.py:2326-2345,

@@ -90,6 +90,10 @@ class JnoBatchItem(C.Structure):
     _fields_ = [("op", JnoOp), ("dinv", C.c_void_p), ("b", C.c_void_p), ("x", C.c_void_p)]
 
 
+class JnoBatchTuning(C.Structure):
+    _fields_ = [("one_kernel", C.c_int32), ("rows_per_entry", C.c_int32), ("grid", C.c_int32), ("ran_one_kernel", C.c_int32)]
+
+
 class JnoError(RuntimeError):
     """A failed call into libjno_fem (carries the status code)."""
 
@@ -173,6 +177,11 @@ SIGNATURES = {
     "jno_batch_pcg_mixed": (
         C.c_int,
         [C.c_int, C.POINTER(JnoBatchItem), D, D, I64, D, C.c_int, C.POINTER(JnoPcgStats), P, SZ, P],
+    ),
+    "jno_batch_pcg_mixed_tuned": (
+        C.c_int,
+        [C.c_int, C.POINTER(JnoBatchItem), D, D, I64, D, C.c_int, C.POINTER(JnoBatchTuning), C.POINTER(JnoPcgStats), P,
+         SZ, P],
     ),
     "jno_promote_p2_workspace_size": (SZ, [I64]),
     "jno_promote_p2": (C.c_int, [P, P, I64, I64, P, P, P, P, C.POINTER(I64), P, SZ, P]),

@@ -113,24 +113,28 @@ def tune_launch(A, b, precision: str = "mixed", iters: int = 100, shapes=CANDIDA
     return _LAUNCH_CACHE[key]
 
 
-def tune_batch(systems, chunks=(4, 8, 16), **solver_kw) -> int:
-    """Steps per hipGraph replay for :class:`jno_amd.batch.BatchSolver` on ``systems`` (a list of
-    ``(A, b)``): each candidate is timed on one whole batched solve (after a warm solve that captures
-    its graph), and the fastest is returned.  Longer chunks mean fewer host polls but more step
-    launches after the last item converged."""
+def tune_batch(systems, chunks=(4, 8, 16), shapes=((4096, 2048),), **solver_kw) -> dict:
+    """Launch shape for :class:`jno_amd.batch.BatchSolver` on ``systems`` (a list of ``(A, b)``):
+    steps per hipGraph replay from ``chunks`` and the one-kernel step's (rows_per_entry, grid) from
+    ``shapes``.  Each candidate is timed on one whole batched solve (after a warm solve that captures
+    its graph) and the fastest is returned as BatchSolver keyword arguments.  Longer chunks mean
+    fewer host polls but more step launches after the last item converged; longer entries mean
+    fewer partial sums for each entry to reduce."""
     import time
 
     from .batch import BatchSolver
 
     best = None
-    for c in chunks:
-        bs = BatchSolver(steps_per_graph=int(c), **solver_kw)
-        bs(systems)  # warm: graph capture, operator layouts
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        bs(systems)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-        if best is None or t < best[0]:
-            best = (t, int(c))
+    for rpe, grid in shapes:
+        for c in chunks:
+            kw = {"steps_per_graph": int(c), "rows_per_entry": int(rpe), "grid": int(grid)}
+            bs = BatchSolver(**kw, **solver_kw)
+            bs(systems)  # warm: graph capture, operator layouts
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bs(systems)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            if best is None or t < best[0]:
+                best = (t, kw)
     return best[1]

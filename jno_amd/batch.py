@@ -20,14 +20,26 @@ from .operator import SolverOperator
 
 
 class BatchSolver:
+    """Batched mixed-precision Jacobi-PCG.
+
+    The launch shape is per solver: ``one_kernel`` runs one kernel per step when every system is a
+    Jacobi-scaled 4-diagonal DIA operator (3-D Kuhn lattices), else the three-kernel step;
+    ``rows_per_entry`` (a multiple of 2048) and ``grid`` (a multiple of 8) shape the one-kernel
+    step's table of (system, row range) entries, 0 taking the library defaults;
+    ``steps_per_graph`` steps are captured in one hipGraph replay (0: direct launches).
+    :func:`jno_amd.autotune.tune_batch` picks them by timing."""
+
     def __init__(self, tol: float = 1e-10, atol: float = 0.0, maxiter: int = 20_000, replace_factor: float = 0.1,
-                 steps_per_graph: int = 8, format: str = "auto"):
+                 steps_per_graph: int = 8, format: str = "auto", one_kernel: bool = True, rows_per_entry: int = 0,
+                 grid: int = 0):
         self.tol, self.atol, self.maxiter = float(tol), float(atol), int(maxiter)
         self.replace_factor = float(replace_factor)
         self.steps_per_graph = int(steps_per_graph)
         self.format = format
+        self.tuning = _lib.JnoBatchTuning(1 if one_kernel else 0, int(rows_per_entry), int(grid), 0)
         self.last_stats: List[dict] = []
         self.last_status = 0
+        self.last_one_kernel = False  # whether the last call ran the one-kernel step
 
     def _op(self, A: CSR) -> SolverOperator:
         # the batch kernels stream DIA and CSR items only: the batch keeps its own cache entry, and
@@ -76,9 +88,11 @@ class BatchSolver:
         wsb = lib.jno_batch_workspace_size(len(systems), items)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         stats = (_lib.JnoPcgStats * len(systems))()
-        rc = lib.jno_batch_pcg_mixed(len(systems), items, self.tol, self.atol, self.maxiter, self.replace_factor,
-                                     self.steps_per_graph, stats, _lib.ptr(ws), wsb, _lib.stream_ptr(dev))
+        rc = lib.jno_batch_pcg_mixed_tuned(len(systems), items, self.tol, self.atol, self.maxiter,
+                                           self.replace_factor, self.steps_per_graph, C.byref(self.tuning), stats,
+                                           _lib.ptr(ws), wsb, _lib.stream_ptr(dev))
         self.last_status = int(rc)
+        self.last_one_kernel = bool(self.tuning.ran_one_kernel)
         self.last_stats = [
             {"iterations": int(s.iterations), "refinements": int(s.refinements), "relres": float(s.relres),
              "seconds": float(s.seconds)} for s in stats

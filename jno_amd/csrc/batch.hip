@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pcg_common.h"
 #include "sparse.h"
 
 namespace jno {
@@ -525,18 +526,438 @@ __global__ void batch_scale_k(const BItem* items, const double* const* dinvs, co
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// One kernel per step (batch_one_k): the single-system one-kernel iteration carried over to the
+// batch, for items on a Jacobi-scaled 4-diagonal DIA operator (the 3-D Kuhn lattice; near offsets
+// o1 < o2 <= B1_HALO, far o3).  Launch k of an item reads r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1} and
+// the partials of its previous launch (r.r, p.q, q.r, q.q); alpha = gamma_{k-1} / p.q,
+// gamma_k = gamma_{k-1} - 2 alpha q.r + alpha^2 q.q, beta = gamma_k / gamma_{k-1}; then per row
+// r_k = r_{k-1} - alpha q_{k-1}, xs += alpha p_{k-1}, p_k = r_k + beta p_{k-1} (near neighbours
+// through LDS, the far ones recomputed from r, q, p), q_k = A p_k and the partials of r_k.r_k,
+// p_k.q_k, q_k.r_k, q_k.q_k.  r, q and p alternate between two buffers by the parity of k.
+// An item's phases, one launch each:
+//   ITER  as above; when r.r has fallen to the replacement threshold (or at the cap, or on a
+//         non-finite scalar) the launch instead folds xs into x64 (x += S xs, xs = 0): phase R2
+//   R2    r64 = b - A64 x64 in FP64, r = S r64 into the buffer the resumed launch reads; partials
+//         r.r (scaled), ||r64||^2, b.b
+//   RES   decides on the true residual (converged, cap, non-finite: DONE); otherwise runs launch
+//         kr = k_stop - 1 again with alpha = 0 and beta = gamma_true / gamma_{kr-1} on p_{kr-1},
+//         which the stopped launch left intact, so the search direction survives the replacement
+//         (kr = 0: the start, beta = 0)
+//   DONE  nothing (the state is carried to the other buffer)
+// The first launch is R2 from x64 = 0.  Per-item state and partial sums are double-buffered by the
+// launch parity (a kernel argument: launches alternate 0, 1, 0, ...), so a launch reads only what
+// the previous launch wrote; every entry of an item derives the same decision from the same
+// values, and the item's first entry stores the next state.  No value crosses workgroups inside a
+// launch (no atomics on the step path apart from the active-item count, no fences).
+// ---------------------------------------------------------------------------------------------
+constexpr int B1_NT = 512;
+constexpr int B1_U = 4;                // rows per thread and tile
+constexpr int B1_TT = B1_U * B1_NT;    // rows per tile
+constexpr int B1_HALO = 128;           // near halo rows staged in LDS (o2 of a Kuhn lattice up to 127 cells)
+constexpr int B1_RPE = 2 * B1_TT;      // default rows per table entry (whole tiles)
+constexpr int B1_GRID = 4096;          // default workgroups per launch (C4: 112.7 ms against 116.7 at 2048)
+
+enum { P1_R2 = 0, P1_RES = 1, P1_ITER = 2, P1_DONE = 3 };
+
+struct B1State {
+  int32_t phase, status, bad, pad;
+  int64_t k;            // ITER: the index of this launch
+  int64_t kr;           // R2 / RES: the launch a resume runs (iterations done)
+  int64_t refinements;
+  double g_hist[2];     // g_hist[k & 1] = gamma_{k-1}, written by the first run of launch k
+  double rr_replace, rr_true, bb, rr_last;
+};
+
+struct B1Args {
+  const BItem* items;
+  B1State* st[2];     // by launch parity
+  const int2* table;  // (item, local entry)
+  int32_t n_entries;
+  int32_t rpe;        // rows per entry (a multiple of B1_TT)
+  float* r[2];
+  float* q[2];
+  float* p[2];
+  float* xs;
+  double* P[2];       // [4][n_entries] by launch parity
+  int32_t* active;
+  double tol2, atol2_abs, repl2;
+  int64_t maxiter;
+};
+
+// Values uniform across the workgroup moved to scalar registers.  The compiler loads the table
+// entry, the item descriptor and the state with vector loads (the kernel stores to global memory,
+// so it cannot prove them unclobbered), which would hold some 50 vector registers per lane.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni(int64_t v) {
+  const int32_t lo = __builtin_amdgcn_readfirstlane((int32_t)(v & 0xffffffffLL));
+  const int32_t hi = __builtin_amdgcn_readfirstlane((int32_t)(v >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+__device__ __forceinline__ double uni(double v) { return __longlong_as_double(uni((int64_t)__double_as_longlong(v))); }
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+  return reinterpret_cast<T*>(uni((int64_t) reinterpret_cast<intptr_t>(p)));
+}
+
+__device__ __forceinline__ ItemView item_view_uni(const BItem* items, int i) {
+  ItemView v = item_view(items, i);
+  v.n = uni(v.n);
+  v.off = uni(v.off);
+  v.format = uni(v.format);
+  v.ndiag = uni(v.ndiag);
+  v.offs = uni(v.offs);
+  v.rowptr = uni(v.rowptr);
+  v.col = uni(v.col);
+  v.v32 = uni(v.v32);
+  v.v64 = uni(v.v64);
+  v.b = uni(v.b);
+  v.x = uni(v.x);
+  v.e0 = uni(v.e0);
+  v.ne = uni(v.ne);
+  v.s32 = uni(v.s32);
+  v.dinv64 = uni(v.dinv64);
+  v.scaled = uni((int32_t)v.scaled) != 0;
+  return v;
+}
+
+__device__ __forceinline__ B1State state_uni(const B1State* S) {
+  B1State s = *S;
+  s.phase = uni(s.phase);
+  s.status = uni(s.status);
+  s.bad = uni(s.bad);
+  s.k = uni(s.k);
+  s.kr = uni(s.kr);
+  s.refinements = uni(s.refinements);
+  s.g_hist[0] = uni(s.g_hist[0]);
+  s.g_hist[1] = uni(s.g_hist[1]);
+  s.rr_replace = uni(s.rr_replace);
+  s.rr_true = uni(s.rr_true);
+  s.bb = uni(s.bb);
+  s.rr_last = uni(s.rr_last);
+  return s;
+}
+
+// item totals of the 4 partial arrays P[m * NE + e0 + j], j < ne, in a fixed order, identical in
+// every thread: array m is summed by the threads m NT/4 .. (m+1) NT/4 - 1, then their waves' sums in
+// wave order
+__device__ __forceinline__ void b1_item_totals(const double* __restrict__ P, int64_t NE, int e0, int ne,
+                                               double* sm, double* out) {
+  constexpr int GS = B1_NT / 4, WPG = GS / 64;
+  const int m = threadIdx.x / GS, l = threadIdx.x % GS;
+  const double* src = P + (int64_t)m * NE + e0;
+  double s = 0.0;
+  for (int j = l; j < ne; j += GS) s += src[j];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+#pragma unroll
+  for (int mm = 0; mm < 4; ++mm) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < WPG; ++w) t += sm[mm * WPG + w];
+    out[mm] = t;
+  }
+  __syncthreads();  // sm is reused
+}
+
+// the entry's 4 partials into P[m * NE + entry]
+__device__ __forceinline__ void b1_store_partials(const double* v, double* __restrict__ P, int64_t NE, int entry,
+                                                  double* sm) {
+  constexpr int NW = B1_NT / 64;
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double s = wave_sum(v[m]);
+    if ((threadIdx.x & 63) == 0) sm[m * NW + w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) t += sm[threadIdx.x * NW + ww];
+    P[(int64_t)threadIdx.x * NE + entry] = t;
+  }
+  __syncthreads();
+}
+
+// a row of an item's vector through a 32-bit byte offset from the item's (uniform) base: the loads
+// then take the scalar-base addressing form, one 32-bit offset register each instead of a 64-bit
+// address (item rows are below 2^29, checked on the host)
+__device__ __forceinline__ float ldr(const float* __restrict__ base, uint32_t row) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (row << 2));
+}
+__device__ __forceinline__ void str(float* __restrict__ base, uint32_t row, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (row << 2)) = v;
+}
+
+// One iteration launch over rows [r0, r1) of an item: launch kk with the given alpha, beta
+// (alpha = 0: a start or resume, r and x left as they are).  part: r.r, p.q, q.r, q.q.
+__device__ __forceinline__ void b1_body(const B1Args& a, const ItemView& it, int64_t r0, int64_t r1, int64_t kk,
+                                        float al, float be, float* pt, double* part) {
+  const int n = (int)it.n;
+  const int o1 = it.offs[1], o2 = it.offs[2], o3 = it.offs[3];
+  const int h = o2;
+  const int64_t ld = dia_ld(it.n);
+  const int cur = (int)(kk & 1), nxt = cur ^ 1;
+  const float* __restrict__ R = a.r[cur] + it.off;
+  const float* __restrict__ Q = a.q[cur] + it.off;
+  const float* __restrict__ Pp = a.p[cur] + it.off;
+  float* __restrict__ Rn = a.r[nxt] + it.off;
+  float* __restrict__ Qn = a.q[nxt] + it.off;
+  float* __restrict__ Pn = a.p[nxt] + it.off;
+  float* __restrict__ XS = a.xs + it.off;
+  const float* __restrict__ v1 = it.s32;
+  const float* __restrict__ v2 = it.s32 + ld;
+  const float* __restrict__ v3 = it.s32 + 2 * ld;
+  const bool upd = al != 0.0f;
+  const int tid = threadIdx.x;
+  for (int t0 = (int)r0; t0 < (int)r1; t0 += B1_TT) {
+    const int t1 = min((int)r1, t0 + B1_TT);
+    // own rows: r, q, p, xs; one near-halo row per thread below 2h; every load unconditional from
+    // a row clamped to 0 where it does not apply (finite values that meet zero coefficients only)
+    float rv[B1_U], qv[B1_U], pv[B1_U], xv[B1_U];
+#pragma unroll
+    for (int u = 0; u < B1_U; ++u) {
+      const int i = t0 + u * B1_NT + tid;
+      const uint32_t j = i < t1 ? (uint32_t)i : 0u;
+      rv[u] = ldr(R, j);
+      qv[u] = ldr(Q, j);
+      pv[u] = ldr(Pp, j);
+      xv[u] = ldr(XS, j);
+    }
+    const bool hal = tid < 2 * h;
+    const int jh = tid < h ? t0 - h + tid : t1 + (tid - h);
+    const bool hok = hal && jh >= 0 && jh < n;
+    const uint32_t jhc = hok ? (uint32_t)jh : 0u;
+    const float rh = ldr(R, jhc), qh = ldr(Q, jhc), ph = ldr(Pp, jhc);
+    float rn[B1_U], pn[B1_U];
+#pragma unroll
+    for (int u = 0; u < B1_U; ++u) {
+      const int i = t0 + u * B1_NT + tid;
+      rn[u] = upd ? rv[u] - al * qv[u] : rv[u];
+      pn[u] = rn[u] + be * pv[u];
+      if (i < t1) {
+        pt[i - t0 + h] = pn[u];
+        str(Rn, (uint32_t)i, rn[u]);
+        str(Pn, (uint32_t)i, pn[u]);
+        if (upd) str(XS, (uint32_t)i, xv[u] + al * pv[u]);
+        part[0] += (double)rn[u] * (double)rn[u];
+      }
+    }
+    if (hal) {
+      const float rr = upd ? rh - al * qh : rh;
+      pt[jh - t0 + h] = hok ? rr + be * ph : 0.0f;
+    }
+    float fu[B1_U], fl[B1_U];
+#pragma unroll
+    for (int u = 0; u < B1_U; ++u) {
+      const int i = t0 + u * B1_NT + tid;
+      const bool ok = i < t1;
+      const uint32_t iu = ok && i + o3 < n ? (uint32_t)(i + o3) : 0u;
+      const uint32_t il = ok && i - o3 >= 0 ? (uint32_t)(i - o3) : 0u;
+      const float ru = ldr(R, iu), qu = ldr(Q, iu), pu = ldr(Pp, iu);
+      const float rl = ldr(R, il), ql = ldr(Q, il), pl = ldr(Pp, il);
+      fu[u] = (upd ? ru - al * qu : ru) + be * pu;
+      fl[u] = (upd ? rl - al * ql : rl) + be * pl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < B1_U; ++u) {
+      const int i = t0 + u * B1_NT + tid;
+      const bool ok = i < t1;
+      const uint32_t ic = ok ? (uint32_t)i : 0u;
+      const bool l1 = ok && i - o1 >= 0, l2 = ok && i - o2 >= 0, l3 = ok && i - o3 >= 0;
+      const float cu1 = ldr(v1, ic), cu2 = ldr(v2, ic), cu3 = ldr(v3, ic);
+      const float c1 = ldr(v1, l1 ? (uint32_t)(i - o1) : 0u);
+      const float c2 = ldr(v2, l2 ? (uint32_t)(i - o2) : 0u);
+      const float c3 = ldr(v3, l3 ? (uint32_t)(i - o3) : 0u);
+      if (ok) {
+        const int li = i - t0 + h;
+        float s = pn[u];
+        s += cu1 * pt[li + o1];
+        s += (l1 ? c1 : 0.0f) * pt[li - o1];
+        s += cu2 * pt[li + o2];
+        s += (l2 ? c2 : 0.0f) * pt[li - o2];
+        s += cu3 * fu[u];
+        s += (l3 ? c3 : 0.0f) * fl[u];
+        str(Qn, (uint32_t)i, s);
+        part[1] += (double)pn[u] * (double)s;
+        part[2] += (double)s * (double)rn[u];
+        part[3] += (double)s * (double)s;
+      }
+    }
+    __syncthreads();  // the next tile rewrites pt
+  }
+}
+
+// At least 4 waves per SIMD (at most 128 VGPRs): two workgroups per CU.  Measured at C4 (256
+// systems, 29.1M rows): 333 us per launch with every system active (3.85 TB/s of algorithmic bytes);
+// loading the coefficients with the far rows before the barrier, or 2 rows per thread and tile,
+// measured 2-6% slower.
+__global__ void __launch_bounds__(B1_NT, 4) batch_one_k(B1Args a, int par) {
+  __shared__ float pt[B1_TT + 2 * B1_HALO];
+  __shared__ double sm[4 * (B1_NT / 64)];
+  const int64_t NE = a.n_entries;
+  // XCD-aware: the entries split into 8 contiguous slabs, one per XCD (workgroups are dealt to the
+  // XCDs round-robin), so an item's neighbouring entries run on one XCD at about the same time and
+  // share their far rows through its L2
+  const int G = gridDim.x;
+  int64_t e_lo = 0, e_hi = NE, e_first = blockIdx.x, e_step = G;
+  if (G % N_XCD == 0) {
+    const int x = blockIdx.x % N_XCD, per = G / N_XCD;
+    e_lo = NE * x / N_XCD;
+    e_hi = NE * (x + 1) / N_XCD;
+    e_first = e_lo + blockIdx.x / N_XCD;
+    e_step = per;
+  }
+  const B1State* __restrict__ Sin = a.st[par];
+  B1State* __restrict__ Sout = a.st[par ^ 1];
+  const double* __restrict__ Pin = a.P[par];
+  double* __restrict__ Pout = a.P[par ^ 1];
+  for (int64_t entry = e_first; entry < e_hi; entry += e_step) {
+    (void)e_lo;
+    int2 te = a.table[entry];
+    te.x = uni(te.x);
+    te.y = uni(te.y);
+    const ItemView it = item_view_uni(a.items, te.x);
+    B1State S = state_uni(Sin + te.x);
+    const bool first = te.y == 0;
+    const bool lead = first && threadIdx.x == 0;
+    if (S.phase == P1_DONE) {
+      if (lead) Sout[te.x] = S;
+      continue;
+    }
+    const int64_t r0 = (int64_t)te.y * a.rpe, r1 = min(it.n, r0 + a.rpe);
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    if (S.phase == P1_R2) {
+      // true residual in FP64 on the unscaled operator, r = S r64 into the resumed launch's buffer
+      float* __restrict__ Rw = a.r[S.kr & 1] + it.off;
+      const int64_t ld = dia_ld(it.n);
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += B1_NT) {
+        const double bi = it.b[i];
+        double ax = it.v64[i] * it.x[i];
+#pragma unroll
+        for (int d = 1; d < 4; ++d) {
+          const int64_t o = it.offs[d];
+          const double* vd = it.v64 + (int64_t)d * ld;
+          if (i + o < it.n) ax += vd[i] * it.x[i + o];
+          if (i - o >= 0) ax += vd[i - o] * it.x[i - o];
+        }
+        const double ri = bi - ax;
+        const float rf = (float)(ri * sqrt(it.dinv64[i]));
+        Rw[i] = rf;
+        part[0] += (double)rf * (double)rf;
+        part[1] += ri * ri;
+        part[2] += bi * bi;
+      }
+      S.phase = P1_RES;
+      b1_store_partials(part, Pout, NE, (int)entry, sm);
+      if (lead) Sout[te.x] = S;
+      continue;
+    }
+    double in[4];
+    b1_item_totals(Pin, NE, it.e0, it.ne, sm, in);
+    int64_t kk;
+    double alpha, beta;
+    if (S.phase == P1_RES) {
+      const double rr = in[0], rt = in[1], bb = in[2];
+      const double atol2 = fmax(a.tol2 * bb, a.atol2_abs);
+      const bool bad = !isfinite(rt) || S.bad;
+      S.refinements += 1;
+      S.rr_true = rt;
+      S.bb = bb;
+      S.rr_last = rr;
+      if (bad || !(rt > atol2) || S.kr >= a.maxiter) {
+        S.phase = P1_DONE;
+        S.status = bad ? BS_NONFINITE : !(rt > atol2) ? BS_CONVERGED : BS_MAXITER;
+        if (lead) {
+          Sout[te.x] = S;
+          atomicSub(a.active, 1);
+        }
+        continue;
+      }
+      // the inner loop stops only to replace; the last cycle aims at the remaining reduction
+      const double need = atol2 / rt;
+      S.rr_replace = need > a.repl2 ? 0.5 * need * rr : a.repl2 * rr;
+      kk = S.kr;
+      alpha = 0.0;
+      // selects, not a runtime index (which would place S in scratch memory)
+      beta = kk == 0 ? 0.0 : rr / ((kk & 1) ? S.g_hist[1] : S.g_hist[0]);
+      S.phase = P1_ITER;
+      S.k = kk + 1;
+    } else {  // P1_ITER
+      kk = S.k;
+      const double rr = in[0];
+      alpha = rr / in[1];
+      beta = (rr - 2.0 * alpha * in[2] + alpha * alpha * in[3]) / rr;
+      const bool stop_repl = rr <= S.rr_replace;
+      const bool bad = !isfinite(rr) || !isfinite(alpha) || !isfinite(beta);
+      if (stop_repl || bad || kk > a.maxiter) {
+        // x64 += S xs, xs = 0 on the entry's rows; the true residual follows in the next launch
+        float* __restrict__ XS = a.xs + it.off;
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += B1_NT) {
+          it.x[i] += sqrt(it.dinv64[i]) * (double)XS[i];
+          XS[i] = 0.0f;
+        }
+        S.phase = P1_R2;
+        S.kr = kk - 1;
+        S.bad = S.bad || (bad && !stop_repl);
+        if (lead) Sout[te.x] = S;
+        continue;
+      }
+      if (kk & 1)
+        S.g_hist[1] = rr;
+      else
+        S.g_hist[0] = rr;
+      S.k = kk + 1;
+    }
+    b1_body(a, it, r0, r1, kk, (float)alpha, (float)beta, pt, part);
+    b1_store_partials(part, Pout, NE, (int)entry, sm);
+    if (lead) Sout[te.x] = S;
+  }
+}
+
+__global__ void batch_one_init_k(B1Args a, int n_items) {
+  // per entry: zero x64, xs, p and q (a start reads p_{-1} and q_{-1} times zero)
+  for (int entry = blockIdx.x; entry < a.n_entries; entry += gridDim.x) {
+    const int2 te = a.table[entry];
+    const BItem& it = a.items[te.x];
+    const int64_t r0 = (int64_t)te.y * a.rpe, r1 = min(it.n, r0 + a.rpe);
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+      it.x[i] = 0.0;
+      a.xs[it.off + i] = 0.0f;
+      a.p[0][it.off + i] = 0.0f;
+      a.p[1][it.off + i] = 0.0f;
+      a.q[0][it.off + i] = 0.0f;
+      a.q[1][it.off + i] = 0.0f;
+    }
+  }
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n_items) {
+    B1State s = {};
+    s.phase = P1_R2;  // the first launch forms r = b and the norms
+    s.status = BS_RUNNING;
+    s.refinements = -1;
+    a.st[0][g] = s;
+    a.st[1][g] = s;
+  }
+}
+
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct BLayout {
   BItem* items;
   BState* st;
+  B1State* st1[2];
   int2* table;
   const double** dinvs;
   double* PA;
-  double* PB;
+  double* PB;        // [4][n_entries]; the one-kernel step uses it as P[0]
+  double* PB1;       // [4][n_entries]: the one-kernel step's P[1]
   int32_t* active;
   int32_t* bad;  // [n_items]: the item's operator cannot be Jacobi-scaled exactly
-  float *rs, *p0, *p1, *q, *xs, *dinv;
+  float *rs, *p0, *p1, *q, *xs, *dinv, *r1, *q1;
   float* sdia;   // scaled upper diagonals of the DIA items, one block per item
   size_t bytes;
 };
@@ -551,10 +972,13 @@ static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, int64_t nsd
   };
   L.items = (BItem*)take(sizeof(BItem) * n_items);
   L.st = (BState*)take(sizeof(BState) * n_items);
+  L.st1[0] = (B1State*)take(sizeof(B1State) * n_items);
+  L.st1[1] = (B1State*)take(sizeof(B1State) * n_items);
   L.table = (int2*)take(sizeof(int2) * n_entries);
   L.dinvs = (const double**)take(sizeof(double*) * n_items);
   L.PA = (double*)take(sizeof(double) * n_entries);
   L.PB = (double*)take(sizeof(double) * 4 * n_entries);
+  L.PB1 = (double*)take(sizeof(double) * 4 * n_entries);
   L.active = (int32_t*)take(256);
   L.rs = (float*)take(4 * ntot);
   L.p0 = (float*)take(4 * ntot);
@@ -562,6 +986,8 @@ static BLayout blayout(int n_items, int64_t n_entries, int64_t ntot, int64_t nsd
   L.q = (float*)take(4 * ntot);
   L.xs = (float*)take(4 * ntot);
   L.dinv = (float*)take(4 * ntot);
+  L.r1 = (float*)take(4 * ntot);
+  L.q1 = (float*)take(4 * ntot);
   L.bad = (int32_t*)take(sizeof(int32_t) * n_items);
   L.sdia = (float*)take(4 * nsd);
   L.bytes = off;
@@ -574,6 +1000,7 @@ static int64_t scaled_rows(const jno_op& op) { return op.format == JNO_FMT_DIA ?
 // each item's rows start 64-aligned in the concatenated FP32 vectors (16-byte vector tiles)
 static int64_t padded_rows(int64_t n) { return (n + 63) & ~(int64_t)63; }
 
+// entries of the three-kernel table (BT_RPB rows each): the most any table of the call has
 static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot, int64_t* nsd) {
   int64_t ne = 0, nt = 0, ns = 0;
   for (int i = 0; i < n; ++i) {
@@ -584,6 +1011,20 @@ static int64_t count_entries(int n, const jno_batch_item* items, int64_t* ntot, 
   if (ntot) *ntot = nt;
   if (nsd) *nsd = ns;
   return ne;
+}
+
+// the item table and (item, local entry) list for entries of rpe rows
+static int64_t build_table(std::vector<BItem>& items, std::vector<int2>& table, int64_t rpe) {
+  int64_t e = 0;
+  table.clear();
+  for (size_t i = 0; i < items.size(); ++i) {
+    BItem& it = items[i];
+    it.e0 = (int32_t)e;
+    it.ne = (int32_t)cdiv(it.n, rpe);
+    for (int j = 0; j < it.ne; ++j) table.push_back(make_int2((int)i, j));
+    e += it.ne;
+  }
+  return e;
 }
 
 // Pinned slots and events of the host's polls of the active-item count: created once per thread
@@ -618,23 +1059,85 @@ struct BatchGraph {
   }
 };
 
-}  // namespace jno
-
-using namespace jno;
-
-extern "C" size_t jno_batch_workspace_size(int n_items, const jno_batch_item* h_items) {
-  if (n_items <= 0 || !h_items) return 0;
-  int64_t ntot, nsd;
-  const int64_t ne = count_entries(n_items, h_items, &ntot, &nsd);
-  return blayout(n_items, ne, ntot, nsd, nullptr).bytes;
+static int check_batch_tuning(const jno_batch_tuning* t) {
+  if (!t) return JNO_OK;
+  JNO_REQUIRE(t->one_kernel == 0 || t->one_kernel == 1, JNO_EINVAL, "jno_batch_tuning: one_kernel is 0 or 1");
+  JNO_REQUIRE(t->rows_per_entry == 0 || (t->rows_per_entry > 0 && t->rows_per_entry % B1_TT == 0 &&
+                                         t->rows_per_entry <= 65536),
+              JNO_EINVAL, "jno_batch_tuning: rows_per_entry is 0 or a multiple of %d up to 65536", B1_TT);
+  JNO_REQUIRE(t->grid == 0 || (t->grid > 0 && t->grid % N_XCD == 0 && t->grid <= 65536), JNO_EINVAL,
+              "jno_batch_tuning: grid is 0 or a multiple of %d up to 65536", N_XCD);
+  return JNO_OK;
 }
 
-extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, double tol, double atol, int64_t maxiter,
-                                   double replace_factor, int steps_per_graph, jno_pcg_stats* h_stats, void* d_ws,
-                                   size_t ws_bytes, void* stream) {
+// every item allows the one-kernel step: a Jacobi-scaled 4-diagonal DIA operator with ascending
+// offsets whose second one fits the LDS halo, and fewer than 2^29 rows (32-bit byte offsets)
+static bool one_kernel_ok(const std::vector<BItem>& items) {
+  for (const BItem& it : items) {
+    if (it.format != JNO_FMT_DIA || it.ndiag != 4 || !it.scaled || it.n >= ((int64_t)1 << 29)) return false;
+    if (!(0 < it.offs[1] && it.offs[1] < it.offs[2] && it.offs[2] <= B1_HALO && it.offs[2] < it.offs[3])) return false;
+  }
+  return true;
+}
+
+// Host loop shared by both step forms: enqueue chunks of steps (replayed from a hipGraph when
+// steps_per_graph > 0) and poll the active-item count one chunk behind, so the queue never drains.
+template <typename Step>
+static int run_steps(Step&& launch_step, int steps_per_graph, int chunk, int64_t max_steps, int32_t* d_active,
+                     hipStream_t s, int64_t* launched_out) {
+  int rc = g_bpoll.init();
+  if (rc) return rc;
+  int32_t* h_act = g_bpoll.h;
+  hipEvent_t* ev = g_bpoll.ev;
+  BatchGraph bg;
+  bg.s = s;
+  if (steps_per_graph > 0) {
+    // capture one chunk of steps on a private stream (default priority), replay it on `s`
+    JNO_HIP(hipStreamCreateWithFlags(&bg.cap, hipStreamNonBlocking));
+    JNO_HIP(hipStreamBeginCapture(bg.cap, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < chunk && rc == JNO_OK; ++i) rc = launch_step(bg.cap, i);
+    hipError_t ce = hipStreamEndCapture(bg.cap, &bg.graph);
+    if (rc) return rc;
+    JNO_HIP(ce);
+    JNO_HIP(hipGraphInstantiate(&bg.exec, bg.graph, nullptr, nullptr, 0));
+  }
+  int slot = 0, pending = 0;
+  int64_t launched = 0;
+  for (;;) {
+    if (bg.exec) {
+      JNO_HIP(hipGraphLaunch(bg.exec, s));
+    } else {
+      for (int i = 0; i < chunk; ++i) {
+        rc = launch_step(s, i);
+        if (rc) return rc;
+      }
+    }
+    launched += chunk;
+    JNO_HIP(hipMemcpyAsync(&h_act[slot], d_active, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipEventRecord(ev[slot], s));
+    if (pending) {
+      JNO_HIP(hipEventSynchronize(ev[slot ^ 1]));
+      if (h_act[slot ^ 1] <= 0) break;
+    }
+    if (launched > max_steps) break;
+    pending = 1;
+    slot ^= 1;
+  }
+  JNO_HIP(hipStreamSynchronize(s));
+  *launched_out = launched;
+  return JNO_OK;
+}
+
+static int batch_impl(int n_items, const jno_batch_item* h_items, double tol, double atol, int64_t maxiter,
+                      double replace_factor, int steps_per_graph, jno_batch_tuning* h_tuning,
+                      jno_pcg_stats* h_stats, void* d_ws, size_t ws_bytes, void* stream) {
   JNO_REQUIRE(n_items > 0 && h_items && h_stats, JNO_EINVAL, "jno_batch_pcg_mixed: empty batch");
   JNO_REQUIRE(replace_factor > 0.0 && replace_factor < 1.0, JNO_EINVAL, "jno_batch_pcg_mixed: replace_factor in (0,1)");
   JNO_REQUIRE(maxiter >= 0 && steps_per_graph >= 0, JNO_EINVAL, "jno_batch_pcg_mixed: bad maxiter/steps");
+  int rc = check_batch_tuning(h_tuning);
+  if (rc) return rc;
+  const bool want_one = !h_tuning || h_tuning->one_kernel;
+  const int64_t rpe = h_tuning && h_tuning->rows_per_entry ? h_tuning->rows_per_entry : B1_RPE;
   int64_t ntot, nsd;
   const int64_t ne = count_entries(n_items, h_items, &ntot, &nsd);
   JNO_REQUIRE(ne < INT32_MAX, JNO_EINVAL, "jno_batch_pcg_mixed: batch too large");
@@ -644,9 +1147,9 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
   BLayout L = blayout(n_items, ne, ntot, nsd, static_cast<char*>(d_ws));
   // host staging of the item descriptors and the entry table
   std::vector<BItem> items(n_items);
-  std::vector<int2> table(ne);
+  std::vector<int2> table;
   std::vector<const double*> dinvs(n_items);
-  int64_t off = 0, e = 0, soff = 0;
+  int64_t off = 0, soff = 0;
   bool any_dia = false, any_csr = false;
   for (int i = 0; i < n_items; ++i) {
     const jno_batch_item& h = h_items[i];
@@ -667,9 +1170,6 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
     it.v64 = h.op.vals64;
     it.b = h.b;
     it.x = h.x;
-    it.e0 = (int32_t)e;
-    it.ne = (int32_t)cdiv(it.n, BT_RPB);
-    for (int j = 0; j < it.ne; ++j) table[e++] = make_int2(i, j);
     dinvs[i] = h.dinv;
     it.dinv64 = h.dinv;
     it.s32 = it.format == JNO_FMT_DIA ? L.sdia + soff : nullptr;
@@ -679,34 +1179,13 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
     any_dia |= it.format == JNO_FMT_DIA;
     any_csr |= it.format == JNO_FMT_CSR;
   }
+  build_table(items, table, BT_RPB);
   JNO_HIP(hipMemcpyAsync(L.items, items.data(), sizeof(BItem) * n_items, hipMemcpyHostToDevice, s));
   JNO_HIP(hipMemcpyAsync(L.table, table.data(), sizeof(int2) * ne, hipMemcpyHostToDevice, s));
   JNO_HIP(hipMemcpyAsync(L.dinvs, dinvs.data(), sizeof(double*) * n_items, hipMemcpyHostToDevice, s));
   const int32_t h_active = n_items;
   JNO_HIP(hipMemcpyAsync(L.active, &h_active, sizeof(int32_t), hipMemcpyHostToDevice, s));
-  BArgs a;
-  a.items = L.items;
-  a.st = L.st;
-  a.table = L.table;
-  a.n_entries = (int32_t)ne;
-  a.rs = L.rs;
-  a.p[0] = L.p0;
-  a.p[1] = L.p1;
-  a.q = L.q;
-  a.xs = L.xs;
-  a.dinv = L.dinv;
-  a.PA = L.PA;
-  a.PB = L.PB;
-  a.active = L.active;
-  a.tol2 = tol * tol;
-  a.atol2_abs = atol * atol;
-  a.repl2 = replace_factor * replace_factor;
-  a.maxiter = maxiter;
   const int grid = (int)std::min<int64_t>(ne, BT_GRID);
-  batch_init_k<<<(unsigned)std::max<int64_t>(grid, cdiv(n_items, BT_NT)), BT_NT, 0, s>>>(a, n_items);
-  JNO_LAUNCH_CHECK();
-  batch_dinv_k<<<grid, BT_NT, 0, s>>>(L.items, L.dinvs, L.table, (int)ne, L.dinv);
-  JNO_LAUNCH_CHECK();
   if (any_dia) {
     // the DIA items' scaled operators; an item whose Jacobi scaling is not exact runs unscaled
     JNO_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t) * n_items, s));
@@ -716,88 +1195,159 @@ extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, d
     JNO_HIP(hipMemcpyAsync(bad.data(), L.bad, sizeof(int32_t) * n_items, hipMemcpyDeviceToHost, s));
     JNO_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < n_items; ++i) items[i].scaled = items[i].format == JNO_FMT_DIA && !bad[i];
-    JNO_HIP(hipMemcpyAsync(L.items, items.data(), sizeof(BItem) * n_items, hipMemcpyHostToDevice, s));
   }
+  const bool one = want_one && one_kernel_ok(items);
+  if (h_tuning) h_tuning->ran_one_kernel = one ? 1 : 0;
+  int64_t ne_run = ne;
+  if (one) ne_run = build_table(items, table, rpe);
+  JNO_HIP(hipMemcpyAsync(L.items, items.data(), sizeof(BItem) * n_items, hipMemcpyHostToDevice, s));
+  if (one) JNO_HIP(hipMemcpyAsync(L.table, table.data(), sizeof(int2) * ne_run, hipMemcpyHostToDevice, s));
   JNO_HIP(hipStreamSynchronize(s));  // host staging vectors go out of scope after this call
 
-  auto launch_kb = [&](hipStream_t st) -> int {
-    if (any_dia) batch_kb<true><<<grid, BT_NT, 0, st>>>(a);
-    if (any_csr) batch_kb<false><<<grid, BT_NT, 0, st>>>(a);
-    JNO_LAUNCH_CHECK();
-    return JNO_OK;
-  };
-  auto launch_step = [&](hipStream_t st) -> int {
-    if (any_dia) batch_ka<true><<<grid, BT_NT, 0, st>>>(a);
-    if (any_csr) batch_ka<false><<<grid, BT_NT, 0, st>>>(a);
-    if (any_dia) batch_ks<true><<<grid, BT_NT, 0, st>>>(a);
-    if (any_csr) batch_ks<false><<<grid, BT_NT, 0, st>>>(a);
-    JNO_LAUNCH_CHECK();
-    return launch_kb(st);
-  };
-
-  int rc = g_bpoll.init();
-  if (rc) return rc;
-  int32_t* h_act = g_bpoll.h;
-  hipEvent_t* ev = g_bpoll.ev;
-  rc = launch_kb(s);  // the first true residual r = b
-  if (rc) return rc;
-  BatchGraph bg;
-  bg.s = s;
-  const int chunk = steps_per_graph > 0 ? steps_per_graph : 8;
-  if (steps_per_graph > 0) {
-    // capture one chunk of steps on a private stream (default priority), replay it on `s`
-    JNO_HIP(hipStreamCreateWithFlags(&bg.cap, hipStreamNonBlocking));
-    JNO_HIP(hipStreamBeginCapture(bg.cap, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk && rc == JNO_OK; ++i) rc = launch_step(bg.cap);
-    hipError_t ce = hipStreamEndCapture(bg.cap, &bg.graph);
-    if (rc) return rc;
-    JNO_HIP(ce);
-    JNO_HIP(hipGraphInstantiate(&bg.exec, bg.graph, nullptr, nullptr, 0));
-  }
-  hipGraphExec_t exec = bg.exec;
   const auto t0 = std::chrono::steady_clock::now();
-  int slot = 0, pending = 0;
   int64_t launched = 0;
-  // every item is DONE after at most maxiter iterations plus one replacement per iteration
-  const int64_t max_steps = 2 * (maxiter + 2) + 4 * (int64_t)chunk;
-  for (;;) {
-    if (exec) {
-      JNO_HIP(hipGraphLaunch(exec, s));
-    } else {
-      for (int i = 0; i < chunk; ++i) {
-        rc = launch_step(s);
-        if (rc) return rc;
-      }
-    }
-    launched += chunk;
-    JNO_HIP(hipMemcpyAsync(&h_act[slot], L.active, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    JNO_HIP(hipEventRecord(ev[slot], s));
-    if (pending) {
-      JNO_HIP(hipEventSynchronize(ev[slot ^ 1]));
-      if (h_act[slot ^ 1] <= 0) break;
-    }
-    if (launched > max_steps) break;
-    pending = 1;
-    slot ^= 1;
-  }
-  JNO_HIP(hipStreamSynchronize(s));
-  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  std::vector<BState> st(n_items);
-  JNO_HIP(hipMemcpyAsync(st.data(), L.st, sizeof(BState) * n_items, hipMemcpyDeviceToHost, s));
-  JNO_HIP(hipStreamSynchronize(s));
   int worst = JNO_OK;
+  std::vector<jno_pcg_stats> out(n_items);
+  if (one) {
+    B1Args a;
+    a.items = L.items;
+    a.st[0] = L.st1[0];
+    a.st[1] = L.st1[1];
+    a.table = L.table;
+    a.n_entries = (int32_t)ne_run;
+    a.rpe = (int32_t)rpe;
+    a.r[0] = L.rs;
+    a.r[1] = L.r1;
+    a.q[0] = L.q;
+    a.q[1] = L.q1;
+    a.p[0] = L.p0;
+    a.p[1] = L.p1;
+    a.xs = L.xs;
+    a.P[0] = L.PB;
+    a.P[1] = L.PB1;
+    a.active = L.active;
+    a.tol2 = tol * tol;
+    a.atol2_abs = atol * atol;
+    a.repl2 = replace_factor * replace_factor;
+    a.maxiter = maxiter;
+    const int g1 = h_tuning && h_tuning->grid ? h_tuning->grid : B1_GRID;
+    batch_one_init_k<<<(unsigned)std::max<int64_t>(std::min<int64_t>(ne_run, 4096), cdiv(n_items, B1_NT)), B1_NT, 0,
+                       s>>>(a, n_items);
+    JNO_LAUNCH_CHECK();
+    // launches alternate the state parity, so a chunk holds an even number of them
+    int chunk = steps_per_graph > 0 ? steps_per_graph : 8;
+    chunk += chunk & 1;
+    auto step = [&](hipStream_t st, int i) -> int {
+      batch_one_k<<<g1, B1_NT, 0, st>>>(a, i & 1);
+      JNO_LAUNCH_CHECK();
+      return JNO_OK;
+    };
+    // every replacement costs three launches (the stop, R2 and the resumed launch)
+    rc = run_steps(step, steps_per_graph > 0 ? chunk : 0, chunk, 4 * (maxiter + 2) + 4 * (int64_t)chunk, L.active, s,
+                   &launched);
+    if (rc) return rc;
+    std::vector<B1State> st(n_items);
+    JNO_HIP(hipMemcpyAsync(st.data(), L.st1[launched & 1], sizeof(B1State) * n_items, hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n_items; ++i) {
+      const double bn = sqrt(st[i].bb);
+      out[i].iterations = st[i].kr;
+      out[i].refinements = st[i].refinements;
+      out[i].relres = bn > 0 ? sqrt(st[i].rr_true) / bn : sqrt(st[i].rr_true);
+      out[i].relres_recursive = bn > 0 ? sqrt(st[i].rr_last) / bn : sqrt(st[i].rr_last);
+      const int status = st[i].phase == P1_DONE ? st[i].status : BS_RUNNING;
+      if (status == BS_NONFINITE) worst = JNO_ENONFINITE;
+      else if (status != BS_CONVERGED && worst == JNO_OK) worst = JNO_ENOTCONV;
+    }
+  } else {
+    BArgs a;
+    a.items = L.items;
+    a.st = L.st;
+    a.table = L.table;
+    a.n_entries = (int32_t)ne;
+    a.rs = L.rs;
+    a.p[0] = L.p0;
+    a.p[1] = L.p1;
+    a.q = L.q;
+    a.xs = L.xs;
+    a.dinv = L.dinv;
+    a.PA = L.PA;
+    a.PB = L.PB;
+    a.active = L.active;
+    a.tol2 = tol * tol;
+    a.atol2_abs = atol * atol;
+    a.repl2 = replace_factor * replace_factor;
+    a.maxiter = maxiter;
+    batch_init_k<<<(unsigned)std::max<int64_t>(grid, cdiv(n_items, BT_NT)), BT_NT, 0, s>>>(a, n_items);
+    JNO_LAUNCH_CHECK();
+    batch_dinv_k<<<grid, BT_NT, 0, s>>>(L.items, L.dinvs, L.table, (int)ne, L.dinv);
+    JNO_LAUNCH_CHECK();
+    auto launch_kb = [&](hipStream_t st) -> int {
+      if (any_dia) batch_kb<true><<<grid, BT_NT, 0, st>>>(a);
+      if (any_csr) batch_kb<false><<<grid, BT_NT, 0, st>>>(a);
+      JNO_LAUNCH_CHECK();
+      return JNO_OK;
+    };
+    auto step = [&](hipStream_t st, int) -> int {
+      if (any_dia) batch_ka<true><<<grid, BT_NT, 0, st>>>(a);
+      if (any_csr) batch_ka<false><<<grid, BT_NT, 0, st>>>(a);
+      if (any_dia) batch_ks<true><<<grid, BT_NT, 0, st>>>(a);
+      if (any_csr) batch_ks<false><<<grid, BT_NT, 0, st>>>(a);
+      JNO_LAUNCH_CHECK();
+      return launch_kb(st);
+    };
+    rc = launch_kb(s);  // the first true residual r = b
+    if (rc) return rc;
+    const int chunk = steps_per_graph > 0 ? steps_per_graph : 8;
+    // every item is DONE after at most maxiter iterations plus one replacement per iteration
+    rc = run_steps(step, steps_per_graph, chunk, 2 * (maxiter + 2) + 4 * (int64_t)chunk, L.active, s, &launched);
+    if (rc) return rc;
+    std::vector<BState> st(n_items);
+    JNO_HIP(hipMemcpyAsync(st.data(), L.st, sizeof(BState) * n_items, hipMemcpyDeviceToHost, s));
+    JNO_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n_items; ++i) {
+      const double bn = sqrt(st[i].bb);
+      out[i].iterations = st[i].k;
+      out[i].refinements = st[i].refinements;
+      out[i].relres = bn > 0 ? sqrt(st[i].rr_true) / bn : sqrt(st[i].rr_true);
+      out[i].relres_recursive = bn > 0 ? sqrt(st[i].rr_last) / bn : sqrt(st[i].rr_last);
+      if (st[i].status == BS_NONFINITE) worst = JNO_ENONFINITE;
+      else if (st[i].status != BS_CONVERGED && worst == JNO_OK) worst = JNO_ENOTCONV;
+    }
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int i = 0; i < n_items; ++i) {
-    const double bn = sqrt(st[i].bb);
-    h_stats[i].iterations = st[i].k;
-    h_stats[i].refinements = st[i].refinements;
-    h_stats[i].relres = bn > 0 ? sqrt(st[i].rr_true) / bn : sqrt(st[i].rr_true);
-    h_stats[i].relres_recursive = bn > 0 ? sqrt(st[i].rr_last) / bn : sqrt(st[i].rr_last);
-    h_stats[i].seconds = secs;
-    if (st[i].status == BS_NONFINITE) worst = JNO_ENONFINITE;
-    else if (st[i].status != BS_CONVERGED && worst == JNO_OK) worst = JNO_ENOTCONV;
+    out[i].seconds = secs;
+    h_stats[i] = out[i];
   }
   if (worst == JNO_ENONFINITE) set_error("jno_batch_pcg_mixed: a non-finite value in at least one system");
   if (worst == JNO_ENOTCONV) set_error("jno_batch_pcg_mixed: at least one system did not converge in %lld iterations",
                                        (long long)maxiter);
   return worst;
+}
+
+}  // namespace jno
+
+using namespace jno;
+
+extern "C" size_t jno_batch_workspace_size(int n_items, const jno_batch_item* h_items) {
+  if (n_items <= 0 || !h_items) return 0;
+  int64_t ntot, nsd;
+  const int64_t ne = count_entries(n_items, h_items, &ntot, &nsd);
+  return blayout(n_items, ne, ntot, nsd, nullptr).bytes;
+}
+
+extern "C" int jno_batch_pcg_mixed(int n_items, const jno_batch_item* h_items, double tol, double atol, int64_t maxiter,
+                                   double replace_factor, int steps_per_graph, jno_pcg_stats* h_stats, void* d_ws,
+                                   size_t ws_bytes, void* stream) {
+  return batch_impl(n_items, h_items, tol, atol, maxiter, replace_factor, steps_per_graph, nullptr, h_stats, d_ws,
+                    ws_bytes, stream);
+}
+
+extern "C" int jno_batch_pcg_mixed_tuned(int n_items, const jno_batch_item* h_items, double tol, double atol,
+                                         int64_t maxiter, double replace_factor, int steps_per_graph,
+                                         jno_batch_tuning* h_tuning, jno_pcg_stats* h_stats, void* d_ws,
+                                         size_t ws_bytes, void* stream) {
+  return batch_impl(n_items, h_items, tol, atol, maxiter, replace_factor, steps_per_graph, h_tuning, h_stats, d_ws,
+                    ws_bytes, stream);
 }

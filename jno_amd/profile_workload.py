@@ -7,7 +7,8 @@ Builds the configuration's system (128^3 P1 Poisson for c2, 64^3 for c3, 32^3 P2
 c5; --format ebe: its element-by-element operator), then runs exactly --iters Jacobi-PCG iterations (no convergence test, no residual guard), and
 finally copies a buffer of --copy-mib MiB with a device-to-device copy kernel: the copy's
 FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels' counters.
---case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver;
+--case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver
+(--batch-step one|three: the one-kernel step, the default, or the three-kernel step);
 --case heat marches --iters backward-Euler steps of the n^3 transient heat block (--n, default 128).
 --case assemble re-assembles the n^3 P1 system (--n, default 128) --iters times on the structured-box
 kernel and then --iters times on the incidence-gather kernel, both writing the DIA layout in the
@@ -134,6 +135,8 @@ def main():
     ap.add_argument("--no-pace", action="store_true", help="fixed chunks for the one-kernel inner loops")
     ap.add_argument("--replace-factor", type=float, default=None,
                     help="residual replacement factor of the heat case (default: the step solver's)")
+    ap.add_argument("--batch-step", default="one", choices=("one", "three"),
+                    help="c4: one kernel per step (default) or the three-kernel step")
     ap.add_argument("--tuning", default=None,
                     help="launch shape block,grid_cap,k1_vector,mode of the iterations (default: the library's)")
     args = ap.parse_args()
@@ -145,15 +148,18 @@ def main():
         from jno_amd.batch import BatchSolver
 
         systems = build_batch(dev)
-        bs = BatchSolver(tol=1e-10, steps_per_graph=8)
+        bs = BatchSolver(tol=1e-10, steps_per_graph=8, one_kernel=args.batch_step == "one")
         bs(systems)  # warm (graphs, code objects)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         bs(systems)
         torch.cuda.synchronize()
         its = [st["iterations"] for st in bs.last_stats]
-        print(f"c4 batch: {len(systems)} systems, {sum(A.shape[0] for A, _ in systems)} rows, "
-              f"{(time.perf_counter() - t0) * 1e3:.1f} ms, iterations {min(its)}..{max(its)}")
+        refs = [st["refinements"] for st in bs.last_stats]
+        print(f"c4 batch ({'one kernel' if bs.last_one_kernel else 'three kernels'} per step): {len(systems)} "
+              f"systems, {sum(A.shape[0] for A, _ in systems)} rows, {(time.perf_counter() - t0) * 1e3:.1f} ms, "
+              f"iterations {min(its)}..{max(its)} (sum {sum(its)}), refinements {min(refs)}..{max(refs)}")
+        calibration_copy(dev, args.copy_mib)
         return
     if args.no_pace:
         from jno_amd.solve import set_pacing

@@ -30,14 +30,16 @@ def _systems(ns, seed0=100, fmt_csr=()):
     return out
 
 
-@pytest.mark.parametrize("graph", [0, 4])
-def test_uneven_batch_matches_oracle(dev, graph):
+@pytest.mark.parametrize("graph,one", [(0, True), (4, True), (0, False), (4, False)])
+def test_uneven_batch_matches_oracle(dev, graph, one):
     from jno_amd.batch import BatchSolver
 
     ns = [3, 5, 8, 12, 4, 9]
     sy = _systems(ns)
-    bs = BatchSolver(tol=1e-10, steps_per_graph=graph)
+    bs = BatchSolver(tol=1e-10, steps_per_graph=graph, one_kernel=one)
     xs = bs([(A, b) for _, _, A, b in sy])
+    # Kuhn lattices on their Jacobi-scaled DIA layout take the one-kernel step when asked to
+    assert bs.last_one_kernel == one
     for (n, k, A, b), x, st in zip(sy, xs, bs.last_stats):
         p, t = O.kuhn_box(n)
         Ao, bo, d = O.poisson_p1_system(p, t, k)
@@ -47,6 +49,42 @@ def test_uneven_batch_matches_oracle(dev, graph):
     # larger systems need more iterations; each system kept its own count
     its = [s["iterations"] for s in bs.last_stats]
     assert its[3] > its[0]
+
+
+@pytest.mark.parametrize("rpe,grid,graph", [(2048, 8, 6), (4096, 0, 8), (8192, 24, 3), (2048, 2048, 0)])
+def test_one_kernel_batch_launch_shapes(dev, rpe, grid, graph):
+    # entries of several tiles, a grid smaller than the entry count (grid-stride within the XCD
+    # slabs), an odd chunk (rounded up to even: launches alternate the state parity), direct
+    # launches; systems of 1 to 58 tiles of 2048 rows, the largest (49^3 = 117,649 rows) with
+    # offsets 1, 49 and 2401
+    from jno_amd.batch import BatchSolver
+
+    ns = [3, 24, 7, 48, 13]
+    sy = _systems(ns, seed0=300)
+    bs = BatchSolver(tol=1e-10, steps_per_graph=graph, rows_per_entry=rpe, grid=grid)
+    xs = bs([(A, b) for _, _, A, b in sy])
+    assert bs.last_one_kernel
+    for (n, k, A, b), x, st in zip(sy, xs, bs.last_stats):
+        assert st["relres"] <= 1e-10, (n, st)
+        if n <= 13:
+            p, t = O.kuhn_box(n)
+            Ao, bo, _ = O.poisson_p1_system(p, t, k)
+            assert _rel(_np(x), O.direct_solve(Ao, bo)) <= 1e-9, (n, st)
+        else:
+            # larger ones against the three-kernel step and the single-system solver
+            from jno_amd import precond, solve
+
+            u = solve.cg(tol=1e-10, precision="fp64")(A, b, M=precond.jacobi())
+            assert _rel(_np(x), _np(u)) <= 1e-9, (n, st)
+
+
+def test_one_kernel_batch_rejects_bad_shapes(dev):
+    from jno_amd.batch import BatchSolver
+
+    sy = _systems([4])
+    for kw in ({"rows_per_entry": 3000}, {"grid": 12}, {"rows_per_entry": 2 * 65536}):
+        with pytest.raises(ValueError, match="jno_batch_tuning"):
+            BatchSolver(tol=1e-10, **kw)([(sy[0][2], sy[0][3])])
 
 
 def test_batch_with_csr_items_and_shared_operator(dev):
@@ -62,6 +100,8 @@ def test_batch_with_csr_items_and_shared_operator(dev):
     rhs = [torch.tensor(rng.standard_normal(A.shape[0]), device="cuda") for _ in range(5)]
     bs = BatchSolver(tol=1e-10, steps_per_graph=8)
     xs = bs([(A, r) for r in rhs] + [(A2, sy[1][3])])
+    # a CSR item: the whole batch takes the three-kernel step
+    assert not bs.last_one_kernel
     p, t = O.kuhn_box(n)
     Ao, bo, d = O.poisson_p1_system(p, t, k)
     for r, x in zip(rhs, xs[:5]):
@@ -106,6 +146,7 @@ def test_c4_batch_256_uneven_and_64_rhs(dev):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     assert all(s["relres"] <= 1e-10 for s in bs.last_stats)
+    assert bs.last_one_kernel
     # spot-check against the single-system mixed solver
     single = solve.cg(tol=1e-10, precision="mixed")
     for i in (0, int(np.argmax(ns)), int(np.argmin(ns))):

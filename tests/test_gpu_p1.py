@@ -602,9 +602,9 @@ def test_autotuner_choices(dev):
     Ao, bo, _ = O.poisson_p1_system(p, t, k)
     assert _rel(_np(u), O.direct_solve(Ao, bo)) <= SOL_TOL
     systems = [_system(nn, O.k_lognormal(6 * nn**3, 60 + nn))[2:] for nn in (5, 7, 6)]
-    c = autotune.tune_batch(systems, chunks=(2, 8), tol=1e-10)
-    assert c in (2, 8)
-    xs = BatchSolver(tol=1e-10, steps_per_graph=c)(systems)
+    kw = autotune.tune_batch(systems, chunks=(2, 8), shapes=((2048, 8), (4096, 2048)), tol=1e-10)
+    assert kw["steps_per_graph"] in (2, 8) and (kw["rows_per_entry"], kw["grid"]) in ((2048, 8), (4096, 2048))
+    xs = BatchSolver(tol=1e-10, **kw)(systems)
     for (Ai, bi), xi in zip(systems, xs):
         # the solver stops on its own FP64 residual at 1e-10; recomputed here in another order
         assert float(torch.linalg.norm(bi - Ai.mv(xi)) / torch.linalg.norm(bi)) <= 1.5e-10
