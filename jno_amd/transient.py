@@ -14,6 +14,11 @@ its Jacobi preconditioner are formed once per (dt, theta) -- here in one HIP pas
 the explicit operator -- and every step is one fused right-hand-side pass plus the Jacobi-PCG solve
 of the steady path (same operator layouts, same kernels), warm-started from the current state.
 
+A time-dependent operator (``operator_fn(t, args)``, backend_blocks.py:335) is evaluated once per
+step at the step's end time t + dt and used for both halves of the step, as the reference does; the
+step and explicit operators and the exact step diagonal for Jacobi are then formed per step (the
+per-step branch of compose_transient_step_solvers, solver_api.py:923-935).
+
 ``theta_march`` makes the march differentiable in the per-element coefficient ``k`` and the initial
 state: the backward sweep solves with the (symmetric) step operator and contracts each step's
 contribution on the device (the reference differentiates its ``lax.scan`` in reverse mode,
@@ -50,11 +55,17 @@ class SemidiscreteTimeBlock:
     when the step operators are formed), ``A``/``affine_bias`` the eliminated operator and load,
     ``g`` the Dirichlet values (None: 0).  ``theta`` is the block's default scheme (the reference's
     ``metadata["theta"]``), ``dt`` its assembled step.
+
+    ``operator_fn(t, args)``, when given, makes the operator time-dependent: it returns the
+    eliminated operator at time t on the block's pattern, or the pair (operator, affine bias) when
+    the bias moves with it (the symmetric elimination's lift -A e(g) does).  ``A`` and
+    ``affine_bias`` are then the operator and bias at t0.
     """
 
     def __init__(self, M: CSR, A: CSR, affine_bias: Optional[torch.Tensor], *, dmask=None, g=None,
                  forcing_vector_fn: Optional[Callable] = None, state0: Optional[torch.Tensor] = None,
-                 t0: float = 0.0, t1: Optional[float] = None, dt: Optional[float] = None, theta: float = 1.0):
+                 t0: float = 0.0, t1: Optional[float] = None, dt: Optional[float] = None, theta: float = 1.0,
+                 operator_fn: Optional[Callable] = None):
         if M.shape != A.shape or M.nnz != A.nnz:
             raise ValueError("M and A must share one CSR pattern")
         if not (torch.equal(M.rowptr, A.rowptr) and torch.equal(M.col, A.col)):
@@ -66,17 +77,26 @@ class SemidiscreteTimeBlock:
         self.c = affine_bias
         self.dmask, self.g = dmask, g
         self.forcing_vector_fn = forcing_vector_fn
+        self.operator_fn = operator_fn
         self.state0 = state0
         self.t0, self.t1, self.dt = float(t0), t1, dt
         self.theta = float(theta)
         self._ops: dict = {}
 
     @classmethod
-    def from_system(cls, system: FemLinearSystem, args: dict, **kw) -> "SemidiscreteTimeBlock":
+    def from_system(cls, system: FemLinearSystem, args: dict, *, coefficient_fn: Optional[Callable] = None,
+                    **kw) -> "SemidiscreteTimeBlock":
         """The heat block of ``system`` at ``args``: ``u_t v + k grad u . grad v - f v`` with the
-        system's Dirichlet set, g and load."""
+        system's Dirichlet set, g and load.  ``coefficient_fn(t, args)`` -> the per-element
+        coefficient at time t makes the operator time-dependent (re-assembled at every step's end
+        time, with its lift)."""
         A, c = system.evaluate(args)
         M = system.mass()
+        if coefficient_fn is not None:
+            def operator_fn(t, a=None, _s=system, _f=coefficient_fn):
+                return _s.evaluate({_s.param: _f(t, a)})
+
+            kw["operator_fn"] = operator_fn
         return cls(M, A, c, dmask=system.dmask, g=system.g, **kw)
 
     # -- step operators ----------------------------------------------------------------------
@@ -89,7 +109,33 @@ class SemidiscreteTimeBlock:
         hit = self._ops.get(key)
         if hit is not None:
             return hit
-        A = self.A
+        self._ops[key] = self._form_step(self.A, dt, theta)
+        return self._ops[key]
+
+    def operator_at(self, t: float, args=None):
+        """(A, c): the operator and affine bias at time t (the block's own unless operator_fn)."""
+        if self.operator_fn is None:
+            return self.A, self.c
+        res = self.operator_fn(t, args)
+        A, c = res if isinstance(res, tuple) else (res, self.c)
+        if not isinstance(A, CSR) or A.shape != self.A.shape or A.nnz != self.A.nnz:
+            raise ValueError("operator_fn must return a CSR operator on the block's pattern")
+        if A.rowptr is not self.A.rowptr and not (torch.equal(A.rowptr, self.A.rowptr) and
+                                                  torch.equal(A.col, self.A.col)):
+            raise ValueError("operator_fn must return a CSR operator on the block's pattern")
+        return A, c
+
+    def step_operators_at(self, t_next: float, dt: float, theta: float, args=None):
+        """(K, E, c) of the step ending at t_next: the cached constant-operator pair, or for a
+        time-dependent operator the pair formed from operator_fn(t_next, args) (not cached)."""
+        if self.operator_fn is None:
+            K, E = self.step_operators(dt, theta)
+            return K, E, self.c
+        A, c = self.operator_at(t_next, args)
+        K, E = self._form_step(A, dt, theta)
+        return K, E, c
+
+    def _form_step(self, A: CSR, dt: float, theta: float):
         kv = torch.empty_like(A.vals)
         ev = torch.empty_like(A.vals)
         kd = torch.empty(self.n, dtype=torch.float64, device=self.device)
@@ -98,7 +144,6 @@ class SemidiscreteTimeBlock:
                   _lib.ptr(kd), _lib.stream_ptr(self.device))
         K = CSR(A.rowptr, A.col, kv, A.shape, kd, family="theta-step")
         E = SolverOperator(CSR(A.rowptr, A.col, ev, A.shape, family="theta-explicit"), fmt="auto", fp32=False)
-        self._ops[key] = (K, E)
         return K, E
 
     def _forcing(self, t, args):
@@ -110,19 +155,22 @@ class SemidiscreteTimeBlock:
             raise ValueError(f"forcing_vector_fn returned {f.shape[0]} entries for {self.n} dofs")
         return f.to(torch.float64).contiguous()
 
-    def rhs(self, u: torch.Tensor, t: float, dt: float, theta: float, args=None, out=None) -> torch.Tensor:
-        """E u + dt c + dt (theta f(t+dt) + (1-theta) f(t)) in one pass."""
-        _, E = self.step_operators(dt, theta)
+    def rhs(self, u: torch.Tensor, t: float, dt: float, theta: float, args=None, out=None, ops=None) -> torch.Tensor:
+        """E u + dt c + dt (theta f(t+dt) + (1-theta) f(t)) in one pass (``ops`` = the step's
+        (K, E, c), formed here when not given)."""
+        _, E, c = ops if ops is not None else self.step_operators_at(t + dt, dt, theta, args)
         f0 = self._forcing(t, args)
         f1 = self._forcing(t + dt, args) if f0 is not None else None
         y = torch.empty(self.n, dtype=torch.float64, device=self.device) if out is None else out
-        _lib.call("jno_theta_rhs", C.byref(E.op), _lib.ptr(u), _lib.ptr(self.c), _lib.ptr(f0), _lib.ptr(f1), float(dt),
+        _lib.call("jno_theta_rhs", C.byref(E.op), _lib.ptr(u), _lib.ptr(c), _lib.ptr(f0), _lib.ptr(f1), float(dt),
                   float(theta), _lib.ptr(y), _lib.stream_ptr(self.device))
         return y
 
-    def apply_explicit(self, x: torch.Tensor, dt: float, theta: float) -> torch.Tensor:
-        """E x alone (the explicit operator is symmetric: also the transposed step)."""
-        _, E = self.step_operators(dt, theta)
+    def apply_explicit(self, x: torch.Tensor, dt: float, theta: float, E=None) -> torch.Tensor:
+        """E x alone (the explicit operator is symmetric: also the transposed step); ``E`` = a
+        step's explicit operator (default: the constant operator's)."""
+        if E is None:
+            _, E = self.step_operators(dt, theta)
         x = x.to(torch.float64).contiguous()
         y = torch.empty(self.n, dtype=torch.float64, device=self.device)
         _lib.call("jno_theta_rhs", C.byref(E.op), _lib.ptr(x), None, None, None, float(dt), float(theta), _lib.ptr(y),
@@ -146,9 +194,10 @@ class SemidiscreteTimeBlock:
         u = u.to(torch.float64).contiguous()
         if u.shape[0] != self.n:
             raise ValueError(f"state has {u.shape[0]} entries for {self.n} dofs")
-        K, _ = self.step_operators(dt, theta)
-        b = self.rhs(u, t, dt, theta, args)
+        ops = self.step_operators_at(t + dt, dt, theta, args)
+        b = self.rhs(u, t, dt, theta, args, ops=ops)
         solver = linear_solve if linear_solve is not None else default_step_solver()
+        K = ops[0]
         return solver(K, b, M=Diagonal(K.dinv), x0=u)
 
     def march(self, u0: torch.Tensor, n_steps: int, dt: Optional[float] = None, args=None,
@@ -173,12 +222,17 @@ class SemidiscreteTimeBlock:
 
 class _ThetaMarch(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, k, u0, system: FemLinearSystem, dt, theta, n_steps, save, solver, forcing):
-        block = SemidiscreteTimeBlock.from_system(system, {system.param: k.detach()}, dt=dt, theta=theta,
-                                                  forcing_vector_fn=forcing)
+    def forward(ctx, k, u0, system: FemLinearSystem, dt, theta, n_steps, save, solver, forcing, profile):
+        kd = k.detach()
+        kw = {}
+        if profile is not None:
+            kw["coefficient_fn"] = lambda t, a=None, _k=kd, _p=profile: float(_p(t)) * _k
+        block = SemidiscreteTimeBlock.from_system(system, {system.param: kd if profile is None else
+                                                           float(profile(0.0)) * kd},
+                                                  dt=dt, theta=theta, forcing_vector_fn=forcing, **kw)
         states = block.march(u0.detach(), n_steps, linear_solve=solver, keep="all")
         ctx.block, ctx.states, ctx.system, ctx.solver, ctx.save = block, states, system, solver, save
-        ctx.dt, ctx.theta = float(dt), float(theta)
+        ctx.dt, ctx.theta, ctx.profile = float(dt), float(theta), profile
         return torch.stack([states[i] for i in save])
 
     @staticmethod
@@ -190,7 +244,6 @@ class _ThetaMarch(torch.autograd.Function):
         seed = {}
         for j, i in enumerate(ctx.save):
             seed[i] = seed[i] + gy[j] if i in seed else gy[j].clone()
-        K, _ = block.step_operators(dt, theta)
         m = system.mesh
         grad = torch.zeros(m.n_cells, dtype=torch.float64, device=m.device)
         lam = torch.zeros(block.n, dtype=torch.float64, device=m.device)
@@ -199,21 +252,28 @@ class _ThetaMarch(torch.autograd.Function):
         for n in range(max(seed) - 1, -1, -1):
             if n + 1 in seed:
                 lam = lam + seed[n + 1]
+            # the step's operators (re-formed at its end time when the operator depends on time); k_e
+            # enters that step's operator scaled by the profile there
+            t_next = block.t0 + (n + 1) * dt
+            K, E, _ = block.step_operators_at(t_next, dt, theta)
+            scale = dt if ctx.profile is None else dt * float(ctx.profile(t_next))
             mu = ctx.solver(K, lam.contiguous(), M=Diagonal(K.dinv))
             _lib.call("jno_theta_mix", block.n, _lib.ptr(states[n + 1]), _lib.ptr(states[n]), _lib.ptr(block.dmask),
                       _lib.ptr(block.g), theta, _lib.ptr(w), _lib.stream_ptr(m.device))
-            grad += dt * system.grad_k(w, mu)
-            lam = block.apply_explicit(mu, dt, theta)
+            grad += scale * system.grad_k(w, mu)
+            lam = block.apply_explicit(mu, dt, theta, E=E)
         if 0 in seed:
             lam = lam + seed[0]
-        return grad, lam, None, None, None, None, None, None, None
+        return grad, lam, None, None, None, None, None, None, None, None
 
 
 def theta_march(system: FemLinearSystem, k: torch.Tensor, u0: torch.Tensor, dt: float, n_steps: int,
                 theta: float = 1.0, save: Optional[Sequence[int]] = None, solver: Optional[LinearSolver] = None,
-                forcing: Optional[Callable] = None) -> torch.Tensor:
+                forcing: Optional[Callable] = None, profile: Optional[Callable] = None) -> torch.Tensor:
     """States ``u_i`` for i in ``save`` (default: the last) of the heat march of ``system`` at the
-    per-element coefficient ``k``, stacked (len(save), n); differentiable in ``k`` and ``u0``."""
+    per-element coefficient ``k``, stacked (len(save), n); differentiable in ``k`` and ``u0``.
+    ``profile(t)`` (a float) makes the coefficient time-dependent, profile(t) * k: the operator is
+    re-assembled at every step's end time, and the gradient carries each step's factor."""
     save = [int(n_steps)] if save is None else [int(i) for i in save]
     if any(i < 0 or i > n_steps for i in save):
         raise ValueError(f"save indices must lie in 0..{n_steps}")
@@ -222,4 +282,4 @@ def theta_march(system: FemLinearSystem, k: torch.Tensor, u0: torch.Tensor, dt: 
     if k.ndim == 0:
         k = k.expand(system.mesh.n_cells)
     u0 = torch.as_tensor(u0, dtype=torch.float64, device=system.mesh.device).reshape(-1)
-    return _ThetaMarch.apply(k, u0, system, float(dt), float(theta), int(n_steps), save, solver, forcing)
+    return _ThetaMarch.apply(k, u0, system, float(dt), float(theta), int(n_steps), save, solver, forcing, profile)

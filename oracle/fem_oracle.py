@@ -632,6 +632,43 @@ def theta_march(Mz, A, c, u0, dt, theta, n_steps, forcing=None, t0=0.0):
     return us
 
 
+def theta_march_tv(Mz, system_at, u0, dt, theta, n_steps, forcing=None, t0=0.0):
+    """States u_0..u_n of a march whose operator depends on time: step n uses ``system_at(t_{n+1})``
+    = (A, c) for both halves of the theta-step, as ``SemidiscreteTimeBlock.step`` evaluates
+    ``operator_fn(t_next, args)`` once per step (jno/utils/solver/backend_blocks.py:335, :342-344;
+    the per-step operator of compose_transient_step_solvers, jno/utils/solver/solver_api.py:
+    923-935), each step solved directly.  A constant ``system_at`` reproduces ``theta_march``."""
+    us = [np.array(u0, float)]
+    t = t0
+    for _ in range(n_steps):
+        A, c = system_at(t + dt)
+        K, _ = theta_operators(Mz, A, dt, theta)
+        fn = (forcing(t), forcing(t + dt)) if forcing is not None else (None, None)
+        us.append(spla.spsolve(K.tocsc(), theta_rhs(Mz, A, c, us[-1], dt, theta, *fn)))
+        t += dt
+    return us
+
+
+def theta_march_tv_grad_k(points, cells, Mz, system_at, profile, us, dt, theta, dofs, g, lam_final, t0=0.0):
+    """dL/dk_e along a march whose coefficient at time t is profile(t) * k (``system_at(t)`` = the
+    eliminated (A, c) at that coefficient): the reverse sweep of ``theta_march_grad_k`` with the
+    step operators of each step and the contraction scaled by profile(t_{n+1}), the factor by
+    which k_e enters that step's operator.  Returns (dL/dk, dL/du_0)."""
+    dofs = np.asarray(dofs, np.int64)
+    grad = np.zeros(cells.shape[0])
+    lam = np.array(lam_final, float)
+    for n in range(len(us) - 2, -1, -1):
+        tn1 = t0 + (n + 1) * dt
+        A, _ = system_at(tn1)
+        K, E = theta_operators(Mz, A, dt, theta)
+        mu = spla.spsolve(K.tocsc(), lam)
+        w = theta * us[n + 1] + (1.0 - theta) * us[n]
+        w[dofs] = g
+        grad += dt * profile(tn1) * adjoint_grad_k_p1(points, cells, w, mu, dofs)
+        lam = E @ mu
+    return grad, lam
+
+
 def theta_march_grad_k(points, cells, Mz, A, us, dt, theta, dofs, g, lam_final):
     """dL/dk_e for L with dL/du_N = ``lam_final`` along the march ``us``: the reverse sweep
     mu = K^-1 lam_{n+1}, dL/dk += dt * adjoint_grad_k(w_n, mu) with w_n = theta u_{n+1} +

@@ -213,6 +213,104 @@ def test_march_gradient_through_saved_states_matches_fd(dev):
         assert abs(float(kt.grad[e]) - fd) <= 1e-5 * max(abs(fd), 1e-4), (e, float(kt.grad[e]), fd)
 
 
+def _tv_oracle(n, k, phi, f=1.0, g=0.0):
+    p, t = O.kuhn_box(n)
+    _, _, d = O.poisson_p1_system(p, t, k, f=f, g=g)
+    Mz = O.transient_dirichlet(O.mass_matrix_p1(p, t), d)
+
+    def system_at(tt):
+        A, c, _ = O.poisson_p1_system(p, t, phi(tt) * k, f=f, g=g)
+        return A, c
+
+    return p, t, d, Mz, system_at
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_time_dependent_operator_matches_oracle(dev, theta):
+    # operator_fn(t, args): the coefficient phi(t) k re-assembled at every step's end time, the
+    # lift with it; states against the oracle's direct-solve march of the same time-dependent block
+    import jno_amd as J
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    n = 8
+    k = O.k_lognormal(6 * n**3, 9)
+    phi = lambda tt: 1.0 + 40.0 * tt  # noqa: E731
+    g, dt, N = 0.3, 0.01, 5
+    p, t, d, Mz, system_at = _tv_oracle(n, k, phi, f=1.0, g=g)
+    u0 = 0.2 * np.random.default_rng(2).standard_normal(p.shape[0])
+    ref = O.theta_march_tv(Mz, system_at, u0, dt, theta, N)
+    m = J.kuhn_box(n)
+    sysg = J.fem_poisson(m, f=1.0, g=g)
+    kd = torch.as_tensor(k, device="cuda")
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": kd}, coefficient_fn=lambda tt, a: phi(tt) * kd, dt=dt,
+                                            theta=theta)
+    us = blk.march(torch.as_tensor(u0, device="cuda"), N, keep="all")
+    for i in (1, 3, N):
+        assert _rel(_np(us[i]), ref[i]) < 1e-8, i
+    # the constant-operator march differs (the check sees the time dependence)
+    const = SemidiscreteTimeBlock.from_system(sysg, {"k": kd}, dt=dt, theta=theta).march(
+        torch.as_tensor(u0, device="cuda"), N)
+    assert _rel(_np(const), ref[N]) > 1e-3
+    # an operator on another pattern is refused
+    bad = SemidiscreteTimeBlock(blk.M, blk.A, blk.c, dmask=blk.dmask, g=blk.g, dt=dt,
+                                operator_fn=lambda tt, a: J.fem_poisson(J.kuhn_box(n - 1)).evaluate({"k": 1.0}))
+    with pytest.raises(ValueError, match="pattern"):
+        bad.step(us[0], 0.0)
+
+
+def test_time_dependent_amplification_closed_form(dev):
+    # a generalized eigenvector of the free block is multiplied by
+    # prod_n (1 - (1-theta) dt phi(t_{n+1}) lam) / (1 + theta dt phi(t_{n+1}) lam) over the march
+    import jno_amd as J
+    import scipy.linalg
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    n, dt, N, theta = 5, 0.004, 6, 0.5
+    k = O.k_lognormal(6 * n**3, 4)
+    phi = lambda tt: 1.0 + 40.0 * tt  # noqa: E731
+    p, t, d, Mz, system_at = _tv_oracle(n, k, phi, f=0.0)
+    A1, _ = system_at(0.0)
+    free = np.setdiff1d(np.arange(p.shape[0]), d)
+    lam, V = scipy.linalg.eigh(A1.toarray()[np.ix_(free, free)], Mz.toarray()[np.ix_(free, free)])
+    sysg = J.fem_poisson(J.kuhn_box(n), f=0.0)
+    kd = torch.as_tensor(k, device="cuda")
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": kd}, coefficient_fn=lambda tt, a: phi(tt) * kd, dt=dt,
+                                            theta=theta)
+    for j in (0, 11):
+        v = np.zeros(p.shape[0])
+        v[free] = V[:, j]
+        uN = blk.march(torch.as_tensor(v, device="cuda"), N, linear_solve=J.solve.cg(tol=1e-12))
+        R = np.prod([(1 - (1 - theta) * dt * phi((i + 1) * dt) * lam[j]) / (1 + theta * dt * phi((i + 1) * dt) * lam[j])
+                     for i in range(N)])
+        # each step solved to 1e-12 of its own right-hand side: errors relative to the initial mode
+        assert np.linalg.norm(_np(uN) - R * v) < 1e-10 * np.linalg.norm(v), j
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_time_dependent_march_gradient_matches_oracle_adjoint(dev, theta):
+    import jno_amd as J
+
+    n = 4
+    p, t = O.kuhn_box(n)
+    T = t.shape[0]
+    k = O.k_lognormal(T, 17)
+    phi = lambda tt: 1.0 + 30.0 * tt + 0.5 * np.sin(200.0 * tt)  # noqa: E731
+    g, dt, N = 0.3, 0.02, 4
+    _, _, d, Mz, system_at = _tv_oracle(n, k, phi, f=1.0, g=g)
+    u0 = 0.1 * np.random.default_rng(5).standard_normal(p.shape[0])
+    w = np.random.default_rng(6).standard_normal(p.shape[0])
+    us = O.theta_march_tv(Mz, system_at, u0, dt, theta, N)
+    gk_ref, gu_ref = O.theta_march_tv_grad_k(p, t, Mz, system_at, phi, us, dt, theta, d, g, w)
+    sysg = J.fem_poisson(J.kuhn_box(n), f=1.0, g=g)
+    kt = torch.as_tensor(k, device="cuda").requires_grad_(True)
+    u0t = torch.as_tensor(u0, device="cuda").requires_grad_(True)
+    out = J.theta_march(sysg, kt, u0t, dt, N, theta=theta, solver=J.solve.cg(tol=1e-12), profile=phi)
+    assert _rel(_np(out[-1]), us[-1]) < 1e-9
+    (out[-1] @ torch.as_tensor(w, device="cuda")).backward()
+    assert _rel(_np(kt.grad), gk_ref) < 1e-6
+    assert _rel(_np(u0t.grad), gu_ref) < 1e-6
+
+
 def test_large_march_properties(dev):
     # 64^3 cells (274,625 dofs): energy decay of the homogeneous backward-Euler march, per-step
     # residual of the step solve, Dirichlet rows kept at g, mixed-precision steps agree with FP64

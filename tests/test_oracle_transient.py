@@ -131,3 +131,85 @@ def test_transient_adjoint_matches_central_fd(theta):
     fd0 = (w @ O.theta_march(Mz, A, c, u0 + h * v, dt, theta, N)[-1]
            - w @ O.theta_march(Mz, A, c, u0 - h * v, dt, theta, N)[-1]) / (2 * h)
     assert abs(gu0 @ v - fd0) < 1e-9 * max(1.0, abs(fd0))
+
+
+def _tv_block(n, k, profile, f=0.0, g=0.0):
+    # the eliminated system at coefficient profile(t) * k, and the transient mass of its Dirichlet set
+    p, t = O.kuhn_box(n)
+    _, _, d = O.poisson_p1_system(p, t, k, f=f, g=g)
+    Mz = O.transient_dirichlet(O.mass_matrix_p1(p, t), d)
+
+    def system_at(tt):
+        A, c, _ = O.poisson_p1_system(p, t, profile(tt) * k, f=f, g=g)
+        return A, c
+
+    return p, t, d, Mz, system_at
+
+
+def test_time_dependent_march_with_a_constant_operator_is_the_march():
+    p, t, d, Mz, system_at = _tv_block(4, O.k_lognormal(6 * 64, 2), lambda tt: 1.0, f=1.0, g=0.2)
+    A, c = system_at(0.0)
+    u0 = 0.3 * np.random.default_rng(1).standard_normal(p.shape[0])
+    for theta in (1.0, 0.5):
+        a = O.theta_march_tv(Mz, system_at, u0, 0.01, theta, 4)
+        b = O.theta_march(Mz, A, c, u0, 0.01, theta, 4)
+        assert max(np.abs(x - y).max() for x, y in zip(a, b)) < 1e-12
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5, 0.7])
+def test_time_dependent_amplification_uses_the_operator_at_the_step_end(theta):
+    # with k(t) = phi(t) k the free block is phi(t) A_ff, so a generalized eigenvector A_ff v = lam M v
+    # is multiplied by prod_n (1 - (1-theta) dt phi(t_{n+1}) lam) / (1 + theta dt phi(t_{n+1}) lam):
+    # pins the evaluation time (t_{n+1} for both halves) and the theta weights
+    n, dt, N = 4, 0.004, 6
+    k = O.k_lognormal(6 * n**3, 3)
+    phi = lambda tt: 1.0 + 40.0 * tt  # noqa: E731
+    p, t, d, Mz, system_at = _tv_block(n, k, phi)
+    A1, _ = system_at(0.0)  # phi(0) = 1: the operator of k itself
+    free = np.setdiff1d(np.arange(p.shape[0]), d)
+    lam, V = scipy.linalg.eigh(A1.toarray()[np.ix_(free, free)], Mz.toarray()[np.ix_(free, free)])
+    for j in (0, 7, len(lam) - 1):
+        v = np.zeros(p.shape[0])
+        v[free] = V[:, j]
+        uN = O.theta_march_tv(Mz, system_at, v, dt, theta, N)[-1]
+        R = np.prod([(1 - (1 - theta) * dt * phi((i + 1) * dt) * lam[j]) / (1 + theta * dt * phi((i + 1) * dt) * lam[j])
+                     for i in range(N)])
+        assert np.abs(uN - R * v).max() < 1e-11 * np.abs(v).max(), j
+        # the operator at the step start would give another factor (the check can tell them apart)
+        R0 = np.prod([(1 - (1 - theta) * dt * phi(i * dt) * lam[j]) / (1 + theta * dt * phi(i * dt) * lam[j])
+                      for i in range(N)])
+        assert abs(R - R0) > 1e-4 * abs(R), j
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_time_dependent_adjoint_matches_central_fd(theta):
+    n = 3
+    p, t = O.kuhn_box(n)
+    T = t.shape[0]
+    k = O.k_lognormal(T, 12)
+    phi = lambda tt: 1.0 + 30.0 * tt + 0.5 * np.sin(200.0 * tt)  # noqa: E731
+    w = np.random.default_rng(3).standard_normal(p.shape[0])
+    dt, N, g = 0.02, 4, 0.3
+    u0 = 0.1 * np.random.default_rng(6).standard_normal(p.shape[0])
+
+    def run(kk):
+        _, _, d, Mz, system_at = _tv_block(n, kk, phi, f=1.0, g=g)
+        return O.theta_march_tv(Mz, system_at, u0, dt, theta, N), d, Mz, system_at
+
+    us, d, Mz, system_at = run(k)
+    grad, gu0 = O.theta_march_tv_grad_k(p, t, Mz, system_at, phi, us, dt, theta, d, g, w)
+    for e in (0, 9, T // 2, T - 1):
+        h = 1e-6 * k[e]
+        kp, km = k.copy(), k.copy()
+        kp[e] += h
+        km[e] -= h
+        fd = (w @ run(kp)[0][-1] - w @ run(km)[0][-1]) / (2 * h)
+        assert abs(grad[e] - fd) <= 1e-6 * max(abs(fd), 1e-3), (e, grad[e], fd)
+    # a profile left out of the contraction would be caught
+    grad1, _ = O.theta_march_tv_grad_k(p, t, Mz, system_at, lambda tt: 1.0, us, dt, theta, d, g, w)
+    assert np.abs(grad1 - grad).max() > 1e-3 * np.abs(grad).max()
+    v = np.random.default_rng(7).standard_normal(p.shape[0])
+    h = 1e-3
+    fd0 = (w @ O.theta_march_tv(Mz, system_at, u0 + h * v, dt, theta, N)[-1]
+           - w @ O.theta_march_tv(Mz, system_at, u0 - h * v, dt, theta, N)[-1]) / (2 * h)
+    assert abs(gu0 @ v - fd0) < 1e-9 * max(1.0, abs(fd0))
