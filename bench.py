@@ -328,6 +328,19 @@ def heat_rate(J, torch, dev, n=128, steps=20):
         res["step_operator"] = sv.last_stats["operator"]
     res["mixed_vs_fp64_rel_l2"] = float((finals["mixed"] - finals["fp64"]).norm() / finals["fp64"].norm())
     res["speedup_mixed_vs_fp64"] = round(res["fp64_ms_per_step"] / res["mixed_ms_per_step"], 3)
+    # a time-dependent coefficient (1 + 50 t) k: the operator re-assembled at every step's end time,
+    # the step and explicit operators re-formed on their diagonal layouts
+    blk_t = J.SemidiscreteTimeBlock.from_system(system, {"k": k}, dt=0.2 / 1500,
+                                                coefficient_fn=lambda t, a: (1.0 + 50.0 * t) * k)
+    sv = default_step_solver()
+    blk_t.march(u0, 3, linear_solve=sv)  # warm: the DIA offsets of the plan, the mass's layout
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    blk_t.march(u0, steps, linear_solve=sv)
+    torch.cuda.synchronize()
+    its = blk_t.last_iterations
+    res["time_dependent_mixed_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 4)
+    res["time_dependent_iterations_per_step"] = round(sum(its) / len(its), 2)
     return res
 
 

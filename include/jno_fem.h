@@ -524,6 +524,18 @@ int jno_compliance_grad_p2(const double* d_points2, const int32_t* d_cells10, in
 int jno_theta_operators(int64_t n_rows, const int32_t* d_rowptr, const int32_t* d_col,
                         const double* d_mass, const double* d_stiff, const uint8_t* d_dmask, double dt,
                         double theta, double* d_step, double* d_expl, double* d_step_dinv, void* stream);
+/* The same operators on the symmetric DIA layout (the time-dependent step on a structured box, where
+ * the operator's assembly writes its DIA layout and the mass's is formed once): d_mass_dia holds
+ * ndiag diagonals at h_offsets (main first), d_stiff_dia ndiag_stiff diagonals at h_offsets_stiff,
+ * each one of the mass's offsets; both with leading dimension n_rows rounded up to 64.  Writes K
+ * (d_step_dia, FP64, and d_step_dia32, FP32, nullable), E (d_expl_dia, nullable) on the mass's
+ * diagonals and 1/diag K, with the values jno_theta_operators gives bitwise.  No reference
+ * counterpart as a separate entry: the per-step operator of compose_transient_step_solvers
+ * (jno/utils/solver/solver_api.py:923-935) on the diagonal layout. */
+int jno_theta_operators_dia(int64_t n_rows, int ndiag, const int32_t* h_offsets, const double* d_mass_dia,
+                            int ndiag_stiff, const int32_t* h_offsets_stiff, const double* d_stiff_dia,
+                            const uint8_t* d_dmask, double dt, double theta, double* d_step_dia,
+                            float* d_step_dia32, double* d_expl_dia, double* d_step_dinv, void* stream);
 
 /* Right-hand side of one step: d_rhs = E u + dt c + dt (theta f(t+dt) + (1-theta) f(t))
  * (backend_blocks.py:342-344), E given as an operator layout (jno_op, FP64 values: symmetric DIA

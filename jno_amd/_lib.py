@@ -191,6 +191,8 @@ SIGNATURES = {
     "jno_traction_p2": (C.c_int, [P, P, I64, P, P, C.c_int, D, C.POINTER(D), P, P]),
     "jno_compliance_grad_p2": (C.c_int, [P, P, I64, D, P, P, P]),
     "jno_theta_operators": (C.c_int, [I64, P, P, P, P, P, D, D, P, P, P, P]),
+    "jno_theta_operators_dia": (C.c_int, [I64, C.c_int, C.POINTER(I32), P, C.c_int, C.POINTER(I32), P, P, D, D, P, P,
+                                          P, P, P]),
     "jno_theta_rhs": (C.c_int, [C.POINTER(JnoOp), P, P, P, P, D, D, P, P]),
     "jno_theta_mix": (C.c_int, [I64, P, P, P, P, D, P, P]),
 }

@@ -43,6 +43,40 @@ __global__ void __launch_bounds__(256) theta_ops_k(int64_t n, const int32_t* __r
   }
 }
 
+// The same operators on the diagonal layout, for a mass and an operator that both come with a
+// symmetric DIA layout (the structured box: the mass's nd diagonals hold the operator's nda):
+// per row and stored diagonal d of the mass, K_d = Mz_d + sk A_d and E_d = Mz_d + se A_d, rounded as
+// written, so the values equal theta_ops_k's bitwise; amap[d] is the operator's diagonal at the
+// mass's offset d (-1: none, a zero there).  The FP32 copy of K and 1/diag K come in the same pass.
+// No CSR is read: the time-dependent step re-forms these every step.
+__global__ void __launch_bounds__(256) theta_ops_dia_k(int64_t n, int nd, DiaOffsets off, DiaOffsets amap,
+                                                       const double* __restrict__ mdia,
+                                                       const double* __restrict__ adia,
+                                                       const uint8_t* __restrict__ dmask, double sk, double se,
+                                                       double* __restrict__ kdia, float* __restrict__ kdia32,
+                                                       double* __restrict__ edia, double* __restrict__ kdinv) {
+  const int64_t ld = dia_ld(n);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const bool ci = dmask && dmask[i];
+    double kd = 0.0;
+    for (int d = 0; d < nd; ++d) {
+      const int64_t j = i + off.o[d];
+      // the mass's entry (i, j) is kept when neither dof is constrained (past the last row the
+      // stored upper value is 0 anyway)
+      const bool keep = !ci && !(j < n && dmask && dmask[j]);
+      const double m = keep ? mdia[d * ld + i] : 0.0;
+      const int da = amap.o[d];
+      const double a = da >= 0 ? adia[da * ld + i] : 0.0;
+      const double k = __dadd_rn(m, __dmul_rn(sk, a));
+      kdia[d * ld + i] = k;
+      if (kdia32) kdia32[d * ld + i] = (float)k;
+      if (edia) edia[d * ld + i] = __dadd_rn(m, __dmul_rn(se, a));
+      if (d == 0) kd = k;
+    }
+    kdinv[i] = fabs(kd) > 1e-30 ? 1.0 / kd : 1.0;
+  }
+}
+
 // y = E x [+ dt c] [+ dt (theta f1 + (1-theta) f0)], L lanes per row
 template <int L>
 __global__ void __launch_bounds__(256) theta_rhs_k(int64_t n, const int32_t* __restrict__ rowptr,
@@ -120,6 +154,40 @@ extern "C" int jno_theta_operators(int64_t n_rows, const int32_t* d_rowptr, cons
   if (n_rows == 0) return JNO_OK;
   theta_ops_k<<<grid_for(n_rows, 256, 8192), 256, 0, as_stream(stream)>>>(
       n_rows, d_rowptr, d_col, d_mass, d_stiff, d_dmask, theta * dt, -(1.0 - theta) * dt, d_step, d_expl, d_step_dinv);
+  JNO_LAUNCH_CHECK();
+  return JNO_OK;
+}
+
+extern "C" int jno_theta_operators_dia(int64_t n_rows, int ndiag, const int32_t* h_offsets, const double* d_mass_dia,
+                                       int ndiag_stiff, const int32_t* h_offsets_stiff, const double* d_stiff_dia,
+                                       const uint8_t* d_dmask, double dt, double theta, double* d_step_dia,
+                                       float* d_step_dia32, double* d_expl_dia, double* d_step_dinv, void* stream) {
+  JNO_REQUIRE(h_offsets && d_mass_dia && h_offsets_stiff && d_stiff_dia && d_step_dia && d_step_dinv, JNO_EINVAL,
+              "jno_theta_operators_dia: null pointer");
+  JNO_REQUIRE(dt > 0.0 && theta > 0.0 && theta <= 1.0, JNO_EINVAL,
+              "jno_theta_operators_dia: need dt > 0 and 0 < theta <= 1 (got dt=%g, theta=%g)", dt, theta);
+  JNO_REQUIRE(ndiag >= 1 && ndiag <= JNO_DIA_MAX && ndiag_stiff >= 1 && ndiag_stiff <= ndiag, JNO_EINVAL,
+              "jno_theta_operators_dia: bad diagonal counts (%d, %d)", ndiag, ndiag_stiff);
+  JNO_REQUIRE(h_offsets[0] == 0 && h_offsets_stiff[0] == 0, JNO_EINVAL,
+              "jno_theta_operators_dia: the main diagonal is stored first");
+  DiaOffsets off, amap;
+  for (int d = 0; d < JNO_DIA_MAX; ++d) {
+    off.o[d] = d < ndiag ? h_offsets[d] : 0;
+    amap.o[d] = -1;
+  }
+  for (int e = 0; e < ndiag_stiff; ++e) {
+    int hit = -1;
+    for (int d = 0; d < ndiag; ++d)
+      if (h_offsets[d] == h_offsets_stiff[e]) hit = d;
+    JNO_REQUIRE(hit >= 0, JNO_EINVAL,
+                "jno_theta_operators_dia: the operator's offset %d is not among the mass's diagonals",
+                h_offsets_stiff[e]);
+    amap.o[hit] = e;
+  }
+  if (n_rows == 0) return JNO_OK;
+  theta_ops_dia_k<<<grid_for(n_rows, 256, 16384), 256, 0, as_stream(stream)>>>(
+      n_rows, ndiag, off, amap, d_mass_dia, d_stiff_dia, d_dmask, theta * dt, -(1.0 - theta) * dt, d_step_dia,
+      d_step_dia32, d_expl_dia, d_step_dinv);
   JNO_LAUNCH_CHECK();
   return JNO_OK;
 }

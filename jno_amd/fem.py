@@ -176,7 +176,7 @@ class CSR:
 
     @property
     def device(self):
-        return self.vals.device
+        return self.rowptr.device
 
     def _spmv(self, x: torch.Tensor) -> torch.Tensor:
         x = x.detach().to(torch.float64).contiguous()

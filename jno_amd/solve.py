@@ -99,7 +99,7 @@ class LinearSolver:
         if not isinstance(A, (CSR, ElementOperator)):
             raise TypeError("A must be an assembled jno_amd CSR operator (FemLinearSystem.evaluate) or an "
                             "ElementOperator")
-        _lib.require_gpu(A.vals if isinstance(A, CSR) else A.E, "A")
+        _lib.require_gpu(A.rowptr if isinstance(A, CSR) else A.E, "A")
         n = A.shape[0]
         b = torch.as_tensor(b).reshape(-1)
         _lib.require_gpu(b, "b")

@@ -38,6 +38,24 @@ from .precond import Diagonal
 from .solve import LinearSolver, cg
 
 
+class _StepCSR(CSR):
+    """A step or explicit operator formed on its DIA layout (the time-dependent step on a structured
+    box): its CSR values are formed from the mass and the operator, bitwise the same, only when
+    something asks for them (a CSR layout, a dense check); the solves use the DIA layout."""
+
+    def __init__(self, rowptr, col, shape, dinv, family, make_vals):
+        super().__init__(rowptr, col, None, shape, dinv, family=family)
+        del self.__dict__["vals"]
+        self.__dict__["_make_vals"] = make_vals
+
+    def __getattr__(self, name):
+        if name == "vals":
+            v = self.__dict__["_make_vals"]()
+            self.__dict__["vals"] = v
+            return v
+        raise AttributeError(name)
+
+
 def default_step_solver() -> LinearSolver:
     """The per-step solve: Jacobi-PCG to rtol 1e-10 on the true FP64 residual, FP32 inner
     iterations with FP64 residual replacement every 1e-4 of reduction (the step operator
@@ -82,6 +100,7 @@ class SemidiscreteTimeBlock:
         self.t0, self.t1, self.dt = float(t0), t1, dt
         self.theta = float(theta)
         self._ops: dict = {}
+        self._mass_dia = None  # (ndiag, offsets, FP64 diagonals) of M, formed on first use
 
     @classmethod
     def from_system(cls, system: FemLinearSystem, args: dict, *, coefficient_fn: Optional[Callable] = None,
@@ -136,6 +155,59 @@ class SemidiscreteTimeBlock:
         return K, E, c
 
     def _form_step(self, A: CSR, dt: float, theta: float):
+        if self.operator_fn is not None:
+            fast = self._form_step_dia(A, dt, theta)
+            if fast is not None:
+                return fast
+        return self._form_step_csr(A, dt, theta)
+
+    def _dia_of_mass(self):
+        if self._mass_dia is None:
+            try:
+                op = SolverOperator(self.M, fmt="dia", fp32=False)
+                self._mass_dia = (op.ndiag, list(op.offsets), op)
+            except ValueError:
+                self._mass_dia = (0, [], None)
+        return self._mass_dia
+
+    def _form_step_dia(self, A: CSR, dt: float, theta: float):
+        """K, E on the mass's diagonals from A's DIA layout (written by the box assembly, or found
+        once here), with no CSR pass; None when either has no DIA layout."""
+        opA = A._cache.get(("solver_op", "auto", True)) or A._cache.get(("solver_op", "auto", False))
+        if opA is None:
+            opA = SolverOperator(A, fmt="auto", fp32=False)  # chooses DIA once: later assemblies fuse it
+            A._cache[("solver_op", "auto", False)] = opA
+        nd, offs, mop = self._dia_of_mass()
+        if opA.format != "dia" or mop is None or not set(opA.offsets) <= set(offs):
+            return None
+        n = self.n
+        ld = (n + 63) // 64 * 64
+        dev = self.device
+        k64 = torch.empty(nd * ld, dtype=torch.float64, device=dev)
+        k32 = torch.empty(nd * ld, dtype=torch.float32, device=dev)
+        e64 = torch.empty(nd * ld, dtype=torch.float64, device=dev)
+        kd = torch.empty(n, dtype=torch.float64, device=dev)
+        I32 = C.c_int32 * _lib.JNO_DIA_MAX
+        _lib.call("jno_theta_operators_dia", n, nd, I32(*offs), mop.op.vals64, opA.ndiag, I32(*opA.offsets),
+                  opA.op.vals64, _lib.ptr(self.dmask), float(dt), float(theta), _lib.ptr(k64), _lib.ptr(k32),
+                  _lib.ptr(e64), _lib.ptr(kd), _lib.stream_ptr(dev))
+        csr_vals = {}
+
+        def vals(which, _A=A):
+            if not csr_vals:
+                Kc, Ec = self._form_step_csr(_A, dt, theta)
+                csr_vals["K"], csr_vals["E"] = Kc.vals, Ec.A.vals
+            return csr_vals[which]
+
+        K = _StepCSR(A.rowptr, A.col, A.shape, kd, "theta-step", lambda: vals("K"))
+        kop = SolverOperator.from_dia(K, nd, offs, k64, k32)
+        K._cache[("solver_op", "auto", True)] = kop
+        K._cache[("solver_op", "auto", False)] = kop
+        E = SolverOperator.from_dia(_StepCSR(A.rowptr, A.col, A.shape, None, "theta-explicit", lambda: vals("E")),
+                                    nd, offs, e64, None)
+        return K, E
+
+    def _form_step_csr(self, A: CSR, dt: float, theta: float):
         kv = torch.empty_like(A.vals)
         ev = torch.empty_like(A.vals)
         kd = torch.empty(self.n, dtype=torch.float64, device=self.device)

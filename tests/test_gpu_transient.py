@@ -258,6 +258,40 @@ def test_time_dependent_operator_matches_oracle(dev, theta):
         bad.step(us[0], 0.0)
 
 
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_time_dependent_step_operators_on_the_diagonal_layout(dev, theta):
+    # the per-step K, E formed on the DIA layouts of the mass and of A(t) (no CSR pass) equal the
+    # CSR path bitwise: values, Jacobi diagonal, the DIA layouts a fill of the CSR values gives, and
+    # the step's right-hand side
+    import jno_amd as J
+    from jno_amd.operator import SolverOperator
+    from jno_amd.transient import SemidiscreteTimeBlock
+
+    n, dt = 7, 0.013
+    m = J.kuhn_box(n)
+    sysg = J.fem_poisson(m, f=1.0, g=0.2)
+    kd = torch.as_tensor(O.k_lognormal(m.n_cells, 31), device="cuda")
+    blk = SemidiscreteTimeBlock.from_system(sysg, {"k": kd}, coefficient_fn=lambda tt, a: (1.0 + 5.0 * tt) * kd,
+                                            dt=dt, theta=theta)
+    for t_next in (dt, 0.4):
+        A, c = blk.operator_at(t_next)
+        K, E = blk._form_step(A, dt, theta)
+        kop = K._cache[("solver_op", "auto", True)]
+        assert kop.format == "dia" and kop.ndiag == 8 and E.format == "dia"
+        Kc, Ec = blk._form_step_csr(A, dt, theta)
+        assert torch.equal(K.dinv, Kc.dinv)
+        assert torch.equal(K.vals, Kc.vals) and torch.equal(E.A.vals, Ec.A.vals)
+        ref = SolverOperator(Kc, fmt="dia", fp32=True)
+        assert ref.offsets == kop.offsets
+        assert torch.equal(kop._keep[0], ref._keep[0]) and torch.equal(kop._keep[1], ref._keep[1])
+        eref = SolverOperator(Ec.A, fmt="dia", fp32=False)
+        assert eref.offsets == E.offsets and torch.equal(E._keep[0], eref._keep[0])
+        u = torch.as_tensor(np.random.default_rng(3).standard_normal(m.n_nodes), device="cuda")
+        r1 = blk.rhs(u, t_next - dt, dt, theta, ops=(K, E, c))
+        r2 = blk.rhs(u, t_next - dt, dt, theta, ops=(Kc, Ec, c))
+        assert torch.equal(r1, r2)
+
+
 def test_time_dependent_amplification_closed_form(dev):
     # a generalized eigenvector of the free block is multiplied by
     # prod_n (1 - (1-theta) dt phi(t_{n+1}) lam) / (1 + theta dt phi(t_{n+1}) lam) over the march
