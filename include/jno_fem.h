@@ -74,6 +74,37 @@ int jno_pattern_kuhn_box_nnz(int nx, int ny, int nz, int64_t* h_nnz, int32_t* h_
 int jno_pattern_kuhn_box(int nx, int ny, int nz, int32_t* d_rowptr, int32_t* d_col, int32_t* d_inc_ptr,
                          int32_t* d_inc, void* stream);
 
+/* Opaque plan handles keyed by a content hash of the mesh's cells: the pattern of
+ * jno_pattern_count/fill and the node incidence, built once per cell content and shared.  The
+ * reference memoizes compress_plan on a digest of its index array
+ * (jno/utils/solver/fem_utils.py:4027-4047); here the key is the device, the two content sums of
+ * jno_hash_i32 and the sizes, and a hit is confirmed by comparing the cells with the plan's own copy
+ * (a sum collision gets a plan of its own).  jno_plan_create returns the cached plan with its
+ * reference count raised, or builds one (it allocates the plan's device memory: a set-up call, not
+ * the hot path; it synchronizes the stream); jno_plan_get fills a view of its device arrays;
+ * jno_plan_destroy drops one reference and frees the plan with the last (after the device is idle);
+ * an unknown or already freed handle is JNO_EINVAL.  Thread-safe: a mutex guards the table. */
+typedef struct jno_plan jno_plan;
+typedef struct jno_plan_view {
+  int64_t n_rows;          /* n_nodes * vec                                   */
+  int64_t nnz;
+  int64_t n_cells;
+  int64_t n_nodes;
+  int32_t nloc, vec;
+  int32_t max_row;         /* longest row                                     */
+  int32_t refs;            /* current reference count                         */
+  const int32_t* rowptr;   /* i32[n_rows+1]                                   */
+  const int32_t* col;      /* i32[nnz]                                        */
+  const int32_t* inc_ptr;  /* i32[n_nodes+1]                                  */
+  const int32_t* inc;      /* i32[n_cells*nloc]                               */
+  uint64_t key[2];         /* the content sums                                */
+} jno_plan_view;
+int jno_plan_create(const int32_t* d_cells, int64_t n_cells, int nloc, int vec, int64_t n_nodes,
+                    jno_plan** h_plan, void* stream);
+int jno_plan_get(const jno_plan* plan, jno_plan_view* h_view);
+int jno_plan_destroy(jno_plan* plan);
+int jno_plan_live_count(int64_t* h_count);
+
 /* Content key of an index array (the plan cache keys patterns on the cells' content, as
  * compress_plan memoizes on a digest of its index array, jno/utils/solver/fem_utils.py:4027-4060):
  * h_hash[2] = two independent 64-bit sums of a position-dependent mix of every entry; and an

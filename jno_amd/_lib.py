@@ -90,6 +90,24 @@ class JnoBatchItem(C.Structure):
     _fields_ = [("op", JnoOp), ("dinv", C.c_void_p), ("b", C.c_void_p), ("x", C.c_void_p)]
 
 
+class JnoPlanView(C.Structure):
+    _fields_ = [
+        ("n_rows", C.c_int64),
+        ("nnz", C.c_int64),
+        ("n_cells", C.c_int64),
+        ("n_nodes", C.c_int64),
+        ("nloc", C.c_int32),
+        ("vec", C.c_int32),
+        ("max_row", C.c_int32),
+        ("refs", C.c_int32),
+        ("rowptr", C.c_void_p),
+        ("col", C.c_void_p),
+        ("inc_ptr", C.c_void_p),
+        ("inc", C.c_void_p),
+        ("key", C.c_uint64 * 2),
+    ]
+
+
 class JnoBatchTuning(C.Structure):
     _fields_ = [("one_kernel", C.c_int32), ("rows_per_entry", C.c_int32), ("grid", C.c_int32), ("ran_one_kernel", C.c_int32)]
 
@@ -173,6 +191,10 @@ SIGNATURES = {
         [C.POINTER(JnoOp), P, P, P, C.c_int, D, D, I64, D, C.POINTER(JnoPcgStats), P],
     ),
     "jno_adjoint_grad_k_p1": (C.c_int, [P, P, I64, P, P, P, P, P]),
+    "jno_plan_create": (C.c_int, [P, I64, C.c_int, C.c_int, I64, C.POINTER(C.c_void_p), P]),
+    "jno_plan_get": (C.c_int, [C.c_void_p, C.POINTER(JnoPlanView)]),
+    "jno_plan_destroy": (C.c_int, [C.c_void_p]),
+    "jno_plan_live_count": (C.c_int, [C.POINTER(I64)]),
     "jno_batch_workspace_size": (SZ, [C.c_int, C.POINTER(JnoBatchItem)]),
     "jno_batch_pcg_mixed": (
         C.c_int,
