@@ -485,7 +485,7 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
         "worst_relres": max(s["relres"] for s in bs.last_stats),
         "rhs_s": round(t_rhs, 6), "rhs_iterations_min_max": [min(its_r), max(its_r)],
         "one_kernel": bs.last_one_kernel,
-        "roofline": {"kernel": "batch_one_k (one kernel per step), whole solve",
+        "roofline": {"kernel": "batch_one_k<256> (one kernel per step, quads), whole solve",
                      "bytes": moved, "achieved": round(gbs, 1), "unit": "GB/s", "peak": HBM_PEAK_GBS,
                      "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "rhs_achieved": round(moved_r / t_rhs / 1e9, 1)},

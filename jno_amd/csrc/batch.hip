@@ -551,12 +551,11 @@ __global__ void batch_scale_k(const BItem* items, const double* const* dinvs, co
 // values, and the item's first entry stores the next state.  No value crosses workgroups inside a
 // launch (no atomics on the step path apart from the active-item count, no fences).
 // ---------------------------------------------------------------------------------------------
-constexpr int B1_NT = 512;
-constexpr int B1_U = 4;                // rows per thread and tile
-constexpr int B1_TT = B1_U * B1_NT;    // rows per tile
+constexpr int B1_NT = 256;
 constexpr int B1_HALO = 128;           // near halo rows staged in LDS (o2 of a Kuhn lattice up to 127 cells)
-constexpr int B1_RPE = 2 * B1_TT;      // default rows per table entry (whole tiles)
-constexpr int B1_GRID = 4096;          // default workgroups per launch (C4: 112.7 ms against 116.7 at 2048)
+constexpr int B1_RPE_UNIT = 2048;      // rows per table entry: a multiple of this (whole tiles of 4 B1_NT rows)
+constexpr int B1_RPE = 4096;           // default rows per table entry
+constexpr int B1_GRID = 4096;          // default workgroups per launch (C4: 106.7 ms against 111.8 at 2048)
 
 enum { P1_R2 = 0, P1_RES = 1, P1_ITER = 2, P1_DONE = 3 };
 
@@ -574,7 +573,7 @@ struct B1Args {
   B1State* st[2];     // by launch parity
   const int2* table;  // (item, local entry)
   int32_t n_entries;
-  int32_t rpe;        // rows per entry (a multiple of B1_TT)
+  int32_t rpe;        // rows per entry (a multiple of B1_RPE_UNIT)
   float* r[2];
   float* q[2];
   float* p[2];
@@ -641,9 +640,10 @@ __device__ __forceinline__ B1State state_uni(const B1State* S) {
 // item totals of the 4 partial arrays P[m * NE + e0 + j], j < ne, in a fixed order, identical in
 // every thread: array m is summed by the threads m NT/4 .. (m+1) NT/4 - 1, then their waves' sums in
 // wave order
+template <int NT>
 __device__ __forceinline__ void b1_item_totals(const double* __restrict__ P, int64_t NE, int e0, int ne,
                                                double* sm, double* out) {
-  constexpr int GS = B1_NT / 4, WPG = GS / 64;
+  constexpr int GS = NT / 4, WPG = GS / 64;
   const int m = threadIdx.x / GS, l = threadIdx.x % GS;
   const double* src = P + (int64_t)m * NE + e0;
   double s = 0.0;
@@ -662,9 +662,10 @@ __device__ __forceinline__ void b1_item_totals(const double* __restrict__ P, int
 }
 
 // the entry's 4 partials into P[m * NE + entry]
+template <int NT>
 __device__ __forceinline__ void b1_store_partials(const double* v, double* __restrict__ P, int64_t NE, int entry,
                                                   double* sm) {
-  constexpr int NW = B1_NT / 64;
+  constexpr int NW = NT / 64;
   const int w = threadIdx.x >> 6;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -691,113 +692,143 @@ __device__ __forceinline__ void str(float* __restrict__ base, uint32_t row, floa
   *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (row << 2)) = v;
 }
 
-// One iteration launch over rows [r0, r1) of an item: launch kk with the given alpha, beta
-// (alpha = 0: a start or resume, r and x left as they are).  part: r.r, p.q, q.r, q.q.
-__device__ __forceinline__ void b1_body(const B1Args& a, const ItemView& it, int64_t r0, int64_t r1, int64_t kk,
-                                        float al, float be, float* pt, double* part) {
+// ldr / str for 16-byte quads of rows: unaligned quads (the neighbours at odd offsets) are served
+// by gfx950's vector loads at 4-byte alignment; the base is the item's vector shifted 4 rows down,
+// so quads reaching up to 3 rows before the item's first row keep an unsigned offset
+typedef float F4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ F4 ldq(const float* __restrict__ base4, uint32_t row4) {
+  F4 r;
+  __builtin_memcpy(&r, reinterpret_cast<const char*>(base4) + (row4 << 2), 16);
+  return r;
+}
+__device__ __forceinline__ void stq(float* __restrict__ base4, uint32_t row4, F4 v) {
+  __builtin_memcpy(reinterpret_cast<char*>(base4) + (row4 << 2), &v, 16);
+}
+
+// The same launch with register-resident quads: a thread owns 4 consecutive rows (16-byte loads and
+// stores) and a tile is one quad per thread, 4 NT rows.  Items sit back to back in the vectors
+// (64-row aligned), so a quad at an item's first or last rows reads rows of the neighbouring item
+// or of the padding, which may hold anything, NaN included: every term of such a row is selected
+// after its product (a coefficient selected to 0 times a NaN would still be NaN).  Per row the sum
+// runs p, then the upper and lower terms of each off-diagonal in ascending offset order.  Rows of
+// the last quad past the item's end are stored as 0.  Every load
+// stays in the workspace: at most 3 rows before an item (the buffer before the first) or 3 past its
+// padded end (the buffer after the last).
+template <int NT>
+__device__ __forceinline__ void b1_body_q(const B1Args& a, const ItemView& it, int64_t r0, int64_t r1, int64_t kk,
+                                          float al, float be, float* pt, double* part) {
+  constexpr int TT = 4 * NT;
   const int n = (int)it.n;
   const int o1 = it.offs[1], o2 = it.offs[2], o3 = it.offs[3];
-  const int h = o2;
+  const int hq = (o2 + 3) & ~3;  // near halo rows on either side, whole quads (<= B1_HALO)
   const int64_t ld = dia_ld(it.n);
   const int cur = (int)(kk & 1), nxt = cur ^ 1;
-  const float* __restrict__ R = a.r[cur] + it.off;
-  const float* __restrict__ Q = a.q[cur] + it.off;
-  const float* __restrict__ Pp = a.p[cur] + it.off;
-  float* __restrict__ Rn = a.r[nxt] + it.off;
-  float* __restrict__ Qn = a.q[nxt] + it.off;
-  float* __restrict__ Pn = a.p[nxt] + it.off;
-  float* __restrict__ XS = a.xs + it.off;
-  const float* __restrict__ v1 = it.s32;
-  const float* __restrict__ v2 = it.s32 + ld;
-  const float* __restrict__ v3 = it.s32 + 2 * ld;
+  // bases 4 rows down: row j is at offset j + 4 >= 1 for every j >= -3
+  const float* __restrict__ R = a.r[cur] + it.off - 4;
+  const float* __restrict__ Q = a.q[cur] + it.off - 4;
+  const float* __restrict__ Pp = a.p[cur] + it.off - 4;
+  float* __restrict__ Rn = a.r[nxt] + it.off - 4;
+  float* __restrict__ Qn = a.q[nxt] + it.off - 4;
+  float* __restrict__ Pn = a.p[nxt] + it.off - 4;
+  float* __restrict__ XS = a.xs + it.off - 4;
+  const float* __restrict__ v1 = it.s32 - 4;
+  const float* __restrict__ v2 = it.s32 + ld - 4;
+  const float* __restrict__ v3 = it.s32 + 2 * ld - 4;
   const bool upd = al != 0.0f;
   const int tid = threadIdx.x;
-  for (int t0 = (int)r0; t0 < (int)r1; t0 += B1_TT) {
-    const int t1 = min((int)r1, t0 + B1_TT);
-    // own rows: r, q, p, xs; one near-halo row per thread below 2h; every load unconditional from
-    // a row clamped to 0 where it does not apply (finite values that meet zero coefficients only)
-    float rv[B1_U], qv[B1_U], pv[B1_U], xv[B1_U];
-#pragma unroll
-    for (int u = 0; u < B1_U; ++u) {
-      const int i = t0 + u * B1_NT + tid;
-      const uint32_t j = i < t1 ? (uint32_t)i : 0u;
-      rv[u] = ldr(R, j);
-      qv[u] = ldr(Q, j);
-      pv[u] = ldr(Pp, j);
-      xv[u] = ldr(XS, j);
-    }
-    const bool hal = tid < 2 * h;
-    const int jh = tid < h ? t0 - h + tid : t1 + (tid - h);
+  for (int t0 = (int)r0; t0 < (int)r1; t0 += TT) {
+    const int t1 = min((int)r1, t0 + TT);
+    const int i = t0 + 4 * tid;
+    const bool own = i < t1;
+    const uint32_t ic4 = (uint32_t)((own ? i : 0) + 4);
+    const F4 rv = ldq(R, ic4), qv = ldq(Q, ic4), pv = ldq(Pp, ic4), xv = ldq(XS, ic4);
+    // one near-halo row per thread below 2 hq (rows outside the item staged as 0)
+    const bool hal = tid < 2 * hq;
+    const int jh = tid < hq ? t0 - hq + tid : t1 + (tid - hq);
     const bool hok = hal && jh >= 0 && jh < n;
-    const uint32_t jhc = hok ? (uint32_t)jh : 0u;
-    const float rh = ldr(R, jhc), qh = ldr(Q, jhc), ph = ldr(Pp, jhc);
-    float rn[B1_U], pn[B1_U];
+    const uint32_t jh4 = (uint32_t)((hok ? jh : 0) + 4);
+    const float rh = ldr(R, jh4), qh = ldr(Q, jh4), ph = ldr(Pp, jh4);
+    bool vr[4];
+    float rn[4], pn[4];
 #pragma unroll
-    for (int u = 0; u < B1_U; ++u) {
-      const int i = t0 + u * B1_NT + tid;
-      rn[u] = upd ? rv[u] - al * qv[u] : rv[u];
-      pn[u] = rn[u] + be * pv[u];
-      if (i < t1) {
-        pt[i - t0 + h] = pn[u];
-        str(Rn, (uint32_t)i, rn[u]);
-        str(Pn, (uint32_t)i, pn[u]);
-        if (upd) str(XS, (uint32_t)i, xv[u] + al * pv[u]);
-        part[0] += (double)rn[u] * (double)rn[u];
+    for (int u = 0; u < 4; ++u) {
+      vr[u] = own && i + u < t1;
+      const float r_ = upd ? rv[u] - al * qv[u] : rv[u];
+      const float p_ = r_ + be * pv[u];
+      rn[u] = vr[u] ? r_ : 0.0f;
+      pn[u] = vr[u] ? p_ : 0.0f;
+    }
+    if (own) {
+      const int li = i - t0 + hq;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pt[li + u] = pn[u];
+      const uint32_t i4 = (uint32_t)(i + 4);
+      stq(Rn, i4, F4{rn[0], rn[1], rn[2], rn[3]});
+      stq(Pn, i4, F4{pn[0], pn[1], pn[2], pn[3]});
+      if (upd) {
+        F4 xn;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xn[u] = vr[u] ? xv[u] + al * pv[u] : 0.0f;
+        stq(XS, i4, xn);
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) part[0] += (double)rn[u] * (double)rn[u];
     }
     if (hal) {
       const float rr = upd ? rh - al * qh : rh;
-      pt[jh - t0 + h] = hok ? rr + be * ph : 0.0f;
+      pt[jh - t0 + hq] = hok ? rr + be * ph : 0.0f;
     }
-    float fu[B1_U], fl[B1_U];
-#pragma unroll
-    for (int u = 0; u < B1_U; ++u) {
-      const int i = t0 + u * B1_NT + tid;
-      const bool ok = i < t1;
-      const uint32_t iu = ok && i + o3 < n ? (uint32_t)(i + o3) : 0u;
-      const uint32_t il = ok && i - o3 >= 0 ? (uint32_t)(i - o3) : 0u;
-      const float ru = ldr(R, iu), qu = ldr(Q, iu), pu = ldr(Pp, iu);
-      const float rl = ldr(R, il), ql = ldr(Q, il), pl = ldr(Pp, il);
-      fu[u] = (upd ? ru - al * qu : ru) + be * pu;
-      fl[u] = (upd ? rl - al * ql : rl) + be * pl;
-    }
+    // far neighbours' p_k and the coefficients (upper at the own rows, lower at i - o), in one batch
+    // of loads; indices that do not reach a valid row are clamped to row 0 (their terms are selected)
+    const int ib = own ? i : 0;
+    const uint32_t ju4 = (uint32_t)((ib + o3 < n ? ib + o3 : 0) + 4);
+    const uint32_t jd4 = (uint32_t)((ib - o3 >= -3 ? ib - o3 : 0) + 4);
+    const F4 fur = ldq(R, ju4), fuq = ldq(Q, ju4), fup = ldq(Pp, ju4);
+    const F4 fdr = ldq(R, jd4), fdq = ldq(Q, jd4), fdp = ldq(Pp, jd4);
+    const F4 cu1 = ldq(v1, ic4), cu2 = ldq(v2, ic4), cu3 = ldq(v3, ic4);
+    const F4 cl1 = ldq(v1, (uint32_t)((ib - o1 >= -3 ? ib - o1 : 0) + 4));
+    const F4 cl2 = ldq(v2, (uint32_t)((ib - o2 >= -3 ? ib - o2 : 0) + 4));
+    const F4 cl3 = ldq(v3, jd4);
     __syncthreads();
+    if (own) {
+      const int li = i - t0 + hq;
+      float qo[4];
 #pragma unroll
-    for (int u = 0; u < B1_U; ++u) {
-      const int i = t0 + u * B1_NT + tid;
-      const bool ok = i < t1;
-      const uint32_t ic = ok ? (uint32_t)i : 0u;
-      const bool l1 = ok && i - o1 >= 0, l2 = ok && i - o2 >= 0, l3 = ok && i - o3 >= 0;
-      const float cu1 = ldr(v1, ic), cu2 = ldr(v2, ic), cu3 = ldr(v3, ic);
-      const float c1 = ldr(v1, l1 ? (uint32_t)(i - o1) : 0u);
-      const float c2 = ldr(v2, l2 ? (uint32_t)(i - o2) : 0u);
-      const float c3 = ldr(v3, l3 ? (uint32_t)(i - o3) : 0u);
-      if (ok) {
-        const int li = i - t0 + h;
-        float s = pn[u];
-        s += cu1 * pt[li + o1];
-        s += (l1 ? c1 : 0.0f) * pt[li - o1];
-        s += cu2 * pt[li + o2];
-        s += (l2 ? c2 : 0.0f) * pt[li - o2];
-        s += cu3 * fu[u];
-        s += (l3 ? c3 : 0.0f) * fl[u];
-        str(Qn, (uint32_t)i, s);
-        part[1] += (double)pn[u] * (double)s;
-        part[2] += (double)s * (double)rn[u];
-        part[3] += (double)s * (double)s;
+      for (int u = 0; u < 4; ++u) {
+        const int iu = i + u;
+        const float fu = (upd ? fur[u] - al * fuq[u] : fur[u]) + be * fup[u];
+        const float fl = (upd ? fdr[u] - al * fdq[u] : fdr[u]) + be * fdp[u];
+        const float t_u1 = cu1[u] * pt[li + u + o1], t_l1 = cl1[u] * pt[li + u - o1];
+        const float t_u2 = cu2[u] * pt[li + u + o2], t_l2 = cl2[u] * pt[li + u - o2];
+        const float t_u3 = cu3[u] * fu, t_l3 = cl3[u] * fl;
+        float s_ = pn[u];
+        s_ += iu + o1 < n ? t_u1 : 0.0f;
+        s_ += iu - o1 >= 0 ? t_l1 : 0.0f;
+        s_ += iu + o2 < n ? t_u2 : 0.0f;
+        s_ += iu - o2 >= 0 ? t_l2 : 0.0f;
+        s_ += iu + o3 < n ? t_u3 : 0.0f;
+        s_ += iu - o3 >= 0 ? t_l3 : 0.0f;
+        qo[u] = vr[u] ? s_ : 0.0f;
+        part[1] += (double)pn[u] * (double)qo[u];
+        part[2] += (double)qo[u] * (double)rn[u];
+        part[3] += (double)qo[u] * (double)qo[u];
       }
+      stq(Qn, (uint32_t)(i + 4), F4{qo[0], qo[1], qo[2], qo[3]});
     }
     __syncthreads();  // the next tile rewrites pt
   }
 }
 
-// At least 4 waves per SIMD (at most 128 VGPRs): two workgroups per CU.  Measured at C4 (256
-// systems, 29.1M rows): 333 us per launch with every system active (3.85 TB/s of algorithmic bytes);
-// loading the coefficients with the far rows before the barrier, or 2 rows per thread and tile,
-// measured 2-6% slower.
-__global__ void __launch_bounds__(B1_NT, 4) batch_one_k(B1Args a, int par) {
-  __shared__ float pt[B1_TT + 2 * B1_HALO];
-  __shared__ double sm[4 * (B1_NT / 64)];
+// At least 3 waves per SIMD (at most 168 VGPRs; the kernel takes 155): three 256-thread
+// workgroups per CU.  C4 (256 systems, 29.1M rows): 106.7 ms per solve, where rows one per thread
+// (512 threads, 4 rows per thread, LDS-staged, 127 VGPRs) took 112.9 ms; issuing the first tile's
+// r, q, p before the reduction of the partial sums pushed the kernel to 168 VGPRs with spills and
+// measured 112.0 ms.
+template <int NT>
+__global__ void __launch_bounds__(NT, 3) batch_one_k(B1Args a, int par) {
+  constexpr int TT = 4 * NT;
+  __shared__ __align__(16) float pt[TT + 2 * B1_HALO];
+  __shared__ double sm[4 * (NT / 64)];
   const int64_t NE = a.n_entries;
   // XCD-aware: the entries split into 8 contiguous slabs, one per XCD (workgroups are dealt to the
   // XCDs round-robin), so an item's neighbouring entries run on one XCD at about the same time and
@@ -834,7 +865,7 @@ __global__ void __launch_bounds__(B1_NT, 4) batch_one_k(B1Args a, int par) {
       // true residual in FP64 on the unscaled operator, r = S r64 into the resumed launch's buffer
       float* __restrict__ Rw = a.r[S.kr & 1] + it.off;
       const int64_t ld = dia_ld(it.n);
-      for (int64_t i = r0 + threadIdx.x; i < r1; i += B1_NT) {
+      for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
         const double bi = it.b[i];
         double ax = it.v64[i] * it.x[i];
 #pragma unroll
@@ -852,12 +883,14 @@ __global__ void __launch_bounds__(B1_NT, 4) batch_one_k(B1Args a, int par) {
         part[2] += bi * bi;
       }
       S.phase = P1_RES;
-      b1_store_partials(part, Pout, NE, (int)entry, sm);
+      b1_store_partials<NT>(part, Pout, NE, (int)entry, sm);
       if (lead) Sout[te.x] = S;
       continue;
     }
+    // the quad form issues its first tile's loads (which need only the buffer parity) before the
+    // reduction of the partial sums, so their round trip overlaps it
     double in[4];
-    b1_item_totals(Pin, NE, it.e0, it.ne, sm, in);
+    b1_item_totals<NT>(Pin, NE, it.e0, it.ne, sm, in);
     int64_t kk;
     double alpha, beta;
     if (S.phase == P1_RES) {
@@ -896,7 +929,7 @@ __global__ void __launch_bounds__(B1_NT, 4) batch_one_k(B1Args a, int par) {
       if (stop_repl || bad || kk > a.maxiter) {
         // x64 += S xs, xs = 0 on the entry's rows; the true residual follows in the next launch
         float* __restrict__ XS = a.xs + it.off;
-        for (int64_t i = r0 + threadIdx.x; i < r1; i += B1_NT) {
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
           it.x[i] += sqrt(it.dinv64[i]) * (double)XS[i];
           XS[i] = 0.0f;
         }
@@ -912,8 +945,8 @@ __global__ void __launch_bounds__(B1_NT, 4) batch_one_k(B1Args a, int par) {
         S.g_hist[0] = rr;
       S.k = kk + 1;
     }
-    b1_body(a, it, r0, r1, kk, (float)alpha, (float)beta, pt, part);
-    b1_store_partials(part, Pout, NE, (int)entry, sm);
+    b1_body_q<NT>(a, it, r0, r1, kk, (float)alpha, (float)beta, pt, part);
+    b1_store_partials<NT>(part, Pout, NE, (int)entry, sm);
     if (lead) Sout[te.x] = S;
   }
 }
@@ -1062,9 +1095,9 @@ struct BatchGraph {
 static int check_batch_tuning(const jno_batch_tuning* t) {
   if (!t) return JNO_OK;
   JNO_REQUIRE(t->one_kernel == 0 || t->one_kernel == 1, JNO_EINVAL, "jno_batch_tuning: one_kernel is 0 or 1");
-  JNO_REQUIRE(t->rows_per_entry == 0 || (t->rows_per_entry > 0 && t->rows_per_entry % B1_TT == 0 &&
+  JNO_REQUIRE(t->rows_per_entry == 0 || (t->rows_per_entry > 0 && t->rows_per_entry % B1_RPE_UNIT == 0 &&
                                          t->rows_per_entry <= 65536),
-              JNO_EINVAL, "jno_batch_tuning: rows_per_entry is 0 or a multiple of %d up to 65536", B1_TT);
+              JNO_EINVAL, "jno_batch_tuning: rows_per_entry is 0 or a multiple of %d up to 65536", B1_RPE_UNIT);
   JNO_REQUIRE(t->grid == 0 || (t->grid > 0 && t->grid % N_XCD == 0 && t->grid <= 65536), JNO_EINVAL,
               "jno_batch_tuning: grid is 0 or a multiple of %d up to 65536", N_XCD);
   return JNO_OK;
@@ -1238,7 +1271,7 @@ static int batch_impl(int n_items, const jno_batch_item* h_items, double tol, do
     int chunk = steps_per_graph > 0 ? steps_per_graph : 8;
     chunk += chunk & 1;
     auto step = [&](hipStream_t st, int i) -> int {
-      batch_one_k<<<g1, B1_NT, 0, st>>>(a, i & 1);
+      batch_one_k<B1_NT><<<g1, B1_NT, 0, st>>>(a, i & 1);
       JNO_LAUNCH_CHECK();
       return JNO_OK;
     };
