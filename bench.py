@@ -260,7 +260,7 @@ def committed_traffic(kernel, workload):
     return None
 
 
-def grad_rate(J, torch, dev, n=64, reps=3):
+def grad_rate(J, torch, dev, n=64, reps=5):
     from jno_amd import adjoint, solve
 
     mesh = J.kuhn_box(n, device=dev)
@@ -286,16 +286,22 @@ def grad_rate(J, torch, dev, n=64, reps=3):
             return k.grad
 
         once()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        # each repetition timed on its own: the rate is the median's (a host stall of tens of ms,
+        # seen now and then on these boxes, would dominate a mean of a few 8 ms gradients)
+        times = []
         for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
             g = once()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
         res[f"{prec}_grads_per_s"] = round(1.0 / dt, 3)
         res[f"{prec}_s_per_grad"] = round(dt, 6)
+        res[f"{prec}_s_per_grad_mean"] = round(float(np.mean(times)), 6)
         res[f"{prec}_grad_norm"] = float(torch.linalg.norm(g))
-    res["workload"] = f"P1 Poisson {n}^3 (T={T}), L = 1/2 (u-u*)^T M (u-u*), dL/dk for all elements"
+    res["workload"] = (f"P1 Poisson {n}^3 (T={T}), L = 1/2 (u-u*)^T M (u-u*), dL/dk for all elements; "
+                       f"median of {reps} timed gradients after one warm-up")
     res["speedup_mixed_vs_fp64"] = round(res["fp64_s_per_grad"] / res["mixed_s_per_grad"], 3)
     return res
 
