@@ -454,13 +454,24 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
         m = J.kuhn_box(int(nn), device=dev)
         k = np.exp(0.5 * np.random.default_rng(1000 + i).standard_normal(m.n_cells))
         systems.append(J.fem_poisson(m).evaluate({"k": torch.tensor(k, device=dev)}))
-    bs = BatchSolver(tol=1e-10, steps_per_graph=8)
-    bs(systems)  # warm: layouts, graph capture, code objects
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    # the launch shape (rows per entry, workgroups, steps per graph replay) picked by the autotuner on
+    # the first call, which also warms the layouts, the graph capture and the code objects
+    bs = BatchSolver(tol=1e-10, autotune=True)
     bs(systems)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs = bs(systems)
+    torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    # the error against FP64 single solves of three of the systems (the largest, the smallest, the first)
+    from jno_amd import precond, solve
+
+    fp64 = solve.cg(tol=1e-10, precision="fp64")
+    errs = []
+    for i in sorted({0, int(np.argmax(ns)), int(np.argmin(ns))}):
+        A, b = systems[i]
+        ref = fp64(A, b, M=precond.jacobi())
+        errs.append(float(torch.linalg.norm(xs[i] - ref) / torch.linalg.norm(ref)))
     its = [s["iterations"] for s in bs.last_stats]
     refs = [s["refinements"] for s in bs.last_stats]
     rows = [A.shape[0] for A, _ in systems]
@@ -474,7 +485,7 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
     A96, _ = J.fem_poisson(m96).evaluate({"k": torch.tensor(k96, device=dev)})
     rng = np.random.default_rng(8)
     rhs = [(A96, torch.tensor(rng.standard_normal(A96.shape[0]), device=dev)) for _ in range(n_rhs)]
-    br = BatchSolver(tol=1e-10, steps_per_graph=8)
+    br = BatchSolver(tol=1e-10, autotune=True)
     br(rhs)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -488,6 +499,8 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
         "workload": f"{count} P1 Poisson systems, n^3 Kuhn boxes with n in [24, 64] (seed {seed}), {sum(rows)} rows; "
                     f"plus {n_rhs} N(0,1) right-hand sides on {n_rhs_mesh}^3; mixed to rtol 1e-10",
         "batch_s": round(t_batch, 6), "iterations_min_max": [min(its), max(its)],
+        "launch_shape": bs.last_launch_shape, "rhs_launch_shape": br.last_launch_shape,
+        "rel_l2_vs_fp64_single_solves": errs,
         "worst_relres": max(s["relres"] for s in bs.last_stats),
         "rhs_s": round(t_rhs, 6), "rhs_iterations_min_max": [min(its_r), max(its_r)],
         "one_kernel": bs.last_one_kernel,

@@ -113,7 +113,10 @@ def tune_launch(A, b, precision: str = "mixed", iters: int = 100, shapes=CANDIDA
     return _LAUNCH_CACHE[key]
 
 
-def tune_batch(systems, chunks=(4, 8, 16), shapes=((4096, 2048),), **solver_kw) -> dict:
+BATCH_SHAPES = ((4096, 4096), (4096, 2048), (2048, 4096), (8192, 4096))
+
+
+def tune_batch(systems, chunks=(8, 16), shapes=BATCH_SHAPES, **solver_kw) -> dict:
     """Launch shape for :class:`jno_amd.batch.BatchSolver` on ``systems`` (a list of ``(A, b)``):
     steps per hipGraph replay from ``chunks`` and the one-kernel step's (rows_per_entry, grid) from
     ``shapes``.  Each candidate is timed on one whole batched solve (after a warm solve that captures

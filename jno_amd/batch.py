@@ -19,6 +19,10 @@ from .fem import CSR
 from .operator import SolverOperator
 
 
+# launch shapes found by BatchSolver(autotune=True), per batch signature
+_TUNED: dict = {}
+
+
 class BatchSolver:
     """Batched mixed-precision Jacobi-PCG.
 
@@ -31,12 +35,16 @@ class BatchSolver:
 
     def __init__(self, tol: float = 1e-10, atol: float = 0.0, maxiter: int = 20_000, replace_factor: float = 0.1,
                  steps_per_graph: int = 8, format: str = "auto", one_kernel: bool = True, rows_per_entry: int = 0,
-                 grid: int = 0):
+                 grid: int = 0, autotune: bool = False):
         self.tol, self.atol, self.maxiter = float(tol), float(atol), int(maxiter)
         self.replace_factor = float(replace_factor)
         self.steps_per_graph = int(steps_per_graph)
         self.format = format
         self.tuning = _lib.JnoBatchTuning(1 if one_kernel else 0, int(rows_per_entry), int(grid), 0)
+        # autotune: the first call on a batch signature (each system's size and layout) times the
+        # launch shapes of autotune.tune_batch and keeps the fastest for later calls
+        self.autotune = bool(autotune)
+        self.last_launch_shape: dict = {}
         self.last_stats: List[dict] = []
         self.last_status = 0
         self.last_one_kernel = False  # whether the last call ran the one-kernel step
@@ -63,10 +71,33 @@ class BatchSolver:
             A._cache[key] = op
         return op
 
+    def _signature(self, systems):
+        sig = []
+        for A, _ in systems:
+            op = self._op(A)
+            sig.append((A.shape[0], op.format, tuple(getattr(op, "offsets", ()) or ())))
+        return (self.tol, self.atol, self.replace_factor, tuple(sig))
+
+    def _tuned(self, systems):
+        key = self._signature(systems)
+        kw = _TUNED.get(key)
+        if kw is None:
+            from .autotune import tune_batch
+
+            kw = tune_batch(list(systems), tol=self.tol, atol=self.atol, maxiter=self.maxiter,
+                            replace_factor=self.replace_factor, format=self.format)
+            _TUNED[key] = kw
+        self.steps_per_graph = kw["steps_per_graph"]
+        self.tuning.rows_per_entry = kw["rows_per_entry"]
+        self.tuning.grid = kw["grid"]
+        self.last_launch_shape = dict(kw)
+
     def __call__(self, systems: Sequence[Tuple[CSR, torch.Tensor]]) -> List[torch.Tensor]:
         """Solve A_i x_i = b_i for every (A_i, b_i); an operator object may repeat (shared)."""
         if not systems:
             return []
+        if self.autotune:
+            self._tuned(systems)
         items = (_lib.JnoBatchItem * len(systems))()
         xs, keep = [], []
         dev = systems[0][0].device

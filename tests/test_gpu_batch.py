@@ -232,3 +232,26 @@ def test_batch_and_single_solver_on_one_p2_operator(dev):
             assert sv.last_stats["operator"]["format"] == choose_format(A)[0], sv.last_stats
         assert bs._op(A).format in ("dia", "csr")
         assert _rel(_np(u), uo) <= 1e-9 and _rel(_np(x), uo) <= 1e-9, order
+
+
+def test_batch_autotune_picks_and_keeps_a_launch_shape(dev):
+    # the first call on a batch signature times the autotuner's candidates and keeps the fastest;
+    # a second solver on the same signature reuses it without timing again
+    from jno_amd import autotune, batch
+    from jno_amd.batch import BatchSolver
+
+    sy = _systems([5, 9, 7], seed0=400)
+    bs = BatchSolver(tol=1e-10, autotune=True)
+    xs = bs([(A, b) for _, _, A, b in sy])
+    kw = bs.last_launch_shape
+    assert (kw["rows_per_entry"], kw["grid"]) in autotune.BATCH_SHAPES and kw["steps_per_graph"] in (8, 16)
+    assert bs.tuning.rows_per_entry == kw["rows_per_entry"] and bs.steps_per_graph == kw["steps_per_graph"]
+    n_tuned = len(batch._TUNED)
+    bs2 = BatchSolver(tol=1e-10, autotune=True)
+    xs2 = bs2([(A, b) for _, _, A, b in sy])
+    assert len(batch._TUNED) == n_tuned and bs2.last_launch_shape == kw
+    for (n, k, A, b), x, x2 in zip(sy, xs, xs2):
+        p, t = O.kuhn_box(n)
+        Ao, bo, _ = O.poisson_p1_system(p, t, k)
+        xd = O.direct_solve(Ao, bo)
+        assert _rel(_np(x), xd) <= 1e-9 and _rel(_np(x2), xd) <= 1e-9
