@@ -158,7 +158,12 @@ def main():
         quad = vec in (1, 2, 3) and blk <= 512 and op.ndiag == 4 and 2 <= sum(o <= 256 for o in op.offsets[1:]) <= 3
         kname = ("pcg_one_p<dia, f32, scaled>" if vec == 3 else "pcg_one_q<dia, f32, scaled>") if quad \
             else "pcg_one<dia, f32, scaled>"
-        kernels = [(kname, ms[1], N * 4 * (op.ndiag - 1) + N * (4 * 4 + 4 * 4))]
+        if kname.startswith("pcg_one_p"):
+            # x every other launch: a deferring launch moves r, q, p (24 B) and the diagonals, a
+            # flushing one also reads x and p_{k-2} and writes x (12 B): 6 B per row on average
+            kernels = [(kname, ms[1], N * 4 * (op.ndiag - 1) + N * (3 * 4 + 3 * 4) + N * 6)]
+        else:
+            kernels = [(kname, ms[1], N * 4 * (op.ndiag - 1) + N * (4 * 4 + 4 * 4))]
     elif mode == 4 and op.format == "dia":  # K1 K2 on the Jacobi-scaled operator: no dinv, unit diagonal
         k1 = "pcg_k1_lds<dia, f32, scaled>" if vec == 3 else "pcg_k1<dia, f32, scaled>"  # vec 3: p staged in LDS
         kernels = [(k1, ms[1], N * 4 * (op.ndiag - 1) + N * (3 * 4 + 3 * 4)),

@@ -1153,8 +1153,21 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
 // the next phase 1.  A workgroup thus always has loads outstanding, instead of alternating a load
 // round trip with a compute phase on every tile.  The far-diagonal terms move to phase 2 (their
 // p_k, two quads, cross the barrier in registers), so phase 1 holds no diagonal values.
+// x is updated every other launch (XM, chosen by the host from the launch parity): a deferring
+// launch (XM_DEFER, even k) neither reads nor writes x and records its alpha_{k-1} in
+// a_pend[(k+1) & 1]; the flushing launch after it (XM_FLUSH) adds both terms,
+// x += a_pend p_{k-2} + alpha_{k-1} p_{k-1}, reading p_{k-2} from the buffer it is about to
+// overwrite with p_k (own rows only, read before the write by the same thread).  Per row and pair
+// of launches: x read and written once and p_{k-2} read, 12 B instead of 16.  The two products are
+// added in the order, and with the same FP32 roundings, as two eager updates, so x is bitwise that
+// of updating every launch.  A stop leaves nothing pending: a flushing launch that stops adds the
+// deferred term itself (a pass over its rows, x += a_pend p_{k-2}), so x is x_{k-1} at every stop,
+// as the residual replacement and the end of the solve need.  Special launches (start, resume)
+// have alpha = 0: they never find a term pending (the stop before them flushed it) and defer none.
 // ---------------------------------------------------------------------------------------------
-template <int NT, int ND, int NN>
+constexpr int XM_EAGER = 0, XM_DEFER = 1, XM_FLUSH = 2;
+
+template <int NT, int ND, int NN, int XM = XM_EAGER>
 __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
   static_assert(ND == 4 && NN >= 2 && NN <= 3, "instantiated for 4-diagonal operators with 2 or 3 near diagonals");
   constexpr int NF = ND - 1 - NN;  // far off-diagonals
@@ -1215,6 +1228,8 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
 #pragma unroll
   for (int u = 0; u < QRED_H; ++u) pv_in[u] = *reinterpret_cast<const double2*>(pin + min(2 * (u * NT + tid), G4 - 2));
   const DecideIn din = decide_in(st, k);
+  // the alpha whose x term the previous (deferring) launch left to this one
+  const float apd = XM == XM_FLUSH ? (float)st->a_pend[k & 1] : 0.0f;
   // every wave's head loads are queued before any wave of the workgroup issues its tile's: the head
   // waits for them, the tile's loads have time until the head is done
   __syncthreads();
@@ -1224,8 +1239,8 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
     int i, t0, t1;
     bool own, hin[UH];
   };
-  struct TF {  // group F: the far neighbours and x
-    Q fur[NFA], fuq[NFA], fup[NFA], fdr[NFA], fdq[NFA], fdp[NFA], xv;
+  struct TF {  // group F: the far neighbours, x and (flushing launches) p_{k-2}
+    Q fur[NFA], fuq[NFA], fup[NFA], fdr[NFA], fdq[NFA], fdp[NFA], xv, p2;
   };
   struct TB {  // group B: the off-diagonal values
     Q vu[ND], vl[ND];
@@ -1265,7 +1280,9 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
       c.fdq[f] = ld16(a.q + jd);
       c.fdp[f] = ld16(a.p_old + jd);
     }
-    c.xv = ld16(a.x + ic);  // used only by an updating launch (every launch but the special ones)
+    // used only by an updating launch (every launch but the special ones)
+    if constexpr (XM != XM_DEFER) c.xv = ld16(a.x + ic);
+    if constexpr (XM == XM_FLUSH) c.p2 = ld16(a.p_new + ic);  // p_{k-2}, before this launch's p_k
   };
   auto load_b = [&](TB& c, const TA& ca) {
     const int ic = ca.own ? ca.i : 0;
@@ -1318,9 +1335,32 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
   }
   double alpha, beta;
   int special;
-  if (!one_decide_pre(st, din, k, in, alpha, beta, special)) return;
+  if (!one_decide_pre(st, din, k, in, alpha, beta, special)) {
+    if constexpr (XM != XM_EAGER) {
+      if (blockIdx.x == 0 && tid == 0) st->a_pend[(k + 1) & 1] = 0.0;
+      // a stopping flushing launch adds the term the launch before it deferred: x = x_{k-1}
+      if (XM == XM_FLUSH && special == 0 && apd != 0.0f) {
+        for (int t = 0; t < nt; ++t) {
+          int t0, t1;
+          tile_at(t, t0, t1);
+          const int i = t0 + 4 * tid;
+          if (i < t1) {
+            const Q xv = ld16(a.x + i), p2 = ld16(a.p_new + i);
+            Q xn;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) xn.v[e] = xv.v[e] + apd * p2.v[e];
+            st16(a.x + i, xn);
+          }
+        }
+      }
+    }
+    return;
+  }
   const float al = (float)alpha, be = (float)beta;
   const bool upd = special == 0;
+  if constexpr (XM != XM_EAGER) {
+    if (blockIdx.x == 0 && tid == 0) st->a_pend[(k + 1) & 1] = XM == XM_DEFER && upd ? (double)al : 0.0;
+  }
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   // one tile: phase 1, the next tile's A and F, the barrier, phase 2, the next tile's B; returns
   // whether a next tile follows.  The loads are issued unconditionally (the last tile reloads its own
@@ -1361,10 +1401,14 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
         fu[f].v[e] = (cf.fur[f].v[e] - al * cf.fuq[f].v[e]) + be * cf.fup[f].v[e];
         fd[f].v[e] = (cf.fdr[f].v[e] - al * cf.fdq[f].v[e]) + be * cf.fdp[f].v[e];
       }
-    if (own && upd) {
+    if (XM != XM_DEFER && own && upd) {
       Q xn;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) xn.v[e] = cf.xv.v[e] + al * ca.pv.v[e];
+      for (int e = 0; e < 4; ++e) {
+        // flushing: the deferred term first, then this launch's, as two eager updates would add them
+        const float xb = XM == XM_FLUSH && apd != 0.0f ? cf.xv.v[e] + apd * cf.p2.v[e] : cf.xv.v[e];
+        xn.v[e] = xb + al * ca.pv.v[e];
+      }
       st16(a.x + i, xn);
     }
     // the next tile's groups A and F go out now, in flight during the barrier and phase 2
@@ -2488,15 +2532,18 @@ static int launch_quad(const jno_op* op, const PcgArgs<T, V>& a, int grid, hipSt
       case 9: pcg_one_q<NT, 4, 2><<<grid, NT, 0, s>>>(a); break;
       case 10: pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a); break;
       // the pipelined kernel reads the partial sums as pairs (an even grid); an odd one runs LF
+      // x every other launch: even launches defer their x term, odd ones flush both
       case 11:
         if (grid % 2) pcg_one_q<NT, 4, 2, true><<<grid, NT, 0, s>>>(a);
-        else pcg_one_p<NT, 4, 2><<<grid, NT, 0, s>>>(a);
+        else if (a.kk & 1) pcg_one_p<NT, 4, 2, XM_FLUSH><<<grid, NT, 0, s>>>(a);
+        else pcg_one_p<NT, 4, 2, XM_DEFER><<<grid, NT, 0, s>>>(a);
         break;
       case 13: pcg_one_q<NT, 4, 3><<<grid, NT, 0, s>>>(a); break;
       case 14: pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a); break;
       default:
         if (grid % 2) pcg_one_q<NT, 4, 3, true><<<grid, NT, 0, s>>>(a);
-        else pcg_one_p<NT, 4, 3><<<grid, NT, 0, s>>>(a);
+        else if (a.kk & 1) pcg_one_p<NT, 4, 3, XM_FLUSH><<<grid, NT, 0, s>>>(a);
+        else pcg_one_p<NT, 4, 3, XM_DEFER><<<grid, NT, 0, s>>>(a);
         break;
     }
     JNO_LAUNCH_CHECK();

@@ -39,6 +39,9 @@ struct PcgState {
   int32_t special;
   int32_t outer_done;  // scaled mixed solves: the outer step decided to stop (convergence, cap, ...)
   double g_hist[2];
+  // one-kernel iterations with x updated every other launch: a_pend[k & 1] = the (FP32) alpha whose
+  // x term launch k-1 deferred to launch k (0 when nothing is pending)
+  double a_pend[2];
 };
 
 // A kernel-side view of the PCG state for SpMV kernels launched by the PCG engine; empty (st =

@@ -10,6 +10,8 @@ FETCH_SIZE/WRITE_SIZE give the counter-to-bytes calibration for the PCG kernels'
 --case c4 instead solves 256 uneven P1 systems (n in [24, 64]) to 1e-10 with the batched solver
 (--batch-step one|three: the one-kernel step, the default, or the three-kernel step);
 --case heat marches --iters backward-Euler steps of the n^3 transient heat block (--n, default 128).
+--case c2step runs --iters whole C2 steps (128^3 assemble + mixed solve to 1e-10 with residual
+replacement after a 100x drop of the FP32 residual), after two warm steps.
 --case assemble re-assembles the n^3 P1 system (--n, default 128) --iters times on the structured-box
 kernel and then --iters times on the incidence-gather kernel, both writing the DIA layout in the
 same pass (one solve first picks the offsets).
@@ -111,6 +113,29 @@ def run_assemble(dev, iters: int, n: int = 128):
               f"evaluate, nnz {A.nnz}, DIA fused {fused is not None and fused.format == 'dia'}")
 
 
+def run_c2_steps(dev, steps: int):
+    import jno_amd as J
+    from jno_amd import precond, solve
+
+    m = J.kuhn_box(128, device=dev)
+    system = J.fem_poisson(m)
+    k = torch.tensor(np.exp(0.5 * np.random.default_rng(0).standard_normal(m.n_cells)), device=dev)
+    sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=0.01)
+    for _ in range(2):  # warm: the layout choice, code objects
+        A, b = system.evaluate({"k": k})
+        sv(A, b, M=precond.jacobi())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        A, b = system.evaluate({"k": k})
+        sv(A, b, M=precond.jacobi())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(steps, 1)
+    st = sv.last_stats
+    print(f"c2 step: {dt * 1e3:.3f} ms per assemble + mixed solve, {st['iterations']} inner launches, "
+          f"{st.get('refinements')} replacements")
+
+
 def calibration_copy(dev, mib: int):
     """A device-to-device copy of ``mib`` MiB: its FETCH_SIZE / WRITE_SIZE per byte calibrate the
     counters of the other kernels."""
@@ -172,6 +197,10 @@ def main():
         set_tuning(args.precision, blk, cap, vec, mode)
     if args.case == "heat":
         run_heat(dev, args.iters, args.precision, args.n, args.replace_factor)
+        return
+    if args.case == "c2step":
+        run_c2_steps(dev, args.iters)
+        calibration_copy(dev, args.copy_mib)
         return
     if args.case == "assemble":
         run_assemble(dev, args.iters, args.n)

@@ -478,6 +478,33 @@ def test_lds_staged_k1(dev, n):
             assert max(its[(vec, 5, rf)]) <= 1.1 * min(its[(3, 4, rf)]) + 12, its
 
 
+@pytest.mark.parametrize("n,cap", [(24, 768), (40, 64), (13, 8)])
+def test_x_every_other_launch_is_bitwise_eager(dev, n, cap):
+    """The pipelined one-kernel iteration (k1_vector 3) updates x every other launch: even launches
+    defer their term, odd ones add both, and a stopping odd launch adds the deferred one.  Its
+    products and sums are those of the eager update (k1_vector 2, x updated by every launch) in the
+    same order, so the solutions are bitwise equal, with the same launch and replacement counts.
+    Each solve stops for several replacements, at launches of either parity; the small grid caps give
+    several tiles per workgroup (the stop's flush pass walks them all)."""
+    from jno_amd import precond, solve
+
+    k = O.k_lognormal(6 * n**3, 21)
+    m, s, A, b = _system(n, k)
+    base = solve.Tuning.defaults()
+    for rf in (0.3, 0.1, 0.01):
+        out = {}
+        for vec in (2, 3):
+            sv = solve.cg(tol=1e-10, precision="mixed", replace_factor=rf,
+                          tuning=base.with_shape("mixed", 256, cap, vec, 5))
+            u = sv(A, b, M=precond.jacobi())
+            out[vec] = (u, dict(sv.last_stats))
+        (u2, s2), (u3, s3) = out[2], out[3]
+        assert s3["relres"] <= 1e-10, s3
+        assert (s2["iterations"], s2["refinements"]) == (s3["iterations"], s3["refinements"]), (s2, s3)
+        assert s3["refinements"] >= 2, s3
+        assert torch.equal(u2, u3), (n, cap, rf, float((u2 - u3).abs().max()))
+
+
 def test_adjoint_gradient_vs_oracle_and_fd(dev):
     import jno_amd as J
     from jno_amd import adjoint, precond, solve
