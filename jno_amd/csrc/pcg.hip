@@ -114,7 +114,26 @@ struct PcgArgs {
   T* q_out;
   PcgState* st;
   Partials part;
+  // one-kernel iterations: host-pinned slot that this launch writes its final state into (the
+  // last launch of each chunk of the pinned poller), or nullptr
+  PcgState* post;
 };
+
+// The solve state for the host's poll, written at the end of the last launch of a chunk instead of
+// a device-to-host copy command after it (a copy costs a launch and a gap of several us before
+// the next kernel).  Only thread 0 of workgroup 0 writes state fields during a launch; the first
+// lanes of its wave then copy the state, one 8-byte word each: the fence waits for the wave's
+// earlier stores (thread 0's), and the loads bypass the L1 (agent scope).  The host reads the slot
+// after an event recorded behind the launch has completed.
+__device__ __forceinline__ void post_state(const PcgState* st, PcgState* post) {
+  static_assert(sizeof(PcgState) % 8 == 0 && sizeof(PcgState) / 8 <= 64, "one 8-byte word per lane of a wave");
+  constexpr int W = (int)(sizeof(PcgState) / 8);
+  if (post == nullptr || blockIdx.x != 0 || threadIdx.x >= W) return;
+  __threadfence_block();
+  const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(st) + threadIdx.x, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(post) + threadIdx.x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // sum of part[0..G) in a fixed order; every thread gets the total
 template <int NT = PCG_NT>
@@ -735,12 +754,18 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
     tile_at(0, f0, f1);
     load1(f0 - h + (int)threadIdx.x, f1 + h);
   }
-  if (st->done) return;  // read after the first loads are issued, as in pcg_one_q
+  if (st->done) {  // read after the first loads are issued, as in pcg_one_q
+    post_state(st, a.post);
+    return;
+  }
   double in[4];
   reduce_in<4>(a.part.C + (k & 1) * 4 * (int64_t)G, G, sm, in);
   double alpha, beta;
   int special;
-  if (!one_decide(st, k, in, alpha, beta, special)) return;
+  if (!one_decide(st, k, in, alpha, beta, special)) {
+    post_state(st, a.post);
+    return;
+  }
   // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const T al = (T)alpha, be = (T)beta;
   const bool upd = special == 0;
@@ -868,6 +893,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
     if (t + 1 < nt && u0 < u1) __syncthreads();  // the next tile rewrites pt and rt
   }
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+  post_state(st, a.post);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1028,7 +1054,10 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   // the stop flag is read after the first loads are issued, so its latency overlaps them (a launch
   // past a stop exits with its loads unused: measured 0.35 us less per working launch, 1% less per
   // 128^3 solve, against the extra traffic of the exiting ones)
-  if (st->done) return;
+  if (st->done) {
+    post_state(st, a.post);
+    return;
+  }
   // head: reduce the partials of launch k-1 (fixed order: per thread, per wave, then the waves)
   double in[4];
   {
@@ -1056,7 +1085,10 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
   }
   double alpha, beta;
   int special;
-  if (!one_decide(st, k, in, alpha, beta, special)) return;
+  if (!one_decide(st, k, in, alpha, beta, special)) {
+    post_state(st, a.post);
+    return;
+  }
   // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const float al = (float)alpha, be = (float)beta;
   const bool upd = special == 0;
@@ -1139,6 +1171,7 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
     }
   }
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+  post_state(st, a.post);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1307,7 +1340,10 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
   load_b(cb, ca);
   load_f(cf, ca);
   // the stop flag is read after the first loads are issued, as in pcg_one_q
-  if (st->done) return;
+  if (st->done) {
+    post_state(st, a.post);
+    return;
+  }
   double in[4];
   {
     const int w = tid >> 6;
@@ -1354,6 +1390,7 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
         }
       }
     }
+    post_state(st, a.post);
     return;
   }
   const float al = (float)alpha, be = (float)beta;
@@ -1457,6 +1494,7 @@ __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
     if (go) go = step(t);
   for (int t = 3; go; ++t) go = step(t);
   store_out<NT, 4>(part, a.part.C + ((k + 1) & 1) * 4 * (int64_t)G, G, sm_out);
+  post_state(st, a.post);
 }
 
 // Zero the pads of up to 8 iteration vectors (PCG_PAD_BYTES before and after each of n elements).
@@ -2322,14 +2360,22 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
 }
 
 struct HostPoll {
-  PcgState* h = nullptr;  // pinned, 2 slots
+  PcgState* h = nullptr;  // pinned, 2 slots (filled by copies)
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // pinned coherent slots the one-kernel iterations write themselves (post_state, system-scope
+  // stores that bypass the GPU caches), and their events: these need no system-scope fence (its L2
+  // writeback delayed the next launch by ~6 us per poll), only the completion of the launch
+  PcgState* hk = nullptr;
+  hipEvent_t evk[2] = {nullptr, nullptr};
   bool ok = false;
   int init() {
     if (ok) return JNO_OK;
     JNO_HIP(hipHostMalloc((void**)&h, 2 * sizeof(PcgState), hipHostMallocDefault));
     JNO_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     JNO_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    JNO_HIP(hipHostMalloc((void**)&hk, 2 * sizeof(PcgState), hipHostMallocCoherent));
+    JNO_HIP(hipEventCreateWithFlags(&evk[0], hipEventDisableTiming | hipEventDisableSystemFence));
+    JNO_HIP(hipEventCreateWithFlags(&evk[1], hipEventDisableTiming | hipEventDisableSystemFence));
     ok = true;
     return JNO_OK;
   }
@@ -2345,12 +2391,18 @@ struct Poller {
   bool sync = false;
   PcgState local[2];
   PcgState* slots() { return sync ? local : g_poll.h; }
+  PcgState* kslots() { return g_poll.hk; }  // written by the launches (pinned poller only)
   int init() {
     if (sync) return JNO_OK;
     JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
     return JNO_OK;
   }
-  int post(int slot, const PcgState* d, hipStream_t s) {
+  // by_kernel: the chunk's last launch wrote the slot itself (post_state); only the event is needed
+  int post(int slot, const PcgState* d, hipStream_t s, bool by_kernel = false) {
+    if (by_kernel && !sync) {
+      JNO_HIP(hipEventRecord(g_poll.evk[slot], s));
+      return JNO_OK;
+    }
     JNO_HIP(hipMemcpyAsync(&slots()[slot], d, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     if (sync) {
       JNO_HIP(hipStreamSynchronize(s));
@@ -2359,8 +2411,8 @@ struct Poller {
     }
     return JNO_OK;
   }
-  int wait(int slot) {
-    if (!sync) JNO_HIP(hipEventSynchronize(g_poll.ev[slot]));
+  int wait(int slot, bool by_kernel = false) {
+    if (!sync) JNO_HIP(hipEventSynchronize(by_kernel ? g_poll.evk[slot] : g_poll.ev[slot]));
     return JNO_OK;
   }
   int read(const PcgState* d, hipStream_t s, PcgState* out) {
@@ -2684,7 +2736,7 @@ constexpr int PACE_MARGIN = 1;
 
 template <typename T, typename V>
 static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid,
-                     hipStream_t s, PcgState* out, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr,
+                     hipStream_t s, PcgState* out, int64_t maxiter, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr,
                      InnerPace* pace = nullptr) {
   int rc0 = poll.init();
   if (rc0) return rc0;
@@ -2699,6 +2751,8 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
                                  // (one-kernel iterations: first launch of each event pair's window)
   int64_t win = -1;              // first launch of the open timing window (one-kernel iterations)
   int n = pace ? std::min(chunk, PACE_FIRST) : chunk;
+  bool polled = false;
+  const bool kpost = rbuf && !poll.sync;  // one-kernel launches write the pinned poll slots
   for (;;) {
     for (int i = 0; i < n; ++i, ++k) {
       a.p_old = pbuf[k & 1];
@@ -2709,6 +2763,8 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
         a.r_out = rbuf[(k + 1) & 1];
         a.q = qbuf[k & 1];
         a.q_out = qbuf[(k + 1) & 1];
+        // the chunk's last launch writes the poll slot itself (pinned poller)
+        a.post = kpost && i == n - 1 ? &poll.kslots()[slot] : nullptr;
       }
       if (prof && rbuf) {
         if (k % g_prof.stride == 0) {
@@ -2736,17 +2792,29 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
         if (rc) return rc;
       }
     }
-    int rcp = poll.post(slot, a.st, s);
+    int rcp = poll.post(slot, a.st, s, kpost);
     if (rcp) return rcp;
+    // the device stops an inner loop within maxiter launches; a host that has not seen the stop
+    // two chunks past that reads a state the device did not write (never expected)
+    JNO_REQUIRE(k - k0 - 2 * (int64_t)chunk - 8 <= maxiter, JNO_EHIP,
+                "pcg: no stop reported by the device after %lld launches", (long long)(k - k0));
     upto[slot] = k;
     ++round;
     n = chunk;
     if (pending) {
       const int prev = slot ^ 1;
-      rcp = poll.wait(prev);
+      rcp = poll.wait(prev, kpost);
       if (rcp) return rcp;
-      const PcgState& h = poll.slots()[prev];
-      if (h.done1) break;
+      const PcgState& h = kpost ? poll.kslots()[prev] : poll.slots()[prev];
+      if (h.done1) {
+        // one-kernel iterations: the launches after the stop exit before writing any state, so
+        // the polled state is final and the queue need not drain before the caller's next commands
+        if (rbuf && !poll.sync) {
+          *out = h;
+          polled = true;
+        }
+        break;
+      }
       if (pace && rbuf) {
         // r.r read by the last launch before that poll (a resuming launch does not record it, but
         // the first chunk holds more than one launch)
@@ -2772,8 +2840,10 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
     pending = 1;
     slot ^= 1;
   }
-  int rcr = poll.read(a.st, s, out);
-  if (rcr) return rcr;
+  if (!polled) {
+    int rcr = poll.read(a.st, s, out);
+    if (rcr) return rcr;
+  }
   if (prof && rbuf) {
     for (size_t t = 0; t < sampled.size(); ++t) {
       if (sampled[t] + PROF_WIN - 1 >= out->k_stop) break;  // every launch of the window did work
@@ -2858,6 +2928,7 @@ static PcgArgs<T, V> make_args(const jno_op* op, const V* vals, const T* dinv, T
   a.x = x;
   a.st = w.st;
   a.part = w.part;
+  a.post = nullptr;
   a.p_old = nullptr;
   a.p_new = nullptr;
   a.unit = 0;
@@ -3041,7 +3112,7 @@ extern "C" int jno_pcg_f64(const jno_op* h_op, const double* d_dinv, const doubl
     if (rc) return rc;
   }
   PcgState f;
-  rc = run_inner<double, double>(poll, h_op, a, pb, 0, ct.chunk, G, s, &f);
+  rc = run_inner<double, double>(poll, h_op, a, pb, 0, ct.chunk, G, s, &f, maxiter);
   if (rc) return rc;
   // true residual ||b - A x|| / ||b||  (the eager guard of _maybe_residual_check)
   rc = op_spmv<double, double>(h_op, h_op->vals64, d_x, w.ax, s, w.ext);
@@ -3143,7 +3214,7 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
   InnerPace pace;
   for (;;) {
     const int64_t k_start = k;
-    rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f, one ? rb : nullptr, one ? qb : nullptr,
+    rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f, maxiter, one ? rb : nullptr, one ? qb : nullptr,
                                  one && ct.pace ? &pace : nullptr);
     if (rc) return rc;
     if (f.status == ST_UNSCALED) return JNO_OK;  // *used stays false: nothing the caller relies on changed
@@ -3244,7 +3315,7 @@ int pcg_mixed_impl(const jno_op* h_op, const double* d_dinv, const double* d_b, 
   PcgState f, e;
   double rr_true = 0.0, bb = 0.0;
   for (;;) {
-    rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f);
+    rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f, maxiter);
     if (rc) return rc;
     k = f.k_stop;
     // FP64 refinement: x += xs, r = b - A x, rs = (float) r
