@@ -120,11 +120,12 @@ struct PcgArgs {
 };
 
 // The solve state for the host's poll, written at the end of the last launch of a chunk instead of
-// a device-to-host copy command after it (a copy costs a launch and a gap of several us before
-// the next kernel).  Only thread 0 of workgroup 0 writes state fields during a launch; the first
-// lanes of its wave then copy the state, one 8-byte word each: the fence waits for the wave's
-// earlier stores (thread 0's), and the loads bypass the L1 (agent scope).  The host reads the slot
-// after an event recorded behind the launch has completed.
+// a device-to-host copy command after it (the copies and the queue drain at each stop cost 0.35 ms
+// of a 128^3 solve).  Only thread 0 of workgroup 0 writes state fields during a launch; the first
+// lanes of its wave then copy the state into the slot, one 8-byte word each: the fence waits for
+// the wave's earlier stores (thread 0's), the loads bypass the L1 (agent scope), and the stores go
+// to coherent pinned host memory at system scope.  The host reads the slot after an event recorded
+// behind the launch has completed.
 __device__ __forceinline__ void post_state(const PcgState* st, PcgState* post) {
   static_assert(sizeof(PcgState) % 8 == 0 && sizeof(PcgState) / 8 <= 64, "one 8-byte word per lane of a wave");
   constexpr int W = (int)(sizeof(PcgState) / 8);
@@ -2095,6 +2096,8 @@ __global__ void __launch_bounds__(PCG_NT) finalize_k(PcgState* st, Partials part
     st->gamma_next = 0.0;
     st->alpha_next = 0.0;
     st->outer_done = 0;
+    st->a_pend[0] = 0.0;
+    st->a_pend[1] = 0.0;
   }
   st->rr_replace = replace2 > 0.0 ? replace2 * rr : -1.0;
   if (check_scale && st->scale_bad) {
@@ -2191,15 +2194,10 @@ __global__ void accumulate_scaled_k(int64_t n, double* __restrict__ x, float* __
 // residual) -- outer_done set, the stop flags left set, so the next inner launches exit at once --
 // or restart the inner loop: the next replacement threshold (aimed at the remaining reduction when
 // that is smaller than a whole cycle, with margin) and, for one-kernel iterations, the rerun of
-// launch kr (special 2; 1 when the stop came at launch 0).
-__global__ void __launch_bounds__(PCG_NT) outer_k(PcgState* st, Partials part, int G, double tol, double atol,
-                                                  double repl2, int64_t k_start, int64_t kr, int special,
-                                                  int64_t maxiter) {
-  __shared__ double sm[PCG_NT / 64];
-  const double rt = reduce_partials(part.P, G, sm);
-  const double bb = reduce_partials(part.P + G, G, sm);
-  const double rr = reduce_partials(part.B + G, G, sm);
-  if (threadIdx.x != 0) return;
+// launch kr (special 2; 1 when the stop came at launch 0).  outer_decide runs on thread 0.
+__device__ __forceinline__ void outer_decide(PcgState* st, double rt, double bb, double rr, double tol, double atol,
+                                             double repl2, int64_t k_start, int64_t kr, int special,
+                                             int64_t maxiter) {
   st->rr_true = rt;
   st->bb = bb;
   st->rr0 = rr;
@@ -2232,6 +2230,18 @@ __global__ void __launch_bounds__(PCG_NT) outer_k(PcgState* st, Partials part, i
   st->status = ST_RUNNING;
   st->done = 0;
   st->done1 = 0;
+}
+
+__global__ void __launch_bounds__(PCG_NT) outer_k(PcgState* st, Partials part, int G, double tol, double atol,
+                                                  double repl2, int64_t k_start, int64_t kr, int special,
+                                                  int64_t maxiter, PcgState* post) {
+  __shared__ double sm[PCG_NT / 64];
+  const double rt = reduce_partials(part.P, G, sm);
+  const double bb = reduce_partials(part.P + G, G, sm);
+  const double rr = reduce_partials(part.B + G, G, sm);
+  if (threadIdx.x == 0) outer_decide(st, rt, bb, rr, tol, atol, repl2, k_start, kr, special, maxiter);
+  // the decision for the host, which reads it while the next inner loop's first launches run
+  post_state(st, post);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2362,20 +2372,20 @@ static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
 struct HostPoll {
   PcgState* h = nullptr;  // pinned, 2 slots (filled by copies)
   hipEvent_t ev[2] = {nullptr, nullptr};
-  // pinned coherent slots the one-kernel iterations write themselves (post_state, system-scope
-  // stores that bypass the GPU caches), and their events: these need no system-scope fence (its L2
-  // writeback delayed the next launch by ~6 us per poll), only the completion of the launch
+  // pinned coherent slots the one-kernel iterations write themselves (post_state: system-scope
+  // stores that bypass the GPU caches), and their events, which need no system-scope fence, only
+  // the completion of the launch.  Slot 2 holds outer_k's decision (scaled mixed solves).
   PcgState* hk = nullptr;
-  hipEvent_t evk[2] = {nullptr, nullptr};
+  hipEvent_t evk[3] = {nullptr, nullptr, nullptr};
   bool ok = false;
   int init() {
     if (ok) return JNO_OK;
     JNO_HIP(hipHostMalloc((void**)&h, 2 * sizeof(PcgState), hipHostMallocDefault));
     JNO_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     JNO_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    JNO_HIP(hipHostMalloc((void**)&hk, 2 * sizeof(PcgState), hipHostMallocCoherent));
-    JNO_HIP(hipEventCreateWithFlags(&evk[0], hipEventDisableTiming | hipEventDisableSystemFence));
-    JNO_HIP(hipEventCreateWithFlags(&evk[1], hipEventDisableTiming | hipEventDisableSystemFence));
+    JNO_HIP(hipHostMalloc((void**)&hk, 3 * sizeof(PcgState), hipHostMallocCoherent));
+    for (int i = 0; i < 3; ++i)
+      JNO_HIP(hipEventCreateWithFlags(&evk[i], hipEventDisableTiming | hipEventDisableSystemFence));
     ok = true;
     return JNO_OK;
   }
@@ -2383,10 +2393,11 @@ struct HostPoll {
 static thread_local HostPoll g_poll;
 
 // How a solve reads the device-side state back.  Pinned (the default): into the thread's pinned
-// slots with an event each, one chunk behind the launches, so the queue never drains.  Sync (the
-// allocation-free entry points): into host memory of the solve itself, read synchronously, so
+// slots with an event each, one chunk behind the launches, so the queue never drains; one-kernel
+// iterations write their slots themselves (kslots), other iterations get a copy per poll.  Sync
+// (the allocation-free entry points): into host memory of the solve itself, read synchronously, so
 // nothing is allocated or created; the queue drains at each poll.  The control flow (and hence the
-// launches and results) is the same in both.
+// launches and results) is the same in all cases.
 struct Poller {
   bool sync = false;
   PcgState local[2];
@@ -2397,7 +2408,7 @@ struct Poller {
     JNO_REQUIRE(g_poll.init() == JNO_OK, JNO_EHIP, "pcg: pinned state allocation failed");
     return JNO_OK;
   }
-  // by_kernel: the chunk's last launch wrote the slot itself (post_state); only the event is needed
+  // by_kernel: the chunk's last launch writes the slot itself (post_state); only the event is needed
   int post(int slot, const PcgState* d, hipStream_t s, bool by_kernel = false) {
     if (by_kernel && !sync) {
       JNO_HIP(hipEventRecord(g_poll.evk[slot], s));
@@ -2737,7 +2748,7 @@ constexpr int PACE_MARGIN = 1;
 template <typename T, typename V>
 static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2], int64_t k0, int chunk, int grid,
                      hipStream_t s, PcgState* out, int64_t maxiter, T* rbuf[2] = nullptr, T* qbuf[2] = nullptr,
-                     InnerPace* pace = nullptr) {
+                     InnerPace* pace = nullptr, bool outer_posted = false) {
   int rc0 = poll.init();
   if (rc0) return rc0;
   const bool prof = g_prof.on && !poll.sync;  // the allocation-free path creates no events
@@ -2746,6 +2757,8 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
   int64_t upto[2] = {k0, k0};  // launches enqueued before each outstanding poll
   int64_t k_ref = -1;          // first sampled launch of this inner loop and its r.r
   double rr_ref = 0.0;
+  int64_t k_last = -1;  // the previous sample
+  double rr_last = 0.0;
   size_t nev = 0;
   std::vector<int64_t> sampled;  // iteration index of each (start, K0, SpMV, K2) event quadruple
                                  // (one-kernel iterations: first launch of each event pair's window)
@@ -2801,6 +2814,17 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
     upto[slot] = k;
     ++round;
     n = chunk;
+    if (outer_posted && round == 1) {
+      // the outer step before this loop posted its decision (slot 2): read it while the first
+      // chunk runs, so that a stopped solve enqueues no second chunk
+      rcp = poll.wait(2, true);
+      if (rcp) return rcp;
+      if (poll.kslots()[2].outer_done) {
+        *out = poll.kslots()[2];
+        polled = true;
+        break;
+      }
+    }
     if (pending) {
       const int prev = slot ^ 1;
       rcp = poll.wait(prev, kpost);
@@ -2826,9 +2850,16 @@ static int run_inner(Poller& poll, const jno_op* op, PcgArgs<T, V> a, T* pbuf[2]
         } else if (kl > k_ref && rr > 0.0 && rr_ref > 0.0) {
           pace->log_rate = std::log(rr / rr_ref) / (double)(kl - k_ref);
         }
+        // CG converges faster as a cycle proceeds: the rate since the previous poll, when it is
+        // the faster one, predicts the stop better than the cycle's average
+        double rate = pace->log_rate;
+        if (k_last >= 0 && kl > k_last && rr > 0.0 && rr_last > 0.0)
+          rate = std::min(rate, std::log(rr / rr_last) / (double)(kl - k_last));
+        k_last = kl;
+        rr_last = rr;
         const double target = std::max(h.atol2, h.rr_replace);
-        if (pace->log_rate < -1e-3 && rr > 0.0 && target > 0.0) {
-          const double need = rr > target ? std::log(target / rr) / pace->log_rate : 0.0;
+        if (rate < -1e-3 && rr > 0.0 && target > 0.0) {
+          const double need = rr > target ? std::log(target / rr) / rate : 0.0;
           const int64_t queued = k - upto[prev];
           const double want = std::ceil(need) - (double)queued + PACE_MARGIN;
           n = (int)std::max(1.0, std::min((double)chunk, want));
@@ -3207,15 +3238,21 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
   a.split = one ? MODE_ONE : tu.split == MODE_SCALED_FUSED || tu.split == MODE_ONE ? MODE_FUSED : MODE_SPLIT;
   // Each cycle: the inner loop runs to its stop; x is accumulated, the FP64 residual prepared and
   // the outer step decides on the device whether to stop or restart, with no host round trip: the
-  // host learns an outer stop from the next inner loop's first poll (its launches exit at once).
+  // host learns an outer stop from outer_k's own post (one-kernel iterations, pinned poller), read
+  // once the next inner loop's first chunk is queued, or else from that loop's first poll (its
+  // launches exit at once).
   int64_t k = 0, refinements = 0, resumes = 0;
   bool last_resume = false;
   PcgState f;
   InnerPace pace;
+  // one-kernel iterations with the pinned poller: outer_k posts its decision to the third slot
+  rc = poll.init();
+  if (rc) return rc;
+  const bool opost = one && !poll.sync;
   for (;;) {
     const int64_t k_start = k;
     rc = run_inner<float, float>(poll, h_op, a, pb, k, ct.chunk, G, s, &f, maxiter, one ? rb : nullptr, one ? qb : nullptr,
-                                 one && ct.pace ? &pace : nullptr);
+                                 one && ct.pace ? &pace : nullptr, opost && refinements > 0);
     if (rc) return rc;
     if (f.status == ST_UNSCALED) return JNO_OK;  // *used stays false: nothing the caller relies on changed
     *used = true;
@@ -3234,8 +3271,10 @@ static int mixed_scaled(Poller& poll, const CallTuning& ct, const jno_op* h_op, 
     ++refinements;
     const bool resume = one && k > 0;
     outer_k<<<1, PCG_NT, 0, s>>>(w.st, w.part, G, tol, atol, repl2, k_start, kr, one ? (resume ? 2 : 1) : 0,
-                                 one ? maxiter - resumes - (resume ? 1 : 0) : maxiter);
+                                 one ? maxiter - resumes - (resume ? 1 : 0) : maxiter,
+                                 opost ? &poll.kslots()[2] : nullptr);
     JNO_LAUNCH_CHECK();
+    if (opost) JNO_HIP(hipEventRecord(g_poll.evk[2], s));
     if (resume) ++resumes;
     last_resume = resume;
     k = kr;
