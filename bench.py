@@ -417,6 +417,23 @@ def c5_rate(J, torch, dev, n=32):
     # x once, products written) + gather (node pointers, products, x, mask, y)
     spmv_bytes = slots * (40 + 4 + 4 + 40) + rows * 4 + T * 30 * 4 + (N + 1) * 4 + T * 30 * 4 + rows * (4 + 1 + 4)
     spmv_ms = prof_ms[1] / max(prof_n.value, 1)
+    # the forward solve plus the gradient of the compliance C = b.u with respect to every element's
+    # E (autograd through the closed-form contraction), on the element-by-element operator
+    from jno_amd.elasticity import compliance
+
+    Eg = E.clone().requires_grad_(True)
+    Cv, _ = compliance(s, Eg, sv)
+    Cv.backward()  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Eg = E.clone().requires_grad_(True)
+    Cv, _ = compliance(s, Eg, sv)
+    Cv.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    g_E = Eg.grad
+    # C falls as any E grows (K_e is positive semidefinite), up to rounding of a near-zero quadratic form
+    grad_finite = bool(torch.isfinite(g_E).all()) and bool((g_E <= 1e-12 * float(g_E.abs().max())).all())
     Ab, bb = s.evaluate({"E": E})
     # the assembled operator on the layout and lanes the autotuner picks (BSR3 at this size)
     svb = solve.cg(tol=1e-10, precision="mixed")
@@ -434,6 +451,7 @@ def c5_rate(J, torch, dev, n=32):
         "assembled_operator": svb.last_stats.get("operator"),
         "speedup_ebe_vs_bsr3": round(t_bsr3 / t_ebe, 3),
         "ebe_vs_bsr3_rel_l2": float(torch.linalg.norm(u - ub) / torch.linalg.norm(ub)),
+        "compliance_grad_s": round(t_grad, 6), "compliance_grad_nonpositive": grad_finite,
         "roofline": {"kernel": "ebe_elem_k<f32> + ebe_gather_k<f32> (one SpMV)", "avg_launch_ms": round(spmv_ms, 5),
                      "bytes_per_launch": spmv_bytes, "achieved": round(gbs, 1) if gbs else None, "unit": "GB/s",
                      "peak": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,

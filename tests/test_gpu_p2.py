@@ -162,8 +162,9 @@ def test_bsr3_solves_every_iteration_mode(dev, precision):
             assert _rel(_np(u), uo) <= 1e-9, (mode, blk, cap, st)
 
 
+@pytest.mark.parametrize("operator", ["ebe", "assembled"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_elasticity_solve_and_compliance_gradient(dev, precision):
+def test_elasticity_solve_and_compliance_gradient(dev, precision, operator):
     import jno_amd as J
     from jno_amd import precond, solve
     from jno_amd.elasticity import ElasticitySystem, compliance
@@ -175,12 +176,13 @@ def test_elasticity_solve_and_compliance_gradient(dev, precision):
     s = ElasticitySystem(J.kuhn_box(n))
     sv = solve.cg(tol=1e-10, precision=precision)
     Et = torch.tensor(E, device="cuda", requires_grad=True)
-    Cval, u = compliance(s, Et, sv)
+    Cval, u = compliance(s, Et, sv, operator=operator)
     Cval.backward()
+    assert (sv.last_stats["operator"]["format"] == "ebe") == (operator == "ebe"), sv.last_stats
     Ao, bo, d = O.elasticity_p2_system(p2, c2, E)
     uo = O.direct_solve(Ao, bo)
     assert _rel(_np(u), uo) <= 1e-9, sv.last_stats
-    assert abs(float(Cval) - bo @ uo) <= 1e-9 * abs(bo @ uo)
+    assert abs(float(Cval.detach()) - bo @ uo) <= 1e-9 * abs(bo @ uo)
     go = O.compliance_grad_p2(p2, c2, E, 0.3, uo)
     assert _rel(_np(Et.grad), go) <= 1e-8
     # central FD (h = 1e-6 E_e) with defect-correction solves to 1e-12 of the defect, against the
