@@ -463,9 +463,10 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
     """C4: ``count`` P1 Poisson systems on n^3 Kuhn boxes, n uniform in [24, 64] (default_rng(seed)),
     each with its own lognormal k (default_rng(1000 + i)), f = 1, solved together to rtol 1e-10; and
     ``n_rhs`` N(0,1) right-hand sides on one n_rhs_mesh^3 operator.  Effective bandwidth: the
-    algorithmic bytes of every launch an item took part in, against the solve time: per row 44 B
-    per one-kernel iteration (r, q, p, xs read and written: 32; the 3 stored upper diagonals of the
-    Jacobi-scaled operator: 12), and per residual replacement 32 B for folding xs into x64 (xs, x64
+    algorithmic bytes of every launch an item took part in, against the solve time: per row 42 B
+    per one-kernel iteration on average (r, q, p read and written: 24; the 3 stored upper diagonals
+    of the Jacobi-scaled operator: 12; xs read and written and p_{k-2} read every other launch: 6),
+    and per residual replacement 32 B for folding xs into x64 (xs, x64
     read and written, dinv read) plus 60 B for the FP64 true residual (b, x64, 4 FP64 diagonals,
     dinv read; r written).  Each item runs iterations + replacements + 1 iteration launches (the
     resumed launch after each replacement and after the start)."""
@@ -500,7 +501,7 @@ def c4_rate(J, torch, dev, count=256, seed=4, n_rhs=64, n_rhs_mesh=96):
     rows = [A.shape[0] for A, _ in systems]
 
     def batch_bytes(r, it, rf):
-        return r * (44 * (it + rf + 1) + (32 + 60) * (rf + 1))
+        return r * (42 * (it + rf + 1) + (32 + 60) * (rf + 1))
 
     moved = sum(batch_bytes(r, it, rf) for r, it, rf in zip(rows, its, refs))
     m96 = J.kuhn_box(n_rhs_mesh, device=dev)

@@ -532,7 +532,8 @@ __global__ void batch_scale_k(const BItem* items, const double* const* dinvs, co
 // o1 < o2 <= B1_HALO, far o3).  Launch k of an item reads r_{k-1}, q_{k-1} = A p_{k-1}, p_{k-1} and
 // the partials of its previous launch (r.r, p.q, q.r, q.q); alpha = gamma_{k-1} / p.q,
 // gamma_k = gamma_{k-1} - 2 alpha q.r + alpha^2 q.q, beta = gamma_k / gamma_{k-1}; then per row
-// r_k = r_{k-1} - alpha q_{k-1}, xs += alpha p_{k-1}, p_k = r_k + beta p_{k-1} (near neighbours
+// r_k = r_{k-1} - alpha q_{k-1}, xs += alpha p_{k-1} (applied every other launch: b1_body_q),
+// p_k = r_k + beta p_{k-1} (near neighbours
 // through LDS, the far ones recomputed from r, q, p), q_k = A p_k and the partials of r_k.r_k,
 // p_k.q_k, q_k.r_k, q_k.q_k.  r, q and p alternate between two buffers by the parity of k.
 // An item's phases, one launch each:
@@ -560,7 +561,8 @@ constexpr int B1_GRID = 4096;          // default workgroups per launch (C4: 106
 enum { P1_R2 = 0, P1_RES = 1, P1_ITER = 2, P1_DONE = 3 };
 
 struct B1State {
-  int32_t phase, status, bad, pad;
+  int32_t phase, status, bad;
+  float a_pend;         // the alpha whose xs term the item's previous (deferring) launch left pending
   int64_t k;            // ITER: the index of this launch
   int64_t kr;           // R2 / RES: the launch a resume runs (iterations done)
   int64_t refinements;
@@ -625,6 +627,7 @@ __device__ __forceinline__ B1State state_uni(const B1State* S) {
   s.phase = uni(s.phase);
   s.status = uni(s.status);
   s.bad = uni(s.bad);
+  s.a_pend = __int_as_float(uni(__float_as_int(s.a_pend)));
   s.k = uni(s.k);
   s.kr = uni(s.kr);
   s.refinements = uni(s.refinements);
@@ -714,9 +717,13 @@ __device__ __forceinline__ void stq(float* __restrict__ base4, uint32_t row4, F4
 // the last quad past the item's end are stored as 0.  Every load
 // stays in the workspace: at most 3 rows before an item (the buffer before the first) or 3 past its
 // padded end (the buffer after the last).
+// xs is updated every other launch of an item: a launch with even kk defers its term alpha p_{k-1}
+// (no xs traffic) and the next one (odd kk, `flush`) adds both, xs += apd p_{k-2} + alpha p_{k-1},
+// reading p_{k-2} from the buffer it is about to overwrite with p_k (own rows, read before the
+// write by the same thread), in the order and with the roundings of two eager updates.
 template <int NT>
 __device__ __forceinline__ void b1_body_q(const B1Args& a, const ItemView& it, int64_t r0, int64_t r1, int64_t kk,
-                                          float al, float be, float* pt, double* part) {
+                                          float al, float be, float apd, float* pt, double* part) {
   constexpr int TT = 4 * NT;
   const int n = (int)it.n;
   const int o1 = it.offs[1], o2 = it.offs[2], o3 = it.offs[3];
@@ -735,13 +742,19 @@ __device__ __forceinline__ void b1_body_q(const B1Args& a, const ItemView& it, i
   const float* __restrict__ v2 = it.s32 + ld - 4;
   const float* __restrict__ v3 = it.s32 + 2 * ld - 4;
   const bool upd = al != 0.0f;
+  const bool flush = (kk & 1) != 0;  // item-uniform: this launch writes xs
   const int tid = threadIdx.x;
   for (int t0 = (int)r0; t0 < (int)r1; t0 += TT) {
     const int t1 = min((int)r1, t0 + TT);
     const int i = t0 + 4 * tid;
     const bool own = i < t1;
     const uint32_t ic4 = (uint32_t)((own ? i : 0) + 4);
-    const F4 rv = ldq(R, ic4), qv = ldq(Q, ic4), pv = ldq(Pp, ic4), xv = ldq(XS, ic4);
+    const F4 rv = ldq(R, ic4), qv = ldq(Q, ic4), pv = ldq(Pp, ic4);
+    F4 xv = F4{0.0f, 0.0f, 0.0f, 0.0f}, p2 = F4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (flush) {
+      xv = ldq(XS, ic4);
+      p2 = ldq(Pn, ic4);  // p_{k-2}, before this launch's p_k
+    }
     // one near-halo row per thread below 2 hq (rows outside the item staged as 0)
     const bool hal = tid < 2 * hq;
     const int jh = tid < hq ? t0 - hq + tid : t1 + (tid - hq);
@@ -765,10 +778,13 @@ __device__ __forceinline__ void b1_body_q(const B1Args& a, const ItemView& it, i
       const uint32_t i4 = (uint32_t)(i + 4);
       stq(Rn, i4, F4{rn[0], rn[1], rn[2], rn[3]});
       stq(Pn, i4, F4{pn[0], pn[1], pn[2], pn[3]});
-      if (upd) {
+      if (flush && upd) {
         F4 xn;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xn[u] = vr[u] ? xv[u] + al * pv[u] : 0.0f;
+        for (int u = 0; u < 4; ++u) {
+          const float xb = apd != 0.0f ? xv[u] + apd * p2[u] : xv[u];
+          xn[u] = vr[u] ? xb + al * pv[u] : 0.0f;
+        }
         stq(XS, i4, xn);
       }
 #pragma unroll
@@ -819,11 +835,11 @@ __device__ __forceinline__ void b1_body_q(const B1Args& a, const ItemView& it, i
   }
 }
 
-// At least 3 waves per SIMD (at most 168 VGPRs; the kernel takes 155): three 256-thread
-// workgroups per CU.  C4 (256 systems, 29.1M rows): 106.7 ms per solve, where rows one per thread
-// (512 threads, 4 rows per thread, LDS-staged, 127 VGPRs) took 112.9 ms; issuing the first tile's
-// r, q, p before the reduction of the partial sums pushed the kernel to 168 VGPRs with spills and
-// measured 112.0 ms.
+// At least 3 waves per SIMD (at most 168 VGPRs; the kernel takes 157): three 256-thread
+// workgroups per CU.  C4 (256 systems, 29.1M rows): 103.2 ms per solve (106.6 with xs updated every
+// launch), where rows one per thread (512 threads, 4 rows per thread, LDS-staged, 127 VGPRs) took
+// 112.9 ms; issuing the first tile's r, q, p before the reduction of the partial sums pushed the
+// kernel to 168 VGPRs with spills and measured 112.0 ms.
 template <int NT>
 __global__ void __launch_bounds__(NT, 3) batch_one_k(B1Args a, int par) {
   constexpr int TT = 4 * NT;
@@ -919,6 +935,7 @@ __global__ void __launch_bounds__(NT, 3) batch_one_k(B1Args a, int par) {
       beta = kk == 0 ? 0.0 : rr / ((kk & 1) ? S.g_hist[1] : S.g_hist[0]);
       S.phase = P1_ITER;
       S.k = kk + 1;
+      S.a_pend = 0.0f;  // the stop's fold left nothing pending, and this launch (alpha = 0) defers nothing
     } else {  // P1_ITER
       kk = S.k;
       const double rr = in[0];
@@ -927,12 +944,18 @@ __global__ void __launch_bounds__(NT, 3) batch_one_k(B1Args a, int par) {
       const bool stop_repl = rr <= S.rr_replace;
       const bool bad = !isfinite(rr) || !isfinite(alpha) || !isfinite(beta);
       if (stop_repl || bad || kk > a.maxiter) {
-        // x64 += S xs, xs = 0 on the entry's rows; the true residual follows in the next launch
+        // x64 += S xs, xs = 0 on the entry's rows, with the term the previous launch deferred
+        // (an odd launch: xs + a_pend p_{k-2}, p_{k-2} in the buffer this launch would have
+        // written); the true residual follows in the next launch
         float* __restrict__ XS = a.xs + it.off;
+        const float apd = (kk & 1) ? S.a_pend : 0.0f;
+        const float* __restrict__ P2 = a.p[(kk & 1) ^ 1] + it.off;
         for (int64_t i = r0 + threadIdx.x; i < r1; i += NT) {
-          it.x[i] += sqrt(it.dinv64[i]) * (double)XS[i];
+          const float xsi = apd != 0.0f ? XS[i] + apd * P2[i] : XS[i];
+          it.x[i] += sqrt(it.dinv64[i]) * (double)xsi;
           XS[i] = 0.0f;
         }
+        S.a_pend = 0.0f;
         S.phase = P1_R2;
         S.kr = kk - 1;
         S.bad = S.bad || (bad && !stop_repl);
@@ -945,7 +968,10 @@ __global__ void __launch_bounds__(NT, 3) batch_one_k(B1Args a, int par) {
         S.g_hist[0] = rr;
       S.k = kk + 1;
     }
-    b1_body_q<NT>(a, it, r0, r1, kk, (float)alpha, (float)beta, pt, part);
+    // even launches defer their xs term to the next launch, odd ones add it (and their own)
+    const float apd_in = S.a_pend;
+    S.a_pend = (kk & 1) ? 0.0f : (float)alpha;
+    b1_body_q<NT>(a, it, r0, r1, kk, (float)alpha, (float)beta, apd_in, pt, part);
     b1_store_partials<NT>(part, Pout, NE, (int)entry, sm);
     if (lead) Sout[te.x] = S;
   }
