@@ -673,6 +673,9 @@ __device__ __forceinline__ bool one_decide_pre(PcgState* st, const DecideIn& din
 // (Interleaving r, p, x and q per row, to fetch a far neighbour in one load, measured slower: the
 // partially written lines.)
 // ---------------------------------------------------------------------------------------------
+// x update of a one-kernel launch: every launch (XM_EAGER), or every other launch -- an even launch
+// defers its term (XM_DEFER), the odd launch after it adds both (XM_FLUSH); see pcg_one_p
+constexpr int XM_EAGER = 0, XM_DEFER = 1, XM_FLUSH = 2;
 constexpr int ONE_STAGE_BYTES = 65536;
 // FW (far window): the far off-diagonals lie within LDS_FAR_SPAN of each other (the mass-carrying
 // 8-diagonal operators of a Kuhn lattice: offsets (nx+1)(ny+1) + {0, 1, nx+1, nx+2}), so p_k of the
@@ -680,7 +683,7 @@ constexpr int ONE_STAGE_BYTES = 65536;
 // instead of one range of TT rows per far diagonal and direction: a quarter of the far loads.
 constexpr int LDS_FAR_SPAN = 256;
 
-template <int NT, int ND, int NN, typename T, typename V, bool FW = false>
+template <int NT, int ND, int NN, typename T, typename V, bool FW = false, int XM = XM_EAGER>
 __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   constexpr int NF = ND > 1 ? ND - 1 - NN : 0;  // far off-diagonals
   static_assert(!FW || NF >= 2, "the far window groups two or more far diagonals");
@@ -708,16 +711,20 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
 #pragma unroll
   for (int d = 1; d < ND; ++d) o[d] = a.off.o[d];
   const int h = NN > 0 ? o[NN] : 0;
-  T rv[U1], qv[U1], pv[U1], xv[U1];
-  auto load1 = [&](int j0, int s1) {
+  T rv[U1], qv[U1], pv[U1], xv[U1], p2[U1];
+  // rows [j0, s1) with the tile's own rows [lo, hi): x (and, flushing, p_{k-2} from the buffer this
+  // launch overwrites with p_k, read before the write by the same thread) only for own rows
+  auto load1 = [&](int j0, int s1, int lo, int hi) {
 #pragma unroll
     for (int u = 0; u < U1; ++u) {
       const int j = j0 + u * NT;
       const bool in = j < s1 && j >= 0 && j < n;
+      const bool own = in && j >= lo && j < hi;
       rv[u] = ldc(a.r, j, in);
       qv[u] = ldc(a.q, j, in);
       pv[u] = ldc(a.p_old, j, in);
-      xv[u] = ldc(a.x, j, in);
+      if constexpr (XM != XM_DEFER) xv[u] = ldc(a.x, j, own);
+      if constexpr (XM == XM_FLUSH) p2[u] = ldc(a.p_new, j, own);
     }
   };
   // Tiles of equal length tl <= TT, nt per workgroup.  When the grid is whole XCD slabs and
@@ -753,8 +760,10 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   {
     int f0, f1;
     tile_at(0, f0, f1);
-    load1(f0 - h + (int)threadIdx.x, f1 + h);
+    load1(f0 - h + (int)threadIdx.x, f1 + h, f0, f1);
   }
+  // the alpha whose x term the previous (deferring) launch left to this one
+  const T apd = XM == XM_FLUSH ? (T)st->a_pend[k & 1] : (T)0;
   if (st->done) {  // read after the first loads are issued, as in pcg_one_q
     post_state(st, a.post);
     return;
@@ -764,28 +773,44 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
   double alpha, beta;
   int special;
   if (!one_decide(st, k, in, alpha, beta, special)) {
+    if constexpr (XM != XM_EAGER) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->a_pend[(k + 1) & 1] = 0.0;
+      // a stopping flushing launch adds the term the launch before it deferred: x = x_{k-1}
+      if (XM == XM_FLUSH && special == 0 && apd != (T)0) {
+        for (int t = 0; t < nt; ++t) {
+          int t0, t1;
+          tile_at(t, t0, t1);
+          for (int j = t0 + (int)threadIdx.x; j < t1; j += NT) a.x[j] = a.x[j] + apd * a.p_new[j];
+        }
+      }
+    }
     post_state(st, a.post);
     return;
   }
   // alpha = 0 (a first or resumed launch) leaves r and x as they are (q times 0)
   const T al = (T)alpha, be = (T)beta;
   const bool upd = special == 0;
+  if constexpr (XM != XM_EAGER) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->a_pend[(k + 1) & 1] = XM == XM_DEFER && upd ? (double)al : 0.0;
+  }
   double part[4] = {0.0, 0.0, 0.0, 0.0};
   for (int t = 0; t < nt; ++t) {
     int t0, t1;
     tile_at(t, t0, t1);
     if (t0 >= t1) break;  // tiles past the last row (the same for every thread)
     const int s0 = t0 - h, s1 = t1 + h;
-    if (t > 0) load1(s0 + (int)threadIdx.x, s1);
+    if (t > 0) load1(s0 + (int)threadIdx.x, s1, t0, t1);
     for (int j0 = s0 + (int)threadIdx.x; j0 < s1; j0 += U1 * NT) {
       T rn[U1], pn[U1], xn[U1];
 #pragma unroll
       for (int u = 0; u < U1; ++u) {
         rn[u] = rv[u] - al * qv[u];
         pn[u] = rn[u] + be * pv[u];
-        xn[u] = xv[u] + al * pv[u];
+        // flushing: the deferred term first, then this launch's, as two eager updates add them
+        const T xb = XM == XM_FLUSH && apd != (T)0 ? xv[u] + apd * p2[u] : xv[u];
+        xn[u] = XM == XM_DEFER ? (T)0 : xb + al * pv[u];
       }
-      load1(j0 + U1 * NT, s1);  // the next batch in flight meanwhile (past the end: row 0, unused)
+      load1(j0 + U1 * NT, s1, t0, t1);  // the next batch in flight meanwhile (past the end: row 0, unused)
 #pragma unroll
       for (int u = 0; u < U1; ++u) {
         const int j = j0 + u * NT;
@@ -795,7 +820,7 @@ __global__ void __launch_bounds__(NT) pcg_one(PcgArgs<T, V> a) {
             rt[j - t0] = rn[u];
             sto(&a.r_out[j], rn[u]);
             sto(&a.p_new[j], pn[u]);
-            if (upd) sto(&a.x[j], xn[u]);
+            if (XM != XM_DEFER && upd) sto(&a.x[j], xn[u]);
             part[0] += (double)rn[u] * (double)rn[u];
           }
         }
@@ -1199,8 +1224,6 @@ __global__ void __launch_bounds__(NT) pcg_one_q(PcgArgs<float, float> a) {
 // as the residual replacement and the end of the solve need.  Special launches (start, resume)
 // have alpha = 0: they never find a term pending (the stop before them flushed it) and defer none.
 // ---------------------------------------------------------------------------------------------
-constexpr int XM_EAGER = 0, XM_DEFER = 1, XM_FLUSH = 2;
-
 template <int NT, int ND, int NN, int XM = XM_EAGER>
 __global__ void __launch_bounds__(NT) pcg_one_p(PcgArgs<float, float> a) {
   static_assert(ND == 4 && NN >= 2 && NN <= 3, "instantiated for 4-diagonal operators with 2 or 3 near diagonals");
@@ -2573,7 +2596,15 @@ static int launch_lds(bool one, const jno_op* op, const PcgArgs<T, V>& a, int gr
       JNO_LDS_CASE(4, 2, false)
       JNO_LDS_CASE(4, 3, false)
       JNO_LDS_CASE(8, 1, false)
-      JNO_LDS_CASE(8, 3, true)
+      // the transient step operator M + theta dt A: x every other launch, as in pcg_one_p
+      case 8 * 8 + 3:
+        if (!one)
+          pcg_k1_lds<NT, 8, 3, T, V><<<grid, NT, 0, s>>>(a);
+        else if (a.kk & 1)
+          pcg_one<NT, 8, 3, T, V, true, XM_FLUSH><<<grid, NT, 0, s>>>(a);
+        else
+          pcg_one<NT, 8, 3, T, V, true, XM_DEFER><<<grid, NT, 0, s>>>(a);
+        break;
       default: JNO_REQUIRE(false, JNO_EINVAL, "pcg: no LDS-staged kernel for this diagonal set");
     }
 #undef JNO_LDS_CASE
