@@ -275,6 +275,18 @@ typedef struct jno_ebe {
                                    -1 for padding (the inverse of inc)                         */
   const uint8_t* dmask;         /* u8[3N]: constrained dofs                                    */
   void* scratch;                /* f64[30 T]: product buffer of jno_op_spmv_* (nullable)       */
+  /* Structured Kuhn box (jno_ebe_box_map found the cells to be the P2 box of this grid, and every
+   * cell of Kuhn type t to have the class box_class[t]): the SpMV runs on the half lattice, with
+   * no product buffer.  box_n[0] = 0: the general path. */
+  int32_t box_n[3];             /* voxels per axis                                             */
+  int32_t box_class[6];         /* class (table block) of each Kuhn tet type                  */
+  int32_t box_pad;
+  const int32_t* box_perm;      /* i32[(2nx+1)(2ny+1)(2nz+1)]: node at each half-lattice point */
+  const uint8_t* box_mask;      /* u8[same]: its constrained-dof bits (bit c: dof 3 node + c)  */
+  const double* box_e64;        /* f64[T]: E per cell, in cell order                           */
+  const float* box_e32;         /* f32[T]                                                      */
+  const float* box_rows32;      /* f32[6*10*10*16]: K_{box_class[t]}[3a + r][3b + c] at
+                                   ((10 t + a) 10 + b) 16 + 3 r + c (9 of 16 used)             */
 } jno_ebe;
 
 typedef struct jno_op {
@@ -321,6 +333,19 @@ int jno_ebe_incidence(int64_t n_entries, const int32_t* d_inc, const int32_t* d_
                       int32_t* d_inc_out, int32_t* d_qpos, void* stream);
 int jno_ebe_diagonal(const jno_op* h_op, int invert, double* d_out, void* stream);
 int jno_ebe_dirichlet_rhs(int64_t n_rows, const uint8_t* d_dmask, double* d_b, void* stream);
+/* The half-lattice map of a P2 mesh that is the structured Kuhn box of nx x ny x nz voxels (the P2
+ * promotion of Shape.box(..).structured(n), jno/domain/geometries.py:204-301 with
+ * jno/utils/solver/fem_utils.py:1979-2044): d_perm[L] = the node at half-lattice point
+ * L = (I (2ny+1) + J)(2nz+1) + K and d_mask_lat[L] its constrained-dof bits from d_dmask (u8[3N]).
+ * *h_ok = 1 when the cells are exactly that box (every cell e = 6 voxel + t holds the nodes of
+ * Kuhn type t's local points, each node at one point, every point covered), else 0 (the general
+ * element-by-element path applies).  d_ws: i32[N + 1].  Synchronizes the stream (one-shot set-up). */
+int jno_ebe_box_map(int nx, int ny, int nz, const int32_t* d_cells10, int64_t n_cells, int64_t n_nodes,
+                    const uint8_t* d_dmask, int32_t* d_perm, uint8_t* d_mask_lat, int32_t* d_ws, int32_t* h_ok,
+                    void* stream);
+/* The coefficient table of the structured-box SpMV: d_rows f32[6*10*10*16] from the FP32 class table of
+ * jno_ebe (d_table32, n_classes blocks) and the class of each Kuhn type (h_class, 6 host ints). */
+int jno_ebe_box_rows(const float* d_table32, int n_classes, const int32_t* h_class, float* d_rows, void* stream);
 
 /* Distinct column-row offsets (up to JNO_DIA_MAX upper ones) and symmetry of a CSR.
  * *h_ndiag = number of distinct offsets >= 0 (0 is always counted), or -1 when there are more than

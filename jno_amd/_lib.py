@@ -56,6 +56,14 @@ class JnoEbe(C.Structure):
         ("qpos", C.c_void_p),
         ("dmask", C.c_void_p),
         ("scratch", C.c_void_p),
+        ("box_n", C.c_int32 * 3),
+        ("box_class", C.c_int32 * 6),
+        ("box_pad", C.c_int32),
+        ("box_perm", C.c_void_p),
+        ("box_mask", C.c_void_p),
+        ("box_e64", C.c_void_p),
+        ("box_e32", C.c_void_p),
+        ("box_rows32", C.c_void_p),
     ]
 
 

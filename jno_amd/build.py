@@ -17,7 +17,7 @@ INCLUDE = PKG.parent / "include"
 LIB = PKG / "libjno_fem.so"
 OBJDIR = PKG / "_build"
 
-SOURCES = ["common.hip", "mesh.hip", "pattern.hip", "plan.hip", "assemble_p1.hip", "sparse.hip", "pcg.hip", "pcg_static.hip", "p2.hip", "ebe.hip", "batch.hip", "transient.hip"]
+SOURCES = ["common.hip", "mesh.hip", "pattern.hip", "plan.hip", "assemble_p1.hip", "sparse.hip", "pcg.hip", "pcg_static.hip", "p2.hip", "ebe.hip", "ebe_box.hip", "batch.hip", "transient.hip"]
 
 HIPFLAGS = [
     "-O3",

@@ -335,6 +335,8 @@ static int ebe_launch(const jno_op* op, const T* x, T* y, T* yloc, double* part,
   int rc = check_ebe(op);
   if (rc) return rc;
   const jno_ebe* e = op->ebe;
+  if constexpr (std::is_same<T, float>::value)
+    if (ebe_box_bricks(e) > 0) return ebe_box_launch<T>(op, x, y, part, st, rpb > 0 ? grid : 0, cg, s);
   JNO_REQUIRE(yloc, JNO_EINVAL, "ebe: no product buffer (scratch)");
   const T* table = std::is_same<T, float>::value ? (const T*)e->table32 : (const T*)e->table64;
   const T* scale = std::is_same<T, float>::value ? (const T*)e->scale32 : (const T*)e->scale64;

@@ -2357,6 +2357,13 @@ static Tuning effective_tuning(const Tuning& tu, const jno_op* op) {
 }
 static void pcg_grid(const jno_op* op, const Tuning& tu, int* G, int64_t* rpb) {
   const int64_t n = op->n;
+  if (op->format == JNO_FMT_EBE && ebe_box_bricks(op->ebe) > 0) {
+    // the structured-box SpMV: one workgroup per brick (whole XCD slabs), its partial per workgroup
+    const int64_t want = std::min<int64_t>(cdiv(ebe_box_bricks(op->ebe), (int64_t)N_XCD) * N_XCD, PCG_MAX_BLOCKS);
+    *G = (int)want;
+    *rpb = cdiv(cdiv(n, want), (int64_t)12) * 12;
+    return;
+  }
   if (tu.split == MODE_ONE && tu.vec >= 1) {
     // pcg_one_q: one tile of 4 block rows per workgroup where the grid cap allows (rounded up to
     // whole XCD slabs), else several equal tiles per workgroup

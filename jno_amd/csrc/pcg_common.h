@@ -102,6 +102,12 @@ template <typename T>
 int ebe_spmv_pcg(const jno_op* op, const T* p, T* q, T* yloc, double* part, PcgStateView st, int grid, int64_t rpb,
                  int nt, bool cg, hipStream_t s);
 size_t ebe_scratch_bytes(const jno_op* op);
+// the structured-box SpMV of an element-by-element operator (ebe_box.hip, FP32): bricks of its grid
+// (0: not a box), and the launch (grid <= 0: one workgroup per brick)
+int64_t ebe_box_bricks(const jno_ebe* e);
+template <typename T>
+int ebe_box_launch(const jno_op* op, const T* x, T* y, double* part, PcgStateView st, int grid, bool cg,
+                   hipStream_t s);
 
 template <int NT>
 __device__ __forceinline__ void store_out_1(double v, double* __restrict__ dst, double* sm_out) {

@@ -87,6 +87,7 @@ class EbeGeometry:
         _lib.call("jno_ebe_incidence", int(plan.inc.shape[0]), _lib.ptr(plan.inc), _lib.ptr(slot_of_d), n_slots,
                   _lib.ptr(self.inc), _lib.ptr(self.qpos), stream)
         self.dmask = system.dmask if system.dmask is not None else torch.zeros(3 * N, dtype=torch.uint8, device=dev)
+        self.box = self._box_map(m, cls, n_classes, stream)
         self.n_slots = n_slots
         self.slot_mask = torch.empty(max(n_slots, 1), dtype=torch.int32, device=dev)
         self.cells = torch.empty(max(10 * n_slots, 1), dtype=torch.int32, device=dev)
@@ -94,9 +95,38 @@ class EbeGeometry:
                   _lib.ptr(self.cells), _lib.ptr(self.slot_mask), stream)
         del lib
 
+    def _box_map(self, m, cls, n_classes, stream):
+        """The half-lattice data of the structured-box SpMV when the mesh is the P2 Kuhn box of its grid
+        and every cell of one Kuhn type has one class (checked on the device), else None."""
+        grid = getattr(m, "grid", None)
+        T, N = m.n_cells, m.n_nodes
+        if grid is None or T == 0:
+            return None
+        nx, ny, nz = grid
+        if T != 6 * nx * ny * nz or N != (2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1):
+            return None
+        per_type = cls.reshape(-1, 6)
+        if not (per_type == per_type[0]).all():
+            return None
+        dev = m.device
+        perm = torch.empty(N, dtype=torch.int32, device=dev)
+        mlat = torch.empty(N, dtype=torch.uint8, device=dev)
+        ws = torch.empty(N + 1, dtype=torch.int32, device=dev)
+        ok = C.c_int32(0)
+        _lib.call("jno_ebe_box_map", nx, ny, nz, _lib.ptr(m.cells), T, N, _lib.ptr(self.dmask), _lib.ptr(perm),
+                  _lib.ptr(mlat), _lib.ptr(ws), C.byref(ok), stream)
+        if not ok.value:
+            return None
+        cls6 = (C.c_int32 * 6)(*[int(c) for c in per_type[0]])
+        rows = torch.empty(6 * 10 * 10 * 16, dtype=torch.float32, device=dev)
+        _lib.call("jno_ebe_box_rows", _lib.ptr(self.table32), n_classes, cls6, _lib.ptr(rows), stream)
+        return {"grid": (nx, ny, nz), "perm": perm, "mask": mlat, "rows": rows, "class": cls6}
+
     @property
     def nbytes(self) -> int:
         ts = (self.order, self.tile_class, self.table64, self.table32, self.inc, self.qpos, self.slot_mask, self.cells)
+        if self.box is not None:
+            ts += (self.box["perm"], self.box["mask"], self.box["rows"])
         return int(sum(t.numel() * t.element_size() for t in ts))
 
 
@@ -106,7 +136,7 @@ class ElementOperator:
 
     format = "ebe"
 
-    def __init__(self, system, E: torch.Tensor):
+    def __init__(self, system, E: torch.Tensor, structured: bool = True):
         geo = getattr(system, "_ebe_geometry", None)
         if geo is None:
             geo = EbeGeometry(system)
@@ -131,6 +161,17 @@ class ElementOperator:
         e.inc_ptr, e.inc, e.dmask = geo.inc_ptr.data_ptr(), geo.inc.data_ptr(), geo.dmask.data_ptr()
         e.qpos = geo.qpos.data_ptr()
         e.scratch = self.scratch.data_ptr()
+        # the structured Kuhn box: FP32 products on the half lattice (ebe_box.hip), no product buffer
+        self.structured = bool(structured and geo.box is not None)
+        if self.structured:
+            bx = geo.box
+            self.e32 = self.E.to(torch.float32)
+            for d in range(3):
+                e.box_n[d] = bx["grid"][d]
+            for t in range(6):
+                e.box_class[t] = bx["class"][t]
+            e.box_perm, e.box_mask, e.box_rows32 = bx["perm"].data_ptr(), bx["mask"].data_ptr(), bx["rows"].data_ptr()
+            e.box_e64, e.box_e32 = self.E.data_ptr(), self.e32.data_ptr()
         self._ebe = e
         op = _lib.JnoOp()
         op.format = _lib.JNO_FMT_EBE
@@ -203,7 +244,8 @@ class ElementOperator:
 
     def describe(self) -> dict:
         g = self.geometry
-        return {"format": "ebe", "classes": g.n_classes, "cells": g.n_cells, "tiles": g.n_tiles}
+        return {"format": "ebe", "classes": g.n_classes, "cells": g.n_cells, "tiles": g.n_tiles,
+                "structured": self.structured}
 
     def __repr__(self) -> str:
         g = self.geometry

@@ -122,10 +122,12 @@ class ElasticitySystem:
                       _lib.ptr(b), _lib.ptr(self.dmask), None, stream)
         return A, b
 
-    def element_operator(self, args: dict):
+    def element_operator(self, args: dict, structured: bool = True):
         """(A, b) with A the element-by-element operator (:class:`jno_amd.ebe.ElementOperator`, the
         reference's ElementOperator, jno/utils/solver/element_operator.py:1-178) instead of the
-        assembled CSR: the same eliminated operator and load, no global matrix."""
+        assembled CSR: the same eliminated operator and load, no global matrix.  On a structured
+        Kuhn box its FP32 products run on the half lattice (``structured=False``: the general
+        element-by-element kernels)."""
         from .ebe import ElementOperator
 
         m = self.mesh
@@ -134,7 +136,7 @@ class ElasticitySystem:
             E = E.expand(m.n_cells)
         if E.shape != (m.n_cells,):
             raise ValueError(f"{self.param} must have one value per cell ({m.n_cells},)")
-        A = ElementOperator(self, E.contiguous())
+        A = ElementOperator(self, E.contiguous(), structured=structured)
         b = self._b0.clone()
         if self.dmask is not None:
             _lib.call("jno_ebe_dirichlet_rhs", self.ndof, _lib.ptr(self.dmask), _lib.ptr(b), _lib.stream_ptr(m.device))
