@@ -173,3 +173,27 @@ def test_ebe_structured_box_solve_matches_general(dev):
             sv = solve.cg(tol=1e-10, precision=precision)
             u = sv(op, b, M=precond.jacobi())
             assert sv.last_stats["relres"] <= 2e-10 and _rel(_np(u), uo) <= 1e-9, (precision, sv.last_stats)
+
+
+def test_ebe_box_map_rejects_reordered_cells(dev):
+    # two voxels' cells of one Kuhn type swapped: every type keeps one geometry class, but the cells
+    # no longer sit where the lattice puts them, so the device check refuses the half-lattice path
+    # and the general kernels run the same operator
+    import jno_amd as J
+    from jno_amd.elasticity import ElasticitySystem, promote_p2
+
+    n = 4
+    m1 = J.kuhn_box(n)
+    m2 = promote_p2(m1)
+    cells = m2.cells.clone()
+    t, v1, v2 = 2, 5, 37
+    cells[[6 * v1 + t, 6 * v2 + t]] = cells[[6 * v2 + t, 6 * v1 + t]]
+    m2s = J.Mesh(points=m2.points, cells=cells.contiguous(), order=2, box=m2.box, grid=m2.grid)
+    s = ElasticitySystem(m2s)
+    E = torch.tensor(np.exp(0.3 * np.random.default_rng(4).standard_normal(s.mesh.n_cells)), device="cuda")
+    A, _ = s.element_operator({"E": E})
+    assert A.geometry.n_classes == 6 and not A.structured
+    p2, c2 = _np(m2s.points), _np(m2s.cells)
+    Ao, _, _ = O.elasticity_p2_system(p2, c2, _np(E), nu=0.3, load=(0, 1.0))
+    x = np.random.default_rng(6).standard_normal(A.shape[0])
+    assert _rel(_np(A.mv(torch.tensor(x, device="cuda"))), Ao @ x) <= 1e-13
