@@ -456,6 +456,15 @@ def c5_rate(J, torch, dev, n=32):
     # C falls as any E grows (K_e is positive semidefinite), up to rounding of a near-zero quadratic form
     grad_finite = bool(torch.isfinite(g_E).all()) and bool((g_E <= 1e-12 * float(g_E.abs().max())).all())
     Ab, bb = s.evaluate({"E": E})
+    # the assembled operator (MFMA element matrices, colored scatter through the pair table)
+    ta = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.evaluate({"E": E})
+        torch.cuda.synchronize()
+        ta.append(time.perf_counter() - t0)
+    t_assemble = float(np.median(ta))
     # the assembled operator on the layout and lanes the autotuner picks (BSR3 at this size)
     svb = solve.cg(tol=1e-10, precision="mixed")
     svb(Ab, bb, M=precond.jacobi())  # warm: layout
@@ -469,6 +478,7 @@ def c5_rate(J, torch, dev, n=32):
         "workload": f"P2 elasticity {n}^3 (T={T}, {rows} dofs), E=exp(0.3 z) seed 0, nu=0.3, mixed to rtol 1e-10",
         "ebe_mixed_s": round(t_ebe, 6), "ebe_iterations": st["iterations"], "ebe_relres_true": st["relres"],
         "ebe_structured": bool(A.structured), "ebe_general_kernels_mixed_s": round(t_general, 6),
+        "assembly_s": round(t_assemble, 6),
         "bsr3_mixed_s": round(t_bsr3, 6), "bsr3_iterations": svb.last_stats["iterations"],
         "assembled_operator": svb.last_stats.get("operator"),
         "speedup_ebe_vs_bsr3": round(t_bsr3 / t_ebe, 3),

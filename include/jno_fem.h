@@ -547,12 +547,21 @@ int jno_color_cells(const int32_t* d_cells, int64_t n_cells, int nloc, const int
  * E_e (lam tr eps(u) tr eps(v) + 2 mu eps(u):eps(v)) (weak form of
  * tests/test_topology_optimisation.py:59-66; jacfwd element kernel fem_native.py:2331-2345)
  * contracted on MFMA (FP64 16x16x4) and scattered color by color into the CSR values d_vals (nnz)
- * of the vec=3 plan.  lam, mu are the Lame parameters of E = 1 and Poisson ratio nu. */
+ * of the vec=3 plan.  lam, mu are the Lame parameters of E = 1 and Poisson ratio nu.  d_rank
+ * (nullable): the pair table of jno_p2_pair_rank for this plan, which replaces the scatter's binary
+ * searches (the values are the same either way). */
 int jno_assemble_p2_elasticity(const double* d_points2, const int32_t* d_cells10, int64_t n_cells,
                                const int32_t* d_rowptr, const int32_t* d_col, int64_t nnz,
                                const double* d_E, double nu, const int32_t* d_order,
-                               const int32_t* h_color_ptr, int ncolors, double* d_vals,
-                               void* stream);
+                               const int32_t* h_color_ptr, int ncolors, const uint8_t* d_rank,
+                               double* d_vals, void* stream);
+/* The place of each (cell, local node a, local node b) pair in node a's row of the vec = 3 plan
+ * (d_rank u8[100 n_cells] at 100 e + 10 a + b: row 3 n_a + i holds node n_b's columns at
+ * rowptr[3 n_a + i] + 3 place + j), found once per plan (the scatter in compress_plan's inverse map,
+ * jno/utils/solver/fem_utils.py:4009-4060, kept per cell).  *h_ok = 0 when a row holds more than 255
+ * nodes or a pair is missing (then the assembly searches).  d_ws: 256 bytes.  Synchronizes. */
+int jno_p2_pair_rank(const int32_t* d_cells10, int64_t n_cells, const int32_t* d_rowptr, const int32_t* d_col,
+                     uint8_t* d_rank, int32_t* h_ok, void* d_ws, size_t ws_bytes, void* stream);
 
 /* A9 surface load: constant traction h_t[3] on the boundary facets lying on x[axis] == value,
  * int_F t phi = |F|/3 t on the facet's edge nodes (_surf_elem_res, fem_native.py:1882-1960).

@@ -288,7 +288,7 @@ __device__ __forceinline__ double p2_kv(const double (*sK)[33], const double (*s
 __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
     const double* __restrict__ pts, const int32_t* __restrict__ cells10, const int32_t* __restrict__ order,
     int32_t n_el, const double* __restrict__ E, double lam, double mu, const int32_t* __restrict__ rowptr,
-    const int32_t* __restrict__ col, double* __restrict__ vals) {
+    const int32_t* __restrict__ col, const uint8_t* __restrict__ rank, double* __restrict__ vals) {
   __shared__ double sK[EL_WAVES][32][33];
   __shared__ double sS[EL_WAVES][16][17];
   const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -303,14 +303,21 @@ __global__ void __launch_bounds__(64 * EL_WAVES) elast_p2_k(
   // ---- colored scatter: 100 node pairs over 64 lanes, 3x3 block each
   for (int pr = lane; pr < 100; pr += 64) {
     const int a = pr / 10, b = pr % 10;
-    const int na = c[a], nb = c[b];
-    const int rb = rowptr[3 * na];
-    const int len = (rowptr[3 * na + 1] - rb) / 3;
-    int lo = 0, hi = len - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (col[rb + 3 * mid] < 3 * nb) lo = mid + 1;
-      else hi = mid;
+    const int na = c[a];
+    int lo;
+    if (rank) {  // the pair's place in node a's row, found once per plan (jno_p2_pair_rank)
+      lo = rank[(int64_t)e * 100 + pr];
+    } else {
+      const int nb = c[b];
+      const int rb = rowptr[3 * na];
+      const int len = (rowptr[3 * na + 1] - rb) / 3;
+      lo = 0;
+      int hi = len - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[rb + 3 * mid] < 3 * nb) lo = mid + 1;
+        else hi = mid;
+      }
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -555,10 +562,54 @@ extern "C" int jno_color_cells(const int32_t* d_cells, int64_t n_cells, int nloc
   return JNO_OK;
 }
 
+// the place of every (cell, local node a, local node b) pair in node a's row of the vec = 3 pattern
+// (in nodes: the row's 3 (place) + d columns), by binary search; a place past 254 flags the table
+// unusable
+__global__ void pair_rank_k(const int32_t* __restrict__ cells10, int64_t T, const int32_t* __restrict__ rowptr,
+                            const int32_t* __restrict__ col, uint8_t* __restrict__ rank, int32_t* __restrict__ bad) {
+  int f = 0;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 100 * T; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = q / 100;
+    const int pr = (int)(q % 100), a = pr / 10, b = pr % 10;
+    const int na = cells10[e * 10 + a], nb = cells10[e * 10 + b];
+    const int rb = rowptr[3 * na];
+    const int len = (rowptr[3 * na + 1] - rb) / 3;
+    int lo = 0, hi = len - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col[rb + 3 * mid] < 3 * nb) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo > 254 || col[rb + 3 * lo] != 3 * nb) f = 1;
+    rank[q] = (uint8_t)(lo > 254 ? 255 : lo);
+  }
+  if (f) atomicOr(bad, 1);
+}
+
+extern "C" int jno_p2_pair_rank(const int32_t* d_cells10, int64_t n_cells, const int32_t* d_rowptr,
+                                const int32_t* d_col, uint8_t* d_rank, int32_t* h_ok, void* d_ws, size_t ws_bytes,
+                                void* stream) {
+  JNO_REQUIRE(d_cells10 && d_rowptr && d_col && d_rank && h_ok && d_ws && ws_bytes >= sizeof(int32_t), JNO_EINVAL,
+              "jno_p2_pair_rank: null pointer or workspace too small");
+  *h_ok = 0;
+  hipStream_t s = as_stream(stream);
+  int32_t* bad = static_cast<int32_t*>(d_ws);
+  JNO_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+  if (n_cells > 0) {
+    pair_rank_k<<<grid_for(100 * n_cells, 256), 256, 0, s>>>(d_cells10, n_cells, d_rowptr, d_col, d_rank, bad);
+    JNO_LAUNCH_CHECK();
+  }
+  int32_t hb = 1;
+  JNO_HIP(hipMemcpyAsync(&hb, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  JNO_HIP(hipStreamSynchronize(s));
+  *h_ok = hb == 0 ? 1 : 0;
+  return JNO_OK;
+}
+
 extern "C" int jno_assemble_p2_elasticity(const double* d_points2, const int32_t* d_cells10, int64_t n_cells,
                                           const int32_t* d_rowptr, const int32_t* d_col, int64_t nnz, const double* d_E,
                                           double nu, const int32_t* d_order, const int32_t* h_color_ptr, int ncolors,
-                                          double* d_vals, void* stream) {
+                                          const uint8_t* d_rank, double* d_vals, void* stream) {
   JNO_REQUIRE(nu > -1.0 && nu < 0.5, JNO_EINVAL, "jno_assemble_p2_elasticity: Poisson ratio must be in (-1, 0.5)");
   JNO_REQUIRE(d_points2 && d_cells10 && d_rowptr && d_col && d_E && d_order && h_color_ptr && d_vals, JNO_EINVAL,
               "jno_assemble_p2_elasticity: null pointer");
@@ -572,7 +623,7 @@ extern "C" int jno_assemble_p2_elasticity(const double* d_points2, const int32_t
     const int32_t n_el = h_color_ptr[c + 1] - h_color_ptr[c];
     if (!n_el) continue;
     elast_p2_k<<<(unsigned)cdiv(n_el, EL_WAVES), 64 * EL_WAVES, 0, s>>>(
-        d_points2, d_cells10, d_order + h_color_ptr[c], n_el, d_E, lam, mu, d_rowptr, d_col, d_vals);
+        d_points2, d_cells10, d_order + h_color_ptr[c], n_el, d_E, lam, mu, d_rowptr, d_col, d_rank, d_vals);
     JNO_LAUNCH_CHECK();
   }
   return JNO_OK;

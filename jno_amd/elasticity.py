@@ -80,6 +80,15 @@ class ElasticitySystem:
         self.param = param
         self.plan = m.plan(3)
         self.coloring = Coloring(m, self.plan)
+        # each (cell, node pair)'s place in its CSR row, found once, so the assembly scatters with no
+        # searches (None when the table does not apply: the assembly searches)
+        self.pair_rank = torch.empty(max(100 * m.n_cells, 1), dtype=torch.uint8, device=m.device)
+        ok = C.c_int32(0)
+        ws = torch.empty(256, dtype=torch.uint8, device=m.device)
+        _lib.call("jno_p2_pair_rank", _lib.ptr(m.cells), m.n_cells, _lib.ptr(self.plan.rowptr), _lib.ptr(self.plan.col),
+                  _lib.ptr(self.pair_rank), C.byref(ok), _lib.ptr(ws), 256, _lib.stream_ptr(m.device))
+        if not ok.value:
+            self.pair_rank = None
         self.ndof = 3 * m.n_nodes
         node_mask = m.boundary_mask(clamp) if clamp else None
         self.dmask = node_mask.repeat_interleave(3).contiguous() if node_mask is not None else None
@@ -115,7 +124,7 @@ class ElasticitySystem:
         stream = _lib.stream_ptr(m.device)
         _lib.call("jno_assemble_p2_elasticity", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, _lib.ptr(plan.rowptr),
                   _lib.ptr(plan.col), plan.nnz, _lib.ptr(E), self.nu, _lib.ptr(self.coloring.order),
-                  self.coloring.color_ptr, self.coloring.ncolors, _lib.ptr(A.vals), stream)
+                  self.coloring.color_ptr, self.coloring.ncolors, _lib.ptr(self.pair_rank), _lib.ptr(A.vals), stream)
         b = self._b0.clone()
         if self.dmask is not None:
             _lib.call("jno_dirichlet_eliminate", n, _lib.ptr(plan.rowptr), _lib.ptr(plan.col), _lib.ptr(A.vals),

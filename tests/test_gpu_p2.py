@@ -109,6 +109,13 @@ def test_elasticity_assembly(dev, n):
     assert np.abs(_np(b) - bo).max() <= 1e-15
     D = _np(A.dense())
     assert np.abs(D - D.T).max() <= 1e-13 * np.abs(D).max()
+    # the scatter through the per-plan pair table writes the same values as the searching scatter
+    assert s.pair_rank is not None
+    table = s.pair_rank
+    s.pair_rank = None
+    A2, _ = s.evaluate({"E": torch.tensor(E, device="cuda")})
+    s.pair_rank = table
+    assert torch.equal(A.vals, A2.vals)
 
 
 def test_bsr3_layout(dev):
