@@ -120,10 +120,6 @@ struct EbeBox {
   PcgStateView st;
 };
 
-// entry (r, k) of a class block in the table order of ebe.hip
-__host__ __device__ constexpr int box_table_index(int r, int k) {
-  return ((r / 16) * 8 + k / 4) * 64 + (k % 4) * 16 + r % 16;
-}
 // FP32 values of the coefficient table: for each Kuhn type t, local node a and other local node b,
 // the 3 x 3 block K_t[3a + r][3b + c] at ((10 t + a) 10 + b) 16 + 3 r + c
 constexpr int BOX_ROWS = 6 * 10 * 10 * 16;
@@ -392,7 +388,7 @@ __global__ void box_map_check_k(int nx, int ny, int nz, const int32_t* __restric
 __global__ void box_rows_k(const float* __restrict__ table, jno_i6 cls, float* __restrict__ rows) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < BOX_ROWS; q += gridDim.x * blockDim.x) {
     const int t = q / 1600, la = (q / 160) % 10, b = (q / 16) % 10, j = q % 16;
-    rows[q] = j < 9 ? table[(int64_t)cls.v[t] * 1024 + box_table_index(3 * la + j / 3, 3 * b + j % 3)] : 0.0f;
+    rows[q] = j < 9 ? table[(int64_t)cls.v[t] * 1024 + ebe_table_index(3 * la + j / 3, 3 * b + j % 3)] : 0.0f;
   }
 }
 

@@ -100,4 +100,10 @@ __device__ __forceinline__ void bsr3_rows(const int32_t* __restrict__ browptr, c
   bsr3_rows_at<L, U, V, X, CH>(bcol, bvals, x, bs, len, sub, s);
 }
 
+// position of entry (r, k) of a 30 x 30 class block (pitch 32) in the element-by-element operator's
+// class table (the MFMA operand order of its element pass)
+__host__ __device__ constexpr int ebe_table_index(int r, int k) {
+  return ((r / 16) * 8 + k / 4) * 64 + (k % 4) * 16 + r % 16;
+}
+
 }  // namespace jno

@@ -74,6 +74,7 @@ class EbeGeometry:
         self.n_cells, self.n_nodes, self.n_classes = T, N, n_classes
         self.n_tiles = int(tile_class.shape[0])
         self.order = torch.tensor(order, device=dev)
+        self.cell_class = torch.tensor(cls.astype(np.int32), device=dev)
         self.tile_class = torch.tensor(tile_class, device=dev)
         slot_of_d = torch.tensor(slot_of.astype(np.int32), device=dev)
         rep = torch.tensor(first.astype(np.int32), device=dev)
@@ -124,7 +125,8 @@ class EbeGeometry:
 
     @property
     def nbytes(self) -> int:
-        ts = (self.order, self.tile_class, self.table64, self.table32, self.inc, self.qpos, self.slot_mask, self.cells)
+        ts = (self.order, self.tile_class, self.table64, self.table32, self.inc, self.qpos, self.slot_mask, self.cells,
+              self.cell_class)
         if self.box is not None:
             ts += (self.box["perm"], self.box["mask"], self.box["rows"])
         return int(sum(t.numel() * t.element_size() for t in ts))

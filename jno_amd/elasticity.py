@@ -89,6 +89,9 @@ class ElasticitySystem:
                   _lib.ptr(self.pair_rank), C.byref(ok), _lib.ptr(ws), 256, _lib.stream_ptr(m.device))
         if not ok.value:
             self.pair_rank = None
+        # assemble from the geometry-class table when the cells repeat their shape (row-pull, no
+        # element matrices per cell); False: MFMA element matrices and the colored scatter
+        self.class_assembly = True
         self.ndof = 3 * m.n_nodes
         node_mask = m.boundary_mask(clamp) if clamp else None
         self.dmask = node_mask.repeat_interleave(3).contiguous() if node_mask is not None else None
@@ -122,14 +125,35 @@ class ElasticitySystem:
             A._cache.clear()
             A.dinv = None
         stream = _lib.stream_ptr(m.device)
-        _lib.call("jno_assemble_p2_elasticity", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells, _lib.ptr(plan.rowptr),
-                  _lib.ptr(plan.col), plan.nnz, _lib.ptr(E), self.nu, _lib.ptr(self.coloring.order),
-                  self.coloring.color_ptr, self.coloring.ncolors, _lib.ptr(self.pair_rank), _lib.ptr(A.vals), stream)
+        geo = self._class_geometry() if self.class_assembly and self.pair_rank is not None else None
+        if geo is not None and plan.max_row <= 384:
+            _lib.call("jno_assemble_p2_classes", m.n_nodes, _lib.ptr(plan.inc_ptr), _lib.ptr(plan.inc),
+                      _lib.ptr(geo.cell_class), _lib.ptr(geo.table64), _lib.ptr(E), _lib.ptr(self.pair_rank),
+                      _lib.ptr(plan.rowptr), plan.max_row, _lib.ptr(A.vals), stream)
+        else:
+            _lib.call("jno_assemble_p2_elasticity", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells,
+                      _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.nnz, _lib.ptr(E), self.nu,
+                      _lib.ptr(self.coloring.order), self.coloring.color_ptr, self.coloring.ncolors,
+                      _lib.ptr(self.pair_rank), _lib.ptr(A.vals), stream)
         b = self._b0.clone()
         if self.dmask is not None:
             _lib.call("jno_dirichlet_eliminate", n, _lib.ptr(plan.rowptr), _lib.ptr(plan.col), _lib.ptr(A.vals),
                       _lib.ptr(b), _lib.ptr(self.dmask), None, stream)
         return A, b
+
+    def _class_geometry(self):
+        """The geometry classes of the cells (shared with the element-by-element operator), or None
+        when the cells do not repeat their shape."""
+        from .ebe import EbeGeometry
+
+        geo = getattr(self, "_ebe_geometry", None)
+        if geo is None and not getattr(self, "_no_classes", False):
+            try:
+                geo = EbeGeometry(self)
+                self._ebe_geometry = geo
+            except ValueError:
+                self._no_classes = True
+        return geo
 
     def element_operator(self, args: dict, structured: bool = True):
         """(A, b) with A the element-by-element operator (:class:`jno_amd.ebe.ElementOperator`, the

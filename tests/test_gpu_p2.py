@@ -109,13 +109,18 @@ def test_elasticity_assembly(dev, n):
     assert np.abs(_np(b) - bo).max() <= 1e-15
     D = _np(A.dense())
     assert np.abs(D - D.T).max() <= 1e-13 * np.abs(D).max()
-    # the scatter through the per-plan pair table writes the same values as the searching scatter
+    # the element-matrix path (MFMA per cell, colored scatter), with and without the per-plan pair
+    # table: the same values as each other bitwise, and as the class-table row pull to rounding
     assert s.pair_rank is not None
+    assert s._class_geometry() is not None  # classes by bitwise shape (fragmented at spacing 1/3)
+    s.class_assembly = False
+    A1, _ = s.evaluate({"E": torch.tensor(E, device="cuda")})
     table = s.pair_rank
     s.pair_rank = None
     A2, _ = s.evaluate({"E": torch.tensor(E, device="cuda")})
-    s.pair_rank = table
-    assert torch.equal(A.vals, A2.vals)
+    s.pair_rank, s.class_assembly = table, True
+    assert torch.equal(A1.vals, A2.vals)
+    assert np.abs(_np(A.vals) - _np(A1.vals)).max() <= 1e-14 * np.abs(_np(A1.vals)).max()
 
 
 def test_bsr3_layout(dev):
