@@ -562,17 +562,19 @@ int jno_assemble_p2_elasticity(const double* d_points2, const int32_t* d_cells10
  * nodes or a pair is missing (then the assembly searches).  d_ws: 256 bytes.  Synchronizes. */
 int jno_p2_pair_rank(const int32_t* d_cells10, int64_t n_cells, const int32_t* d_rowptr, const int32_t* d_col,
                      uint8_t* d_rank, int32_t* h_ok, void* d_ws, size_t ws_bytes, void* stream);
-/* A6, A8 for P2 elasticity when the cells repeat their shape: the CSR values (before the
- * elimination) from the geometry-class table of jno_ebe_class_blocks (d_table64, blocks of E = 1
- * in jno_ebe's table order) with d_cell_class[e] the class of cell e, d_E per cell and the pair
- * table of jno_p2_pair_rank: each node's three rows summed over its incidences (plan d_inc_ptr /
- * d_inc, entries e*10 + a) in ascending cell order in LDS and written once -- the segment_sum of the
- * triplets (jno/utils/solver/fem_native.py:2318-2329) as a row pull.  max_row (values per row) at
- * most 384. */
+/* A6, A8 (and A11 with d_dmask) for P2 elasticity when the cells repeat their shape: the CSR values
+ * from the geometry-class table of jno_ebe_class_blocks (d_table64, blocks of E = 1 in jno_ebe's
+ * table order) with d_cell_class[e] the class of cell e, d_E per cell and the pair table of
+ * jno_p2_pair_rank: each node's three rows summed over its incidences (plan d_inc_ptr / d_inc,
+ * entries e*10 + a; d_cells10 the cells) in ascending cell order in LDS and written once -- the
+ * segment_sum of the triplets (jno/utils/solver/fem_native.py:2318-2329) as a row pull.  d_dmask
+ * (u8[3N], nullable): the symmetric elimination with zero values fused, exactly as
+ * jno_dirichlet_eliminate applies it to the matrix (_apply_dirichlet_symmetric, fem_1d.py:667-686);
+ * the caller sets b[d] = 0.  max_row (values per row) at most 384. */
 int jno_assemble_p2_classes(int64_t n_nodes, const int32_t* d_inc_ptr, const int32_t* d_inc,
-                            const int32_t* d_cell_class, const double* d_table64, const double* d_E,
-                            const uint8_t* d_rank, const int32_t* d_rowptr, int max_row, double* d_vals,
-                            void* stream);
+                            const int32_t* d_cells10, const int32_t* d_cell_class, const double* d_table64,
+                            const double* d_E, const uint8_t* d_rank, const uint8_t* d_dmask,
+                            const int32_t* d_rowptr, int max_row, double* d_vals, void* stream);
 
 /* A9 surface load: constant traction h_t[3] on the boundary facets lying on x[axis] == value,
  * int_F t phi = |F|/3 t on the facet's edge nodes (_surf_elem_res, fem_native.py:1882-1960).

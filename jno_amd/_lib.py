@@ -219,7 +219,7 @@ SIGNATURES = {
     "jno_color_cells": (C.c_int, [P, I64, C.c_int, P, P, P, C.POINTER(I32), I32, C.POINTER(I32), P, SZ, P]),
     "jno_assemble_p2_elasticity": (C.c_int, [P, P, I64, P, P, I64, P, D, P, C.POINTER(I32), C.c_int, P, P, P]),
     "jno_p2_pair_rank": (C.c_int, [P, I64, P, P, P, C.POINTER(I32), P, C.c_size_t, P]),
-    "jno_assemble_p2_classes": (C.c_int, [I64, P, P, P, P, P, P, P, C.c_int, P, P]),
+    "jno_assemble_p2_classes": (C.c_int, [I64, P, P, P, P, P, P, P, P, P, C.c_int, P, P]),
     "jno_traction_p2": (C.c_int, [P, P, I64, P, P, C.c_int, D, C.POINTER(D), P, P]),
     "jno_compliance_grad_p2": (C.c_int, [P, P, I64, D, P, P, P]),
     "jno_theta_operators": (C.c_int, [I64, P, P, P, P, P, D, D, P, P, P, P]),

@@ -319,29 +319,41 @@ __global__ void ebe_rhs_k(int64_t n, const uint8_t* __restrict__ dmask, double* 
 // The scatter of elast_p2_k reads and writes every value once per contributing cell instead.
 constexpr int AC_WAVES = 4;
 constexpr int AC_MAXR = 384;  // LDS capacity of a row, in values (rows of up to 128 nodes)
+// With dmask (the constrained dofs, zero Dirichlet values) the symmetric elimination is fused as
+// jno_dirichlet_eliminate applies it: contributions to constrained columns are left out (stored
+// zeros), and a constrained row is written as 0 with 1 on its diagonal.
 __global__ void __launch_bounds__(64 * AC_WAVES) assemble_p2_classes_k(
     int64_t n_nodes, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-    const int32_t* __restrict__ cell_class, const double* __restrict__ table, const double* __restrict__ E,
-    const uint8_t* __restrict__ rank, const int32_t* __restrict__ rowptr, double* __restrict__ vals) {
+    const int32_t* __restrict__ cells10, const int32_t* __restrict__ cell_class, const double* __restrict__ table,
+    const double* __restrict__ E, const uint8_t* __restrict__ rank, const uint8_t* __restrict__ dmask,
+    const int32_t* __restrict__ rowptr, double* __restrict__ vals) {
   __shared__ double acc[AC_WAVES][3][AC_MAXR];
   const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
   double(*ra)[AC_MAXR] = acc[wid];
   for (int64_t nd = (int64_t)blockIdx.x * AC_WAVES + wid; nd < n_nodes; nd += (int64_t)gridDim.x * AC_WAVES) {
     const int r0 = rowptr[3 * nd], len = rowptr[3 * nd + 1] - r0;  // the node's three rows are equally long
     for (int i = lane; i < 3 * len; i += 64) ra[i / len][i % len] = 0.0;
+    int self = 0;  // the node's own place in its rows
     for (int q = inc_ptr[nd]; q < inc_ptr[nd + 1]; ++q) {
       const int v = inc[q];
       const int e = v / 10, a = v - 10 * e;
       const double s = E[e];
       const double* __restrict__ K = table + (int64_t)cell_class[e] * 1024;
+      self = rank[(int64_t)e * 100 + 11 * a];
       // the 90 values of rows 3a..3a+2: distinct places within one cell, so no two lanes collide
       for (int l = lane; l < 90; l += 64) {
         const int b = l / 9, i = (l / 3) % 3, j = l % 3;
+        if (dmask && dmask[3 * (int64_t)cells10[(int64_t)e * 10 + b] + j]) continue;
         const int pos = 3 * rank[(int64_t)e * 100 + 10 * a + b] + j;
         ra[i][pos] += s * K[ebe_table_index(3 * a + i, 3 * b + j)];
       }
     }
-    for (int i = lane; i < 3 * len; i += 64) vals[rowptr[3 * nd + i / len] + i % len] = ra[i / len][i % len];
+    for (int i = lane; i < 3 * len; i += 64) {
+      const int c = i / len, pos = i % len;
+      double v = ra[c][pos];
+      if (dmask && dmask[3 * nd + c]) v = pos == 3 * self + c ? 1.0 : 0.0;
+      vals[rowptr[3 * nd + c] + pos] = v;
+    }
   }
 }
 
@@ -477,17 +489,17 @@ extern "C" int jno_ebe_dirichlet_rhs(int64_t n_rows, const uint8_t* d_dmask, dou
 }
 
 extern "C" int jno_assemble_p2_classes(int64_t n_nodes, const int32_t* d_inc_ptr, const int32_t* d_inc,
-                                       const int32_t* d_cell_class, const double* d_table64, const double* d_E,
-                                       const uint8_t* d_rank, const int32_t* d_rowptr, int max_row, double* d_vals,
-                                       void* stream) {
-  JNO_REQUIRE(d_inc_ptr && d_inc && d_cell_class && d_table64 && d_E && d_rank && d_rowptr && d_vals, JNO_EINVAL,
-              "jno_assemble_p2_classes: null pointer");
+                                       const int32_t* d_cells10, const int32_t* d_cell_class, const double* d_table64,
+                                       const double* d_E, const uint8_t* d_rank, const uint8_t* d_dmask,
+                                       const int32_t* d_rowptr, int max_row, double* d_vals, void* stream) {
+  JNO_REQUIRE(d_inc_ptr && d_inc && d_cells10 && d_cell_class && d_table64 && d_E && d_rank && d_rowptr && d_vals,
+              JNO_EINVAL, "jno_assemble_p2_classes: null pointer");
   JNO_REQUIRE(max_row >= 0 && max_row <= AC_MAXR, JNO_EINVAL,
               "jno_assemble_p2_classes: rows of %d values exceed the %d the kernel stages", max_row, AC_MAXR);
   if (n_nodes == 0) return JNO_OK;
   const int64_t g = std::min<int64_t>(cdiv(n_nodes, (int64_t)AC_WAVES), 1 << 16);
-  assemble_p2_classes_k<<<(unsigned)g, 64 * AC_WAVES, 0, as_stream(stream)>>>(n_nodes, d_inc_ptr, d_inc, d_cell_class,
-                                                                               d_table64, d_E, d_rank, d_rowptr, d_vals);
+  assemble_p2_classes_k<<<(unsigned)g, 64 * AC_WAVES, 0, as_stream(stream)>>>(
+      n_nodes, d_inc_ptr, d_inc, d_cells10, d_cell_class, d_table64, d_E, d_rank, d_dmask, d_rowptr, d_vals);
   JNO_LAUNCH_CHECK();
   return JNO_OK;
 }

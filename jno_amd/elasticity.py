@@ -126,16 +126,20 @@ class ElasticitySystem:
             A.dinv = None
         stream = _lib.stream_ptr(m.device)
         geo = self._class_geometry() if self.class_assembly and self.pair_rank is not None else None
-        if geo is not None and plan.max_row <= 384:
-            _lib.call("jno_assemble_p2_classes", m.n_nodes, _lib.ptr(plan.inc_ptr), _lib.ptr(plan.inc),
-                      _lib.ptr(geo.cell_class), _lib.ptr(geo.table64), _lib.ptr(E), _lib.ptr(self.pair_rank),
-                      _lib.ptr(plan.rowptr), plan.max_row, _lib.ptr(A.vals), stream)
-        else:
-            _lib.call("jno_assemble_p2_elasticity", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells,
-                      _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.nnz, _lib.ptr(E), self.nu,
-                      _lib.ptr(self.coloring.order), self.coloring.color_ptr, self.coloring.ncolors,
-                      _lib.ptr(self.pair_rank), _lib.ptr(A.vals), stream)
         b = self._b0.clone()
+        if geo is not None and plan.max_row <= 384:
+            # the class-table row pull with the elimination fused (zero Dirichlet values: b[d] = 0)
+            _lib.call("jno_assemble_p2_classes", m.n_nodes, _lib.ptr(plan.inc_ptr), _lib.ptr(plan.inc),
+                      _lib.ptr(m.cells), _lib.ptr(geo.cell_class), _lib.ptr(geo.table64), _lib.ptr(E),
+                      _lib.ptr(self.pair_rank), _lib.ptr(self.dmask), _lib.ptr(plan.rowptr), plan.max_row,
+                      _lib.ptr(A.vals), stream)
+            if self.dmask is not None:
+                _lib.call("jno_ebe_dirichlet_rhs", n, _lib.ptr(self.dmask), _lib.ptr(b), stream)
+            return A, b
+        _lib.call("jno_assemble_p2_elasticity", _lib.ptr(m.points), _lib.ptr(m.cells), m.n_cells,
+                  _lib.ptr(plan.rowptr), _lib.ptr(plan.col), plan.nnz, _lib.ptr(E), self.nu,
+                  _lib.ptr(self.coloring.order), self.coloring.color_ptr, self.coloring.ncolors,
+                  _lib.ptr(self.pair_rank), _lib.ptr(A.vals), stream)
         if self.dmask is not None:
             _lib.call("jno_dirichlet_eliminate", n, _lib.ptr(plan.rowptr), _lib.ptr(plan.col), _lib.ptr(A.vals),
                       _lib.ptr(b), _lib.ptr(self.dmask), None, stream)
