@@ -309,9 +309,11 @@ __global__ void __launch_bounds__(PB_NT) __attribute__((amdgpu_waves_per_eu(6)))
           const int64_t L = ((int64_t)I * NJ + J) * NK + K;
           const int64_t nd = a.perm[L];
           const uint8_t m = a.mlat[L];
+          // the node's own x was staged (as 0 on its constrained dofs, which read it from memory)
+          const typename Vec4<T>::type xo = xs[P * SUB + ((li + 1) * HY + (lj + 1)) * HZ + lk + 4 * u + 1];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const T xd = a.x[3 * nd + c];
+            const T xd = ((m >> c) & 1) ? a.x[3 * nd + c] : xo[c];
             const T v = ((m >> c) & 1) ? xd : acc[u][c];
             a.y[3 * nd + c] = v;
             part += (double)xd * (double)v;
