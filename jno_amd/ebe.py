@@ -167,7 +167,8 @@ class ElementOperator:
         self.structured = bool(structured and geo.box is not None)
         if self.structured:
             bx = geo.box
-            self.e32 = self.E.to(torch.float32)
+            self.e32 = torch.empty(max(geo.n_cells, 1), dtype=torch.float32, device=dev)
+            _lib.call("jno_cast_f32", geo.n_cells, _lib.ptr(self.E), _lib.ptr(self.e32), _lib.stream_ptr(dev))
             for d in range(3):
                 e.box_n[d] = bx["grid"][d]
             for t in range(6):
