@@ -491,6 +491,10 @@ def c5_rate(J, torch, dev, n=32):
                      "fp32_tflops": round(spmv_flops / (spmv_ms * 1e-3) / 1e12, 2) if spmv_ms > 0 else None,
                      "fp32_peak_tflops": FP32_PEAK_TFLOPS,
                      "fp32_frac": round(spmv_flops / (spmv_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4) if spmv_ms > 0 else None,
+                     # the peak without packed FP32 instructions, which this build does not emit
+                     "fp32_unpacked_peak_tflops": FP32_PEAK_TFLOPS / 2,
+                     "fp32_unpacked_frac": round(spmv_flops / (spmv_ms * 1e-3) / 1e12 / (FP32_PEAK_TFLOPS / 2), 4)
+                     if spmv_ms > 0 else None,
                      "sampled_iterations": prof_n.value},
     }
 
