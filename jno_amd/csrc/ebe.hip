@@ -328,25 +328,56 @@ __global__ void __launch_bounds__(64 * AC_WAVES) assemble_p2_classes_k(
     const double* __restrict__ E, const uint8_t* __restrict__ rank, const uint8_t* __restrict__ dmask,
     const int32_t* __restrict__ rowptr, double* __restrict__ vals) {
   __shared__ double acc[AC_WAVES][3][AC_MAXR];
+  __shared__ int si[AC_WAVES][64];     // a batch of the node's incidences (cell * 10 + local node)
+  __shared__ int sc[AC_WAVES][64];     // their cells' classes
+  __shared__ double se[AC_WAVES][64];  // and moduli
+  __shared__ uint32_t sm[AC_WAVES][64];  // and constrained-dof bits (bit 3b + j: local node b, comp j)
   const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
   double(*ra)[AC_MAXR] = acc[wid];
   for (int64_t nd = (int64_t)blockIdx.x * AC_WAVES + wid; nd < n_nodes; nd += (int64_t)gridDim.x * AC_WAVES) {
     const int r0 = rowptr[3 * nd], len = rowptr[3 * nd + 1] - r0;  // the node's three rows are equally long
     for (int i = lane; i < 3 * len; i += 64) ra[i / len][i % len] = 0.0;
     int self = 0;  // the node's own place in its rows
-    for (int q = inc_ptr[nd]; q < inc_ptr[nd + 1]; ++q) {
-      const int v = inc[q];
-      const int e = v / 10, a = v - 10 * e;
-      const double s = E[e];
-      const double* __restrict__ K = table + (int64_t)cell_class[e] * 1024;
-      self = rank[(int64_t)e * 100 + 11 * a];
-      // the 90 values of rows 3a..3a+2: distinct places within one cell, so no two lanes collide
-      for (int l = lane; l < 90; l += 64) {
-        const int b = l / 9, i = (l / 3) % 3, j = l % 3;
-        if (dmask && dmask[3 * (int64_t)cells10[(int64_t)e * 10 + b] + j]) continue;
-        const int pos = 3 * rank[(int64_t)e * 100 + 10 * a + b] + j;
-        ra[i][pos] += s * K[ebe_table_index(3 * a + i, 3 * b + j)];
+    const int q0 = inc_ptr[nd], q1 = inc_ptr[nd + 1];
+    for (int qb = q0; qb < q1; qb += 64) {
+      // the batch's incidence, class and modulus loads in flight together, one per lane
+      if (qb + lane < q1) {
+        const int v = inc[qb + lane];
+        const int e = v / 10;
+        si[wid][lane] = v;
+        sc[wid][lane] = cell_class[e];
+        se[wid][lane] = E[e];
+        uint32_t bits = 0;
+        if (dmask)
+          for (int b = 0; b < 10; ++b) {
+            const int64_t d = 3 * (int64_t)cells10[(int64_t)e * 10 + b];
+            bits |= (dmask[d] ? 1u : 0u) << (3 * b) | (dmask[d + 1] ? 2u : 0u) << (3 * b) |
+                    (dmask[d + 2] ? 4u : 0u) << (3 * b);
+          }
+        sm[wid][lane] = bits;
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int q = qb; q < min(q1, qb + 64); ++q) {
+        const int v = si[wid][q - qb];
+        const int e = v / 10, a = v - 10 * e;
+        const double s = se[wid][q - qb];
+        const double* __restrict__ K = table + (int64_t)sc[wid][q - qb] * 1024;
+        const uint32_t cbits = sm[wid][q - qb];
+        self = rank[(int64_t)e * 100 + 11 * a];
+        // the 90 values of rows 3a..3a+2: distinct places within one cell, so no two lanes collide
+        for (int l = lane; l < 90; l += 64) {
+          const int b = l / 9, i = (l / 3) % 3, j = l % 3;
+          if ((cbits >> (3 * b + j)) & 1u) continue;
+          const int pos = 3 * rank[(int64_t)e * 100 + 10 * a + b] + j;
+          ra[i][pos] += s * K[ebe_table_index(3 * a + i, 3 * b + j)];
+        }
+      }
+      // the next batch rewrites the staged incidences after every lane has read them
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     for (int i = lane; i < 3 * len; i += 64) {
       const int c = i / len, pos = i % len;
