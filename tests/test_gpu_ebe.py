@@ -118,7 +118,20 @@ def test_c5_32_ebe_solve(dev):
     s = ElasticitySystem(J.kuhn_box(n))
     E = torch.tensor(np.exp(0.3 * np.random.default_rng(0).standard_normal(s.mesh.n_cells)), device="cuda")
     A, b = s.element_operator({"E": E})
-    assert A.geometry.n_classes == 6 and A.shape == (823_875, 823_875)
+    assert A.geometry.n_classes == 6 and A.shape == (823_875, 823_875) and A.structured
+    # the half-lattice product against the general element pass + gather at full size
+    G, _ = s.element_operator({"E": E}, structured=False)
+    xr = torch.randn(A.shape[0], device="cuda", dtype=torch.float32, generator=torch.Generator("cuda").manual_seed(3))
+    assert _rel(_np(A.mv(xr)).astype(np.float64), _np(G.mv(xr)).astype(np.float64)) <= 1e-6
+    del G
+    # the class-table assembly (elimination fused) against the MFMA element matrices + colored scatter
+    Ac, _ = s.evaluate({"E": E})
+    s.class_assembly = False
+    Ae, _ = s.evaluate({"E": E})
+    s.class_assembly = True
+    assert torch.equal(Ac.col, Ae.col)
+    assert float((Ac.vals - Ae.vals).abs().max()) <= 1e-13 * float(Ae.vals.abs().max())
+    del Ac, Ae
     mixed = solve.cg(tol=1e-10, precision="mixed")
     u = mixed(A, b, M=precond.jacobi())
     torch.cuda.synchronize()
