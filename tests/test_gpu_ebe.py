@@ -150,7 +150,7 @@ def test_c5_32_ebe_solve(dev):
     assert abs(float(torch.dot(E, g)) + C) <= 1e-8 * abs(C)
 
 
-@pytest.mark.parametrize("n,box", [(2, None), (4, None), ((5, 3, 6), ((0.0, 0.0, 0.0), (5.0, 3.0, 6.0))),
+@pytest.mark.parametrize("n,box", [(1, None), (2, None), (4, None), ((5, 3, 6), ((0.0, 0.0, 0.0), (5.0, 3.0, 6.0))),
                                    ((9, 4, 7), ((0.0, 0.0, 0.0), (4.5, 2.0, 3.5)))])
 def test_ebe_structured_box_spmv(dev, n, box):
     # the half-lattice SpMV of a structured Kuhn box (grids that are not multiples of the 4-voxel
