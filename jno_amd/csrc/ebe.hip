@@ -359,19 +359,37 @@ __global__ void __launch_bounds__(64 * AC_WAVES) assemble_p2_classes_k(
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int q = qb; q < min(q1, qb + 64); ++q) {
-        const int v = si[wid][q - qb];
-        const int e = v / 10, a = v - 10 * e;
-        const double s = se[wid][q - qb];
-        const double* __restrict__ K = table + (int64_t)sc[wid][q - qb] * 1024;
-        const uint32_t cbits = sm[wid][q - qb];
-        self = rank[(int64_t)e * 100 + 11 * a];
+      const int qe = min(q1, qb + 64);
+      // two incidences per step: both cells' loads in flight together, their sums added in order
+      for (int q = qb; q < qe; q += 2) {
+        const bool two = q + 1 < qe;
+        int e[2], a[2];
+        double s2[2];
+        const double* __restrict__ K[2];
+        uint32_t cbits[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int qq = two ? q + h : q;
+          const int v = si[wid][qq - qb];
+          e[h] = v / 10;
+          a[h] = v - 10 * e[h];
+          s2[h] = se[wid][qq - qb];
+          K[h] = table + (int64_t)sc[wid][qq - qb] * 1024;
+          cbits[h] = sm[wid][qq - qb];
+        }
+        self = rank[(int64_t)e[0] * 100 + 11 * a[0]];
         // the 90 values of rows 3a..3a+2: distinct places within one cell, so no two lanes collide
         for (int l = lane; l < 90; l += 64) {
           const int b = l / 9, i = (l / 3) % 3, j = l % 3;
-          if ((cbits >> (3 * b + j)) & 1u) continue;
-          const int pos = 3 * rank[(int64_t)e * 100 + 10 * a + b] + j;
-          ra[i][pos] += s * K[ebe_table_index(3 * a + i, 3 * b + j)];
+          double val[2];
+          int pos[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            pos[h] = 3 * rank[(int64_t)e[h] * 100 + 10 * a[h] + b] + j;
+            val[h] = s2[h] * K[h][ebe_table_index(3 * a[h] + i, 3 * b + j)];
+          }
+          if (!((cbits[0] >> (3 * b + j)) & 1u)) ra[i][pos[0]] += val[0];
+          if (two && !((cbits[1] >> (3 * b + j)) & 1u)) ra[i][pos[1]] += val[1];
         }
       }
       // the next batch rewrites the staged incidences after every lane has read them
