@@ -32,11 +32,11 @@ class BatchSolver:
     step's table of (system, row range) entries, 0 taking the library defaults;
     ``steps_per_graph`` steps are captured in one hipGraph replay (0: direct launches).
     :func:`jno_amd.autotune.tune_batch` picks them by timing.  ``replace_factor``: the reduction of
-    each system's FP32 residual norm between its FP64 residual replacements (0.01: five replacements
-    to rtol 1e-10; on 256 uneven Kuhn-box systems 94.6 ms against 103.4 ms at 0.1, the same
-    iterations)."""
+    each system's FP32 residual norm between its FP64 residual replacements (0.003: four
+    replacements to rtol 1e-10; on 256 uneven Kuhn-box systems 92.9 ms against 95.2 ms at 0.01 and
+    103.4 ms at 0.1, the same iterations)."""
 
-    def __init__(self, tol: float = 1e-10, atol: float = 0.0, maxiter: int = 20_000, replace_factor: float = 0.01,
+    def __init__(self, tol: float = 1e-10, atol: float = 0.0, maxiter: int = 20_000, replace_factor: float = 0.003,
                  steps_per_graph: int = 8, format: str = "auto", one_kernel: bool = True, rows_per_entry: int = 0,
                  grid: int = 0, autotune: bool = False):
         self.tol, self.atol, self.maxiter = float(tol), float(atol), int(maxiter)
